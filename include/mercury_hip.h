@@ -1,0 +1,143 @@
+/*
+ * mercury_hip.h -- C ABI of the MI355X-native Mercury explicit step.
+ *
+ * The reference has no plugin/FFI layer: its "boundary" is the set of C++
+ * class APIs that Z4_Mercury/main.cpp:27-99 wires together.  Each entry point
+ * below replaces the reference interface named in its comment; all take plain
+ * pointers and sizes, return 0 on success and -1 on failure (message in
+ * mercury_last_error()), and never let an exception cross the boundary.
+ *
+ * Storage convention for every per-block array crossing this ABI: the box
+ * [-g, m+g] on each axis of a block with m cells, k fastest,
+ * element (i,j,k) at ((i+g)*(my+1+2g) + (j+g))*(mz+1+2g) + (k+g), one array per
+ * field component.  It is the reference's FieldBlock(i,j,k,m) index space
+ * (include/3_field/FieldBlock.h:38-49) with a common allocation for all
+ * stagger locations.
+ */
+#ifndef MERCURY_HIP_H
+#define MERCURY_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MERCURY_HIP_ABI_VERSION 1
+
+/* Face boundary codes, order -xi,+xi,-eta,+eta,-zeta,+zeta.
+ * PERIODIC: an inner self-connection with a negative source block id
+ *           (src/1_grid/1_MPCNS_Grid.cpp:455-465).
+ * FARFIELD: the "Farfield" physical patch (Z4_Mercury/src/boundary/Boundary_Build.cpp:45-203).
+ * WALL:     the fluid side of the solid planet surface, i.e. the "Coupled-Solid"
+ *           handlers (Boundary_Build.cpp:59-71, 111-119) and the stationary-wall EMF
+ *           (Z4_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:5-89). */
+enum { MERCURY_BC_PERIODIC = 0, MERCURY_BC_FARFIELD = 1, MERCURY_BC_WALL = 2 };
+
+/* Integrators: forward Euler (the reference StepOnce) or SSP-RK2 (two Euler
+ * stages and the average; no reference counterpart). */
+enum { MERCURY_EULER = 0, MERCURY_SSPRK2 = 1 };
+
+/* Non-dimensional constants, derived as in MercurySolver::calc_physical_constant
+ * (Z4_Mercury/src/physics/MercurySolver_Tools.cpp:3-48) and
+ * BuildMercuryBackgroundState (Z4_Mercury/include/0_BackgroundState.h:18-84). */
+typedef struct MercuryPhysics {
+    double gamma;
+    double M_H, M_Na, M_ref;
+    double state_coeff_H, state_coeff_Na;
+    double T_ref;
+    double inver_MA2;
+    double hall_coef;
+    double ne_floor;
+    double c_ind, c_hall;            /* momentum_induce_coeff, momentum_hall_coeff */
+    double a1_Na, a4, a5, a6;        /* AddSourceToRHS_Fluid coefficients */
+    double Tn0;
+    double nu_lit, nu_shadow;        /* drag frequencies sk1, sk2 */
+    double cfl;
+    double taper_rmin, taper_rmax;   /* Hall taper radii r_min, r_max */
+    double arti_eta_max, J_start, J_on;
+    double local_enabled, local_eta_max, local_center[3], local_r_decay, local_r_cutoff;
+    double ambipolar_enabled, pressure_coef;
+    double qinf[5], qinfs[5], q_pv_inf[5], B_imf[3];
+    double wall_rho_floor_H, wall_rho_floor_Na, wall_p_floor;
+    double reserved[8];
+} MercuryPhysics;
+
+typedef struct MercuryContext MercuryContext;
+
+const char *mercury_last_error(void);
+int mercury_version(void);
+
+/* Replaces Grid::Grid_Preprocess (include/1_grid/1_MPCNS_Grid.h:152-226) +
+ * Field(Grid*,Param*,ngg) with METRIC::build_field_geometry (src/7_metric/Metric.cpp:1416)
+ * + MercurySolver::RegisterFields/RegisterHaloFields and the solver constructor's
+ * static setup (Z4_Mercury/src/MercurySolver.cpp:11-247, SetupCellReconstructionWeights_).
+ * dims: 3 ints per block (cells); bc: 6 codes per block; owned_nodes: per block
+ * x, y, z planes of the (mx+1)(my+1)(mz+1) owned nodes, k fastest. */
+int mercury_create(MercuryContext **ctx, int nblocks, const int *dims, int nghost, const int *bc,
+                   const double *owned_nodes, const MercuryPhysics *phys);
+/* Replaces the deletes at the end of Z4_Mercury/main.cpp:86-89. */
+int mercury_destroy(MercuryContext *ctx);
+
+/* Replaces Field::field_id / descriptor (include/3_field/Field.h:16-96). */
+int mercury_field_info(const char *name, int *slot, int *ncomp, int *location);
+/* Elements of one per-block array (the box volume). */
+long long mercury_block_volume(MercuryContext *ctx, int block);
+long long mercury_total_volume(MercuryContext *ctx);
+
+/* Replace direct FieldBlock(i,j,k,m) access on host storage
+ * (include/3_field/FieldBlock.h:38-49) and the restart injection of
+ * IOModule::ReadRestartBinFile (src/6_io/IOModule_Bin.cpp:234-420). */
+int mercury_upload(MercuryContext *ctx, const char *name, int block, int comp, const double *host);
+int mercury_download(MercuryContext *ctx, const char *name, int block, int comp, double *host);
+
+/* Initial face fluxes: B = Bu . JD + d1(A) and Badd = B_imf . JD on the allocated
+ * face range; A holds 3 edge arrays per block (or NULL).  The IMF part stands in
+ * for DipoleField::Build_Badd_FaceFlux (Z4_Mercury/include/operators/Dipole.h:89-374). */
+int mercury_init_bface(MercuryContext *ctx, const double *Bu, const double *const *A);
+
+/* Replaces the constructor tail Sync("Badd"), Sync("Bface") (MercurySolver.cpp:141-145)
+ * and the UpdateDerivedFields_ call at the head of Advance (MercurySolver_Driver.cpp:6-19). */
+int mercury_prepare(MercuryContext *ctx);
+
+/* Replaces the while(!if_stop) StepOnce() loop of MercurySolver::Advance
+ * (Z4_Mercury/src/MercurySolver_Driver.cpp:6-61) for nsteps steps; dt_out (nsteps
+ * doubles, may be NULL) receives each step's dt.  Fails on the reference's dt
+ * abort rule (MercurySolver_Timestep.cpp:129-140). */
+int mercury_step(MercuryContext *ctx, int nsteps, int integrator, double *dt_out);
+
+/* Individual operators (one reference function each), for parity tests and
+ * drop-in use: pv_uplus, bcell, jedge, jcell, derived, flux, sources, eface,
+ * edge_emf, hall_cell, hall_edge, induction, alias_e, alias_j, timestep, stage,
+ * dec_d0, dec_d1 (arg = multiplier), dec_d2, sync_ucell, sync_bface, sync_eedge,
+ * sync_ehall, sync_jedge, sync_bcell, sync_jcell, sync_badd. */
+int mercury_op(MercuryContext *ctx, const char *op, double arg);
+
+/* Replaces MercurySolver::Compute_Timestep (MercurySolver_Timestep.cpp:8-141). */
+int mercury_timestep(MercuryContext *ctx, double *dt);
+
+/* Replaces calc_divB (MercurySolver_MagneticCell.cpp:151-241); returns
+ * max|d2 B| / max|B_face| and max|d2 B| over owned cells/faces. */
+int mercury_divb(MercuryContext *ctx, double *normalised, double *maxabs);
+
+int mercury_synchronize(MercuryContext *ctx);
+
+/* One explicit step captured as a HIP graph and replayed (no reference
+ * counterpart: the reference loops on the host). */
+int mercury_graph_capture(MercuryContext *ctx, int integrator);
+int mercury_graph_launch(MercuryContext *ctx, int nsteps);
+
+/* Average device time (ms) of an op over reps launches, HIP events on the
+ * solver's stream; op "step_graph" replays the captured step. */
+int mercury_time_op(MercuryContext *ctx, const char *op, int reps, double *ms_avg);
+
+/* Host-only static geometry of one block (no GPU needed): writes component
+ * comp of the named metric / reconstruction field (JDxi, Jac, beta_*, alpha_*,
+ * dr_*, Bcell_from_Bface_w, ...) over the block box.  Replaces reading the
+ * fields of METRIC::compute_metric_fields (src/7_metric/Metric.cpp:78-1414) and
+ * SetupCellReconstructionWeights_ (Z4_Mercury/src/MercurySolver_ReconstructionSetup.cpp:29-193). */
+int mercury_geometry_block(const int *dims, int nghost, const double *owned_nodes, const MercuryPhysics *phys,
+                           const char *name, int comp, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

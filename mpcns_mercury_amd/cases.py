@@ -1,0 +1,214 @@
+"""Synthetic cases of the Mercury explicit step (grids, topology, seeded initial states).
+
+The reference reads its grids from plot3D files and injects initial states by
+restart (src/1_grid/1_MPCNS_Grid.cpp:125-230, src/6_io/IOModule_Bin.cpp:234-420).
+No data files exist for the benchmark configurations, so this module builds the
+same inputs in memory from a counter-based PRNG (splitmix64), identically for
+the HIP path and for the CPU oracle:
+
+* owned node coordinates per block (ghost nodes are extrapolated by the solver,
+  as ``Block::Add_ghostmesh`` does);
+* the face boundary codes of every block (periodic self-connection, far field,
+  or the fluid side of the solid planet surface);
+* the conserved fluid state of both ion species on the whole allocated cell box
+  (ghosts included, as a restart file holds them);
+* an edge vector potential ``A`` whose discrete curl is the divergence-free
+  face-B perturbation, plus a uniform part ``Bu`` (face flux ``Bu . JD``);
+* the static photo-ionisation source field.
+
+Layout of every per-block array: the uniform box [-g, m+g] per axis, k fastest.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .params import MercuryPhysics, MercuryUnits, derive, RHO_SMALL
+
+BC_PERIODIC, BC_FARFIELD, BC_WALL = 0, 1, 2
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """splitmix64 finaliser, vectorised over uint64 (wrap-around arithmetic)."""
+    with np.errstate(over="ignore"):
+        z = (x.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, stream: int, key: np.ndarray) -> np.ndarray:
+    """Deterministic U(-1, 1) per integer key."""
+    h = splitmix64(np.uint64(seed) ^ splitmix64(np.uint64(stream) ^ splitmix64(key.astype(np.uint64))))
+    return ((h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0
+
+
+@dataclass
+class Block:
+    dims: tuple  # cells (mx, my, mz)
+    bc: tuple  # 6 face codes: -xi, +xi, -eta, +eta, -zeta, +zeta
+    nodes: np.ndarray  # (3, mx+1, my+1, mz+1)
+
+
+@dataclass
+class Case:
+    name: str
+    g: int
+    blocks: List[Block]
+    phys: MercuryPhysics
+    fields: List[Dict[str, np.ndarray]]  # per block: name -> (ncomp, NI, NJ, NK)
+    A: Optional[List[np.ndarray]]  # per block (3, NI, NJ, NK) edge potential, or None
+    Bu: tuple = (0.0, 0.0, 0.0)
+    steps: int = 100
+    integrator: int = 0  # 0 forward Euler (reference), 1 SSP-RK2
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nb(self) -> int:
+        return len(self.blocks)
+
+    def box(self, ib: int):
+        m = self.blocks[ib].dims
+        return tuple(mm + 1 + 2 * self.g for mm in m)
+
+    def ncells(self) -> int:
+        return int(sum(np.prod(b.dims) for b in self.blocks))
+
+    def dims_array(self) -> np.ndarray:
+        return np.array([b.dims for b in self.blocks], dtype=np.int32).reshape(-1)
+
+    def bc_array(self) -> np.ndarray:
+        return np.array([b.bc for b in self.blocks], dtype=np.int32).reshape(-1)
+
+    def nodes_flat(self) -> np.ndarray:
+        return np.concatenate([np.ascontiguousarray(b.nodes).reshape(-1) for b in self.blocks]).astype(np.float64)
+
+
+def _box_index(dims, g):
+    """Integer (i, j, k) index grids of the box [-g, m+g]."""
+    rng = [np.arange(-g, m + g + 1) for m in dims]
+    return np.meshgrid(*rng, indexing="ij")
+
+
+def _conserved(rho, u, v, w, p, gamma):
+    E = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    return np.stack([rho, rho * u, rho * v, rho * w, E])
+
+
+# ---------------------------------------------------------------------------
+# C1: single-fluid ideal MHD on one periodic curvilinear block
+# ---------------------------------------------------------------------------
+
+def make_periodic_cube(n=(48, 48, 48), g: int = 4, seed: int = 1, amp: float = 1e-3,
+                       integrator: int = 0, steps: int = 100, hall: bool = False) -> Case:
+    """One block mapped from the unit cube by x += 0.05 sin(2 pi y) sin(2 pi z),
+    periodic on every axis; rho = 1, p = 0.1, B = (1,0,0) + curl of a random edge
+    potential, seeded velocity perturbations; the Na+ fluid sits at rho_small.
+    Hall is off (taper radii beyond the domain) unless ``hall``."""
+    mx, my, mz = n
+    i, j, k = np.meshgrid(np.arange(mx + 1), np.arange(my + 1), np.arange(mz + 1), indexing="ij")
+    y = j / my
+    z = k / mz
+    x = i / mx + 0.05 * np.sin(2 * np.pi * y) * np.sin(2 * np.pi * z)
+    nodes = np.stack([x, y, z]).astype(np.float64)
+    units = MercuryUnits().alfven_units()
+    units.CFL = 0.4
+    if hall:
+        units.r_min, units.r_max = -1.0, -1.0
+        units.hall_coef_override = 0.02
+    else:
+        units.r_min, units.r_max = 100.0, 100.0
+    P = derive(units)
+    gamma = P.gamma
+    I, J, K = _box_index(n, g)
+    key = ((np.mod(I, mx) * my + np.mod(J, my)) * mz + np.mod(K, mz)).astype(np.uint64)
+    du = [amp * uniform(seed, s, key) for s in range(3)]
+    shape = I.shape
+    rho = np.ones(shape)
+    p = np.full(shape, 0.1)
+    UH = _conserved(rho, du[0], du[1], du[2], p, gamma)
+    UNa = _conserved(RHO_SMALL * rho, du[0], du[1], du[2], RHO_SMALL * p, gamma)
+    h = 1.0 / mx
+    A = np.stack([amp * h * uniform(seed, 10 + a, key) for a in range(3)])
+    fields = [{"U_H": UH, "U_Na": UNa, "Photo_rate": np.zeros((1,) + shape)}]
+    blk = Block(dims=(mx, my, mz), bc=(BC_PERIODIC,) * 6, nodes=nodes)
+    return Case(name=f"periodic_cube_{mx}x{my}x{mz}", g=g, blocks=[blk], phys=P, fields=fields, A=[A],
+                Bu=(1.0, 0.0, 0.0), steps=steps, integrator=integrator,
+                meta={"seed": seed, "species": 2, "hall": hall})
+
+
+# ---------------------------------------------------------------------------
+# C2/C3: two-fluid Hall MHD on six spherical-shell sector blocks
+# ---------------------------------------------------------------------------
+
+C2_BLOCKS = ((96, 64, 64), (128, 48, 64), (64, 96, 80), (80, 80, 48), (112, 56, 72), (72, 72, 96))
+# panel axes (centre direction, first and second tangent) of the six sectors
+_PANELS = (((1, 0, 0), (0, 1, 0), (0, 0, 1)), ((-1, 0, 0), (0, 0, 1), (0, 1, 0)),
+           ((0, 1, 0), (0, 0, 1), (1, 0, 0)), ((0, -1, 0), (1, 0, 0), (0, 0, 1)),
+           ((0, 0, 1), (1, 0, 0), (0, 1, 0)), ((0, 0, -1), (0, 1, 0), (1, 0, 0)))
+
+
+def sector_nodes(dims, panel, r0=1.0, r1=4.0, half_angle=np.deg2rad(40.0)):
+    """Gnomonic sector of the shell r in [r0, r1]: xi radial, eta/zeta angular."""
+    mx, my, mz = dims
+    c, e1, e2 = (np.asarray(v, dtype=np.float64) for v in panel)
+    r = r0 + (r1 - r0) * np.arange(mx + 1) / mx
+    ta = np.tan(-half_angle + 2 * half_angle * np.arange(my + 1) / my)
+    tb = np.tan(-half_angle + 2 * half_angle * np.arange(mz + 1) / mz)
+    R, TA, TB = np.meshgrid(r, ta, tb, indexing="ij")
+    d = c[:, None, None, None] + TA[None] * e1[:, None, None, None] + TB[None] * e2[:, None, None, None]
+    d = d / np.sqrt((d * d).sum(axis=0))[None]
+    return (R[None] * d).astype(np.float64)
+
+
+def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, integrator: int = 0,
+               scale: float = 1.0, hall_coef: float | None = None) -> Case:
+    """Two-fluid Hall MHD: H+ and Na+ (mass ratio 1:23, density ratio 1:0.01) plus
+    electron pressure (ambipolar term), inflow u = (-1,0,0), IMF B = (0,0,-0.2),
+    the fluid side of a perfectly conducting sphere at r = 1 on every block's
+    xi- face and far field on the other five faces; seeded 1e-3 density noise.
+
+    The listed block sizes admit no conformal 1:1 tiling of the shell, so every
+    block is a separate gnomonic sector (no inter-block interfaces)."""
+    units = MercuryUnits().alfven_units()
+    Bref = units.REF["B_ref"]
+    units.INITIAL = {"U": units.REF["U"], "T": units.REF["T"], "n": units.REF["n"],
+                     "Molecular_mass": units.REF["Molecular_mass"], "c_y": 0.0, "c_z": 0.0,
+                     "Bx": 0.0, "By": 0.0, "Bz": -0.2 * Bref}
+    units.CFL = 0.4
+    units.r_min, units.r_max = 1.1, 1.2
+    units.is_Ambipolar_Efield = True
+    units.is_local_artificial_resistivity = True
+    units.hall_coef_override = hall_coef
+    P = derive(units)
+    gamma = P.gamma
+    blks, fields = [], []
+    for ib, dims0 in enumerate(blocks):
+        dims = tuple(max(4, int(round(d * scale))) for d in dims0)
+        nodes = sector_nodes(dims, _PANELS[ib % 6])
+        blks.append(Block(dims=dims, bc=(BC_WALL,) + (BC_FARFIELD,) * 5, nodes=nodes))
+        I, J, K = _box_index(dims, g)
+        key = (((ib * 4096 + (I + g)) * 4096 + (J + g)) * 4096 + (K + g)).astype(np.uint64)
+        shape = I.shape
+        rhoH = P.qinf[0] * (1.0 + 1e-3 * uniform(seed, 20, key))
+        u0, v0, w0, p0 = P.q_pv_inf[0], P.q_pv_inf[1], P.q_pv_inf[2], P.q_pv_inf[3]
+        UH = _conserved(rhoH, np.full(shape, u0), np.full(shape, v0), np.full(shape, w0), np.full(shape, p0), gamma)
+        rhoN = 0.01 * rhoH
+        UNa = _conserved(rhoN, np.full(shape, u0), np.full(shape, v0), np.full(shape, w0),
+                         np.full(shape, p0 * 0.01 / 23.0), gamma)
+        # cell-centre radius from the owned nodes (photo-ionisation profile of the exosphere)
+        cc = np.zeros((3,) + shape)
+        nd = nodes
+        ccore = 0.125 * (nd[:, :-1, :-1, :-1] + nd[:, 1:, :-1, :-1] + nd[:, :-1, 1:, :-1] + nd[:, 1:, 1:, :-1] +
+                         nd[:, :-1, :-1, 1:] + nd[:, 1:, :-1, 1:] + nd[:, :-1, 1:, 1:] + nd[:, 1:, 1:, 1:])
+        cc[:, g:g + dims[0], g:g + dims[1], g:g + dims[2]] = ccore
+        r = np.sqrt((cc * cc).sum(axis=0))
+        photo = np.where(r > 1.0, 1e-2 * np.exp(-(r - 1.0) / 0.2), 0.0)
+        fields.append({"U_H": UH, "U_Na": UNa, "Photo_rate": photo[None]})
+    return Case(name="shell6" if scale == 1.0 else f"shell6_x{scale:g}", g=g, blocks=blks, phys=P, fields=fields,
+                A=None, Bu=(0.0, 0.0, 0.0), steps=steps, integrator=integrator,
+                meta={"seed": seed, "species": 2, "hall": True})

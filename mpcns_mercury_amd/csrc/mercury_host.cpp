@@ -1,0 +1,1110 @@
+// Host runtime of the MI355X Mercury explicit step and its C ABI.
+//
+// C++ classes keep the roles of the reference's host interfaces:
+//   Field        field registry + device SoA arena      (include/3_field/Field.h:16-96)
+//   Halo         periodic halo and owner/alias plans     (include/4_halo/Halo.h:15-68)
+//   BoundaryCore physical ghost-fill plans               (include/5_boundary/Boundary.h:19-122)
+//   DEC          d0 / d1 / d2 on device forms            (include/8_dec/DecOps.h:8-68)
+//   MercurySolver the explicit step                      (Z4_Mercury/include/MercurySolver.h:31-48)
+// The C ABI at the bottom (declared in include/mercury_hip.h) is the drop-in
+// boundary; every entry returns an int status and never throws.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mercury_hip.h"
+#include "mercury_common.h"
+#include "mercury_geometry.h"
+#include "mercury_kernels.h"
+
+static_assert(sizeof(MercuryPhysics) == sizeof(MercPhys), "MercuryPhysics / MercPhys layout mismatch");
+
+namespace merc {
+
+static thread_local std::string g_last_error;
+
+#define HIPCHK(x)                                                                                      \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static const int DELTA[8][3] = {{1, 1, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, 0}};
+
+// ---------------------------------------------------------------------------
+// field catalogue (names of Z4_Mercury/include/0_MercuryFieldCatalog.h:70-202 and
+// src/7_metric/Metric.cpp:12-76; reconstruction weights keep their names)
+// ---------------------------------------------------------------------------
+struct FieldSpec {
+    const char *name;
+    int slot, ncomp, loc;
+};
+static const FieldSpec kFields[] = {
+    {"U_H", SL_UH, 5, LOC_CELL}, {"U_Na", SL_UN, 5, LOC_CELL}, {"PV_H", SL_PVH, 5, LOC_CELL}, {"PV_Na", SL_PVN, 5, LOC_CELL},
+    {"U_plus", SL_UP, 3, LOC_CELL}, {"B_cell", SL_BC, 3, LOC_CELL}, {"Bind_cell", SL_BI, 3, LOC_CELL}, {"J_cell", SL_JC, 3, LOC_CELL},
+    {"RHS_H", SL_RH, 5, LOC_CELL}, {"RHS_Na", SL_RN, 5, LOC_CELL}, {"Ehc", SL_EHC, 3, LOC_CELL}, {"hall_beta", SL_HB, 1, LOC_CELL},
+    {"Jac", SL_JAC, 1, LOC_CELL}, {"Bcell_from_Bface_w", SL_WB, 18, LOC_CELL}, {"Jcell_from_Jedge_w", SL_WJ, 36, LOC_CELL},
+    {"Photo_rate", SL_PHOTO, 1, LOC_CELL}, {"cell_center", SL_CC, 3, LOC_CELL}, {"divB", SL_DIVB, 1, LOC_CELL},
+    {"cell_scalar", SL_CELLS, 1, LOC_CELL},
+    {"B_xi", SL_B, 1, LOC_FX}, {"B_eta", SL_B + 1, 1, LOC_FY}, {"B_zeta", SL_B + 2, 1, LOC_FZ},
+    {"Badd_xi", SL_BADD, 1, LOC_FX}, {"Badd_eta", SL_BADD + 1, 1, LOC_FY}, {"Badd_zeta", SL_BADD + 2, 1, LOC_FZ},
+    {"RHS_B_xi", SL_RHSB, 1, LOC_FX}, {"RHS_B_eta", SL_RHSB + 1, 1, LOC_FY}, {"RHS_B_zeta", SL_RHSB + 2, 1, LOC_FZ},
+    {"JDxi", SL_JD, 3, LOC_FX}, {"JDet", SL_JD + 3, 3, LOC_FY}, {"JDze", SL_JD + 6, 3, LOC_FZ},
+    {"Area_xi", SL_AREA, 1, LOC_FX}, {"Area_eta", SL_AREA + 1, 1, LOC_FY}, {"Area_zeta", SL_AREA + 2, 1, LOC_FZ},
+    {"beta_xi", SL_BETA, 1, LOC_FX}, {"beta_eta", SL_BETA + 1, 1, LOC_FY}, {"beta_zeta", SL_BETA + 2, 1, LOC_FZ},
+    {"Eface_xi", SL_EF, 3, LOC_FX}, {"Eface_eta", SL_EF + 3, 3, LOC_FY}, {"Eface_zeta", SL_EF + 6, 3, LOC_FZ},
+    {"face_xi", SL_FACEF, 1, LOC_FX}, {"face_eta", SL_FACEF + 1, 1, LOC_FY}, {"face_zeta", SL_FACEF + 2, 1, LOC_FZ},
+    {"E_xi", SL_E, 1, LOC_EX}, {"E_eta", SL_E + 1, 1, LOC_EY}, {"E_zeta", SL_E + 2, 1, LOC_EZ},
+    {"Ehall_xi", SL_EH, 1, LOC_EX}, {"Ehall_eta", SL_EH + 1, 1, LOC_EY}, {"Ehall_zeta", SL_EH + 2, 1, LOC_EZ},
+    {"J_xi", SL_J, 1, LOC_EX}, {"J_eta", SL_J + 1, 1, LOC_EY}, {"J_zeta", SL_J + 2, 1, LOC_EZ},
+    {"dr_xi", SL_DR, 3, LOC_EX}, {"dr_eta", SL_DR + 3, 3, LOC_EY}, {"dr_zeta", SL_DR + 6, 3, LOC_EZ},
+    {"dl_xi", SL_DL, 1, LOC_EX}, {"dl_eta", SL_DL + 1, 1, LOC_EY}, {"dl_zeta", SL_DL + 2, 1, LOC_EZ},
+    {"alpha_xi", SL_ALPHA, 1, LOC_EX}, {"alpha_eta", SL_ALPHA + 1, 1, LOC_EY}, {"alpha_zeta", SL_ALPHA + 2, 1, LOC_EZ},
+    {"taper_xi", SL_TAPER, 1, LOC_EX}, {"taper_eta", SL_TAPER + 1, 1, LOC_EY}, {"taper_zeta", SL_TAPER + 2, 1, LOC_EZ},
+    {"eta_local_xi", SL_LETA, 1, LOC_EX}, {"eta_local_eta", SL_LETA + 1, 1, LOC_EY}, {"eta_local_zeta", SL_LETA + 2, 1, LOC_EZ},
+    {"edge_xi", SL_EDGEF, 1, LOC_EX}, {"edge_eta", SL_EDGEF + 1, 1, LOC_EY}, {"edge_zeta", SL_EDGEF + 2, 1, LOC_EZ},
+    {"node_scalar", SL_NODEF, 1, LOC_NODE},
+};
+
+static const FieldSpec *find_field(const char *name) {
+    for (const auto &f : kFields)
+        if (!std::strcmp(f.name, name)) return &f;
+    return nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    long long n = 0;
+    void upload(const std::vector<T> &h) {
+        release();
+        n = (long long)h.size();
+        if (n == 0) return;
+        HIPCHK(hipMalloc(&p, sizeof(T) * n));
+        HIPCHK(hipMemcpy(p, h.data(), sizeof(T) * n, hipMemcpyHostToDevice));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { release(); }
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+};
+
+// ---------------------------------------------------------------------------
+// Field: one SoA arena, slot-major, every slot spanning all blocks
+// ---------------------------------------------------------------------------
+class Field {
+public:
+    std::vector<MercBlk> blk;
+    long long Vtot = 0;
+    double *arena = nullptr;
+    double **dptr = nullptr;  // device table of slot base pointers
+    std::vector<double *> hptr;
+    MercBlk *dblk = nullptr;
+
+    Field(int nb, const int *dims, int g, const int *bc) {
+        blk.resize(nb);
+        long long off = 0;
+        for (int b = 0; b < nb; ++b) {
+            MercBlk &k = blk[b];
+            k.off = off;
+            k.g = g;
+            for (int a = 0; a < 3; ++a) {
+                k.m[a] = dims[3 * b + a];
+                k.N[a] = k.m[a] + 1 + 2 * g;
+            }
+            k.sJ = k.N[2];
+            k.sI = k.N[1] * k.N[2];
+            for (int f = 0; f < 6; ++f) k.bc[f] = bc[6 * b + f];
+            off += (long long)k.N[0] * k.N[1] * k.N[2];
+        }
+        Vtot = (off + 63) / 64 * 64;
+        if (Vtot >= (1LL << 31)) throw std::runtime_error("total box volume exceeds the int32 index range");
+        HIPCHK(hipMalloc(&arena, sizeof(double) * Vtot * SL_NSLOT));
+        HIPCHK(hipMemset(arena, 0, sizeof(double) * Vtot * SL_NSLOT));
+        hptr.resize(SL_NSLOT);
+        for (int s = 0; s < SL_NSLOT; ++s) hptr[s] = arena + (long long)s * Vtot;
+        HIPCHK(hipMalloc(&dptr, sizeof(double *) * SL_NSLOT));
+        HIPCHK(hipMemcpy(dptr, hptr.data(), sizeof(double *) * SL_NSLOT, hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&dblk, sizeof(MercBlk) * nb));
+        HIPCHK(hipMemcpy(dblk, blk.data(), sizeof(MercBlk) * nb, hipMemcpyHostToDevice));
+    }
+    ~Field() {
+        if (arena) (void)hipFree(arena);
+        if (dptr) (void)hipFree(dptr);
+        if (dblk) (void)hipFree(dblk);
+    }
+    int nb() const { return (int)blk.size(); }
+    long long vol(int b) const { return (long long)blk[b].N[0] * blk[b].N[1] * blk[b].N[2]; }
+    int gidx(int b, int i, int j, int k) const {
+        const MercBlk &q = blk[b];
+        return (int)(q.off + (long long)(i + q.g) * q.sI + (long long)(j + q.g) * q.sJ + (k + q.g));
+    }
+    void owned_hi(int b, int loc, int hi[3]) const {
+        for (int a = 0; a < 3; ++a) hi[a] = blk[b].m[a] + 1 - DELTA[loc][a];
+    }
+    void upload_block(int slot, int b, const double *h) {
+        HIPCHK(hipMemcpy(hptr[slot] + blk[b].off, h, sizeof(double) * vol(b), hipMemcpyHostToDevice));
+    }
+    void download_block(int slot, int b, double *h) const {
+        HIPCHK(hipMemcpy(h, hptr[slot] + blk[b].off, sizeof(double) * vol(b), hipMemcpyDeviceToHost));
+    }
+};
+
+// ---------------------------------------------------------------------------
+// launch ranges
+// ---------------------------------------------------------------------------
+struct RangeOwner {
+    RangeH h;
+    DBuf<long long> start;
+    DBuf<int> lo, n;
+};
+
+static void make_range(const Field &F, int lo_off, int hi_off, RangeOwner &R) {
+    // per block box [lo_off, m + hi_off) on every axis
+    std::vector<long long> st(F.nb() + 1, 0);
+    std::vector<int> lo(3 * F.nb()), n(3 * F.nb());
+    for (int b = 0; b < F.nb(); ++b) {
+        long long cnt = 1;
+        for (int a = 0; a < 3; ++a) {
+            lo[3 * b + a] = lo_off;
+            n[3 * b + a] = F.blk[b].m[a] + hi_off - lo_off;
+            cnt *= n[3 * b + a];
+        }
+        st[b + 1] = st[b] + cnt;
+    }
+    R.start.upload(st);
+    R.lo.upload(lo);
+    R.n.upload(n);
+    R.h.dev.nb = F.nb();
+    R.h.dev.start = R.start.p;
+    R.h.dev.lo = R.lo.p;
+    R.h.dev.n = R.n.p;
+    R.h.total = st.back();
+}
+
+// ---------------------------------------------------------------------------
+// synchronisation programs
+// ---------------------------------------------------------------------------
+enum OpType { OP_COPY, OP_FILL, OP_SETV, OP_WALLF, OP_ZERO };
+
+struct SyncOp {
+    OpType type;
+    std::vector<int> slots;
+    std::vector<double> fillv;
+    int is_na = 0;
+    std::vector<int> hidx;     // host list (pairs for COPY)
+    std::vector<double> hval;  // SETV values
+    DBuf<int> idx;
+    DBuf<double> val;
+    long long count() const { return type == OP_COPY ? (long long)hidx.size() / 2 : (type == OP_WALLF ? -1 : (long long)hidx.size()); }
+};
+
+struct GroupProg {
+    std::vector<std::unique_ptr<SyncOp>> ops;
+};
+
+enum GroupId { G_UCELL = 0, G_BFACE, G_EEDGE, G_EHALL, G_JEDGE, G_BCELL, G_JCELL, G_BADD, G_NGROUP };
+enum Kind { K_UH, K_UNA, K_BFACE, K_EEDGE, K_EHALL, K_JEDGE, K_BCELL, K_BIND, K_JCELL, K_BADD };
+
+struct GField {
+    int kind, slot, ncomp, loc, level, owner_sync;
+};
+
+// sync groups (Z4_Mercury/src/boundary/Boundary_Build.cpp:564-703) with each
+// field's halo level and owner policy (0_MercuryFieldCatalog.h:77-178)
+static std::vector<GField> group_fields(int gid) {
+    std::vector<GField> v;
+    switch (gid) {
+    case G_UCELL: v = {{K_UH, SL_UH, 5, LOC_CELL, 1, 0}, {K_UNA, SL_UN, 5, LOC_CELL, 1, 0}}; break;
+    case G_BFACE: for (int a = 0; a < 3; ++a) v.push_back({K_BFACE, SL_B + a, 1, LOC_FX + a, 3, 1}); break;
+    case G_BADD: for (int a = 0; a < 3; ++a) v.push_back({K_BADD, SL_BADD + a, 1, LOC_FX + a, 3, 1}); break;
+    case G_EEDGE: for (int a = 0; a < 3; ++a) v.push_back({K_EEDGE, SL_E + a, 1, LOC_EX + a, 3, 1}); break;
+    case G_EHALL: for (int a = 0; a < 3; ++a) v.push_back({K_EHALL, SL_EH + a, 1, LOC_EX + a, 3, 1}); break;
+    case G_JEDGE: for (int a = 0; a < 3; ++a) v.push_back({K_JEDGE, SL_J + a, 1, LOC_EX + a, 3, 1}); break;
+    case G_BCELL: v = {{K_BCELL, SL_BC, 3, LOC_CELL, 3, 0}, {K_BIND, SL_BI, 3, LOC_CELL, 3, 0}}; break;
+    case G_JCELL: v = {{K_JCELL, SL_JC, 3, LOC_CELL, 3, 0}}; break;
+    }
+    return v;
+}
+
+static std::vector<int> slots_of(const GField &f) {
+    std::vector<int> s;
+    for (int c = 0; c < f.ncomp; ++c) s.push_back(f.slot + c);
+    return s;
+}
+
+// Boxes ---------------------------------------------------------------------
+struct Box {
+    int lo[3], hi[3];
+};
+
+// boundary-adjacent inner slab of a full face (LayoutTraits.cpp:434-465)
+static Box inner_slab(const Field &F, int b, int loc, int dir) {
+    int ohi[3];
+    F.owned_hi(b, loc, ohi);
+    Box x;
+    for (int a = 0; a < 3; ++a) {
+        x.lo[a] = 0;
+        x.hi[a] = ohi[a];
+    }
+    const int ax = std::abs(dir) - 1;
+    if (dir < 0) {
+        x.lo[ax] = 0;
+        x.hi[ax] = 1;
+    } else {
+        x.lo[ax] = ohi[ax] - 1;
+        x.hi[ax] = ohi[ax];
+    }
+    return x;
+}
+// ghost slab outside an inner slab (LayoutTraits.cpp:467-485)
+static Box ghost_slab(const Box &in, int dir, int G) {
+    Box x = in;
+    const int ax = std::abs(dir) - 1;
+    if (dir < 0) {
+        x.hi[ax] = in.lo[ax];
+        x.lo[ax] = in.lo[ax] - G;
+    } else {
+        x.lo[ax] = in.hi[ax];
+        x.hi[ax] = in.hi[ax] + G;
+    }
+    return x;
+}
+
+template <class Fn>
+static void for_box(const Box &x, Fn fn) {
+    for (int i = x.lo[0]; i < x.hi[0]; ++i)
+        for (int j = x.lo[1]; j < x.hi[1]; ++j)
+            for (int k = x.lo[2]; k < x.hi[2]; ++k) fn(i, j, k);
+}
+
+static inline int face_dir(int f) { return (f / 2 + 1) * ((f & 1) ? 1 : -1); }
+
+// ---------------------------------------------------------------------------
+// BoundaryCore: physical handlers as index lists
+// ---------------------------------------------------------------------------
+class BoundaryCore {
+public:
+    // BoundaryCore::DefaultPhysicalCopy (src/5_boundary/Boundary_Physical.cpp:288-315)
+    static void default_copy(const Field &F, int b, int loc, int dir, std::vector<int> &pairs) {
+        const Box in = inner_slab(F, b, loc, dir);
+        const Box gs = ghost_slab(in, dir, F.blk[b].g);
+        const int ax = std::abs(dir) - 1;
+        const int ref = dir < 0 ? in.lo[ax] : in.hi[ax] - 1;
+        for_box(gs, [&](int i, int j, int k) {
+            int s[3] = {i, j, k};
+            s[ax] = ref;
+            pairs.push_back(F.gidx(b, i, j, k));
+            pairs.push_back(F.gidx(b, s[0], s[1], s[2]));
+        });
+    }
+    static void ghost_list(const Field &F, int b, int loc, int dir, std::vector<int> &dst) {
+        const Box gs = ghost_slab(inner_slab(F, b, loc, dir), dir, F.blk[b].g);
+        for_box(gs, [&](int i, int j, int k) { dst.push_back(F.gidx(b, i, j, k)); });
+    }
+    // fill_physical_corner_ghost on the 2D / 3D corner ghost boxes of physical
+    // edges and vertices (src/5_boundary/Boundary_Physical.cpp:39-59, 206-264)
+    static void corner_default(const Field &F, int b, int loc, std::vector<int> &pairs) {
+        const MercBlk &k = F.blk[b];
+        int ohi[3];
+        F.owned_hi(b, loc, ohi);
+        const int G = k.g;
+        auto strip = [&](int ax, int side, int &lo, int &hi) {
+            if (side == 0) { lo = -G; hi = 0; }
+            else { lo = ohi[ax]; hi = ohi[ax] + G; }
+        };
+        auto add = [&](const Box &x) {
+            for_box(x, [&](int i, int j, int kk) {
+                const int ii = std::min(std::max(i, 0), ohi[0] - 1), jj = std::min(std::max(j, 0), ohi[1] - 1),
+                          k2 = std::min(std::max(kk, 0), ohi[2] - 1);
+                pairs.push_back(F.gidx(b, i, j, kk));
+                pairs.push_back(F.gidx(b, ii, jj, k2));
+            });
+        };
+        auto phys = [&](int f) { return k.bc[f] != MBC_PERIODIC; };
+        for (int a = 0; a < 3; ++a)
+            for (int c = a + 1; c < 3; ++c)
+                for (int sa = 0; sa < 2; ++sa)
+                    for (int sc = 0; sc < 2; ++sc) {
+                        if (!phys(2 * a + sa) || !phys(2 * c + sc)) continue;
+                        const int t = 3 - a - c;
+                        Box x;
+                        strip(a, sa, x.lo[a], x.hi[a]);
+                        strip(c, sc, x.lo[c], x.hi[c]);
+                        x.lo[t] = 0;
+                        x.hi[t] = ohi[t];
+                        add(x);
+                    }
+        for (int s0 = 0; s0 < 2; ++s0)
+            for (int s1 = 0; s1 < 2; ++s1)
+                for (int s2 = 0; s2 < 2; ++s2) {
+                    if (!phys(s0) || !phys(2 + s1) || !phys(4 + s2)) continue;
+                    Box x;
+                    strip(0, s0, x.lo[0], x.hi[0]);
+                    strip(1, s1, x.lo[1], x.hi[1]);
+                    strip(2, s2, x.lo[2], x.hi[2]);
+                    add(x);
+                }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Halo: periodic self-connections with the reference's staged strip rules
+// (Halo::build_inner_1DCorner_pattern, src/4_halo/build/HaloBuildFace.cpp:6-57;
+// ghost / inner strip boxes, src/0_basic/LayoutTraits.cpp:93-130; copies with the
+// inverse index transform, src/4_halo/component_copy/HaloExchangeInner.cpp:90-117)
+// ---------------------------------------------------------------------------
+class Halo {
+public:
+    static bool periodic(const MercBlk &k, int a) { return k.bc[2 * a] == MBC_PERIODIC && k.bc[2 * a + 1] == MBC_PERIODIC; }
+
+    static void periodic_pairs(const Field &F, int b, int loc, int level, std::vector<int> &pairs) {
+        const MercBlk &k = F.blk[b];
+        int ohi[3];
+        F.owned_hi(b, loc, ohi);
+        const int G = k.g;
+        std::vector<int> axes;
+        for (int a = 0; a < 3; ++a)
+            if (periodic(k, a)) axes.push_back(a);
+        const int na = (int)axes.size();
+        // stage s: every subset of s periodic axes, every side combination
+        for (int s = 1; s <= std::min(level, na); ++s)
+            for (int mask = 0; mask < (1 << na); ++mask) {
+                if (__builtin_popcount(mask) != s) continue;
+                std::vector<int> sel;
+                for (int t = 0; t < na; ++t)
+                    if (mask & (1 << t)) sel.push_back(axes[t]);
+                for (int sides = 0; sides < (1 << s); ++sides) {
+                    Box recv;
+                    int shift[3] = {0, 0, 0};
+                    for (int a = 0; a < 3; ++a) {
+                        recv.lo[a] = 0;
+                        recv.hi[a] = ohi[a];
+                    }
+                    for (int t = 0; t < s; ++t) {
+                        const int a = sel[t], side = (sides >> t) & 1;
+                        if (side == 0) {
+                            recv.lo[a] = -G;
+                            recv.hi[a] = 0;
+                            shift[a] = +k.m[a];
+                        } else {
+                            recv.lo[a] = ohi[a];
+                            recv.hi[a] = ohi[a] + G;
+                            shift[a] = -k.m[a];
+                        }
+                    }
+                    for_box(recv, [&](int i, int j, int kk) {
+                        pairs.push_back(F.gidx(b, i, j, kk));
+                        pairs.push_back(F.gidx(b, i + shift[0], j + shift[1], kk + shift[2]));
+                    });
+                }
+            }
+    }
+
+    // owner -> alias (Halo::sync_owner_alias_group, src/4_halo/core/HaloSyncFramework.cpp:695-712):
+    // the seam DOF at index m on a periodic axis (delta = 0) aliases the owner,
+    // the smallest EntityKey of its class, i.e. every seam index set to 0
+    static void owner_pairs(const Field &F, int b, int loc, std::vector<int> &pairs) {
+        const MercBlk &k = F.blk[b];
+        int ohi[3];
+        F.owned_hi(b, loc, ohi);
+        for (int i = 0; i < ohi[0]; ++i)
+            for (int j = 0; j < ohi[1]; ++j)
+                for (int kk = 0; kk < ohi[2]; ++kk) {
+                    int idx[3] = {i, j, kk}, o[3] = {i, j, kk};
+                    bool alias = false;
+                    for (int a = 0; a < 3; ++a)
+                        if (DELTA[loc][a] == 0 && periodic(k, a) && idx[a] == k.m[a]) {
+                            o[a] = 0;
+                            alias = true;
+                        }
+                    if (alias) {
+                        pairs.push_back(F.gidx(b, i, j, kk));
+                        pairs.push_back(F.gidx(b, o[0], o[1], o[2]));
+                    }
+                }
+    }
+};
+
+// edge alias classes (ReduceEdgeAliasCandidatesToOwners_, MercurySolver_Current.cpp:3-57)
+struct AliasClasses {
+    DBuf<int> start, member, msign, owner, osign, slot_of;
+    long long n = 0;
+    void build(const Field &F) {
+        std::vector<int> st{0}, mem, ms, ow, os, sl;
+        for (int b = 0; b < F.nb(); ++b) {
+            const MercBlk &k = F.blk[b];
+            for (int a = 0; a < 3; ++a) {
+                const int loc = LOC_EX + a;
+                int ohi[3];
+                F.owned_hi(b, loc, ohi);
+                bool per[3];
+                for (int c = 0; c < 3; ++c) per[c] = DELTA[loc][c] == 0 && Halo::periodic(k, c);
+                for (int i = 0; i < ohi[0]; ++i)
+                    for (int j = 0; j < ohi[1]; ++j)
+                        for (int kk = 0; kk < ohi[2]; ++kk) {
+                            int idx[3] = {i, j, kk}, seam[3], ns = 0;
+                            bool owner = true;
+                            for (int c = 0; c < 3; ++c) {
+                                if (!per[c]) continue;
+                                if (idx[c] == k.m[c]) owner = false;
+                                if (idx[c] == 0) seam[ns++] = c;
+                            }
+                            if (!owner || ns == 0) continue;
+                            // members in ascending (i, j, k) order
+                            for (int mask = 0; mask < (1 << ns); ++mask) {
+                                int q[3] = {i, j, kk};
+                                for (int t = 0; t < ns; ++t)
+                                    if (mask & (1 << (ns - 1 - t))) q[seam[t]] = k.m[seam[t]];
+                                mem.push_back(F.gidx(b, q[0], q[1], q[2]));
+                                ms.push_back(1);
+                            }
+                            st.push_back((int)mem.size());
+                            ow.push_back(F.gidx(b, i, j, kk));
+                            os.push_back(1);
+                            sl.push_back(a);
+                        }
+            }
+        }
+        n = (long long)ow.size();
+        start.upload(st);
+        member.upload(mem);
+        msign.upload(ms);
+        owner.upload(ow);
+        osign.upload(os);
+        slot_of.upload(sl);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// MercurySolver
+// ---------------------------------------------------------------------------
+class MercurySolver {
+public:
+    std::unique_ptr<Field> F;
+    MercPhys P{};
+    std::vector<std::unique_ptr<BlockGeom>> geom;
+    RangeOwner R_cell, R_box, R_node, R_eface;
+    GroupProg prog[G_NGROUP];
+    AliasClasses alias;
+    hipStream_t st = nullptr;
+    unsigned long long *d_dt = nullptr;       // dt word (double bits)
+    unsigned long long *d_red = nullptr;      // reduction scratch (2 words)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_integrator = -1;
+
+    MercurySolver(int nb, const int *dims, int g, const int *bc, const double *nodes, const MercPhys &phys) : P(phys) {
+        if (g < 2) throw std::runtime_error("nghost must be >= 2");
+        bool any_per = false, any_phys = false;
+        for (int b = 0; b < nb; ++b) {
+            bool bp = false, bq = false;
+            for (int f = 0; f < 6; ++f) {
+                const int c = bc[6 * b + f];
+                if (c < 0 || c > 2) throw std::runtime_error("unknown boundary code");
+                if (c == MBC_PERIODIC) bp = true; else bq = true;
+                if (c == MBC_PERIODIC && bc[6 * b + (f ^ 1)] != MBC_PERIODIC) throw std::runtime_error("periodic faces must be paired");
+            }
+            if (bp && bq) throw std::runtime_error("blocks mixing periodic and physical faces are not supported");
+            any_per |= bp;
+            any_phys |= bq;
+            for (int a = 0; a < 3; ++a)
+                if (dims[3 * b + a] < 2) throw std::runtime_error("every block needs at least 2 cells per axis");
+        }
+        (void)any_per;
+        (void)any_phys;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        F.reset(new Field(nb, dims, g, bc));
+        HIPCHK(hipMalloc(&d_dt, sizeof(unsigned long long)));
+        HIPCHK(hipMalloc(&d_red, 2 * sizeof(unsigned long long)));
+        // static geometry per block
+        long long noff = 0;
+        for (int b = 0; b < nb; ++b) {
+            const int m[3] = {dims[3 * b], dims[3 * b + 1], dims[3 * b + 2]};
+            geom.emplace_back(new BlockGeom(m, g, nodes + noff));
+            noff += 3LL * (m[0] + 1) * (m[1] + 1) * (m[2] + 1);
+            struct W : SlotWriter {
+                Field *F;
+                int b;
+                void put(int slot, const std::vector<double> &v) override { F->upload_block(slot, b, v.data()); }
+            } w;
+            w.F = F.get();
+            w.b = b;
+            geom[b]->compute(P, w);
+        }
+        make_range(*F, 0, 0, R_cell);
+        make_range(*F, -g, g + 1, R_box);
+        make_range(*F, 0, 1, R_node);
+        make_range(*F, -1, 2, R_eface);
+        build_programs();
+        alias.build(*F);
+    }
+    ~MercurySolver() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (d_dt) (void)hipFree(d_dt);
+        if (d_red) (void)hipFree(d_red);
+        F.reset();
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    double *const *Fd() const { return F->dptr; }
+
+    // Sync programs: for every group the physical handlers (far field before
+    // the wall, the order of the sorted physical patch list), the corner
+    // defaults, the staged periodic halo and the owner -> alias copies
+    // (MercuryBoundary::Sync_, Z4_Mercury/src/boundary/Boundary_Sync_.cpp:4-110;
+    // handlers Boundary_Build.cpp:5-300)
+    void build_programs() {
+        const Field &Fl = *F;
+        for (int gid = 0; gid < G_NGROUP; ++gid) {
+            std::vector<std::unique_ptr<SyncOp>> A, Bc, C, D, E;
+            auto add = [](std::vector<std::unique_ptr<SyncOp>> &ph, OpType t, const std::vector<int> &slots, std::vector<int> &&idx,
+                          std::vector<double> &&val = {}, const double *fill = nullptr, int is_na = 0) {
+                if (idx.empty()) return;
+                if (t == OP_COPY)
+                    for (auto &o : ph)
+                        if (o->type == OP_COPY && o->hidx == idx && o->slots.size() + slots.size() <= MERC_MAX_SYNC_SLOTS) {
+                            o->slots.insert(o->slots.end(), slots.begin(), slots.end());
+                            return;
+                        }
+                std::unique_ptr<SyncOp> o(new SyncOp());
+                o->type = t;
+                o->slots = slots;
+                o->hidx = std::move(idx);
+                o->hval = std::move(val);
+                o->is_na = is_na;
+                if (fill) o->fillv.assign(fill, fill + slots.size());
+                ph.push_back(std::move(o));
+            };
+            for (const GField &gf : group_fields(gid)) {
+                const auto sl = slots_of(gf);
+                std::vector<int> fillA, copyB, setd, wallC, zeroC, copyD, ownE;
+                std::vector<double> setv;
+                const double *fill = nullptr;
+                static const double zero3[3] = {0, 0, 0};
+                for (int b = 0; b < Fl.nb(); ++b) {
+                    const MercBlk &k = Fl.blk[b];
+                    for (int f = 0; f < 6; ++f) {
+                        if (k.bc[f] != MBC_FARFIELD) continue;
+                        const int dir = face_dir(f);
+                        switch (gf.kind) {
+                        case K_UH: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = P.qinf; break;
+                        case K_UNA: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = P.qinfs; break;
+                        case K_BFACE: break;
+                        case K_BCELL: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = P.B_imf; break;
+                        case K_BIND: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = zero3; break;
+                        case K_EEDGE: {
+                            // BC_Farfield_Eedge_set_zerocurl (Boundary_Condition.cpp:573-623)
+                            const double *u = P.q_pv_inf, *Bi = P.B_imf;
+                            const double Ei[3] = {Bi[1] * u[2] - Bi[2] * u[1], Bi[2] * u[0] - Bi[0] * u[2], Bi[0] * u[1] - Bi[1] * u[0]};
+                            const int a = gf.loc - LOC_EX;
+                            const BlockGeom &G = *geom[b];
+                            for_box(inner_slab(Fl, b, gf.loc, dir), [&](int i, int j, int kk) {
+                                const auto r0 = G.node(i, j, kk), r1 = G.node(i + (a == 0), j + (a == 1), kk + (a == 2));
+                                setd.push_back(Fl.gidx(b, i, j, kk));
+                                setv.push_back(Ei[0] * (r1[0] - r0[0]) + Ei[1] * (r1[1] - r0[1]) + Ei[2] * (r1[2] - r0[2]));
+                            });
+                            BoundaryCore::default_copy(Fl, b, gf.loc, dir, copyB);
+                            break;
+                        }
+                        default: BoundaryCore::default_copy(Fl, b, gf.loc, dir, copyB); break;
+                        }
+                    }
+                    for (int f = 0; f < 6; ++f) {
+                        if (k.bc[f] != MBC_WALL) continue;
+                        const int dir = face_dir(f);
+                        const int ax = std::abs(dir) - 1, sgn = dir > 0 ? 1 : -1;
+                        if (gf.kind == K_UH || gf.kind == K_UNA) {
+                            // BC_Solid_Surface_ (Boundary_Condition_Fluid.cpp:36-260): (first, deeper, ghosts)
+                            int lo[3], hi[3];
+                            for (int a = 0; a < 3; ++a) { lo[a] = -k.g; hi[a] = k.m[a] + k.g; }
+                            auto cl = [&](int v, int a) { return std::min(std::max(v, lo[a]), hi[a] - 1); };
+                            for_box(inner_slab(Fl, b, LOC_CELL, dir), [&](int i, int j, int kk) {
+                                int c[3] = {i, j, kk};
+                                wallC.push_back(Fl.gidx(b, i, j, kk));
+                                int r[3] = {c[0], c[1], c[2]};
+                                r[ax] -= sgn;
+                                wallC.push_back(Fl.gidx(b, cl(r[0], 0), cl(r[1], 1), cl(r[2], 2)));
+                                for (int l = 1; l <= k.g; ++l) {
+                                    int q[3] = {c[0], c[1], c[2]};
+                                    q[ax] += l * sgn;
+                                    wallC.push_back(Fl.gidx(b, cl(q[0], 0), cl(q[1], 1), cl(q[2], 2)));
+                                }
+                            });
+                        } else if (gf.kind == K_JCELL) {
+                            // BC_Solid_Surface_Jcell (Boundary_Condition_EM.cpp:382-427)
+                            for_box(inner_slab(Fl, b, LOC_CELL, dir), [&](int i, int j, int kk) {
+                                int c[3] = {i, j, kk};
+                                zeroC.push_back(Fl.gidx(b, i, j, kk));
+                                c[ax] -= sgn;
+                                zeroC.push_back(Fl.gidx(b, c[0], c[1], c[2]));
+                            });
+                            BoundaryCore::default_copy(Fl, b, gf.loc, dir, copyD);
+                        }
+                    }
+                    BoundaryCore::corner_default(Fl, b, gf.loc, copyD);
+                    Halo::periodic_pairs(Fl, b, gf.loc, gf.level, copyD);
+                    if (gf.owner_sync) Halo::owner_pairs(Fl, b, gf.loc, ownE);
+                }
+                if (!fillA.empty()) add(A, OP_FILL, sl, std::move(fillA), {}, fill);
+                if (!setd.empty()) add(A, OP_SETV, sl, std::move(setd), std::move(setv));
+                add(Bc, OP_COPY, sl, std::move(copyB));
+                if (!wallC.empty()) add(C, OP_WALLF, sl, std::move(wallC), {}, nullptr, gf.kind == K_UNA);
+                if (!zeroC.empty()) {
+                    static const double z[10] = {0};
+                    add(C, OP_FILL, sl, std::move(zeroC), {}, z);
+                }
+                add(D, OP_COPY, sl, std::move(copyD));
+                add(E, OP_COPY, sl, std::move(ownE));
+            }
+            for (auto *ph : {&A, &Bc, &C, &D, &E})
+                for (auto &o : *ph) {
+                    o->idx.upload(o->hidx);
+                    if (!o->hval.empty()) o->val.upload(o->hval);
+                    prog[gid].ops.push_back(std::move(o));
+                }
+        }
+    }
+
+    void sync(int gid) {
+        for (auto &o : prog[gid].ops) {
+            switch (o->type) {
+            case OP_COPY: launch_copy(st, o->idx.p, o->idx.n / 2, Fd(), o->slots.data(), (int)o->slots.size()); break;
+            case OP_FILL: launch_fill(st, o->idx.p, o->idx.n, Fd(), o->slots.data(), (int)o->slots.size(), o->fillv.data()); break;
+            case OP_SETV: launch_setv(st, o->idx.p, o->val.p, o->idx.n, F->hptr[o->slots[0]]); break;
+            case OP_WALLF: {
+                const int g = F->blk[0].g;
+                launch_wall_fluid(st, o->idx.p, o->idx.n / (2 + g), g, Fd(), o->slots[0], o->is_na, P);
+                break;
+            }
+            default: break;
+            }
+        }
+    }
+
+    const MercBlk *Bd() const { return F->dblk; }
+    const double *dtp() const { return reinterpret_cast<const double *>(d_dt); }
+
+    // UpdateDerivedFields_ (MercurySolver_Tools.cpp:303-321)
+    void derived() {
+        launch_bcell(st, R_cell.h, Bd(), Fd());
+        sync(G_BCELL);
+        launch_jedge(st, R_node.h, Bd(), Fd());
+        launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
+        sync(G_JEDGE);
+        launch_jcell(st, R_cell.h, Bd(), Fd());
+        sync(G_JCELL);
+        launch_pv_uplus(st, R_box.h, Bd(), Fd(), P);
+    }
+
+    // AssembleRHS_Induction_CT_ (MercurySolver_MagneticRHS.cpp:90-171) up to Sync(Eedge)
+    void induction_emf() {
+        const int es[3] = {SL_E, SL_E + 1, SL_E + 2};
+        launch_zero_span(st, Fd(), es, 3, F->Vtot);
+        launch_eface(st, R_eface.h, Bd(), Fd());
+        launch_edge_emf(st, R_node.h, Bd(), Fd(), P);
+        launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_E);
+        launch_hall_cell(st, R_box.h, Bd(), Fd(), P);
+        launch_hall_edge(st, R_node.h, Bd(), Fd());
+        sync(G_EHALL);
+        launch_add_hall_wall(st, R_node.h, Bd(), Fd());
+        sync(G_EEDGE);
+    }
+
+    // StepOnce (MercurySolver_Driver.cpp:21-61) without the time-step computation
+    void stage() {
+        launch_flux_div(st, R_cell.h, Bd(), Fd(), P);
+        launch_sources(st, R_cell.h, Bd(), Fd(), P);
+        induction_emf();
+        launch_curl_update(st, R_node.h, Bd(), Fd(), dtp(), 1);
+        launch_update_u(st, R_cell.h, Bd(), Fd(), dtp());
+        sync(G_UCELL);
+        sync(G_BFACE);
+        derived();
+    }
+
+    void enqueue_step(int integrator) {
+        launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt);
+        if (integrator == 0) {
+            stage();
+        } else {
+            launch_rk2(st, R_node.h, Bd(), Fd(), 1);
+            stage();
+            stage();
+            launch_rk2(st, R_node.h, Bd(), Fd(), 0);
+            sync(G_UCELL);
+            sync(G_BFACE);
+            derived();
+        }
+    }
+
+    double read_dt() {
+        unsigned long long bits;
+        HIPCHK(hipMemcpyAsync(&bits, d_dt, sizeof(bits), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        double dt;
+        std::memcpy(&dt, &bits, sizeof(dt));
+        return dt;
+    }
+
+    void check_dt(double dt) {
+        // Compute_Timestep abort rule (MercurySolver_Timestep.cpp:129-140)
+        if (!std::isfinite(dt) || dt < 1e-7) throw std::runtime_error("dt too small/NaN (dt underflow)");
+    }
+
+    void step(int n, int integrator, double *dt_out) {
+        for (int s = 0; s < n; ++s) {
+            enqueue_step(integrator);
+            if (dt_out) {
+                dt_out[s] = read_dt();
+                check_dt(dt_out[s]);
+            }
+        }
+        HIPCHK(hipGetLastError());
+        if (!dt_out && n > 0) check_dt(read_dt());
+    }
+
+    void capture(int integrator) {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        enqueue_step(integrator);
+        HIPCHK(hipStreamEndCapture(st, &graph));
+        HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        graph_integrator = integrator;
+    }
+    void graph_launch(int n) {
+        if (!gexec) throw std::runtime_error("no captured step graph");
+        for (int s = 0; s < n; ++s) HIPCHK(hipGraphLaunch(gexec, st));
+    }
+
+    void init_bface(const double Bu[3], const double *const *A) {
+        for (int b = 0; b < F->nb(); ++b) {
+            const BlockGeom &G = *geom[b];
+            const long long V = G.V;
+            const double *Ax = A ? A[3 * b] : nullptr, *Ay = A ? A[3 * b + 1] : nullptr, *Az = A ? A[3 * b + 2] : nullptr;
+            auto AA = [&](const double *p, int i, int j, int k) { return p ? p[G.idx(i, j, k)] : 0.0; };
+            for (int a = 0; a < 3; ++a) {
+                std::vector<double> Bf(V, 0.0), Ba(V, 0.0);
+                int lo[3], hi[3];
+                for (int c = 0; c < 3; ++c) {
+                    lo[c] = -G.g;
+                    hi[c] = G.m[c] + 1 - DELTA[LOC_FX + a][c] + G.g;
+                }
+                for (int i = lo[0]; i < hi[0]; ++i)
+                    for (int j = lo[1]; j < hi[1]; ++j)
+                        for (int k = lo[2]; k < hi[2]; ++k) {
+                            const long long q = G.idx(i, j, k);
+                            const double j0 = G.JD_[3 * a][q], j1 = G.JD_[3 * a + 1][q], j2 = G.JD_[3 * a + 2][q];
+                            double c;
+                            if (a == 0) c = (AA(Ay, i, j, k) - AA(Ay, i, j, k + 1)) + (AA(Az, i, j + 1, k) - AA(Az, i, j, k));
+                            else if (a == 1) c = (AA(Ax, i, j, k + 1) - AA(Ax, i, j, k)) + (AA(Az, i, j, k) - AA(Az, i + 1, j, k));
+                            else c = (AA(Ax, i, j, k) - AA(Ax, i, j + 1, k)) + (AA(Ay, i + 1, j, k) - AA(Ay, i, j, k));
+                            Bf[q] = (Bu[0] * j0 + Bu[1] * j1 + Bu[2] * j2) + c;
+                            Ba[q] = P.B_imf[0] * j0 + P.B_imf[1] * j1 + P.B_imf[2] * j2;
+                        }
+                F->upload_block(SL_B + a, b, Bf.data());
+                F->upload_block(SL_BADD + a, b, Ba.data());
+            }
+        }
+    }
+
+    // MercurySolver ctor tail + Advance head (MercurySolver.cpp:141-145, MercurySolver_Driver.cpp:6-19)
+    void prepare() {
+        sync(G_BADD);
+        sync(G_BFACE);
+        derived();
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
+    void run_op(const std::string &op, double arg) {
+        if (op == "pv_uplus") launch_pv_uplus(st, R_box.h, Bd(), Fd(), P);
+        else if (op == "bcell") launch_bcell(st, R_cell.h, Bd(), Fd());
+        else if (op == "jedge") launch_jedge(st, R_node.h, Bd(), Fd());
+        else if (op == "jcell") launch_jcell(st, R_cell.h, Bd(), Fd());
+        else if (op == "derived") derived();
+        else if (op == "flux") launch_flux_div(st, R_cell.h, Bd(), Fd(), P);
+        else if (op == "sources") launch_sources(st, R_cell.h, Bd(), Fd(), P);
+        else if (op == "eface") launch_eface(st, R_eface.h, Bd(), Fd());
+        else if (op == "edge_emf") launch_edge_emf(st, R_node.h, Bd(), Fd(), P);
+        else if (op == "hall_cell") launch_hall_cell(st, R_box.h, Bd(), Fd(), P);
+        else if (op == "hall_edge") launch_hall_edge(st, R_node.h, Bd(), Fd());
+        else if (op == "induction") {
+            induction_emf();
+            launch_curl_update(st, R_node.h, Bd(), Fd(), dtp(), 0);
+        } else if (op == "alias_e")
+            launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_E);
+        else if (op == "alias_j")
+            launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
+        else if (op == "timestep") launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt);
+        else if (op == "stage") stage();
+        else if (op == "dec_d0") launch_dec_d0(st, R_node.h, Bd(), Fd(), SL_NODEF, SL_EDGEF);
+        else if (op == "dec_d1") launch_dec_d1(st, R_node.h, Bd(), Fd(), SL_EDGEF, SL_FACEF, arg == 0.0 ? 1.0 : arg);
+        else if (op == "dec_d2") launch_dec_d2(st, R_cell.h, Bd(), Fd(), SL_FACEF, SL_CELLS);
+        else if (op == "sync_ucell") sync(G_UCELL);
+        else if (op == "sync_bface") sync(G_BFACE);
+        else if (op == "sync_eedge") sync(G_EEDGE);
+        else if (op == "sync_ehall") sync(G_EHALL);
+        else if (op == "sync_jedge") sync(G_JEDGE);
+        else if (op == "sync_bcell") sync(G_BCELL);
+        else if (op == "sync_jcell") sync(G_JCELL);
+        else if (op == "sync_badd") sync(G_BADD);
+        else throw std::runtime_error("unknown op: " + op);
+        HIPCHK(hipGetLastError());
+    }
+
+    void divb(double *norm, double *maxabs) {
+        HIPCHK(hipMemsetAsync(d_red, 0, 2 * sizeof(unsigned long long), st));
+        launch_divb(st, R_node.h, Bd(), Fd(), d_red);
+        unsigned long long h[2];
+        HIPCHK(hipMemcpyAsync(h, d_red, sizeof(h), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        double dm, bm;
+        std::memcpy(&dm, &h[0], 8);
+        std::memcpy(&bm, &h[1], 8);
+        if (maxabs) *maxabs = dm;
+        if (norm) *norm = bm > 0.0 ? dm / bm : dm;
+    }
+
+    // average device time of one op over reps launches, HIP events on the solver stream
+    double time_op(const std::string &op, int reps) {
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        run_op(op, 0.0);
+        HIPCHK(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) {
+            if (op == "step_graph") graph_launch(1);
+            else run_op(op, 0.0);
+        }
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return (double)ms / reps;
+    }
+};
+
+}  // namespace merc
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+struct MercuryContext {
+    std::unique_ptr<merc::MercurySolver> s;
+};
+
+#define ABI_TRY(body)                                  \
+    try {                                              \
+        body;                                          \
+        return 0;                                      \
+    } catch (const std::exception &e) {                \
+        merc::g_last_error = e.what();                 \
+        return -1;                                     \
+    } catch (...) {                                    \
+        merc::g_last_error = "unknown error";          \
+        return -1;                                     \
+    }
+
+#define CHECK_CTX(c)                                                   \
+    if (!(c) || !(c)->s) {                                             \
+        merc::g_last_error = "null or destroyed context";              \
+        return -1;                                                     \
+    }
+
+extern "C" {
+
+const char *mercury_last_error(void) { return merc::g_last_error.c_str(); }
+
+int mercury_version(void) { return MERCURY_HIP_ABI_VERSION; }
+
+int mercury_create(MercuryContext **out, int nblocks, const int *dims, int nghost, const int *bc, const double *owned_nodes,
+                   const MercuryPhysics *phys) {
+    if (!out || nblocks <= 0 || !dims || !bc || !owned_nodes || !phys) {
+        merc::g_last_error = "mercury_create: invalid argument";
+        return -1;
+    }
+    ABI_TRY({
+        MercPhys P;
+        std::memcpy(&P, phys, sizeof(P));
+        std::unique_ptr<MercuryContext> c(new MercuryContext());
+        c->s.reset(new merc::MercurySolver(nblocks, dims, nghost, bc, owned_nodes, P));
+        *out = c.release();
+    })
+}
+
+int mercury_destroy(MercuryContext *ctx) {
+    ABI_TRY({ delete ctx; })
+}
+
+int mercury_field_info(const char *name, int *slot, int *ncomp, int *location) {
+    const merc::FieldSpec *f = name ? merc::find_field(name) : nullptr;
+    if (!f) {
+        merc::g_last_error = std::string("unknown field: ") + (name ? name : "(null)");
+        return -1;
+    }
+    if (slot) *slot = f->slot;
+    if (ncomp) *ncomp = f->ncomp;
+    if (location) *location = f->loc;
+    return 0;
+}
+
+long long mercury_block_volume(MercuryContext *ctx, int block) {
+    if (!ctx || !ctx->s || block < 0 || block >= ctx->s->F->nb()) return -1;
+    return ctx->s->F->vol(block);
+}
+
+static int field_comp_slot(const char *name, int comp) {
+    const merc::FieldSpec *f = name ? merc::find_field(name) : nullptr;
+    if (!f) throw std::runtime_error(std::string("unknown field: ") + (name ? name : "(null)"));
+    if (comp < 0 || comp >= f->ncomp) throw std::runtime_error("component out of range");
+    return f->slot + comp;
+}
+
+int mercury_upload(MercuryContext *ctx, const char *name, int block, int comp, const double *host) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        if (block < 0 || block >= ctx->s->F->nb() || !host) throw std::runtime_error("invalid block or buffer");
+        HIPCHK(hipStreamSynchronize(ctx->s->st));
+        ctx->s->F->upload_block(field_comp_slot(name, comp), block, host);
+    })
+}
+
+int mercury_download(MercuryContext *ctx, const char *name, int block, int comp, double *host) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        if (block < 0 || block >= ctx->s->F->nb() || !host) throw std::runtime_error("invalid block or buffer");
+        HIPCHK(hipStreamSynchronize(ctx->s->st));
+        ctx->s->F->download_block(field_comp_slot(name, comp), block, host);
+    })
+}
+
+int mercury_init_bface(MercuryContext *ctx, const double *Bu, const double *const *A) {
+    CHECK_CTX(ctx);
+    if (!Bu) {
+        merc::g_last_error = "mercury_init_bface: Bu is null";
+        return -1;
+    }
+    ABI_TRY({
+        HIPCHK(hipStreamSynchronize(ctx->s->st));
+        ctx->s->init_bface(Bu, A);
+    })
+}
+
+int mercury_prepare(MercuryContext *ctx) {
+    CHECK_CTX(ctx);
+    ABI_TRY({ ctx->s->prepare(); })
+}
+
+int mercury_step(MercuryContext *ctx, int nsteps, int integrator, double *dt_out) {
+    CHECK_CTX(ctx);
+    if (nsteps < 0 || (integrator != 0 && integrator != 1)) {
+        merc::g_last_error = "mercury_step: invalid nsteps or integrator";
+        return -1;
+    }
+    ABI_TRY({ ctx->s->step(nsteps, integrator, dt_out); })
+}
+
+int mercury_op(MercuryContext *ctx, const char *op, double arg) {
+    CHECK_CTX(ctx);
+    if (!op) {
+        merc::g_last_error = "mercury_op: null op";
+        return -1;
+    }
+    ABI_TRY({ ctx->s->run_op(op, arg); })
+}
+
+int mercury_timestep(MercuryContext *ctx, double *dt) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        ctx->s->run_op("timestep", 0.0);
+        const double v = ctx->s->read_dt();
+        if (dt) *dt = v;
+    })
+}
+
+int mercury_divb(MercuryContext *ctx, double *normalised, double *maxabs) {
+    CHECK_CTX(ctx);
+    ABI_TRY({ ctx->s->divb(normalised, maxabs); })
+}
+
+int mercury_synchronize(MercuryContext *ctx) {
+    CHECK_CTX(ctx);
+    ABI_TRY({ HIPCHK(hipStreamSynchronize(ctx->s->st)); })
+}
+
+int mercury_graph_capture(MercuryContext *ctx, int integrator) {
+    CHECK_CTX(ctx);
+    ABI_TRY({ ctx->s->capture(integrator); })
+}
+
+int mercury_graph_launch(MercuryContext *ctx, int nsteps) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        ctx->s->graph_launch(nsteps);
+        HIPCHK(hipGetLastError());
+    })
+}
+
+int mercury_time_op(MercuryContext *ctx, const char *op, int reps, double *ms_avg) {
+    CHECK_CTX(ctx);
+    if (!op || reps <= 0 || !ms_avg) {
+        merc::g_last_error = "mercury_time_op: invalid argument";
+        return -1;
+    }
+    ABI_TRY({ *ms_avg = ctx->s->time_op(op, reps); })
+}
+
+int mercury_geometry_block(const int *dims, int nghost, const double *owned_nodes, const MercuryPhysics *phys, const char *name,
+                           int comp, double *out) {
+    if (!dims || !owned_nodes || !phys || !name || !out || nghost < 2) {
+        merc::g_last_error = "mercury_geometry_block: invalid argument";
+        return -1;
+    }
+    ABI_TRY({
+        MercPhys P;
+        std::memcpy(&P, phys, sizeof(P));
+        const int slot = field_comp_slot(name, comp);
+        merc::BlockGeom G(dims, nghost, owned_nodes);
+        struct W : merc::SlotWriter {
+            int want;
+            double *out;
+            long long V;
+            void put(int s, const std::vector<double> &v) override {
+                if (s == want) std::memcpy(out, v.data(), sizeof(double) * V);
+            }
+        } w;
+        w.want = slot;
+        w.out = out;
+        w.V = G.V;
+        G.compute(P, w);
+    })
+}
+
+long long mercury_total_volume(MercuryContext *ctx) {
+    if (!ctx || !ctx->s) return -1;
+    return ctx->s->F->Vtot;
+}
+
+}  // extern "C"

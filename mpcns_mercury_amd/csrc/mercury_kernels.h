@@ -1,0 +1,43 @@
+// Host-side launch wrappers of the FP64 kernels (mercury_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mercury_common.h"
+
+namespace merc {
+
+// a device MercRange plus its total thread count
+struct RangeH {
+    MercRange dev{};
+    long long total = 0;
+};
+
+void launch_pv_uplus(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P);
+void launch_bcell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
+void launch_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
+void launch_jcell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
+void launch_timestep(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P, unsigned long long *dt_bits);
+void launch_flux_div(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P);
+void launch_sources(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P);
+void launch_eface(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
+void launch_edge_emf(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P);
+void launch_hall_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P);
+void launch_hall_edge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
+void launch_add_hall_wall(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
+void launch_curl_update(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const double *dt, int do_update);
+void launch_update_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const double *dt);
+void launch_rk2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int save);
+void launch_dec_d0(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int s_node, int s_edge);
+void launch_dec_d1(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int s_edge, int s_face, double mult);
+void launch_dec_d2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int s_face, int s_cell);
+void launch_divb(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, unsigned long long *out);
+
+void launch_copy(hipStream_t st, const int *pairs, long long n, double *const *F, const int *slots, int ns);
+void launch_fill(hipStream_t st, const int *dst, long long n, double *const *F, const int *slots, int ns, const double *vals);
+void launch_setv(hipStream_t st, const int *dst, const double *val, long long n, double *f);
+void launch_alias_reduce(hipStream_t st, const int *cls_start, const int *member, const int *msign, const int *owner, const int *osign,
+                         const int *slot_of, long long ncls, double *const *F, int slot0);
+void launch_wall_fluid(hipStream_t st, const int *ent, long long n, int g, double *const *F, int slot, int is_na, const MercPhys &P);
+void launch_zero_span(hipStream_t st, double *const *F, const int *slots, int ns, long long n);
+
+}  // namespace merc

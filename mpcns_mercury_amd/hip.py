@@ -1,0 +1,185 @@
+"""ctypes front end of ``libmercury_hip.so`` (the C ABI of include/mercury_hip.h).
+
+The HIP library is the only compute path: if it is missing or no GPU is
+present, every entry point raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import MercuryPhysics
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmercury_hip.so")
+_lib = None
+
+# every symbol declared in include/mercury_hip.h
+ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercury_destroy", "mercury_field_info",
+               "mercury_block_volume", "mercury_total_volume", "mercury_upload", "mercury_download",
+               "mercury_init_bface", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
+               "mercury_divb", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
+               "mercury_time_op", "mercury_geometry_block")
+
+
+class MercuryError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the in-tree HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MercuryError(f"{LIB_PATH} is missing: run __graft_entry__.build() (or python -m mpcns_mercury_amd.build)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, dp, cp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), ctypes.c_char_p
+    L.mercury_last_error.restype = ctypes.c_char_p
+    L.mercury_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ip, ctypes.c_int, ip, dp, ctypes.POINTER(MercuryPhysics)]
+    L.mercury_destroy.argtypes = [vp]
+    L.mercury_field_info.argtypes = [cp, ip, ip, ip]
+    L.mercury_block_volume.restype = ctypes.c_longlong
+    L.mercury_block_volume.argtypes = [vp, ctypes.c_int]
+    L.mercury_total_volume.restype = ctypes.c_longlong
+    L.mercury_total_volume.argtypes = [vp]
+    L.mercury_upload.argtypes = [vp, cp, ctypes.c_int, ctypes.c_int, dp]
+    L.mercury_download.argtypes = [vp, cp, ctypes.c_int, ctypes.c_int, dp]
+    L.mercury_init_bface.argtypes = [vp, dp, ctypes.POINTER(dp)]
+    L.mercury_prepare.argtypes = [vp]
+    L.mercury_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+    L.mercury_op.argtypes = [vp, cp, ctypes.c_double]
+    L.mercury_timestep.argtypes = [vp, dp]
+    L.mercury_divb.argtypes = [vp, dp, dp]
+    L.mercury_synchronize.argtypes = [vp]
+    L.mercury_graph_capture.argtypes = [vp, ctypes.c_int]
+    L.mercury_graph_launch.argtypes = [vp, ctypes.c_int]
+    L.mercury_time_op.argtypes = [vp, cp, ctypes.c_int, dp]
+    L.mercury_geometry_block.argtypes = [ip, ctypes.c_int, dp, ctypes.POINTER(MercuryPhysics), cp, ctypes.c_int, dp]
+    _lib = L
+    return L
+
+
+def _chk(rc):
+    if rc != 0:
+        raise MercuryError(load().mercury_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def field_info(name):
+    s, n, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _chk(load().mercury_field_info(name.encode(), ctypes.byref(s), ctypes.byref(n), ctypes.byref(l)))
+    return s.value, n.value, l.value
+
+
+def geometry_block(case, ib, name):
+    """Host-side static geometry of one block (no GPU involved)."""
+    _, nc, _ = field_info(name)
+    b = case.blocks[ib]
+    dims = np.array(b.dims, dtype=np.int32)
+    nodes = np.ascontiguousarray(b.nodes, dtype=np.float64).reshape(-1)
+    shape = case.box(ib)
+    out = np.empty((nc,) + shape)
+    for c in range(nc):
+        buf = np.empty(shape)
+        _chk(load().mercury_geometry_block(dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), case.g, _dp(nodes),
+                                           ctypes.byref(case.phys), name.encode(), c, _dp(buf)))
+        out[c] = buf
+    return out
+
+
+class Mercury:
+    """All blocks of one case resident on the GPU."""
+
+    def __init__(self, case):
+        L = load()
+        self.case = case
+        self.phys = case.phys
+        self._h = ctypes.c_void_p()
+        dims = case.dims_array()
+        bc = case.bc_array()
+        nodes = case.nodes_flat()
+        _chk(L.mercury_create(ctypes.byref(self._h), case.nb, dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), case.g,
+                              bc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(nodes), ctypes.byref(self.phys)))
+        for ib, fl in enumerate(case.fields):
+            for name, arr in fl.items():
+                self.set(name, ib, arr)
+        Bu = np.array(case.Bu, dtype=np.float64)
+        if case.A is not None:
+            flat = [np.ascontiguousarray(case.A[ib][a], dtype=np.float64) for ib in range(case.nb) for a in range(3)]
+            self._A = flat
+            ptrs = (ctypes.POINTER(ctypes.c_double) * len(flat))(*[_dp(x) for x in flat])
+            _chk(L.mercury_init_bface(self._h, _dp(Bu), ptrs))
+        else:
+            _chk(L.mercury_init_bface(self._h, _dp(Bu), None))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            load().mercury_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get(self, name, ib=0):
+        _, nc, _ = field_info(name)
+        shape = self.case.box(ib)
+        out = np.empty((nc,) + shape)
+        for c in range(nc):
+            buf = np.empty(shape)
+            _chk(load().mercury_download(self._h, name.encode(), ib, c, _dp(buf)))
+            out[c] = buf
+        return out
+
+    def set(self, name, ib, arr):
+        shape = self.case.box(ib)
+        arr = np.asarray(arr, dtype=np.float64).reshape((-1,) + shape)
+        for c in range(arr.shape[0]):
+            buf = np.ascontiguousarray(arr[c])
+            _chk(load().mercury_upload(self._h, name.encode(), ib, c, _dp(buf)))
+
+    def prepare(self):
+        _chk(load().mercury_prepare(self._h))
+
+    def step(self, n=1, integrator=0, record_dt=True):
+        if record_dt:
+            dts = np.zeros(max(n, 1))
+            _chk(load().mercury_step(self._h, n, integrator, _dp(dts)))
+            return dts[:n]
+        _chk(load().mercury_step(self._h, n, integrator, None))
+        return None
+
+    def op(self, name, arg=0.0):
+        _chk(load().mercury_op(self._h, name.encode(), float(arg)))
+
+    def timestep(self):
+        v = ctypes.c_double()
+        _chk(load().mercury_timestep(self._h, ctypes.byref(v)))
+        return v.value
+
+    def divb(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _chk(load().mercury_divb(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def synchronize(self):
+        _chk(load().mercury_synchronize(self._h))
+
+    def graph_capture(self, integrator=0):
+        _chk(load().mercury_graph_capture(self._h, integrator))
+
+    def graph_launch(self, n=1):
+        _chk(load().mercury_graph_launch(self._h, n))
+
+    def time_op(self, op, reps=10):
+        v = ctypes.c_double()
+        _chk(load().mercury_time_op(self._h, op.encode(), reps, ctypes.byref(v)))
+        return v.value

@@ -1,0 +1,1875 @@
+/*
+ * mercury_oracle.c -- CPU restatement of the MPCNS/Mercury explicit step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls this
+ * file: it is the checker that the HIP path is compared against in tests/, in
+ * __graft_entry__.smoke() and in the cpu_baseline leg of bench.py.
+ *
+ * Every function restates one reference function and cites it (paths are
+ * relative to the reference repository root).  The reference's Mercury TUs all
+ * include <petscksp.h> (Z4_Mercury/include/MercurySolver.h:14) and the image has
+ * no PETSc, so the reference cannot be compiled here; this restatement is the
+ * oracle and is pinned by the closed-form / invariant / known-answer tests under
+ * tests/ (see the header of each function for what pins it).
+ *
+ * Compile with -ffp-contract=off: the reference is built without FMA
+ * contraction on x86-64, and the device FP64 path is built the same way, so the
+ * two can be compared bit for bit wherever they perform the same operations.
+ *
+ * Layout: per block, every field component is one array over the box
+ * [-g, m+g] on each axis (m cells), k fastest:  idx = ((i+g)*NJ+(j+g))*NK+(k+g).
+ * Node coordinates live on a larger box [-g-1, m+g+1] (ghost depth g+1), like
+ * the reference grid arrays (src/1_grid/1_MPCNS_Grid.cpp:92-101).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* physics constants, same layout as MercuryPhysics in include/mercury_hip.h   */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    double gamma;
+    double M_H, M_Na, M_ref;
+    double state_coeff_H, state_coeff_Na;
+    double T_ref;
+    double inver_MA2;
+    double hall_coef;
+    double ne_floor;
+    double c_ind, c_hall;
+    double a1_Na, a4, a5, a6;
+    double Tn0;
+    double nu_lit, nu_shadow;
+    double cfl;
+    double taper_rmin, taper_rmax;
+    double arti_eta_max, J_start, J_on;
+    double local_enabled, local_eta_max, local_center[3], local_r_decay, local_r_cutoff;
+    double ambipolar_enabled, pressure_coef;
+    double qinf[5], qinfs[5], q_pv_inf[5], B_imf[3];
+    double wall_rho_floor_H, wall_rho_floor_Na, wall_p_floor;
+    double reserved[8];
+} OrcPhys;
+
+enum { BC_PERIODIC = 0, BC_FARFIELD = 1, BC_WALL = 2 };
+
+/* stagger locations */
+enum { L_CELL = 0, L_FX, L_FY, L_FZ, L_EX, L_EY, L_EZ, L_NODE };
+static const int DELTA[8][3] = {{1, 1, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0},
+                                {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, 0}};
+
+/* field slots ------------------------------------------------------------- */
+enum {
+    /* cell */
+    S_UH = 0, S_UN = S_UH + 5, S_PVH = S_UN + 5, S_PVN = S_PVH + 5,
+    S_UP = S_PVN + 5, S_BC = S_UP + 3, S_BI = S_BC + 3, S_JC = S_BI + 3,
+    S_RH = S_JC + 3, S_RN = S_RH + 5, S_EHC = S_RN + 5, S_HB = S_EHC + 3,
+    S_JAC = S_HB + 1, S_WB = S_JAC + 1, S_WJ = S_WB + 18, S_PHOTO = S_WJ + 36,
+    S_CC = S_PHOTO + 1, S_DIVB = S_CC + 3, S_U0H = S_DIVB + 1, S_U0N = S_U0H + 5,
+    /* faces (3 axes each) */
+    S_B = S_U0N + 5, S_BADD = S_B + 3, S_RHSB = S_BADD + 3, S_JD = S_RHSB + 3,
+    S_AREA = S_JD + 9, S_BETA = S_AREA + 3, S_EF = S_BETA + 3, S_F = S_EF + 9,
+    S_B0 = S_F + 15,
+    /* edges */
+    S_E = S_B0 + 3, S_EH = S_E + 3, S_J = S_EH + 3, S_DR = S_J + 3, S_DL = S_DR + 9,
+    S_ALPHA = S_DL + 3, S_TAPER = S_ALPHA + 3, S_LETA = S_TAPER + 3,
+    S_NSLOT = S_LETA + 3
+};
+
+typedef struct {
+    int m[3], g;
+    int N[3];
+    long V;
+    int bc[6];
+    int Nn[3];
+    long Vn;
+    double *x, *y, *z;
+    double *f[S_NSLOT];
+} OBlock;
+
+typedef struct {
+    int nb;
+    OBlock *b;
+    OrcPhys P;
+    double dt;
+    double time;
+    int step;
+    int integrator; /* 0 Euler (reference), 1 SSP-RK2 */
+} Orc;
+
+#define IX(b, i, j, k) ((((long)(i) + (b)->g) * (b)->N[1] + ((j) + (b)->g)) * (b)->N[2] + ((k) + (b)->g))
+#define NX(b, i, j, k) ((((long)(i) + (b)->g + 1) * (b)->Nn[1] + ((j) + (b)->g + 1)) * (b)->Nn[2] + ((k) + (b)->g + 1))
+#define F(b, s, i, j, k) ((b)->f[(s)][IX(b, i, j, k)])
+#define XN(b, i, j, k) ((b)->x[NX(b, i, j, k)])
+#define YN(b, i, j, k) ((b)->y[NX(b, i, j, k)])
+#define ZN(b, i, j, k) ((b)->z[NX(b, i, j, k)])
+
+static inline double dmax(double a, double b) { return (a < b) ? b : a; } /* std::max */
+static inline double dmin(double a, double b) { return (b < a) ? b : a; } /* std::min */
+
+/* owned range of a location: [0, m+1-delta)  (include/3_field/FieldBlock.h:34-35) */
+static void owned_hi(const OBlock *b, int loc, int hi[3]) {
+    for (int a = 0; a < 3; ++a) hi[a] = b->m[a] + 1 - DELTA[loc][a];
+}
+/* allocated range with ghost depth G (src/0_basic/LayoutTraits.cpp:278-294) */
+static void alloc_box(const OBlock *b, int loc, int G, int lo[3], int hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = -G;
+        hi[a] = b->m[a] + 1 - DELTA[loc][a] + G;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* geometry                                                                   */
+/* ------------------------------------------------------------------------- */
+
+/* Block::Add_ghostmesh, 3-D branch (src/1_grid/1_MPCNS_Grid.cpp:591-818).
+ * Ghost nodes are mirror-extrapolated one layer at a time; periodic interfaces
+ * do not overwrite them (src/1_grid/1_MPCNS_Grid.cpp:1268-1269). */
+static void add_ghostmesh(OBlock *b) {
+    const int mx = b->m[0], my = b->m[1], mz = b->m[2];
+    double *X[3] = {b->x, b->y, b->z};
+    for (int gh = 1; gh <= b->g + 1; ++gh) {
+        const int gimin = -gh, gimax = mx + gh, gjmin = -gh, gjmax = my + gh, gkmin = -gh, gkmax = mz + gh;
+        for (int c = 0; c < 3; ++c) {
+            double *q = X[c];
+#define Q(i, j, k) q[NX(b, i, j, k)]
+            for (int j = gjmin + 1; j < gjmax; j++)
+                for (int k = gkmin + 1; k < gkmax; k++) {
+                    Q(gimin, j, k) = 2 * Q(0, j, k) - Q(-gimin, j, k);
+                    Q(gimax, j, k) = 2 * Q(mx, j, k) - Q(2 * mx - gimax, j, k);
+                }
+            for (int i = gimin + 1; i < gimax; i++)
+                for (int k = gkmin + 1; k < gkmax; k++) {
+                    Q(i, gjmin, k) = 2 * Q(i, 0, k) - Q(i, -gjmin, k);
+                    Q(i, gjmax, k) = 2 * Q(i, my, k) - Q(i, 2 * my - gjmax, k);
+                }
+            for (int i = gimin + 1; i < gimax; i++)
+                for (int j = gjmin + 1; j < gjmax; j++) {
+                    Q(i, j, gkmin) = 2 * Q(i, j, 0) - Q(i, j, -gkmin);
+                    Q(i, j, gkmax) = 2 * Q(i, j, mz) - Q(i, j, 2 * mz - gkmax);
+                }
+            for (int k = gkmin + 1; k < gkmax; k++) {
+                Q(gimin, gjmin, k) = Q(gimin + 1, gjmin, k) + Q(gimin, gjmin + 1, k) - Q(gimin + 1, gjmin + 1, k);
+                Q(gimax, gjmin, k) = Q(gimax - 1, gjmin, k) + Q(gimax, gjmin + 1, k) - Q(gimax - 1, gjmin + 1, k);
+                Q(gimin, gjmax, k) = Q(gimin + 1, gjmax, k) + Q(gimin, gjmax - 1, k) - Q(gimin + 1, gjmax - 1, k);
+                Q(gimax, gjmax, k) = Q(gimax - 1, gjmax, k) + Q(gimax, gjmax - 1, k) - Q(gimax - 1, gjmax - 1, k);
+            }
+            for (int j = gjmin + 1; j < gjmax; j++) {
+                Q(gimin, j, gkmin) = Q(gimin + 1, j, gkmin) + Q(gimin, j, gkmin + 1) - Q(gimin + 1, j, gkmin + 1);
+                Q(gimax, j, gkmin) = Q(gimax - 1, j, gkmin) + Q(gimax, j, gkmin + 1) - Q(gimax - 1, j, gkmin + 1);
+                Q(gimin, j, gkmax) = Q(gimin + 1, j, gkmax) + Q(gimin, j, gkmax - 1) - Q(gimin + 1, j, gkmax - 1);
+                Q(gimax, j, gkmax) = Q(gimax - 1, j, gkmax) + Q(gimax, j, gkmax - 1) - Q(gimax - 1, j, gkmax - 1);
+            }
+            for (int i = gimin; i < gimax + 1; i++) {
+                Q(i, gjmin, gkmin) = Q(i, gjmin + 1, gkmin) + Q(i, gjmin, gkmin + 1) - Q(i, gjmin + 1, gkmin + 1);
+                Q(i, gjmax, gkmin) = Q(i, gjmax - 1, gkmin) + Q(i, gjmax, gkmin + 1) - Q(i, gjmax - 1, gkmin + 1);
+                Q(i, gjmin, gkmax) = Q(i, gjmin + 1, gkmax) + Q(i, gjmin, gkmax - 1) - Q(i, gjmin + 1, gkmax - 1);
+                Q(i, gjmax, gkmax) = Q(i, gjmax - 1, gkmax) + Q(i, gjmax, gkmax - 1) - Q(i, gjmax - 1, gkmax - 1);
+            }
+#undef Q
+        }
+    }
+}
+
+static inline void cross3(const double a[3], const double b[3], double c[3]) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline double norm3(const double a[3]) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+static inline double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+static void node(const OBlock *b, int i, int j, int k, double p[3]) {
+    p[0] = XN(b, i, j, k);
+    p[1] = YN(b, i, j, k);
+    p[2] = ZN(b, i, j, k);
+}
+
+/* Block::calc_Dual_Grids (src/1_grid/1_MPCNS_Grid_GCL.cpp:6-24): centre of
+ * cell (i,j,k) = mean of its 8 nodes, stored at the cell's own index. */
+static void dual_grid(OBlock *b) {
+    for (int i = -b->g; i < b->m[0] + b->g; ++i)
+        for (int j = -b->g; j < b->m[1] + b->g; ++j)
+            for (int k = -b->g; k < b->m[2] + b->g; ++k) {
+                double *q[3] = {b->x, b->y, b->z};
+                for (int c = 0; c < 3; ++c) {
+                    const double *Q = q[c];
+#define QQ(a1, a2, a3) Q[NX(b, a1, a2, a3)]
+                    F(b, S_CC + c, i, j, k) = 0.125 * (QQ(i + 1, j + 1, k + 1) + QQ(i, j + 1, k + 1) + QQ(i + 1, j, k + 1) + QQ(i, j, k + 1) +
+                                                       QQ(i + 1, j + 1, k) + QQ(i, j + 1, k) + QQ(i + 1, j, k) + QQ(i, j, k));
+#undef QQ
+                }
+            }
+}
+
+static void cellc(const OBlock *b, int i, int j, int k, double p[3]) {
+    p[0] = F(b, S_CC + 0, i, j, k);
+    p[1] = F(b, S_CC + 1, i, j, k);
+    p[2] = F(b, S_CC + 2, i, j, k);
+}
+
+/* quad area vector, 0.5*((p10-p00)x(p01-p00) + (p01-p11)x(p10-p11))
+ * (src/7_metric/Metric.cpp:109-141 and :380-389) */
+static void quad_area(const double p00[3], const double p10[3], const double p01[3], const double p11[3], double A[3]) {
+    double r1[3], r2[3], r3[3], r4[3], c1[3], c2[3];
+    for (int c = 0; c < 3; ++c) {
+        r1[c] = p10[c] - p00[c];
+        r2[c] = p01[c] - p00[c];
+        r3[c] = p01[c] - p11[c];
+        r4[c] = p10[c] - p11[c];
+    }
+    cross3(r1, r2, c1);
+    cross3(r3, r4, c2);
+    for (int c = 0; c < 3; ++c) A[c] = 0.5 * (c1[c] + c2[c]);
+}
+
+/* METRIC::compute_metric_fields (src/7_metric/Metric.cpp:78-1414): JD (face
+ * area vectors, ghost g), Jac (ghost g-1), Area / face Hodge beta (ghost g-1),
+ * dl / edge inverse Hodge alpha with the dual-collapse cap (ghost g-1), dr
+ * (ghost g-1). */
+static void metric(OBlock *b) {
+    const int g = b->g;
+    int lo[3], hi[3];
+    /* JDxi / JDet / JDze (Metric.cpp:109-211) */
+    for (int a = 0; a < 3; ++a) {
+        alloc_box(b, L_FX + a, g, lo, hi);
+        for (int i = lo[0]; i < hi[0]; ++i)
+            for (int j = lo[1]; j < hi[1]; ++j)
+                for (int k = lo[2]; k < hi[2]; ++k) {
+                    double o[3], pa[3], pb[3], pab[3];
+                    node(b, i, j, k, o);
+                    if (a == 0) {
+                        node(b, i, j + 1, k, pa); node(b, i, j, k + 1, pb); node(b, i, j + 1, k + 1, pab);
+                    } else if (a == 1) {
+                        node(b, i, j, k + 1, pa); node(b, i + 1, j, k, pb); node(b, i + 1, j, k + 1, pab);
+                    } else {
+                        node(b, i + 1, j, k, pa); node(b, i, j + 1, k, pb); node(b, i + 1, j + 1, k, pab);
+                    }
+                    double rx[3], ry[3], rx_[3], ry_[3], c1[3], c2[3];
+                    for (int c = 0; c < 3; ++c) {
+                        rx[c] = pa[c] - o[c];
+                        ry[c] = pb[c] - o[c];
+                        rx_[c] = pb[c] - pab[c];
+                        ry_[c] = pa[c] - pab[c];
+                    }
+                    cross3(rx, ry, c1);
+                    cross3(rx_, ry_, c2);
+                    for (int c = 0; c < 3; ++c) F(b, S_JD + 3 * a + c, i, j, k) = 0.5 * (c1[c] + c2[c]);
+                }
+    }
+    /* Jac = 1/3 sum_f A_f.(x_f - x_c) (Metric.cpp:214-326) */
+    alloc_box(b, L_CELL, g - 1, lo, hi);
+    for (int i = lo[0]; i < hi[0]; ++i)
+        for (int j = lo[1]; j < hi[1]; ++j)
+            for (int k = lo[2]; k < hi[2]; ++k) {
+                double xc[3];
+                cellc(b, i, j, k, xc);
+                double V = 0.0;
+                /* faces in the reference order: xi-, xi+, eta-, eta+, zeta-, zeta+ */
+                for (int a = 0; a < 3; ++a)
+                    for (int s = 0; s < 2; ++s) {
+                        int fi = i + (a == 0 ? s : 0), fj = j + (a == 1 ? s : 0), fk = k + (a == 2 ? s : 0);
+                        double Af[3];
+                        for (int c = 0; c < 3; ++c) {
+                            double v = F(b, S_JD + 3 * a + c, fi, fj, fk);
+                            Af[c] = s ? v : -v;
+                        }
+                        /* four corner nodes of the face, averaged in the reference's order */
+                        double n0[3], n1[3], n2[3], n3[3], xf[3];
+                        if (a == 0) {
+                            node(b, fi, j, k, n0); node(b, fi, j + 1, k, n1); node(b, fi, j, k + 1, n2); node(b, fi, j + 1, k + 1, n3);
+                        } else if (a == 1) {
+                            node(b, i, fj, k, n0); node(b, i + 1, fj, k, n1); node(b, i, fj, k + 1, n2); node(b, i + 1, fj, k + 1, n3);
+                        } else {
+                            node(b, i, j, fk, n0); node(b, i + 1, j, fk, n1); node(b, i, j + 1, fk, n2); node(b, i + 1, j + 1, fk, n3);
+                        }
+                        for (int c = 0; c < 3; ++c) xf[c] = 0.25 * (n0[c] + n1[c] + n2[c] + n3[c]);
+                        double d[3] = {xf[0] - xc[0], xf[1] - xc[1], xf[2] - xc[2]};
+                        V += dot3(Af, d);
+                    }
+                F(b, S_JAC, i, j, k) = V / 3.0;
+            }
+    /* Area, face Hodge beta = |l*|/|S| (Metric.cpp:397-489) */
+    const double eps = 1e-25;
+    for (int a = 0; a < 3; ++a) {
+        alloc_box(b, L_FX + a, g - 1, lo, hi);
+        for (int i = lo[0]; i < hi[0]; ++i)
+            for (int j = lo[1]; j < hi[1]; ++j)
+                for (int k = lo[2]; k < hi[2]; ++k) {
+                    double S[3] = {F(b, S_JD + 3 * a, i, j, k), F(b, S_JD + 3 * a + 1, i, j, k), F(b, S_JD + 3 * a + 2, i, j, k)};
+                    double Smag = norm3(S);
+                    F(b, S_AREA + a, i, j, k) = Smag;
+                    double cL[3], cR[3], d[3];
+                    cellc(b, i - (a == 0), j - (a == 1), k - (a == 2), cL);
+                    cellc(b, i, j, k, cR);
+                    for (int c = 0; c < 3; ++c) d[c] = cR[c] - cL[c];
+                    double Lstar = norm3(d);
+                    F(b, S_BETA + a, i, j, k) = (Smag < eps) ? 0.0 : Lstar / Smag;
+                }
+    }
+    /* dl, capped inverse Hodge alpha = |e|/|S*| (Metric.cpp:511-680) */
+    const double RATIO = 1e-3, EABS = 1e-12;
+    for (int a = 0; a < 3; ++a) {
+        alloc_box(b, L_EX + a, g - 1, lo, hi);
+        for (int i = lo[0]; i < hi[0]; ++i)
+            for (int j = lo[1]; j < hi[1]; ++j)
+                for (int k = lo[2]; k < hi[2]; ++k) {
+                    double r0[3], r1[3], d[3];
+                    node(b, i, j, k, r0);
+                    node(b, i + (a == 0), j + (a == 1), k + (a == 2), r1);
+                    for (int c = 0; c < 3; ++c) d[c] = r1[c] - r0[c];
+                    double L = norm3(d);
+                    F(b, S_DL + a, i, j, k) = L;
+                    /* dual quad: the 4 cells around the edge */
+                    double p00[3], p10[3], p01[3], p11[3];
+                    if (a == 0) {
+                        cellc(b, i, j - 1, k - 1, p00); cellc(b, i, j, k - 1, p10); cellc(b, i, j - 1, k, p01); cellc(b, i, j, k, p11);
+                    } else if (a == 1) {
+                        cellc(b, i - 1, j, k - 1, p00); cellc(b, i, j, k - 1, p10); cellc(b, i - 1, j, k, p01); cellc(b, i, j, k, p11);
+                    } else {
+                        cellc(b, i - 1, j - 1, k, p00); cellc(b, i, j - 1, k, p10); cellc(b, i - 1, j, k, p01); cellc(b, i, j, k, p11);
+                    }
+                    double Av[3];
+                    quad_area(p00, p10, p01, p11, Av);
+                    double Amag = norm3(Av);
+                    double raw = (Amag < eps) ? 0.0 : (L / Amag);
+                    double t[3];
+#define DIST(P, Qp) (t[0] = Qp[0] - P[0], t[1] = Qp[1] - P[1], t[2] = Qp[2] - P[2], norm3(t))
+                    double d00_10 = DIST(p00, p10), d00_01 = DIST(p00, p01), d10_11 = DIST(p10, p11), d01_11 = DIST(p01, p11);
+                    double d00_11 = DIST(p00, p11), d10_01 = DIST(p10, p01);
+#undef DIST
+                    double href = dmax(dmax(dmax(d00_10, d00_01), dmax(d10_11, d01_11)), dmax(dmax(d00_11, d10_01), dmax(L, eps)));
+                    double side_min = dmin(dmin(d00_10, d00_01), dmin(d10_11, d01_11));
+                    double diag_min = dmin(d00_11, d10_01);
+                    int touch = (L < EABS) || (side_min < RATIO * href) || (diag_min < RATIO * href) || (Amag < RATIO * href * href);
+                    F(b, S_ALPHA + a, i, j, k) = touch ? 0.0 : raw;
+                    for (int c = 0; c < 3; ++c) F(b, S_DR + 3 * a + c, i, j, k) = d[c];
+                }
+    }
+}
+
+/* 3x3 symmetric Jacobi eigen-decomposition with 18 sweeps
+ * (Z4_Mercury/src/MercurySolver_ReconstructionSetup.cpp:34-61) */
+static void eigen_sym3(double A[3][3], double V[3][3], double d[3]) {
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) V[a][c] = (a == c) ? 1.0 : 0.0;
+    for (int it = 0; it < 18; ++it) {
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                const double apq = A[p][q];
+                if (fabs(apq) <= 1.0e-16 * (fabs(A[p][p]) + fabs(A[q][q]) + 1.0)) continue;
+                const double phi = 0.5 * atan2(2.0 * apq, A[q][q] - A[p][p]);
+                const double c = cos(phi), s = sin(phi);
+                const double app = A[p][p], aqq = A[q][q];
+                for (int k = 0; k < 3; ++k)
+                    if (k != p && k != q) {
+                        const double akp = A[k][p], akq = A[k][q];
+                        A[k][p] = A[p][k] = c * akp - s * akq;
+                        A[k][q] = A[q][k] = s * akp + c * akq;
+                    }
+                A[p][p] = c * c * app - 2.0 * s * c * apq + s * s * aqq;
+                A[q][q] = s * s * app + 2.0 * s * c * apq + c * c * aqq;
+                A[p][q] = A[q][p] = 0.0;
+                for (int k = 0; k < 3; ++k) {
+                    const double vkp = V[k][p], vkq = V[k][q];
+                    V[k][p] = c * vkp - s * vkq;
+                    V[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    d[0] = A[0][0];
+    d[1] = A[1][1];
+    d[2] = A[2][2];
+}
+
+/* SetupCellReconstructionWeights_ (Z4_Mercury/src/MercurySolver_ReconstructionSetup.cpp:63-191):
+ * W_b (18 per owned cell) and W_j (36 per owned cell). */
+static void recon_weights(OBlock *b) {
+    const double eps = 1.0e-30, eig_tol = 1.0e-10;
+    for (int i = 0; i < b->m[0]; ++i)
+        for (int j = 0; j < b->m[1]; ++j)
+            for (int k = 0; k < b->m[2]; ++k) {
+                double s[6][3];
+                for (int a = 0; a < 3; ++a) {
+                    s[0][a] = -F(b, S_JD + 0 + a, i, j, k);
+                    s[1][a] = F(b, S_JD + 0 + a, i + 1, j, k);
+                    s[2][a] = -F(b, S_JD + 3 + a, i, j, k);
+                    s[3][a] = F(b, S_JD + 3 + a, i, j + 1, k);
+                    s[4][a] = -F(b, S_JD + 6 + a, i, j, k);
+                    s[5][a] = F(b, S_JD + 6 + a, i, j, k + 1);
+                }
+                double X[8][3];
+                int cnt = 0;
+                /* node order of avg8 in the reference: (i,j,k),(i+1,j,k),(i,j+1,k),(i+1,j+1,k),(i,j,k+1),... */
+                for (int dk = 0; dk < 2; ++dk)
+                    for (int dj = 0; dj < 2; ++dj)
+                        for (int di = 0; di < 2; ++di) node(b, i + di, j + dj, k + dk, X[cnt++]);
+                double xc[3];
+                for (int c = 0; c < 3; ++c)
+                    xc[c] = 0.125 * (X[0][c] + X[1][c] + X[2][c] + X[3][c] + X[4][c] + X[5][c] + X[6][c] + X[7][c]);
+                /* face centres: setfc(n, ...) four nodes each */
+                static const int FN[6][4][3] = {
+                    {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}}, {{1, 0, 0}, {1, 1, 0}, {1, 0, 1}, {1, 1, 1}},
+                    {{0, 0, 0}, {1, 0, 0}, {0, 0, 1}, {1, 0, 1}}, {{0, 1, 0}, {1, 1, 0}, {0, 1, 1}, {1, 1, 1}},
+                    {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}}, {{0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}}};
+                double fc[6][3];
+                for (int n = 0; n < 6; ++n) {
+                    double P[4][3];
+                    for (int q = 0; q < 4; ++q) node(b, i + FN[n][q][0], j + FN[n][q][1], k + FN[n][q][2], P[q]);
+                    for (int c = 0; c < 3; ++c) fc[n][c] = 0.25 * (P[0][c] + P[1][c] + P[2][c] + P[3][c]);
+                }
+                double smag[6], mu[6], sref = 0.0;
+                for (int n = 0; n < 6; ++n) {
+                    smag[n] = sqrt(s[n][0] * s[n][0] + s[n][1] * s[n][1] + s[n][2] * s[n][2]);
+                    sref = dmax(sref, smag[n]);
+                    const double dx = fc[n][0] - xc[0], dy = fc[n][1] - xc[1], dz = fc[n][2] - xc[2];
+                    mu[n] = 1.0 / dmax(sqrt(dx * dx + dy * dy + dz * dz), 1.0e-14);
+                }
+                double M[3][3] = {{0}};
+                int active[6];
+                for (int n = 0; n < 6; ++n) {
+                    active[n] = smag[n] > 1.0e-10 * sref;
+                    if (!active[n]) continue;
+                    for (int a = 0; a < 3; ++a)
+                        for (int c = 0; c < 3; ++c) M[a][c] += mu[n] * s[n][a] * s[n][c];
+                }
+                double A[3][3], V[3][3], lam[3];
+                memcpy(A, M, sizeof(A));
+                eigen_sym3(A, V, lam);
+                const double lmax = dmax(lam[0], dmax(lam[1], lam[2]));
+                for (int m = 0; m < 18; ++m) F(b, S_WB + m, i, j, k) = 0.0;
+                if (!(lmax <= eps)) {
+                    double Mi[3][3] = {{0}};
+                    for (int q = 0; q < 3; ++q) {
+                        const double inv = (lam[q] > eig_tol * lmax) ? 1.0 / lam[q] : 0.0;
+                        for (int a = 0; a < 3; ++a)
+                            for (int c = 0; c < 3; ++c) Mi[a][c] += V[a][q] * inv * V[c][q];
+                    }
+                    for (int n = 0; n < 6; ++n)
+                        if (active[n])
+                            for (int a = 0; a < 3; ++a)
+                                F(b, S_WB + 6 * a + n, i, j, k) = mu[n] * (Mi[a][0] * s[n][0] + Mi[a][1] * s[n][1] + Mi[a][2] * s[n][2]);
+                }
+                /* --- W_j from the 12 cell edges --- */
+                static const int ED[12][6] = {
+                    {0, 0, 0, 1, 0, 0}, {0, 1, 0, 1, 1, 0}, {0, 0, 1, 1, 0, 1}, {0, 1, 1, 1, 1, 1},
+                    {0, 0, 0, 0, 1, 0}, {1, 0, 0, 1, 1, 0}, {0, 0, 1, 0, 1, 1}, {1, 0, 1, 1, 1, 1},
+                    {0, 0, 0, 0, 0, 1}, {1, 0, 0, 1, 0, 1}, {0, 1, 0, 0, 1, 1}, {1, 1, 0, 1, 1, 1}};
+                double l[12][3];
+                for (int n = 0; n < 12; ++n) {
+                    double p0[3], p1[3];
+                    node(b, i + ED[n][0], j + ED[n][1], k + ED[n][2], p0);
+                    node(b, i + ED[n][3], j + ED[n][4], k + ED[n][5], p1);
+                    for (int c = 0; c < 3; ++c) l[n][c] = p1[c] - p0[c];
+                }
+                double Mj[3][3] = {{0}};
+                double lmaxl = 0.0;
+                for (int n = 0; n < 12; ++n) lmaxl = dmax(lmaxl, sqrt(l[n][0] * l[n][0] + l[n][1] * l[n][1] + l[n][2] * l[n][2]));
+                for (int m = 0; m < 36; ++m) F(b, S_WJ + m, i, j, k) = 0.0;
+                if (lmaxl <= eps) continue;
+                int act[12];
+                for (int n = 0; n < 12; ++n) {
+                    const double ln = sqrt(l[n][0] * l[n][0] + l[n][1] * l[n][1] + l[n][2] * l[n][2]);
+                    act[n] = ln > 1.0e-10 * lmaxl;
+                    if (!act[n]) continue;
+                    for (int a = 0; a < 3; ++a)
+                        for (int c = 0; c < 3; ++c) Mj[a][c] += l[n][a] * l[n][c];
+                }
+                memcpy(A, Mj, sizeof(A));
+                eigen_sym3(A, V, lam);
+                const double lme = dmax(lam[0], dmax(lam[1], lam[2]));
+                if (lme <= eps) continue;
+                double Mi[3][3] = {{0}};
+                for (int q = 0; q < 3; ++q) {
+                    const double inv = (lam[q] > eig_tol * lme) ? 1.0 / lam[q] : 0.0;
+                    for (int a = 0; a < 3; ++a)
+                        for (int c = 0; c < 3; ++c) Mi[a][c] += V[a][q] * inv * V[c][q];
+                }
+                for (int n = 0; n < 12; ++n)
+                    if (act[n])
+                        for (int a = 0; a < 3; ++a)
+                            F(b, S_WJ + 12 * a + n, i, j, k) = Mi[a][0] * l[n][0] + Mi[a][1] * l[n][1] + Mi[a][2] * l[n][2];
+            }
+}
+
+/* HallRadialTaper_/HallRadialTaperEdge_ (MercurySolver_MagExplicit_Hall.cpp:6-25)
+ * and the cached local artificial eta (MercurySolver_MagExplicit_Resis.cpp:87-144),
+ * both per owned edge. */
+static void edge_statics(OBlock *b, const OrcPhys *P) {
+    int hi[3];
+    for (int a = 0; a < 3; ++a) {
+        owned_hi(b, L_EX + a, hi);
+        for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+                for (int k = 0; k < hi[2]; ++k) {
+                    int ip = i + (a == 0), jp = j + (a == 1), kp = k + (a == 2);
+                    double xm = 0.5 * (XN(b, i, j, k) + XN(b, ip, jp, kp));
+                    double ym = 0.5 * (YN(b, i, j, k) + YN(b, ip, jp, kp));
+                    double zm = 0.5 * (ZN(b, i, j, k) + ZN(b, ip, jp, kp));
+                    double r = sqrt(xm * xm + ym * ym + zm * zm);
+                    double width = P->taper_rmax - P->taper_rmin, tp;
+                    if (width <= 0.0)
+                        tp = (r >= P->taper_rmin) ? 1.0 : 0.0;
+                    else {
+                        double mid = 0.5 * (P->taper_rmin + P->taper_rmax);
+                        tp = 0.5 * (1.0 + tanh(2.0 * (r - mid) / width));
+                    }
+                    F(b, S_TAPER + a, i, j, k) = tp;
+                    double eta = 0.0;
+                    if (P->local_enabled != 0.0 && P->local_eta_max > 0.0 && P->local_r_decay > 0.0) {
+                        double dx = xm - P->local_center[0], dy = ym - P->local_center[1], dz = zm - P->local_center[2];
+                        double rr = sqrt(dx * dx + dy * dy + dz * dz);
+                        if (P->local_r_cutoff > 0.0 && rr >= P->local_r_cutoff)
+                            eta = 0.0;
+                        else
+                            eta = P->local_eta_max * exp(-rr / P->local_r_decay);
+                    }
+                    F(b, S_LETA + a, i, j, k) = eta;
+                }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* synchronisation: physical handlers, corner default, periodic halo, owner   */
+/* ------------------------------------------------------------------------- */
+
+/* which slots belong to which field, with location and component count */
+typedef struct {
+    int slot, ncomp, loc;
+} FDesc;
+
+/* boundary-adjacent one-layer inner slab of a full block face
+ * (LAYOUT::boundary_inner_slab_one_layer_from_cells, LayoutTraits.cpp:434-465) */
+static void inner_slab(const OBlock *b, int loc, int dir, int lo[3], int hi[3]) {
+    int ax = abs(dir) - 1, ohi[3];
+    owned_hi(b, loc, ohi);
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = 0;
+        hi[a] = ohi[a];
+    }
+    if (dir < 0) {
+        lo[ax] = 0;
+        hi[ax] = 1;
+    } else {
+        lo[ax] = ohi[ax] - 1;
+        hi[ax] = ohi[ax];
+    }
+}
+/* LAYOUT::ghost_slab_from_inner (LayoutTraits.cpp:467-485) */
+static void ghost_slab(const int ilo[3], const int ihi[3], int dir, int G, int lo[3], int hi[3]) {
+    int ax = abs(dir) - 1;
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = ilo[a];
+        hi[a] = ihi[a];
+    }
+    if (dir < 0) {
+        hi[ax] = ilo[ax];
+        lo[ax] = ilo[ax] - G;
+    } else {
+        lo[ax] = ihi[ax];
+        hi[ax] = ihi[ax] + G;
+    }
+}
+
+/* BoundaryCore::DefaultPhysicalCopy (src/5_boundary/Boundary_Physical.cpp:288-315) */
+static void default_copy(OBlock *b, const FDesc *fd, int dir) {
+    int ilo[3], ihi[3], lo[3], hi[3];
+    inner_slab(b, fd->loc, dir, ilo, ihi);
+    ghost_slab(ilo, ihi, dir, b->g, lo, hi);
+    int ax = abs(dir) - 1;
+    int ref = (dir < 0) ? ilo[ax] : ihi[ax] - 1;
+    for (int i = lo[0]; i < hi[0]; ++i)
+        for (int j = lo[1]; j < hi[1]; ++j)
+            for (int k = lo[2]; k < hi[2]; ++k) {
+                int ii = ax == 0 ? ref : i, jj = ax == 1 ? ref : j, kk = ax == 2 ? ref : k;
+                for (int m = 0; m < fd->ncomp; ++m) F(b, fd->slot + m, i, j, k) = F(b, fd->slot + m, ii, jj, kk);
+            }
+}
+
+static void fill_const(OBlock *b, const FDesc *fd, int dir, const double *val) {
+    int ilo[3], ihi[3], lo[3], hi[3];
+    inner_slab(b, fd->loc, dir, ilo, ihi);
+    ghost_slab(ilo, ihi, dir, b->g, lo, hi);
+    for (int i = lo[0]; i < hi[0]; ++i)
+        for (int j = lo[1]; j < hi[1]; ++j)
+            for (int k = lo[2]; k < hi[2]; ++k)
+                for (int m = 0; m < fd->ncomp; ++m) F(b, fd->slot + m, i, j, k) = val[m];
+}
+
+/* MercuryBoundary::BC_Farfield_Eedge_set_zerocurl (Boundary_Condition.cpp:573-623) */
+static void farfield_eedge(OBlock *b, const OrcPhys *P, const FDesc *fd, int dir) {
+    const double *u = P->q_pv_inf, *Bi = P->B_imf;
+    double Ei[3];
+    Ei[0] = Bi[1] * u[2] - Bi[2] * u[1];
+    Ei[1] = Bi[2] * u[0] - Bi[0] * u[2];
+    Ei[2] = Bi[0] * u[1] - Bi[1] * u[0];
+    int a = fd->loc - L_EX;
+    int ilo[3], ihi[3];
+    inner_slab(b, fd->loc, dir, ilo, ihi);
+    for (int i = ilo[0]; i < ihi[0]; ++i)
+        for (int j = ilo[1]; j < ihi[1]; ++j)
+            for (int k = ilo[2]; k < ihi[2]; ++k) {
+                int ir = i + (a == 0), jr = j + (a == 1), kr = k + (a == 2);
+                F(b, fd->slot, i, j, k) = Ei[0] * (XN(b, ir, jr, kr) - XN(b, i, j, k)) + Ei[1] * (YN(b, ir, jr, kr) - YN(b, i, j, k)) +
+                                          Ei[2] * (ZN(b, ir, jr, kr) - ZN(b, i, j, k));
+            }
+    default_copy(b, fd, dir);
+}
+
+/* MercuryBoundary::BC_Solid_Surface_ (Boundary_Condition_Fluid.cpp:36-260):
+ * non-Na species: first active layer <- (rho, p) one cell deeper, zero velocity;
+ * ghosts <- first layer.  Na species: ghosts <- floor state. */
+static void wall_fluid(OBlock *b, const OrcPhys *P, int slot, int is_na, int dir) {
+    int ax = abs(dir), sgn = dir > 0 ? 1 : -1;
+    int cyc[3] = {ax == 1 ? sgn : 0, ax == 2 ? sgn : 0, ax == 3 ? sgn : 0};
+    const double rho_floor = is_na ? P->wall_rho_floor_Na : P->wall_rho_floor_H;
+    const double p_floor = P->wall_p_floor;
+    const double gm1 = P->gamma - 1.0;
+    int ilo[3], ihi[3];
+    inner_slab(b, L_CELL, dir, ilo, ihi);
+    int ulo[3], uhi[3];
+    alloc_box(b, L_CELL, b->g, ulo, uhi);
+#define CLAMP(v, a) ((v) < ulo[a] ? ulo[a] : ((v) > uhi[a] - 1 ? uhi[a] - 1 : (v)))
+    for (int ii = ilo[0]; ii < ihi[0]; ++ii)
+        for (int jj = ilo[1]; jj < ihi[1]; ++jj)
+            for (int kk = ilo[2]; kk < ihi[2]; ++kk) {
+                if (is_na) {
+                    for (int ng = 1; ng <= b->g; ++ng) {
+                        int ig = CLAMP(ii + ng * cyc[0], 0), jg = CLAMP(jj + ng * cyc[1], 1), kg = CLAMP(kk + ng * cyc[2], 2);
+                        F(b, slot + 0, ig, jg, kg) = rho_floor;
+                        F(b, slot + 1, ig, jg, kg) = 0.0;
+                        F(b, slot + 2, ig, jg, kg) = 0.0;
+                        F(b, slot + 3, ig, jg, kg) = 0.0;
+                        F(b, slot + 4, ig, jg, kg) = p_floor / gm1;
+                    }
+                    continue;
+                }
+                for (int layer = 0; layer <= b->g; ++layer) {
+                    int ir, jr, kr, ig, jg, kg;
+                    if (layer == 0) {
+                        ir = CLAMP(ii - cyc[0], 0); jr = CLAMP(jj - cyc[1], 1); kr = CLAMP(kk - cyc[2], 2);
+                        ig = ii; jg = jj; kg = kk;
+                    } else {
+                        ir = CLAMP(ii, 0); jr = CLAMP(jj, 1); kr = CLAMP(kk, 2);
+                        ig = CLAMP(ii + layer * cyc[0], 0); jg = CLAMP(jj + layer * cyc[1], 1); kg = CLAMP(kk + layer * cyc[2], 2);
+                    }
+                    /* pressure_from_cons (Boundary_Condition_Fluid.cpp:79-93) */
+                    double rho_r = dmax(F(b, slot, ir, jr, kr), rho_floor);
+                    double mxr = F(b, slot + 1, ir, jr, kr), myr = F(b, slot + 2, ir, jr, kr), mzr = F(b, slot + 3, ir, jr, kr);
+                    double Er = F(b, slot + 4, ir, jr, kr);
+                    double rho2 = dmax(rho_r, rho_floor);
+                    double ke = 0.5 * (mxr * mxr + myr * myr + mzr * mzr) / rho2;
+                    double eint = Er - ke;
+                    if (eint < 0.0) eint = 0.0;
+                    double p_r = dmax(gm1 * eint, p_floor);
+                    F(b, slot + 0, ig, jg, kg) = rho_r;
+                    F(b, slot + 1, ig, jg, kg) = rho_r * 0.0;
+                    F(b, slot + 2, ig, jg, kg) = rho_r * 0.0;
+                    F(b, slot + 3, ig, jg, kg) = rho_r * 0.0;
+                    const double ke_g = 0.5 * rho_r * 0.0;
+                    F(b, slot + 4, ig, jg, kg) = p_r / gm1 + ke_g;
+                }
+            }
+#undef CLAMP
+}
+
+/* MercuryBoundary::BC_Solid_Surface_Jcell (Boundary_Condition_EM.cpp:382-427) */
+static void wall_jcell(OBlock *b, const FDesc *fd, int dir) {
+    int ax = abs(dir), sgn = dir > 0 ? 1 : -1, inward = -sgn;
+    int ilo[3], ihi[3];
+    inner_slab(b, L_CELL, dir, ilo, ihi);
+    for (int i = ilo[0]; i < ihi[0]; ++i)
+        for (int j = ilo[1]; j < ihi[1]; ++j)
+            for (int k = ilo[2]; k < ihi[2]; ++k) {
+                for (int n = 0; n < 3; ++n) F(b, fd->slot + n, i, j, k) = 0.0;
+                int is = i + (ax == 1 ? inward : 0), js = j + (ax == 2 ? inward : 0), ks = k + (ax == 3 ? inward : 0);
+                for (int n = 0; n < 3; ++n) F(b, fd->slot + n, is, js, ks) = 0.0;
+            }
+    default_copy(b, fd, dir);
+}
+
+enum { G_UCELL, G_BFACE, G_EEDGE, G_EHALL, G_JEDGE, G_BCELL, G_JCELL, G_BADD, G_NGROUP };
+/* field kinds for handler dispatch */
+enum { K_UH, K_UNA, K_BFACE, K_EEDGE, K_EHALL, K_JEDGE, K_BCELL, K_BIND, K_JCELL, K_BADD };
+
+typedef struct {
+    int kind, slot, ncomp, loc, level; /* level: 1 = Corner1D, 3 = Corner3D */
+    int owner_sync;
+} GField;
+
+static int group_fields(int gid, GField out[6]) {
+    int n = 0;
+#define ADD(K, S, NC, L, LV, OS) out[n].kind = K, out[n].slot = S, out[n].ncomp = NC, out[n].loc = L, out[n].level = LV, out[n].owner_sync = OS, ++n
+    switch (gid) {
+    case G_UCELL: ADD(K_UH, S_UH, 5, L_CELL, 1, 0); ADD(K_UNA, S_UN, 5, L_CELL, 1, 0); break;
+    case G_BFACE: for (int a = 0; a < 3; ++a) ADD(K_BFACE, S_B + a, 1, L_FX + a, 3, 1); break;
+    case G_BADD: for (int a = 0; a < 3; ++a) ADD(K_BADD, S_BADD + a, 1, L_FX + a, 3, 1); break;
+    case G_EEDGE: for (int a = 0; a < 3; ++a) ADD(K_EEDGE, S_E + a, 1, L_EX + a, 3, 1); break;
+    case G_EHALL: for (int a = 0; a < 3; ++a) ADD(K_EHALL, S_EH + a, 1, L_EX + a, 3, 1); break;
+    case G_JEDGE: for (int a = 0; a < 3; ++a) ADD(K_JEDGE, S_J + a, 1, L_EX + a, 3, 1); break;
+    case G_BCELL: ADD(K_BCELL, S_BC, 3, L_CELL, 3, 0); ADD(K_BIND, S_BI, 3, L_CELL, 3, 0); break;
+    case G_JCELL: ADD(K_JCELL, S_JC, 3, L_CELL, 3, 0); break;
+    }
+#undef ADD
+    return n;
+}
+
+/* one physical region handler (Z4_Mercury/src/boundary/Boundary_Build.cpp:5-300) */
+static void apply_handler(OBlock *b, const OrcPhys *P, const GField *gf, int bc, int dir) {
+    FDesc fd = {gf->slot, gf->ncomp, gf->loc};
+    static const double zero3[3] = {0, 0, 0};
+    if (bc == BC_FARFIELD) {
+        switch (gf->kind) {
+        case K_UH: fill_const(b, &fd, dir, P->qinf); break;
+        case K_UNA: fill_const(b, &fd, dir, P->qinfs); break;
+        case K_BFACE: break; /* nop */
+        case K_BCELL: fill_const(b, &fd, dir, P->B_imf); break;
+        case K_BIND: fill_const(b, &fd, dir, zero3); break;
+        case K_EEDGE: farfield_eedge(b, P, &fd, dir); break;
+        default: default_copy(b, &fd, dir); break; /* Ehall, J edges, J_cell, Badd */
+        }
+    } else if (bc == BC_WALL) {
+        switch (gf->kind) {
+        case K_UH: wall_fluid(b, P, gf->slot, 0, dir); break;
+        case K_UNA: wall_fluid(b, P, gf->slot, 1, dir); break;
+        case K_JCELL: wall_jcell(b, &fd, dir); break;
+        default: break; /* nop for every other field on the fluid side of the wall */
+        }
+    }
+}
+
+/* fill_physical_corner_ghost (src/5_boundary/Boundary_Physical.cpp:39-59) on the
+ * corner ghost boxes of physical edges (2 physical faces) and vertices (3). */
+static void corner_default(OBlock *b, const GField *gf) {
+    int ohi[3];
+    owned_hi(b, gf->loc, ohi);
+    int lo[3], hi[3];
+    alloc_box(b, gf->loc, b->g, lo, hi);
+    for (int i = lo[0]; i < hi[0]; ++i)
+        for (int j = lo[1]; j < hi[1]; ++j)
+            for (int k = lo[2]; k < hi[2]; ++k) {
+                int idx[3] = {i, j, k}, nout = 0, ok = 1;
+                for (int a = 0; a < 3; ++a) {
+                    int side = idx[a] < 0 ? 0 : (idx[a] >= ohi[a] ? 1 : -1);
+                    if (side < 0) continue;
+                    ++nout;
+                    if (b->bc[2 * a + side] == BC_PERIODIC) ok = 0;
+                }
+                if (nout < 2 || !ok) continue;
+                int ii = i < 0 ? 0 : (i > ohi[0] - 1 ? ohi[0] - 1 : i);
+                int jj = j < 0 ? 0 : (j > ohi[1] - 1 ? ohi[1] - 1 : j);
+                int kk = k < 0 ? 0 : (k > ohi[2] - 1 ? ohi[2] - 1 : k);
+                for (int m = 0; m < gf->ncomp; ++m) F(b, gf->slot + m, i, j, k) = F(b, gf->slot + m, ii, jj, kk);
+            }
+}
+
+/* Periodic self-connection halo (Halo::sync_group, HaloSyncFramework.cpp:969-1035,
+ * with the strip rules of LayoutTraits.cpp:93-130): a ghost DOF whose
+ * out-of-range axes are all periodic and number at most `level` takes the value
+ * of its translate by whole periods.  Along an axis with delta = 0 the shared
+ * boundary DOF is excluded from the source strip, i.e. i<0 -> i+m and i>m -> i-m. */
+static void periodic_halo(OBlock *b, const GField *gf) {
+    int ohi[3], lo[3], hi[3];
+    owned_hi(b, gf->loc, ohi);
+    alloc_box(b, gf->loc, b->g, lo, hi);
+    for (int i = lo[0]; i < hi[0]; ++i)
+        for (int j = lo[1]; j < hi[1]; ++j)
+            for (int k = lo[2]; k < hi[2]; ++k) {
+                int idx[3] = {i, j, k}, src[3], nout = 0, ok = 1;
+                for (int a = 0; a < 3; ++a) {
+                    src[a] = idx[a];
+                    if (idx[a] >= 0 && idx[a] < ohi[a]) continue;
+                    ++nout;
+                    int side = idx[a] < 0 ? 0 : 1;
+                    if (b->bc[2 * a + side] != BC_PERIODIC) { ok = 0; continue; }
+                    src[a] = idx[a] < 0 ? idx[a] + b->m[a] : idx[a] - b->m[a];
+                }
+                if (nout == 0 || nout > gf->level || !ok) continue;
+                for (int m = 0; m < gf->ncomp; ++m) F(b, gf->slot + m, i, j, k) = F(b, gf->slot + m, src[0], src[1], src[2]);
+            }
+}
+
+/* owner -> alias (Halo::sync_owner_alias_group, HaloSyncFramework.cpp:695-712):
+ * on a periodic axis the DOF at index m (delta = 0) aliases the one at 0; the
+ * owner of an equivalence class is its smallest EntityKey, i.e. every seam index
+ * replaced by 0; orientation signs are +1 for translations. */
+static void owner_sync(OBlock *b, const GField *gf) {
+    int ohi[3];
+    owned_hi(b, gf->loc, ohi);
+    for (int i = 0; i < ohi[0]; ++i)
+        for (int j = 0; j < ohi[1]; ++j)
+            for (int k = 0; k < ohi[2]; ++k) {
+                int idx[3] = {i, j, k}, o[3], alias = 0;
+                for (int a = 0; a < 3; ++a) {
+                    o[a] = idx[a];
+                    if (DELTA[gf->loc][a] == 0 && b->bc[2 * a] == BC_PERIODIC && idx[a] == b->m[a]) {
+                        o[a] = 0;
+                        alias = 1;
+                    }
+                }
+                if (alias) F(b, gf->slot, i, j, k) = F(b, gf->slot, o[0], o[1], o[2]);
+            }
+}
+
+/* MercuryBoundary::Sync_ (Z4_Mercury/src/boundary/Boundary_Sync_.cpp:4-110):
+ * physical handlers region by region (Farfield regions before the wall, the
+ * order of the sorted physical patch list), corner defaults, staged halo, then
+ * owner -> alias for the face/edge form groups. */
+static void sync_group(Orc *o, int gid) {
+    GField gf[6];
+    int nf = group_fields(gid, gf);
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int n = 0; n < nf; ++n) {
+            for (int pass = 0; pass < 2; ++pass) {
+                int want = pass == 0 ? BC_FARFIELD : BC_WALL;
+                for (int f = 0; f < 6; ++f) {
+                    if (b->bc[f] != want) continue;
+                    int dir = (f / 2 + 1) * ((f & 1) ? 1 : -1);
+                    apply_handler(b, &o->P, &gf[n], want, dir);
+                }
+            }
+            corner_default(b, &gf[n]);
+            periodic_halo(b, &gf[n]);
+            if (gf[n].owner_sync) owner_sync(b, &gf[n]);
+        }
+    }
+}
+
+/* ReduceEdgeAliasCandidatesToOwners_ (MercurySolver_Current.cpp:3-57): the owner
+ * of each shared periodic-seam edge class takes the signed mean of all members,
+ * summed in ascending EntityKey order. */
+static void alias_reduce_edges(Orc *o, int slot0) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int a = 0; a < 3; ++a) {
+            int loc = L_EX + a, ohi[3];
+            owned_hi(b, loc, ohi);
+            int per[3];
+            for (int c = 0; c < 3; ++c) per[c] = (DELTA[loc][c] == 0 && b->bc[2 * c] == BC_PERIODIC);
+            for (int i = 0; i < ohi[0]; ++i)
+                for (int j = 0; j < ohi[1]; ++j)
+                    for (int k = 0; k < ohi[2]; ++k) {
+                        int idx[3] = {i, j, k}, nseam = 0, ax[3];
+                        int isowner = 1;
+                        for (int c = 0; c < 3; ++c) {
+                            if (!per[c]) continue;
+                            if (idx[c] == b->m[c]) isowner = 0;
+                            if (idx[c] == 0) ax[nseam++] = c;
+                        }
+                        if (!isowner || nseam == 0) continue;
+                        /* members: each seam axis index in {0, m}; ascending key order */
+                        int nm = 1 << nseam;
+                        double sum = 0.0;
+                        for (int mask = 0; mask < nm; ++mask) {
+                            int q[3] = {i, j, k};
+                            for (int t = 0; t < nseam; ++t)
+                                if (mask & (1 << (nseam - 1 - t))) q[ax[t]] = b->m[ax[t]];
+                            sum += 1 * F(b, slot0 + a, q[0], q[1], q[2]);
+                        }
+                        F(b, slot0 + a, i, j, k) = 1 * sum / (double)nm;
+                    }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* physics                                                                    */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    double nH, nNa, ne, ne_eff, chiH, chiNa, wH, wNa;
+} NumInfo;
+
+/* Hall_Num_Limiter (MercurySolver_Tools.cpp:111-157) */
+static NumInfo num_limiter(const OrcPhys *P, double rhoH, double rhoNa) {
+    NumInfo n;
+    const double rhoH_pos = dmax(rhoH, 0.0), rhoNa_pos = dmax(rhoNa, 0.0);
+    n.nH = rhoH_pos / P->M_H * P->M_ref;
+    n.nNa = rhoNa_pos / P->M_Na * P->M_ref;
+    n.ne = n.nH + n.nNa;
+    n.ne_eff = sqrt(n.ne * n.ne + P->ne_floor * P->ne_floor);
+    if (n.ne > 1.0e-300) {
+        n.chiH = n.nH / n.ne;
+        n.chiNa = n.nNa / n.ne;
+    } else {
+        n.chiH = 1.0;
+        n.chiNa = 0.0;
+    }
+    n.wH = n.nH / n.ne_eff;
+    n.wNa = n.nNa / n.ne_eff;
+    return n;
+}
+
+/* calc_PV (MercurySolver_Tools.cpp:159-213), full allocated range */
+static void calc_pv(Orc *o) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int s = 0; s < 2; ++s) {
+            int su = s ? S_UN : S_UH, sp = s ? S_PVN : S_PVH;
+            double coeff = s ? o->P.state_coeff_Na : o->P.state_coeff_H;
+            for (long q = 0; q < b->V; ++q) {
+                const double rho = dmax(b->f[su][q], 1e-12);
+                const double u = b->f[su + 1][q] / rho, v = b->f[su + 2][q] / rho, w = b->f[su + 3][q] / rho;
+                const double E = b->f[su + 4][q];
+                const double ke = 0.5 * rho * (u * u + v * v + w * w);
+                double eint = E - ke;
+                if (eint < 0.0) eint = 0.0;
+                double p = (o->P.gamma - 1.0) * eint;
+                p = dmax(p, 1e-12);
+                b->f[sp][q] = u;
+                b->f[sp + 1][q] = v;
+                b->f[sp + 2][q] = w;
+                b->f[sp + 3][q] = p;
+                b->f[sp + 4][q] = p / (rho * coeff);
+            }
+        }
+    }
+}
+
+/* calc_Uplus (MercurySolver_Tools.cpp:215-301), full allocated range */
+static void calc_uplus(Orc *o) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (long q = 0; q < b->V; ++q) {
+            const double rH = dmax(b->f[S_UH][q], 0.0), rN = dmax(b->f[S_UN][q], 0.0);
+            double uH[3], uN[3];
+            for (int c = 0; c < 3; ++c) {
+                uH[c] = (rH > 1e-20) ? b->f[S_PVH + c][q] : 0.0;
+                uN[c] = (rN > 1e-20) ? b->f[S_PVN + c][q] : 0.0;
+            }
+            NumInfo n = num_limiter(&o->P, rH, rN);
+            for (int c = 0; c < 3; ++c) b->f[S_UP + c][q] = n.chiH * uH[c] + n.chiNa * uN[c];
+        }
+    }
+}
+
+/* calc_Bcell (MercurySolver_MagneticCell.cpp:3-81), owned cells */
+static void calc_bcell(Orc *o) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int i = 0; i < b->m[0]; ++i)
+            for (int j = 0; j < b->m[1]; ++j)
+                for (int k = 0; k < b->m[2]; ++k) {
+                    double pt[6], pi[6];
+                    pt[0] = -F(b, S_B + 0, i, j, k) - F(b, S_BADD + 0, i, j, k);
+                    pt[1] = F(b, S_B + 0, i + 1, j, k) + F(b, S_BADD + 0, i + 1, j, k);
+                    pt[2] = -F(b, S_B + 1, i, j, k) - F(b, S_BADD + 1, i, j, k);
+                    pt[3] = F(b, S_B + 1, i, j + 1, k) + F(b, S_BADD + 1, i, j + 1, k);
+                    pt[4] = -F(b, S_B + 2, i, j, k) - F(b, S_BADD + 2, i, j, k);
+                    pt[5] = F(b, S_B + 2, i, j, k + 1) + F(b, S_BADD + 2, i, j, k + 1);
+                    pi[0] = -F(b, S_B + 0, i, j, k);
+                    pi[1] = F(b, S_B + 0, i + 1, j, k);
+                    pi[2] = -F(b, S_B + 1, i, j, k);
+                    pi[3] = F(b, S_B + 1, i, j + 1, k);
+                    pi[4] = -F(b, S_B + 2, i, j, k);
+                    pi[5] = F(b, S_B + 2, i, j, k + 1);
+                    double bt[3] = {0, 0, 0}, bi[3] = {0, 0, 0};
+                    for (int n = 0; n < 6; ++n)
+                        for (int c = 0; c < 3; ++c) {
+                            const double w = F(b, S_WB + 6 * c + n, i, j, k);
+                            bt[c] += w * pt[n];
+                            bi[c] += w * pi[n];
+                        }
+                    for (int c = 0; c < 3; ++c) {
+                        F(b, S_BC + c, i, j, k) = bt[c];
+                        F(b, S_BI + c, i, j, k) = bi[c];
+                    }
+                }
+    }
+}
+
+/* CTOperators::CurlAdjFaceToEdge (Z4_Mercury/src/operators/CTOperators.cpp:59-110)
+ * then J *= alpha (MercurySolver_Current.cpp:140-201), owned edges */
+static void calc_jedge(Orc *o) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        int hi[3];
+#define BB(a, i, j, k) (F(b, S_BETA + (a), i, j, k) * F(b, S_B + (a), i, j, k))
+        owned_hi(b, L_EX, hi);
+        for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+                for (int k = 0; k < hi[2]; ++k) {
+                    double v = (BB(1, i, j, k - 1) - BB(1, i, j, k)) + (BB(2, i, j, k) - BB(2, i, j - 1, k));
+                    F(b, S_J + 0, i, j, k) = 1.0 * v;
+                    F(b, S_J + 0, i, j, k) *= F(b, S_ALPHA + 0, i, j, k);
+                }
+        owned_hi(b, L_EY, hi);
+        for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+                for (int k = 0; k < hi[2]; ++k) {
+                    double v = (BB(0, i, j, k) - BB(0, i, j, k - 1)) + (BB(2, i - 1, j, k) - BB(2, i, j, k));
+                    F(b, S_J + 1, i, j, k) = 1.0 * v;
+                    F(b, S_J + 1, i, j, k) *= F(b, S_ALPHA + 1, i, j, k);
+                }
+        owned_hi(b, L_EZ, hi);
+        for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+                for (int k = 0; k < hi[2]; ++k) {
+                    double v = (BB(0, i, j - 1, k) - BB(0, i, j, k)) + (BB(1, i, j, k) - BB(1, i - 1, j, k));
+                    F(b, S_J + 2, i, j, k) = 1.0 * v;
+                    F(b, S_J + 2, i, j, k) *= F(b, S_ALPHA + 2, i, j, k);
+                }
+#undef BB
+    }
+}
+
+/* calc_Jcell (MercurySolver_Current.cpp:203-259), owned cells */
+static void calc_jcell(Orc *o) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int i = 0; i < b->m[0]; ++i)
+            for (int j = 0; j < b->m[1]; ++j)
+                for (int k = 0; k < b->m[2]; ++k) {
+                    double s[12] = {
+                        F(b, S_J + 0, i, j, k), F(b, S_J + 0, i, j + 1, k), F(b, S_J + 0, i, j, k + 1), F(b, S_J + 0, i, j + 1, k + 1),
+                        F(b, S_J + 1, i, j, k), F(b, S_J + 1, i + 1, j, k), F(b, S_J + 1, i, j, k + 1), F(b, S_J + 1, i + 1, j, k + 1),
+                        F(b, S_J + 2, i, j, k), F(b, S_J + 2, i + 1, j, k), F(b, S_J + 2, i, j + 1, k), F(b, S_J + 2, i + 1, j + 1, k)};
+                    double J[3] = {0, 0, 0};
+                    for (int n = 0; n < 12; ++n)
+                        for (int c = 0; c < 3; ++c) J[c] += F(b, S_WJ + 12 * c + n, i, j, k) * s[n];
+                    for (int c = 0; c < 3; ++c) F(b, S_JC + c, i, j, k) = J[c];
+                }
+    }
+}
+
+/* UpdateDerivedFields_ (MercurySolver_Tools.cpp:303-321) */
+static void update_derived(Orc *o) {
+    calc_bcell(o);
+    sync_group(o, G_BCELL);
+    calc_jedge(o);
+    alias_reduce_edges(o, S_J);
+    sync_group(o, G_JEDGE);
+    calc_jcell(o);
+    sync_group(o, G_JCELL);
+    calc_pv(o);
+    calc_uplus(o);
+}
+
+/* Compute_Timestep (MercurySolver_Timestep.cpp:8-141) */
+static double compute_timestep(Orc *o) {
+    const OrcPhys *P = &o->P;
+    double dt_local = 1e100;
+    for (int s = 0; s < 2; ++s)
+        for (int ib = 0; ib < o->nb; ++ib) {
+            OBlock *b = &o->b[ib];
+            int su = s ? S_UN : S_UH;
+            for (int i = 0; i < b->m[0]; ++i)
+                for (int j = 0; j < b->m[1]; ++j)
+                    for (int k = 0; k < b->m[2]; ++k) {
+                        double V = fabs(F(b, S_JAC, i, j, k));
+                        if (V <= 0.0) continue;
+                        double rho = dmax(F(b, su, i, j, k), 1e-2);
+                        double ux = F(b, su + 1, i, j, k) / rho, uy = F(b, su + 2, i, j, k) / rho, uz = F(b, su + 3, i, j, k) / rho;
+                        const double E = F(b, su + 4, i, j, k);
+                        const double ke = 0.5 * rho * (ux * ux + uy * uy + uz * uz);
+                        double p = (P->gamma - 1.0) * (E - ke);
+                        double cs2 = P->gamma * p / rho;
+                        double Bx = F(b, S_BC, i, j, k), By = F(b, S_BC + 1, i, j, k), Bz = F(b, S_BC + 2, i, j, k);
+                        double B2 = Bx * Bx + By * By + Bz * Bz;
+                        double denom = 0.0;
+                        for (int a = 0; a < 3; ++a)
+                            for (int sd = 0; sd < 2; ++sd) {
+                                int fi = i + (a == 0 && sd), fj = j + (a == 1 && sd), fk = k + (a == 2 && sd);
+                                double ax = F(b, S_JD + 3 * a, fi, fj, fk), ay = F(b, S_JD + 3 * a + 1, fi, fj, fk), az = F(b, S_JD + 3 * a + 2, fi, fj, fk);
+                                double A = sqrt(ax * ax + ay * ay + az * az);
+                                double term = 0.0;
+                                if (!(A <= 0.0)) {
+                                    double nx = ax / A, ny = ay / A, nz = az / A;
+                                    if (!sd) { nx = -nx; ny = -ny; nz = -nz; }
+                                    double un = ux * nx + uy * ny + uz * nz;
+                                    double Bn = Bx * nx + By * ny + Bz * nz;
+                                    double vA2 = P->inver_MA2 * B2 / rho;
+                                    double vAn2 = P->inver_MA2 * (Bn * Bn) / rho;
+                                    double t = cs2 + vA2;
+                                    double disc = t * t - 4.0 * cs2 * vAn2;
+                                    if (disc < 0.0) disc = 0.0;
+                                    double cf = sqrt(0.5 * (t + sqrt(disc)));
+                                    term = (fabs(un) + cf) * A;
+                                }
+                                denom += term;
+                            }
+                        if (denom > 0.0) {
+                            double dtc = P->cfl * V / denom;
+                            dt_local = dmin(dt_local, dtc);
+                        }
+                    }
+        }
+    return dt_local;
+}
+
+/* Scheme_U_ (MercurySolver_FluidRHS.cpp:3-202): Rusanov face fluxes on owned
+ * faces of each axis, then RHS -= dF/Jac on owned cells.  RHS starts at zero
+ * (ZeroRHS_, MercurySolver_Advance.cpp:4-71). */
+static void scheme_u(Orc *o) {
+    const double gm = o->P.gamma;
+    for (int s = 0; s < 2; ++s)
+        for (int ib = 0; ib < o->nb; ++ib) {
+            OBlock *b = &o->b[ib];
+            int su = s ? S_UN : S_UH, sp = s ? S_PVN : S_PVH, sr = s ? S_RN : S_RH;
+            for (int a = 0; a < 3; ++a) {
+                int hi[3];
+                owned_hi(b, L_FX + a, hi);
+                for (int i = 0; i < hi[0]; ++i)
+                    for (int j = 0; j < hi[1]; ++j)
+                        for (int k = 0; k < hi[2]; ++k) {
+                            const double k1 = F(b, S_JD + 3 * a, i, j, k), k2 = F(b, S_JD + 3 * a + 1, i, j, k), k3 = F(b, S_JD + 3 * a + 2, i, j, k);
+                            int iL = i - (a == 0), jL = j - (a == 1), kL = k - (a == 2);
+                            double W[2][5];
+                            for (int side = 0; side < 2; ++side) {
+                                int ii = side ? i : iL, jj = side ? j : jL, kk = side ? k : kL;
+                                W[side][0] = F(b, su, ii, jj, kk);
+                                for (int c = 0; c < 4; ++c) W[side][c + 1] = F(b, sp + c, ii, jj, kk);
+                            }
+                            double Uc[2][5], Fl[2][5], rad[2];
+                            for (int side = 0; side < 2; ++side) {
+                                const double rho = W[side][0], u = W[side][1], v = W[side][2], w = W[side][3], p = W[side][4];
+                                Uc[side][0] = rho;
+                                Uc[side][1] = rho * u;
+                                Uc[side][2] = rho * v;
+                                Uc[side][3] = rho * w;
+                                Uc[side][4] = p / (gm - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
+                                const double uvw = k1 * u + k2 * v + k3 * w;
+                                const double kk2 = (k1 * k1 + k2 * k2 + k3 * k3);
+                                const double c = sqrt(dmax(0.0, gm * p / dmax(rho, 1e-30)));
+                                rad[side] = fabs(uvw) + c * sqrt(kk2);
+                                const double rhoe = p / (gm - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
+                                Fl[side][0] = rho * uvw;
+                                Fl[side][1] = rho * uvw * u + k1 * p;
+                                Fl[side][2] = rho * uvw * v + k2 * p;
+                                Fl[side][3] = rho * uvw * w + k3 * p;
+                                Fl[side][4] = uvw * (rhoe + p);
+                            }
+                            const double r = dmax(rad[0], rad[1]);
+                            for (int m = 0; m < 5; ++m)
+                                F(b, S_F + 5 * a + m, i, j, k) = 0.5 * (Fl[0][m] + Fl[1][m]) - 0.5 * r * (Uc[1][m] - Uc[0][m]);
+                        }
+            }
+            for (int i = 0; i < b->m[0]; ++i)
+                for (int j = 0; j < b->m[1]; ++j)
+                    for (int k = 0; k < b->m[2]; ++k) {
+                        const double invJ = 1.0 / F(b, S_JAC, i, j, k);
+                        for (int m = 0; m < 5; ++m) {
+                            double r = 0.0;
+                            r -= (F(b, S_F + 0 + m, i + 1, j, k) - F(b, S_F + 0 + m, i, j, k)) * invJ;
+                            r -= (F(b, S_F + 5 + m, i, j + 1, k) - F(b, S_F + 5 + m, i, j, k)) * invJ;
+                            r -= (F(b, S_F + 10 + m, i, j, k + 1) - F(b, S_F + 10 + m, i, j, k)) * invJ;
+                            F(b, sr + m, i, j, k) = r;
+                        }
+                    }
+        }
+}
+
+/* AddSourceToRHS_Fluid (MercurySolver_FluidRHS.cpp:206-527) */
+static void add_sources(Orc *o) {
+    const OrcPhys *P = &o->P;
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int i = 0; i < b->m[0]; ++i)
+            for (int j = 0; j < b->m[1]; ++j)
+                for (int k = 0; k < b->m[2]; ++k) {
+                    const double upx = F(b, S_UP, i, j, k), upy = F(b, S_UP + 1, i, j, k), upz = F(b, S_UP + 2, i, j, k);
+                    const double Bx = F(b, S_BC, i, j, k), By = F(b, S_BC + 1, i, j, k), Bz = F(b, S_BC + 2, i, j, k);
+                    const double Jx = F(b, S_JC, i, j, k), Jy = F(b, S_JC + 1, i, j, k), Jz = F(b, S_JC + 2, i, j, k);
+                    const double sjbx = Jy * Bz - Jz * By, sjby = Jz * Bx - Jx * Bz, sjbz = Jx * By - Jy * Bx;
+                    NumInfo num = num_limiter(P, F(b, S_UH, i, j, k), F(b, S_UN, i, j, k));
+                    double dpex = 0.0, dpey = 0.0, dpez = 0.0;
+                    if (P->ambipolar_enabled != 0.0) {
+                        double V = fabs(F(b, S_JAC, i, j, k));
+                        double d[9];
+                        if (V <= 0.0) {
+                            for (int q = 0; q < 9; ++q) d[q] = 0.0;
+                        } else {
+                            for (int c = 0; c < 3; ++c) {
+                                d[c] = 0.5 * (F(b, S_JD + c, i - 1, j, k) + F(b, S_JD + c, i, j, k)) / V;
+                                d[3 + c] = 0.5 * (F(b, S_JD + 3 + c, i, j - 1, k) + F(b, S_JD + 3 + c, i, j, k)) / V;
+                                d[6 + c] = 0.5 * (F(b, S_JD + 6 + c, i, j, k - 1) + F(b, S_JD + 6 + c, i, j, k)) / V;
+                            }
+                        }
+#define PE(a1, a2, a3) (F(b, S_PVH + 3, a1, a2, a3) + F(b, S_PVN + 3, a1, a2, a3))
+                        const double pip = PE(i + 1, j, k), pim = PE(i - 1, j, k), pjp = PE(i, j + 1, k), pjm = PE(i, j - 1, k), pkp = PE(i, j, k + 1), pkm = PE(i, j, k - 1);
+#undef PE
+                        dpex = 0.5 * (d[0] * (pip - pim) + d[3] * (pjp - pjm) + d[6] * (pkp - pkm));
+                        dpey = 0.5 * (d[1] * (pip - pim) + d[4] * (pjp - pjm) + d[7] * (pkp - pkm));
+                        dpez = 0.5 * (d[2] * (pip - pim) + d[5] * (pjp - pjm) + d[8] * (pkp - pkm));
+                    }
+                    const double sss = F(b, S_PHOTO, i, j, k);
+                    const double x = F(b, S_CC, i, j, k), y = F(b, S_CC + 1, i, j, k), z = F(b, S_CC + 2, i, j, k);
+                    const double rho_cyl = sqrt(y * y + z * z);
+                    const int in_shadow = (x < 0.0 && rho_cyl < 1.0);
+                    const double vst = in_shadow ? P->nu_shadow : P->nu_lit;
+                    for (int s = 0; s < 2; ++s) {
+                        int su = s ? S_UN : S_UH, sp = s ? S_PVN : S_PVH, sr = s ? S_RN : S_RH;
+                        const double rho = dmax(F(b, su, i, j, k), 0.0);
+                        const double u = F(b, sp, i, j, k), v = F(b, sp + 1, i, j, k), w = F(b, sp + 2, i, j, k);
+                        const double u2 = u * u + v * v + w * w;
+                        const double subx = (v - upy) * Bz - (w - upz) * By;
+                        const double suby = (w - upz) * Bx - (u - upx) * Bz;
+                        const double subz = (u - upx) * By - (v - upy) * Bx;
+                        const double subu = subx * u + suby * v + subz * w;
+                        const double sjbu = sjbx * u + sjby * v + sjbz * w;
+                        const double dpeu = dpex * u + dpey * v + dpez * w;
+                        const double Ts0 = F(b, sp + 4, i, j, k) * P->T_ref;
+                        const double Ms = s ? P->M_Na : P->M_H;
+                        const double Mo = P->M_Na; /* H+ pairs with (M_H, M_Na), Na+ with (M_Na, M_Na) */
+                        const double b1 = vst * (P->Tn0 - Ts0) * (Ms / (Ms + Mo));
+                        const double b2 = (Mo / (Ms + Mo)) * vst;
+                        const double ns = s ? num.nNa : num.nH;
+                        const double chi = s ? num.wNa : num.wH;
+                        F(b, sr + 1, i, j, k) += P->c_ind * ns * subx + P->c_hall * chi * sjbx;
+                        F(b, sr + 2, i, j, k) += P->c_ind * ns * suby + P->c_hall * chi * sjby;
+                        F(b, sr + 3, i, j, k) += P->c_ind * ns * subz + P->c_hall * chi * sjbz;
+                        F(b, sr + 4, i, j, k) += P->c_ind * ns * subu + P->c_hall * chi * sjbu;
+                        if (P->ambipolar_enabled != 0.0) {
+                            F(b, sr + 1, i, j, k) -= chi * dpex;
+                            F(b, sr + 2, i, j, k) -= chi * dpey;
+                            F(b, sr + 3, i, j, k) -= chi * dpez;
+                            F(b, sr + 4, i, j, k) -= chi * dpeu;
+                        }
+                        if (s) {
+                            F(b, sr + 0, i, j, k) += P->a1_Na * sss;
+                            F(b, sr + 4, i, j, k) += P->a6 * sss * P->Tn0;
+                        }
+                        F(b, sr + 1, i, j, k) -= P->a4 * rho * u * vst;
+                        F(b, sr + 2, i, j, k) -= P->a4 * rho * v * vst;
+                        F(b, sr + 3, i, j, k) -= P->a4 * rho * w * vst;
+                        F(b, sr + 4, i, j, k) -= P->a4 * rho * u2 * vst;
+                        F(b, sr + 4, i, j, k) += P->a5 * ns * b1;
+                        F(b, sr + 4, i, j, k) += P->a4 * rho * u2 * b2;
+                    }
+                }
+    }
+}
+
+/* helpers of AssembleOneDirectionEMF_ (MercurySolver_MagExplicit_Ideal.cpp:433-594) */
+static inline double u_contra(const OBlock *b, int axis, int i, int j, int k) {
+    int ip = i + (axis == 0), jp = j + (axis == 1), kp = k + (axis == 2);
+    const int s0 = S_JD + 3 * axis;
+    const double kx = 0.5 * (F(b, s0, i, j, k) + F(b, s0, ip, jp, kp));
+    const double ky = 0.5 * (F(b, s0 + 1, i, j, k) + F(b, s0 + 1, ip, jp, kp));
+    const double kz = 0.5 * (F(b, s0 + 2, i, j, k) + F(b, s0 + 2, ip, jp, kp));
+    const double inv_jac = 1.0 / (fabs(F(b, S_JAC, i, j, k)) + 1.0e-30);
+    return (kx * F(b, S_UP, i, j, k) + ky * F(b, S_UP + 1, i, j, k) + kz * F(b, S_UP + 2, i, j, k)) * inv_jac;
+}
+static inline double beta_ind(const OBlock *b, int axis, int i, int j, int k) {
+    return 0.5 * (F(b, S_B + axis, i, j, k) + F(b, S_B + axis, i + (axis == 0), j + (axis == 1), k + (axis == 2)));
+}
+static inline double beta_tot(const OBlock *b, int axis, int i, int j, int k) {
+    int ip = i + (axis == 0), jp = j + (axis == 1), kp = k + (axis == 2);
+    return 0.5 * ((F(b, S_B + axis, i, j, k) + F(b, S_BADD + axis, i, j, k)) + (F(b, S_B + axis, ip, jp, kp) + F(b, S_BADD + axis, ip, jp, kp)));
+}
+static inline double emf_flux(const OBlock *b, int ba, int fa, int iL, int jL, int kL, int iR, int jR, int kR, double bff) {
+    const double btL = beta_tot(b, ba, iL, jL, kL), btR = beta_tot(b, ba, iR, jR, kR);
+    const double biL = beta_ind(b, ba, iL, jL, kL), biR = beta_ind(b, ba, iR, jR, kR);
+    const double ufL = u_contra(b, fa, iL, jL, kL), ufR = u_contra(b, fa, iR, jR, kR);
+    const double ubL = u_contra(b, ba, iL, jL, kL), ubR = u_contra(b, ba, iR, jR, kR);
+    const double GL = ufL * btL - ubL * bff, GR = ufR * btR - ubR * bff;
+    const double radius = dmax(fabs(ufL), fabs(ufR));
+    return 0.5 * (GL + GR) - 0.5 * radius * (biR - biL);
+}
+
+/* AddIdealEdgeEMF_ (MercurySolver_MagExplicit_Ideal.cpp:7-55): face candidates on
+ * owned faces +-1 (clamped to the allocation +-1), then
+ * AssembleEdgeEMF_FromFaceE_Ideal_ (:138-372) on owned edges. */
+static void ideal_emf(Orc *o) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int fa = 0; fa < 3; ++fa) {
+            int ohi[3], alo[3], ahi[3], lo[3], hi[3];
+            owned_hi(b, L_FX + fa, ohi);
+            alloc_box(b, L_FX + fa, b->g, alo, ahi);
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = dmax(0 - 1, alo[a] + 1);
+                hi[a] = dmin(ohi[a] + 1, ahi[a] - 1);
+            }
+            for (int i = lo[0]; i < hi[0]; ++i)
+                for (int j = lo[1]; j < hi[1]; ++j)
+                    for (int k = lo[2]; k < hi[2]; ++k) {
+                        int iL = i - (fa == 0), jL = j - (fa == 1), kL = k - (fa == 2);
+                        const double bff = F(b, S_B + fa, i, j, k) + F(b, S_BADD + fa, i, j, k);
+                        double e[3] = {0, 0, 0};
+                        if (fa == 0) {
+                            double Gex = emf_flux(b, 1, 0, iL, jL, kL, i, j, k, bff);
+                            double Gzx = emf_flux(b, 2, 0, iL, jL, kL, i, j, k, bff);
+                            e[1] = Gzx;
+                            e[2] = -Gex;
+                        } else if (fa == 1) {
+                            double Gxe = emf_flux(b, 0, 1, iL, jL, kL, i, j, k, bff);
+                            double Gze = emf_flux(b, 2, 1, iL, jL, kL, i, j, k, bff);
+                            e[0] = -Gze;
+                            e[2] = Gxe;
+                        } else {
+                            double Gxz = emf_flux(b, 0, 2, iL, jL, kL, i, j, k, bff);
+                            double Gez = emf_flux(b, 1, 2, iL, jL, kL, i, j, k, bff);
+                            e[0] = Gez;
+                            e[1] = -Gxz;
+                        }
+                        for (int c = 0; c < 3; ++c) F(b, S_EF + 3 * fa + c, i, j, k) = e[c];
+                    }
+        }
+        const double Cuct = 0.5;
+        int hi[3];
+#define MAXU4(ax, a0, a1, a2, b0, b1, b2, c0, c1, c2, d0, d1, d2)                                                    \
+    dmax(dmax(dmax(dmax(0.0, fabs(u_contra(b, ax, a0, a1, a2))), fabs(u_contra(b, ax, b0, b1, b2))), fabs(u_contra(b, ax, c0, c1, c2))), \
+         fabs(u_contra(b, ax, d0, d1, d2)))
+        owned_hi(b, L_EX, hi);
+        for (int i = 0; i < hi[0]; i++)
+            for (int j = 0; j < hi[1]; j++)
+                for (int k = 0; k < hi[2]; k++) {
+                    const double Ec = 0.25 * (F(b, S_EF + 3, i, j, k) + F(b, S_EF + 3, i, j, k - 1) + F(b, S_EF + 6, i, j, k) + F(b, S_EF + 6, i, j - 1, k));
+                    const double ae = MAXU4(1, i, j, k, i, j - 1, k, i, j, k - 1, i, j - 1, k - 1);
+                    const double az = MAXU4(2, i, j, k, i, j - 1, k, i, j, k - 1, i, j - 1, k - 1);
+                    const double dBz = F(b, S_B + 2, i, j, k) - F(b, S_B + 2, i, j - 1, k);
+                    const double dBe = F(b, S_B + 1, i, j, k) - F(b, S_B + 1, i, j, k - 1);
+                    F(b, S_E + 0, i, j, k) = Ec + Cuct * (0.5 * ae * dBz - 0.5 * az * dBe);
+                }
+        owned_hi(b, L_EY, hi);
+        for (int i = 0; i < hi[0]; i++)
+            for (int j = 0; j < hi[1]; j++)
+                for (int k = 0; k < hi[2]; k++) {
+                    const double Ec = 0.25 * (F(b, S_EF + 1, i, j, k) + F(b, S_EF + 1, i, j, k - 1) + F(b, S_EF + 7, i, j, k) + F(b, S_EF + 7, i - 1, j, k));
+                    const double az = MAXU4(2, i, j, k, i - 1, j, k, i, j, k - 1, i - 1, j, k - 1);
+                    const double ax = MAXU4(0, i, j, k, i - 1, j, k, i, j, k - 1, i - 1, j, k - 1);
+                    const double dBx = F(b, S_B + 0, i, j, k) - F(b, S_B + 0, i, j, k - 1);
+                    const double dBz = F(b, S_B + 2, i, j, k) - F(b, S_B + 2, i - 1, j, k);
+                    F(b, S_E + 1, i, j, k) = Ec + Cuct * (0.5 * az * dBx - 0.5 * ax * dBz);
+                }
+        owned_hi(b, L_EZ, hi);
+        for (int i = 0; i < hi[0]; i++)
+            for (int j = 0; j < hi[1]; j++)
+                for (int k = 0; k < hi[2]; k++) {
+                    const double Ec = 0.25 * (F(b, S_EF + 2, i, j, k) + F(b, S_EF + 2, i, j - 1, k) + F(b, S_EF + 5, i, j, k) + F(b, S_EF + 5, i - 1, j, k));
+                    const double ax = MAXU4(0, i, j, k, i - 1, j, k, i, j - 1, k, i - 1, j - 1, k);
+                    const double ae = MAXU4(1, i, j, k, i - 1, j, k, i, j - 1, k, i - 1, j - 1, k);
+                    const double dBe = F(b, S_B + 1, i, j, k) - F(b, S_B + 1, i - 1, j, k);
+                    const double dBx = F(b, S_B + 0, i, j, k) - F(b, S_B + 0, i, j - 1, k);
+                    F(b, S_E + 2, i, j, k) = Ec + Cuct * (0.5 * ax * dBe - 0.5 * ae * dBx);
+                }
+#undef MAXU4
+    }
+}
+
+/* AddAmbipolarEdgeEMF_ (MercurySolver_MagExplicit_Ambipo.cpp:3-143) */
+static void ambipolar_emf(Orc *o) {
+    const OrcPhys *P = &o->P;
+    if (P->ambipolar_enabled == 0.0) return;
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+#define PEC(a1, a2, a3) (F(b, S_PVH + 3, a1, a2, a3) + F(b, S_PVN + 3, a1, a2, a3))
+        for (int a = 0; a < 3; ++a) {
+            int hi[3];
+            owned_hi(b, L_EX + a, hi);
+            for (int i = 0; i < hi[0]; ++i)
+                for (int j = 0; j < hi[1]; ++j)
+                    for (int k = 0; k < hi[2]; ++k) {
+                        double pe[2];
+                        for (int e = 0; e < 2; ++e) {
+                            int ni = i + (e && a == 0), nj = j + (e && a == 1), nk = k + (e && a == 2);
+                            double sum = 0.0;
+                            for (int di = -1; di <= 0; ++di)
+                                for (int dj = -1; dj <= 0; ++dj)
+                                    for (int dk = -1; dk <= 0; ++dk) sum += PEC(ni + di, nj + dj, nk + dk);
+                            pe[e] = 0.125 * sum;
+                        }
+                        double rH = 0.0, rN = 0.0;
+                        for (int d1 = -1; d1 <= 0; ++d1)
+                            for (int d2 = -1; d2 <= 0; ++d2) {
+                                int ii = i, jj = j, kk = k;
+                                if (a == 0) { jj += d1; kk += d2; }
+                                else if (a == 1) { ii += d1; kk += d2; }
+                                else { ii += d1; jj += d2; }
+                                rH += F(b, S_UH, ii, jj, kk);
+                                rN += F(b, S_UN, ii, jj, kk);
+                            }
+                        NumInfo n = num_limiter(P, 0.25 * rH, 0.25 * rN);
+                        F(b, S_E + a, i, j, k) -= P->pressure_coef * (pe[1] - pe[0]) / n.ne_eff;
+                    }
+        }
+#undef PEC
+    }
+}
+
+/* AddArtificialResistivityToEdgeEMF_ and AddLocalArtificialResistivityToEdgeEMF_
+ * (MercurySolver_MagExplicit_Resis.cpp:6-185) */
+static void arti_emf(Orc *o) {
+    const OrcPhys *P = &o->P;
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int a = 0; a < 3; ++a) {
+            int hi[3];
+            owned_hi(b, L_EX + a, hi);
+            for (int i = 0; i < hi[0]; ++i)
+                for (int j = 0; j < hi[1]; ++j)
+                    for (int k = 0; k < hi[2]; ++k) {
+                        if (P->arti_eta_max > 0.0) {
+                            const double J1 = F(b, S_J + a, i, j, k);
+                            const double dx = F(b, S_DR + 3 * a, i, j, k), dy = F(b, S_DR + 3 * a + 1, i, j, k), dz = F(b, S_DR + 3 * a + 2, i, j, k);
+                            const double drl = sqrt(dx * dx + dy * dy + dz * dz);
+                            const double Jm = fabs(J1) / dmax(drl, 1.0e-30);
+                            double eta;
+                            if (P->J_on <= P->J_start)
+                                eta = (Jm > P->J_start) ? P->arti_eta_max : 0.0;
+                            else {
+                                double q = (Jm - P->J_start) / (P->J_on - P->J_start);
+                                q = dmax(0.0, dmin(1.0, q));
+                                eta = P->arti_eta_max * (q * q * (3.0 - 2.0 * q));
+                            }
+                            F(b, S_E + a, i, j, k) += eta * J1;
+                        }
+                    }
+        }
+    }
+    if (!(P->local_enabled != 0.0 && P->local_eta_max > 0.0 && P->local_r_decay > 0.0)) return;
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int a = 0; a < 3; ++a) {
+            int hi[3];
+            owned_hi(b, L_EX + a, hi);
+            for (int i = 0; i < hi[0]; ++i)
+                for (int j = 0; j < hi[1]; ++j)
+                    for (int k = 0; k < hi[2]; ++k) F(b, S_E + a, i, j, k) += F(b, S_LETA + a, i, j, k) * F(b, S_J + a, i, j, k);
+        }
+    }
+}
+
+/* BuildHallFaceEMF_Rusanov_diff_ (MercurySolver_MagExplicit_Hall.cpp:58-433), no Pole patches */
+static void hall_emf(Orc *o) {
+    const OrcPhys *P = &o->P;
+    const double eps = 1e-14, Cwh = 0.1;
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (long q = 0; q < b->V; ++q) {
+            NumInfo n = num_limiter(P, b->f[S_UH][q], b->f[S_UN][q]);
+            const double alpha = P->hall_coef / n.ne_eff;
+            const double Jx = b->f[S_JC][q], Jy = b->f[S_JC + 1][q], Jz = b->f[S_JC + 2][q];
+            const double Bx = b->f[S_BC][q], By = b->f[S_BC + 1][q], Bz = b->f[S_BC + 2][q];
+            b->f[S_EHC][q] = alpha * (Jy * Bz - Jz * By);
+            b->f[S_EHC + 1][q] = alpha * (Jz * Bx - Jx * Bz);
+            b->f[S_EHC + 2][q] = alpha * (Jx * By - Jy * Bx);
+            b->f[S_HB][q] = fabs(alpha) * sqrt(Bx * Bx + By * By + Bz * Bz);
+        }
+        for (int a = 0; a < 3; ++a) {
+            int hi[3];
+            owned_hi(b, L_EX + a, hi);
+            /* the four cells around the edge, in the reference's order */
+            int c4[4][3];
+            if (a == 0) { int t[4][3] = {{0, -1, -1}, {0, -1, 0}, {0, 0, -1}, {0, 0, 0}}; memcpy(c4, t, sizeof(t)); }
+            else if (a == 1) { int t[4][3] = {{-1, 0, -1}, {-1, 0, 0}, {0, 0, -1}, {0, 0, 0}}; memcpy(c4, t, sizeof(t)); }
+            else { int t[4][3] = {{-1, -1, 0}, {-1, 0, 0}, {0, -1, 0}, {0, 0, 0}}; memcpy(c4, t, sizeof(t)); }
+            for (int i = 0; i < hi[0]; ++i)
+                for (int j = 0; j < hi[1]; ++j)
+                    for (int k = 0; k < hi[2]; ++k) {
+                        double Ec[3];
+                        for (int c = 0; c < 3; ++c)
+                            Ec[c] = 0.25 * (F(b, S_EHC + c, i + c4[0][0], j + c4[0][1], k + c4[0][2]) + F(b, S_EHC + c, i + c4[1][0], j + c4[1][1], k + c4[1][2]) +
+                                            F(b, S_EHC + c, i + c4[2][0], j + c4[2][1], k + c4[2][2]) + F(b, S_EHC + c, i + c4[3][0], j + c4[3][1], k + c4[3][2]));
+                        const double Ecen = Ec[0] * F(b, S_DR + 3 * a, i, j, k) + Ec[1] * F(b, S_DR + 3 * a + 1, i, j, k) + Ec[2] * F(b, S_DR + 3 * a + 2, i, j, k);
+                        const double be = dmax(dmax(F(b, S_HB, i + c4[0][0], j + c4[0][1], k + c4[0][2]), F(b, S_HB, i + c4[1][0], j + c4[1][1], k + c4[1][2])),
+                                               dmax(F(b, S_HB, i + c4[2][0], j + c4[2][1], k + c4[2][2]), F(b, S_HB, i + c4[3][0], j + c4[3][1], k + c4[3][2])));
+                        const double L = dmax(F(b, S_DL + a, i, j, k), eps);
+                        double val;
+                        if (a == 0) {
+                            const double Az = 0.5 * (F(b, S_AREA + 2, i, j, k) + F(b, S_AREA + 2, i, j - 1, k));
+                            const double ihe = L / dmax(Az, eps);
+                            const double mue = Cwh * be * ihe * ihe;
+                            const double Ae = 0.5 * (F(b, S_AREA + 1, i, j, k) + F(b, S_AREA + 1, i, j, k - 1));
+                            const double ihz = L / dmax(Ae, eps);
+                            const double muz = Cwh * be * ihz * ihz;
+                            const double d1 = F(b, S_B + 2, i, j, k) - F(b, S_B + 2, i, j - 1, k);
+                            const double d2 = F(b, S_B + 1, i, j, k) - F(b, S_B + 1, i, j, k - 1);
+                            val = Ecen + 0.5 * mue * d1 - 0.5 * muz * d2;
+                        } else if (a == 1) {
+                            const double Ax = 0.5 * (F(b, S_AREA + 0, i, j, k) + F(b, S_AREA + 0, i, j, k - 1));
+                            const double ihz = L / dmax(Ax, eps);
+                            const double muz = Cwh * be * ihz * ihz;
+                            const double Az = 0.5 * (F(b, S_AREA + 2, i, j, k) + F(b, S_AREA + 2, i - 1, j, k));
+                            const double ihx = L / dmax(Az, eps);
+                            const double mux = Cwh * be * ihx * ihx;
+                            const double d1 = F(b, S_B + 0, i, j, k) - F(b, S_B + 0, i, j, k - 1);
+                            const double d2 = F(b, S_B + 2, i, j, k) - F(b, S_B + 2, i - 1, j, k);
+                            val = Ecen + 0.5 * muz * d1 - 0.5 * mux * d2;
+                        } else {
+                            const double Ae = 0.5 * (F(b, S_AREA + 1, i, j, k) + F(b, S_AREA + 1, i - 1, j, k));
+                            const double ihx = L / dmax(Ae, eps);
+                            const double mux = Cwh * be * ihx * ihx;
+                            const double Ax = 0.5 * (F(b, S_AREA + 0, i, j, k) + F(b, S_AREA + 0, i, j - 1, k));
+                            const double ihe = L / dmax(Ax, eps);
+                            const double mue = Cwh * be * ihe * ihe;
+                            const double d1 = F(b, S_B + 1, i, j, k) - F(b, S_B + 1, i - 1, j, k);
+                            const double d2 = F(b, S_B + 0, i, j, k) - F(b, S_B + 0, i, j - 1, k);
+                            val = Ecen + 0.5 * mux * d1 - 0.5 * mue * d2;
+                        }
+                        F(b, S_EH + a, i, j, k) = F(b, S_TAPER + a, i, j, k) * val;
+                    }
+        }
+    }
+}
+
+/* ApplyStationaryWallNonResistiveEMF_ (MercurySolver_MagneticRHS.cpp:5-89):
+ * tangential E on the owned wall edges is zeroed. */
+static void wall_zero_e(Orc *o) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int f = 0; f < 6; ++f) {
+            if (b->bc[f] != BC_WALL) continue;
+            int nax = f / 2;
+            for (int a = 0; a < 3; ++a) {
+                if (a == nax) continue;
+                int hi[3], lo[3] = {0, 0, 0};
+                owned_hi(b, L_EX + a, hi);
+                if (f & 1) lo[nax] = b->m[nax]; else hi[nax] = 1;
+                if (f & 1) hi[nax] = b->m[nax] + 1;
+                for (int i = lo[0]; i < hi[0]; ++i)
+                    for (int j = lo[1]; j < hi[1]; ++j)
+                        for (int k = lo[2]; k < hi[2]; ++k) F(b, S_E + a, i, j, k) = 0.0;
+            }
+        }
+    }
+}
+
+/* AssembleRHS_Induction_CT_ (MercurySolver_MagneticRHS.cpp:90-171) with
+ * CTOperators::CurlEdgeToFace(-1) (CTOperators.cpp:6-57) */
+static void induction_ct(Orc *o) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int a = 0; a < 3; ++a) memset(b->f[S_E + a], 0, sizeof(double) * b->V);
+    }
+    ideal_emf(o);
+    ambipolar_emf(o);
+    arti_emf(o);
+    alias_reduce_edges(o, S_E);
+    hall_emf(o);
+    sync_group(o, G_EHALL);
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int a = 0; a < 3; ++a) {
+            int hi[3];
+            owned_hi(b, L_EX + a, hi);
+            for (int i = 0; i < hi[0]; ++i)
+                for (int j = 0; j < hi[1]; ++j)
+                    for (int k = 0; k < hi[2]; ++k) F(b, S_E + a, i, j, k) += F(b, S_EH + a, i, j, k);
+        }
+    }
+    wall_zero_e(o);
+    sync_group(o, G_EEDGE);
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        int hi[3];
+        owned_hi(b, L_FX, hi);
+        for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+                for (int k = 0; k < hi[2]; ++k) {
+                    const double c = (F(b, S_E + 1, i, j, k) - F(b, S_E + 1, i, j, k + 1)) + (F(b, S_E + 2, i, j + 1, k) - F(b, S_E + 2, i, j, k));
+                    F(b, S_RHSB + 0, i, j, k) = -1.0 * c;
+                }
+        owned_hi(b, L_FY, hi);
+        for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+                for (int k = 0; k < hi[2]; ++k) {
+                    const double c = (F(b, S_E + 0, i, j, k + 1) - F(b, S_E + 0, i, j, k)) + (F(b, S_E + 2, i, j, k) - F(b, S_E + 2, i + 1, j, k));
+                    F(b, S_RHSB + 1, i, j, k) = -1.0 * c;
+                }
+        owned_hi(b, L_FZ, hi);
+        for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+                for (int k = 0; k < hi[2]; ++k) {
+                    const double c = (F(b, S_E + 0, i, j, k) - F(b, S_E + 0, i, j + 1, k)) + (F(b, S_E + 1, i + 1, j, k) - F(b, S_E + 1, i, j, k));
+                    F(b, S_RHSB + 2, i, j, k) = -1.0 * c;
+                }
+    }
+}
+
+/* ApplyUpdate_Euler_ (MercurySolver_Advance.cpp:73-147), owned DOFs */
+static void apply_update(Orc *o, double dt) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int i = 0; i < b->m[0]; ++i)
+            for (int j = 0; j < b->m[1]; ++j)
+                for (int k = 0; k < b->m[2]; ++k)
+                    for (int m = 0; m < 5; ++m) {
+                        F(b, S_UH + m, i, j, k) += dt * F(b, S_RH + m, i, j, k);
+                        F(b, S_UN + m, i, j, k) += dt * F(b, S_RN + m, i, j, k);
+                    }
+        for (int a = 0; a < 3; ++a) {
+            int hi[3];
+            owned_hi(b, L_FX + a, hi);
+            for (int i = 0; i < hi[0]; ++i)
+                for (int j = 0; j < hi[1]; ++j)
+                    for (int k = 0; k < hi[2]; ++k) F(b, S_B + a, i, j, k) += dt * F(b, S_RHSB + a, i, j, k);
+        }
+    }
+}
+
+/* SSP-RK2 second-stage average U <- 0.5*(U0 + U), owned DOFs (no reference
+ * counterpart: the reference integrator is forward Euler) */
+static void rk2_average(Orc *o) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int i = 0; i < b->m[0]; ++i)
+            for (int j = 0; j < b->m[1]; ++j)
+                for (int k = 0; k < b->m[2]; ++k)
+                    for (int m = 0; m < 5; ++m) {
+                        F(b, S_UH + m, i, j, k) = 0.5 * (F(b, S_U0H + m, i, j, k) + F(b, S_UH + m, i, j, k));
+                        F(b, S_UN + m, i, j, k) = 0.5 * (F(b, S_U0N + m, i, j, k) + F(b, S_UN + m, i, j, k));
+                    }
+        for (int a = 0; a < 3; ++a) {
+            int hi[3];
+            owned_hi(b, L_FX + a, hi);
+            for (int i = 0; i < hi[0]; ++i)
+                for (int j = 0; j < hi[1]; ++j)
+                    for (int k = 0; k < hi[2]; ++k) F(b, S_B + a, i, j, k) = 0.5 * (F(b, S_B0 + a, i, j, k) + F(b, S_B + a, i, j, k));
+        }
+    }
+}
+
+static void save_state(Orc *o) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int m = 0; m < 5; ++m) {
+            memcpy(b->f[S_U0H + m], b->f[S_UH + m], sizeof(double) * b->V);
+            memcpy(b->f[S_U0N + m], b->f[S_UN + m], sizeof(double) * b->V);
+        }
+        for (int a = 0; a < 3; ++a) memcpy(b->f[S_B0 + a], b->f[S_B + a], sizeof(double) * b->V);
+    }
+}
+
+/* one evaluation of the stage operator and Euler update:
+ * StepOnce (MercurySolver_Driver.cpp:21-61) without the dt computation */
+static void stage(Orc *o, double dt) {
+    scheme_u(o);
+    add_sources(o);
+    induction_ct(o);
+    apply_update(o, dt);
+    sync_group(o, G_UCELL);
+    sync_group(o, G_BFACE);
+    update_derived(o);
+}
+
+/* ------------------------------------------------------------------------- */
+/* C ABI for the tests                                                        */
+/* ------------------------------------------------------------------------- */
+
+void *orc_create(int nb, const int *dims, int g, const int *bc, const double *nodes, const OrcPhys *P) {
+    Orc *o = (Orc *)calloc(1, sizeof(Orc));
+    o->nb = nb;
+    o->b = (OBlock *)calloc(nb, sizeof(OBlock));
+    o->P = *P;
+    long noff = 0;
+    for (int ib = 0; ib < nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        b->g = g;
+        for (int a = 0; a < 3; ++a) {
+            b->m[a] = dims[3 * ib + a];
+            b->N[a] = b->m[a] + 1 + 2 * g;
+            b->Nn[a] = b->m[a] + 1 + 2 * (g + 1);
+        }
+        for (int f = 0; f < 6; ++f) b->bc[f] = bc[6 * ib + f];
+        b->V = (long)b->N[0] * b->N[1] * b->N[2];
+        b->Vn = (long)b->Nn[0] * b->Nn[1] * b->Nn[2];
+        b->x = (double *)calloc(b->Vn, sizeof(double));
+        b->y = (double *)calloc(b->Vn, sizeof(double));
+        b->z = (double *)calloc(b->Vn, sizeof(double));
+        for (int s = 0; s < S_NSLOT; ++s) b->f[s] = (double *)calloc(b->V, sizeof(double));
+        /* owned nodes (m+1)^3, k fastest, x then y then z planes */
+        long nn = (long)(b->m[0] + 1) * (b->m[1] + 1) * (b->m[2] + 1);
+        long q = 0;
+        for (int i = 0; i <= b->m[0]; ++i)
+            for (int j = 0; j <= b->m[1]; ++j)
+                for (int k = 0; k <= b->m[2]; ++k, ++q) {
+                    XN(b, i, j, k) = nodes[noff + q];
+                    YN(b, i, j, k) = nodes[noff + nn + q];
+                    ZN(b, i, j, k) = nodes[noff + 2 * nn + q];
+                }
+        noff += 3 * nn;
+        add_ghostmesh(b);
+        dual_grid(b);
+        metric(b);
+        recon_weights(b);
+        edge_statics(b, &o->P);
+    }
+    return o;
+}
+
+void orc_destroy(void *h) {
+    Orc *o = (Orc *)h;
+    if (!o) return;
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        free(b->x); free(b->y); free(b->z);
+        for (int s = 0; s < S_NSLOT; ++s) free(b->f[s]);
+    }
+    free(o->b);
+    free(o);
+}
+
+int orc_nslot(void) { return S_NSLOT; }
+long orc_block_volume(void *h, int ib) { return ((Orc *)h)->b[ib].V; }
+
+/* slot lookup by reference field name (Z4_Mercury/include/0_MercuryFieldCatalog.h:70-202,
+ * metric names of src/7_metric/Metric.cpp:12-76) */
+int orc_slot(const char *name) {
+    static const struct { const char *n; int s; } T[] = {
+        {"U_H", S_UH}, {"U_Na", S_UN}, {"PV_H", S_PVH}, {"PV_Na", S_PVN}, {"U_plus", S_UP}, {"B_cell", S_BC},
+        {"Bind_cell", S_BI}, {"J_cell", S_JC}, {"RHS_H", S_RH}, {"RHS_Na", S_RN}, {"Ehc", S_EHC}, {"hall_beta", S_HB},
+        {"Jac", S_JAC}, {"Bcell_from_Bface_w", S_WB}, {"Jcell_from_Jedge_w", S_WJ}, {"Photo_rate", S_PHOTO},
+        {"cell_center", S_CC}, {"divB", S_DIVB},
+        {"B_xi", S_B}, {"B_eta", S_B + 1}, {"B_zeta", S_B + 2}, {"Badd_xi", S_BADD}, {"Badd_eta", S_BADD + 1}, {"Badd_zeta", S_BADD + 2},
+        {"RHS_B_xi", S_RHSB}, {"RHS_B_eta", S_RHSB + 1}, {"RHS_B_zeta", S_RHSB + 2},
+        {"JDxi", S_JD}, {"JDet", S_JD + 3}, {"JDze", S_JD + 6}, {"Area_xi", S_AREA}, {"Area_eta", S_AREA + 1}, {"Area_zeta", S_AREA + 2},
+        {"beta_xi", S_BETA}, {"beta_eta", S_BETA + 1}, {"beta_zeta", S_BETA + 2},
+        {"Eface_xi", S_EF}, {"Eface_eta", S_EF + 3}, {"Eface_zeta", S_EF + 6},
+        {"E_xi", S_E}, {"E_eta", S_E + 1}, {"E_zeta", S_E + 2}, {"Ehall_xi", S_EH}, {"Ehall_eta", S_EH + 1}, {"Ehall_zeta", S_EH + 2},
+        {"J_xi", S_J}, {"J_eta", S_J + 1}, {"J_zeta", S_J + 2}, {"dr_xi", S_DR}, {"dr_eta", S_DR + 3}, {"dr_zeta", S_DR + 6},
+        {"dl_xi", S_DL}, {"dl_eta", S_DL + 1}, {"dl_zeta", S_DL + 2}, {"alpha_xi", S_ALPHA}, {"alpha_eta", S_ALPHA + 1}, {"alpha_zeta", S_ALPHA + 2},
+        {"taper_xi", S_TAPER}, {"taper_eta", S_TAPER + 1}, {"taper_zeta", S_TAPER + 2},
+        {"eta_local_xi", S_LETA}, {"eta_local_eta", S_LETA + 1}, {"eta_local_zeta", S_LETA + 2}};
+    for (unsigned q = 0; q < sizeof(T) / sizeof(T[0]); ++q)
+        if (!strcmp(T[q].n, name)) return T[q].s;
+    return -1;
+}
+
+int orc_get(void *h, int ib, int slot, double *out) {
+    Orc *o = (Orc *)h;
+    if (ib < 0 || ib >= o->nb || slot < 0 || slot >= S_NSLOT) return -1;
+    memcpy(out, o->b[ib].f[slot], sizeof(double) * o->b[ib].V);
+    return 0;
+}
+int orc_set(void *h, int ib, int slot, const double *in) {
+    Orc *o = (Orc *)h;
+    if (ib < 0 || ib >= o->nb || slot < 0 || slot >= S_NSLOT) return -1;
+    memcpy(o->b[ib].f[slot], in, sizeof(double) * o->b[ib].V);
+    return 0;
+}
+int orc_get_nodes(void *h, int ib, double *out) {
+    Orc *o = (Orc *)h;
+    OBlock *b = &o->b[ib];
+    memcpy(out, b->x, sizeof(double) * b->Vn);
+    memcpy(out + b->Vn, b->y, sizeof(double) * b->Vn);
+    memcpy(out + 2 * b->Vn, b->z, sizeof(double) * b->Vn);
+    return 0;
+}
+
+/* B = Bu . JD + d1(A) over the allocated face ranges; Badd = B_imf . JD.
+ * (initial condition of the configurations; the face flux of a uniform field is
+ * its dot product with the face area vector, as in DipoleField::Build_Badd_FaceFlux,
+ * Z4_Mercury/include/operators/Dipole.h:89-374, for the IMF part) */
+int orc_init_bface(void *h, const double *Bu, const double *const *A) {
+    Orc *o = (Orc *)h;
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        const double *Ax = A ? A[3 * ib] : NULL, *Ay = A ? A[3 * ib + 1] : NULL, *Az = A ? A[3 * ib + 2] : NULL;
+#define AA(p, i, j, k) ((p) ? (p)[IX(b, i, j, k)] : 0.0)
+        for (int a = 0; a < 3; ++a) {
+            int lo[3], hi[3];
+            alloc_box(b, L_FX + a, b->g, lo, hi);
+            for (int i = lo[0]; i < hi[0]; ++i)
+                for (int j = lo[1]; j < hi[1]; ++j)
+                    for (int k = lo[2]; k < hi[2]; ++k) {
+                        double jd0 = F(b, S_JD + 3 * a, i, j, k), jd1 = F(b, S_JD + 3 * a + 1, i, j, k), jd2 = F(b, S_JD + 3 * a + 2, i, j, k);
+                        double c;
+                        if (a == 0) c = (AA(Ay, i, j, k) - AA(Ay, i, j, k + 1)) + (AA(Az, i, j + 1, k) - AA(Az, i, j, k));
+                        else if (a == 1) c = (AA(Ax, i, j, k + 1) - AA(Ax, i, j, k)) + (AA(Az, i, j, k) - AA(Az, i + 1, j, k));
+                        else c = (AA(Ax, i, j, k) - AA(Ax, i, j + 1, k)) + (AA(Ay, i + 1, j, k) - AA(Ay, i, j, k));
+                        F(b, S_B + a, i, j, k) = (Bu[0] * jd0 + Bu[1] * jd1 + Bu[2] * jd2) + c;
+                        F(b, S_BADD + a, i, j, k) = o->P.B_imf[0] * jd0 + o->P.B_imf[1] * jd1 + o->P.B_imf[2] * jd2;
+                    }
+        }
+#undef AA
+    }
+    return 0;
+}
+
+/* MercurySolver ctor tail + Advance head: Sync(Badd), Sync(Bface),
+ * UpdateDerivedFields_ (MercurySolver.cpp:141-145, MercurySolver_Driver.cpp:6-19) */
+int orc_prepare(void *h) {
+    Orc *o = (Orc *)h;
+    sync_group(o, G_BADD);
+    sync_group(o, G_BFACE);
+    update_derived(o);
+    return 0;
+}
+
+/* n steps; integrator 0 = forward Euler (reference StepOnce), 1 = SSP-RK2 */
+int orc_step(void *h, int nsteps, int integrator, double *dt_out) {
+    Orc *o = (Orc *)h;
+    for (int n = 0; n < nsteps; ++n) {
+        double dt = compute_timestep(o);
+        if (!isfinite(dt) || dt < 1e-7) return -2;
+        o->dt = dt;
+        if (integrator == 0) {
+            stage(o, dt);
+        } else {
+            save_state(o);
+            stage(o, dt);
+            stage(o, dt);
+            rk2_average(o);
+            sync_group(o, G_UCELL);
+            sync_group(o, G_BFACE);
+            update_derived(o);
+        }
+        o->time += dt;
+        o->step += 1;
+        if (dt_out) dt_out[n] = dt;
+    }
+    return 0;
+}
+
+/* individual operators, for per-kernel parity tests */
+int orc_op(void *h, const char *op, double arg) {
+    Orc *o = (Orc *)h;
+    if (!strcmp(op, "pv")) calc_pv(o);
+    else if (!strcmp(op, "uplus")) calc_uplus(o);
+    else if (!strcmp(op, "bcell")) calc_bcell(o);
+    else if (!strcmp(op, "jedge")) calc_jedge(o);
+    else if (!strcmp(op, "jcell")) calc_jcell(o);
+    else if (!strcmp(op, "derived")) update_derived(o);
+    else if (!strcmp(op, "flux")) scheme_u(o);
+    else if (!strcmp(op, "sources")) add_sources(o);
+    else if (!strcmp(op, "ideal_emf")) ideal_emf(o);
+    else if (!strcmp(op, "hall_emf")) hall_emf(o);
+    else if (!strcmp(op, "induction")) induction_ct(o);
+    else if (!strcmp(op, "alias_e")) alias_reduce_edges(o, S_E);
+    else if (!strcmp(op, "alias_j")) alias_reduce_edges(o, S_J);
+    else if (!strcmp(op, "update")) apply_update(o, arg);
+    else if (!strcmp(op, "sync_ucell")) sync_group(o, G_UCELL);
+    else if (!strcmp(op, "sync_bface")) sync_group(o, G_BFACE);
+    else if (!strcmp(op, "sync_eedge")) sync_group(o, G_EEDGE);
+    else if (!strcmp(op, "sync_ehall")) sync_group(o, G_EHALL);
+    else if (!strcmp(op, "sync_jedge")) sync_group(o, G_JEDGE);
+    else if (!strcmp(op, "sync_bcell")) sync_group(o, G_BCELL);
+    else if (!strcmp(op, "sync_jcell")) sync_group(o, G_JCELL);
+    else if (!strcmp(op, "sync_badd")) sync_group(o, G_BADD);
+    else return -1;
+    return 0;
+}
+
+double orc_timestep(void *h) { return compute_timestep((Orc *)h); }
+
+/* calc_divB (MercurySolver_MagneticCell.cpp:151-241) and the normalised
+ * diagnostic max|d2 B| / max|B_face| over owned cells / faces */
+double orc_divb(void *h, double *maxabs_out) {
+    Orc *o = (Orc *)h;
+    double dmaxv = 0.0, bmax = 0.0;
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int i = 0; i < b->m[0]; ++i)
+            for (int j = 0; j < b->m[1]; ++j)
+                for (int k = 0; k < b->m[2]; ++k) {
+                    double d = (F(b, S_B, i + 1, j, k) - F(b, S_B, i, j, k) + F(b, S_B + 1, i, j + 1, k) - F(b, S_B + 1, i, j, k) +
+                                F(b, S_B + 2, i, j, k + 1) - F(b, S_B + 2, i, j, k));
+                    F(b, S_DIVB, i, j, k) = d / F(b, S_JAC, i, j, k);
+                    dmaxv = dmax(dmaxv, fabs(d));
+                }
+        for (int a = 0; a < 3; ++a) {
+            int hi[3];
+            owned_hi(b, L_FX + a, hi);
+            for (int i = 0; i < hi[0]; ++i)
+                for (int j = 0; j < hi[1]; ++j)
+                    for (int k = 0; k < hi[2]; ++k) bmax = dmax(bmax, fabs(F(b, S_B + a, i, j, k)));
+        }
+    }
+    if (maxabs_out) *maxabs_out = dmaxv;
+    return bmax > 0.0 ? dmaxv / bmax : dmaxv;
+}

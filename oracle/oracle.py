@@ -1,0 +1,141 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, always as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from mpcns_mercury_amd.params import MercuryPhysics
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_NCOMP = {"U_H": 5, "U_Na": 5, "PV_H": 5, "PV_Na": 5, "U_plus": 3, "B_cell": 3, "Bind_cell": 3, "J_cell": 3,
+          "RHS_H": 5, "RHS_Na": 5, "Ehc": 3, "Bcell_from_Bface_w": 18, "Jcell_from_Jedge_w": 36,
+          "cell_center": 3, "JDxi": 3, "JDet": 3, "JDze": 3, "Eface_xi": 3, "Eface_eta": 3, "Eface_zeta": 3,
+          "dr_xi": 3, "dr_eta": 3, "dr_zeta": 3}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [ctypes.c_int, ip, ctypes.c_int, ip, dp, ctypes.POINTER(MercuryPhysics)]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_slot.argtypes = [ctypes.c_char_p]
+        L.orc_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.orc_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.orc_get_nodes.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_block_volume.restype = ctypes.c_long
+        L.orc_block_volume.argtypes = [vp, ctypes.c_int]
+        L.orc_init_bface.argtypes = [vp, dp, ctypes.POINTER(dp)]
+        L.orc_prepare.argtypes = [vp]
+        L.orc_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.orc_op.argtypes = [vp, ctypes.c_char_p, ctypes.c_double]
+        L.orc_timestep.restype = ctypes.c_double
+        L.orc_timestep.argtypes = [vp]
+        L.orc_divb.restype = ctypes.c_double
+        L.orc_divb.argtypes = [vp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """One oracle instance holding every block of a case."""
+
+    def __init__(self, case):
+        L = lib()
+        self.case = case
+        self.phys = case.phys
+        dims = case.dims_array()
+        bc = case.bc_array()
+        nodes = case.nodes_flat()
+        self.h = L.orc_create(case.nb, dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), case.g,
+                              bc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(nodes), ctypes.byref(self.phys))
+        for ib, fl in enumerate(case.fields):
+            for name, arr in fl.items():
+                self.set(name, ib, arr)
+        self._A = None
+        if case.A is not None:
+            flat = [np.ascontiguousarray(case.A[ib][a]) for ib in range(case.nb) for a in range(3)]
+            self._A = flat
+            ptrs = (ctypes.POINTER(ctypes.c_double) * len(flat))(*[_dp(x) for x in flat])
+            L.orc_init_bface(self.h, _dp(np.array(case.Bu, dtype=np.float64)), ptrs)
+        else:
+            L.orc_init_bface(self.h, _dp(np.array(case.Bu, dtype=np.float64)), None)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().orc_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def slot(self, name):
+        s = lib().orc_slot(name.encode())
+        if s < 0:
+            raise KeyError(name)
+        return s
+
+    def get(self, name, ib=0):
+        nc = _NCOMP.get(name, 1)
+        s = self.slot(name)
+        shape = self.case.box(ib)
+        out = np.empty((nc,) + shape)
+        for c in range(nc):
+            buf = np.empty(shape)
+            lib().orc_get(self.h, ib, s + c, _dp(buf))
+            out[c] = buf
+        return out
+
+    def set(self, name, ib, arr):
+        s = self.slot(name)
+        arr = np.asarray(arr, dtype=np.float64)
+        shape = self.case.box(ib)
+        arr = arr.reshape((-1,) + shape)
+        for c in range(arr.shape[0]):
+            buf = np.ascontiguousarray(arr[c])
+            lib().orc_set(self.h, ib, s + c, _dp(buf))
+
+    def prepare(self):
+        lib().orc_prepare(self.h)
+
+    def step(self, n=1, integrator=0):
+        dts = np.zeros(max(n, 1))
+        rc = lib().orc_step(self.h, n, integrator, _dp(dts))
+        if rc != 0:
+            raise RuntimeError(f"oracle step failed ({rc})")
+        return dts[:n]
+
+    def op(self, name, arg=0.0):
+        rc = lib().orc_op(self.h, name.encode(), float(arg))
+        if rc != 0:
+            raise KeyError(name)
+
+    def timestep(self):
+        return lib().orc_timestep(self.h)
+
+    def divb(self):
+        m = ctypes.c_double(0.0)
+        r = lib().orc_divb(self.h, ctypes.byref(m))
+        return r, m.value
