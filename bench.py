@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Benchmark of the Mercury explicit step on one MI355X.
+
+Workload (default): two-fluid Hall MHD, FP64 plain GPU form -- H+ and Na+ plus
+electron pressure on six spherical-shell sector blocks of 96x64x64, 128x48x64,
+64x96x80, 80x80x48, 112x56x72 and 72x72x96 cells (2 534 400 cells), r in [1,4],
+inflow u = (-1,0,0), IMF (0,0,-0.2), conducting inner sphere, far field
+elsewhere, seeded 1e-3 density noise.  One "step" is one explicit step of every
+block (forward Euler, the reference integrator), replayed from a captured HIP
+graph with all inputs resident in HBM.
+
+Prints one JSON line:  value = cell-updates/s of the whole job; roofline = the
+dominant kernel's algorithmic HBM bytes / its average duration (HIP events on the
+solver stream); cpu_baseline = the CPU restatement (single thread) on a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from mpcns_mercury_amd import cases  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic (compulsory) HBM bytes per unit of each kernel of the FP64 plain
+# path: every input array read once, every output written once, neighbours
+# served from cache.  Units: owned cells (cell kernels) or node-box points
+# (edge/face kernels, one point carries up to 3 edges or faces).
+ALGO_BYTES = {
+    # U_s (2x5) + PV_s u,v,w,p (2x4) + JD (9) + Jac (1) read, RHS_s (2x5) written
+    "flux": 8 * (10 + 8 + 9 + 1 + 10),
+    # U rho (2) + PV (2x5) + U+ (3) + B_cell (3) + J_cell (3) + Photo + centre (3) + Jac + JD (9) + RHS r/w (2x10)
+    "sources": 8 * (2 + 10 + 3 + 3 + 3 + 1 + 3 + 1 + 9 + 20),
+    # B (3) + Badd (3) + U+ (3) + JD (9) + Jac (1) read, Eface (9) written
+    "eface": 8 * (3 + 3 + 3 + 9 + 1 + 9),
+    # Eface (9) + U+ (3) + JD (9) + Jac (1) + B (3) + PV p (2) + U rho (2) + J (3) + eta_local (3) read, E (3) written
+    "edge_emf": 8 * (9 + 3 + 9 + 1 + 3 + 2 + 2 + 3 + 3 + 3),
+    # U rho (2) + J_cell (3) + B_cell (3) read, Ehc (3) + beta (1) written
+    "hall_cell": 8 * (2 + 3 + 3 + 3 + 1),
+    # Ehc (3) + beta (1) + dr (9) + Area (3) + dl (3) + B (3) + taper (3) read, Ehall (3) written
+    "hall_edge": 8 * (3 + 1 + 9 + 3 + 3 + 3 + 3 + 3),
+    # U_s (10) read, PV_s (10) + U+ (3) written
+    "pv_uplus": 8 * (10 + 10 + 3),
+    # B (3) + Badd (3) + W_b (18) read, B_cell + Bind (6) written
+    "bcell": 8 * (3 + 3 + 18 + 6),
+    # B (3) + beta (3) + alpha (3) read, J (3) written
+    "jedge": 8 * (3 + 3 + 3 + 3),
+    # J (3) + W_j (36) read, J_cell (3) written
+    "jcell": 8 * (3 + 36 + 3),
+    # U_s (10) + B_cell (3) + JD (9) + Jac (1)
+    "timestep": 8 * (10 + 3 + 9 + 1),
+}
+# whole-stage compulsory bytes per cell of this configuration (N_s = 2, Hall, Badd):
+# 8 * [2(5N_s+3) + 10 + 60 + 2 + 3 + 18] = 952
+STAGE_BYTES = 8 * (2 * (5 * 2 + 3) + 10 + 60 + 2 + 3 + 18)
+
+
+def make_case(name: str):
+    if name == "shell6":
+        return cases.make_shell(steps=200)
+    if name.startswith("shell6x"):
+        return cases.make_shell(scale=float(name[7:]), steps=200)
+    if name == "cube48":
+        return cases.make_periodic_cube(n=(48, 48, 48), steps=100)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def rel_l2(a, b):
+    d = np.sqrt(np.sum((a - b) ** 2))
+    n = np.sqrt(np.sum(b ** 2))
+    return float(d / n) if n > 0 else float(d)
+
+
+def cpu_baseline(case, sample_steps: int):
+    """Time the CPU restatement (single thread) on `sample_steps` full steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # noqa: E402
+    oracle.build()
+    o = oracle.Oracle(case)
+    o.prepare()
+    t0 = time.perf_counter()
+    o.step(sample_steps, case.integrator)
+    dt = time.perf_counter() - t0
+    return o, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="shell6")
+    ap.add_argument("--cpu-sample-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-ops", action="store_true", help="print per-op timings to stderr")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); every block of the "
+              "workload lives on one MI355X and there is no multi-GPU path.", file=sys.stderr)
+        sys.exit(2)
+
+    from mpcns_mercury_amd.hip import Mercury  # noqa: E402
+
+    case = make_case(args.workload)
+    ncells = case.ncells()
+    print(f"[bench] workload={case.name} blocks={[b.dims for b in case.blocks]} cells={ncells}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    g = Mercury(case)
+    g.prepare()
+    print(f"[bench] setup {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    # parity sample: the same steps on GPU and CPU from the same initial state
+    err = None
+    cpu = None
+    if not args.no_cpu:
+        o, tcpu = cpu_baseline(case, args.cpu_sample_steps)
+        g.step(args.cpu_sample_steps, case.integrator)
+        err = {}
+        for name in ("U_H", "U_Na", "B_xi", "B_eta", "B_zeta"):
+            num, den = 0.0, 0.0
+            for ib in range(case.nb):
+                a, b = g.get(name, ib), o.get(name, ib)
+                num += float(np.sum((a - b) ** 2))
+                den += float(np.sum(b ** 2))
+            err[name] = (num / den) ** 0.5 if den > 0 else num ** 0.5
+        cpu = {"value": ncells * args.cpu_sample_steps / tcpu, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_sample_steps} full steps of {case.name} ({ncells} cells), CPU oracle, 1 thread, "
+                         f"{tcpu:.1f} s"}
+        print(f"[bench] cpu sample {tcpu:.1f}s, rel-L2 vs oracle {err}", file=sys.stderr, flush=True)
+        del o
+
+    g.graph_capture(case.integrator)
+    g.graph_launch(args.warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.graph_launch(args.steps)
+    g.synchronize()
+    elapsed = time.perf_counter() - t0
+    dt_last = g.timestep()
+    if not (np.isfinite(dt_last) and dt_last > 1e-7):
+        raise SystemExit(f"bench: dt underflow after the timed steps (dt={dt_last})")
+    ms_step = elapsed * 1e3 / args.steps
+    value = ncells * args.steps / elapsed
+    divb_norm, _ = g.divb()
+    print(f"[bench] {args.steps} steps {elapsed:.3f}s  {ms_step:.3f} ms/step  {value:.4e} cell-updates/s  divB={divb_norm:.2e}",
+          file=sys.stderr, flush=True)
+
+    # per-kernel device times (HIP events on the solver stream)
+    ops = ["flux", "sources", "eface", "edge_emf", "hall_cell", "hall_edge", "pv_uplus", "bcell", "jedge", "jcell",
+           "timestep"]
+    times = {op: g.time_op(op, 20) for op in ops}
+    step_graph_ms = g.time_op("step_graph", 10)
+    dom = max(times, key=times.get)
+    units = ncells if dom in ("flux", "sources", "bcell", "jcell", "timestep") else ncells  # node-box points ~ cells
+    algo = ALGO_BYTES[dom] * units
+    achieved = algo / (times[dom] * 1e-3) / 1e9
+    if args.profile_ops:
+        for op in ops:
+            print(f"[bench] op {op:10s} {times[op] * 1e3:9.1f} us  {ALGO_BYTES[op] * ncells / (times[op] * 1e-3) / 1e9:8.1f} GB/s",
+                  file=sys.stderr)
+        print(f"[bench] step (graph) {step_graph_ms:.3f} ms", file=sys.stderr)
+    step_gbs = STAGE_BYTES * ncells / (step_graph_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": "cell-updates/s (mixed vs FP64 GPU vs CPU) + % HBM roofline, err vs FP64",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{case.name}: two-fluid Hall MHD, FP64 plain GPU form, 6 curvilinear sector blocks "
+                               f"{[list(b.dims) for b in case.blocks]}, forward Euler, CFL 0.4",
+                   "cells": ncells, "blocks": case.nb, "form": "fp64", "integrator": "euler"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "algo_bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": units,
+                     "avg_launch_ms": times[dom]},
+        "step_roofline": {"stage_bytes_per_cell": STAGE_BYTES, "ms_per_step": step_graph_ms, "achieved_GBs": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS},
+        "cpu_baseline": cpu,
+        "err_vs_fp64_oracle": err,
+        "divB_normalised": divb_norm,
+        "kernel_ms": times,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -212,3 +212,28 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
     return Case(name="shell6" if scale == 1.0 else f"shell6_x{scale:g}", g=g, blocks=blks, phys=P, fields=fields,
                 A=None, Bu=(0.0, 0.0, 0.0), steps=steps, integrator=integrator,
                 meta={"seed": seed, "species": 2, "hall": True})
+
+
+def make_uniform_block(nodes: np.ndarray, bc, g: int = 4, rho: float = 1.0, u=(0.0, 0.0, 0.0), p: float = 0.1,
+                       rho_na_ratio: float = RHO_SMALL, units: MercuryUnits | None = None, name: str = "block") -> Case:
+    """One block with given owned nodes (3, mx+1, my+1, mz+1) and face codes,
+    a uniform state on the whole cell box, B = 0 (tests set fields directly)."""
+    dims = tuple(int(n) - 1 for n in nodes.shape[1:])
+    if units is None:
+        units = MercuryUnits().alfven_units()
+        units.r_min, units.r_max = 100.0, 100.0
+    P = derive(units)
+    I, J, K = _box_index(dims, g)
+    shape = I.shape
+    one = np.ones(shape)
+    UH = _conserved(rho * one, u[0] * one, u[1] * one, u[2] * one, p * one, P.gamma)
+    UNa = _conserved(rho_na_ratio * rho * one, u[0] * one, u[1] * one, u[2] * one, rho_na_ratio * p * one, P.gamma)
+    fields = [{"U_H": UH, "U_Na": UNa, "Photo_rate": np.zeros((1,) + shape)}]
+    return Case(name=name, g=g, blocks=[Block(dims=dims, bc=tuple(bc), nodes=nodes.astype(np.float64))], phys=P,
+                fields=fields, A=None, Bu=(0.0, 0.0, 0.0), steps=1)
+
+
+def cartesian_nodes(n=(6, 5, 4), h=(0.1, 0.1, 0.1), origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """Owned nodes of a uniform Cartesian block."""
+    i, j, k = np.meshgrid(np.arange(n[0] + 1), np.arange(n[1] + 1), np.arange(n[2] + 1), indexing="ij")
+    return np.stack([origin[0] + h[0] * i, origin[1] + h[1] * j, origin[2] + h[2] * k]).astype(np.float64)

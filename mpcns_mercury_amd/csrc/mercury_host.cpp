@@ -890,7 +890,8 @@ public:
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        run_op(op, 0.0);
+        if (op == "step_graph") graph_launch(1);
+        else run_op(op, 0.0);
         HIPCHK(hipEventRecord(e0, st));
         for (int r = 0; r < reps; ++r) {
             if (op == "step_graph") graph_launch(1);

@@ -36,8 +36,8 @@ DEVI bool locate(const MercRange R, const MercBlk *__restrict__ B, long long t, 
         if (R.start[mid] <= t) lo = mid; else hi = mid - 1;
     }
     const int b = lo;
-    long long r = t - R.start[b];
-    const int n1 = R.n[3 * b + 1], n2 = R.n[3 * b + 2];
+    unsigned r = (unsigned)(t - R.start[b]);  // per-block boxes stay below 2^31 points
+    const unsigned n1 = (unsigned)R.n[3 * b + 1], n2 = (unsigned)R.n[3 * b + 2];
     const int kk = (int)(r % n2);
     r /= n2;
     const int jj = (int)(r % n1);
@@ -53,7 +53,17 @@ DEVI bool locate(const MercRange R, const MercBlk *__restrict__ B, long long t, 
     return true;
 }
 
-#define TID ((long long)blockIdx.x * blockDim.x + threadIdx.x)
+// XCD-aware workgroup order: the dispatcher deals workgroups round-robin over
+// the 8 XCDs, each with its own L2; remapping gives every XCD one contiguous
+// slab of the index space, so the i/j stencil neighbours of a workgroup are
+// mostly resident in the same L2.
+DEVI long long xcd_tid() {
+    const unsigned nwg = gridDim.x, wg = blockIdx.x;
+    const unsigned q = nwg >> 3, r = nwg & 7u, x = wg & 7u, s = wg >> 3;
+    const unsigned logical = x * q + (x < r ? x : r) + s;
+    return (long long)logical * blockDim.x + threadIdx.x;
+}
+#define TID xcd_tid()
 #define LOCATE(R)            \
     Pt p;                    \
     if (!locate(R, B, TID, p)) return;

@@ -74,7 +74,9 @@ enum {
     /* edges */
     S_E = S_B0 + 3, S_EH = S_E + 3, S_J = S_EH + 3, S_DR = S_J + 3, S_DL = S_DR + 9,
     S_ALPHA = S_DL + 3, S_TAPER = S_ALPHA + 3, S_LETA = S_TAPER + 3,
-    S_NSLOT = S_LETA + 3
+    /* generic DEC forms: node 0-form, edge 1-forms, face 2-forms, cell 3-form */
+    S_NODEF = S_LETA + 3, S_EDGEF = S_NODEF + 1, S_FACEF = S_EDGEF + 3, S_CELLS = S_FACEF + 3,
+    S_NSLOT = S_CELLS + 1
 };
 
 typedef struct {
@@ -1724,7 +1726,9 @@ int orc_slot(const char *name) {
         {"J_xi", S_J}, {"J_eta", S_J + 1}, {"J_zeta", S_J + 2}, {"dr_xi", S_DR}, {"dr_eta", S_DR + 3}, {"dr_zeta", S_DR + 6},
         {"dl_xi", S_DL}, {"dl_eta", S_DL + 1}, {"dl_zeta", S_DL + 2}, {"alpha_xi", S_ALPHA}, {"alpha_eta", S_ALPHA + 1}, {"alpha_zeta", S_ALPHA + 2},
         {"taper_xi", S_TAPER}, {"taper_eta", S_TAPER + 1}, {"taper_zeta", S_TAPER + 2},
-        {"eta_local_xi", S_LETA}, {"eta_local_eta", S_LETA + 1}, {"eta_local_zeta", S_LETA + 2}};
+        {"eta_local_xi", S_LETA}, {"eta_local_eta", S_LETA + 1}, {"eta_local_zeta", S_LETA + 2},
+        {"node_scalar", S_NODEF}, {"edge_xi", S_EDGEF}, {"edge_eta", S_EDGEF + 1}, {"edge_zeta", S_EDGEF + 2},
+        {"face_xi", S_FACEF}, {"face_eta", S_FACEF + 1}, {"face_zeta", S_FACEF + 2}, {"cell_scalar", S_CELLS}};
     for (unsigned q = 0; q < sizeof(T) / sizeof(T[0]); ++q)
         if (!strcmp(T[q].n, name)) return T[q].s;
     return -1;
@@ -1816,6 +1820,52 @@ int orc_step(void *h, int nsteps, int integrator, double *dt_out) {
     return 0;
 }
 
+/* DEC exterior derivatives on owned DOFs (src/8_dec/DecExteriorDerivative.cpp:
+ * d0 node->edge, d1 edge->face = CurlEdgeToFace, d2 face->cell) */
+static void dec_d0(Orc *o) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int a = 0; a < 3; ++a) {
+            int hi[3];
+            owned_hi(b, L_EX + a, hi);
+            for (int i = 0; i < hi[0]; ++i)
+                for (int j = 0; j < hi[1]; ++j)
+                    for (int k = 0; k < hi[2]; ++k)
+                        F(b, S_EDGEF + a, i, j, k) = F(b, S_NODEF, i + (a == 0), j + (a == 1), k + (a == 2)) - F(b, S_NODEF, i, j, k);
+        }
+    }
+}
+static void dec_d1(Orc *o, double mult) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        int hi[3];
+#define EE(a, i, j, k) F(b, S_EDGEF + (a), i, j, k)
+        owned_hi(b, L_FX, hi);
+        for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+                for (int k = 0; k < hi[2]; ++k) F(b, S_FACEF, i, j, k) = mult * ((EE(1, i, j, k) - EE(1, i, j, k + 1)) + (EE(2, i, j + 1, k) - EE(2, i, j, k)));
+        owned_hi(b, L_FY, hi);
+        for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+                for (int k = 0; k < hi[2]; ++k) F(b, S_FACEF + 1, i, j, k) = mult * ((EE(0, i, j, k + 1) - EE(0, i, j, k)) + (EE(2, i, j, k) - EE(2, i + 1, j, k)));
+        owned_hi(b, L_FZ, hi);
+        for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+                for (int k = 0; k < hi[2]; ++k) F(b, S_FACEF + 2, i, j, k) = mult * ((EE(0, i, j, k) - EE(0, i, j + 1, k)) + (EE(1, i + 1, j, k) - EE(1, i, j, k)));
+#undef EE
+    }
+}
+static void dec_d2(Orc *o) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int i = 0; i < b->m[0]; ++i)
+            for (int j = 0; j < b->m[1]; ++j)
+                for (int k = 0; k < b->m[2]; ++k)
+                    F(b, S_CELLS, i, j, k) = F(b, S_FACEF, i + 1, j, k) - F(b, S_FACEF, i, j, k) + F(b, S_FACEF + 1, i, j + 1, k) - F(b, S_FACEF + 1, i, j, k) +
+                                             F(b, S_FACEF + 2, i, j, k + 1) - F(b, S_FACEF + 2, i, j, k);
+    }
+}
+
 /* individual operators, for per-kernel parity tests */
 int orc_op(void *h, const char *op, double arg) {
     Orc *o = (Orc *)h;
@@ -1841,6 +1891,10 @@ int orc_op(void *h, const char *op, double arg) {
     else if (!strcmp(op, "sync_bcell")) sync_group(o, G_BCELL);
     else if (!strcmp(op, "sync_jcell")) sync_group(o, G_JCELL);
     else if (!strcmp(op, "sync_badd")) sync_group(o, G_BADD);
+    else if (!strcmp(op, "dec_d0")) dec_d0(o);
+    else if (!strcmp(op, "dec_d1")) dec_d1(o, arg == 0.0 ? 1.0 : arg);
+    else if (!strcmp(op, "dec_d2")) dec_d2(o);
+    else if (!strcmp(op, "eface")) ideal_emf(o);
     else return -1;
     return 0;
 }

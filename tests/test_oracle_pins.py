@@ -1,0 +1,355 @@
+"""Pins of the CPU oracle (CPU only).
+
+The reference's own tests for this path hold invariants, not numeric vectors
+(SURVEY of Z0_NULL/test: DEC chain d1.d0 = 0 and d2.d1 = 0, halo NaN-overwrite
+with unique integer codes, manufactured J cases).  These tests restate them and
+add closed forms chosen so that a plausible mistake in the oracle (a dropped
+term, a wrong sign or index, a transposed operand) fails one of them.
+"""
+import numpy as np
+import pytest
+
+from mpcns_mercury_amd import cases
+from mpcns_mercury_amd.params import MercuryUnits, derive
+
+
+def owned(a, g, dims, delta):
+    """Owned sub-box of a box array (any leading component axis)."""
+    sl = tuple(slice(g, g + m + 1 - d) for m, d in zip(dims, delta))
+    return a[(...,) + sl]
+
+
+DELTA = {"cell": (1, 1, 1), "fx": (0, 1, 1), "fy": (1, 0, 1), "fz": (1, 1, 0),
+         "ex": (1, 0, 0), "ey": (0, 1, 0), "ez": (0, 0, 1), "node": (0, 0, 0)}
+
+
+@pytest.fixture(scope="module")
+def O(oracle_mod):
+    return oracle_mod
+
+
+def cube(n=(6, 5, 4), **kw):
+    return cases.make_periodic_cube(n=n, g=4, **kw)
+
+
+# --------------------------------------------------------------------------
+# DEC chain (Z0_NULL/test/dec_chain/Z0_DecChainTest.cpp:77-105)
+# --------------------------------------------------------------------------
+def test_dec_chain_exact(O):
+    c = cube()
+    o = O.Oracle(c)
+    rng = np.random.default_rng(3)
+    shape = c.box(0)
+    o.set("node_scalar", 0, rng.integers(-1000, 1000, size=shape).astype(float))
+    o.op("dec_d0")
+    o.op("dec_d1")
+    g, d = c.g, c.blocks[0].dims
+    for name, dl in (("face_xi", "fx"), ("face_eta", "fy"), ("face_zeta", "fz")):
+        assert np.all(owned(o.get(name)[0], g, d, DELTA[dl]) == 0.0), name
+    for a, dl in zip(("edge_xi", "edge_eta", "edge_zeta"), ("ex", "ey", "ez")):
+        o.set(a, 0, rng.integers(-1000, 1000, size=shape).astype(float))
+    o.op("dec_d1")
+    o.op("dec_d2")
+    assert np.all(owned(o.get("cell_scalar")[0], g, d, DELTA["cell"]) == 0.0)
+    # d1 of a nonzero edge form is not identically zero (the test can fail)
+    assert np.abs(owned(o.get("face_xi")[0], g, d, DELTA["fx"])).max() > 0
+
+
+# --------------------------------------------------------------------------
+# halo: NaN fill + unique integer codes (Z0_NULL/test/halo_comm/Z0_HaloCommTest.cpp:701-844,
+# code 1e9(rank+1)+1e6(ib+1)+1e3(m+1)+100i+10j+k, Z0_TestCommon.h:93-120)
+# --------------------------------------------------------------------------
+def _code_field(shape, g, dims, delta, comp=0):
+    f = np.full(shape, np.nan)
+    I, J, K = np.meshgrid(*[np.arange(s) - g for s in shape], indexing="ij")
+    code = 1e9 + 1e6 + 1e3 * (comp + 1) + 100.0 * I + 10.0 * J + K
+    sl = tuple(slice(g, g + m + 1 - dd) for m, dd in zip(dims, delta))
+    f[sl] = code[sl]
+    return f, code
+
+
+def test_halo_periodic_unique_codes(O):
+    c = cube(n=(6, 5, 4))
+    o = O.Oracle(c)
+    g, dims = c.g, c.blocks[0].dims
+    shape = c.box(0)
+    # Corner3D face 2-form: every allocated ghost gets the translate of an owned code
+    f, code = _code_field(shape, g, dims, DELTA["fx"])
+    o.set("B_xi", 0, f)
+    o.op("sync_bface")
+    got = o.get("B_xi")[0]
+    alloc = tuple(slice(0, g + m + 1 - dd + g) for m, dd in zip(dims, DELTA["fx"]))
+    sub = got[alloc]
+    assert np.isfinite(sub).all()
+    # ghost i<0 <- i+m (the shared face i=m is excluded from the source strip)
+    mx = dims[0]
+    assert got[g - 1, g + 1, g + 1] == code[g - 1 + mx, g + 1, g + 1]
+    assert got[g + mx + 1, g + 1, g + 1] == code[g + 1, g + 1, g + 1]
+    # owner/alias: the seam face i=m takes the owner's (i=0) value
+    np.testing.assert_array_equal(got[g + mx, g:g + dims[1], g:g + dims[2]], code[g, g:g + dims[1], g:g + dims[2]])
+    # Corner1D cell field: face strips filled, 2-D corners untouched (NaN)
+    U = np.stack([_code_field(shape, g, dims, DELTA["cell"], comp=q)[0] for q in range(5)])
+    o.set("U_H", 0, U)
+    o.op("sync_ucell")
+    got = o.get("U_H")
+    assert got[0, g - 1, g + 2, g + 1] == U[0, g - 1 + dims[0], g + 2, g + 1]
+    assert np.isnan(got[0, g - 1, g - 1, g + 1])  # 2-D corner ghost is not exchanged
+    assert np.isfinite(got[:, g:g + dims[0], g - 1, g:g + dims[2]]).all()
+
+
+def test_edge_alias_mean(O):
+    c = cube(n=(4, 4, 4))
+    o = O.Oracle(c)
+    g, dims = c.g, c.blocks[0].dims
+    rng = np.random.default_rng(5)
+    E = rng.standard_normal(c.box(0))
+    o.set("E_xi", 0, E)
+    o.op("alias_e")
+    got = o.get("E_xi")[0]
+    my, mz = dims[1], dims[2]
+    # class of the edge (i, 0, 0): members (i,0,0), (i,0,mz), (i,my,0), (i,my,mz)
+    i = 2
+    mem = [E[g + i, g + jj, g + kk] for jj in (0, my) for kk in (0, mz)]
+    assert got[g + i, g, g] == ((mem[0] + mem[1]) + mem[2]) / 4.0 + mem[3] / 4.0 or \
+        abs(got[g + i, g, g] - np.mean(mem)) < 1e-15
+    # class of (i, j, 0), 0<j<my: two members
+    assert got[g + i, g + 2, g] == (E[g + i, g + 2, g] + E[g + i, g + 2, g + mz]) / 2.0
+    # non-seam edges untouched
+    assert got[g + i, g + 2, g + 2] == E[g + i, g + 2, g + 2]
+
+
+# --------------------------------------------------------------------------
+# metric on a uniform Cartesian block: closed forms
+# --------------------------------------------------------------------------
+def test_metric_cartesian_closed_form(O):
+    h = (0.1, 0.2, 0.25)
+    nodes = cases.cartesian_nodes((5, 4, 3), h)
+    c = cases.make_uniform_block(nodes, (1,) * 6)
+    o = O.Oracle(c)
+    g, dims = c.g, c.blocks[0].dims
+    vol = h[0] * h[1] * h[2]
+    jac = owned(o.get("Jac")[0], g, dims, DELTA["cell"])
+    np.testing.assert_allclose(jac, vol, rtol=1e-13)
+    areas = (h[1] * h[2], h[0] * h[2], h[0] * h[1])
+    for a, (nm, dl) in enumerate((("JDxi", "fx"), ("JDet", "fy"), ("JDze", "fz"))):
+        JD = owned(o.get(nm), g, dims, DELTA[dl])
+        np.testing.assert_allclose(JD[a], areas[a], rtol=1e-13)
+        for b in range(3):
+            if b != a:
+                assert np.abs(JD[b]).max() < 1e-15
+    for a, (nm, dl) in enumerate((("beta_xi", "fx"), ("beta_eta", "fy"), ("beta_zeta", "fz"))):
+        np.testing.assert_allclose(owned(o.get(nm)[0], g, dims, DELTA[dl]), h[a] / areas[a], rtol=1e-13)
+    for a, (nm, dl) in enumerate((("alpha_xi", "ex"), ("alpha_eta", "ey"), ("alpha_zeta", "ez"))):
+        np.testing.assert_allclose(owned(o.get(nm)[0], g, dims, DELTA[dl]), h[a] / areas[a], rtol=1e-13)
+        dr = owned(o.get(["dr_xi", "dr_eta", "dr_zeta"][a]), g, dims, DELTA[dl])
+        np.testing.assert_allclose(dr[a], h[a], rtol=1e-13)
+
+
+def test_gcl_closure_curvilinear(O):
+    c = cube(n=(6, 5, 4))
+    o = O.Oracle(c)
+    g, (mx, my, mz) = c.g, c.blocks[0].dims
+    JX, JY, JZ = o.get("JDxi"), o.get("JDet"), o.get("JDze")
+    s = (JX[:, g + 1:g + mx + 1, g:g + my, g:g + mz] - JX[:, g:g + mx, g:g + my, g:g + mz] +
+         JY[:, g:g + mx, g + 1:g + my + 1, g:g + mz] - JY[:, g:g + mx, g:g + my, g:g + mz] +
+         JZ[:, g:g + mx, g:g + my, g + 1:g + mz + 1] - JZ[:, g:g + mx, g:g + my, g:g + mz])
+    assert np.abs(s).max() < 1e-15
+    jac = owned(o.get("Jac")[0], g, (mx, my, mz), DELTA["cell"])
+    np.testing.assert_allclose(jac.sum(), 1.0, rtol=1e-12)  # the warped unit cube keeps its volume
+
+
+# --------------------------------------------------------------------------
+# reconstruction weights reproduce uniform fields; manufactured J
+# (Z4_Mercury/src/MPCNSSolver_Debug.cpp:1158-1200)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("tid", range(6))
+def test_manufactured_j_cartesian(O, tid):
+    h = 0.1
+    n = (5, 5, 5)
+    nodes = cases.cartesian_nodes(n, (h, h, h), origin=(-0.25, -0.25, -0.25))
+    c = cases.make_uniform_block(nodes, (1,) * 6)
+    o = O.Oracle(c)
+    g = c.g
+    shape = c.box(0)
+    I, J, K = np.meshgrid(*[np.arange(s) - g for s in shape], indexing="ij")
+    x0 = -0.25
+
+    def B(x, y, z):
+        one = np.ones_like(x)
+        return [(one, 0 * one, 0 * one), (0 * one, one, 0 * one), (0 * one, 0 * one, one),
+                (0 * one, -0.5 * z, 0.5 * y), (0.5 * z, 0 * one, -0.5 * x), (-0.5 * y, 0.5 * x, 0 * one)][tid]
+
+    Jexp = [(0, 0, 0)] * 3 + [(1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    # face flux of a linear field through a planar face = B(centre) . area vector
+    area = h * h
+    for a, nm in enumerate(("B_xi", "B_eta", "B_zeta")):
+        cx = x0 + h * (I + (0.0 if a == 0 else 0.5))
+        cy = x0 + h * (J + (0.0 if a == 1 else 0.5))
+        cz = x0 + h * (K + (0.0 if a == 2 else 0.5))
+        o.set(nm, 0, B(cx, cy, cz)[a] * area)
+    o.op("jedge")
+    o.op("jcell")
+    o.op("bcell")
+    Jc = owned(o.get("J_cell"), g, n, DELTA["cell"])
+    for q in range(3):
+        np.testing.assert_allclose(Jc[q], Jexp[tid][q], atol=1e-12)
+    # uniform B is reproduced exactly by the face -> cell reconstruction
+    if tid < 3:
+        Bc = owned(o.get("B_cell"), g, n, DELTA["cell"])
+        np.testing.assert_allclose(Bc[tid], 1.0, rtol=1e-13)
+
+
+# --------------------------------------------------------------------------
+# fluid: free-stream preservation, time step closed form
+# --------------------------------------------------------------------------
+def test_free_stream_rhs_zero_curvilinear(O):
+    c = cube(n=(6, 5, 4), amp=0.0)
+    o = O.Oracle(c)
+    o.prepare()
+    o.op("flux")
+    R = owned(o.get("RHS_H"), c.g, c.blocks[0].dims, DELTA["cell"])
+    assert np.abs(R).max() < 1e-12
+
+
+def test_timestep_closed_form(O):
+    h = 0.1
+    n = (4, 4, 4)
+    u = (0.3, 0.0, 0.0)
+    c = cases.make_uniform_block(cases.cartesian_nodes(n, (h, h, h)), (1,) * 6, u=u, p=0.1, rho_na_ratio=0.5)
+    o = O.Oracle(c)
+    o.prepare()  # B = 0 everywhere
+    P = c.phys
+    gam = P.gamma
+    dts = []
+    for rho, p in ((1.0, 0.1), (0.5, 0.05)):
+        r = max(rho, 1e-2)
+        cs = np.sqrt(gam * p / r)
+        denom = 2 * (abs(u[0]) + cs) * h * h + 4 * cs * h * h
+        dts.append(P.cfl * h ** 3 / denom)
+    np.testing.assert_allclose(o.timestep(), min(dts), rtol=1e-14)
+
+
+def test_rusanov_consistency_uniform_flow(O):
+    # uniform moving state on a Cartesian periodic block: every face flux is the
+    # physical flux, so the divergence vanishes exactly
+    h = 0.1
+    n = (4, 4, 4)
+    c = cases.make_uniform_block(cases.cartesian_nodes(n, (h, h, h)), (0,) * 6, u=(0.2, -0.1, 0.05))
+    o = O.Oracle(c)
+    o.prepare()
+    o.op("flux")
+    R = owned(o.get("RHS_H"), c.g, n, DELTA["cell"])
+    assert np.abs(R).max() < 1e-14
+
+
+# --------------------------------------------------------------------------
+# sources: special states isolating single terms
+# --------------------------------------------------------------------------
+def test_sources_thermal_only(O):
+    h = 0.1
+    n = (4, 4, 4)
+    c = cases.make_uniform_block(cases.cartesian_nodes(n, (h, h, h), origin=(1.0, 0, 0)), (0,) * 6, p=0.1)
+    o = O.Oracle(c)
+    o.prepare()
+    o.op("flux")
+    o.op("sources")
+    P = c.phys
+    R = owned(o.get("RHS_H"), c.g, n, DELTA["cell"])
+    pv = owned(o.get("PV_H"), c.g, n, DELTA["cell"])
+    nH = 1.0 / P.M_H * P.M_ref
+    b1 = P.nu_lit * (P.Tn0 - pv[4] * P.T_ref) * (P.M_H / (P.M_H + P.M_Na))
+    np.testing.assert_allclose(R[4], P.a5 * nH * b1, rtol=1e-12)
+    assert np.abs(R[1:4]).max() == 0.0
+
+
+def test_sources_lorentz_only(O):
+    # static plasma (u = U+ = 0), uniform B_cell and J_cell: momentum gains
+    # c_hall * w_s * (J x B), energy gains nothing from it (u = 0)
+    h = 0.1
+    n = (4, 4, 4)
+    c = cases.make_uniform_block(cases.cartesian_nodes(n, (h, h, h), origin=(1.0, 0, 0)), (0,) * 6, p=0.1)
+    o = O.Oracle(c)
+    o.prepare()
+    shape = c.box(0)
+    Bv, Jv = (0.2, -0.4, 0.7), (1.5, 0.3, -0.2)
+    o.set("B_cell", 0, np.stack([np.full(shape, v) for v in Bv]))
+    o.set("J_cell", 0, np.stack([np.full(shape, v) for v in Jv]))
+    o.op("flux")
+    o.op("sources")
+    P = c.phys
+    jxb = np.cross(Jv, Bv)
+    nH = 1.0 / P.M_H * P.M_ref
+    nNa = 1e-6 / P.M_Na * P.M_ref
+    ne_eff = np.sqrt((nH + nNa) ** 2 + P.ne_floor ** 2)
+    R = owned(o.get("RHS_H"), c.g, n, DELTA["cell"])
+    for q in range(3):
+        np.testing.assert_allclose(R[1 + q], P.c_hall * (nH / ne_eff) * jxb[q], rtol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# ideal EMF of a uniform flow in a uniform field: E_edge = -(u x B) . dr
+# --------------------------------------------------------------------------
+def test_ideal_emf_uniform(O):
+    h = 0.1
+    n = (4, 4, 4)
+    u = (0.3, -0.2, 0.1)
+    c = cases.make_uniform_block(cases.cartesian_nodes(n, (h, h, h)), (0,) * 6, u=u, rho_na_ratio=1e-6)
+    c.Bu = (0.5, 0.25, -0.75)
+    o = O.Oracle(c)
+    o.prepare()
+    o.op("eface")
+    E = -np.cross(u, c.Bu)
+    for a, (nm, dl) in enumerate((("E_xi", "ex"), ("E_eta", "ey"), ("E_zeta", "ez"))):
+        got = owned(o.get(nm)[0], c.g, n, DELTA[dl])
+        np.testing.assert_allclose(got, E[a] * h, rtol=1e-10, atol=1e-15)
+
+
+# --------------------------------------------------------------------------
+# physical boundaries on a sector block
+# --------------------------------------------------------------------------
+def test_farfield_and_wall_handlers(O):
+    c = cases.make_shell(scale=0.06)
+    o = O.Oracle(c)
+    o.prepare()
+    o.op("sync_ucell")
+    P = c.phys
+    g = c.g
+    U = o.get("U_H", 0)
+    N = o.get("U_Na", 0)
+    mx, my, mz = c.blocks[0].dims
+    # far field (+xi face): ghost slab holds qinf
+    np.testing.assert_array_equal(U[:, g + mx + 1, g + 1, g + 1], np.array(P.qinf[:]))
+    # wall (-xi face): first active layer has zero momentum, ghosts copy it
+    assert np.all(U[1:4, g, g:g + my, g:g + mz] == 0.0)
+    np.testing.assert_array_equal(U[:, g - 2, g + 1, g + 1], U[:, g, g + 1, g + 1])
+    # Na ghosts at the wall: floor state
+    assert N[0, g - 1, g + 1, g + 1] == P.wall_rho_floor_Na
+    # corner default: 2-D corner ghost = clamped owned value
+    np.testing.assert_array_equal(U[:, g + mx + 1, g - 1, g + 2], U[:, g + mx - 1, g, g + 2])
+    # far-field edge EMF: boundary-layer tangential edges carry E_inf . dr
+    o.op("induction")
+    E = o.get("E_eta", 0)[0]
+    u, Bi = P.q_pv_inf, P.B_imf
+    Ei = np.array([Bi[1] * u[2] - Bi[2] * u[1], Bi[2] * u[0] - Bi[0] * u[2], Bi[0] * u[1] - Bi[1] * u[0]])
+    dr = o.get("dr_eta", 0)
+    np.testing.assert_allclose(E[g + mx, g + 1, g + 1], Ei @ dr[:, g + mx, g + 1, g + 1], rtol=1e-12)
+    # conducting wall: tangential E on the wall is zero, except on the wall's rim,
+    # which the far-field faces' zero-curl handler rewrites during Sync(Eedge)
+    assert np.all(E[g, g + 1:g + my - 1, g + 1:g + mz] == 0.0)
+    np.testing.assert_allclose(E[g, g + 1, g], Ei @ dr[:, g, g + 1, g], rtol=1e-12)
+
+
+def test_divb_preserved_over_steps(O):
+    c = cube(n=(8, 6, 5))
+    o = O.Oracle(c)
+    o.prepare()
+    o.step(10)
+    norm, _ = o.divb()
+    assert norm < 1e-14
+
+
+def test_params_alfven_units():
+    P = derive(MercuryUnits().alfven_units())
+    assert abs(P.inver_MA2 - 1.0) < 1e-12 and abs(P.c_hall - 1.0) < 1e-12
+    # the ambipolar pressure coefficient is the inverse of the induce coefficient times n-ratio
+    assert abs(P.pressure_coef * P.c_ind - 1.0) < 1e-12
