@@ -36,6 +36,21 @@ HBM_PEAK_GBS = 8000.0
 # served from cache.  Units: owned cells (cell kernels) or node-box points
 # (edge/face kernels, one point carries up to 3 edges or faces).
 ALGO_BYTES = {
+    # fused passes
+    # flux + sources: U_s (10) + PV_s (10) + JD (9) + Jac + U+ (3) + B_cell (3) + J_cell (3) + Photo + centre (3) read,
+    # RHS_s (10) written
+    "rhs_cell": 8 * (10 + 10 + 9 + 1 + 3 + 3 + 3 + 1 + 3 + 10),
+    # B (3) + Badd (3) + U_con (3) read, Eface (9) written
+    "eface_u": 8 * (3 + 3 + 3 + 9),
+    # Eface (9) + U_con (3) + B (3) + PV p (2) + U rho (2) + J (3) + eta_local (3) + E_hc (3) + beta (1) + dr (9)
+    # + Area (3) + dl (3) + taper (3) read, E (3) + Ehall (3) written
+    "edge_total": 8 * (9 + 3 + 3 + 2 + 2 + 3 + 3 + 3 + 1 + 9 + 3 + 3 + 3 + 6),
+    # U_s (10) + RHS_s (10) + E (3) + B (3) read, U_s (10) + B (3) + RHS_B (3) written
+    "update_all": 8 * (10 + 10 + 3 + 3 + 10 + 3 + 3),
+    # B (3) + Badd (3) + W_b (18) + beta (3) + alpha (3) read, B_cell + Bind (6) + J (3) written
+    "bcell_jedge": 8 * (3 + 3 + 18 + 3 + 3 + 6 + 3),
+    # U_s (10) + J_cell (3) + B_cell (3) + JD (9) + Jac read, PV_s (10) + U+ (3) + E_hc (3) + beta + U_con (3) written
+    "cell_end": 8 * (10 + 3 + 3 + 9 + 1 + 10 + 3 + 3 + 1 + 3),
     # U_s (2x5) + PV_s u,v,w,p (2x4) + JD (9) + Jac (1) read, RHS_s (2x5) written
     "flux": 8 * (10 + 8 + 9 + 1 + 10),
     # U rho (2) + PV (2x5) + U+ (3) + B_cell (3) + J_cell (3) + Photo + centre (3) + Jac + JD (9) + RHS r/w (2x10)
@@ -154,19 +169,22 @@ def main():
     print(f"[bench] {args.steps} steps {elapsed:.3f}s  {ms_step:.3f} ms/step  {value:.4e} cell-updates/s  divB={divb_norm:.2e}",
           file=sys.stderr, flush=True)
 
-    # per-kernel device times (HIP events on the solver stream)
-    ops = ["flux", "sources", "eface", "edge_emf", "hall_cell", "hall_edge", "pv_uplus", "bcell", "jedge", "jcell",
-           "timestep"]
+    # per-kernel device times of the fused passes (HIP events on the solver stream)
+    ops = ["rhs_cell", "eface_u", "edge_total", "update_all", "bcell_jedge", "jcell", "cell_end"]
     times = {op: g.time_op(op, 20) for op in ops}
+    sync_times = {op: g.time_op(op, 20) for op in ("sync_ucell", "sync_bface", "sync_eedge", "sync_ehall", "sync_jedge",
+                                                   "sync_bcell", "sync_jcell")}
     step_graph_ms = g.time_op("step_graph", 10)
     dom = max(times, key=times.get)
-    units = ncells if dom in ("flux", "sources", "bcell", "jcell", "timestep") else ncells  # node-box points ~ cells
+    units = ncells
     algo = ALGO_BYTES[dom] * units
     achieved = algo / (times[dom] * 1e-3) / 1e9
     if args.profile_ops:
         for op in ops:
             print(f"[bench] op {op:10s} {times[op] * 1e3:9.1f} us  {ALGO_BYTES[op] * ncells / (times[op] * 1e-3) / 1e9:8.1f} GB/s",
                   file=sys.stderr)
+        for op, t in sync_times.items():
+            print(f"[bench] {op:12s} {t * 1e3:9.1f} us", file=sys.stderr)
         print(f"[bench] step (graph) {step_graph_ms:.3f} ms", file=sys.stderr)
     step_gbs = STAGE_BYTES * ncells / (step_graph_ms * 1e-3) / 1e9
 

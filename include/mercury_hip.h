@@ -120,6 +120,10 @@ int mercury_divb(MercuryContext *ctx, double *normalised, double *maxabs);
 
 int mercury_synchronize(MercuryContext *ctx);
 
+/* Kernel grouping of the FP64 step: 0 = one kernel per reference function
+ * (the drop-in form), 1 = fused passes (default).  Both give identical results. */
+int mercury_set_form(MercuryContext *ctx, int form);
+
 /* One explicit step captured as a HIP graph and replayed (no reference
  * counterpart: the reference loops on the host). */
 int mercury_graph_capture(MercuryContext *ctx, int integrator);

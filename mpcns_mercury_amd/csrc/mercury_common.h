@@ -35,7 +35,9 @@ enum MercSlot {
     SL_ALPHA = SL_DL + 3, SL_TAPER = SL_ALPHA + 3, SL_LETA = SL_TAPER + 3, SL_EDGEF = SL_LETA + 3,
     // nodes
     SL_NODEF = SL_EDGEF + 3,
-    SL_NSLOT = SL_NODEF + 1
+    // cell: contravariant velocity of U+ per axis (fused path)
+    SL_UCON = SL_NODEF + 1,
+    SL_NSLOT = SL_UCON + 3
 };
 
 // physics constants (same layout as MercuryPhysics in include/mercury_hip.h)

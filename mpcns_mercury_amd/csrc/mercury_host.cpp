@@ -24,6 +24,7 @@
 #include "mercury_common.h"
 #include "mercury_geometry.h"
 #include "mercury_kernels.h"
+#include "mercury_fused.h"
 
 static_assert(sizeof(MercuryPhysics) == sizeof(MercPhys), "MercuryPhysics / MercPhys layout mismatch");
 
@@ -505,6 +506,8 @@ public:
     AliasClasses alias;
     hipStream_t st = nullptr;
     unsigned long long *d_dt = nullptr;       // dt word (double bits)
+    unsigned long long *d_dt_next = nullptr;  // next step's dt, reduced at the end of the derived fields
+    int form = 1;                             // 0: one kernel per reference function, 1: fused FP64
     unsigned long long *d_red = nullptr;      // reduction scratch (2 words)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -532,6 +535,7 @@ public:
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         F.reset(new Field(nb, dims, g, bc));
         HIPCHK(hipMalloc(&d_dt, sizeof(unsigned long long)));
+        HIPCHK(hipMalloc(&d_dt_next, sizeof(unsigned long long)));
         HIPCHK(hipMalloc(&d_red, 2 * sizeof(unsigned long long)));
         // static geometry per block
         long long noff = 0;
@@ -559,6 +563,7 @@ public:
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         if (d_dt) (void)hipFree(d_dt);
+        if (d_dt_next) (void)hipFree(d_dt_next);
         if (d_red) (void)hipFree(d_red);
         F.reset();
         if (st) (void)hipStreamDestroy(st);
@@ -702,6 +707,40 @@ public:
     const MercBlk *Bd() const { return F->dblk; }
     const double *dtp() const { return reinterpret_cast<const double *>(d_dt); }
 
+    // UpdateDerivedFields_ (MercurySolver_Tools.cpp:303-321), fused passes; the
+    // last derived pass of a step also reduces the next step's dt
+    void derived_fused(bool final) {
+        launch_bcell_jedge(st, R_node.h, Bd(), Fd());
+        sync(G_BCELL);
+        launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
+        sync(G_JEDGE);
+        launch_jcell(st, R_cell.h, Bd(), Fd());
+        sync(G_JCELL);
+        launch_cell_end(st, R_box.h, Bd(), Fd(), P, d_dt_next, final ? 1 : 0);
+    }
+
+    // StepOnce without the time step, fused passes
+    void stage_fused(bool final) {
+        launch_rhs_cell(st, R_cell.h, Bd(), Fd(), P);
+        launch_eface_u(st, R_eface.h, Bd(), Fd());
+        if (alias.n == 0) {
+            launch_edge_total(st, R_node.h, Bd(), Fd(), P, 1);
+            sync(G_EHALL);
+        } else {
+            // periodic seams: alias mean of the non-Hall EMF before the Hall part
+            launch_edge_total(st, R_node.h, Bd(), Fd(), P, 0);
+            launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_E);
+            launch_hall_edge(st, R_node.h, Bd(), Fd());
+            sync(G_EHALL);
+            launch_add_hall_wall(st, R_node.h, Bd(), Fd());
+        }
+        sync(G_EEDGE);
+        launch_update_all(st, R_node.h, Bd(), Fd(), dtp());
+        sync(G_UCELL);
+        sync(G_BFACE);
+        derived_fused(final);
+    }
+
     // UpdateDerivedFields_ (MercurySolver_Tools.cpp:303-321)
     void derived() {
         launch_bcell(st, R_cell.h, Bd(), Fd());
@@ -741,6 +780,21 @@ public:
     }
 
     void enqueue_step(int integrator) {
+        if (form == 1) {
+            launch_dt_commit(st, d_dt, d_dt_next);
+            if (integrator == 0) {
+                stage_fused(true);
+            } else {
+                launch_rk2(st, R_node.h, Bd(), Fd(), 1);
+                stage_fused(false);
+                stage_fused(false);
+                launch_rk2(st, R_node.h, Bd(), Fd(), 0);
+                sync(G_UCELL);
+                sync(G_BFACE);
+                derived_fused(true);
+            }
+            return;
+        }
         launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt);
         if (integrator == 0) {
             stage();
@@ -833,7 +887,20 @@ public:
         sync(G_BADD);
         sync(G_BFACE);
         derived();
+        launch_cell_end(st, R_box.h, Bd(), Fd(), P, d_dt_next, 1);
         HIPCHK(hipStreamSynchronize(st));
+    }
+
+    void set_form(int f) {
+        if (f != 0 && f != 1) throw std::runtime_error("form must be 0 (per-function FP64) or 1 (fused FP64)");
+        if (f == 1 && form != 1) launch_cell_end(st, R_box.h, Bd(), Fd(), P, d_dt_next, 1);
+        form = f;
+        if (gexec) {
+            (void)hipGraphExecDestroy(gexec);
+            (void)hipGraphDestroy(graph);
+            gexec = nullptr;
+            graph = nullptr;
+        }
     }
 
     void run_op(const std::string &op, double arg) {
@@ -857,6 +924,15 @@ public:
             launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
         else if (op == "timestep") launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt);
         else if (op == "stage") stage();
+        else if (op == "rhs_cell") launch_rhs_cell(st, R_cell.h, Bd(), Fd(), P);
+        else if (op == "cell_end") launch_cell_end(st, R_box.h, Bd(), Fd(), P, d_dt_next, 1);
+        else if (op == "eface_u") launch_eface_u(st, R_eface.h, Bd(), Fd());
+        else if (op == "edge_total") launch_edge_total(st, R_node.h, Bd(), Fd(), P, alias.n == 0 ? 1 : 0);
+        else if (op == "update_all") launch_update_all(st, R_node.h, Bd(), Fd(), dtp());
+        else if (op == "bcell_jedge") launch_bcell_jedge(st, R_node.h, Bd(), Fd());
+        else if (op == "sync_all") {
+            for (int gid = 0; gid < G_NGROUP; ++gid) sync(gid);
+        }
         else if (op == "dec_d0") launch_dec_d0(st, R_node.h, Bd(), Fd(), SL_NODEF, SL_EDGEF);
         else if (op == "dec_d1") launch_dec_d1(st, R_node.h, Bd(), Fd(), SL_EDGEF, SL_FACEF, arg == 0.0 ? 1.0 : arg);
         else if (op == "dec_d2") launch_dec_d2(st, R_cell.h, Bd(), Fd(), SL_FACEF, SL_CELLS);
@@ -1053,6 +1129,11 @@ int mercury_divb(MercuryContext *ctx, double *normalised, double *maxabs) {
 int mercury_synchronize(MercuryContext *ctx) {
     CHECK_CTX(ctx);
     ABI_TRY({ HIPCHK(hipStreamSynchronize(ctx->s->st)); })
+}
+
+int mercury_set_form(MercuryContext *ctx, int form) {
+    CHECK_CTX(ctx);
+    ABI_TRY({ ctx->s->set_form(form); })
 }
 
 int mercury_graph_capture(MercuryContext *ctx, int integrator) {

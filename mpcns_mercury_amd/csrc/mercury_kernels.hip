@@ -15,87 +15,10 @@
 #include "mercury_common.h"
 #include "mercury_kernels.h"
 
-#define DEVI __device__ __forceinline__
+#include "mercury_device.h"
 
 namespace {
-
-DEVI double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
-DEVI double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
-
-struct Pt {
-    int b, i, j, k;
-    long long q;
-    int sI, sJ;
-};
-
-DEVI bool locate(const MercRange R, const MercBlk *__restrict__ B, long long t, Pt &p) {
-    if (t >= R.start[R.nb]) return false;
-    int lo = 0, hi = R.nb - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (R.start[mid] <= t) lo = mid; else hi = mid - 1;
-    }
-    const int b = lo;
-    unsigned r = (unsigned)(t - R.start[b]);  // per-block boxes stay below 2^31 points
-    const unsigned n1 = (unsigned)R.n[3 * b + 1], n2 = (unsigned)R.n[3 * b + 2];
-    const int kk = (int)(r % n2);
-    r /= n2;
-    const int jj = (int)(r % n1);
-    const int ii = (int)(r / n1);
-    const MercBlk &bk = B[b];
-    p.b = b;
-    p.i = R.lo[3 * b] + ii;
-    p.j = R.lo[3 * b + 1] + jj;
-    p.k = R.lo[3 * b + 2] + kk;
-    p.sI = bk.sI;
-    p.sJ = bk.sJ;
-    p.q = bk.off + (long long)(p.i + bk.g) * bk.sI + (long long)(p.j + bk.g) * bk.sJ + (p.k + bk.g);
-    return true;
-}
-
-// XCD-aware workgroup order: the dispatcher deals workgroups round-robin over
-// the 8 XCDs, each with its own L2; remapping gives every XCD one contiguous
-// slab of the index space, so the i/j stencil neighbours of a workgroup are
-// mostly resident in the same L2.
-DEVI long long xcd_tid() {
-    const unsigned nwg = gridDim.x, wg = blockIdx.x;
-    const unsigned q = nwg >> 3, r = nwg & 7u, x = wg & 7u, s = wg >> 3;
-    const unsigned logical = x * q + (x < r ? x : r) + s;
-    return (long long)logical * blockDim.x + threadIdx.x;
-}
-#define TID xcd_tid()
-#define LOCATE(R)            \
-    Pt p;                    \
-    if (!locate(R, B, TID, p)) return;
-
-// neighbour offsets
-#define OI(p) ((long long)(p).sI)
-#define OJ(p) ((long long)(p).sJ)
-#define AX(p, a) ((a) == 0 ? OI(p) : ((a) == 1 ? OJ(p) : 1LL))
-
-struct Num {
-    double nH, nNa, ne, ne_eff, chiH, chiNa, wH, wNa;
-};
-
-// Hall_Num_Limiter (Z4_Mercury/src/physics/MercurySolver_Tools.cpp:111-157)
-DEVI Num num_limiter(const MercPhys &P, double rhoH, double rhoNa) {
-    Num n;
-    const double rp = dmax(rhoH, 0.0), rn = dmax(rhoNa, 0.0);
-    n.nH = rp / P.M_H * P.M_ref;
-    n.nNa = rn / P.M_Na * P.M_ref;
-    n.ne = n.nH + n.nNa;
-    n.ne_eff = sqrt(n.ne * n.ne + P.ne_floor * P.ne_floor);
-    if (n.ne > 1.0e-300) {
-        n.chiH = n.nH / n.ne;
-        n.chiNa = n.nNa / n.ne;
-    } else {
-        n.chiH = 1.0;
-        n.chiNa = 0.0;
-    }
-    n.wH = n.nH / n.ne_eff;
-    n.wNa = n.nNa / n.ne_eff;
-    return n;
-}
+using namespace merc_dev;
 
 // ---------------------------------------------------------------------------
 // derived fields
@@ -270,35 +193,6 @@ __global__ void k_timestep(const MercRange R, const MercBlk *__restrict__ B, dou
 // accumulates -dF/Jac in the reference's xi, eta, zeta order; no face flux
 // array round trip.
 // ---------------------------------------------------------------------------
-DEVI void rusanov_face(const double *__restrict__ const *F, int su, int sp, long long qL, long long qR, long long qf, int a,
-                       double gm, double out[5]) {
-    const double k1 = F[SL_JD + 3 * a][qf], k2 = F[SL_JD + 3 * a + 1][qf], k3 = F[SL_JD + 3 * a + 2][qf];
-    double Uc[2][5], Fl[2][5], rad[2];
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-        const long long qq = side ? qR : qL;
-        const double rho = F[su][qq], u = F[sp][qq], v = F[sp + 1][qq], w = F[sp + 2][qq], pr = F[sp + 3][qq];
-        Uc[side][0] = rho;
-        Uc[side][1] = rho * u;
-        Uc[side][2] = rho * v;
-        Uc[side][3] = rho * w;
-        Uc[side][4] = pr / (gm - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
-        const double uvw = k1 * u + k2 * v + k3 * w;
-        const double kk2 = (k1 * k1 + k2 * k2 + k3 * k3);
-        const double c = sqrt(dmax(0.0, gm * pr / dmax(rho, 1e-30)));
-        rad[side] = fabs(uvw) + c * sqrt(kk2);
-        const double rhoe = pr / (gm - 1.0) + 0.5 * rho * (u * u + v * v + w * w);
-        Fl[side][0] = rho * uvw;
-        Fl[side][1] = rho * uvw * u + k1 * pr;
-        Fl[side][2] = rho * uvw * v + k2 * pr;
-        Fl[side][3] = rho * uvw * w + k3 * pr;
-        Fl[side][4] = uvw * (rhoe + pr);
-    }
-    const double r = dmax(rad[0], rad[1]);
-#pragma unroll
-    for (int m = 0; m < 5; ++m) out[m] = 0.5 * (Fl[0][m] + Fl[1][m]) - 0.5 * r * (Uc[1][m] - Uc[0][m]);
-}
-
 __global__ void k_flux_div(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, const MercPhys P) {
     LOCATE(R);
     const long long q = p.q;
@@ -400,32 +294,6 @@ __global__ void k_sources(const MercRange R, const MercBlk *__restrict__ B, doub
 // ---------------------------------------------------------------------------
 // ideal EMF: AssembleOneDirectionEMF_ (MercurySolver_MagExplicit_Ideal.cpp:433-594)
 // ---------------------------------------------------------------------------
-DEVI double u_contra(double *const *__restrict__ F, const Pt &p, int axis, long long c) {
-    const long long cp = c + AX(p, axis);
-    const int s0 = SL_JD + 3 * axis;
-    const double kx = 0.5 * (F[s0][c] + F[s0][cp]);
-    const double ky = 0.5 * (F[s0 + 1][c] + F[s0 + 1][cp]);
-    const double kz = 0.5 * (F[s0 + 2][c] + F[s0 + 2][cp]);
-    const double inv_jac = 1.0 / (fabs(F[SL_JAC][c]) + 1.0e-30);
-    return (kx * F[SL_UP][c] + ky * F[SL_UP + 1][c] + kz * F[SL_UP + 2][c]) * inv_jac;
-}
-DEVI double beta_ind(double *const *__restrict__ F, const Pt &p, int axis, long long c) {
-    return 0.5 * (F[SL_B + axis][c] + F[SL_B + axis][c + AX(p, axis)]);
-}
-DEVI double beta_tot(double *const *__restrict__ F, const Pt &p, int axis, long long c) {
-    const long long cp = c + AX(p, axis);
-    return 0.5 * ((F[SL_B + axis][c] + F[SL_BADD + axis][c]) + (F[SL_B + axis][cp] + F[SL_BADD + axis][cp]));
-}
-DEVI double emf_flux(double *const *__restrict__ F, const Pt &p, int ba, int fa, long long L, long long Rr, double bff) {
-    const double btL = beta_tot(F, p, ba, L), btR = beta_tot(F, p, ba, Rr);
-    const double biL = beta_ind(F, p, ba, L), biR = beta_ind(F, p, ba, Rr);
-    const double ufL = u_contra(F, p, fa, L), ufR = u_contra(F, p, fa, Rr);
-    const double ubL = u_contra(F, p, ba, L), ubR = u_contra(F, p, ba, Rr);
-    const double GL = ufL * btL - ubL * bff, GR = ufR * btR - ubR * bff;
-    const double radius = dmax(fabs(ufL), fabs(ufR));
-    return 0.5 * (GL + GR) - 0.5 * radius * (biR - biL);
-}
-
 // face candidates on owned faces +-1 (clamped to the allocation +-1); range is
 // the box [-1, m+2)^3 and each axis keeps its own valid sub-range
 __global__ void k_eface(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F) {
@@ -464,15 +332,6 @@ __global__ void k_eface(const MercRange R, const MercBlk *__restrict__ B, double
         F[SL_EF + 3 * fa + 1][Rq] = e1;
         F[SL_EF + 3 * fa + 2][Rq] = e2;
     }
-}
-
-DEVI double maxu4(double *const *__restrict__ F, const Pt &p, int ax, long long c0, long long c1, long long c2, long long c3) {
-    double a = 0.0;
-    a = dmax(a, fabs(u_contra(F, p, ax, c0)));
-    a = dmax(a, fabs(u_contra(F, p, ax, c1)));
-    a = dmax(a, fabs(u_contra(F, p, ax, c2)));
-    a = dmax(a, fabs(u_contra(F, p, ax, c3)));
-    return a;
 }
 
 // Edge EMF on owned edges: AssembleEdgeEMF_FromFaceE_Ideal_ (Ideal.cpp:138-372,

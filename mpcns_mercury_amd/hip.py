@@ -21,7 +21,7 @@ ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercu
                "mercury_block_volume", "mercury_total_volume", "mercury_upload", "mercury_download",
                "mercury_init_bface", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
                "mercury_divb", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
-               "mercury_time_op", "mercury_geometry_block")
+               "mercury_time_op", "mercury_geometry_block", "mercury_set_form")
 
 
 class MercuryError(RuntimeError):
@@ -55,6 +55,7 @@ def load():
     L.mercury_divb.argtypes = [vp, dp, dp]
     L.mercury_synchronize.argtypes = [vp]
     L.mercury_graph_capture.argtypes = [vp, ctypes.c_int]
+    L.mercury_set_form.argtypes = [vp, ctypes.c_int]
     L.mercury_graph_launch.argtypes = [vp, ctypes.c_int]
     L.mercury_time_op.argtypes = [vp, cp, ctypes.c_int, dp]
     L.mercury_geometry_block.argtypes = [ip, ctypes.c_int, dp, ctypes.POINTER(MercuryPhysics), cp, ctypes.c_int, dp]
@@ -172,6 +173,10 @@ class Mercury:
 
     def synchronize(self):
         _chk(load().mercury_synchronize(self._h))
+
+    def set_form(self, form):
+        """0: one kernel per reference function; 1: fused FP64 passes."""
+        _chk(load().mercury_set_form(self._h, int(form)))
 
     def graph_capture(self, integrator=0):
         _chk(load().mercury_graph_capture(self._h, integrator))
