@@ -116,7 +116,11 @@ def main():
     ap.add_argument("--workload", default="shell6")
     ap.add_argument("--cpu-sample-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-ops", action="store_true", help="print per-op timings to stderr")
+    ap.add_argument("--profile-ops", action="store_true", help="print per-op timings of the fused forms to stderr")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run three device-to-device copies of MIB MiB (a kernel of known traffic for HBM counters)")
+    ap.add_argument("--forms", default="fp64_plain,mixed,fp64_fused",
+                    help="comma list of forms to run (fp64_fused is always run: it gives the value)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); every block of the "
@@ -125,53 +129,91 @@ def main():
 
     from mpcns_mercury_amd.hip import Mercury  # noqa: E402
 
+    if args.calibrate_copy > 0:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copies: 3 x {args.calibrate_copy} MiB read + written", file=sys.stderr, flush=True)
+        del src, dst
+
     case = make_case(args.workload)
     ncells = case.ncells()
     print(f"[bench] workload={case.name} blocks={[b.dims for b in case.blocks]} cells={ncells}", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    g = Mercury(case)
-    g.prepare()
-    print(f"[bench] setup {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
 
-    # parity sample: the same steps on GPU and CPU from the same initial state
-    err = None
-    cpu = None
+    # CPU oracle sample: the baseline timing and the reference state for the error of each form
+    cpu, oref = None, None
     if not args.no_cpu:
-        o, tcpu = cpu_baseline(case, args.cpu_sample_steps)
-        g.step(args.cpu_sample_steps, case.integrator)
-        err = {}
-        for name in ("U_H", "U_Na", "B_xi", "B_eta", "B_zeta"):
-            num, den = 0.0, 0.0
-            for ib in range(case.nb):
-                a, b = g.get(name, ib), o.get(name, ib)
-                num += float(np.sum((a - b) ** 2))
-                den += float(np.sum(b ** 2))
-            err[name] = (num / den) ** 0.5 if den > 0 else num ** 0.5
+        oref, tcpu = cpu_baseline(case, args.cpu_sample_steps)
         cpu = {"value": ncells * args.cpu_sample_steps / tcpu, "unit": "cell-updates/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_sample_steps} full steps of {case.name} ({ncells} cells), CPU oracle, 1 thread, "
                          f"{tcpu:.1f} s"}
-        print(f"[bench] cpu sample {tcpu:.1f}s, rel-L2 vs oracle {err}", file=sys.stderr, flush=True)
-        del o
+        print(f"[bench] cpu sample {tcpu:.1f}s -> {cpu['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
 
-    g.graph_capture(case.integrator)
-    g.graph_launch(args.warmup)
-    g.synchronize()
-    t0 = time.perf_counter()
-    g.graph_launch(args.steps)
-    g.synchronize()
-    elapsed = time.perf_counter() - t0
-    dt_last = g.timestep()
-    if not (np.isfinite(dt_last) and dt_last > 1e-7):
-        raise SystemExit(f"bench: dt underflow after the timed steps (dt={dt_last})")
-    ms_step = elapsed * 1e3 / args.steps
-    value = ncells * args.steps / elapsed
-    divb_norm, _ = g.divb()
-    print(f"[bench] {args.steps} steps {elapsed:.3f}s  {ms_step:.3f} ms/step  {value:.4e} cell-updates/s  divB={divb_norm:.2e}",
-          file=sys.stderr, flush=True)
+    forms = {}
+    op_times = {}
+    g = None
+    fused_ops = ["rhs_cell", "eface_u", "edge_total", "update_all", "bcell_jedge", "jcell", "cell_end"]
+    wanted = set(args.forms.split(",")) | {"fp64_fused"}
+    for form, label in ((0, "fp64_plain"), (2, "mixed"), (1, "fp64_fused")):
+        if label not in wanted:
+            continue
+        if g is not None:
+            g.close()
+        t0 = time.perf_counter()
+        g = Mercury(case)
+        g.prepare()
+        g.set_form(form)
+        info = {}
+        if oref is not None:
+            g.step(args.cpu_sample_steps, case.integrator)
+            err, emax = {}, {}
+            for name in ("U_H", "U_Na", "B_xi", "B_eta", "B_zeta"):
+                num, den, mx, scale = 0.0, 0.0, 0.0, 0.0
+                for ib in range(case.nb):
+                    a_, b_ = g.get(name, ib), oref.get(name, ib)
+                    num += float(np.sum((a_ - b_) ** 2))
+                    den += float(np.sum(b_ ** 2))
+                    mx = max(mx, float(np.abs(a_ - b_).max()))
+                    scale = max(scale, float(np.abs(b_).max()))
+                err[name] = (num / den) ** 0.5 if den > 0 else num ** 0.5
+                emax[name] = mx / scale if scale > 0 else mx
+            info["rel_l2_vs_fp64_oracle"] = err
+            info["rel_max_vs_fp64_oracle"] = emax
+        g.graph_capture(case.integrator)
+        g.graph_launch(args.warmup)
+        g.synchronize()
+        t1 = time.perf_counter()
+        g.graph_launch(args.steps)
+        g.synchronize()
+        el = time.perf_counter() - t1
+        dt_last = g.timestep()
+        if not (np.isfinite(dt_last) and dt_last > 1e-7):
+            raise SystemExit(f"bench: dt underflow after the timed steps (form {label}, dt={dt_last})")
+        info["ms_per_step"] = el * 1e3 / args.steps
+        info["cell_updates_per_s"] = ncells * args.steps / el
+        info["divB_normalised"] = g.divb()[0]
+        forms[label] = info
+        print(f"[bench] {label}: {info['ms_per_step']:.3f} ms/step {info['cell_updates_per_s']:.4e} cell-updates/s "
+              f"divB={info['divB_normalised']:.2e} err={info.get('rel_l2_vs_fp64_oracle')} (setup+run {time.perf_counter() - t0:.1f}s)",
+              file=sys.stderr, flush=True)
+        if form >= 1:
+            # per-kernel device times of the fused passes (HIP events on the solver stream)
+            op_times[label] = {op: g.time_op(op, 20) for op in fused_ops}
+            if args.profile_ops:
+                for op in fused_ops:
+                    t = op_times[label][op]
+                    print(f"[bench] {label} op {op:12s} {t * 1e3:9.1f} us  {ALGO_BYTES[op] * ncells / (t * 1e-3) / 1e9:8.1f} GB/s",
+                          file=sys.stderr)
+    ms_step = forms["fp64_fused"]["ms_per_step"]
+    value = forms["fp64_fused"]["cell_updates_per_s"]
+    divb_norm = forms["fp64_fused"]["divB_normalised"]
+    err = forms["fp64_fused"].get("rel_l2_vs_fp64_oracle")
 
-    # per-kernel device times of the fused passes (HIP events on the solver stream)
-    ops = ["rhs_cell", "eface_u", "edge_total", "update_all", "bcell_jedge", "jcell", "cell_end"]
-    times = {op: g.time_op(op, 20) for op in ops}
+    times = op_times["fp64_fused"]
     sync_times = {op: g.time_op(op, 20) for op in ("sync_ucell", "sync_bface", "sync_eedge", "sync_ehall", "sync_jedge",
                                                    "sync_bcell", "sync_jcell")}
     step_graph_ms = g.time_op("step_graph", 10)
@@ -180,9 +222,6 @@ def main():
     algo = ALGO_BYTES[dom] * units
     achieved = algo / (times[dom] * 1e-3) / 1e9
     if args.profile_ops:
-        for op in ops:
-            print(f"[bench] op {op:10s} {times[op] * 1e3:9.1f} us  {ALGO_BYTES[op] * ncells / (times[op] * 1e-3) / 1e9:8.1f} GB/s",
-                  file=sys.stderr)
         for op, t in sync_times.items():
             print(f"[bench] {op:12s} {t * 1e3:9.1f} us", file=sys.stderr)
         print(f"[bench] step (graph) {step_graph_ms:.3f} ms", file=sys.stderr)
@@ -201,9 +240,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{case.name}: two-fluid Hall MHD, FP64 plain GPU form, 6 curvilinear sector blocks "
+        "config": {"workload": f"{case.name}: two-fluid Hall MHD, FP64 fused GPU form, {case.nb} curvilinear blocks "
                                f"{[list(b.dims) for b in case.blocks]}, forward Euler, CFL 0.4",
-                   "cells": ncells, "blocks": case.nb, "form": "fp64", "integrator": "euler"},
+                   "cells": ncells, "blocks": case.nb, "form": "fp64_fused", "integrator": "euler"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
                      "algo_bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": units,
@@ -211,9 +250,12 @@ def main():
         "step_roofline": {"stage_bytes_per_cell": STAGE_BYTES, "ms_per_step": step_graph_ms, "achieved_GBs": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
+        "forms": forms,
+        "mixed_over_fp64_plain": (forms["mixed"]["cell_updates_per_s"] / forms["fp64_plain"]["cell_updates_per_s"]
+                                  if "mixed" in forms and "fp64_plain" in forms else None),
         "err_vs_fp64_oracle": err,
         "divB_normalised": divb_norm,
-        "kernel_ms": times,
+        "kernel_ms": op_times,
     }
     print(json.dumps(out), flush=True)
 

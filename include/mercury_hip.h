@@ -120,8 +120,11 @@ int mercury_divb(MercuryContext *ctx, double *normalised, double *maxabs);
 
 int mercury_synchronize(MercuryContext *ctx);
 
-/* Kernel grouping of the FP64 step: 0 = one kernel per reference function
- * (the drop-in form), 1 = fused passes (default).  Both give identical results. */
+/* Form of the step: 0 = one kernel per reference function (the plain FP64
+ * drop-in), 1 = fused FP64 passes (default; identical results to 0),
+ * 2 = mixed precision: fluxes, EMFs, Hall terms and reconstructions in FP32
+ * (compensated flux divergence) with FP32 static arrays; state, metric,
+ * CT update, floors and dt in FP64. */
 int mercury_set_form(MercuryContext *ctx, int form);
 
 /* One explicit step captured as a HIP graph and replayed (no reference

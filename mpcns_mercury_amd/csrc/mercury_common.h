@@ -83,3 +83,11 @@ struct MercRange {
 };
 
 enum MercBC { MBC_PERIODIC = 0, MBC_FARFIELD = 1, MBC_WALL = 2 };
+
+// slot sets of a merged sync op (passed by value as a kernel argument)
+#define MERC_MAX_SETS 12
+struct MultiSets {
+    int n[MERC_MAX_SETS];
+    int s[MERC_MAX_SETS][MERC_MAX_SYNC_SLOTS];
+    double v[MERC_MAX_SETS][MERC_MAX_SYNC_SLOTS];
+};

@@ -1,14 +1,19 @@
-// Launch wrappers of the fused FP64 kernels (mercury_fused.hip).
+// Launch wrappers of the fused kernels (mercury_fused.hip); mix selects the
+// mixed-precision instantiation (FP32 fluxes / EMFs / Hall / reconstruction).
 #pragma once
 #include "mercury_kernels.h"
 
 namespace merc {
-void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P);
-void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P, unsigned long long *dt_next,
-                     int compute_dt);
-void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
-void launch_edge_total(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P, int with_hall);
+void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
+                     bool mix);
+void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
+                     unsigned long long *dt_next, int compute_dt, bool mix);
+void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, bool mix);
+void launch_edge_total(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
+                       int with_hall, bool mix);
 void launch_update_all(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const double *dt);
-void launch_bcell_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
+void launch_bcell_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);
+void launch_jcell_t(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);
+void launch_to_float(hipStream_t st, const double *src, float *dst, long long n);
 void launch_dt_commit(hipStream_t st, unsigned long long *dt, unsigned long long *dt_next);
 }  // namespace merc

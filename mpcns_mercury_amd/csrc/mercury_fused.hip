@@ -21,26 +21,103 @@
 namespace {
 using namespace merc_dev;
 
+// precision policy: T = double (bit-exact FP64 form) or float (mixed form).
+// In the mixed form the static arrays marked FP32 are read from the float
+// table G, divisions and square roots use the hardware reciprocal / square
+// root (about 1 ulp in FP32), and multi-term sums are compensated.
+template <bool MIX> struct Prec;
+template <> struct Prec<false> {
+    using T = double;
+    static DEVI T div(T a, T b) { return a / b; }
+    static DEVI T sq(T a) { return sqrt(a); }
+};
+template <> struct Prec<true> {
+    using T = float;
+    static DEVI T div(T a, T b) { return a * __builtin_amdgcn_rcpf(b); }
+    static DEVI T sq(T a) { return __builtin_amdgcn_sqrtf(a); }
+};
+// static array element: FP64 table in the FP64 form, FP32 table in the mixed form
+template <bool MIX>
+DEVI typename Prec<MIX>::T st(double *const *__restrict__ F, const float *const *__restrict__ G, int s, long long q) {
+    if constexpr (MIX) return G[s][q];
+    else return F[s][q];
+}
+template <typename T> DEVI T tmax(T a, T b) { return (a < b) ? b : a; }
+template <typename T> DEVI T tmin(T a, T b) { return (b < a) ? b : a; }
+
+// compensated accumulation (Kahan) used by the mixed form for the flux
+// divergence; the FP64 form keeps the reference's plain sum
+template <bool MIX> struct Acc {
+    using T = typename Prec<MIX>::T;
+    T s = T(0), c = T(0);
+    DEVI void sub(T x) {
+        if constexpr (MIX) {
+            const T y = -x - c;
+            const T t = s + y;
+            c = (t - s) - y;
+            s = t;
+        } else {
+            s -= x;
+        }
+    }
+    DEVI double value() const { return MIX ? (double)s - (double)c : (double)s; }
+};
+
+template <bool MIX>
+DEVI void rusanov_face_t(double *const *__restrict__ F, int su, int sp, long long qL, long long qR, long long qf, int a,
+                         typename Prec<MIX>::T gm, typename Prec<MIX>::T out[5]) {
+    using T = typename Prec<MIX>::T;
+    using PR = Prec<MIX>;
+    const T k1 = (T)F[SL_JD + 3 * a][qf], k2 = (T)F[SL_JD + 3 * a + 1][qf], k3 = (T)F[SL_JD + 3 * a + 2][qf];
+    T Uc[2][5], Fl[2][5], rad[2];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const long long qq = side ? qR : qL;
+        const T rho = (T)F[su][qq], u = (T)F[sp][qq], v = (T)F[sp + 1][qq], w = (T)F[sp + 2][qq], pr = (T)F[sp + 3][qq];
+        Uc[side][0] = rho;
+        Uc[side][1] = rho * u;
+        Uc[side][2] = rho * v;
+        Uc[side][3] = rho * w;
+        Uc[side][4] = PR::div(pr, gm - T(1)) + T(0.5) * rho * (u * u + v * v + w * w);
+        const T uvw = k1 * u + k2 * v + k3 * w;
+        const T kk2 = (k1 * k1 + k2 * k2 + k3 * k3);
+        const T c = PR::sq(tmax(T(0), PR::div(gm * pr, tmax(rho, T(1e-30)))));
+        rad[side] = fabs(uvw) + c * PR::sq(kk2);
+        const T rhoe = PR::div(pr, gm - T(1)) + T(0.5) * rho * (u * u + v * v + w * w);
+        Fl[side][0] = rho * uvw;
+        Fl[side][1] = rho * uvw * u + k1 * pr;
+        Fl[side][2] = rho * uvw * v + k2 * pr;
+        Fl[side][3] = rho * uvw * w + k3 * pr;
+        Fl[side][4] = uvw * (rhoe + pr);
+    }
+    const T r = tmax(rad[0], rad[1]);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) out[m] = T(0.5) * (Fl[0][m] + Fl[1][m]) - T(0.5) * r * (Uc[1][m] - Uc[0][m]);
+}
+
 // ---------------------------------------------------------------------------
-__global__ void k_rhs_cell(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, const MercPhys P) {
+template <bool MIX>
+__global__ void k_rhs_cell(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
+                           const float *const *__restrict__ G, const MercPhys P) {
+    using T = typename Prec<MIX>::T;
     LOCATE(R);
     const long long q = p.q, di = OI(p), dj = OJ(p);
-    const double gm = P.gamma;
-    const double invJ = 1.0 / F[SL_JAC][q];
+    const T gm = (T)P.gamma;
+    const T invJ = (T)(1.0 / F[SL_JAC][q]);
     // --- Scheme_U_ (MercurySolver_FluidRHS.cpp:3-202)
     double rr[2][5];
     for (int s = 0; s < 2; ++s) {
         const int su = s ? SL_UN : SL_UH, sp = s ? SL_PVN : SL_PVH;
-        double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        Acc<MIX> r[5];
         for (int a = 0; a < 3; ++a) {
             const long long d = AX(p, a);
-            double Fm[5], Fp[5];
-            rusanov_face((const double *const *)F, su, sp, q - d, q, q, a, gm, Fm);
-            rusanov_face((const double *const *)F, su, sp, q, q + d, q + d, a, gm, Fp);
+            T Fm[5], Fp[5];
+            rusanov_face_t<MIX>(F, su, sp, q - d, q, q, a, gm, Fm);
+            rusanov_face_t<MIX>(F, su, sp, q, q + d, q + d, a, gm, Fp);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) r[m] -= (Fp[m] - Fm[m]) * invJ;
+            for (int m = 0; m < 5; ++m) r[m].sub((Fp[m] - Fm[m]) * invJ);
         }
-        for (int m = 0; m < 5; ++m) rr[s][m] = r[m];
+        for (int m = 0; m < 5; ++m) rr[s][m] = r[m].value();
     }
     // --- AddSourceToRHS_Fluid (MercurySolver_FluidRHS.cpp:206-527)
     const double upx = F[SL_UP][q], upy = F[SL_UP + 1][q], upz = F[SL_UP + 2][q];
@@ -69,7 +146,7 @@ __global__ void k_rhs_cell(const MercRange R, const MercBlk *__restrict__ B, dou
         dpey = 0.5 * (d[1] * (pip - pim) + d[4] * (pjp - pjm) + d[7] * (pkp - pkm));
         dpez = 0.5 * (d[2] * (pip - pim) + d[5] * (pjp - pjm) + d[8] * (pkp - pkm));
     }
-    const double sss = F[SL_PHOTO][q];
+    const double sss = (double)st<MIX>(F, G, SL_PHOTO, q);
     const double x = F[SL_CC][q], y = F[SL_CC + 1][q], z = F[SL_CC + 2][q];
     const double rho_cyl = sqrt(y * y + z * z);
     const bool in_shadow = (x < 0.0 && rho_cyl < 1.0);
@@ -122,8 +199,12 @@ __global__ void k_rhs_cell(const MercRange R, const MercBlk *__restrict__ B, dou
 // PV_s, U+ (Tools.cpp:159-301), Hall E_hc and beta (Hall.cpp:87-124), the
 // contravariant velocity of U+ per axis (cell_u_contra, Ideal.cpp:513-526) and,
 // on owned cells when requested, the CFL candidate of Compute_Timestep.
-__global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, const MercPhys P,
-                           unsigned long long *__restrict__ dt_bits, int compute_dt) {
+template <bool MIX>
+__global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
+                           const float *const *__restrict__ G, const MercPhys P, unsigned long long *__restrict__ dt_bits,
+                           int compute_dt) {
+    using T = typename Prec<MIX>::T;
+    using PR = Prec<MIX>;
     double best = 1e100;
     Pt p;
     if (locate(R, B, TID, p)) {
@@ -156,24 +237,27 @@ __global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, dou
         }
         // Hall cell terms (the limiter of the raw densities, Hall.cpp:92-95)
         const Num nh = num_limiter(P, F[SL_UH][q], F[SL_UN][q]);
-        const double alpha = P.hall_coef / nh.ne_eff;
-        const double Jx = F[SL_JC][q], Jy = F[SL_JC + 1][q], Jz = F[SL_JC + 2][q];
         const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
-        F[SL_EHC][q] = alpha * (Jy * Bz - Jz * By);
-        F[SL_EHC + 1][q] = alpha * (Jz * Bx - Jx * Bz);
-        F[SL_EHC + 2][q] = alpha * (Jx * By - Jy * Bx);
-        F[SL_HB][q] = fabs(alpha) * sqrt(Bx * Bx + By * By + Bz * Bz);
+        {
+            const T alpha = (T)(P.hall_coef / nh.ne_eff);
+            const T Jx = (T)F[SL_JC][q], Jy = (T)F[SL_JC + 1][q], Jz = (T)F[SL_JC + 2][q];
+            const T bx = (T)Bx, by = (T)By, bz = (T)Bz;
+            F[SL_EHC][q] = alpha * (Jy * bz - Jz * by);
+            F[SL_EHC + 1][q] = alpha * (Jz * bx - Jx * bz);
+            F[SL_EHC + 2][q] = alpha * (Jx * by - Jy * bx);
+            F[SL_HB][q] = fabs(alpha) * PR::sq(bx * bx + by * by + bz * bz);
+        }
         // contravariant velocities (need the face of the next index on each axis)
         const int idx[3] = {p.i, p.j, p.k};
-        const double inv_jac = 1.0 / (fabs(F[SL_JAC][q]) + 1.0e-30);
+        const T inv_jac = MIX ? (T)(1.0 / (fabs(F[SL_JAC][q]) + 1.0e-30)) : (T)(1.0 / (fabs(F[SL_JAC][q]) + 1.0e-30));
         for (int a = 0; a < 3; ++a) {
             if (idx[a] + 1 > bk.m[a] + bk.g) continue;
             const long long cp = q + AX(p, a);
             const int s0 = SL_JD + 3 * a;
-            const double kx = 0.5 * (F[s0][q] + F[s0][cp]);
-            const double ky = 0.5 * (F[s0 + 1][q] + F[s0 + 1][cp]);
-            const double kz = 0.5 * (F[s0 + 2][q] + F[s0 + 2][cp]);
-            F[SL_UCON + a][q] = (kx * up[0] + ky * up[1] + kz * up[2]) * inv_jac;
+            const T kx = T(0.5) * ((T)F[s0][q] + (T)F[s0][cp]);
+            const T ky = T(0.5) * ((T)F[s0 + 1][q] + (T)F[s0 + 1][cp]);
+            const T kz = T(0.5) * ((T)F[s0 + 2][q] + (T)F[s0 + 2][cp]);
+            F[SL_UCON + a][q] = (kx * (T)up[0] + ky * (T)up[1] + kz * (T)up[2]) * inv_jac;
         }
         // time-step candidate on owned cells
         if (compute_dt && p.i >= 0 && p.j >= 0 && p.k >= 0 && p.i < bk.m[0] && p.j < bk.m[1] && p.k < bk.m[2]) {
@@ -229,17 +313,25 @@ __global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, dou
 }
 
 // ---------------------------------------------------------------------------
-DEVI double emf_flux_u(double *const *__restrict__ F, const Pt &p, int ba, int fa, long long L, long long Rr, double bff) {
-    const double btL = beta_tot(F, p, ba, L), btR = beta_tot(F, p, ba, Rr);
-    const double biL = beta_ind(F, p, ba, L), biR = beta_ind(F, p, ba, Rr);
-    const double ufL = F[SL_UCON + fa][L], ufR = F[SL_UCON + fa][Rr];
-    const double ubL = F[SL_UCON + ba][L], ubR = F[SL_UCON + ba][Rr];
-    const double GL = ufL * btL - ubL * bff, GR = ufR * btR - ubR * bff;
-    const double radius = dmax(fabs(ufL), fabs(ufR));
-    return 0.5 * (GL + GR) - 0.5 * radius * (biR - biL);
+template <bool MIX>
+DEVI typename Prec<MIX>::T emf_flux_u(double *const *__restrict__ F, const Pt &p, int ba, int fa, long long L, long long Rr,
+                                      typename Prec<MIX>::T bff) {
+    using T = typename Prec<MIX>::T;
+    const long long dL = AX(p, ba);
+    const T btL = T(0.5) * (((T)F[SL_B + ba][L] + (T)F[SL_BADD + ba][L]) + ((T)F[SL_B + ba][L + dL] + (T)F[SL_BADD + ba][L + dL]));
+    const T btR = T(0.5) * (((T)F[SL_B + ba][Rr] + (T)F[SL_BADD + ba][Rr]) + ((T)F[SL_B + ba][Rr + dL] + (T)F[SL_BADD + ba][Rr + dL]));
+    const T biL = T(0.5) * ((T)F[SL_B + ba][L] + (T)F[SL_B + ba][L + dL]);
+    const T biR = T(0.5) * ((T)F[SL_B + ba][Rr] + (T)F[SL_B + ba][Rr + dL]);
+    const T ufL = (T)F[SL_UCON + fa][L], ufR = (T)F[SL_UCON + fa][Rr];
+    const T ubL = (T)F[SL_UCON + ba][L], ubR = (T)F[SL_UCON + ba][Rr];
+    const T GL = ufL * btL - ubL * bff, GR = ufR * btR - ubR * bff;
+    const T radius = tmax(fabs(ufL), fabs(ufR));
+    return T(0.5) * (GL + GR) - T(0.5) * radius * (biR - biL);
 }
 
+template <bool MIX>
 __global__ void k_eface_u(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F) {
+    using T = typename Prec<MIX>::T;
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const int idx[3] = {p.i, p.j, p.k};
@@ -253,21 +345,21 @@ __global__ void k_eface_u(const MercRange R, const MercBlk *__restrict__ B, doub
         }
         if (!ok) continue;
         const long long Rq = p.q, Lq = p.q - AX(p, fa);
-        const double bff = F[SL_B + fa][Rq] + F[SL_BADD + fa][Rq];
-        double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+        const T bff = (T)F[SL_B + fa][Rq] + (T)F[SL_BADD + fa][Rq];
+        T e0 = T(0), e1 = T(0), e2 = T(0);
         if (fa == 0) {
-            const double Gex = emf_flux_u(F, p, 1, 0, Lq, Rq, bff);
-            const double Gzx = emf_flux_u(F, p, 2, 0, Lq, Rq, bff);
+            const T Gex = emf_flux_u<MIX>(F, p, 1, 0, Lq, Rq, bff);
+            const T Gzx = emf_flux_u<MIX>(F, p, 2, 0, Lq, Rq, bff);
             e1 = Gzx;
             e2 = -Gex;
         } else if (fa == 1) {
-            const double Gxe = emf_flux_u(F, p, 0, 1, Lq, Rq, bff);
-            const double Gze = emf_flux_u(F, p, 2, 1, Lq, Rq, bff);
+            const T Gxe = emf_flux_u<MIX>(F, p, 0, 1, Lq, Rq, bff);
+            const T Gze = emf_flux_u<MIX>(F, p, 2, 1, Lq, Rq, bff);
             e0 = -Gze;
             e2 = Gxe;
         } else {
-            const double Gxz = emf_flux_u(F, p, 0, 2, Lq, Rq, bff);
-            const double Gez = emf_flux_u(F, p, 1, 2, Lq, Rq, bff);
+            const T Gxz = emf_flux_u<MIX>(F, p, 0, 2, Lq, Rq, bff);
+            const T Gez = emf_flux_u<MIX>(F, p, 1, 2, Lq, Rq, bff);
             e0 = Gez;
             e1 = -Gxz;
         }
@@ -277,66 +369,71 @@ __global__ void k_eface_u(const MercRange R, const MercBlk *__restrict__ B, doub
     }
 }
 
-DEVI double maxu4_p(double *const *__restrict__ F, int ax, long long c0, long long c1, long long c2, long long c3) {
-    double a = 0.0;
-    a = dmax(a, fabs(F[SL_UCON + ax][c0]));
-    a = dmax(a, fabs(F[SL_UCON + ax][c1]));
-    a = dmax(a, fabs(F[SL_UCON + ax][c2]));
-    a = dmax(a, fabs(F[SL_UCON + ax][c3]));
+template <typename T>
+DEVI T maxu4_p(double *const *__restrict__ F, int ax, long long c0, long long c1, long long c2, long long c3) {
+    T a = T(0);
+    a = tmax(a, (T)fabs(F[SL_UCON + ax][c0]));
+    a = tmax(a, (T)fabs(F[SL_UCON + ax][c1]));
+    a = tmax(a, (T)fabs(F[SL_UCON + ax][c2]));
+    a = tmax(a, (T)fabs(F[SL_UCON + ax][c3]));
     return a;
 }
 
 // edge EMF, all contributions, owned edges; with_hall adds the Hall EMF and
 // applies the wall condition (used when no periodic seam needs the alias mean
 // of the non-Hall part first)
-__global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, const MercPhys P,
-                             int with_hall) {
+template <bool MIX>
+__global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
+                             const float *const *__restrict__ G, const MercPhys P, int with_hall) {
+    using T = typename Prec<MIX>::T;
+    using PR = Prec<MIX>;
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
     const int idx[3] = {p.i, p.j, p.k};
-    const double Cuct = 0.5;
+    const T Cuct = T(0.5);
     const bool ambi = P.ambipolar_enabled != 0.0;
     const bool arti = P.arti_eta_max > 0.0;
     const bool local = P.local_enabled != 0.0 && P.local_eta_max > 0.0 && P.local_r_decay > 0.0;
+#define FB(s, o) ((T)F[(s)][q + (o)])
     for (int a = 0; a < 3; ++a) {
         if (idx[a] >= bk.m[a]) continue;
-        double E;
+        T E;
         if (a == 0) {
-            const double Ec = 0.25 * (F[SL_EF + 3][q] + F[SL_EF + 3][q - dk] + F[SL_EF + 6][q] + F[SL_EF + 6][q - dj]);
-            const double ae = maxu4_p(F, 1, q, q - dj, q - dk, q - dj - dk);
-            const double az = maxu4_p(F, 2, q, q - dj, q - dk, q - dj - dk);
-            const double dBz = F[SL_B + 2][q] - F[SL_B + 2][q - dj];
-            const double dBe = F[SL_B + 1][q] - F[SL_B + 1][q - dk];
-            E = Ec + Cuct * (0.5 * ae * dBz - 0.5 * az * dBe);
+            const T Ec = T(0.25) * (FB(SL_EF + 3, 0) + FB(SL_EF + 3, -dk) + FB(SL_EF + 6, 0) + FB(SL_EF + 6, -dj));
+            const T ae = maxu4_p<T>(F, 1, q, q - dj, q - dk, q - dj - dk);
+            const T az = maxu4_p<T>(F, 2, q, q - dj, q - dk, q - dj - dk);
+            const T dBz = FB(SL_B + 2, 0) - FB(SL_B + 2, -dj);
+            const T dBe = FB(SL_B + 1, 0) - FB(SL_B + 1, -dk);
+            E = Ec + Cuct * (T(0.5) * ae * dBz - T(0.5) * az * dBe);
         } else if (a == 1) {
-            const double Ec = 0.25 * (F[SL_EF + 1][q] + F[SL_EF + 1][q - dk] + F[SL_EF + 7][q] + F[SL_EF + 7][q - di]);
-            const double az = maxu4_p(F, 2, q, q - di, q - dk, q - di - dk);
-            const double ax = maxu4_p(F, 0, q, q - di, q - dk, q - di - dk);
-            const double dBx = F[SL_B + 0][q] - F[SL_B + 0][q - dk];
-            const double dBz = F[SL_B + 2][q] - F[SL_B + 2][q - di];
-            E = Ec + Cuct * (0.5 * az * dBx - 0.5 * ax * dBz);
+            const T Ec = T(0.25) * (FB(SL_EF + 1, 0) + FB(SL_EF + 1, -dk) + FB(SL_EF + 7, 0) + FB(SL_EF + 7, -di));
+            const T az = maxu4_p<T>(F, 2, q, q - di, q - dk, q - di - dk);
+            const T ax = maxu4_p<T>(F, 0, q, q - di, q - dk, q - di - dk);
+            const T dBx = FB(SL_B + 0, 0) - FB(SL_B + 0, -dk);
+            const T dBz = FB(SL_B + 2, 0) - FB(SL_B + 2, -di);
+            E = Ec + Cuct * (T(0.5) * az * dBx - T(0.5) * ax * dBz);
         } else {
-            const double Ec = 0.25 * (F[SL_EF + 2][q] + F[SL_EF + 2][q - dj] + F[SL_EF + 5][q] + F[SL_EF + 5][q - di]);
-            const double ax = maxu4_p(F, 0, q, q - di, q - dj, q - di - dj);
-            const double ae = maxu4_p(F, 1, q, q - di, q - dj, q - di - dj);
-            const double dBe = F[SL_B + 1][q] - F[SL_B + 1][q - di];
-            const double dBx = F[SL_B + 0][q] - F[SL_B + 0][q - dj];
-            E = Ec + Cuct * (0.5 * ax * dBe - 0.5 * ae * dBx);
+            const T Ec = T(0.25) * (FB(SL_EF + 2, 0) + FB(SL_EF + 2, -dj) + FB(SL_EF + 5, 0) + FB(SL_EF + 5, -di));
+            const T ax = maxu4_p<T>(F, 0, q, q - di, q - dj, q - di - dj);
+            const T ae = maxu4_p<T>(F, 1, q, q - di, q - dj, q - di - dj);
+            const T dBe = FB(SL_B + 1, 0) - FB(SL_B + 1, -di);
+            const T dBx = FB(SL_B + 0, 0) - FB(SL_B + 0, -dj);
+            E = Ec + Cuct * (T(0.5) * ax * dBe - T(0.5) * ae * dBx);
         }
         if (ambi) {
             const long long ea = AX(p, a);
-            double pe[2];
+            T pe[2];
             for (int e = 0; e < 2; ++e) {
                 const long long n0 = q + (e ? ea : 0);
-                double sum = 0.0;
+                T sum = T(0);
                 for (int ddi = -1; ddi <= 0; ++ddi)
                     for (int ddj = -1; ddj <= 0; ++ddj)
                         for (int ddk = -1; ddk <= 0; ++ddk) {
                             const long long c = n0 + ddi * di + ddj * dj + ddk;
-                            sum += F[SL_PVH + 3][c] + F[SL_PVN + 3][c];
+                            sum += (T)F[SL_PVH + 3][c] + (T)F[SL_PVN + 3][c];
                         }
-                pe[e] = 0.125 * sum;
+                pe[e] = T(0.125) * sum;
             }
             double rH = 0.0, rN = 0.0;
             const long long t1 = (a == 0) ? dj : di, t2 = (a == 2) ? dj : 1;
@@ -347,81 +444,86 @@ __global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, d
                     rN += F[SL_UN][c];
                 }
             const Num n = num_limiter(P, 0.25 * rH, 0.25 * rN);
-            E -= P.pressure_coef * (pe[1] - pe[0]) / n.ne_eff;
+            E -= PR::div((T)P.pressure_coef * (pe[1] - pe[0]), (T)n.ne_eff);
         }
         if (arti) {
-            const double J1 = F[SL_J + a][q];
-            const double dx = F[SL_DR + 3 * a][q], dy = F[SL_DR + 3 * a + 1][q], dz = F[SL_DR + 3 * a + 2][q];
-            const double drl = sqrt(dx * dx + dy * dy + dz * dz);
-            const double Jm = fabs(J1) / dmax(drl, 1.0e-30);
-            double eta;
+            const T J1 = FB(SL_J + a, 0);
+            const T dx = st<MIX>(F, G, SL_DR + 3 * a, q), dy = st<MIX>(F, G, SL_DR + 3 * a + 1, q), dz = st<MIX>(F, G, SL_DR + 3 * a + 2, q);
+            const T drl = PR::sq(dx * dx + dy * dy + dz * dz);
+            const T Jm = PR::div(fabs(J1), tmax(drl, T(1.0e-30)));
+            T eta;
             if (P.J_on <= P.J_start)
-                eta = (Jm > P.J_start) ? P.arti_eta_max : 0.0;
+                eta = (Jm > (T)P.J_start) ? (T)P.arti_eta_max : T(0);
             else {
-                double qq = (Jm - P.J_start) / (P.J_on - P.J_start);
-                qq = dmax(0.0, dmin(1.0, qq));
-                eta = P.arti_eta_max * (qq * qq * (3.0 - 2.0 * qq));
+                T qq = PR::div(Jm - (T)P.J_start, (T)(P.J_on - P.J_start));
+                qq = tmax(T(0), tmin(T(1), qq));
+                eta = (T)P.arti_eta_max * (qq * qq * (T(3) - T(2) * qq));
             }
             E += eta * J1;
         }
-        if (local) E += F[SL_LETA + a][q] * F[SL_J + a][q];
+        if (local) E += st<MIX>(F, G, SL_LETA + a, q) * FB(SL_J + a, 0);
         if (!with_hall) {
-            F[SL_E + a][q] = E;
+            F[SL_E + a][q] = (double)E;
             continue;
         }
         // Hall EMF of this edge (Hall.cpp:126-385)
-        const double eps = 1e-14, Cwh = 0.1;
+        const T eps = T(1e-14), Cwh = T(0.1);
         long long c[4];
         if (a == 0) { c[0] = q - dj - dk; c[1] = q - dj; c[2] = q - dk; c[3] = q; }
         else if (a == 1) { c[0] = q - di - dk; c[1] = q - di; c[2] = q - dk; c[3] = q; }
         else { c[0] = q - di - dj; c[1] = q - di; c[2] = q - dj; c[3] = q; }
-        double Ecc[3];
-        for (int d = 0; d < 3; ++d) Ecc[d] = 0.25 * (F[SL_EHC + d][c[0]] + F[SL_EHC + d][c[1]] + F[SL_EHC + d][c[2]] + F[SL_EHC + d][c[3]]);
-        const double Ecen = Ecc[0] * F[SL_DR + 3 * a][q] + Ecc[1] * F[SL_DR + 3 * a + 1][q] + Ecc[2] * F[SL_DR + 3 * a + 2][q];
-        const double be = dmax(dmax(F[SL_HB][c[0]], F[SL_HB][c[1]]), dmax(F[SL_HB][c[2]], F[SL_HB][c[3]]));
-        const double L = dmax(F[SL_DL + a][q], eps);
-        double val;
+        T Ecc[3];
+        for (int d = 0; d < 3; ++d)
+            Ecc[d] = T(0.25) * ((T)F[SL_EHC + d][c[0]] + (T)F[SL_EHC + d][c[1]] + (T)F[SL_EHC + d][c[2]] + (T)F[SL_EHC + d][c[3]]);
+        const T Ecen = Ecc[0] * st<MIX>(F, G, SL_DR + 3 * a, q) + Ecc[1] * st<MIX>(F, G, SL_DR + 3 * a + 1, q) +
+                       Ecc[2] * st<MIX>(F, G, SL_DR + 3 * a + 2, q);
+        const T be = tmax(tmax((T)F[SL_HB][c[0]], (T)F[SL_HB][c[1]]), tmax((T)F[SL_HB][c[2]], (T)F[SL_HB][c[3]]));
+        const T L = tmax(st<MIX>(F, G, SL_DL + a, q), eps);
+#define AR(s, o) st<MIX>(F, G, (s), q + (o))
+        T val;
         if (a == 0) {
-            const double Az = 0.5 * (F[SL_AREA + 2][q] + F[SL_AREA + 2][q - dj]);
-            const double ihe = L / dmax(Az, eps);
-            const double mue = Cwh * be * ihe * ihe;
-            const double Ae = 0.5 * (F[SL_AREA + 1][q] + F[SL_AREA + 1][q - dk]);
-            const double ihz = L / dmax(Ae, eps);
-            const double muz = Cwh * be * ihz * ihz;
-            const double d1 = F[SL_B + 2][q] - F[SL_B + 2][q - dj];
-            const double d2 = F[SL_B + 1][q] - F[SL_B + 1][q - dk];
-            val = Ecen + 0.5 * mue * d1 - 0.5 * muz * d2;
+            const T Az = T(0.5) * (AR(SL_AREA + 2, 0) + AR(SL_AREA + 2, -dj));
+            const T ihe = PR::div(L, tmax(Az, eps));
+            const T mue = Cwh * be * ihe * ihe;
+            const T Ae = T(0.5) * (AR(SL_AREA + 1, 0) + AR(SL_AREA + 1, -dk));
+            const T ihz = PR::div(L, tmax(Ae, eps));
+            const T muz = Cwh * be * ihz * ihz;
+            const T d1 = FB(SL_B + 2, 0) - FB(SL_B + 2, -dj);
+            const T d2 = FB(SL_B + 1, 0) - FB(SL_B + 1, -dk);
+            val = Ecen + T(0.5) * mue * d1 - T(0.5) * muz * d2;
         } else if (a == 1) {
-            const double Ax = 0.5 * (F[SL_AREA + 0][q] + F[SL_AREA + 0][q - dk]);
-            const double ihz = L / dmax(Ax, eps);
-            const double muz = Cwh * be * ihz * ihz;
-            const double Az = 0.5 * (F[SL_AREA + 2][q] + F[SL_AREA + 2][q - di]);
-            const double ihx = L / dmax(Az, eps);
-            const double mux = Cwh * be * ihx * ihx;
-            const double d1 = F[SL_B + 0][q] - F[SL_B + 0][q - dk];
-            const double d2 = F[SL_B + 2][q] - F[SL_B + 2][q - di];
-            val = Ecen + 0.5 * muz * d1 - 0.5 * mux * d2;
+            const T Ax = T(0.5) * (AR(SL_AREA + 0, 0) + AR(SL_AREA + 0, -dk));
+            const T ihz = PR::div(L, tmax(Ax, eps));
+            const T muz = Cwh * be * ihz * ihz;
+            const T Az = T(0.5) * (AR(SL_AREA + 2, 0) + AR(SL_AREA + 2, -di));
+            const T ihx = PR::div(L, tmax(Az, eps));
+            const T mux = Cwh * be * ihx * ihx;
+            const T d1 = FB(SL_B + 0, 0) - FB(SL_B + 0, -dk);
+            const T d2 = FB(SL_B + 2, 0) - FB(SL_B + 2, -di);
+            val = Ecen + T(0.5) * muz * d1 - T(0.5) * mux * d2;
         } else {
-            const double Ae = 0.5 * (F[SL_AREA + 1][q] + F[SL_AREA + 1][q - di]);
-            const double ihx = L / dmax(Ae, eps);
-            const double mux = Cwh * be * ihx * ihx;
-            const double Ax = 0.5 * (F[SL_AREA + 0][q] + F[SL_AREA + 0][q - dj]);
-            const double ihe = L / dmax(Ax, eps);
-            const double mue = Cwh * be * ihe * ihe;
-            const double d1 = F[SL_B + 1][q] - F[SL_B + 1][q - di];
-            const double d2 = F[SL_B + 0][q] - F[SL_B + 0][q - dj];
-            val = Ecen + 0.5 * mux * d1 - 0.5 * mue * d2;
+            const T Ae = T(0.5) * (AR(SL_AREA + 1, 0) + AR(SL_AREA + 1, -di));
+            const T ihx = PR::div(L, tmax(Ae, eps));
+            const T mux = Cwh * be * ihx * ihx;
+            const T Ax = T(0.5) * (AR(SL_AREA + 0, 0) + AR(SL_AREA + 0, -dj));
+            const T ihe = PR::div(L, tmax(Ax, eps));
+            const T mue = Cwh * be * ihe * ihe;
+            const T d1 = FB(SL_B + 1, 0) - FB(SL_B + 1, -di);
+            const T d2 = FB(SL_B + 0, 0) - FB(SL_B + 0, -dj);
+            val = Ecen + T(0.5) * mux * d1 - T(0.5) * mue * d2;
         }
-        const double eh = F[SL_TAPER + a][q] * val;
-        F[SL_EH + a][q] = eh;
-        double e = E + eh;
+#undef AR
+        const T eh = st<MIX>(F, G, SL_TAPER + a, q) * val;
+        F[SL_EH + a][q] = (double)eh;
+        T e = E + eh;
         for (int f = 0; f < 6; ++f) {
             const int nax = f >> 1;
             if (bk.bc[f] != MBC_WALL || nax == a) continue;
-            if ((f & 1) ? (idx[nax] == bk.m[nax]) : (idx[nax] == 0)) e = 0.0;
+            if ((f & 1) ? (idx[nax] == bk.m[nax]) : (idx[nax] == 0)) e = T(0);
         }
-        F[SL_E + a][q] = e;
+        F[SL_E + a][q] = (double)e;
     }
+#undef FB
 }
 
 // ---------------------------------------------------------------------------
@@ -456,52 +558,79 @@ __global__ void k_update_all(const MercRange R, const MercBlk *__restrict__ B, d
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_bcell_jedge(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F) {
+template <bool MIX>
+__global__ void k_bcell_jedge(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
+                              const float *const *__restrict__ G) {
+    using T = typename Prec<MIX>::T;
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
     if (p.i < bk.m[0] && p.j < bk.m[1] && p.k < bk.m[2]) {
-        double pt[6], pi[6];
-        pt[0] = -F[SL_B + 0][q] - F[SL_BADD + 0][q];
-        pt[1] = F[SL_B + 0][q + di] + F[SL_BADD + 0][q + di];
-        pt[2] = -F[SL_B + 1][q] - F[SL_BADD + 1][q];
-        pt[3] = F[SL_B + 1][q + dj] + F[SL_BADD + 1][q + dj];
-        pt[4] = -F[SL_B + 2][q] - F[SL_BADD + 2][q];
-        pt[5] = F[SL_B + 2][q + 1] + F[SL_BADD + 2][q + 1];
-        pi[0] = -F[SL_B + 0][q];
-        pi[1] = F[SL_B + 0][q + di];
-        pi[2] = -F[SL_B + 1][q];
-        pi[3] = F[SL_B + 1][q + dj];
-        pi[4] = -F[SL_B + 2][q];
-        pi[5] = F[SL_B + 2][q + 1];
-        double bt[3] = {0, 0, 0}, bi[3] = {0, 0, 0};
+        T pt[6], pi[6];
+        pt[0] = -(T)F[SL_B + 0][q] - (T)F[SL_BADD + 0][q];
+        pt[1] = (T)F[SL_B + 0][q + di] + (T)F[SL_BADD + 0][q + di];
+        pt[2] = -(T)F[SL_B + 1][q] - (T)F[SL_BADD + 1][q];
+        pt[3] = (T)F[SL_B + 1][q + dj] + (T)F[SL_BADD + 1][q + dj];
+        pt[4] = -(T)F[SL_B + 2][q] - (T)F[SL_BADD + 2][q];
+        pt[5] = (T)F[SL_B + 2][q + 1] + (T)F[SL_BADD + 2][q + 1];
+        pi[0] = -(T)F[SL_B + 0][q];
+        pi[1] = (T)F[SL_B + 0][q + di];
+        pi[2] = -(T)F[SL_B + 1][q];
+        pi[3] = (T)F[SL_B + 1][q + dj];
+        pi[4] = -(T)F[SL_B + 2][q];
+        pi[5] = (T)F[SL_B + 2][q + 1];
+        T bt[3] = {T(0), T(0), T(0)}, bi[3] = {T(0), T(0), T(0)};
 #pragma unroll
         for (int n = 0; n < 6; ++n)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const double w = F[SL_WB + 6 * c + n][q];
+                const T w = st<MIX>(F, G, SL_WB + 6 * c + n, q);
                 bt[c] += w * pt[n];
                 bi[c] += w * pi[n];
             }
         for (int c = 0; c < 3; ++c) {
-            F[SL_BC + c][q] = bt[c];
-            F[SL_BI + c][q] = bi[c];
+            F[SL_BC + c][q] = (double)bt[c];
+            F[SL_BI + c][q] = (double)bi[c];
         }
     }
-#define BB(a, o) (F[SL_BETA + (a)][q + (o)] * F[SL_B + (a)][q + (o)])
+#define BB(a, o) (st<MIX>(F, G, SL_BETA + (a), q + (o)) * (T)F[SL_B + (a)][q + (o)])
     if (p.i < bk.m[0]) {
-        const double v = (BB(1, -dk) - BB(1, 0)) + (BB(2, 0) - BB(2, -dj));
-        F[SL_J + 0][q] = (1.0 * v) * F[SL_ALPHA + 0][q];
+        const T v = (BB(1, -dk) - BB(1, 0)) + (BB(2, 0) - BB(2, -dj));
+        F[SL_J + 0][q] = (double)((T(1) * v) * st<MIX>(F, G, SL_ALPHA + 0, q));
     }
     if (p.j < bk.m[1]) {
-        const double v = (BB(0, 0) - BB(0, -dk)) + (BB(2, -di) - BB(2, 0));
-        F[SL_J + 1][q] = (1.0 * v) * F[SL_ALPHA + 1][q];
+        const T v = (BB(0, 0) - BB(0, -dk)) + (BB(2, -di) - BB(2, 0));
+        F[SL_J + 1][q] = (double)((T(1) * v) * st<MIX>(F, G, SL_ALPHA + 1, q));
     }
     if (p.k < bk.m[2]) {
-        const double v = (BB(0, -dj) - BB(0, 0)) + (BB(1, 0) - BB(1, -di));
-        F[SL_J + 2][q] = (1.0 * v) * F[SL_ALPHA + 2][q];
+        const T v = (BB(0, -dj) - BB(0, 0)) + (BB(1, 0) - BB(1, -di));
+        F[SL_J + 2][q] = (double)((T(1) * v) * st<MIX>(F, G, SL_ALPHA + 2, q));
     }
 #undef BB
+}
+
+// calc_Jcell (MercurySolver_Current.cpp:203-259), owned cells
+template <bool MIX>
+__global__ void k_jcell_t(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
+                          const float *const *__restrict__ G) {
+    using T = typename Prec<MIX>::T;
+    LOCATE(R);
+    const long long q = p.q, di = OI(p), dj = OJ(p);
+    const T s[12] = {(T)F[SL_J + 0][q], (T)F[SL_J + 0][q + dj], (T)F[SL_J + 0][q + 1], (T)F[SL_J + 0][q + dj + 1],
+                     (T)F[SL_J + 1][q], (T)F[SL_J + 1][q + di], (T)F[SL_J + 1][q + 1], (T)F[SL_J + 1][q + di + 1],
+                     (T)F[SL_J + 2][q], (T)F[SL_J + 2][q + di], (T)F[SL_J + 2][q + dj], (T)F[SL_J + 2][q + di + dj]};
+    T J[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int n = 0; n < 12; ++n)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) J[c] += st<MIX>(F, G, SL_WJ + 12 * c + n, q) * s[n];
+    for (int c = 0; c < 3; ++c) F[SL_JC + c][q] = (double)J[c];
+}
+
+// FP32 copy of a static FP64 slot (mixed form)
+__global__ void k_to_float(const double *__restrict__ src, float *__restrict__ dst, long long n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = (float)src[t];
 }
 
 __global__ void k_dt_commit(unsigned long long *dt, unsigned long long *dt_next) {
@@ -521,18 +650,38 @@ static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1
         if ((Rh).total > 0) hipLaunchKernelGGL(kern, dim3(nblk((Rh).total, 256)), dim3(256), 0, st, (Rh).dev, __VA_ARGS__); \
     } while (0)
 
-void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P) { LAUNCH_R(k_rhs_cell, R, B, F, P); }
-void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P, unsigned long long *dt_next,
-                     int compute_dt) {
-    if (compute_dt) hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, st, dt_next);
-    LAUNCH_R(k_cell_end, R, B, F, P, dt_next, compute_dt);
+void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
+                     bool mix) {
+    if (mix) LAUNCH_R(k_rhs_cell<true>, R, B, F, G, P);
+    else LAUNCH_R(k_rhs_cell<false>, R, B, F, G, P);
 }
-void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F) { LAUNCH_R(k_eface_u, R, B, F); }
-void launch_edge_total(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P, int with_hall) {
-    LAUNCH_R(k_edge_total, R, B, F, P, with_hall);
+void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
+                     unsigned long long *dt_next, int compute_dt, bool mix) {
+    if (compute_dt) hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, st, dt_next);
+    if (mix) LAUNCH_R(k_cell_end<true>, R, B, F, G, P, dt_next, compute_dt);
+    else LAUNCH_R(k_cell_end<false>, R, B, F, G, P, dt_next, compute_dt);
+}
+void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, bool mix) {
+    if (mix) LAUNCH_R(k_eface_u<true>, R, B, F);
+    else LAUNCH_R(k_eface_u<false>, R, B, F);
+}
+void launch_edge_total(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
+                       int with_hall, bool mix) {
+    if (mix) LAUNCH_R(k_edge_total<true>, R, B, F, G, P, with_hall);
+    else LAUNCH_R(k_edge_total<false>, R, B, F, G, P, with_hall);
 }
 void launch_update_all(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const double *dt) { LAUNCH_R(k_update_all, R, B, F, dt); }
-void launch_bcell_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F) { LAUNCH_R(k_bcell_jedge, R, B, F); }
+void launch_bcell_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix) {
+    if (mix) LAUNCH_R(k_bcell_jedge<true>, R, B, F, G);
+    else LAUNCH_R(k_bcell_jedge<false>, R, B, F, G);
+}
+void launch_jcell_t(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix) {
+    if (mix) LAUNCH_R(k_jcell_t<true>, R, B, F, G);
+    else LAUNCH_R(k_jcell_t<false>, R, B, F, G);
+}
+void launch_to_float(hipStream_t st, const double *src, float *dst, long long n) {
+    if (n > 0) hipLaunchKernelGGL(k_to_float, dim3(nblk(n, 256)), dim3(256), 0, st, src, dst, n);
+}
 void launch_dt_commit(hipStream_t st, unsigned long long *dt, unsigned long long *dt_next) {
     hipLaunchKernelGGL(k_dt_commit, dim3(1), dim3(64), 0, st, dt, dt_next);
 }

@@ -204,20 +204,34 @@ static void make_range(const Field &F, int lo_off, int hi_off, RangeOwner &R) {
 // ---------------------------------------------------------------------------
 enum OpType { OP_COPY, OP_FILL, OP_SETV, OP_WALLF, OP_ZERO };
 
+// one operation of one field (or of fields sharing an index list) in one
+// phase of a group's program; host lists only
 struct SyncOp {
     OpType type;
+    int phase = 0;
     std::vector<int> slots;
     std::vector<double> fillv;
     int is_na = 0;
-    std::vector<int> hidx;     // host list (pairs for COPY)
+    std::vector<int> hidx;     // pairs for COPY, (first, deeper, ghosts) for WALLF
     std::vector<double> hval;  // SETV values
-    DBuf<int> idx;
-    DBuf<double> val;
-    long long count() const { return type == OP_COPY ? (long long)hidx.size() / 2 : (type == OP_WALLF ? -1 : (long long)hidx.size()); }
 };
 
 struct GroupProg {
     std::vector<std::unique_ptr<SyncOp>> ops;
+};
+
+// A merged launch: every op of one type in one phase of a set of independent
+// groups. Entries name their slot set: COPY (dst, src, set), FILL (dst, set),
+// SETV (dst, slot) + value, WALLF (slot, is_na, first, deeper, g ghosts).
+struct MergedOp {
+    OpType type;
+    MultiSets S{};
+    DBuf<int> ent;
+    DBuf<double> val;
+    long long n = 0;
+};
+struct MergedProg {
+    std::vector<std::unique_ptr<MergedOp>> ops;
 };
 
 enum GroupId { G_UCELL = 0, G_BFACE, G_EEDGE, G_EHALL, G_JEDGE, G_BCELL, G_JCELL, G_BADD, G_NGROUP };
@@ -503,11 +517,15 @@ public:
     std::vector<std::unique_ptr<BlockGeom>> geom;
     RangeOwner R_cell, R_box, R_node, R_eface;
     GroupProg prog[G_NGROUP];
+    std::map<unsigned, MergedProg> mprog;  // keyed by the bit set of groups
     AliasClasses alias;
     hipStream_t st = nullptr;
     unsigned long long *d_dt = nullptr;       // dt word (double bits)
     unsigned long long *d_dt_next = nullptr;  // next step's dt, reduced at the end of the derived fields
-    int form = 1;                             // 0: one kernel per reference function, 1: fused FP64
+    int form = 1;                             // 0: one kernel per reference function, 1: fused FP64, 2: mixed
+    float *farena = nullptr;                  // FP32 copies of the static arrays (mixed form)
+    float **fptr = nullptr;                   // device table indexed like the FP64 slots (null where absent)
+    std::vector<int> fslots;
     unsigned long long *d_red = nullptr;      // reduction scratch (2 words)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -552,6 +570,25 @@ public:
             w.b = b;
             geom[b]->compute(P, w);
         }
+        // FP32 static copies: reconstruction weights, Hodge stars, edge geometry, taper, eta, photo rate
+        for (int c = 0; c < 18; ++c) fslots.push_back(SL_WB + c);
+        for (int c = 0; c < 36; ++c) fslots.push_back(SL_WJ + c);
+        for (int c = 0; c < 3; ++c) {
+            fslots.push_back(SL_BETA + c);
+            fslots.push_back(SL_ALPHA + c);
+            fslots.push_back(SL_DL + c);
+            fslots.push_back(SL_AREA + c);
+            fslots.push_back(SL_TAPER + c);
+            fslots.push_back(SL_LETA + c);
+        }
+        for (int c = 0; c < 9; ++c) fslots.push_back(SL_DR + c);
+        fslots.push_back(SL_PHOTO);
+        HIPCHK(hipMalloc(&farena, sizeof(float) * F->Vtot * fslots.size()));
+        std::vector<float *> hf(SL_NSLOT, nullptr);
+        for (size_t n = 0; n < fslots.size(); ++n) hf[fslots[n]] = farena + (long long)n * F->Vtot;
+        HIPCHK(hipMalloc(&fptr, sizeof(float *) * SL_NSLOT));
+        HIPCHK(hipMemcpy(fptr, hf.data(), sizeof(float *) * SL_NSLOT, hipMemcpyHostToDevice));
+        fptr_h = hf;
         make_range(*F, 0, 0, R_cell);
         make_range(*F, -g, g + 1, R_box);
         make_range(*F, 0, 1, R_node);
@@ -564,12 +601,21 @@ public:
         if (graph) (void)hipGraphDestroy(graph);
         if (d_dt) (void)hipFree(d_dt);
         if (d_dt_next) (void)hipFree(d_dt_next);
+        if (farena) (void)hipFree(farena);
+        if (fptr) (void)hipFree(fptr);
         if (d_red) (void)hipFree(d_red);
         F.reset();
         if (st) (void)hipStreamDestroy(st);
     }
 
     double *const *Fd() const { return F->dptr; }
+    const float *const *Gd() const { return fptr; }
+    std::vector<float *> fptr_h;
+    bool mix() const { return form == 2; }
+    // refresh the FP32 static copies from the FP64 arrays
+    void refresh_float_statics() {
+        for (int s : fslots) launch_to_float(st, F->hptr[s], fptr_h[s], F->Vtot);
+    }
 
     // Sync programs: for every group the physical handlers (far field before
     // the wall, the order of the sorted physical patch list), the corner
@@ -679,30 +725,96 @@ public:
                 add(D, OP_COPY, sl, std::move(copyD));
                 add(E, OP_COPY, sl, std::move(ownE));
             }
-            for (auto *ph : {&A, &Bc, &C, &D, &E})
+            int phase = 0;
+            for (auto *ph : {&A, &Bc, &C, &D, &E}) {
                 for (auto &o : *ph) {
-                    o->idx.upload(o->hidx);
-                    if (!o->hval.empty()) o->val.upload(o->hval);
+                    o->phase = phase;
                     prog[gid].ops.push_back(std::move(o));
                 }
+                ++phase;
+            }
         }
+        for (int gid = 0; gid < G_NGROUP; ++gid) build_merged(1u << gid);
+        build_merged(MG_EDGE_PAIR);
+        build_merged(MG_STATE_PAIR);
+        build_merged(MG_DERIVED_PAIR);
+        build_merged(MG_PREPARE_PAIR);
     }
 
-    void sync(int gid) {
-        for (auto &o : prog[gid].ops) {
-            switch (o->type) {
-            case OP_COPY: launch_copy(st, o->idx.p, o->idx.n / 2, Fd(), o->slots.data(), (int)o->slots.size()); break;
-            case OP_FILL: launch_fill(st, o->idx.p, o->idx.n, Fd(), o->slots.data(), (int)o->slots.size(), o->fillv.data()); break;
-            case OP_SETV: launch_setv(st, o->idx.p, o->val.p, o->idx.n, F->hptr[o->slots[0]]); break;
-            case OP_WALLF: {
-                const int g = F->blk[0].g;
-                launch_wall_fluid(st, o->idx.p, o->idx.n / (2 + g), g, Fd(), o->slots[0], o->is_na, P);
-                break;
+    // Groups that read and write disjoint slots and that the step syncs back
+    // to back run as one program: phase by phase, one launch per op type.
+    static constexpr unsigned MG_EDGE_PAIR = (1u << G_EHALL) | (1u << G_EEDGE);
+    static constexpr unsigned MG_STATE_PAIR = (1u << G_UCELL) | (1u << G_BFACE);
+    static constexpr unsigned MG_DERIVED_PAIR = (1u << G_BCELL) | (1u << G_JEDGE);
+    static constexpr unsigned MG_PREPARE_PAIR = (1u << G_BADD) | (1u << G_BFACE);
+
+    void build_merged(unsigned gset) {
+        MergedProg &mp = mprog[gset];
+        const int g = F->blk[0].g;
+        for (int phase = 0; phase < 5; ++phase)
+            for (OpType t : {OP_FILL, OP_SETV, OP_WALLF, OP_COPY}) {
+                std::unique_ptr<MergedOp> m(new MergedOp());
+                m->type = t;
+                int nset = 0;
+                std::vector<int> ent;
+                std::vector<double> val;
+                for (int gid = 0; gid < G_NGROUP; ++gid) {
+                    if (!(gset & (1u << gid))) continue;
+                    for (auto &o : prog[gid].ops) {
+                        if (o->phase != phase || o->type != t) continue;
+                        if (t == OP_COPY || t == OP_FILL) {
+                            if (nset >= MERC_MAX_SETS) throw std::runtime_error("merged sync: too many slot sets");
+                            m->S.n[nset] = (int)o->slots.size();
+                            for (size_t c = 0; c < o->slots.size(); ++c) {
+                                m->S.s[nset][c] = o->slots[c];
+                                m->S.v[nset][c] = t == OP_FILL ? o->fillv[c] : 0.0;
+                            }
+                        }
+                        if (t == OP_COPY) {
+                            for (size_t e = 0; e + 1 < o->hidx.size(); e += 2) ent.insert(ent.end(), {o->hidx[e], o->hidx[e + 1], nset});
+                        } else if (t == OP_FILL) {
+                            for (int d : o->hidx) ent.insert(ent.end(), {d, nset});
+                        } else if (t == OP_SETV) {
+                            for (size_t e = 0; e < o->hidx.size(); ++e) {
+                                ent.insert(ent.end(), {o->hidx[e], o->slots[0]});
+                                val.push_back(o->hval[e]);
+                            }
+                        } else {
+                            const size_t w = 2 + g;
+                            for (size_t e = 0; e + w <= o->hidx.size(); e += w) {
+                                ent.push_back(o->slots[0]);
+                                ent.push_back(o->is_na);
+                                ent.insert(ent.end(), o->hidx.begin() + e, o->hidx.begin() + e + w);
+                            }
+                        }
+                        ++nset;
+                    }
+                }
+                if (ent.empty()) continue;
+                const int width = t == OP_COPY ? 3 : (t == OP_WALLF ? 4 + g : 2);
+                m->n = (long long)ent.size() / width;
+                m->ent.upload(ent);
+                if (!val.empty()) m->val.upload(val);
+                mp.ops.push_back(std::move(m));
             }
+    }
+
+    void run_merged(unsigned gset) {
+        auto it = mprog.find(gset);
+        if (it == mprog.end()) throw std::runtime_error("merged sync program missing");
+        const int g = F->blk[0].g;
+        for (auto &m : it->second.ops) {
+            switch (m->type) {
+            case OP_COPY: launch_copy_multi(st, m->ent.p, m->n, Fd(), m->S); break;
+            case OP_FILL: launch_fill_multi(st, m->ent.p, m->n, Fd(), m->S); break;
+            case OP_SETV: launch_setv_multi(st, m->ent.p, m->val.p, m->n, Fd()); break;
+            case OP_WALLF: launch_wall_multi(st, m->ent.p, m->n, g, Fd(), P); break;
             default: break;
             }
         }
     }
+
+    void sync(int gid) { run_merged(1u << gid); }
 
     const MercBlk *Bd() const { return F->dblk; }
     const double *dtp() const { return reinterpret_cast<const double *>(d_dt); }
@@ -710,25 +822,28 @@ public:
     // UpdateDerivedFields_ (MercurySolver_Tools.cpp:303-321), fused passes; the
     // last derived pass of a step also reduces the next step's dt
     void derived_fused(bool final) {
-        launch_bcell_jedge(st, R_node.h, Bd(), Fd());
-        sync(G_BCELL);
+        launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix());
         launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
-        sync(G_JEDGE);
-        launch_jcell(st, R_cell.h, Bd(), Fd());
+        run_merged(MG_DERIVED_PAIR);
+        launch_jcell_t(st, R_cell.h, Bd(), Fd(), Gd(), mix());
         sync(G_JCELL);
-        launch_cell_end(st, R_box.h, Bd(), Fd(), P, d_dt_next, final ? 1 : 0);
+        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, final ? 1 : 0, mix());
     }
 
     // StepOnce without the time step, fused passes
     void stage_fused(bool final) {
-        launch_rhs_cell(st, R_cell.h, Bd(), Fd(), P);
-        launch_eface_u(st, R_eface.h, Bd(), Fd());
+        launch_rhs_cell(st, R_cell.h, Bd(), Fd(), Gd(), P, mix());
+        launch_eface_u(st, R_eface.h, Bd(), Fd(), mix());
         if (alias.n == 0) {
-            launch_edge_total(st, R_node.h, Bd(), Fd(), P, 1);
-            sync(G_EHALL);
+            launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, 1, mix());
+            run_merged(MG_EDGE_PAIR);
+            launch_update_all(st, R_node.h, Bd(), Fd(), dtp());
+            run_merged(MG_STATE_PAIR);
+            derived_fused(final);
+            return;
         } else {
             // periodic seams: alias mean of the non-Hall EMF before the Hall part
-            launch_edge_total(st, R_node.h, Bd(), Fd(), P, 0);
+            launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, 0, mix());
             launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_E);
             launch_hall_edge(st, R_node.h, Bd(), Fd());
             sync(G_EHALL);
@@ -736,8 +851,7 @@ public:
         }
         sync(G_EEDGE);
         launch_update_all(st, R_node.h, Bd(), Fd(), dtp());
-        sync(G_UCELL);
-        sync(G_BFACE);
+        run_merged(MG_STATE_PAIR);
         derived_fused(final);
     }
 
@@ -774,13 +888,12 @@ public:
         induction_emf();
         launch_curl_update(st, R_node.h, Bd(), Fd(), dtp(), 1);
         launch_update_u(st, R_cell.h, Bd(), Fd(), dtp());
-        sync(G_UCELL);
-        sync(G_BFACE);
+        run_merged(MG_STATE_PAIR);
         derived();
     }
 
     void enqueue_step(int integrator) {
-        if (form == 1) {
+        if (form >= 1) {
             launch_dt_commit(st, d_dt, d_dt_next);
             if (integrator == 0) {
                 stage_fused(true);
@@ -789,8 +902,7 @@ public:
                 stage_fused(false);
                 stage_fused(false);
                 launch_rk2(st, R_node.h, Bd(), Fd(), 0);
-                sync(G_UCELL);
-                sync(G_BFACE);
+                run_merged(MG_STATE_PAIR);
                 derived_fused(true);
             }
             return;
@@ -803,8 +915,7 @@ public:
             stage();
             stage();
             launch_rk2(st, R_node.h, Bd(), Fd(), 0);
-            sync(G_UCELL);
-            sync(G_BFACE);
+            run_merged(MG_STATE_PAIR);
             derived();
         }
     }
@@ -884,17 +995,20 @@ public:
 
     // MercurySolver ctor tail + Advance head (MercurySolver.cpp:141-145, MercurySolver_Driver.cpp:6-19)
     void prepare() {
-        sync(G_BADD);
-        sync(G_BFACE);
+        run_merged(MG_PREPARE_PAIR);
         derived();
-        launch_cell_end(st, R_box.h, Bd(), Fd(), P, d_dt_next, 1);
+        refresh_float_statics();
+        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix());
         HIPCHK(hipStreamSynchronize(st));
     }
 
     void set_form(int f) {
-        if (f != 0 && f != 1) throw std::runtime_error("form must be 0 (per-function FP64) or 1 (fused FP64)");
-        if (f == 1 && form != 1) launch_cell_end(st, R_box.h, Bd(), Fd(), P, d_dt_next, 1);
+        if (f < 0 || f > 2) throw std::runtime_error("form must be 0 (per-function FP64), 1 (fused FP64) or 2 (mixed)");
         form = f;
+        if (f >= 1) {
+            refresh_float_statics();
+            launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix());
+        }
         if (gexec) {
             (void)hipGraphExecDestroy(gexec);
             (void)hipGraphDestroy(graph);
@@ -924,12 +1038,13 @@ public:
             launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
         else if (op == "timestep") launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt);
         else if (op == "stage") stage();
-        else if (op == "rhs_cell") launch_rhs_cell(st, R_cell.h, Bd(), Fd(), P);
-        else if (op == "cell_end") launch_cell_end(st, R_box.h, Bd(), Fd(), P, d_dt_next, 1);
-        else if (op == "eface_u") launch_eface_u(st, R_eface.h, Bd(), Fd());
-        else if (op == "edge_total") launch_edge_total(st, R_node.h, Bd(), Fd(), P, alias.n == 0 ? 1 : 0);
+        else if (op == "rhs_cell") launch_rhs_cell(st, R_cell.h, Bd(), Fd(), Gd(), P, mix());
+        else if (op == "cell_end") launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix());
+        else if (op == "eface_u") launch_eface_u(st, R_eface.h, Bd(), Fd(), mix());
+        else if (op == "edge_total") launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, alias.n == 0 ? 1 : 0, mix());
         else if (op == "update_all") launch_update_all(st, R_node.h, Bd(), Fd(), dtp());
-        else if (op == "bcell_jedge") launch_bcell_jedge(st, R_node.h, Bd(), Fd());
+        else if (op == "bcell_jedge") launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix());
+        else if (op == "jcell_t") launch_jcell_t(st, R_cell.h, Bd(), Fd(), Gd(), mix());
         else if (op == "sync_all") {
             for (int gid = 0; gid < G_NGROUP; ++gid) sync(gid);
         }

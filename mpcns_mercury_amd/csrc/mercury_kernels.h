@@ -32,12 +32,12 @@ void launch_dec_d1(hipStream_t st, const RangeH &R, const MercBlk *B, double *co
 void launch_dec_d2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int s_face, int s_cell);
 void launch_divb(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, unsigned long long *out);
 
-void launch_copy(hipStream_t st, const int *pairs, long long n, double *const *F, const int *slots, int ns);
-void launch_fill(hipStream_t st, const int *dst, long long n, double *const *F, const int *slots, int ns, const double *vals);
-void launch_setv(hipStream_t st, const int *dst, const double *val, long long n, double *f);
 void launch_alias_reduce(hipStream_t st, const int *cls_start, const int *member, const int *msign, const int *owner, const int *osign,
                          const int *slot_of, long long ncls, double *const *F, int slot0);
-void launch_wall_fluid(hipStream_t st, const int *ent, long long n, int g, double *const *F, int slot, int is_na, const MercPhys &P);
 void launch_zero_span(hipStream_t st, double *const *F, const int *slots, int ns, long long n);
+void launch_copy_multi(hipStream_t st, const int *ent, long long n, double *const *F, const MultiSets &S);
+void launch_fill_multi(hipStream_t st, const int *ent, long long n, double *const *F, const MultiSets &S);
+void launch_setv_multi(hipStream_t st, const int *ent, const double *val, long long n, double *const *F);
+void launch_wall_multi(hipStream_t st, const int *ent, long long n, int g, double *const *F, const MercPhys &P);
 
 }  // namespace merc

@@ -572,58 +572,37 @@ __global__ void k_rk2(const MercRange R, const MercBlk *__restrict__ B, double *
 // ---------------------------------------------------------------------------
 // synchronisation ops (halo / physical boundary / owner-alias), all index lists
 // ---------------------------------------------------------------------------
-struct SlotSet {
-    int n;
-    int s[MERC_MAX_SYNC_SLOTS];
-};
-
-// dst <- src for every slot; sources are never destinations of the same list
-__global__ void k_copy(const int *__restrict__ pairs, long long npair, double *const *__restrict__ F, const SlotSet S) {
-    const long long t = TID;
-    if (t >= npair) return;
-    const int d = pairs[2 * t], s = pairs[2 * t + 1];
-    for (int c = 0; c < S.n; ++c) F[S.s[c]][d] = F[S.s[c]][s];
-}
-
-// dst <- val[c] (constant per slot)
-struct ValSet {
-    double v[MERC_MAX_SYNC_SLOTS];
-};
-__global__ void k_fill(const int *__restrict__ dst, long long n, double *const *__restrict__ F, const SlotSet S, const ValSet V) {
+// Every entry names its slot set, so one launch serves the lists of several
+// fields (and of independent sync groups). Within one launch no destination is
+// the source of another entry.
+// COPY: entry (dst, src, set), dst <- src for every slot of the set
+__global__ void k_copy_multi(const int *__restrict__ ent, long long n, double *const *__restrict__ F, const MultiSets S) {
     const long long t = TID;
     if (t >= n) return;
-    const int d = dst[t];
-    for (int c = 0; c < S.n; ++c) F[S.s[c]][d] = V.v[c];
+    const int d = ent[3 * t], s = ent[3 * t + 1], k = ent[3 * t + 2];
+    for (int c = 0; c < S.n[k]; ++c) F[S.s[k][c]][d] = F[S.s[k][c]][s];
 }
-
-// dst <- per-entry value (single slot)
-__global__ void k_setv(const int *__restrict__ dst, const double *__restrict__ val, long long n, double *__restrict__ f) {
+// FILL: entry (dst, set), dst <- the set's constant per slot
+__global__ void k_fill_multi(const int *__restrict__ ent, long long n, double *const *__restrict__ F, const MultiSets S) {
     const long long t = TID;
     if (t >= n) return;
-    f[dst[t]] = val[t];
+    const int d = ent[2 * t], k = ent[2 * t + 1];
+    for (int c = 0; c < S.n[k]; ++c) F[S.s[k][c]][d] = S.v[k][c];
 }
-
-// owner <- signed mean of an equivalence class (ReduceEdgeAliasCandidatesToOwners_,
-// MercurySolver_Current.cpp:3-57); members summed in the listed order
-__global__ void k_alias_reduce(const int *__restrict__ cls_start, const int *__restrict__ member, const int *__restrict__ msign,
-                               const int *__restrict__ owner, const int *__restrict__ osign, const int *__restrict__ slot_of,
-                               long long ncls, double *const *__restrict__ F, int slot0) {
-    const long long t = TID;
-    if (t >= ncls) return;
-    const int b = cls_start[t], e = cls_start[t + 1];
-    double *f = F[slot0 + slot_of[t]];
-    double sum = 0.0;
-    for (int m = b; m < e; ++m) sum += msign[m] * f[member[m]];
-    f[owner[t]] = osign[t] * sum / (double)(e - b);
-}
-
-// BC_Solid_Surface_ (Boundary_Condition_Fluid.cpp:36-260) for one species: per
-// wall cell the entry holds (first layer, one deeper, g ghost indices)
-__global__ void k_wall_fluid(const int *__restrict__ ent, long long n, int g, double *const *__restrict__ F, int slot, int is_na,
-                             const MercPhys P) {
+// SETV: entry (dst, slot) with its own value
+__global__ void k_setv_multi(const int *__restrict__ ent, const double *__restrict__ val, long long n, double *const *__restrict__ F) {
     const long long t = TID;
     if (t >= n) return;
-    const int *e = ent + (long long)t * (2 + g);
+    F[ent[2 * t + 1]][ent[2 * t]] = val[t];
+}
+// BC_Solid_Surface_ (Boundary_Condition_Fluid.cpp:36-260): per wall cell the
+// entry holds (species base slot, is_na, first layer, one deeper, g ghosts)
+__global__ void k_wall_multi(const int *__restrict__ ent, long long n, int g, double *const *__restrict__ F, const MercPhys P) {
+    const long long t = TID;
+    if (t >= n) return;
+    const int *e = ent + (long long)t * (4 + g);
+    const int slot = e[0], is_na = e[1];
+    e += 2;
     const double gm1 = P.gamma - 1.0;
     const double rho_floor = is_na ? P.wall_rho_floor_Na : P.wall_rho_floor_H;
     const double p_floor = P.wall_p_floor;
@@ -656,6 +635,20 @@ __global__ void k_wall_fluid(const int *__restrict__ ent, long long n, int g, do
         const double ke_g = 0.5 * rho_r * 0.0;
         F[slot + 4][d] = p_r / gm1 + ke_g;
     }
+}
+
+// owner <- signed mean of an equivalence class (ReduceEdgeAliasCandidatesToOwners_,
+// MercurySolver_Current.cpp:3-57); members summed in the listed order
+__global__ void k_alias_reduce(const int *__restrict__ cls_start, const int *__restrict__ member, const int *__restrict__ msign,
+                               const int *__restrict__ owner, const int *__restrict__ osign, const int *__restrict__ slot_of,
+                               long long ncls, double *const *__restrict__ F, int slot0) {
+    const long long t = TID;
+    if (t >= ncls) return;
+    const int b = cls_start[t], e = cls_start[t + 1];
+    double *f = F[slot0 + slot_of[t]];
+    double sum = 0.0;
+    for (int m = b; m < e; ++m) sum += msign[m] * f[member[m]];
+    f[owner[t]] = osign[t] * sum / (double)(e - b);
 }
 
 // ---------------------------------------------------------------------------
@@ -715,6 +708,10 @@ __global__ void k_divb(const MercRange R, const MercBlk *__restrict__ B, double 
     }
 }
 
+struct SlotSet {
+    int n;
+    int s[MERC_MAX_SYNC_SLOTS];
+};
 // zero a set of slots over whole blocks (replaces memset inside a captured step)
 __global__ void k_zero_span(double *const *__restrict__ F, const SlotSet S, long long n) {
     const long long t = TID;
@@ -765,36 +762,22 @@ void launch_dec_d1(hipStream_t st, const RangeH &R, const MercBlk *B, double *co
 void launch_dec_d2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int sf, int sc) { LAUNCH_R(k_dec_d2, R, B, F, sf, sc); }
 void launch_divb(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, unsigned long long *out) { LAUNCH_R(k_divb, R, B, F, out); }
 
-void launch_copy(hipStream_t st, const int *pairs, long long n, double *const *F, const int *slots, int ns) {
-    if (n <= 0) return;
-    SlotSet S;
-    S.n = ns;
-    for (int c = 0; c < ns; ++c) S.s[c] = slots[c];
-    hipLaunchKernelGGL(k_copy, dim3(nblk(n, 256)), dim3(256), 0, st, pairs, n, F, S);
-}
-void launch_fill(hipStream_t st, const int *dst, long long n, double *const *F, const int *slots, int ns, const double *vals) {
-    if (n <= 0) return;
-    SlotSet S;
-    ValSet V;
-    S.n = ns;
-    for (int c = 0; c < ns; ++c) {
-        S.s[c] = slots[c];
-        V.v[c] = vals[c];
-    }
-    hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, st, dst, n, F, S, V);
-}
-void launch_setv(hipStream_t st, const int *dst, const double *val, long long n, double *f) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_setv, dim3(nblk(n, 256)), dim3(256), 0, st, dst, val, n, f);
-}
 void launch_alias_reduce(hipStream_t st, const int *cls_start, const int *member, const int *msign, const int *owner, const int *osign,
                          const int *slot_of, long long ncls, double *const *F, int slot0) {
     if (ncls <= 0) return;
     hipLaunchKernelGGL(k_alias_reduce, dim3(nblk(ncls, 256)), dim3(256), 0, st, cls_start, member, msign, owner, osign, slot_of, ncls, F, slot0);
 }
-void launch_wall_fluid(hipStream_t st, const int *ent, long long n, int g, double *const *F, int slot, int is_na, const MercPhys &P) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_wall_fluid, dim3(nblk(n, 256)), dim3(256), 0, st, ent, n, g, F, slot, is_na, P);
+void launch_copy_multi(hipStream_t st, const int *ent, long long n, double *const *F, const MultiSets &S) {
+    if (n > 0) hipLaunchKernelGGL(k_copy_multi, dim3(nblk(n, 256)), dim3(256), 0, st, ent, n, F, S);
+}
+void launch_fill_multi(hipStream_t st, const int *ent, long long n, double *const *F, const MultiSets &S) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_multi, dim3(nblk(n, 256)), dim3(256), 0, st, ent, n, F, S);
+}
+void launch_setv_multi(hipStream_t st, const int *ent, const double *val, long long n, double *const *F) {
+    if (n > 0) hipLaunchKernelGGL(k_setv_multi, dim3(nblk(n, 256)), dim3(256), 0, st, ent, val, n, F);
+}
+void launch_wall_multi(hipStream_t st, const int *ent, long long n, int g, double *const *F, const MercPhys &P) {
+    if (n > 0) hipLaunchKernelGGL(k_wall_multi, dim3(nblk(n, 256)), dim3(256), 0, st, ent, n, g, F, P);
 }
 void launch_zero_span(hipStream_t st, double *const *F, const int *slots, int ns, long long n) {
     if (n <= 0) return;
