@@ -71,6 +71,7 @@ ALGO_BYTES = {
     "jedge": 8 * (3 + 3 + 3 + 3),
     # J (3) + W_j (36) read, J_cell (3) written
     "jcell": 8 * (3 + 36 + 3),
+    "jcell_t": 8 * (3 + 36 + 3),
     # U_s (10) + B_cell (3) + JD (9) + Jac (1)
     "timestep": 8 * (10 + 3 + 9 + 1),
 }
@@ -156,7 +157,7 @@ def main():
     forms = {}
     op_times = {}
     g = None
-    fused_ops = ["rhs_cell", "eface_u", "edge_total", "update_all", "bcell_jedge", "jcell", "cell_end"]
+    fused_ops = ["rhs_cell", "eface_u", "edge_total", "update_all", "bcell_jedge", "jcell_t", "cell_end"]
     wanted = set(args.forms.split(",")) | {"fp64_fused"}
     for form, label in ((0, "fp64_plain"), (2, "mixed"), (1, "fp64_fused")):
         if label not in wanted:

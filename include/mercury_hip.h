@@ -58,7 +58,11 @@ typedef struct MercuryPhysics {
     double ambipolar_enabled, pressure_coef;
     double qinf[5], qinfs[5], q_pv_inf[5], B_imf[3];
     double wall_rho_floor_H, wall_rho_floor_Na, wall_p_floor;
-    double reserved[8];
+    /* implicit resistive correction: is_Mercury_resistance, eta_max_mercury/(mu0 U_ref L_ref),
+     * mercury_eta_r_inner/outer/width (MercurySolver.cpp:173-184, MercurySolver_Tools.cpp:44-69)
+     * and the CG stopping rule |b - A x| <= max(rtol |b|, atol), at most max_it iterations */
+    double resist_enabled, inver_Rem, resist_r_inner, resist_r_outer, resist_width;
+    double resist_rtol, resist_atol, resist_max_it;
 } MercuryPhysics;
 
 typedef struct MercuryContext MercuryContext;

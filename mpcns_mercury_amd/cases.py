@@ -103,12 +103,27 @@ def _conserved(rho, u, v, w, p, gamma):
 # C1: single-fluid ideal MHD on one periodic curvilinear block
 # ---------------------------------------------------------------------------
 
+def set_resistivity(units: MercuryUnits, inver_Rem: float, r_inner: float, r_outer: float, width: float,
+                    rtol: float = 1e-10) -> MercuryUnits:
+    """Switch the interior resistivity on with the non-dimensional amplitude
+    inver_Rem (eta_max_mercury = inver_Rem * mu0 * U_ref * L_ref)."""
+    c, r = units.constant, units.REF
+    units.is_Mercury_resistance = True
+    units.eta_max_mercury = inver_Rem * c["mu_mag"] * r["U"] * r["L_ref"]
+    units.mercury_eta_r_inner, units.mercury_eta_r_outer, units.mercury_eta_width = r_inner, r_outer, width
+    units.implicit_resistive_rtol = rtol
+    return units
+
+
 def make_periodic_cube(n=(48, 48, 48), g: int = 4, seed: int = 1, amp: float = 1e-3,
-                       integrator: int = 0, steps: int = 100, hall: bool = False) -> Case:
+                       integrator: int = 0, steps: int = 100, hall: bool = False,
+                       resist: float | None = None) -> Case:
     """One block mapped from the unit cube by x += 0.05 sin(2 pi y) sin(2 pi z),
     periodic on every axis; rho = 1, p = 0.1, B = (1,0,0) + curl of a random edge
     potential, seeded velocity perturbations; the Na+ fluid sits at rho_small.
-    Hall is off (taper radii beyond the domain) unless ``hall``."""
+    Hall is off (taper radii beyond the domain) unless ``hall``.  ``resist``
+    switches the interior resistivity on with that inver_Rem on the window
+    r in [0.5, 0.9] (width 0.05), which crosses the periodic seams."""
     mx, my, mz = n
     i, j, k = np.meshgrid(np.arange(mx + 1), np.arange(my + 1), np.arange(mz + 1), indexing="ij")
     y = j / my
@@ -122,6 +137,8 @@ def make_periodic_cube(n=(48, 48, 48), g: int = 4, seed: int = 1, amp: float = 1
         units.hall_coef_override = 0.02
     else:
         units.r_min, units.r_max = 100.0, 100.0
+    if resist is not None:
+        set_resistivity(units, resist, 0.5, 0.9, 0.05)
     P = derive(units)
     gamma = P.gamma
     I, J, K = _box_index(n, g)
@@ -136,9 +153,9 @@ def make_periodic_cube(n=(48, 48, 48), g: int = 4, seed: int = 1, amp: float = 1
     A = np.stack([amp * h * uniform(seed, 10 + a, key) for a in range(3)])
     fields = [{"U_H": UH, "U_Na": UNa, "Photo_rate": np.zeros((1,) + shape)}]
     blk = Block(dims=(mx, my, mz), bc=(BC_PERIODIC,) * 6, nodes=nodes)
-    return Case(name=f"periodic_cube_{mx}x{my}x{mz}", g=g, blocks=[blk], phys=P, fields=fields, A=[A],
-                Bu=(1.0, 0.0, 0.0), steps=steps, integrator=integrator,
-                meta={"seed": seed, "species": 2, "hall": hall})
+    return Case(name=f"periodic_cube_{mx}x{my}x{mz}" + ("_resist" if resist is not None else ""), g=g, blocks=[blk],
+                phys=P, fields=fields, A=[A], Bu=(1.0, 0.0, 0.0), steps=steps, integrator=integrator,
+                meta={"seed": seed, "species": 2, "hall": hall, "resist": resist})
 
 
 # ---------------------------------------------------------------------------
@@ -166,11 +183,14 @@ def sector_nodes(dims, panel, r0=1.0, r1=4.0, half_angle=np.deg2rad(40.0)):
 
 
 def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, integrator: int = 0,
-               scale: float = 1.0, hall_coef: float | None = None) -> Case:
+               scale: float = 1.0, hall_coef: float | None = None, resist: float | None = None) -> Case:
     """Two-fluid Hall MHD: H+ and Na+ (mass ratio 1:23, density ratio 1:0.01) plus
     electron pressure (ambipolar term), inflow u = (-1,0,0), IMF B = (0,0,-0.2),
     the fluid side of a perfectly conducting sphere at r = 1 on every block's
     xi- face and far field on the other five faces; seeded 1e-3 density noise.
+    ``resist`` switches the interior resistivity on with that inver_Rem on the
+    reference's window (r_inner 0.84, r_outer 1.04, width 0.02,
+    Z4_Mercury/9999setup/3_MHD_Control.txt:12-22).
 
     The listed block sizes admit no conformal 1:1 tiling of the shell, so every
     block is a separate gnomonic sector (no inter-block interfaces)."""
@@ -184,6 +204,8 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
     units.is_Ambipolar_Efield = True
     units.is_local_artificial_resistivity = True
     units.hall_coef_override = hall_coef
+    if resist is not None:
+        set_resistivity(units, resist, 0.84, 1.04, 0.02)
     P = derive(units)
     gamma = P.gamma
     blks, fields = [], []
@@ -209,9 +231,10 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
         r = np.sqrt((cc * cc).sum(axis=0))
         photo = np.where(r > 1.0, 1e-2 * np.exp(-(r - 1.0) / 0.2), 0.0)
         fields.append({"U_H": UH, "U_Na": UNa, "Photo_rate": photo[None]})
-    return Case(name="shell6" if scale == 1.0 else f"shell6_x{scale:g}", g=g, blocks=blks, phys=P, fields=fields,
+    name = "shell6" if scale == 1.0 else f"shell6_x{scale:g}"
+    return Case(name=name + ("_resist" if resist is not None else ""), g=g, blocks=blks, phys=P, fields=fields,
                 A=None, Bu=(0.0, 0.0, 0.0), steps=steps, integrator=integrator,
-                meta={"seed": seed, "species": 2, "hall": True})
+                meta={"seed": seed, "species": 2, "hall": True, "resist": resist})
 
 
 def make_uniform_block(nodes: np.ndarray, bc, g: int = 4, rho: float = 1.0, u=(0.0, 0.0, 0.0), p: float = 0.1,

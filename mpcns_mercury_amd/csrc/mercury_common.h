@@ -60,7 +60,9 @@ struct MercPhys {
     double ambipolar_enabled, pressure_coef;
     double qinf[5], qinfs[5], q_pv_inf[5], B_imf[3];
     double wall_rho_floor_H, wall_rho_floor_Na, wall_p_floor;
-    double reserved[8];
+    // implicit resistive correction (MercurySolver.cpp:173-184, MercurySolver_Tools.cpp:44-69)
+    double resist_enabled, inver_Rem, resist_r_inner, resist_r_outer, resist_width;
+    double resist_rtol, resist_atol, resist_max_it;
 };
 
 // per-block geometry of the box

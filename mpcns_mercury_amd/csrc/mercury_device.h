@@ -46,12 +46,12 @@ DEVI bool locate(const MercRange R, const MercBlk *__restrict__ B, long long t, 
 // the 8 XCDs, each with its own L2; remapping gives every XCD one contiguous
 // slab of the index space, so the i/j stencil neighbours of a workgroup are
 // mostly resident in the same L2.
-DEVI long long xcd_tid() {
+DEVI unsigned xcd_block() {
     const unsigned nwg = gridDim.x, wg = blockIdx.x;
     const unsigned q = nwg >> 3, r = nwg & 7u, x = wg & 7u, s = wg >> 3;
-    const unsigned logical = x * q + (x < r ? x : r) + s;
-    return (long long)logical * blockDim.x + threadIdx.x;
+    return x * q + (x < r ? x : r) + s;
 }
+DEVI long long xcd_tid() { return (long long)xcd_block() * blockDim.x + threadIdx.x; }
 #define TID xcd_tid()
 #define LOCATE(R)            \
     Pt p;                    \

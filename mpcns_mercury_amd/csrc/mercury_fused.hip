@@ -109,13 +109,20 @@ __global__ void k_rhs_cell(const MercRange R, const MercBlk *__restrict__ B, dou
     for (int s = 0; s < 2; ++s) {
         const int su = s ? SL_UN : SL_UH, sp = s ? SL_PVN : SL_PVH;
         Acc<MIX> r[5];
-        for (int a = 0; a < 3; ++a) {
+        auto axis = [&](int a) {
             const long long d = AX(p, a);
             T Fm[5], Fp[5];
             rusanov_face_t<MIX>(F, su, sp, q - d, q, q, a, gm, Fm);
             rusanov_face_t<MIX>(F, su, sp, q, q + d, q + d, a, gm, Fp);
 #pragma unroll
             for (int m = 0; m < 5; ++m) r[m].sub((Fp[m] - Fm[m]) * invJ);
+        };
+        if constexpr (MIX) {
+            // the mixed body keeps one axis live at a time (unrolled it spills)
+#pragma unroll 1
+            for (int a = 0; a < 3; ++a) axis(a);
+        } else {
+            for (int a = 0; a < 3; ++a) axis(a);
         }
         for (int m = 0; m < 5; ++m) rr[s][m] = r[m].value();
     }
@@ -227,6 +234,8 @@ __global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, dou
             for (int c = 0; c < 5; ++c) F[sp + c][q] = pv[s][c];
         }
         const double rH = dmax(F[SL_UH][q], 0.0), rN = dmax(F[SL_UN][q], 0.0);
+        // one limiter serves U+ (floored densities) and the Hall terms (raw
+        // densities, Hall.cpp:92-95): it floors its inputs at 0 itself
         const Num n = num_limiter(P, rH, rN);
         double up[3];
         for (int c = 0; c < 3; ++c) {
@@ -235,8 +244,7 @@ __global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, dou
             up[c] = n.chiH * uH + n.chiNa * uN;
             F[SL_UP + c][q] = up[c];
         }
-        // Hall cell terms (the limiter of the raw densities, Hall.cpp:92-95)
-        const Num nh = num_limiter(P, F[SL_UH][q], F[SL_UN][q]);
+        const Num &nh = n;
         const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
         {
             const T alpha = (T)(P.hall_coef / nh.ne_eff);
@@ -263,39 +271,48 @@ __global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, dou
         if (compute_dt && p.i >= 0 && p.j >= 0 && p.k >= 0 && p.i < bk.m[0] && p.j < bk.m[1] && p.k < bk.m[2]) {
             const double V = fabs(F[SL_JAC][q]);
             if (!(V <= 0.0)) {
+                // per species state first, then one pass over the six faces:
+                // the face normal and |S| (the stored area, bitwise the same
+                // sqrt) are shared by the species
                 const double B2 = Bx * Bx + By * By + Bz * Bz;
+                double rho[2], ux[2], uy[2], uz[2], cs2[2], vA2[2], denom[2] = {0.0, 0.0};
                 for (int s = 0; s < 2; ++s) {
                     const int su = s ? SL_UN : SL_UH;
-                    const double rho = dmax(F[su][q], 1e-2);
-                    const double ux = F[su + 1][q] / rho, uy = F[su + 2][q] / rho, uz = F[su + 3][q] / rho;
+                    rho[s] = dmax(F[su][q], 1e-2);
+                    ux[s] = F[su + 1][q] / rho[s];
+                    uy[s] = F[su + 2][q] / rho[s];
+                    uz[s] = F[su + 3][q] / rho[s];
                     const double E = F[su + 4][q];
-                    const double ke = 0.5 * rho * (ux * ux + uy * uy + uz * uz);
+                    const double ke = 0.5 * rho[s] * (ux[s] * ux[s] + uy[s] * uy[s] + uz[s] * uz[s]);
                     const double pr = (P.gamma - 1.0) * (E - ke);
-                    const double cs2 = P.gamma * pr / rho;
-                    double denom = 0.0;
-                    for (int a = 0; a < 3; ++a)
-                        for (int sd = 0; sd < 2; ++sd) {
-                            const long long fq = q + (sd ? AX(p, a) : 0LL);
-                            const double ax = F[SL_JD + 3 * a][fq], ay = F[SL_JD + 3 * a + 1][fq], az = F[SL_JD + 3 * a + 2][fq];
-                            const double A = sqrt(ax * ax + ay * ay + az * az);
-                            double term = 0.0;
-                            if (!(A <= 0.0)) {
-                                double nx = ax / A, ny = ay / A, nz = az / A;
-                                if (!sd) { nx = -nx; ny = -ny; nz = -nz; }
-                                const double un = ux * nx + uy * ny + uz * nz;
-                                const double Bn = Bx * nx + By * ny + Bz * nz;
-                                const double vA2 = P.inver_MA2 * B2 / rho;
-                                const double vAn2 = P.inver_MA2 * (Bn * Bn) / rho;
-                                const double t = cs2 + vA2;
-                                double disc = t * t - 4.0 * cs2 * vAn2;
-                                if (disc < 0.0) disc = 0.0;
-                                const double cf = sqrt(0.5 * (t + sqrt(disc)));
-                                term = (fabs(un) + cf) * A;
-                            }
-                            denom += term;
-                        }
-                    if (denom > 0.0) best = dmin(best, P.cfl * V / denom);
+                    cs2[s] = P.gamma * pr / rho[s];
+                    vA2[s] = P.inver_MA2 * B2 / rho[s];
                 }
+                for (int a = 0; a < 3; ++a)
+                    for (int sd = 0; sd < 2; ++sd) {
+                        const long long fq = q + (sd ? AX(p, a) : 0LL);
+                        const double ax = F[SL_JD + 3 * a][fq], ay = F[SL_JD + 3 * a + 1][fq], az = F[SL_JD + 3 * a + 2][fq];
+                        const double A = F[SL_AREA + a][fq];
+                        if (A <= 0.0) {
+                            denom[0] += 0.0;
+                            denom[1] += 0.0;
+                            continue;
+                        }
+                        double nx = ax / A, ny = ay / A, nz = az / A;
+                        if (!sd) { nx = -nx; ny = -ny; nz = -nz; }
+                        const double Bn = Bx * nx + By * ny + Bz * nz;
+                        for (int s = 0; s < 2; ++s) {
+                            const double un = ux[s] * nx + uy[s] * ny + uz[s] * nz;
+                            const double vAn2 = P.inver_MA2 * (Bn * Bn) / rho[s];
+                            const double t = cs2[s] + vA2[s];
+                            double disc = t * t - 4.0 * cs2[s] * vAn2;
+                            if (disc < 0.0) disc = 0.0;
+                            const double cf = sqrt(0.5 * (t + sqrt(disc)));
+                            denom[s] += (fabs(un) + cf) * A;
+                        }
+                    }
+                for (int s = 0; s < 2; ++s)
+                    if (denom[s] > 0.0) best = dmin(best, P.cfl * V / denom[s]);
             }
         }
     }

@@ -46,7 +46,10 @@ class MercuryPhysics(ctypes.Structure):
         ("q_pv_inf", ctypes.c_double * 5), ("B_imf", ctypes.c_double * 3),
         ("wall_rho_floor_H", ctypes.c_double), ("wall_rho_floor_Na", ctypes.c_double),
         ("wall_p_floor", ctypes.c_double),
-        ("reserved", ctypes.c_double * 8),
+        # implicit resistive correction (MercurySolver.cpp:173-184, MercurySolver_Tools.cpp:44)
+        ("resist_enabled", ctypes.c_double), ("inver_Rem", ctypes.c_double),
+        ("resist_r_inner", ctypes.c_double), ("resist_r_outer", ctypes.c_double), ("resist_width", ctypes.c_double),
+        ("resist_rtol", ctypes.c_double), ("resist_atol", ctypes.c_double), ("resist_max_it", ctypes.c_double),
     ]
 
 
@@ -82,6 +85,16 @@ class MercuryUnits:
     local_arti_r_cutoff: float = 0.05
     is_Ambipolar_Efield: bool = False
     hall_coef_override: float | None = None
+    # interior resistivity (Z4_Mercury/9999setup/3_MHD_Control.txt:12-22; eta_max_mercury
+    # in Omega m).  The iteration tolerances are those of the build's CG solve.
+    is_Mercury_resistance: bool = False
+    eta_max_mercury: float = 0.0
+    mercury_eta_r_inner: float = 0.84
+    mercury_eta_r_outer: float = 1.04
+    mercury_eta_width: float = 0.02
+    implicit_resistive_rtol: float = 1.0e-10
+    implicit_resistive_atol: float = 1.0e-30
+    implicit_resistive_max_it: int = 500
 
     def alfven_units(self) -> "MercuryUnits":
         """Choose B_ref so that inver_MA2 = c_hall = 1 (B in Alfven units)."""
@@ -150,4 +163,10 @@ def derive(u: MercuryUnits) -> MercuryPhysics:
             P.qinfs[m] = v
     # wall floors of BC_Solid_Surface_ (Boundary_Condition_Fluid.cpp:58-60)
     P.wall_rho_floor_H, P.wall_rho_floor_Na, P.wall_p_floor = 2.5e-3, 1.0e-12, 1.0e-12
+    # interior resistivity: inver_Rem = eta_max_mercury / (mu0 U_ref L_ref) (MercurySolver_Tools.cpp:44)
+    P.resist_enabled = 1.0 if u.is_Mercury_resistance else 0.0
+    P.inver_Rem = u.eta_max_mercury / (mu0 * U_ref * L_ref)
+    P.resist_r_inner, P.resist_r_outer, P.resist_width = u.mercury_eta_r_inner, u.mercury_eta_r_outer, u.mercury_eta_width
+    P.resist_rtol, P.resist_atol = u.implicit_resistive_rtol, u.implicit_resistive_atol
+    P.resist_max_it = float(u.implicit_resistive_max_it)
     return P

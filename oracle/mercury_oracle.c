@@ -49,7 +49,9 @@ typedef struct {
     double ambipolar_enabled, pressure_coef;
     double qinf[5], qinfs[5], q_pv_inf[5], B_imf[3];
     double wall_rho_floor_H, wall_rho_floor_Na, wall_p_floor;
-    double reserved[8];
+    /* implicit resistive correction (MercurySolver.cpp:173-184, MercurySolver_Tools.cpp:44) */
+    double resist_enabled, inver_Rem, resist_r_inner, resist_r_outer, resist_width;
+    double resist_rtol, resist_atol, resist_max_it;
 } OrcPhys;
 
 enum { BC_PERIODIC = 0, BC_FARFIELD = 1, BC_WALL = 2 };
@@ -76,7 +78,11 @@ enum {
     S_ALPHA = S_DL + 3, S_TAPER = S_ALPHA + 3, S_LETA = S_TAPER + 3,
     /* generic DEC forms: node 0-form, edge 1-forms, face 2-forms, cell 3-form */
     S_NODEF = S_LETA + 3, S_EDGEF = S_NODEF + 1, S_FACEF = S_EDGEF + 3, S_CELLS = S_FACEF + 3,
-    S_NSLOT = S_CELLS + 1
+    /* implicit resistive correction: Eres, dJ, eta on the unknown edges, the
+     * solution kept as the next initial guess (edges); dB, RHS_Bres (faces) */
+    S_ERES = S_CELLS + 1, S_DJ = S_ERES + 3, S_ETAR = S_DJ + 3, S_XRES = S_ETAR + 3,
+    S_DB = S_XRES + 3, S_RHSBR = S_DB + 3,
+    S_NSLOT = S_RHSBR + 3
 };
 
 typedef struct {
@@ -620,6 +626,16 @@ static void farfield_eedge(OBlock *b, const OrcPhys *P, const FDesc *fd, int dir
     default_copy(b, fd, dir);
 }
 
+/* the far-field Eres handler with a zero far field: boundary-layer edges <- 0 */
+static void farfield_eedge_zero(OBlock *b, const FDesc *fd, int dir) {
+    int ilo[3], ihi[3];
+    inner_slab(b, fd->loc, dir, ilo, ihi);
+    for (int i = ilo[0]; i < ihi[0]; ++i)
+        for (int j = ilo[1]; j < ihi[1]; ++j)
+            for (int k = ilo[2]; k < ihi[2]; ++k) F(b, fd->slot, i, j, k) = 0.0;
+    default_copy(b, fd, dir);
+}
+
 /* MercuryBoundary::BC_Solid_Surface_ (Boundary_Condition_Fluid.cpp:36-260):
  * non-Na species: first active layer <- (rho, p) one cell deeper, zero velocity;
  * ghosts <- first layer.  Na species: ghosts <- floor state. */
@@ -692,9 +708,10 @@ static void wall_jcell(OBlock *b, const FDesc *fd, int dir) {
     default_copy(b, fd, dir);
 }
 
-enum { G_UCELL, G_BFACE, G_EEDGE, G_EHALL, G_JEDGE, G_BCELL, G_JCELL, G_BADD, G_NGROUP };
-/* field kinds for handler dispatch */
-enum { K_UH, K_UNA, K_BFACE, K_EEDGE, K_EHALL, K_JEDGE, K_BCELL, K_BIND, K_JCELL, K_BADD };
+enum { G_UCELL, G_BFACE, G_EEDGE, G_EHALL, G_JEDGE, G_BCELL, G_JCELL, G_BADD, G_ERES, G_ERES0, G_DB, G_DJ, G_NGROUP };
+/* field kinds for handler dispatch (K_ERES0: Eres with a zero far-field
+ * field, the linear part of the resistive operator) */
+enum { K_UH, K_UNA, K_BFACE, K_EEDGE, K_EHALL, K_JEDGE, K_BCELL, K_BIND, K_JCELL, K_BADD, K_ERES, K_ERES0, K_DB, K_DJ };
 
 typedef struct {
     int kind, slot, ncomp, loc, level; /* level: 1 = Corner1D, 3 = Corner3D */
@@ -713,6 +730,11 @@ static int group_fields(int gid, GField out[6]) {
     case G_JEDGE: for (int a = 0; a < 3; ++a) ADD(K_JEDGE, S_J + a, 1, L_EX + a, 3, 1); break;
     case G_BCELL: ADD(K_BCELL, S_BC, 3, L_CELL, 3, 0); ADD(K_BIND, S_BI, 3, L_CELL, 3, 0); break;
     case G_JCELL: ADD(K_JCELL, S_JC, 3, L_CELL, 3, 0); break;
+    /* Eres / dB / dJ groups (Boundary_Build.cpp:605-613, 674-693) */
+    case G_ERES: for (int a = 0; a < 3; ++a) ADD(K_ERES, S_ERES + a, 1, L_EX + a, 3, 1); break;
+    case G_ERES0: for (int a = 0; a < 3; ++a) ADD(K_ERES0, S_ERES + a, 1, L_EX + a, 3, 1); break;
+    case G_DB: for (int a = 0; a < 3; ++a) ADD(K_DB, S_DB + a, 1, L_FX + a, 3, 1); break;
+    case G_DJ: for (int a = 0; a < 3; ++a) ADD(K_DJ, S_DJ + a, 1, L_EX + a, 3, 1); break;
     }
 #undef ADD
     return n;
@@ -730,7 +752,9 @@ static void apply_handler(OBlock *b, const OrcPhys *P, const GField *gf, int bc,
         case K_BCELL: fill_const(b, &fd, dir, P->B_imf); break;
         case K_BIND: fill_const(b, &fd, dir, zero3); break;
         case K_EEDGE: farfield_eedge(b, P, &fd, dir); break;
-        default: default_copy(b, &fd, dir); break; /* Ehall, J edges, J_cell, Badd */
+        case K_ERES: farfield_eedge(b, P, &fd, dir); break; /* Boundary_Build.cpp:198-200 */
+        case K_ERES0: farfield_eedge_zero(b, &fd, dir); break;
+        default: default_copy(b, &fd, dir); break; /* Ehall, J edges, J_cell, Badd, dB, dJ */
         }
     } else if (bc == BC_WALL) {
         switch (gf->kind) {
@@ -985,38 +1009,43 @@ static void calc_bcell(Orc *o) {
 
 /* CTOperators::CurlAdjFaceToEdge (Z4_Mercury/src/operators/CTOperators.cpp:59-110)
  * then J *= alpha (MercurySolver_Current.cpp:140-201), owned edges */
-static void calc_jedge(Orc *o) {
+/* CTOperators::CurlAdjFaceToEdge (Z4_Mercury/src/operators/CTOperators.cpp:59-110)
+ * of the face 2-form in slots sf.. times alpha, into the edge slots se.. */
+static void adj_curl(Orc *o, int sf, int se) {
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
         int hi[3];
-#define BB(a, i, j, k) (F(b, S_BETA + (a), i, j, k) * F(b, S_B + (a), i, j, k))
+#define BB(a, i, j, k) (F(b, S_BETA + (a), i, j, k) * F(b, sf + (a), i, j, k))
         owned_hi(b, L_EX, hi);
         for (int i = 0; i < hi[0]; ++i)
             for (int j = 0; j < hi[1]; ++j)
                 for (int k = 0; k < hi[2]; ++k) {
                     double v = (BB(1, i, j, k - 1) - BB(1, i, j, k)) + (BB(2, i, j, k) - BB(2, i, j - 1, k));
-                    F(b, S_J + 0, i, j, k) = 1.0 * v;
-                    F(b, S_J + 0, i, j, k) *= F(b, S_ALPHA + 0, i, j, k);
+                    F(b, se + 0, i, j, k) = 1.0 * v;
+                    F(b, se + 0, i, j, k) *= F(b, S_ALPHA + 0, i, j, k);
                 }
         owned_hi(b, L_EY, hi);
         for (int i = 0; i < hi[0]; ++i)
             for (int j = 0; j < hi[1]; ++j)
                 for (int k = 0; k < hi[2]; ++k) {
                     double v = (BB(0, i, j, k) - BB(0, i, j, k - 1)) + (BB(2, i - 1, j, k) - BB(2, i, j, k));
-                    F(b, S_J + 1, i, j, k) = 1.0 * v;
-                    F(b, S_J + 1, i, j, k) *= F(b, S_ALPHA + 1, i, j, k);
+                    F(b, se + 1, i, j, k) = 1.0 * v;
+                    F(b, se + 1, i, j, k) *= F(b, S_ALPHA + 1, i, j, k);
                 }
         owned_hi(b, L_EZ, hi);
         for (int i = 0; i < hi[0]; ++i)
             for (int j = 0; j < hi[1]; ++j)
                 for (int k = 0; k < hi[2]; ++k) {
                     double v = (BB(0, i, j - 1, k) - BB(0, i, j, k)) + (BB(1, i, j, k) - BB(1, i - 1, j, k));
-                    F(b, S_J + 2, i, j, k) = 1.0 * v;
-                    F(b, S_J + 2, i, j, k) *= F(b, S_ALPHA + 2, i, j, k);
+                    F(b, se + 2, i, j, k) = 1.0 * v;
+                    F(b, se + 2, i, j, k) *= F(b, S_ALPHA + 2, i, j, k);
                 }
 #undef BB
     }
 }
+
+/* Calc_J_Edge (MercurySolver_Current.cpp:140-201): J = alpha * CurlAdjFaceToEdge(beta * B) */
+static void calc_jedge(Orc *o) { adj_curl(o, S_B, S_J); }
 
 /* calc_Jcell (MercurySolver_Current.cpp:203-259), owned cells */
 static void calc_jcell(Orc *o) {
@@ -1637,16 +1666,274 @@ static void save_state(Orc *o) {
     }
 }
 
+/* ------------------------------------------------------------------------- */
+/* implicit resistive correction (Z4_Mercury/src/implicit/Mercury_ImplicitResistive.cpp) */
+/* ------------------------------------------------------------------------- */
+
+/* CTOperators::CurlEdgeToFace (Z4_Mercury/src/operators/CTOperators.cpp:6-57):
+ * face = mult * circulation of the edge 1-form, owned faces */
+static void curl_e2f(Orc *o, int se, int sf, double mult) {
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        int hi[3];
+#define EE(a, i, j, k) F(b, se + (a), i, j, k)
+        owned_hi(b, L_FX, hi);
+        for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+                for (int k = 0; k < hi[2]; ++k) F(b, sf, i, j, k) = mult * ((EE(1, i, j, k) - EE(1, i, j, k + 1)) + (EE(2, i, j + 1, k) - EE(2, i, j, k)));
+        owned_hi(b, L_FY, hi);
+        for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+                for (int k = 0; k < hi[2]; ++k) F(b, sf + 1, i, j, k) = mult * ((EE(0, i, j, k + 1) - EE(0, i, j, k)) + (EE(2, i, j, k) - EE(2, i + 1, j, k)));
+        owned_hi(b, L_FZ, hi);
+        for (int i = 0; i < hi[0]; ++i)
+            for (int j = 0; j < hi[1]; ++j)
+                for (int k = 0; k < hi[2]; ++k) F(b, sf + 2, i, j, k) = mult * ((EE(0, i, j, k) - EE(0, i, j + 1, k)) + (EE(1, i + 1, j, k) - EE(1, i, j, k)));
+#undef EE
+    }
+}
+
+/* MercuryResistivityShape_ (Z4_Mercury/src/physics/MercurySolver_Tools.cpp:50-69) */
+static double resist_shape(const OrcPhys *P, double r) {
+    const double shape = 0.5 * (tanh((r - P->resist_r_inner) / P->resist_width) - tanh((r - P->resist_r_outer) / P->resist_width));
+    if (!isfinite(shape) || shape <= 1.0e-10) return 0.0;
+    return dmin(1.0, shape);
+}
+
+/* Unknown edges and their eta (BuildImplicitResistiveEdgeDofMap_ :165-217,
+ * ImplicitResistiveEtaAtEdge_ :219-252): owned edges that own their periodic
+ * class, with dl > 1e-12 and eta = inver_Rem * shape(|edge midpoint|) > 0.
+ * Two kinds are left out of the unknowns because the exact solution does not
+ * depend on them: edges with alpha = 0 (their operator row is the identity and
+ * their right-hand side eta*alpha*(...) is 0, so they solve to 0) and edges of
+ * a far-field boundary layer (the Eres far-field handler overwrites them before
+ * they reach any face).  Writes eta into S_ETAR (0 off the unknowns). */
+static void resist_dofs(Orc *o) {
+    const OrcPhys *P = &o->P;
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        for (int a = 0; a < 3; ++a) {
+            const int loc = L_EX + a;
+            memset(b->f[S_ETAR + a], 0, sizeof(double) * b->V);
+            int hi[3];
+            owned_hi(b, loc, hi);
+            for (int i = 0; i < hi[0]; ++i)
+                for (int j = 0; j < hi[1]; ++j)
+                    for (int k = 0; k < hi[2]; ++k) {
+                        int idx[3] = {i, j, k}, skip = 0;
+                        for (int c = 0; c < 3; ++c)
+                            if (DELTA[loc][c] == 0 && b->bc[2 * c] == BC_PERIODIC && idx[c] == b->m[c]) skip = 1;
+                        if (skip) continue;
+                        if (fabs(F(b, S_DL + a, i, j, k)) <= 1.0e-12) continue;
+                        const int ir = i + (a == 0), jr = j + (a == 1), kr = k + (a == 2);
+                        const double xm = 0.5 * (XN(b, i, j, k) + XN(b, ir, jr, kr));
+                        const double ym = 0.5 * (YN(b, i, j, k) + YN(b, ir, jr, kr));
+                        const double zm = 0.5 * (ZN(b, i, j, k) + ZN(b, ir, jr, kr));
+                        const double eta = P->inver_Rem * resist_shape(P, sqrt(xm * xm + ym * ym + zm * zm));
+                        if (!(eta > 0.0) || F(b, S_ALPHA + a, i, j, k) == 0.0) continue;
+                        F(b, S_ETAR + a, i, j, k) = eta;
+                    }
+            /* far-field boundary layers */
+            for (int f = 0; f < 6; ++f) {
+                if (b->bc[f] != BC_FARFIELD) continue;
+                int dir = (f / 2 + 1) * ((f & 1) ? 1 : -1), lo[3], sh[3];
+                inner_slab(b, loc, dir, lo, sh);
+                for (int i = lo[0]; i < sh[0]; ++i)
+                    for (int j = lo[1]; j < sh[1]; ++j)
+                        for (int k = lo[2]; k < sh[2]; ++k) F(b, S_ETAR + a, i, j, k) = 0.0;
+            }
+        }
+    }
+}
+
+typedef struct {
+    int n;
+    int *blk, *ax, *i, *j, *k;
+} DofList;
+
+static void dof_list(Orc *o, DofList *d) {
+    d->n = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        int n = 0;
+        for (int ib = 0; ib < o->nb; ++ib) {
+            OBlock *b = &o->b[ib];
+            for (int a = 0; a < 3; ++a) {
+                int hi[3];
+                owned_hi(b, L_EX + a, hi);
+                for (int i = 0; i < hi[0]; ++i)
+                    for (int j = 0; j < hi[1]; ++j)
+                        for (int k = 0; k < hi[2]; ++k) {
+                            if (!(F(b, S_ETAR + a, i, j, k) > 0.0)) continue;
+                            if (pass) d->blk[n] = ib, d->ax[n] = a, d->i[n] = i, d->j[n] = j, d->k[n] = k;
+                            ++n;
+                        }
+            }
+        }
+        if (!pass) {
+            d->n = n;
+            d->blk = (int *)malloc(sizeof(int) * (n + 1));
+            d->ax = (int *)malloc(sizeof(int) * (n + 1));
+            d->i = (int *)malloc(sizeof(int) * (n + 1));
+            d->j = (int *)malloc(sizeof(int) * (n + 1));
+            d->k = (int *)malloc(sizeof(int) * (n + 1));
+        }
+    }
+}
+static void dof_free(DofList *d) { free(d->blk); free(d->ax); free(d->i); free(d->j); free(d->k); }
+#define DOF(o, d, s, n) F(&(o)->b[(d)->blk[n]], (s) + (d)->ax[n], (d)->i[n], (d)->j[n], (d)->k[n])
+
+/* MatMultImplicitResistive_ (Mercury_ImplicitResistive.cpp:406-465):
+ * Eres <- x on the unknowns (0 elsewhere), Sync(Eres), dB = CurlEdgeToFace(Eres)
+ * * (-dt), Sync(dB), dJ = alpha * CurlAdjFaceToEdge(beta * dB), alias mean,
+ * Sync(dJ), y = Eres - eta * dJ on the unknowns.  lin = 1 applies the far-field
+ * Eres handler with a zero field: the linear part of the (affine) operator. */
+static void res_apply(Orc *o, const DofList *d, const double *x, double *y, double dt, int lin) {
+    for (int ib = 0; ib < o->nb; ++ib)
+        for (int a = 0; a < 3; ++a) {
+            memset(o->b[ib].f[S_ERES + a], 0, sizeof(double) * o->b[ib].V);
+            memset(o->b[ib].f[S_DB + a], 0, sizeof(double) * o->b[ib].V);
+            memset(o->b[ib].f[S_DJ + a], 0, sizeof(double) * o->b[ib].V);
+        }
+    for (int n = 0; n < d->n; ++n) DOF(o, d, S_ERES, n) = x[n];
+    sync_group(o, lin ? G_ERES0 : G_ERES);
+    curl_e2f(o, S_ERES, S_DB, -dt);
+    sync_group(o, G_DB);
+    adj_curl(o, S_DB, S_DJ);
+    alias_reduce_edges(o, S_DJ);
+    sync_group(o, G_DJ);
+    for (int n = 0; n < d->n; ++n) y[n] = DOF(o, d, S_ERES, n) - DOF(o, d, S_ETAR, n) * DOF(o, d, S_DJ, n);
+}
+
+/* diagonal of the operator for the Jacobi preconditioner: 1 + dt*eta*alpha*(sum
+ * of beta over the four faces around the edge) */
+static double res_diag(Orc *o, const DofList *d, int n, double dt) {
+    OBlock *b = &o->b[d->blk[n]];
+    const int a = d->ax[n], i = d->i[n], j = d->j[n], k = d->k[n];
+    double s;
+    if (a == 0) s = F(b, S_BETA + 1, i, j, k - 1) + F(b, S_BETA + 1, i, j, k) + F(b, S_BETA + 2, i, j, k) + F(b, S_BETA + 2, i, j - 1, k);
+    else if (a == 1) s = F(b, S_BETA + 0, i, j, k) + F(b, S_BETA + 0, i, j, k - 1) + F(b, S_BETA + 2, i - 1, j, k) + F(b, S_BETA + 2, i, j, k);
+    else s = F(b, S_BETA + 0, i, j - 1, k) + F(b, S_BETA + 0, i, j, k) + F(b, S_BETA + 1, i, j, k) + F(b, S_BETA + 1, i - 1, j, k);
+    return 1.0 + dt * (DOF(o, d, S_ETAR, n) * F(b, S_ALPHA + a, i, j, k)) * s;
+}
+
+/* SolveImplicitResistiveDiffusion_ (Mercury_ImplicitResistive.cpp:467-547) and
+ * ApplyImplicitResistiveUpdate_ (:549-575).  The system A(x) = b, b = eta*J(B*),
+ * is solved by conjugate gradients on D^-1 A_lin x = D^-1 (b - A(0)), D =
+ * diag(eta*alpha): D^-1 A_lin = D^-1 + dt C^T diag(beta) C is symmetric positive
+ * definite.  Jacobi preconditioner; stop when |b - A(x)|_2 <= max(rtol |b|_2,
+ * atol).  The previous solution is the initial guess.  The reference runs
+ * PETSc GMRES without preconditioner on the same equation. */
+static int res_solve(Orc *o, double dt, int *its_out, double *rel_out) {
+    const OrcPhys *P = &o->P;
+    DofList d;
+    dof_list(o, &d);
+    const int n = d.n;
+    double *b = calloc(n + 1, sizeof(double)), *c = calloc(n + 1, sizeof(double)), *x = calloc(n + 1, sizeof(double));
+    double *r = calloc(n + 1, sizeof(double)), *z = calloc(n + 1, sizeof(double)), *p = calloc(n + 1, sizeof(double));
+    double *Ap = calloc(n + 1, sizeof(double)), *dg = calloc(n + 1, sizeof(double)), *w = calloc(n + 1, sizeof(double));
+    /* Calc_J_Edge on B* (the synchronised state after the explicit update) */
+    calc_jedge(o);
+    alias_reduce_edges(o, S_J);
+    sync_group(o, G_JEDGE);
+    for (int q = 0; q < n; ++q) {
+        b[q] = 0.0 + DOF(o, &d, S_ETAR, q) * DOF(o, &d, S_J, q);
+        x[q] = DOF(o, &d, S_XRES, q);
+        dg[q] = res_diag(o, &d, q, dt);
+        w[q] = 1.0 / (DOF(o, &d, S_ETAR, q) * F(&o->b[d.blk[q]], S_ALPHA + d.ax[q], d.i[q], d.j[q], d.k[q]));
+    }
+    double *zero = calloc(n + 1, sizeof(double));
+    res_apply(o, &d, zero, c, dt, 0);
+    res_apply(o, &d, x, Ap, dt, 0);
+    double bn = 0.0, rn = 0.0, rz = 0.0;
+    for (int q = 0; q < n; ++q) {
+        r[q] = b[q] - Ap[q];
+        z[q] = r[q] / dg[q];
+        p[q] = z[q];
+        bn += b[q] * b[q];
+        rn += r[q] * r[q];
+        rz += r[q] * z[q] * w[q];
+    }
+    bn = sqrt(bn);
+    rn = sqrt(rn);
+    const double tol = dmax(P->resist_rtol * bn, P->resist_atol);
+    int its = 0, rc = 0;
+    while (rn > tol) {
+        if (its >= (int)P->resist_max_it) { rc = -3; break; }
+        res_apply(o, &d, p, Ap, dt, 1);
+        double pAp = 0.0;
+        for (int q = 0; q < n; ++q) pAp += p[q] * Ap[q] * w[q];
+        const double al = rz / pAp;
+        double rzn = 0.0;
+        rn = 0.0;
+        for (int q = 0; q < n; ++q) {
+            x[q] += al * p[q];
+            r[q] -= al * Ap[q];
+            z[q] = r[q] / dg[q];
+            rn += r[q] * r[q];
+            rzn += r[q] * z[q] * w[q];
+        }
+        rn = sqrt(rn);
+        const double be = rzn / rz;
+        rz = rzn;
+        for (int q = 0; q < n; ++q) p[q] = z[q] + be * p[q];
+        ++its;
+    }
+    if (its_out) *its_out = its;
+    if (rel_out) *rel_out = bn > 0.0 ? rn / bn : rn;
+    for (int ib = 0; ib < o->nb; ++ib)
+        for (int a = 0; a < 3; ++a) memset(o->b[ib].f[S_XRES + a], 0, sizeof(double) * o->b[ib].V);
+    for (int q = 0; q < n; ++q) DOF(o, &d, S_XRES, q) = x[q];
+    /* UnpackVecToImplicitEres_ + ApplyImplicitResistiveUpdate_: Eres = x (synced),
+     * RHS_Bres = -CurlEdgeToFace(Eres), B += dt * RHS_Bres on owned faces, Sync(Bface) */
+    for (int ib = 0; ib < o->nb; ++ib)
+        for (int a = 0; a < 3; ++a) memset(o->b[ib].f[S_ERES + a], 0, sizeof(double) * o->b[ib].V);
+    for (int q = 0; q < n; ++q) DOF(o, &d, S_ERES, q) = x[q];
+    sync_group(o, G_ERES);
+    curl_e2f(o, S_ERES, S_RHSBR, -1.0);
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *bk = &o->b[ib];
+        for (int a = 0; a < 3; ++a) {
+            int hi[3];
+            owned_hi(bk, L_FX + a, hi);
+            for (int i = 0; i < hi[0]; ++i)
+                for (int j = 0; j < hi[1]; ++j)
+                    for (int k = 0; k < hi[2]; ++k) F(bk, S_B + a, i, j, k) += dt * F(bk, S_RHSBR + a, i, j, k);
+        }
+    }
+    sync_group(o, G_BFACE);
+    free(b); free(c); free(x); free(r); free(z); free(p); free(Ap); free(dg); free(w); free(zero);
+    dof_free(&d);
+    return rc;
+}
+
+/* the operator on fields, for parity tests: x from the generic edge 1-form
+ * slots on the unknowns, y written back there (0 elsewhere) */
+static void res_apply_fields(Orc *o, double dt, int lin) {
+    DofList d;
+    dof_list(o, &d);
+    double *x = calloc(d.n + 1, sizeof(double)), *y = calloc(d.n + 1, sizeof(double));
+    for (int q = 0; q < d.n; ++q) x[q] = DOF(o, &d, S_EDGEF, q);
+    res_apply(o, &d, x, y, dt, lin);
+    for (int ib = 0; ib < o->nb; ++ib)
+        for (int a = 0; a < 3; ++a) memset(o->b[ib].f[S_EDGEF + a], 0, sizeof(double) * o->b[ib].V);
+    for (int q = 0; q < d.n; ++q) DOF(o, &d, S_EDGEF, q) = y[q];
+    free(x); free(y);
+    dof_free(&d);
+}
+
 /* one evaluation of the stage operator and Euler update:
  * StepOnce (MercurySolver_Driver.cpp:21-61) without the dt computation */
-static void stage(Orc *o, double dt) {
+static int stage(Orc *o, double dt, int resist) {
     scheme_u(o);
     add_sources(o);
     induction_ct(o);
     apply_update(o, dt);
     sync_group(o, G_UCELL);
     sync_group(o, G_BFACE);
+    int rc = 0;
+    if (resist) rc = res_solve(o, dt, NULL, NULL);
     update_derived(o);
+    return rc;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1691,6 +1978,7 @@ void *orc_create(int nb, const int *dims, int g, const int *bc, const double *no
         recon_weights(b);
         edge_statics(b, &o->P);
     }
+    resist_dofs(o);
     return o;
 }
 
@@ -1728,7 +2016,12 @@ int orc_slot(const char *name) {
         {"taper_xi", S_TAPER}, {"taper_eta", S_TAPER + 1}, {"taper_zeta", S_TAPER + 2},
         {"eta_local_xi", S_LETA}, {"eta_local_eta", S_LETA + 1}, {"eta_local_zeta", S_LETA + 2},
         {"node_scalar", S_NODEF}, {"edge_xi", S_EDGEF}, {"edge_eta", S_EDGEF + 1}, {"edge_zeta", S_EDGEF + 2},
-        {"face_xi", S_FACEF}, {"face_eta", S_FACEF + 1}, {"face_zeta", S_FACEF + 2}, {"cell_scalar", S_CELLS}};
+        {"face_xi", S_FACEF}, {"face_eta", S_FACEF + 1}, {"face_zeta", S_FACEF + 2}, {"cell_scalar", S_CELLS},
+        {"Eres_xi", S_ERES}, {"Eres_eta", S_ERES + 1}, {"Eres_zeta", S_ERES + 2}, {"dJ_xi", S_DJ}, {"dJ_eta", S_DJ + 1}, {"dJ_zeta", S_DJ + 2},
+        {"eta_res_xi", S_ETAR}, {"eta_res_eta", S_ETAR + 1}, {"eta_res_zeta", S_ETAR + 2},
+        {"Eres_guess_xi", S_XRES}, {"Eres_guess_eta", S_XRES + 1}, {"Eres_guess_zeta", S_XRES + 2},
+        {"dB_xi", S_DB}, {"dB_eta", S_DB + 1}, {"dB_zeta", S_DB + 2},
+        {"RHS_Bres_xi", S_RHSBR}, {"RHS_Bres_eta", S_RHSBR + 1}, {"RHS_Bres_zeta", S_RHSBR + 2}};
     for (unsigned q = 0; q < sizeof(T) / sizeof(T[0]); ++q)
         if (!strcmp(T[q].n, name)) return T[q].s;
     return -1;
@@ -1802,17 +2095,24 @@ int orc_step(void *h, int nsteps, int integrator, double *dt_out) {
         double dt = compute_timestep(o);
         if (!isfinite(dt) || dt < 1e-7) return -2;
         o->dt = dt;
+        const int resist = o->P.resist_enabled != 0.0;
+        int rc = 0;
         if (integrator == 0) {
-            stage(o, dt);
+            /* StepOnce (MercurySolver_Driver.cpp:21-61): the implicit resistive
+             * correction follows the explicit update and Sync(Bface) */
+            rc = stage(o, dt, resist);
         } else {
+            /* SSP-RK2: the correction once per step, after the average */
             save_state(o);
-            stage(o, dt);
-            stage(o, dt);
+            stage(o, dt, 0);
+            stage(o, dt, 0);
             rk2_average(o);
             sync_group(o, G_UCELL);
             sync_group(o, G_BFACE);
+            if (resist) rc = res_solve(o, dt, NULL, NULL);
             update_derived(o);
         }
+        if (rc) return rc;
         o->time += dt;
         o->step += 1;
         if (dt_out) dt_out[n] = dt;
@@ -1835,26 +2135,7 @@ static void dec_d0(Orc *o) {
         }
     }
 }
-static void dec_d1(Orc *o, double mult) {
-    for (int ib = 0; ib < o->nb; ++ib) {
-        OBlock *b = &o->b[ib];
-        int hi[3];
-#define EE(a, i, j, k) F(b, S_EDGEF + (a), i, j, k)
-        owned_hi(b, L_FX, hi);
-        for (int i = 0; i < hi[0]; ++i)
-            for (int j = 0; j < hi[1]; ++j)
-                for (int k = 0; k < hi[2]; ++k) F(b, S_FACEF, i, j, k) = mult * ((EE(1, i, j, k) - EE(1, i, j, k + 1)) + (EE(2, i, j + 1, k) - EE(2, i, j, k)));
-        owned_hi(b, L_FY, hi);
-        for (int i = 0; i < hi[0]; ++i)
-            for (int j = 0; j < hi[1]; ++j)
-                for (int k = 0; k < hi[2]; ++k) F(b, S_FACEF + 1, i, j, k) = mult * ((EE(0, i, j, k + 1) - EE(0, i, j, k)) + (EE(2, i, j, k) - EE(2, i + 1, j, k)));
-        owned_hi(b, L_FZ, hi);
-        for (int i = 0; i < hi[0]; ++i)
-            for (int j = 0; j < hi[1]; ++j)
-                for (int k = 0; k < hi[2]; ++k) F(b, S_FACEF + 2, i, j, k) = mult * ((EE(0, i, j, k) - EE(0, i, j + 1, k)) + (EE(1, i + 1, j, k) - EE(1, i, j, k)));
-#undef EE
-    }
-}
+static void dec_d1(Orc *o, double mult) { curl_e2f(o, S_EDGEF, S_FACEF, mult); }
 static void dec_d2(Orc *o) {
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
@@ -1895,11 +2176,20 @@ int orc_op(void *h, const char *op, double arg) {
     else if (!strcmp(op, "dec_d1")) dec_d1(o, arg == 0.0 ? 1.0 : arg);
     else if (!strcmp(op, "dec_d2")) dec_d2(o);
     else if (!strcmp(op, "eface")) ideal_emf(o);
+    else if (!strcmp(op, "res_matvec")) res_apply_fields(o, arg, 0);
+    else if (!strcmp(op, "res_matvec_lin")) res_apply_fields(o, arg, 1);
+    else if (!strcmp(op, "sync_eres")) sync_group(o, G_ERES);
+    else if (!strcmp(op, "sync_db")) sync_group(o, G_DB);
+    else if (!strcmp(op, "sync_dj")) sync_group(o, G_DJ);
     else return -1;
     return 0;
 }
 
 double orc_timestep(void *h) { return compute_timestep((Orc *)h); }
+
+/* one implicit resistive correction with time step dt on the current state;
+ * returns 0, or -3 when the iteration limit is reached */
+int orc_res_solve(void *h, double dt, int *its, double *rel) { return res_solve((Orc *)h, dt, its, rel); }
 
 /* calc_divB (MercurySolver_MagneticCell.cpp:151-241) and the normalised
  * diagnostic max|d2 B| / max|B_face| over owned cells / faces */

@@ -51,6 +51,7 @@ def lib():
         L.orc_timestep.argtypes = [vp]
         L.orc_divb.restype = ctypes.c_double
         L.orc_divb.argtypes = [vp, dp]
+        L.orc_res_solve.argtypes = [vp, ctypes.c_double, ip, dp]
         _lib = L
     return _lib
 
@@ -139,3 +140,12 @@ class Oracle:
         m = ctypes.c_double(0.0)
         r = lib().orc_divb(self.h, ctypes.byref(m))
         return r, m.value
+
+    def res_solve(self, dt):
+        """One implicit resistive correction; returns (iterations, relative residual)."""
+        its = ctypes.c_int(0)
+        rel = ctypes.c_double(0.0)
+        rc = lib().orc_res_solve(self.h, float(dt), ctypes.byref(its), ctypes.byref(rel))
+        if rc != 0:
+            raise RuntimeError(f"oracle resistive solve failed ({rc})")
+        return its.value, rel.value

@@ -86,6 +86,28 @@ def test_fluid_rhs_bit_exact(pair):
     _cmp(g, o, ["RHS_H", "RHS_Na"], "sources")
 
 
+@pytest.mark.parametrize("shape", ["cube", "cube_wide", "shell"])
+def test_fused_rhs_bit_exact(oracle_mod, shape):
+    """The fused flux + sources pass equals the oracle's Scheme_U_ followed by
+    AddSourceToRHS_Fluid, on three block shapes (the wide cube has rows longer
+    than a wavefront and an odd row length)."""
+    case = {"cube": small_cube, "cube_wide": lambda: cases.make_periodic_cube(n=(12, 24, 30), g=4),
+            "shell": lambda: cases.make_shell(scale=0.25)}[shape]()
+    print(f"[case] {case.name} dims={[b.dims for b in case.blocks]}", flush=True)
+    g = Mercury(case)
+    o = oracle_mod.Oracle(case)
+    try:
+        g.prepare()
+        o.prepare()
+        g.set_form(1)
+        g.op("rhs_cell")
+        o.op("flux")
+        o.op("sources")
+        _cmp(g, o, ["RHS_H", "RHS_Na"], "fused rhs")
+    finally:
+        g.close()
+
+
 def test_induction_bit_exact(pair):
     case, g, o = pair
     g.prepare()
