@@ -80,11 +80,11 @@ ALGO_BYTES = {
 STAGE_BYTES = 8 * (2 * (5 * 2 + 3) + 10 + 60 + 2 + 3 + 18)
 
 
-def make_case(name: str):
+def make_case(name: str, resist=None):
     if name == "shell6":
-        return cases.make_shell(steps=200)
+        return cases.make_shell(steps=200, resist=resist)
     if name.startswith("shell6x"):
-        return cases.make_shell(scale=float(name[7:]), steps=200)
+        return cases.make_shell(scale=float(name[7:]), steps=200, resist=resist)
     if name == "cube48":
         return cases.make_periodic_cube(n=(48, 48, 48), steps=100)
     raise SystemExit(f"unknown workload {name}")
@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--profile-ops", action="store_true", help="print per-op timings of the fused forms to stderr")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run three device-to-device copies of MIB MiB (a kernel of known traffic for HBM counters)")
+    ap.add_argument("--resist-steps", type=int, default=20,
+                    help="steps of the shell with the implicit resistive correction (0: skip that measurement)")
     ap.add_argument("--forms", default="fp64_plain,mixed,fp64_fused",
                     help="comma list of forms to run (fp64_fused is always run: it gives the value)")
     args = ap.parse_args()
@@ -214,6 +216,35 @@ def main():
     divb_norm = forms["fp64_fused"]["divB_normalised"]
     err = forms["fp64_fused"].get("rel_l2_vs_fp64_oracle")
 
+    # the same workload with the implicit resistive correction (reference sample window
+    # r 0.84-1.04, inver_Rem 0.05): host-driven CG, so not graph captured
+    resist = None
+    if args.resist_steps > 0 and args.workload.startswith("shell6"):
+        g.close()
+        rcase = make_case(args.workload, resist=0.05)
+        g = Mercury(rcase)
+        g.prepare()
+        g.set_form(1)
+        g.step(2, 0)
+        g.synchronize()
+        its = []
+        t1 = time.perf_counter()
+        for _ in range(args.resist_steps):
+            g.step(1, 0)
+            its.append(g.resist_stats()[1])
+        g.synchronize()
+        el = time.perf_counter() - t1
+        n_unk, _, rel = g.resist_stats()
+        resist = {"ms_per_step": el * 1e3 / args.resist_steps, "cell_updates_per_s": ncells * args.resist_steps / el,
+                  "unknown_edges": n_unk, "cg_iterations_mean": float(np.mean(its)), "last_rel_residual": rel,
+                  "form": "fp64_fused", "inver_Rem": 0.05}
+        print(f"[bench] resistive: {resist['ms_per_step']:.3f} ms/step, {n_unk} unknown edges, "
+              f"{resist['cg_iterations_mean']:.1f} CG iterations/step, rel residual {rel:.1e}", file=sys.stderr, flush=True)
+        g.close()
+        g = Mercury(case)
+        g.prepare()
+        g.set_form(1)
+        g.graph_capture(case.integrator)
     times = op_times["fp64_fused"]
     sync_times = {op: g.time_op(op, 20) for op in ("sync_ucell", "sync_bface", "sync_eedge", "sync_ehall", "sync_jedge",
                                                    "sync_bcell", "sync_jcell")}
@@ -257,6 +288,7 @@ def main():
         "err_vs_fp64_oracle": err,
         "divB_normalised": divb_norm,
         "kernel_ms": op_times,
+        "resistive": resist,
     }
     print(json.dumps(out), flush=True)
 

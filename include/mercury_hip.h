@@ -131,8 +131,20 @@ int mercury_synchronize(MercuryContext *ctx);
  * CT update, floors and dt in FP64. */
 int mercury_set_form(MercuryContext *ctx, int form);
 
+/* Implicit resistive correction (is_Mercury_resistance; replaces
+ * MercurySolver::SolveImplicitResistiveDiffusion_, Mercury_ImplicitResistive.cpp:467-547,
+ * with ApplyImplicitResistiveUpdate_ :549-575).  mercury_step runs it after
+ * every Euler update (once per SSP-RK2 step) when the physics struct enables
+ * it; op "res_solve" (arg = dt) runs one correction on the current state.
+ * The reference's PETSc GMRES becomes conjugate gradients in the
+ * diag(eta*alpha)^-1 inner product with a Jacobi preconditioner.  Reports the
+ * number of unknown edges and, for the last solve, the iterations and the
+ * relative residual |b - A x| / |b|. */
+int mercury_resist_stats(MercuryContext *ctx, int *n_unknowns, int *iterations, double *rel_residual);
+
 /* One explicit step captured as a HIP graph and replayed (no reference
- * counterpart: the reference loops on the host). */
+ * counterpart: the reference loops on the host).  Not available with the
+ * implicit resistive correction (its iteration count is data dependent). */
 int mercury_graph_capture(MercuryContext *ctx, int integrator);
 int mercury_graph_launch(MercuryContext *ctx, int nsteps);
 

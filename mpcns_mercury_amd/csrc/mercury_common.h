@@ -37,7 +37,10 @@ enum MercSlot {
     SL_NODEF = SL_EDGEF + 3,
     // cell: contravariant velocity of U+ per axis (fused path)
     SL_UCON = SL_NODEF + 1,
-    SL_NSLOT = SL_UCON + 3
+    // implicit resistive correction: Eres, dJ and eta on the unknown edges;
+    // dB and RHS_Bres faces
+    SL_ERES = SL_UCON + 3, SL_DJ = SL_ERES + 3, SL_ETAR = SL_DJ + 3, SL_DB = SL_ETAR + 3, SL_RHSBR = SL_DB + 3,
+    SL_NSLOT = SL_RHSBR + 3
 };
 
 // physics constants (same layout as MercuryPhysics in include/mercury_hip.h)

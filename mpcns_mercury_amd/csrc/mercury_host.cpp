@@ -26,6 +26,7 @@
 #include "mercury_geometry.h"
 #include "mercury_kernels.h"
 #include "mercury_fused.h"
+#include "mercury_resist.h"
 
 static_assert(sizeof(MercuryPhysics) == sizeof(MercPhys), "MercuryPhysics / MercPhys layout mismatch");
 
@@ -56,6 +57,11 @@ static const FieldSpec kFields[] = {
     {"Jac", SL_JAC, 1, LOC_CELL}, {"Bcell_from_Bface_w", SL_WB, 18, LOC_CELL}, {"Jcell_from_Jedge_w", SL_WJ, 36, LOC_CELL},
     {"Photo_rate", SL_PHOTO, 1, LOC_CELL}, {"cell_center", SL_CC, 3, LOC_CELL}, {"divB", SL_DIVB, 1, LOC_CELL},
     {"cell_scalar", SL_CELLS, 1, LOC_CELL},
+    {"Eres_xi", SL_ERES, 1, LOC_EX}, {"Eres_eta", SL_ERES + 1, 1, LOC_EY}, {"Eres_zeta", SL_ERES + 2, 1, LOC_EZ},
+    {"dJ_xi", SL_DJ, 1, LOC_EX}, {"dJ_eta", SL_DJ + 1, 1, LOC_EY}, {"dJ_zeta", SL_DJ + 2, 1, LOC_EZ},
+    {"eta_res_xi", SL_ETAR, 1, LOC_EX}, {"eta_res_eta", SL_ETAR + 1, 1, LOC_EY}, {"eta_res_zeta", SL_ETAR + 2, 1, LOC_EZ},
+    {"dB_xi", SL_DB, 1, LOC_FX}, {"dB_eta", SL_DB + 1, 1, LOC_FY}, {"dB_zeta", SL_DB + 2, 1, LOC_FZ},
+    {"RHS_Bres_xi", SL_RHSBR, 1, LOC_FX}, {"RHS_Bres_eta", SL_RHSBR + 1, 1, LOC_FY}, {"RHS_Bres_zeta", SL_RHSBR + 2, 1, LOC_FZ},
     {"B_xi", SL_B, 1, LOC_FX}, {"B_eta", SL_B + 1, 1, LOC_FY}, {"B_zeta", SL_B + 2, 1, LOC_FZ},
     {"Badd_xi", SL_BADD, 1, LOC_FX}, {"Badd_eta", SL_BADD + 1, 1, LOC_FY}, {"Badd_zeta", SL_BADD + 2, 1, LOC_FZ},
     {"RHS_B_xi", SL_RHSB, 1, LOC_FX}, {"RHS_B_eta", SL_RHSB + 1, 1, LOC_FY}, {"RHS_B_zeta", SL_RHSB + 2, 1, LOC_FZ},
@@ -235,8 +241,9 @@ struct MergedProg {
     std::vector<std::unique_ptr<MergedOp>> ops;
 };
 
-enum GroupId { G_UCELL = 0, G_BFACE, G_EEDGE, G_EHALL, G_JEDGE, G_BCELL, G_JCELL, G_BADD, G_NGROUP };
-enum Kind { K_UH, K_UNA, K_BFACE, K_EEDGE, K_EHALL, K_JEDGE, K_BCELL, K_BIND, K_JCELL, K_BADD };
+enum GroupId { G_UCELL = 0, G_BFACE, G_EEDGE, G_EHALL, G_JEDGE, G_BCELL, G_JCELL, G_BADD, G_ERES, G_ERES0, G_DB, G_NGROUP };
+// K_ERES0: Eres with a zero far field (the linear part of the resistive operator)
+enum Kind { K_UH, K_UNA, K_BFACE, K_EEDGE, K_EHALL, K_JEDGE, K_BCELL, K_BIND, K_JCELL, K_BADD, K_ERES, K_ERES0, K_DB };
 
 struct GField {
     int kind, slot, ncomp, loc, level, owner_sync;
@@ -255,6 +262,10 @@ static std::vector<GField> group_fields(int gid) {
     case G_JEDGE: for (int a = 0; a < 3; ++a) v.push_back({K_JEDGE, SL_J + a, 1, LOC_EX + a, 3, 1}); break;
     case G_BCELL: v = {{K_BCELL, SL_BC, 3, LOC_CELL, 3, 0}, {K_BIND, SL_BI, 3, LOC_CELL, 3, 0}}; break;
     case G_JCELL: v = {{K_JCELL, SL_JC, 3, LOC_CELL, 3, 0}}; break;
+    // Eres / dB groups of the implicit correction (Boundary_Build.cpp:605-613, 674-683)
+    case G_ERES: for (int a = 0; a < 3; ++a) v.push_back({K_ERES, SL_ERES + a, 1, LOC_EX + a, 3, 1}); break;
+    case G_ERES0: for (int a = 0; a < 3; ++a) v.push_back({K_ERES0, SL_ERES + a, 1, LOC_EX + a, 3, 1}); break;
+    case G_DB: for (int a = 0; a < 3; ++a) v.push_back({K_DB, SL_DB + a, 1, LOC_FX + a, 3, 1}); break;
     }
     return v;
 }
@@ -531,6 +542,16 @@ public:
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_integrator = -1;
+    // implicit resistive correction: unknown edges and the CG work vectors
+    bool resist = false;
+    ResDofs rd;
+    DBuf<int> r_q, r_ax, r_sI, r_sJ;
+    DBuf<double> r_eta, r_vec, r_part, r_S;
+    double *rx = nullptr, *rr = nullptr, *rz = nullptr, *rp = nullptr, *rAp = nullptr, *rb = nullptr, *rdg = nullptr, *rw = nullptr,
+           *rt = nullptr;
+    int res_its = 0;
+    double res_rel = 0.0;
+    static constexpr int RES_CHECK = 4;  // the residual is read back every RES_CHECK iterations
 
     MercurySolver(int nb, const int *dims, int g, const int *bc, const double *nodes, const MercPhys &phys) : P(phys) {
         if (g < 2) throw std::runtime_error("nghost must be >= 2");
@@ -596,6 +617,7 @@ public:
         make_range(*F, -1, 2, R_eface);
         build_programs();
         alias.build(*F);
+        build_resist();
     }
     ~MercurySolver() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -662,16 +684,20 @@ public:
                         case K_BFACE: break;
                         case K_BCELL: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = P.B_imf; break;
                         case K_BIND: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = zero3; break;
-                        case K_EEDGE: {
-                            // BC_Farfield_Eedge_set_zerocurl (Boundary_Condition.cpp:573-623)
+                        case K_EEDGE:
+                        case K_ERES:
+                        case K_ERES0: {
+                            // BC_Farfield_Eedge_set_zerocurl (Boundary_Condition.cpp:573-623), also the
+                            // Eres far-field handler (Boundary_Build.cpp:198-200); K_ERES0 sets 0
                             const double *u = P.q_pv_inf, *Bi = P.B_imf;
+                            const bool zero = gf.kind == K_ERES0;
                             const double Ei[3] = {Bi[1] * u[2] - Bi[2] * u[1], Bi[2] * u[0] - Bi[0] * u[2], Bi[0] * u[1] - Bi[1] * u[0]};
                             const int a = gf.loc - LOC_EX;
                             const BlockGeom &G = *geom[b];
                             for_box(inner_slab(Fl, b, gf.loc, dir), [&](int i, int j, int kk) {
                                 const auto r0 = G.node(i, j, kk), r1 = G.node(i + (a == 0), j + (a == 1), kk + (a == 2));
                                 setd.push_back(Fl.gidx(b, i, j, kk));
-                                setv.push_back(Ei[0] * (r1[0] - r0[0]) + Ei[1] * (r1[1] - r0[1]) + Ei[2] * (r1[2] - r0[2]));
+                                setv.push_back(zero ? 0.0 : Ei[0] * (r1[0] - r0[0]) + Ei[1] * (r1[1] - r0[1]) + Ei[2] * (r1[2] - r0[2]));
                             });
                             BoundaryCore::default_copy(Fl, b, gf.loc, dir, copyB);
                             break;
@@ -817,6 +843,143 @@ public:
 
     void sync(int gid) { run_merged(1u << gid); }
 
+    // Unknowns of the implicit resistive correction (BuildImplicitResistiveEdgeDofMap_,
+    // Mercury_ImplicitResistive.cpp:165-217; ImplicitResistiveEtaAtEdge_ :219-252;
+    // MercuryResistivityShape_, MercurySolver_Tools.cpp:50-69): owned edges that own
+    // their periodic class, |dl| > 1e-12, eta = inver_Rem * shape(|edge midpoint|) > 0.
+    // Edges with alpha = 0 (identity row, zero right-hand side: they solve to 0)
+    // and far-field boundary-layer edges (overwritten by the Eres far-field
+    // handler before reaching a face) are left out: the solution on the other
+    // edges and the updated B do not depend on them.
+    void build_resist() {
+        resist = P.resist_enabled != 0.0;
+        if (!resist) return;
+        std::vector<int> q, ax, si, sj;
+        std::vector<double> eta;
+        const Field &Fl = *F;
+        for (int b = 0; b < Fl.nb(); ++b) {
+            const MercBlk &k = Fl.blk[b];
+            const BlockGeom &G = *geom[b];
+            std::vector<double> dl(Fl.vol(b)), al(Fl.vol(b)), etab(Fl.vol(b));
+            for (int a = 0; a < 3; ++a) {
+                const int loc = LOC_EX + a;
+                Fl.download_block(SL_DL + a, b, dl.data());
+                Fl.download_block(SL_ALPHA + a, b, al.data());
+                std::fill(etab.begin(), etab.end(), 0.0);
+                std::vector<char> ff(Fl.vol(b), 0);
+                for (int f = 0; f < 6; ++f)
+                    if (k.bc[f] == MBC_FARFIELD)
+                        for_box(inner_slab(Fl, b, loc, face_dir(f)), [&](int i, int j, int kk) { ff[Fl.gidx(b, i, j, kk) - k.off] = 1; });
+                int hi[3];
+                Fl.owned_hi(b, loc, hi);
+                for (int i = 0; i < hi[0]; ++i)
+                    for (int j = 0; j < hi[1]; ++j)
+                        for (int kk = 0; kk < hi[2]; ++kk) {
+                            const int idx[3] = {i, j, kk};
+                            bool alias_e = false;
+                            for (int c = 0; c < 3; ++c)
+                                if (DELTA[loc][c] == 0 && k.bc[2 * c] == MBC_PERIODIC && idx[c] == k.m[c]) alias_e = true;
+                            if (alias_e) continue;
+                            const long long l = Fl.gidx(b, i, j, kk) - k.off;
+                            if (std::fabs(dl[l]) <= 1.0e-12 || al[l] == 0.0 || ff[l]) continue;
+                            const auto r0 = G.node(i, j, kk), r1 = G.node(i + (a == 0), j + (a == 1), kk + (a == 2));
+                            const double xm = 0.5 * (r0[0] + r1[0]), ym = 0.5 * (r0[1] + r1[1]), zm = 0.5 * (r0[2] + r1[2]);
+                            const double r = std::sqrt(xm * xm + ym * ym + zm * zm);
+                            const double shape = 0.5 * (std::tanh((r - P.resist_r_inner) / P.resist_width) -
+                                                        std::tanh((r - P.resist_r_outer) / P.resist_width));
+                            const double sh = (!std::isfinite(shape) || shape <= 1.0e-10) ? 0.0 : std::min(1.0, shape);
+                            const double e = P.inver_Rem * sh;
+                            if (!(e > 0.0)) continue;
+                            q.push_back(Fl.gidx(b, i, j, kk));
+                            ax.push_back(a);
+                            si.push_back(k.sI);
+                            sj.push_back(k.sJ);
+                            eta.push_back(e);
+                            etab[l] = e;
+                        }
+                F->upload_block(SL_ETAR + a, b, etab.data());
+            }
+        }
+        const int n = (int)q.size();
+        rd.n = n;
+        r_q.upload(q);
+        r_ax.upload(ax);
+        r_sI.upload(si);
+        r_sJ.upload(sj);
+        r_eta.upload(eta);
+        rd.q = r_q.p;
+        rd.ax = r_ax.p;
+        rd.sI = r_sI.p;
+        rd.sJ = r_sJ.p;
+        rd.eta = r_eta.p;
+        r_vec.upload(std::vector<double>(9 * (size_t)std::max(n, 1), 0.0));
+        double *v = r_vec.p;
+        const size_t m = (size_t)std::max(n, 1);
+        rx = v; rr = v + m; rz = v + 2 * m; rp = v + 3 * m; rAp = v + 4 * m; rb = v + 5 * m; rdg = v + 6 * m; rw = v + 7 * m; rt = v + 8 * m;
+        r_part.upload(std::vector<double>(3 * (size_t)std::max(res_blocks(n), 1), 0.0));
+        r_S.upload(std::vector<double>(CG_NSCALAR, 0.0));
+    }
+
+    // MatMultImplicitResistive_ (Mercury_ImplicitResistive.cpp:406-465): Eres <- x on
+    // the unknowns, Sync(Eres), dB = -dt * CurlEdgeToFace(Eres), Sync(dB), dJ =
+    // alpha * CurlAdjFaceToEdge(beta * dB), alias mean, y = Eres - eta * dJ.  The
+    // reference's Sync(dJ) touches no unknown and is left out.  lin: the far-field
+    // Eres handler with a zero field (the linear part).  With w, the partial sums of
+    // x.y.w go to r_part.
+    void res_apply(const double *x, double *y, double dt, bool lin, const double *w) {
+        launch_res_scatter(st, rd, x, Fd());
+        run_merged(1u << (lin ? G_ERES0 : G_ERES));
+        launch_dec_d1(st, R_node.h, Bd(), Fd(), SL_ERES, SL_DB, -dt);
+        run_merged(1u << G_DB);
+        launch_jedge(st, R_node.h, Bd(), Fd(), SL_DB, SL_DJ);
+        launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_DJ);
+        launch_res_gather(st, rd, Fd(), y, x, w, r_part.p);
+    }
+
+    // SolveImplicitResistiveDiffusion_ (Mercury_ImplicitResistive.cpp:467-547) and
+    // ApplyImplicitResistiveUpdate_ (:549-575) on the synchronised state B*:
+    // b = eta * J(B*); conjugate gradients on D^-1 A_lin x = D^-1 (b - A(0)),
+    // D = diag(eta*alpha), with the Jacobi preconditioner (D^-1 A_lin = D^-1 +
+    // dt C^T diag(beta) C is symmetric positive definite); stop when |b - A(x)|_2
+    // <= max(rtol |b|_2, atol), tested every RES_CHECK iterations; the previous
+    // solution is the initial guess.  Then Eres = Sync(x), B += dt * (-curl Eres)
+    // on owned faces, Sync(Bface).  The reference runs PETSc GMRES without a
+    // preconditioner on the same equation.
+    void res_solve(double dt) {
+        const int n = rd.n;
+        // Calc_J_Edge on B* (its Sync(Jedge) touches no unknown)
+        launch_jedge(st, R_node.h, Bd(), Fd());
+        launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
+        launch_res_rhs(st, rd, Fd(), dt, rb, rdg, rw);
+        res_apply(rx, rAp, dt, false, nullptr);
+        launch_cg_init(st, n, rb, rAp, rdg, rw, rr, rz, rp, r_part.p, r_S.p);
+        int its = 0;
+        double S[CG_NSCALAR];
+        for (;;) {
+            if (its % RES_CHECK == 0) {
+                HIPCHK(hipMemcpyAsync(S, r_S.p, sizeof(S), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                const double rn = std::sqrt(S[CG_RR]), bn = std::sqrt(S[CG_BB]);
+                res_rel = bn > 0.0 ? rn / bn : rn;
+                if (!std::isfinite(rn)) throw std::runtime_error("implicit resistive CG: residual is not finite");
+                if (!(rn > std::max(P.resist_rtol * bn, P.resist_atol)) || n == 0) break;
+                if (its >= (int)P.resist_max_it)
+                    throw std::runtime_error("implicit resistive CG did not converge within " + std::to_string(its) + " iterations");
+            }
+            res_apply(rp, rAp, dt, true, rw);
+            launch_cg_pap(st, n, r_part.p, r_S.p);
+            launch_cg_xr(st, n, r_S.p, rp, rAp, rdg, rw, rx, rr, rz, r_part.p, r_S.p);
+            launch_cg_p(st, n, r_S.p, rz, rp);
+            ++its;
+        }
+        res_its = its;
+        launch_res_scatter(st, rd, rx, Fd());
+        run_merged(1u << G_ERES);
+        launch_dec_d1(st, R_node.h, Bd(), Fd(), SL_ERES, SL_RHSBR, -1.0);
+        launch_bres_update(st, R_node.h, Bd(), Fd(), dt);
+        run_merged(1u << G_BFACE);
+    }
+
     const MercBlk *Bd() const { return F->dblk; }
     const double *dtp() const { return reinterpret_cast<const double *>(d_dt); }
 
@@ -840,6 +1003,7 @@ public:
             run_merged(MG_EDGE_PAIR);
             launch_update_all(st, R_node.h, Bd(), Fd(), dtp());
             run_merged(MG_STATE_PAIR);
+            if (res_now) res_solve(res_dt);
             derived_fused(final);
             return;
         } else {
@@ -853,6 +1017,7 @@ public:
         sync(G_EEDGE);
         launch_update_all(st, R_node.h, Bd(), Fd(), dtp());
         run_merged(MG_STATE_PAIR);
+        if (res_now) res_solve(res_dt);
         derived_fused(final);
     }
 
@@ -890,12 +1055,25 @@ public:
         launch_curl_update(st, R_node.h, Bd(), Fd(), dtp(), 1);
         launch_update_u(st, R_cell.h, Bd(), Fd(), dtp());
         run_merged(MG_STATE_PAIR);
+        if (res_now) res_solve(res_dt);
         derived();
     }
 
+    // With the implicit resistive correction the step reads dt back (the CG
+    // solve is host driven); Euler: after the update and Sync(Bface) as in
+    // StepOnce (MercurySolver_Driver.cpp:31-38); SSP-RK2: once, after the average.
+    bool res_now = false;
+    double res_dt = 0.0;
+
     void enqueue_step(int integrator) {
+        if (form >= 1) launch_dt_commit(st, d_dt, d_dt_next);
+        else launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt);
+        if (resist) {
+            res_dt = read_dt();
+            check_dt(res_dt);
+        }
+        res_now = resist && integrator == 0;
         if (form >= 1) {
-            launch_dt_commit(st, d_dt, d_dt_next);
             if (integrator == 0) {
                 stage_fused(true);
             } else {
@@ -904,11 +1082,12 @@ public:
                 stage_fused(false);
                 launch_rk2(st, R_node.h, Bd(), Fd(), 0);
                 run_merged(MG_STATE_PAIR);
+                if (resist) res_solve(res_dt);
                 derived_fused(true);
             }
+            res_now = false;
             return;
         }
-        launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt);
         if (integrator == 0) {
             stage();
         } else {
@@ -917,8 +1096,10 @@ public:
             stage();
             launch_rk2(st, R_node.h, Bd(), Fd(), 0);
             run_merged(MG_STATE_PAIR);
+            if (resist) res_solve(res_dt);
             derived();
         }
+        res_now = false;
     }
 
     double read_dt() {
@@ -948,6 +1129,7 @@ public:
     }
 
     void capture(int integrator) {
+        if (resist) throw std::runtime_error("a step with the implicit resistive correction cannot be captured (its CG iteration count is data dependent)");
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         gexec = nullptr;
@@ -1051,6 +1233,20 @@ public:
         }
         else if (op == "dec_d0") launch_dec_d0(st, R_node.h, Bd(), Fd(), SL_NODEF, SL_EDGEF);
         else if (op == "dec_d1") launch_dec_d1(st, R_node.h, Bd(), Fd(), SL_EDGEF, SL_FACEF, arg == 0.0 ? 1.0 : arg);
+        else if (op == "res_matvec" || op == "res_matvec_lin") {
+            // the resistive operator with time step arg on the generic edge 1-form:
+            // x taken on the unknowns, y written back there (0 elsewhere)
+            if (!resist) throw std::runtime_error("the implicit resistive correction is off");
+            launch_res_pick(st, rd, Fd(), SL_EDGEF, rt);
+            res_apply(rt, rb, arg, op == "res_matvec_lin", nullptr);
+            const int es[3] = {SL_EDGEF, SL_EDGEF + 1, SL_EDGEF + 2};
+            launch_zero_span(st, Fd(), es, 3, F->Vtot);
+            launch_res_scatter(st, rd, rb, Fd(), SL_EDGEF);
+        } else if (op == "res_solve") {
+            if (!resist) throw std::runtime_error("the implicit resistive correction is off");
+            res_solve(arg);
+        } else if (op == "sync_eres") sync(G_ERES);
+        else if (op == "sync_db") sync(G_DB);
         else if (op == "dec_d2") launch_dec_d2(st, R_cell.h, Bd(), Fd(), SL_FACEF, SL_CELLS);
         else if (op == "sync_ucell") sync(G_UCELL);
         else if (op == "sync_bface") sync(G_BFACE);
@@ -1262,6 +1458,15 @@ int mercury_graph_launch(MercuryContext *ctx, int nsteps) {
     ABI_TRY({
         ctx->s->graph_launch(nsteps);
         HIPCHK(hipGetLastError());
+    })
+}
+
+int mercury_resist_stats(MercuryContext *ctx, int *n_unknowns, int *iterations, double *rel_residual) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        if (n_unknowns) *n_unknowns = ctx->s->resist ? ctx->s->rd.n : 0;
+        if (iterations) *iterations = ctx->s->res_its;
+        if (rel_residual) *rel_residual = ctx->s->res_rel;
     })
 }
 

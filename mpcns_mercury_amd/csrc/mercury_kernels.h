@@ -14,7 +14,7 @@ struct RangeH {
 
 void launch_pv_uplus(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P);
 void launch_bcell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
-void launch_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
+void launch_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int s_face = SL_B, int s_edge = SL_J);
 void launch_jcell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
 void launch_timestep(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P, unsigned long long *dt_bits);
 void launch_flux_div(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P);

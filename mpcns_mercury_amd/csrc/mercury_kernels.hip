@@ -88,22 +88,23 @@ __global__ void k_bcell(const MercRange R, const MercBlk *__restrict__ B, double
 // CTOperators::CurlAdjFaceToEdge (Z4_Mercury/src/operators/CTOperators.cpp:59-110)
 // followed by J *= alpha (MercurySolver_Current.cpp:140-201); owned edges of the
 // three axes, one thread per node-box point
-__global__ void k_jedge(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F) {
+// (face slots sf.., edge slots se..: B -> J, or dB -> dJ of the implicit correction)
+__global__ void k_jedge(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, int sf, int se) {
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
-#define BB(a, o) (F[SL_BETA + (a)][q + (o)] * F[SL_B + (a)][q + (o)])
+#define BB(a, o) (F[SL_BETA + (a)][q + (o)] * F[sf + (a)][q + (o)])
     if (p.i < bk.m[0]) {
         const double v = (BB(1, -dk) - BB(1, 0)) + (BB(2, 0) - BB(2, -dj));
-        F[SL_J + 0][q] = (1.0 * v) * F[SL_ALPHA + 0][q];
+        F[se + 0][q] = (1.0 * v) * F[SL_ALPHA + 0][q];
     }
     if (p.j < bk.m[1]) {
         const double v = (BB(0, 0) - BB(0, -dk)) + (BB(2, -di) - BB(2, 0));
-        F[SL_J + 1][q] = (1.0 * v) * F[SL_ALPHA + 1][q];
+        F[se + 1][q] = (1.0 * v) * F[SL_ALPHA + 1][q];
     }
     if (p.k < bk.m[2]) {
         const double v = (BB(0, -dj) - BB(0, 0)) + (BB(1, 0) - BB(1, -di));
-        F[SL_J + 2][q] = (1.0 * v) * F[SL_ALPHA + 2][q];
+        F[se + 2][q] = (1.0 * v) * F[SL_ALPHA + 2][q];
     }
 #undef BB
 }
@@ -739,7 +740,7 @@ static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1
 
 void launch_pv_uplus(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P) { LAUNCH_R(k_pv_uplus, R, B, F, P); }
 void launch_bcell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F) { LAUNCH_R(k_bcell, R, B, F); }
-void launch_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F) { LAUNCH_R(k_jedge, R, B, F); }
+void launch_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int sf, int se) { LAUNCH_R(k_jedge, R, B, F, sf, se); }
 void launch_jcell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F) { LAUNCH_R(k_jcell, R, B, F); }
 void launch_timestep(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P, unsigned long long *dt_bits) {
     hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(64), 0, st, dt_bits);

@@ -1,0 +1,216 @@
+// Implicit resistive correction (Z4_Mercury/src/implicit/Mercury_ImplicitResistive.cpp):
+// the per-unknown kernels of the matrix-free operator and of the conjugate
+// gradient iteration.  The unknowns are a list of owned edges (box index q and
+// axis); the field parts of the operator (curls, synchronisation, alias mean)
+// reuse the generic kernels on the Eres / dB / dJ slots.
+//
+// Reductions are deterministic: every 256-thread block writes its partial sum
+// (fixed tree order) and one block adds the partials in block order.
+#include <hip/hip_runtime.h>
+
+#include "mercury_device.h"
+#include "mercury_resist.h"
+
+namespace {
+using namespace merc_dev;
+using namespace merc;
+
+constexpr int RB = 256;  // threads per block of the unknown-vector kernels
+
+// block tree sum of v into part[blockIdx.x]
+DEVI void block_sum(double v, double *__restrict__ part) {
+    __shared__ double sh[RB];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = RB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// edge slots s.. <- x on the unknowns (x == nullptr: 0)
+__global__ void k_scatter(int n, const int *__restrict__ q, const int *__restrict__ ax, const double *__restrict__ x,
+                          double *const *__restrict__ F, int s) {
+    const int t = blockIdx.x * RB + threadIdx.x;
+    if (t >= n) return;
+    F[s + ax[t]][q[t]] = x ? x[t] : 0.0;
+}
+// x <- edge slots s.. on the unknowns
+__global__ void k_pick(int n, const int *__restrict__ q, const int *__restrict__ ax, double *const *__restrict__ F, int s,
+                       double *__restrict__ x) {
+    const int t = blockIdx.x * RB + threadIdx.x;
+    if (t >= n) return;
+    x[t] = F[s + ax[t]][q[t]];
+}
+
+// y = Eres - eta * dJ on the unknowns (MatMultImplicitResistive_ :449-459);
+// with w != nullptr also the block partials of sum(p * y * w)
+__global__ void k_gather(int n, const int *__restrict__ q, const int *__restrict__ ax, const double *__restrict__ eta,
+                         double *const *__restrict__ F, double *__restrict__ y, const double *__restrict__ p,
+                         const double *__restrict__ w, double *__restrict__ part) {
+    const int t = blockIdx.x * RB + threadIdx.x;
+    double v = 0.0;
+    if (t < n) {
+        const double yy = F[SL_ERES + ax[t]][q[t]] - eta[t] * F[SL_DJ + ax[t]][q[t]];
+        y[t] = yy;
+        if (w) v = p[t] * yy * w[t];
+    }
+    if (w) block_sum(v, part);
+}
+
+// right-hand side b = eta * J(B*) (PackImplicitJedgeToVec_ :387-404), the
+// Jacobi diagonal 1 + dt*eta*alpha*sum(beta of the 4 faces around the edge)
+// and the inner-product weight 1/(eta*alpha)
+__global__ void k_rhs(int n, const int *__restrict__ q, const int *__restrict__ ax, const int *__restrict__ sI,
+                      const int *__restrict__ sJ, const double *__restrict__ eta, double *const *__restrict__ F, double dt,
+                      double *__restrict__ b, double *__restrict__ dg, double *__restrict__ w) {
+    const int t = blockIdx.x * RB + threadIdx.x;
+    if (t >= n) return;
+    const long long e = q[t], di = sI[t], dj = sJ[t];
+    const int a = ax[t];
+    const double *be = F[SL_BETA], *bh = F[SL_BETA + 1], *bz = F[SL_BETA + 2];
+    double s;
+    if (a == 0) s = bh[e - 1] + bh[e] + bz[e] + bz[e - dj];
+    else if (a == 1) s = be[e] + be[e - 1] + bz[e - di] + bz[e];
+    else s = be[e - dj] + be[e] + bh[e] + bh[e - di];
+    const double ea = eta[t] * F[SL_ALPHA + a][e];
+    b[t] = 0.0 + eta[t] * F[SL_J + a][e];
+    dg[t] = 1.0 + dt * ea * s;
+    w[t] = 1.0 / ea;
+}
+
+// r = b - Ax, z = r / dg, p = z; partials of b.b, r.r, r.z.w
+__global__ void k_cg_init(int n, const double *__restrict__ b, const double *__restrict__ Ax, const double *__restrict__ dg,
+                          const double *__restrict__ w, double *__restrict__ r, double *__restrict__ z, double *__restrict__ p,
+                          double *__restrict__ part) {
+    const int t = blockIdx.x * RB + threadIdx.x;
+    double bb = 0.0, rr = 0.0, rz = 0.0;
+    if (t < n) {
+        const double rv = b[t] - Ax[t], zv = rv / dg[t];
+        r[t] = rv;
+        z[t] = zv;
+        p[t] = zv;
+        bb = b[t] * b[t];
+        rr = rv * rv;
+        rz = rv * zv * w[t];
+    }
+    block_sum(bb, part);
+    __syncthreads();
+    block_sum(rr, part + gridDim.x);
+    __syncthreads();
+    block_sum(rz, part + 2 * gridDim.x);
+}
+
+// x += alpha p, r -= alpha Ap, z = r / dg; partials of r.r and r.z.w
+__global__ void k_cg_xr(int n, const double *__restrict__ S, const double *__restrict__ p, const double *__restrict__ Ap,
+                        const double *__restrict__ dg, const double *__restrict__ w, double *__restrict__ x,
+                        double *__restrict__ r, double *__restrict__ z, double *__restrict__ part) {
+    const int t = blockIdx.x * RB + threadIdx.x;
+    const double al = S[CG_ALPHA];
+    double rr = 0.0, rz = 0.0;
+    if (t < n) {
+        x[t] += al * p[t];
+        const double rv = r[t] - al * Ap[t], zv = rv / dg[t];
+        r[t] = rv;
+        z[t] = zv;
+        rr = rv * rv;
+        rz = rv * zv * w[t];
+    }
+    block_sum(rr, part);
+    __syncthreads();
+    block_sum(rz, part + gridDim.x);
+}
+
+// p = z + beta p
+__global__ void k_cg_p(int n, const double *__restrict__ S, const double *__restrict__ z, double *__restrict__ p) {
+    const int t = blockIdx.x * RB + threadIdx.x;
+    if (t < n) p[t] = z[t] + S[CG_BETA] * p[t];
+}
+
+// sum of nq partial arrays of length nb (in block order) and the scalar update
+// mode 0 (init): bb, rr, rz; mode 1: alpha = rz / pAp; mode 2: rr, beta = rz'/rz, rz = rz'
+__global__ void k_cg_scalars(int nb, int nq, const double *__restrict__ part, double *__restrict__ S, int mode) {
+    __shared__ double sh[RB];
+    double tot[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < nq; ++c) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < nb; i += RB) v += part[c * nb + i];
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = RB / 2; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+            __syncthreads();
+        }
+        tot[c] = sh[0];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (mode == 0) {
+        S[CG_BB] = tot[0];
+        S[CG_RR] = tot[1];
+        S[CG_RZ] = tot[2];
+    } else if (mode == 1) {
+        S[CG_PAP] = tot[0];
+        S[CG_ALPHA] = S[CG_RZ] / tot[0];
+    } else {
+        S[CG_RR] = tot[0];
+        S[CG_BETA] = tot[1] / S[CG_RZ];
+        S[CG_RZ] = tot[1];
+    }
+}
+
+// B += dt * RHS_Bres on owned faces (ApplyUpdate_Euler_BfaceOnly_,
+// MercurySolver_MagExplicit_Hall.cpp:435-473)
+__global__ void k_bres_update(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, double dt) {
+    LOCATE(R);
+    const MercBlk &bk = B[p.b];
+    const long long q = p.q;
+    if (p.j < bk.m[1] && p.k < bk.m[2]) F[SL_B + 0][q] += dt * F[SL_RHSBR + 0][q];
+    if (p.i < bk.m[0] && p.k < bk.m[2]) F[SL_B + 1][q] += dt * F[SL_RHSBR + 1][q];
+    if (p.i < bk.m[0] && p.j < bk.m[1]) F[SL_B + 2][q] += dt * F[SL_RHSBR + 2][q];
+}
+
+}  // namespace
+
+namespace merc {
+
+static inline unsigned rblk(int n) { return (unsigned)((n + RB - 1) / RB); }
+
+int res_blocks(int n) { return (int)rblk(n); }
+
+void launch_res_scatter(hipStream_t st, const ResDofs &d, const double *x, double *const *F, int slot) {
+    if (d.n > 0) hipLaunchKernelGGL(k_scatter, dim3(rblk(d.n)), dim3(RB), 0, st, d.n, d.q, d.ax, x, F, slot);
+}
+void launch_res_pick(hipStream_t st, const ResDofs &d, double *const *F, int slot, double *x) {
+    if (d.n > 0) hipLaunchKernelGGL(k_pick, dim3(rblk(d.n)), dim3(RB), 0, st, d.n, d.q, d.ax, F, slot, x);
+}
+void launch_res_gather(hipStream_t st, const ResDofs &d, double *const *F, double *y, const double *p, const double *w, double *part) {
+    if (d.n > 0) hipLaunchKernelGGL(k_gather, dim3(rblk(d.n)), dim3(RB), 0, st, d.n, d.q, d.ax, d.eta, F, y, p, w, part);
+}
+void launch_res_rhs(hipStream_t st, const ResDofs &d, double *const *F, double dt, double *b, double *dg, double *w) {
+    if (d.n > 0) hipLaunchKernelGGL(k_rhs, dim3(rblk(d.n)), dim3(RB), 0, st, d.n, d.q, d.ax, d.sI, d.sJ, d.eta, F, dt, b, dg, w);
+}
+void launch_cg_init(hipStream_t st, int n, const double *b, const double *Ax, const double *dg, const double *w, double *r, double *z,
+                    double *p, double *part, double *S) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cg_init, dim3(rblk(n)), dim3(RB), 0, st, n, b, Ax, dg, w, r, z, p, part);
+    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(RB), 0, st, (int)rblk(n), 3, part, S, 0);
+}
+void launch_cg_pap(hipStream_t st, int n, const double *part, double *S) {
+    if (n > 0) hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(RB), 0, st, (int)rblk(n), 1, part, S, 1);
+}
+void launch_cg_xr(hipStream_t st, int n, const double *S, const double *p, const double *Ap, const double *dg, const double *w, double *x,
+                  double *r, double *z, double *part, double *Sw) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cg_xr, dim3(rblk(n)), dim3(RB), 0, st, n, S, p, Ap, dg, w, x, r, z, part);
+    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(RB), 0, st, (int)rblk(n), 2, part, Sw, 2);
+}
+void launch_cg_p(hipStream_t st, int n, const double *S, const double *z, double *p) {
+    if (n > 0) hipLaunchKernelGGL(k_cg_p, dim3(rblk(n)), dim3(RB), 0, st, n, S, z, p);
+}
+void launch_bres_update(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, double dt) {
+    if (R.total > 0) hipLaunchKernelGGL(k_bres_update, dim3((unsigned)((R.total + 255) / 256)), dim3(256), 0, st, R.dev, B, F, dt);
+}
+
+}  // namespace merc

@@ -21,7 +21,7 @@ ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercu
                "mercury_block_volume", "mercury_total_volume", "mercury_upload", "mercury_download",
                "mercury_init_bface", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
                "mercury_divb", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
-               "mercury_time_op", "mercury_geometry_block", "mercury_set_form")
+               "mercury_time_op", "mercury_geometry_block", "mercury_set_form", "mercury_resist_stats")
 
 
 class MercuryError(RuntimeError):
@@ -58,6 +58,7 @@ def load():
     L.mercury_set_form.argtypes = [vp, ctypes.c_int]
     L.mercury_graph_launch.argtypes = [vp, ctypes.c_int]
     L.mercury_time_op.argtypes = [vp, cp, ctypes.c_int, dp]
+    L.mercury_resist_stats.argtypes = [vp, ip, ip, dp]
     L.mercury_geometry_block.argtypes = [ip, ctypes.c_int, dp, ctypes.POINTER(MercuryPhysics), cp, ctypes.c_int, dp]
     _lib = L
     return L
@@ -183,6 +184,12 @@ class Mercury:
 
     def graph_launch(self, n=1):
         _chk(load().mercury_graph_launch(self._h, n))
+
+    def resist_stats(self):
+        """(unknown edges, iterations of the last solve, its relative residual)."""
+        n, it, rel = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _chk(load().mercury_resist_stats(self._h, ctypes.byref(n), ctypes.byref(it), ctypes.byref(rel)))
+        return n.value, it.value, rel.value
 
     def time_op(self, op, reps=10):
         v = ctypes.c_double()

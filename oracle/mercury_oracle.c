@@ -1828,7 +1828,7 @@ static int res_solve(Orc *o, double dt, int *its_out, double *rel_out) {
     DofList d;
     dof_list(o, &d);
     const int n = d.n;
-    double *b = calloc(n + 1, sizeof(double)), *c = calloc(n + 1, sizeof(double)), *x = calloc(n + 1, sizeof(double));
+    double *b = calloc(n + 1, sizeof(double)), *x = calloc(n + 1, sizeof(double));
     double *r = calloc(n + 1, sizeof(double)), *z = calloc(n + 1, sizeof(double)), *p = calloc(n + 1, sizeof(double));
     double *Ap = calloc(n + 1, sizeof(double)), *dg = calloc(n + 1, sizeof(double)), *w = calloc(n + 1, sizeof(double));
     /* Calc_J_Edge on B* (the synchronised state after the explicit update) */
@@ -1841,8 +1841,6 @@ static int res_solve(Orc *o, double dt, int *its_out, double *rel_out) {
         dg[q] = res_diag(o, &d, q, dt);
         w[q] = 1.0 / (DOF(o, &d, S_ETAR, q) * F(&o->b[d.blk[q]], S_ALPHA + d.ax[q], d.i[q], d.j[q], d.k[q]));
     }
-    double *zero = calloc(n + 1, sizeof(double));
-    res_apply(o, &d, zero, c, dt, 0);
     res_apply(o, &d, x, Ap, dt, 0);
     double bn = 0.0, rn = 0.0, rz = 0.0;
     for (int q = 0; q < n; ++q) {
@@ -1901,7 +1899,7 @@ static int res_solve(Orc *o, double dt, int *its_out, double *rel_out) {
         }
     }
     sync_group(o, G_BFACE);
-    free(b); free(c); free(x); free(r); free(z); free(p); free(Ap); free(dg); free(w); free(zero);
+    free(b); free(x); free(r); free(z); free(p); free(Ap); free(dg); free(w);
     dof_free(&d);
     return rc;
 }
