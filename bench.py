@@ -80,6 +80,18 @@ ALGO_BYTES = {
 STAGE_BYTES = 8 * (2 * (5 * 2 + 3) + 10 + 60 + 2 + 3 + 18)
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of a fused pass from the committed rocprofv3 counter
+    summary (2*FETCH_SIZE + WRITE_SIZE, see profiles/shell6_hbm_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", "shell6_hbm_traffic.md")
+    try:
+        text = open(path).read()
+        blob = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(blob).get(kernel)
+    except (OSError, IndexError, ValueError):
+        return None
+
+
 def make_case(name: str, resist=None):
     if name == "shell6":
         return cases.make_shell(steps=200, resist=resist)
@@ -276,7 +288,8 @@ def main():
                                f"{[list(b.dims) for b in case.blocks]}, forward Euler, CFL 0.4",
                    "cells": ncells, "blocks": case.nb, "form": "fp64_fused", "integrator": "euler"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom) if args.workload == "shell6" else None, "kernel": dom,
                      "algo_bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": units,
                      "avg_launch_ms": times[dom]},
         "step_roofline": {"stage_bytes_per_cell": STAGE_BYTES, "ms_per_step": step_graph_ms, "achieved_GBs": step_gbs,

@@ -134,6 +134,29 @@ def test_steps_bit_exact(pair, integrator, form):
 
 
 @pytest.mark.parametrize("integrator", [0, 1])
+def test_hundred_steps_bit_exact(oracle_mod, integrator):
+    """100 steps of the periodic cube (the single-block configuration's
+    length), fused FP64 form replayed from a graph: the state equals the
+    oracle's bit for bit (relative L2 = 0 <= 1e-12) and div B stays at roundoff."""
+    case = cases.make_periodic_cube(n=(12, 10, 8), g=4, integrator=integrator)
+    print(f"[case] {case.name} 100 steps integrator={integrator}", flush=True)
+    g = Mercury(case)
+    o = oracle_mod.Oracle(case)
+    try:
+        g.prepare()
+        o.prepare()
+        g.set_form(1)
+        g.graph_capture(integrator)
+        g.graph_launch(100)
+        g.synchronize()
+        o.step(100, integrator)
+        _cmp(g, o, STATE + DERIVED, "100 steps")
+        assert g.divb()[0] <= 1e-12
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
 def test_graph_replay_equals_oracle(pair, integrator):
     """A captured step replayed from a HIP graph gives the oracle's state."""
     case, g, o = pair
