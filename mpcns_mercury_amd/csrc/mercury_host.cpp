@@ -183,6 +183,20 @@ struct RangeOwner {
     DBuf<int> lo, n;
 };
 
+// per block box [lo, lo + n) (n = 0: no point of the block)
+static void make_range_boxes(const Field &F, const std::vector<int> &lo, const std::vector<int> &n, RangeOwner &R) {
+    std::vector<long long> st(F.nb() + 1, 0);
+    for (int b = 0; b < F.nb(); ++b) st[b + 1] = st[b] + (long long)n[3 * b] * n[3 * b + 1] * n[3 * b + 2];
+    R.start.upload(st);
+    R.lo.upload(lo);
+    R.n.upload(n);
+    R.h.dev.nb = F.nb();
+    R.h.dev.start = R.start.p;
+    R.h.dev.lo = R.lo.p;
+    R.h.dev.n = R.n.p;
+    R.h.total = st.back();
+}
+
 static void make_range(const Field &F, int lo_off, int hi_off, RangeOwner &R) {
     // per block box [lo_off, m + hi_off) on every axis
     std::vector<long long> st(F.nb() + 1, 0);
@@ -528,6 +542,7 @@ public:
     MercPhys P{};
     std::vector<std::unique_ptr<BlockGeom>> geom;
     RangeOwner R_cell, R_box, R_node, R_eface;
+    RangeOwner R_res;  // node-box points around the resistive unknowns (the CG's curls)
     GroupProg prog[G_NGROUP];
     std::map<unsigned, MergedProg> mprog;  // keyed by the bit set of groups
     AliasClasses alias;
@@ -900,6 +915,36 @@ public:
                 F->upload_block(SL_ETAR + a, b, etab.data());
             }
         }
+        // the CG applies the linear operator, whose faces and edges lie within two
+        // points of an unknown (and of its periodic images): per block the bounding
+        // box of the unknowns grown by 2, the whole axis where the block is periodic
+        {
+            std::vector<int> lo(3 * Fl.nb(), 0), nn(3 * Fl.nb(), 0);
+            std::vector<int> bmin(3 * Fl.nb(), 1 << 30), bmax(3 * Fl.nb(), -1);
+            for (size_t e = 0; e < q.size(); ++e) {
+                int b = 0;
+                while (b + 1 < Fl.nb() && Fl.blk[b + 1].off <= q[e]) ++b;
+                const MercBlk &k = Fl.blk[b];
+                long long r = q[e] - k.off;
+                const int ii = (int)(r / k.sI) - k.g, jj = (int)((r % k.sI) / k.sJ) - k.g, kq = (int)(r % k.sJ) - k.g;
+                const int id[3] = {ii, jj, kq};
+                for (int c = 0; c < 3; ++c) {
+                    bmin[3 * b + c] = std::min(bmin[3 * b + c], id[c]);
+                    bmax[3 * b + c] = std::max(bmax[3 * b + c], id[c]);
+                }
+            }
+            for (int b = 0; b < Fl.nb(); ++b) {
+                if (bmax[3 * b] < 0) continue;
+                const MercBlk &k = Fl.blk[b];
+                for (int c = 0; c < 3; ++c) {
+                    int l = std::max(0, bmin[3 * b + c] - 2), h = std::min(k.m[c] + 1, bmax[3 * b + c] + 3);
+                    if (k.bc[2 * c] == MBC_PERIODIC) l = 0, h = k.m[c] + 1;
+                    lo[3 * b + c] = l;
+                    nn[3 * b + c] = h - l;
+                }
+            }
+            make_range_boxes(Fl, lo, nn, R_res);
+        }
         const int n = (int)q.size();
         rd.n = n;
         r_q.upload(q);
@@ -929,9 +974,9 @@ public:
     void res_apply(const double *x, double *y, double dt, bool lin, const double *w) {
         launch_res_scatter(st, rd, x, Fd());
         run_merged(1u << (lin ? G_ERES0 : G_ERES));
-        launch_dec_d1(st, R_node.h, Bd(), Fd(), SL_ERES, SL_DB, -dt);
+        launch_dec_d1(st, R_res.h, Bd(), Fd(), SL_ERES, SL_DB, -dt);
         run_merged(1u << G_DB);
-        launch_jedge(st, R_node.h, Bd(), Fd(), SL_DB, SL_DJ);
+        launch_jedge(st, R_res.h, Bd(), Fd(), SL_DB, SL_DJ);
         launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_DJ);
         launch_res_gather(st, rd, Fd(), y, x, w, r_part.p);
     }
@@ -947,8 +992,9 @@ public:
     // preconditioner on the same equation.
     void res_solve(double dt) {
         const int n = rd.n;
-        // Calc_J_Edge on B* (its Sync(Jedge) touches no unknown)
-        launch_jedge(st, R_node.h, Bd(), Fd());
+        // Calc_J_Edge on B* around the unknowns (its Sync(Jedge) touches no
+        // unknown; the derived pass recomputes J everywhere afterwards)
+        launch_jedge(st, R_res.h, Bd(), Fd());
         launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
         launch_res_rhs(st, rd, Fd(), dt, rb, rdg, rw);
         res_apply(rx, rAp, dt, false, nullptr);
