@@ -134,6 +134,8 @@ def main():
                     help="first run three device-to-device copies of MIB MiB (a kernel of known traffic for HBM counters)")
     ap.add_argument("--resist-steps", type=int, default=20,
                     help="steps of the shell with the implicit resistive correction (0: skip that measurement)")
+    ap.add_argument("--ensemble-runs", type=int, default=8,
+                    help="members of the batched-ensemble measurement (0: skip; the full configuration has 64)")
     ap.add_argument("--forms", default="fp64_plain,mixed,fp64_fused",
                     help="comma list of forms to run (fp64_fused is always run: it gives the value)")
     args = ap.parse_args()
@@ -257,6 +259,48 @@ def main():
         g.prepare()
         g.set_form(1)
         g.graph_capture(case.integrator)
+    # batched ensemble of independent single-fluid + Hall runs (seeded sizes in
+    # [32,96]^3): one graph steps every member; serial member graphs for comparison
+    ensemble = None
+    if args.ensemble_runs > 0:
+        from mpcns_mercury_amd.hip import Ensemble  # noqa: E402
+        t0 = time.perf_counter()
+        runs = [cases.make_ensemble_run(r) for r in range(args.ensemble_runs)]
+        ecells = sum(c.ncells() for c in runs)
+        members = [Mercury(c) for c in runs]
+        for m in members:
+            m.prepare()
+        ensemble = {"runs": args.ensemble_runs, "cells": ecells, "blocks": sum(c.nb for c in runs)}
+        for form, label in ((1, "fp64_fused"), (2, "mixed")):
+            for m in members:
+                m.set_form(form)
+            ens = Ensemble(members, 0)
+            ens.launch(3)
+            t1 = time.perf_counter()
+            ens.launch(20)
+            el = time.perf_counter() - t1
+            ens.close()
+            for m in members:
+                m.graph_capture(0)
+                m.graph_launch(2)
+            for m in members:
+                m.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(5):
+                for m in members:
+                    m.graph_launch(1)
+            for m in members:
+                m.synchronize()
+            el_serial = (time.perf_counter() - t1) / 5
+            ensemble[label] = {"ms_per_step": el * 1e3 / 20, "cell_updates_per_s": ecells * 20 / el,
+                               "serial_member_graphs_ms_per_step": el_serial * 1e3}
+            print(f"[bench] ensemble {label}: {args.ensemble_runs} runs, {ecells} cells: {el * 1e3 / 20:.3f} ms/step "
+                  f"({ecells * 20 / el:.3e} cell-updates/s), member graphs one after another {el_serial * 1e3:.3f} ms",
+                  file=sys.stderr, flush=True)
+        ensemble["mixed_over_fp64"] = ensemble["mixed"]["cell_updates_per_s"] / ensemble["fp64_fused"]["cell_updates_per_s"]
+        for m in members:
+            m.close()
+        print(f"[bench] ensemble setup+run {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     times = op_times["fp64_fused"]
     sync_times = {op: g.time_op(op, 20) for op in ("sync_ucell", "sync_bface", "sync_eedge", "sync_ehall", "sync_jedge",
                                                    "sync_bcell", "sync_jcell")}
@@ -302,6 +346,7 @@ def main():
         "divB_normalised": divb_norm,
         "kernel_ms": op_times,
         "resistive": resist,
+        "ensemble": ensemble,
     }
     print(json.dumps(out), flush=True)
 

@@ -148,6 +148,17 @@ int mercury_resist_stats(MercuryContext *ctx, int *n_unknowns, int *iterations, 
 int mercury_graph_capture(MercuryContext *ctx, int integrator);
 int mercury_graph_launch(MercuryContext *ctx, int nsteps);
 
+/* Ensemble of independent runs (no reference counterpart: the reference runs
+ * one case per MPI job).  One HIP graph forks into every context's stream,
+ * captures one step of each (integrator as in mercury_graph_capture) and joins
+ * back; mercury_ensemble_launch replays it nsteps times and waits.  Contexts
+ * keep their own physics, dt and state and must outlive the ensemble; contexts
+ * with the implicit resistive correction are refused. */
+typedef struct MercuryEnsemble MercuryEnsemble;
+int mercury_ensemble_create(MercuryContext *const *ctxs, int n, int integrator, MercuryEnsemble **out);
+int mercury_ensemble_launch(MercuryEnsemble *ens, int nsteps);
+int mercury_ensemble_destroy(MercuryEnsemble *ens);
+
 /* Average device time (ms) of an op over reps launches, HIP events on the
  * solver's stream; op "step_graph" replays the captured step. */
 int mercury_time_op(MercuryContext *ctx, const char *op, int reps, double *ms_avg);

@@ -183,7 +183,9 @@ def sector_nodes(dims, panel, r0=1.0, r1=4.0, half_angle=np.deg2rad(40.0)):
 
 
 def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, integrator: int = 0,
-               scale: float = 1.0, hall_coef: float | None = None, resist: float | None = None) -> Case:
+               scale: float = 1.0, hall_coef: float | None = None, resist: float | None = None,
+               inflow: float = 1.0, imf=(0.0, 0.0, -0.2), na_fraction: float = 0.01, panels=None,
+               name: str | None = None) -> Case:
     """Two-fluid Hall MHD: H+ and Na+ (mass ratio 1:23, density ratio 1:0.01) plus
     electron pressure (ambipolar term), inflow u = (-1,0,0), IMF B = (0,0,-0.2),
     the fluid side of a perfectly conducting sphere at r = 1 on every block's
@@ -196,9 +198,9 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
     block is a separate gnomonic sector (no inter-block interfaces)."""
     units = MercuryUnits().alfven_units()
     Bref = units.REF["B_ref"]
-    units.INITIAL = {"U": units.REF["U"], "T": units.REF["T"], "n": units.REF["n"],
+    units.INITIAL = {"U": inflow * units.REF["U"], "T": units.REF["T"], "n": units.REF["n"],
                      "Molecular_mass": units.REF["Molecular_mass"], "c_y": 0.0, "c_z": 0.0,
-                     "Bx": 0.0, "By": 0.0, "Bz": -0.2 * Bref}
+                     "Bx": imf[0] * Bref, "By": imf[1] * Bref, "Bz": imf[2] * Bref}
     units.CFL = 0.4
     units.r_min, units.r_max = 1.1, 1.2
     units.is_Ambipolar_Efield = True
@@ -211,7 +213,7 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
     blks, fields = [], []
     for ib, dims0 in enumerate(blocks):
         dims = tuple(max(4, int(round(d * scale))) for d in dims0)
-        nodes = sector_nodes(dims, _PANELS[ib % 6])
+        nodes = sector_nodes(dims, _PANELS[(panels[ib] if panels is not None else ib) % 6])
         blks.append(Block(dims=dims, bc=(BC_WALL,) + (BC_FARFIELD,) * 5, nodes=nodes))
         I, J, K = _box_index(dims, g)
         key = (((ib * 4096 + (I + g)) * 4096 + (J + g)) * 4096 + (K + g)).astype(np.uint64)
@@ -219,9 +221,9 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
         rhoH = P.qinf[0] * (1.0 + 1e-3 * uniform(seed, 20, key))
         u0, v0, w0, p0 = P.q_pv_inf[0], P.q_pv_inf[1], P.q_pv_inf[2], P.q_pv_inf[3]
         UH = _conserved(rhoH, np.full(shape, u0), np.full(shape, v0), np.full(shape, w0), np.full(shape, p0), gamma)
-        rhoN = 0.01 * rhoH
+        rhoN = na_fraction * rhoH
         UNa = _conserved(rhoN, np.full(shape, u0), np.full(shape, v0), np.full(shape, w0),
-                         np.full(shape, p0 * 0.01 / 23.0), gamma)
+                         np.full(shape, p0 * na_fraction / 23.0), gamma)
         # cell-centre radius from the owned nodes (photo-ionisation profile of the exosphere)
         cc = np.zeros((3,) + shape)
         nd = nodes
@@ -231,7 +233,8 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
         r = np.sqrt((cc * cc).sum(axis=0))
         photo = np.where(r > 1.0, 1e-2 * np.exp(-(r - 1.0) / 0.2), 0.0)
         fields.append({"U_H": UH, "U_Na": UNa, "Photo_rate": photo[None]})
-    name = "shell6" if scale == 1.0 else f"shell6_x{scale:g}"
+    if name is None:
+        name = "shell6" if scale == 1.0 else f"shell6_x{scale:g}"
     return Case(name=name + ("_resist" if resist is not None else ""), g=g, blocks=blks, phys=P, fields=fields,
                 A=None, Bu=(0.0, 0.0, 0.0), steps=steps, integrator=integrator,
                 meta={"seed": seed, "species": 2, "hall": True, "resist": resist})
@@ -260,3 +263,27 @@ def cartesian_nodes(n=(6, 5, 4), h=(0.1, 0.1, 0.1), origin=(0.0, 0.0, 0.0)) -> n
     """Owned nodes of a uniform Cartesian block."""
     i, j, k = np.meshgrid(np.arange(n[0] + 1), np.arange(n[1] + 1), np.arange(n[2] + 1), indexing="ij")
     return np.stack([origin[0] + h[0] * i, origin[1] + h[1] * j, origin[2] + h[2] * k]).astype(np.float64)
+
+
+# ---------------------------------------------------------------------------
+# C4: ensemble member r of independent single-fluid + Hall runs
+# ---------------------------------------------------------------------------
+
+def make_ensemble_run(r: int, scale: float = 1.0, g: int = 4, steps: int = 100) -> Case:
+    """Run r of the batched ensemble (seed 1000 + r): 1-4 shell-sector blocks with
+    seeded sizes in [32, 96]^3 (times ``scale``), inflow speed U(0.6, 1.4), IMF of
+    strength 0.2 at angle U(0, 2 pi) in the y-z plane, Hall coefficient U(0.02, 0.1);
+    single fluid (the Na+ fluid at rho_small)."""
+    seed = 1000 + r
+    u = lambda k: float(uniform(seed, 90 + k, np.arange(1, dtype=np.uint64))[0] * 0.5 + 0.5)  # U(0, 1)
+    nblk = 1 + int(u(0) * 4) % 4
+    dims = []
+    for b in range(nblk):
+        dims.append(tuple(int(round((32 + u(1 + 3 * b + a) * 64) * scale)) for a in range(3)))
+    speed = 0.6 + 0.8 * u(20)
+    ang = 2.0 * np.pi * u(21)
+    hall = 0.02 + 0.08 * u(22)
+    panels = [int(u(30 + b) * 6) % 6 for b in range(nblk)]
+    return make_shell(blocks=tuple(dims), g=g, seed=seed, steps=steps, scale=1.0, hall_coef=hall, inflow=speed,
+                      imf=(0.0, 0.2 * np.sin(ang), -0.2 * np.cos(ang)), na_fraction=RHO_SMALL, panels=panels,
+                      name=f"ensemble_run{r}")

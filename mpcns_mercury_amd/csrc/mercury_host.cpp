@@ -1494,6 +1494,73 @@ int mercury_set_form(MercuryContext *ctx, int form) {
     ABI_TRY({ ctx->s->set_form(form); })
 }
 
+// ---------------------------------------------------------------------------
+// ensembles: independent contexts stepped by one graph
+// ---------------------------------------------------------------------------
+struct MercuryEnsemble {
+    std::vector<MercuryContext *> ctx;
+    hipStream_t root = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~MercuryEnsemble() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto e : join)
+            if (e) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (root) (void)hipStreamDestroy(root);
+    }
+};
+
+int mercury_ensemble_create(MercuryContext *const *ctxs, int n, int integrator, MercuryEnsemble **out) {
+    if (!ctxs || n <= 0 || !out) {
+        merc::g_last_error = "mercury_ensemble_create: invalid argument";
+        return -1;
+    }
+    for (int i = 0; i < n; ++i) CHECK_CTX(ctxs[i]);
+    std::unique_ptr<MercuryEnsemble> e(new MercuryEnsemble());
+    ABI_TRY({
+        for (int i = 0; i < n; ++i)
+            if (ctxs[i]->s->resist) throw std::runtime_error("an ensemble cannot hold a context with the implicit resistive correction");
+        e->ctx.assign(ctxs, ctxs + n);
+        HIPCHK(hipStreamCreateWithFlags(&e->root, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+        e->join.assign(n, nullptr);
+        for (int i = 0; i < n; ++i) HIPCHK(hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming));
+        // fork from the root stream into every context's stream, join back
+        HIPCHK(hipStreamBeginCapture(e->root, hipStreamCaptureModeThreadLocal));
+        HIPCHK(hipEventRecord(e->fork, e->root));
+        for (int i = 0; i < n; ++i) {
+            hipStream_t cs = e->ctx[i]->s->st;
+            HIPCHK(hipStreamWaitEvent(cs, e->fork, 0));
+            e->ctx[i]->s->enqueue_step(integrator);
+            HIPCHK(hipEventRecord(e->join[i], cs));
+            HIPCHK(hipStreamWaitEvent(e->root, e->join[i], 0));
+        }
+        HIPCHK(hipStreamEndCapture(e->root, &e->graph));
+        HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+        *out = e.release();
+    })
+}
+
+int mercury_ensemble_launch(MercuryEnsemble *e, int nsteps) {
+    if (!e || !e->exec || nsteps < 0) {
+        merc::g_last_error = "mercury_ensemble_launch: invalid ensemble";
+        return -1;
+    }
+    ABI_TRY({
+        for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(e->exec, e->root));
+        HIPCHK(hipStreamSynchronize(e->root));
+    })
+}
+
+int mercury_ensemble_destroy(MercuryEnsemble *e) {
+    if (!e) return 0;
+    ABI_TRY({ delete e; })
+}
+
 int mercury_graph_capture(MercuryContext *ctx, int integrator) {
     CHECK_CTX(ctx);
     ABI_TRY({ ctx->s->capture(integrator); })

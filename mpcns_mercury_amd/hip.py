@@ -21,7 +21,8 @@ ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercu
                "mercury_block_volume", "mercury_total_volume", "mercury_upload", "mercury_download",
                "mercury_init_bface", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
                "mercury_divb", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
-               "mercury_time_op", "mercury_geometry_block", "mercury_set_form", "mercury_resist_stats")
+               "mercury_time_op", "mercury_geometry_block", "mercury_set_form", "mercury_resist_stats",
+               "mercury_ensemble_create", "mercury_ensemble_launch", "mercury_ensemble_destroy")
 
 
 class MercuryError(RuntimeError):
@@ -59,6 +60,9 @@ def load():
     L.mercury_graph_launch.argtypes = [vp, ctypes.c_int]
     L.mercury_time_op.argtypes = [vp, cp, ctypes.c_int, dp]
     L.mercury_resist_stats.argtypes = [vp, ip, ip, dp]
+    L.mercury_ensemble_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+    L.mercury_ensemble_launch.argtypes = [vp, ctypes.c_int]
+    L.mercury_ensemble_destroy.argtypes = [vp]
     L.mercury_geometry_block.argtypes = [ip, ctypes.c_int, dp, ctypes.POINTER(MercuryPhysics), cp, ctypes.c_int, dp]
     _lib = L
     return L
@@ -195,3 +199,28 @@ class Mercury:
         v = ctypes.c_double()
         _chk(load().mercury_time_op(self._h, op.encode(), reps, ctypes.byref(v)))
         return v.value
+
+
+class Ensemble:
+    """Independent contexts stepped together by one captured graph."""
+
+    def __init__(self, members, integrator=0):
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        h = ctypes.c_void_p()
+        _chk(load().mercury_ensemble_create(arr, len(self.members), int(integrator), ctypes.byref(h)))
+        self._h = h
+
+    def launch(self, nsteps=1):
+        _chk(load().mercury_ensemble_launch(self._h, int(nsteps)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().mercury_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
