@@ -4,8 +4,9 @@
 #include "mercury_kernels.h"
 
 namespace merc {
+// R: per block ceil(m0 / L) x m1 x m2 points; each thread marches over L cells along i
 void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
-                     bool mix);
+                     int L, bool mix);
 void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                      unsigned long long *dt_next, int compute_dt, bool mix);
 void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, bool mix);

@@ -96,108 +96,130 @@ DEVI void rusanov_face_t(double *const *__restrict__ F, int su, int sp, long lon
 }
 
 // ---------------------------------------------------------------------------
+// Scheme_U_ + AddSourceToRHS_Fluid.  One thread marches over a column of up to
+// L owned cells along i: the i+1/2 face flux of a cell is the i-1/2 flux of the
+// next one (the same face evaluated once, with the same operands), and the
+// i-neighbour planes are read while they are still in cache.  The launch range
+// has ceil(m0 / L) points along i per block.
 template <bool MIX>
-__global__ void k_rhs_cell(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
-                           const float *const *__restrict__ G, const MercPhys P) {
+__global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
+                           const float *const *__restrict__ G, const MercPhys P, int L) {
     using T = typename Prec<MIX>::T;
     LOCATE(R);
-    const long long q = p.q, di = OI(p), dj = OJ(p);
+    const MercBlk &bk = B[p.b];
+    const long long di = OI(p), dj = OJ(p);
+    const int i0 = p.i * L, i1 = (bk.m[0] < i0 + L) ? bk.m[0] : i0 + L;
+    const long long q0 = p.q + (long long)(i0 - p.i) * di;
     const T gm = (T)P.gamma;
-    const T invJ = (T)(1.0 / F[SL_JAC][q]);
     // --- Scheme_U_ (MercurySolver_FluidRHS.cpp:3-202)
-    double rr[2][5];
-    for (int s = 0; s < 2; ++s) {
-        const int su = s ? SL_UN : SL_UH, sp = s ? SL_PVN : SL_PVH;
-        Acc<MIX> r[5];
-        auto axis = [&](int a) {
-            const long long d = AX(p, a);
-            T Fm[5], Fp[5];
-            rusanov_face_t<MIX>(F, su, sp, q - d, q, q, a, gm, Fm);
-            rusanov_face_t<MIX>(F, su, sp, q, q + d, q + d, a, gm, Fp);
+    T Fx[2][5];  // i-1/2 flux of the current cell, per species
+    for (int s = 0; s < 2; ++s) rusanov_face_t<MIX>(F, s ? SL_UN : SL_UH, s ? SL_PVN : SL_PVH, q0 - di, q0, q0, 0, gm, Fx[s]);
+    for (int i = i0; i < i1; ++i) {
+        const long long q = q0 + (long long)(i - i0) * di;
+        const T invJ = (T)(1.0 / F[SL_JAC][q]);
+        double rr[2][5];
+        for (int s = 0; s < 2; ++s) {
+            const int su = s ? SL_UN : SL_UH, sp = s ? SL_PVN : SL_PVH;
+            Acc<MIX> r[5];
+            {
+                T Fp[5];
+                rusanov_face_t<MIX>(F, su, sp, q, q + di, q + di, 0, gm, Fp);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) r[m].sub((Fp[m] - Fm[m]) * invJ);
-        };
-        if constexpr (MIX) {
-            // the mixed body keeps one axis live at a time (unrolled it spills)
-#pragma unroll 1
-            for (int a = 0; a < 3; ++a) axis(a);
-        } else {
-            for (int a = 0; a < 3; ++a) axis(a);
-        }
-        for (int m = 0; m < 5; ++m) rr[s][m] = r[m].value();
-    }
-    // --- AddSourceToRHS_Fluid (MercurySolver_FluidRHS.cpp:206-527)
-    const double upx = F[SL_UP][q], upy = F[SL_UP + 1][q], upz = F[SL_UP + 2][q];
-    const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
-    const double Jx = F[SL_JC][q], Jy = F[SL_JC + 1][q], Jz = F[SL_JC + 2][q];
-    const double sjbx = Jy * Bz - Jz * By, sjby = Jz * Bx - Jx * Bz, sjbz = Jx * By - Jy * Bx;
-    const Num num = num_limiter(P, F[SL_UH][q], F[SL_UN][q]);
-    double dpex = 0.0, dpey = 0.0, dpez = 0.0;
-    const bool ambi = P.ambipolar_enabled != 0.0;
-    if (ambi) {
-        const double V = fabs(F[SL_JAC][q]);
-        double d[9];
-        if (V <= 0.0) {
-            for (int t = 0; t < 9; ++t) d[t] = 0.0;
-        } else {
-            for (int c = 0; c < 3; ++c) {
-                d[c] = 0.5 * (F[SL_JD + c][q - di] + F[SL_JD + c][q]) / V;
-                d[3 + c] = 0.5 * (F[SL_JD + 3 + c][q - dj] + F[SL_JD + 3 + c][q]) / V;
-                d[6 + c] = 0.5 * (F[SL_JD + 6 + c][q - 1] + F[SL_JD + 6 + c][q]) / V;
+                for (int m = 0; m < 5; ++m) {
+                    r[m].sub((Fp[m] - Fx[s][m]) * invJ);
+                    Fx[s][m] = Fp[m];
+                }
             }
+            auto axis = [&](int a) {
+                const long long d = AX(p, a);
+                T Fm[5], Fp[5];
+                rusanov_face_t<MIX>(F, su, sp, q - d, q, q, a, gm, Fm);
+                rusanov_face_t<MIX>(F, su, sp, q, q + d, q + d, a, gm, Fp);
+#pragma unroll
+                for (int m = 0; m < 5; ++m) r[m].sub((Fp[m] - Fm[m]) * invJ);
+            };
+            if constexpr (MIX) {
+                // the mixed body keeps one axis live at a time (unrolled it spills)
+#pragma unroll 1
+                for (int a = 1; a < 3; ++a) axis(a);
+            } else {
+                for (int a = 1; a < 3; ++a) axis(a);
+            }
+            for (int m = 0; m < 5; ++m) rr[s][m] = r[m].value();
         }
-#define PE(o) (F[SL_PVH + 3][q + (o)] + F[SL_PVN + 3][q + (o)])
-        const double pip = PE(di), pim = PE(-di), pjp = PE(dj), pjm = PE(-dj), pkp = PE(1), pkm = PE(-1);
-#undef PE
-        dpex = 0.5 * (d[0] * (pip - pim) + d[3] * (pjp - pjm) + d[6] * (pkp - pkm));
-        dpey = 0.5 * (d[1] * (pip - pim) + d[4] * (pjp - pjm) + d[7] * (pkp - pkm));
-        dpez = 0.5 * (d[2] * (pip - pim) + d[5] * (pjp - pjm) + d[8] * (pkp - pkm));
-    }
-    const double sss = (double)st<MIX>(F, G, SL_PHOTO, q);
-    const double x = F[SL_CC][q], y = F[SL_CC + 1][q], z = F[SL_CC + 2][q];
-    const double rho_cyl = sqrt(y * y + z * z);
-    const bool in_shadow = (x < 0.0 && rho_cyl < 1.0);
-    const double vst = in_shadow ? P.nu_shadow : P.nu_lit;
-    for (int s = 0; s < 2; ++s) {
-        const int su = s ? SL_UN : SL_UH, sp = s ? SL_PVN : SL_PVH, sr = s ? SL_RN : SL_RH;
-        const double rho = dmax(F[su][q], 0.0);
-        const double u = F[sp][q], v = F[sp + 1][q], w = F[sp + 2][q];
-        const double u2 = u * u + v * v + w * w;
-        const double subx = (v - upy) * Bz - (w - upz) * By;
-        const double suby = (w - upz) * Bx - (u - upx) * Bz;
-        const double subz = (u - upx) * By - (v - upy) * Bx;
-        const double subu = subx * u + suby * v + subz * w;
-        const double sjbu = sjbx * u + sjby * v + sjbz * w;
-        const double dpeu = dpex * u + dpey * v + dpez * w;
-        const double Ts0 = F[sp + 4][q] * P.T_ref;
-        const double Ms = s ? P.M_Na : P.M_H;
-        const double Mo = P.M_Na;
-        const double b1 = vst * (P.Tn0 - Ts0) * (Ms / (Ms + Mo));
-        const double b2 = (Mo / (Ms + Mo)) * vst;
-        const double ns = s ? num.nNa : num.nH;
-        const double chi = s ? num.wNa : num.wH;
-        double r0 = rr[s][0], r1 = rr[s][1], r2 = rr[s][2], r3 = rr[s][3], r4 = rr[s][4];
-        r1 += P.c_ind * ns * subx + P.c_hall * chi * sjbx;
-        r2 += P.c_ind * ns * suby + P.c_hall * chi * sjby;
-        r3 += P.c_ind * ns * subz + P.c_hall * chi * sjbz;
-        r4 += P.c_ind * ns * subu + P.c_hall * chi * sjbu;
+        // --- AddSourceToRHS_Fluid (MercurySolver_FluidRHS.cpp:206-527)
+        const double upx = F[SL_UP][q], upy = F[SL_UP + 1][q], upz = F[SL_UP + 2][q];
+        const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
+        const double Jx = F[SL_JC][q], Jy = F[SL_JC + 1][q], Jz = F[SL_JC + 2][q];
+        const double sjbx = Jy * Bz - Jz * By, sjby = Jz * Bx - Jx * Bz, sjbz = Jx * By - Jy * Bx;
+        const Num num = num_limiter(P, F[SL_UH][q], F[SL_UN][q]);
+        double dpex = 0.0, dpey = 0.0, dpez = 0.0;
+        const bool ambi = P.ambipolar_enabled != 0.0;
         if (ambi) {
-            r1 -= chi * dpex;
-            r2 -= chi * dpey;
-            r3 -= chi * dpez;
-            r4 -= chi * dpeu;
+            const double V = fabs(F[SL_JAC][q]);
+            double d[9];
+            if (V <= 0.0) {
+                for (int t = 0; t < 9; ++t) d[t] = 0.0;
+            } else {
+                for (int c = 0; c < 3; ++c) {
+                    d[c] = 0.5 * (F[SL_JD + c][q - di] + F[SL_JD + c][q]) / V;
+                    d[3 + c] = 0.5 * (F[SL_JD + 3 + c][q - dj] + F[SL_JD + 3 + c][q]) / V;
+                    d[6 + c] = 0.5 * (F[SL_JD + 6 + c][q - 1] + F[SL_JD + 6 + c][q]) / V;
+                }
+            }
+    #define PE(o) (F[SL_PVH + 3][q + (o)] + F[SL_PVN + 3][q + (o)])
+            const double pip = PE(di), pim = PE(-di), pjp = PE(dj), pjm = PE(-dj), pkp = PE(1), pkm = PE(-1);
+    #undef PE
+            dpex = 0.5 * (d[0] * (pip - pim) + d[3] * (pjp - pjm) + d[6] * (pkp - pkm));
+            dpey = 0.5 * (d[1] * (pip - pim) + d[4] * (pjp - pjm) + d[7] * (pkp - pkm));
+            dpez = 0.5 * (d[2] * (pip - pim) + d[5] * (pjp - pjm) + d[8] * (pkp - pkm));
         }
-        if (s) {
-            r0 += P.a1_Na * sss;
-            r4 += P.a6 * sss * P.Tn0;
+        const double sss = (double)st<MIX>(F, G, SL_PHOTO, q);
+        const double x = F[SL_CC][q], y = F[SL_CC + 1][q], z = F[SL_CC + 2][q];
+        const double rho_cyl = sqrt(y * y + z * z);
+        const bool in_shadow = (x < 0.0 && rho_cyl < 1.0);
+        const double vst = in_shadow ? P.nu_shadow : P.nu_lit;
+        for (int s = 0; s < 2; ++s) {
+            const int su = s ? SL_UN : SL_UH, sp = s ? SL_PVN : SL_PVH, sr = s ? SL_RN : SL_RH;
+            const double rho = dmax(F[su][q], 0.0);
+            const double u = F[sp][q], v = F[sp + 1][q], w = F[sp + 2][q];
+            const double u2 = u * u + v * v + w * w;
+            const double subx = (v - upy) * Bz - (w - upz) * By;
+            const double suby = (w - upz) * Bx - (u - upx) * Bz;
+            const double subz = (u - upx) * By - (v - upy) * Bx;
+            const double subu = subx * u + suby * v + subz * w;
+            const double sjbu = sjbx * u + sjby * v + sjbz * w;
+            const double dpeu = dpex * u + dpey * v + dpez * w;
+            const double Ts0 = F[sp + 4][q] * P.T_ref;
+            const double Ms = s ? P.M_Na : P.M_H;
+            const double Mo = P.M_Na;
+            const double b1 = vst * (P.Tn0 - Ts0) * (Ms / (Ms + Mo));
+            const double b2 = (Mo / (Ms + Mo)) * vst;
+            const double ns = s ? num.nNa : num.nH;
+            const double chi = s ? num.wNa : num.wH;
+            double r0 = rr[s][0], r1 = rr[s][1], r2 = rr[s][2], r3 = rr[s][3], r4 = rr[s][4];
+            r1 += P.c_ind * ns * subx + P.c_hall * chi * sjbx;
+            r2 += P.c_ind * ns * suby + P.c_hall * chi * sjby;
+            r3 += P.c_ind * ns * subz + P.c_hall * chi * sjbz;
+            r4 += P.c_ind * ns * subu + P.c_hall * chi * sjbu;
+            if (ambi) {
+                r1 -= chi * dpex;
+                r2 -= chi * dpey;
+                r3 -= chi * dpez;
+                r4 -= chi * dpeu;
+            }
+            if (s) {
+                r0 += P.a1_Na * sss;
+                r4 += P.a6 * sss * P.Tn0;
+            }
+            r1 -= P.a4 * rho * u * vst;
+            r2 -= P.a4 * rho * v * vst;
+            r3 -= P.a4 * rho * w * vst;
+            r4 -= P.a4 * rho * u2 * vst;
+            r4 += P.a5 * ns * b1;
+            r4 += P.a4 * rho * u2 * b2;
+            F[sr][q] = r0; F[sr + 1][q] = r1; F[sr + 2][q] = r2; F[sr + 3][q] = r3; F[sr + 4][q] = r4;
         }
-        r1 -= P.a4 * rho * u * vst;
-        r2 -= P.a4 * rho * v * vst;
-        r3 -= P.a4 * rho * w * vst;
-        r4 -= P.a4 * rho * u2 * vst;
-        r4 += P.a5 * ns * b1;
-        r4 += P.a4 * rho * u2 * b2;
-        F[sr][q] = r0; F[sr + 1][q] = r1; F[sr + 2][q] = r2; F[sr + 3][q] = r3; F[sr + 4][q] = r4;
     }
 }
 
@@ -668,9 +690,9 @@ static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1
     } while (0)
 
 void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
-                     bool mix) {
-    if (mix) LAUNCH_R(k_rhs_cell<true>, R, B, F, G, P);
-    else LAUNCH_R(k_rhs_cell<false>, R, B, F, G, P);
+                     int L, bool mix) {
+    if (mix) LAUNCH_R(k_rhs_cell<true>, R, B, F, G, P, L);
+    else LAUNCH_R(k_rhs_cell<false>, R, B, F, G, P, L);
 }
 void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                      unsigned long long *dt_next, int compute_dt, bool mix) {

@@ -543,6 +543,8 @@ public:
     std::vector<std::unique_ptr<BlockGeom>> geom;
     RangeOwner R_cell, R_box, R_node, R_eface;
     RangeOwner R_res;  // node-box points around the resistive unknowns (the CG's curls)
+    RangeOwner R_march;  // flux pass: ceil(m0 / march) x m1 x m2 per block
+    int march = 4;       // owned cells per thread along i in the flux pass
     GroupProg prog[G_NGROUP];
     std::map<unsigned, MergedProg> mprog;  // keyed by the bit set of groups
     AliasClasses alias;
@@ -630,6 +632,8 @@ public:
         make_range(*F, -g, g + 1, R_box);
         make_range(*F, 0, 1, R_node);
         make_range(*F, -1, 2, R_eface);
+        if (const char *e = std::getenv("MERCURY_RHS_MARCH")) march = std::max(1, std::atoi(e));
+        set_march(march);
         build_programs();
         alias.build(*F);
         build_resist();
@@ -650,6 +654,17 @@ public:
     const float *const *Gd() const { return fptr; }
     std::vector<float *> fptr_h;
     bool mix() const { return form == 2; }
+    void set_march(int L) {
+        march = L;
+        std::vector<int> lo(3 * F->nb(), 0), n(3 * F->nb());
+        for (int b = 0; b < F->nb(); ++b) {
+            n[3 * b] = (F->blk[b].m[0] + L - 1) / L;
+            n[3 * b + 1] = F->blk[b].m[1];
+            n[3 * b + 2] = F->blk[b].m[2];
+        }
+        make_range_boxes(*F, lo, n, R_march);
+    }
+
     // refresh the FP32 static copies from the FP64 arrays
     void refresh_float_statics() {
         for (int s : fslots) launch_to_float(st, F->hptr[s], fptr_h[s], F->Vtot);
@@ -1042,7 +1057,7 @@ public:
 
     // StepOnce without the time step, fused passes
     void stage_fused(bool final) {
-        launch_rhs_cell(st, R_cell.h, Bd(), Fd(), Gd(), P, mix());
+        launch_rhs_cell(st, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
         launch_eface_u(st, R_eface.h, Bd(), Fd(), mix());
         if (alias.n == 0) {
             launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, 1, mix());
@@ -1267,7 +1282,7 @@ public:
             launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
         else if (op == "timestep") launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt);
         else if (op == "stage") stage();
-        else if (op == "rhs_cell") launch_rhs_cell(st, R_cell.h, Bd(), Fd(), Gd(), P, mix());
+        else if (op == "rhs_cell") launch_rhs_cell(st, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
         else if (op == "cell_end") launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix());
         else if (op == "eface_u") launch_eface_u(st, R_eface.h, Bd(), Fd(), mix());
         else if (op == "edge_total") launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, alias.n == 0 ? 1 : 0, mix());
