@@ -8,11 +8,13 @@ namespace merc {
 void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                      int L, bool mix);
 void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
-                     unsigned long long *dt_next, int compute_dt, bool mix);
-void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, bool mix);
+                     unsigned long long *dt_next, int compute_dt, bool mix, int hall64 = 0);
+void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);
 void launch_edge_total(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                        int with_hall, bool mix);
-void launch_update_all(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const double *dt);
+void launch_update_all(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const double *dt,
+                       bool mix);
+void launch_to_double(hipStream_t st, const float *src, double *dst, long long n);
 void launch_bcell_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);
 void launch_jcell_t(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);
 void launch_to_float(hipStream_t st, const double *src, float *dst, long long n);

@@ -36,11 +36,19 @@ template <> struct Prec<true> {
     static DEVI T div(T a, T b) { return a * __builtin_amdgcn_rcpf(b); }
     static DEVI T sq(T a) { return __builtin_amdgcn_sqrtf(a); }
 };
-// static array element: FP64 table in the FP64 form, FP32 table in the mixed form
+// static array or derived intermediate element: FP64 table in the FP64 form,
+// FP32 table G (same slot index) in the mixed form.  The derived intermediates
+// kept in FP32 by the mixed form are PV u, v, w, T (pressure stays FP64), U+,
+// the Hall cell terms, the contravariant U+, the face EMFs and the fluid RHS.
 template <bool MIX>
 DEVI typename Prec<MIX>::T st(double *const *__restrict__ F, const float *const *__restrict__ G, int s, long long q) {
     if constexpr (MIX) return G[s][q];
     else return F[s][q];
+}
+template <bool MIX, typename V>
+DEVI void wr(double *const *__restrict__ F, const float *const *__restrict__ G, int s, long long q, V v) {
+    if constexpr (MIX) const_cast<float *>(G[s])[q] = (float)v;
+    else F[s][q] = (double)v;
 }
 template <typename T> DEVI T tmax(T a, T b) { return (a < b) ? b : a; }
 template <typename T> DEVI T tmin(T a, T b) { return (b < a) ? b : a; }
@@ -64,8 +72,8 @@ template <bool MIX> struct Acc {
 };
 
 template <bool MIX>
-DEVI void rusanov_face_t(double *const *__restrict__ F, int su, int sp, long long qL, long long qR, long long qf, int a,
-                         typename Prec<MIX>::T gm, typename Prec<MIX>::T out[5]) {
+DEVI void rusanov_face_t(double *const *__restrict__ F, const float *const *__restrict__ G, int su, int sp, long long qL, long long qR,
+                         long long qf, int a, typename Prec<MIX>::T gm, typename Prec<MIX>::T out[5]) {
     using T = typename Prec<MIX>::T;
     using PR = Prec<MIX>;
     const T k1 = (T)F[SL_JD + 3 * a][qf], k2 = (T)F[SL_JD + 3 * a + 1][qf], k3 = (T)F[SL_JD + 3 * a + 2][qf];
@@ -73,7 +81,8 @@ DEVI void rusanov_face_t(double *const *__restrict__ F, int su, int sp, long lon
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
         const long long qq = side ? qR : qL;
-        const T rho = (T)F[su][qq], u = (T)F[sp][qq], v = (T)F[sp + 1][qq], w = (T)F[sp + 2][qq], pr = (T)F[sp + 3][qq];
+        const T rho = (T)F[su][qq], u = st<MIX>(F, G, sp, qq), v = st<MIX>(F, G, sp + 1, qq), w = st<MIX>(F, G, sp + 2, qq);
+        const T pr = (T)F[sp + 3][qq];
         Uc[side][0] = rho;
         Uc[side][1] = rho * u;
         Uc[side][2] = rho * v;
@@ -113,7 +122,7 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
     const T gm = (T)P.gamma;
     // --- Scheme_U_ (MercurySolver_FluidRHS.cpp:3-202)
     T Fx[2][5];  // i-1/2 flux of the current cell, per species
-    for (int s = 0; s < 2; ++s) rusanov_face_t<MIX>(F, s ? SL_UN : SL_UH, s ? SL_PVN : SL_PVH, q0 - di, q0, q0, 0, gm, Fx[s]);
+    for (int s = 0; s < 2; ++s) rusanov_face_t<MIX>(F, G, s ? SL_UN : SL_UH, s ? SL_PVN : SL_PVH, q0 - di, q0, q0, 0, gm, Fx[s]);
     for (int i = i0; i < i1; ++i) {
         const long long q = q0 + (long long)(i - i0) * di;
         const T invJ = (T)(1.0 / F[SL_JAC][q]);
@@ -123,7 +132,7 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
             Acc<MIX> r[5];
             {
                 T Fp[5];
-                rusanov_face_t<MIX>(F, su, sp, q, q + di, q + di, 0, gm, Fp);
+                rusanov_face_t<MIX>(F, G, su, sp, q, q + di, q + di, 0, gm, Fp);
 #pragma unroll
                 for (int m = 0; m < 5; ++m) {
                     r[m].sub((Fp[m] - Fx[s][m]) * invJ);
@@ -133,8 +142,8 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
             auto axis = [&](int a) {
                 const long long d = AX(p, a);
                 T Fm[5], Fp[5];
-                rusanov_face_t<MIX>(F, su, sp, q - d, q, q, a, gm, Fm);
-                rusanov_face_t<MIX>(F, su, sp, q, q + d, q + d, a, gm, Fp);
+                rusanov_face_t<MIX>(F, G, su, sp, q - d, q, q, a, gm, Fm);
+                rusanov_face_t<MIX>(F, G, su, sp, q, q + d, q + d, a, gm, Fp);
 #pragma unroll
                 for (int m = 0; m < 5; ++m) r[m].sub((Fp[m] - Fm[m]) * invJ);
             };
@@ -148,7 +157,7 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
             for (int m = 0; m < 5; ++m) rr[s][m] = r[m].value();
         }
         // --- AddSourceToRHS_Fluid (MercurySolver_FluidRHS.cpp:206-527)
-        const double upx = F[SL_UP][q], upy = F[SL_UP + 1][q], upz = F[SL_UP + 2][q];
+        const double upx = st<MIX>(F, G, SL_UP, q), upy = st<MIX>(F, G, SL_UP + 1, q), upz = st<MIX>(F, G, SL_UP + 2, q);
         const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
         const double Jx = F[SL_JC][q], Jy = F[SL_JC + 1][q], Jz = F[SL_JC + 2][q];
         const double sjbx = Jy * Bz - Jz * By, sjby = Jz * Bx - Jx * Bz, sjbz = Jx * By - Jy * Bx;
@@ -182,7 +191,7 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
         for (int s = 0; s < 2; ++s) {
             const int su = s ? SL_UN : SL_UH, sp = s ? SL_PVN : SL_PVH, sr = s ? SL_RN : SL_RH;
             const double rho = dmax(F[su][q], 0.0);
-            const double u = F[sp][q], v = F[sp + 1][q], w = F[sp + 2][q];
+            const double u = st<MIX>(F, G, sp, q), v = st<MIX>(F, G, sp + 1, q), w = st<MIX>(F, G, sp + 2, q);
             const double u2 = u * u + v * v + w * w;
             const double subx = (v - upy) * Bz - (w - upz) * By;
             const double suby = (w - upz) * Bx - (u - upx) * Bz;
@@ -190,7 +199,7 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
             const double subu = subx * u + suby * v + subz * w;
             const double sjbu = sjbx * u + sjby * v + sjbz * w;
             const double dpeu = dpex * u + dpey * v + dpez * w;
-            const double Ts0 = F[sp + 4][q] * P.T_ref;
+            const double Ts0 = (double)st<MIX>(F, G, sp + 4, q) * P.T_ref;
             const double Ms = s ? P.M_Na : P.M_H;
             const double Mo = P.M_Na;
             const double b1 = vst * (P.Tn0 - Ts0) * (Ms / (Ms + Mo));
@@ -218,7 +227,8 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
             r4 -= P.a4 * rho * u2 * vst;
             r4 += P.a5 * ns * b1;
             r4 += P.a4 * rho * u2 * b2;
-            F[sr][q] = r0; F[sr + 1][q] = r1; F[sr + 2][q] = r2; F[sr + 3][q] = r3; F[sr + 4][q] = r4;
+            wr<MIX>(F, G, sr, q, r0); wr<MIX>(F, G, sr + 1, q, r1); wr<MIX>(F, G, sr + 2, q, r2);
+            wr<MIX>(F, G, sr + 3, q, r3); wr<MIX>(F, G, sr + 4, q, r4);
         }
     }
 }
@@ -231,7 +241,7 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
 template <bool MIX>
 __global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
                            const float *const *__restrict__ G, const MercPhys P, unsigned long long *__restrict__ dt_bits,
-                           int compute_dt) {
+                           int compute_dt, int hall64) {
     using T = typename Prec<MIX>::T;
     using PR = Prec<MIX>;
     double best = 1e100;
@@ -253,7 +263,10 @@ __global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, dou
             pr = dmax(pr, 1e-12);
             pv[s][0] = u; pv[s][1] = v; pv[s][2] = w; pv[s][3] = pr; pv[s][4] = pr / (rho * coeff);
             const int sp = s ? SL_PVN : SL_PVH;
-            for (int c = 0; c < 5; ++c) F[sp + c][q] = pv[s][c];
+            for (int c = 0; c < 5; ++c) {
+                if (c == 3) F[sp + 3][q] = pv[s][3];
+                else wr<MIX>(F, G, sp + c, q, pv[s][c]);
+            }
         }
         const double rH = dmax(F[SL_UH][q], 0.0), rN = dmax(F[SL_UN][q], 0.0);
         // one limiter serves U+ (floored densities) and the Hall terms (raw
@@ -264,7 +277,7 @@ __global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, dou
             const double uH = (rH > 1e-20) ? pv[0][c] : 0.0;
             const double uN = (rN > 1e-20) ? pv[1][c] : 0.0;
             up[c] = n.chiH * uH + n.chiNa * uN;
-            F[SL_UP + c][q] = up[c];
+            wr<MIX>(F, G, SL_UP + c, q, up[c]);
         }
         const Num &nh = n;
         const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
@@ -272,10 +285,19 @@ __global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, dou
             const T alpha = (T)(P.hall_coef / nh.ne_eff);
             const T Jx = (T)F[SL_JC][q], Jy = (T)F[SL_JC + 1][q], Jz = (T)F[SL_JC + 2][q];
             const T bx = (T)Bx, by = (T)By, bz = (T)Bz;
-            F[SL_EHC][q] = alpha * (Jy * bz - Jz * by);
-            F[SL_EHC + 1][q] = alpha * (Jz * bx - Jx * bz);
-            F[SL_EHC + 2][q] = alpha * (Jx * by - Jy * bx);
-            F[SL_HB][q] = fabs(alpha) * PR::sq(bx * bx + by * by + bz * bz);
+            const T e0 = alpha * (Jy * bz - Jz * by), e1 = alpha * (Jz * bx - Jx * bz), e2 = alpha * (Jx * by - Jy * bx);
+            const T hb = fabs(alpha) * PR::sq(bx * bx + by * by + bz * bz);
+            wr<MIX>(F, G, SL_EHC, q, e0);
+            wr<MIX>(F, G, SL_EHC + 1, q, e1);
+            wr<MIX>(F, G, SL_EHC + 2, q, e2);
+            wr<MIX>(F, G, SL_HB, q, hb);
+            if (MIX && hall64) {
+                // the per-function Hall edge pass of the periodic-seam path reads FP64
+                F[SL_EHC][q] = e0;
+                F[SL_EHC + 1][q] = e1;
+                F[SL_EHC + 2][q] = e2;
+                F[SL_HB][q] = hb;
+            }
         }
         // contravariant velocities (need the face of the next index on each axis)
         const int idx[3] = {p.i, p.j, p.k};
@@ -287,7 +309,7 @@ __global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, dou
             const T kx = T(0.5) * ((T)F[s0][q] + (T)F[s0][cp]);
             const T ky = T(0.5) * ((T)F[s0 + 1][q] + (T)F[s0 + 1][cp]);
             const T kz = T(0.5) * ((T)F[s0 + 2][q] + (T)F[s0 + 2][cp]);
-            F[SL_UCON + a][q] = (kx * (T)up[0] + ky * (T)up[1] + kz * (T)up[2]) * inv_jac;
+            wr<MIX>(F, G, SL_UCON + a, q, (kx * (T)up[0] + ky * (T)up[1] + kz * (T)up[2]) * inv_jac);
         }
         // time-step candidate on owned cells
         if (compute_dt && p.i >= 0 && p.j >= 0 && p.k >= 0 && p.i < bk.m[0] && p.j < bk.m[1] && p.k < bk.m[2]) {
@@ -353,23 +375,24 @@ __global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, dou
 
 // ---------------------------------------------------------------------------
 template <bool MIX>
-DEVI typename Prec<MIX>::T emf_flux_u(double *const *__restrict__ F, const Pt &p, int ba, int fa, long long L, long long Rr,
-                                      typename Prec<MIX>::T bff) {
+DEVI typename Prec<MIX>::T emf_flux_u(double *const *__restrict__ F, const float *const *__restrict__ G, const Pt &p, int ba, int fa,
+                                      long long L, long long Rr, typename Prec<MIX>::T bff) {
     using T = typename Prec<MIX>::T;
     const long long dL = AX(p, ba);
     const T btL = T(0.5) * (((T)F[SL_B + ba][L] + (T)F[SL_BADD + ba][L]) + ((T)F[SL_B + ba][L + dL] + (T)F[SL_BADD + ba][L + dL]));
     const T btR = T(0.5) * (((T)F[SL_B + ba][Rr] + (T)F[SL_BADD + ba][Rr]) + ((T)F[SL_B + ba][Rr + dL] + (T)F[SL_BADD + ba][Rr + dL]));
     const T biL = T(0.5) * ((T)F[SL_B + ba][L] + (T)F[SL_B + ba][L + dL]);
     const T biR = T(0.5) * ((T)F[SL_B + ba][Rr] + (T)F[SL_B + ba][Rr + dL]);
-    const T ufL = (T)F[SL_UCON + fa][L], ufR = (T)F[SL_UCON + fa][Rr];
-    const T ubL = (T)F[SL_UCON + ba][L], ubR = (T)F[SL_UCON + ba][Rr];
+    const T ufL = st<MIX>(F, G, SL_UCON + fa, L), ufR = st<MIX>(F, G, SL_UCON + fa, Rr);
+    const T ubL = st<MIX>(F, G, SL_UCON + ba, L), ubR = st<MIX>(F, G, SL_UCON + ba, Rr);
     const T GL = ufL * btL - ubL * bff, GR = ufR * btR - ubR * bff;
     const T radius = tmax(fabs(ufL), fabs(ufR));
     return T(0.5) * (GL + GR) - T(0.5) * radius * (biR - biL);
 }
 
 template <bool MIX>
-__global__ void k_eface_u(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F) {
+__global__ void k_eface_u(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
+                          const float *const *__restrict__ G) {
     using T = typename Prec<MIX>::T;
     LOCATE(R);
     const MercBlk &bk = B[p.b];
@@ -387,34 +410,36 @@ __global__ void k_eface_u(const MercRange R, const MercBlk *__restrict__ B, doub
         const T bff = (T)F[SL_B + fa][Rq] + (T)F[SL_BADD + fa][Rq];
         T e0 = T(0), e1 = T(0), e2 = T(0);
         if (fa == 0) {
-            const T Gex = emf_flux_u<MIX>(F, p, 1, 0, Lq, Rq, bff);
-            const T Gzx = emf_flux_u<MIX>(F, p, 2, 0, Lq, Rq, bff);
+            const T Gex = emf_flux_u<MIX>(F, G, p, 1, 0, Lq, Rq, bff);
+            const T Gzx = emf_flux_u<MIX>(F, G, p, 2, 0, Lq, Rq, bff);
             e1 = Gzx;
             e2 = -Gex;
         } else if (fa == 1) {
-            const T Gxe = emf_flux_u<MIX>(F, p, 0, 1, Lq, Rq, bff);
-            const T Gze = emf_flux_u<MIX>(F, p, 2, 1, Lq, Rq, bff);
+            const T Gxe = emf_flux_u<MIX>(F, G, p, 0, 1, Lq, Rq, bff);
+            const T Gze = emf_flux_u<MIX>(F, G, p, 2, 1, Lq, Rq, bff);
             e0 = -Gze;
             e2 = Gxe;
         } else {
-            const T Gxz = emf_flux_u<MIX>(F, p, 0, 2, Lq, Rq, bff);
-            const T Gez = emf_flux_u<MIX>(F, p, 1, 2, Lq, Rq, bff);
+            const T Gxz = emf_flux_u<MIX>(F, G, p, 0, 2, Lq, Rq, bff);
+            const T Gez = emf_flux_u<MIX>(F, G, p, 1, 2, Lq, Rq, bff);
             e0 = Gez;
             e1 = -Gxz;
         }
-        F[SL_EF + 3 * fa][Rq] = e0;
-        F[SL_EF + 3 * fa + 1][Rq] = e1;
-        F[SL_EF + 3 * fa + 2][Rq] = e2;
+        wr<MIX>(F, G, SL_EF + 3 * fa, Rq, e0);
+        wr<MIX>(F, G, SL_EF + 3 * fa + 1, Rq, e1);
+        wr<MIX>(F, G, SL_EF + 3 * fa + 2, Rq, e2);
     }
 }
 
-template <typename T>
-DEVI T maxu4_p(double *const *__restrict__ F, int ax, long long c0, long long c1, long long c2, long long c3) {
+template <bool MIX>
+DEVI typename Prec<MIX>::T maxu4_p(double *const *__restrict__ F, const float *const *__restrict__ G, int ax, long long c0, long long c1,
+                                   long long c2, long long c3) {
+    using T = typename Prec<MIX>::T;
     T a = T(0);
-    a = tmax(a, (T)fabs(F[SL_UCON + ax][c0]));
-    a = tmax(a, (T)fabs(F[SL_UCON + ax][c1]));
-    a = tmax(a, (T)fabs(F[SL_UCON + ax][c2]));
-    a = tmax(a, (T)fabs(F[SL_UCON + ax][c3]));
+    a = tmax(a, (T)fabs(st<MIX>(F, G, SL_UCON + ax, c0)));
+    a = tmax(a, (T)fabs(st<MIX>(F, G, SL_UCON + ax, c1)));
+    a = tmax(a, (T)fabs(st<MIX>(F, G, SL_UCON + ax, c2)));
+    a = tmax(a, (T)fabs(st<MIX>(F, G, SL_UCON + ax, c3)));
     return a;
 }
 
@@ -435,27 +460,28 @@ __global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, d
     const bool arti = P.arti_eta_max > 0.0;
     const bool local = P.local_enabled != 0.0 && P.local_eta_max > 0.0 && P.local_r_decay > 0.0;
 #define FB(s, o) ((T)F[(s)][q + (o)])
+#define FE(s, o) (st<MIX>(F, G, (s), q + (o)))
     for (int a = 0; a < 3; ++a) {
         if (idx[a] >= bk.m[a]) continue;
         T E;
         if (a == 0) {
-            const T Ec = T(0.25) * (FB(SL_EF + 3, 0) + FB(SL_EF + 3, -dk) + FB(SL_EF + 6, 0) + FB(SL_EF + 6, -dj));
-            const T ae = maxu4_p<T>(F, 1, q, q - dj, q - dk, q - dj - dk);
-            const T az = maxu4_p<T>(F, 2, q, q - dj, q - dk, q - dj - dk);
+            const T Ec = T(0.25) * (FE(SL_EF + 3, 0) + FE(SL_EF + 3, -dk) + FE(SL_EF + 6, 0) + FE(SL_EF + 6, -dj));
+            const T ae = maxu4_p<MIX>(F, G, 1, q, q - dj, q - dk, q - dj - dk);
+            const T az = maxu4_p<MIX>(F, G, 2, q, q - dj, q - dk, q - dj - dk);
             const T dBz = FB(SL_B + 2, 0) - FB(SL_B + 2, -dj);
             const T dBe = FB(SL_B + 1, 0) - FB(SL_B + 1, -dk);
             E = Ec + Cuct * (T(0.5) * ae * dBz - T(0.5) * az * dBe);
         } else if (a == 1) {
-            const T Ec = T(0.25) * (FB(SL_EF + 1, 0) + FB(SL_EF + 1, -dk) + FB(SL_EF + 7, 0) + FB(SL_EF + 7, -di));
-            const T az = maxu4_p<T>(F, 2, q, q - di, q - dk, q - di - dk);
-            const T ax = maxu4_p<T>(F, 0, q, q - di, q - dk, q - di - dk);
+            const T Ec = T(0.25) * (FE(SL_EF + 1, 0) + FE(SL_EF + 1, -dk) + FE(SL_EF + 7, 0) + FE(SL_EF + 7, -di));
+            const T az = maxu4_p<MIX>(F, G, 2, q, q - di, q - dk, q - di - dk);
+            const T ax = maxu4_p<MIX>(F, G, 0, q, q - di, q - dk, q - di - dk);
             const T dBx = FB(SL_B + 0, 0) - FB(SL_B + 0, -dk);
             const T dBz = FB(SL_B + 2, 0) - FB(SL_B + 2, -di);
             E = Ec + Cuct * (T(0.5) * az * dBx - T(0.5) * ax * dBz);
         } else {
-            const T Ec = T(0.25) * (FB(SL_EF + 2, 0) + FB(SL_EF + 2, -dj) + FB(SL_EF + 5, 0) + FB(SL_EF + 5, -di));
-            const T ax = maxu4_p<T>(F, 0, q, q - di, q - dj, q - di - dj);
-            const T ae = maxu4_p<T>(F, 1, q, q - di, q - dj, q - di - dj);
+            const T Ec = T(0.25) * (FE(SL_EF + 2, 0) + FE(SL_EF + 2, -dj) + FE(SL_EF + 5, 0) + FE(SL_EF + 5, -di));
+            const T ax = maxu4_p<MIX>(F, G, 0, q, q - di, q - dj, q - di - dj);
+            const T ae = maxu4_p<MIX>(F, G, 1, q, q - di, q - dj, q - di - dj);
             const T dBe = FB(SL_B + 1, 0) - FB(SL_B + 1, -di);
             const T dBx = FB(SL_B + 0, 0) - FB(SL_B + 0, -dj);
             E = Ec + Cuct * (T(0.5) * ax * dBe - T(0.5) * ae * dBx);
@@ -513,10 +539,11 @@ __global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, d
         else { c[0] = q - di - dj; c[1] = q - di; c[2] = q - dj; c[3] = q; }
         T Ecc[3];
         for (int d = 0; d < 3; ++d)
-            Ecc[d] = T(0.25) * ((T)F[SL_EHC + d][c[0]] + (T)F[SL_EHC + d][c[1]] + (T)F[SL_EHC + d][c[2]] + (T)F[SL_EHC + d][c[3]]);
+            Ecc[d] = T(0.25) * (st<MIX>(F, G, SL_EHC + d, c[0]) + st<MIX>(F, G, SL_EHC + d, c[1]) + st<MIX>(F, G, SL_EHC + d, c[2]) +
+                                st<MIX>(F, G, SL_EHC + d, c[3]));
         const T Ecen = Ecc[0] * st<MIX>(F, G, SL_DR + 3 * a, q) + Ecc[1] * st<MIX>(F, G, SL_DR + 3 * a + 1, q) +
                        Ecc[2] * st<MIX>(F, G, SL_DR + 3 * a + 2, q);
-        const T be = tmax(tmax((T)F[SL_HB][c[0]], (T)F[SL_HB][c[1]]), tmax((T)F[SL_HB][c[2]], (T)F[SL_HB][c[3]]));
+        const T be = tmax(tmax(st<MIX>(F, G, SL_HB, c[0]), st<MIX>(F, G, SL_HB, c[1])), tmax(st<MIX>(F, G, SL_HB, c[2]), st<MIX>(F, G, SL_HB, c[3])));
         const T L = tmax(st<MIX>(F, G, SL_DL + a, q), eps);
 #define AR(s, o) st<MIX>(F, G, (s), q + (o))
         T val;
@@ -563,18 +590,22 @@ __global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, d
         F[SL_E + a][q] = (double)e;
     }
 #undef FB
+#undef FE
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_update_all(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, const double *__restrict__ dtp) {
+// (the RHS comes from the FP32 table in the mixed form; the accumulation is FP64)
+template <bool MIX>
+__global__ void k_update_all(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
+                             const float *const *__restrict__ G, const double *__restrict__ dtp) {
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
     const double dt = *dtp;
     if (p.i < bk.m[0] && p.j < bk.m[1] && p.k < bk.m[2])
         for (int m = 0; m < 5; ++m) {
-            F[SL_UH + m][q] += dt * F[SL_RH + m][q];
-            F[SL_UN + m][q] += dt * F[SL_RN + m][q];
+            F[SL_UH + m][q] += dt * (double)st<MIX>(F, G, SL_RH + m, q);
+            F[SL_UN + m][q] += dt * (double)st<MIX>(F, G, SL_RN + m, q);
         }
     if (p.j < bk.m[1] && p.k < bk.m[2]) {
         const double c = (F[SL_E + 1][q] - F[SL_E + 1][q + dk]) + (F[SL_E + 2][q + dj] - F[SL_E + 2][q]);
@@ -667,6 +698,10 @@ __global__ void k_jcell_t(const MercRange R, const MercBlk *__restrict__ B, doub
 }
 
 // FP32 copy of a static FP64 slot (mixed form)
+__global__ void k_to_double(const float *__restrict__ src, double *__restrict__ dst, long long n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = (double)src[t];
+}
 __global__ void k_to_float(const double *__restrict__ src, float *__restrict__ dst, long long n) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) dst[t] = (float)src[t];
@@ -695,21 +730,28 @@ void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *
     else LAUNCH_R(k_rhs_cell<false>, R, B, F, G, P, L);
 }
 void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
-                     unsigned long long *dt_next, int compute_dt, bool mix) {
+                     unsigned long long *dt_next, int compute_dt, bool mix, int hall64) {
     if (compute_dt) hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, st, dt_next);
-    if (mix) LAUNCH_R(k_cell_end<true>, R, B, F, G, P, dt_next, compute_dt);
-    else LAUNCH_R(k_cell_end<false>, R, B, F, G, P, dt_next, compute_dt);
+    if (mix) LAUNCH_R(k_cell_end<true>, R, B, F, G, P, dt_next, compute_dt, hall64);
+    else LAUNCH_R(k_cell_end<false>, R, B, F, G, P, dt_next, compute_dt, hall64);
 }
-void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, bool mix) {
-    if (mix) LAUNCH_R(k_eface_u<true>, R, B, F);
-    else LAUNCH_R(k_eface_u<false>, R, B, F);
+void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix) {
+    if (mix) LAUNCH_R(k_eface_u<true>, R, B, F, G);
+    else LAUNCH_R(k_eface_u<false>, R, B, F, G);
 }
 void launch_edge_total(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                        int with_hall, bool mix) {
     if (mix) LAUNCH_R(k_edge_total<true>, R, B, F, G, P, with_hall);
     else LAUNCH_R(k_edge_total<false>, R, B, F, G, P, with_hall);
 }
-void launch_update_all(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const double *dt) { LAUNCH_R(k_update_all, R, B, F, dt); }
+void launch_update_all(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const double *dt,
+                       bool mix) {
+    if (mix) LAUNCH_R(k_update_all<true>, R, B, F, G, dt);
+    else LAUNCH_R(k_update_all<false>, R, B, F, G, dt);
+}
+void launch_to_double(hipStream_t st, const float *src, double *dst, long long n) {
+    if (n > 0) hipLaunchKernelGGL(k_to_double, dim3(nblk(n, 256)), dim3(256), 0, st, src, dst, n);
+}
 void launch_bcell_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix) {
     if (mix) LAUNCH_R(k_bcell_jedge<true>, R, B, F, G);
     else LAUNCH_R(k_bcell_jedge<false>, R, B, F, G);

@@ -554,7 +554,8 @@ public:
     int form = 1;                             // 0: one kernel per reference function, 1: fused FP64, 2: mixed
     float *farena = nullptr;                  // FP32 copies of the static arrays (mixed form)
     float **fptr = nullptr;                   // device table indexed like the FP64 slots (null where absent)
-    std::vector<int> fslots;
+    std::vector<int> fslots;                  // FP32 static copies
+    std::vector<int> islots;                  // FP32 derived intermediates of the mixed form
     unsigned long long *d_red = nullptr;      // reduction scratch (2 words)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -622,9 +623,24 @@ public:
         }
         for (int c = 0; c < 9; ++c) fslots.push_back(SL_DR + c);
         fslots.push_back(SL_PHOTO);
-        HIPCHK(hipMalloc(&farena, sizeof(float) * F->Vtot * fslots.size()));
+        // FP32 derived intermediates of the mixed form (written by its passes):
+        // PV u, v, w, T (the pressure stays FP64), U+, Hall cell terms,
+        // contravariant U+, face EMFs, fluid RHS
+        for (int sp : {SL_PVH, SL_PVN})
+            for (int c : {0, 1, 2, 4}) islots.push_back(sp + c);
+        for (int c = 0; c < 3; ++c) islots.push_back(SL_UP + c);
+        for (int c = 0; c < 3; ++c) islots.push_back(SL_EHC + c);
+        islots.push_back(SL_HB);
+        for (int c = 0; c < 3; ++c) islots.push_back(SL_UCON + c);
+        for (int c = 0; c < 9; ++c) islots.push_back(SL_EF + c);
+        for (int c = 0; c < 5; ++c) islots.push_back(SL_RH + c);
+        for (int c = 0; c < 5; ++c) islots.push_back(SL_RN + c);
+        const size_t nf = fslots.size() + islots.size();
+        HIPCHK(hipMalloc(&farena, sizeof(float) * F->Vtot * nf));
+        HIPCHK(hipMemset(farena, 0, sizeof(float) * F->Vtot * nf));
         std::vector<float *> hf(SL_NSLOT, nullptr);
         for (size_t n = 0; n < fslots.size(); ++n) hf[fslots[n]] = farena + (long long)n * F->Vtot;
+        for (size_t n = 0; n < islots.size(); ++n) hf[islots[n]] = farena + (long long)(fslots.size() + n) * F->Vtot;
         HIPCHK(hipMalloc(&fptr, sizeof(float *) * SL_NSLOT));
         HIPCHK(hipMemcpy(fptr, hf.data(), sizeof(float *) * SL_NSLOT, hipMemcpyHostToDevice));
         fptr_h = hf;
@@ -652,6 +668,16 @@ public:
 
     double *const *Fd() const { return F->dptr; }
     const float *const *Gd() const { return fptr; }
+    int hall64() const { return (mix() && alias.n > 0) ? 1 : 0; }
+    // the FP64 copy of an intermediate the mixed form keeps in FP32 (downloads)
+    void materialize(int slot) {
+        if (form != 2) return;
+        for (int s : islots)
+            if (s == slot) {
+                launch_to_double(st, fptr_h[s], F->hptr[s], F->Vtot);
+                HIPCHK(hipStreamSynchronize(st));
+            }
+    }
     std::vector<float *> fptr_h;
     bool mix() const { return form == 2; }
     void set_march(int L) {
@@ -1052,17 +1078,17 @@ public:
         run_merged(MG_DERIVED_PAIR);
         launch_jcell_t(st, R_cell.h, Bd(), Fd(), Gd(), mix());
         sync(G_JCELL);
-        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, final ? 1 : 0, mix());
+        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, final ? 1 : 0, mix(), hall64());
     }
 
     // StepOnce without the time step, fused passes
     void stage_fused(bool final) {
         launch_rhs_cell(st, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
-        launch_eface_u(st, R_eface.h, Bd(), Fd(), mix());
+        launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix());
         if (alias.n == 0) {
             launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, 1, mix());
             run_merged(MG_EDGE_PAIR);
-            launch_update_all(st, R_node.h, Bd(), Fd(), dtp());
+            launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix());
             run_merged(MG_STATE_PAIR);
             if (res_now) res_solve(res_dt);
             derived_fused(final);
@@ -1076,7 +1102,7 @@ public:
             launch_add_hall_wall(st, R_node.h, Bd(), Fd());
         }
         sync(G_EEDGE);
-        launch_update_all(st, R_node.h, Bd(), Fd(), dtp());
+        launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix());
         run_merged(MG_STATE_PAIR);
         if (res_now) res_solve(res_dt);
         derived_fused(final);
@@ -1242,7 +1268,7 @@ public:
         run_merged(MG_PREPARE_PAIR);
         derived();
         refresh_float_statics();
-        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix());
+        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix(), hall64());
         HIPCHK(hipStreamSynchronize(st));
     }
 
@@ -1251,7 +1277,7 @@ public:
         form = f;
         if (f >= 1) {
             refresh_float_statics();
-            launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix());
+            launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix(), hall64());
         }
         if (gexec) {
             (void)hipGraphExecDestroy(gexec);
@@ -1283,10 +1309,10 @@ public:
         else if (op == "timestep") launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt);
         else if (op == "stage") stage();
         else if (op == "rhs_cell") launch_rhs_cell(st, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
-        else if (op == "cell_end") launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix());
-        else if (op == "eface_u") launch_eface_u(st, R_eface.h, Bd(), Fd(), mix());
+        else if (op == "cell_end") launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix(), hall64());
+        else if (op == "eface_u") launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix());
         else if (op == "edge_total") launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, alias.n == 0 ? 1 : 0, mix());
-        else if (op == "update_all") launch_update_all(st, R_node.h, Bd(), Fd(), dtp());
+        else if (op == "update_all") launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix());
         else if (op == "bcell_jedge") launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix());
         else if (op == "jcell_t") launch_jcell_t(st, R_cell.h, Bd(), Fd(), Gd(), mix());
         else if (op == "sync_all") {
@@ -1446,7 +1472,9 @@ int mercury_download(MercuryContext *ctx, const char *name, int block, int comp,
     ABI_TRY({
         if (block < 0 || block >= ctx->s->F->nb() || !host) throw std::runtime_error("invalid block or buffer");
         HIPCHK(hipStreamSynchronize(ctx->s->st));
-        ctx->s->F->download_block(field_comp_slot(name, comp), block, host);
+        const int slot = field_comp_slot(name, comp);
+        ctx->s->materialize(slot);
+        ctx->s->F->download_block(slot, block, host);
     })
 }
 
