@@ -184,7 +184,7 @@ def main():
         g = Mercury(case)
         g.prepare()
         g.set_form(form)
-        info = {}
+        info = {"march": g.autotune() if form >= 1 else None}
         if oref is not None:
             g.step(args.cpu_sample_steps, case.integrator)
             err, emax = {}, {}

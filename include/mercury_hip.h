@@ -159,6 +159,12 @@ int mercury_ensemble_create(MercuryContext *const *ctxs, int n, int integrator, 
 int mercury_ensemble_launch(MercuryEnsemble *ens, int nsteps);
 int mercury_ensemble_destroy(MercuryEnsemble *ens);
 
+/* Launch-shape autotuner (no reference counterpart): times the flux pass for
+ * several i-march lengths in the current form and keeps the fastest; results
+ * are unchanged by the choice.  Call after mercury_set_form; it drops a
+ * captured step graph (capture again afterwards).  Writes the chosen length. */
+int mercury_autotune(MercuryContext *ctx, int *march_out);
+
 /* Average device time (ms) of an op over reps launches, HIP events on the
  * solver's stream; op "step_graph" replays the captured step. */
 int mercury_time_op(MercuryContext *ctx, const char *op, int reps, double *ms_avg);

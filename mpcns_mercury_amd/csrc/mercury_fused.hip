@@ -317,46 +317,49 @@ __global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, dou
             if (!(V <= 0.0)) {
                 // per species state first, then one pass over the six faces:
                 // the face normal and |S| (the stored area, bitwise the same
-                // sqrt) are shared by the species
-                const double B2 = Bx * Bx + By * By + Bz * Bz;
-                double rho[2], ux[2], uy[2], uz[2], cs2[2], vA2[2], denom[2] = {0.0, 0.0};
+                // sqrt) are shared by the species.  FP64 in the FP64 forms; the
+                // mixed form evaluates the candidate in FP32 (the global minimum
+                // stays an exact reduction of FP64 values)
+                const T bx = (T)Bx, by = (T)By, bz = (T)Bz, gam = (T)P.gamma, mi = (T)P.inver_MA2;
+                const T B2 = bx * bx + by * by + bz * bz;
+                T rho[2], ux[2], uy[2], uz[2], cs2[2], vA2[2], denom[2] = {T(0), T(0)};
                 for (int s = 0; s < 2; ++s) {
                     const int su = s ? SL_UN : SL_UH;
-                    rho[s] = dmax(F[su][q], 1e-2);
-                    ux[s] = F[su + 1][q] / rho[s];
-                    uy[s] = F[su + 2][q] / rho[s];
-                    uz[s] = F[su + 3][q] / rho[s];
-                    const double E = F[su + 4][q];
-                    const double ke = 0.5 * rho[s] * (ux[s] * ux[s] + uy[s] * uy[s] + uz[s] * uz[s]);
-                    const double pr = (P.gamma - 1.0) * (E - ke);
-                    cs2[s] = P.gamma * pr / rho[s];
-                    vA2[s] = P.inver_MA2 * B2 / rho[s];
+                    rho[s] = tmax((T)F[su][q], T(1e-2));
+                    ux[s] = PR::div((T)F[su + 1][q], rho[s]);
+                    uy[s] = PR::div((T)F[su + 2][q], rho[s]);
+                    uz[s] = PR::div((T)F[su + 3][q], rho[s]);
+                    const T E = (T)F[su + 4][q];
+                    const T ke = T(0.5) * rho[s] * (ux[s] * ux[s] + uy[s] * uy[s] + uz[s] * uz[s]);
+                    const T pr = (gam - T(1)) * (E - ke);
+                    cs2[s] = PR::div(gam * pr, rho[s]);
+                    vA2[s] = PR::div(mi * B2, rho[s]);
                 }
                 for (int a = 0; a < 3; ++a)
                     for (int sd = 0; sd < 2; ++sd) {
                         const long long fq = q + (sd ? AX(p, a) : 0LL);
-                        const double ax = F[SL_JD + 3 * a][fq], ay = F[SL_JD + 3 * a + 1][fq], az = F[SL_JD + 3 * a + 2][fq];
-                        const double A = F[SL_AREA + a][fq];
-                        if (A <= 0.0) {
-                            denom[0] += 0.0;
-                            denom[1] += 0.0;
+                        const T ax = (T)F[SL_JD + 3 * a][fq], ay = (T)F[SL_JD + 3 * a + 1][fq], az = (T)F[SL_JD + 3 * a + 2][fq];
+                        const T A = st<MIX>(F, G, SL_AREA + a, fq);
+                        if (A <= T(0)) {
+                            denom[0] += T(0);
+                            denom[1] += T(0);
                             continue;
                         }
-                        double nx = ax / A, ny = ay / A, nz = az / A;
+                        T nx = PR::div(ax, A), ny = PR::div(ay, A), nz = PR::div(az, A);
                         if (!sd) { nx = -nx; ny = -ny; nz = -nz; }
-                        const double Bn = Bx * nx + By * ny + Bz * nz;
+                        const T Bn = bx * nx + by * ny + bz * nz;
                         for (int s = 0; s < 2; ++s) {
-                            const double un = ux[s] * nx + uy[s] * ny + uz[s] * nz;
-                            const double vAn2 = P.inver_MA2 * (Bn * Bn) / rho[s];
-                            const double t = cs2[s] + vA2[s];
-                            double disc = t * t - 4.0 * cs2[s] * vAn2;
-                            if (disc < 0.0) disc = 0.0;
-                            const double cf = sqrt(0.5 * (t + sqrt(disc)));
+                            const T un = ux[s] * nx + uy[s] * ny + uz[s] * nz;
+                            const T vAn2 = PR::div(mi * (Bn * Bn), rho[s]);
+                            const T t = cs2[s] + vA2[s];
+                            T disc = t * t - T(4) * cs2[s] * vAn2;
+                            if (disc < T(0)) disc = T(0);
+                            const T cf = PR::sq(T(0.5) * (t + PR::sq(disc)));
                             denom[s] += (fabs(un) + cf) * A;
                         }
                     }
                 for (int s = 0; s < 2; ++s)
-                    if (denom[s] > 0.0) best = dmin(best, P.cfl * V / denom[s]);
+                    if (denom[s] > T(0)) best = dmin(best, P.cfl * V / (double)denom[s]);
             }
         }
     }

@@ -680,6 +680,27 @@ public:
     }
     std::vector<float *> fptr_h;
     bool mix() const { return form == 2; }
+    // Launch-shape autotuner: times the flux pass (HIP events on the solver
+    // stream) for march lengths 1, 2, 4, 8 in the current form and keeps the
+    // fastest; the choice never changes a result.  Returns the chosen length.
+    int autotune() {
+        int best = march;
+        double tbest = 1e300;
+        for (int L : {1, 2, 4, 8}) {
+            set_march(L);
+            const double t = time_op("rhs_cell", 5);
+            if (t < tbest) tbest = t, best = L;
+        }
+        set_march(best);
+        if (gexec) {  // a captured step keeps the old launch shape
+            (void)hipGraphExecDestroy(gexec);
+            (void)hipGraphDestroy(graph);
+            gexec = nullptr;
+            graph = nullptr;
+        }
+        return best;
+    }
+
     void set_march(int L) {
         march = L;
         std::vector<int> lo(3 * F->nb(), 0), n(3 * F->nb());
@@ -1623,6 +1644,14 @@ int mercury_resist_stats(MercuryContext *ctx, int *n_unknowns, int *iterations, 
         if (n_unknowns) *n_unknowns = ctx->s->resist ? ctx->s->rd.n : 0;
         if (iterations) *iterations = ctx->s->res_its;
         if (rel_residual) *rel_residual = ctx->s->res_rel;
+    })
+}
+
+int mercury_autotune(MercuryContext *ctx, int *march_out) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        const int L = ctx->s->autotune();
+        if (march_out) *march_out = L;
     })
 }
 

@@ -22,7 +22,7 @@ ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercu
                "mercury_init_bface", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
                "mercury_divb", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
                "mercury_time_op", "mercury_geometry_block", "mercury_set_form", "mercury_resist_stats",
-               "mercury_ensemble_create", "mercury_ensemble_launch", "mercury_ensemble_destroy")
+               "mercury_ensemble_create", "mercury_ensemble_launch", "mercury_ensemble_destroy", "mercury_autotune")
 
 
 class MercuryError(RuntimeError):
@@ -63,6 +63,7 @@ def load():
     L.mercury_ensemble_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
     L.mercury_ensemble_launch.argtypes = [vp, ctypes.c_int]
     L.mercury_ensemble_destroy.argtypes = [vp]
+    L.mercury_autotune.argtypes = [vp, ip]
     L.mercury_geometry_block.argtypes = [ip, ctypes.c_int, dp, ctypes.POINTER(MercuryPhysics), cp, ctypes.c_int, dp]
     _lib = L
     return L
@@ -194,6 +195,12 @@ class Mercury:
         n, it, rel = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         _chk(load().mercury_resist_stats(self._h, ctypes.byref(n), ctypes.byref(it), ctypes.byref(rel)))
         return n.value, it.value, rel.value
+
+    def autotune(self):
+        """Pick the fastest launch shape of the flux pass; returns the march length."""
+        v = ctypes.c_int()
+        _chk(load().mercury_autotune(self._h, ctypes.byref(v)))
+        return v.value
 
     def time_op(self, op, reps=10):
         v = ctypes.c_double()

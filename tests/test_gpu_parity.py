@@ -156,6 +156,18 @@ def test_hundred_steps_bit_exact(oracle_mod, integrator):
         g.close()
 
 
+def test_autotune_keeps_results(pair):
+    """The launch-shape autotuner changes no result: steps after it equal the oracle."""
+    case, g, o = pair
+    g.prepare()
+    o.prepare()
+    g.set_form(1)
+    L = g.autotune()
+    assert L in (1, 2, 4, 8)
+    np.testing.assert_array_equal(g.step(3, 0), o.step(3, 0))
+    _cmp(g, o, STATE + DERIVED, f"3 steps after autotune (march {L})")
+
+
 @pytest.mark.parametrize("integrator", [0, 1])
 def test_graph_replay_equals_oracle(pair, integrator):
     """A captured step replayed from a HIP graph gives the oracle's state."""
