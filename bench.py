@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the Mercury explicit step on one MI355X.
 
-Workload (default): two-fluid Hall MHD, FP64 plain GPU form -- H+ and Na+ plus
+Workload (default): two-fluid Hall MHD -- H+ and Na+ plus
 electron pressure on six spherical-shell sector blocks of 96x64x64, 128x48x64,
 64x96x80, 80x80x48, 112x56x72 and 72x72x96 cells (2 534 400 cells), r in [1,4],
 inflow u = (-1,0,0), IMF (0,0,-0.2), conducting inner sphere, far field
@@ -9,10 +9,11 @@ elsewhere, seeded 1e-3 density noise.  One "step" is one explicit step of every
 block (forward Euler, the reference integrator), replayed from a captured HIP
 graph with all inputs resident in HBM.
 
-Prints one JSON line:  value = cell-updates/s of the whole job; roofline = the
-dominant kernel's algorithmic HBM bytes / its average duration (HIP events on the
-solver stream); cpu_baseline = the CPU restatement (single thread) on a bounded
-sample of the same workload.
+Prints one JSON line:  value = cell-updates/s of the whole job in the fused FP64
+form (bit-identical to the CPU restatement); the plain FP64 and the mixed forms
+are reported beside it.  roofline = the dominant kernel's algorithmic HBM bytes /
+its average duration (HIP events on the solver stream); cpu_baseline = the CPU
+restatement (OpenMP threads) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -109,7 +110,9 @@ def rel_l2(a, b):
 
 
 def cpu_baseline(case, sample_steps: int):
-    """Time the CPU restatement (single thread) on `sample_steps` full steps."""
+    """Time the CPU restatement on `sample_steps` full steps; its per-point loops
+    run on OpenMP threads (OMP_NUM_THREADS, bit-identical to one thread).
+    Returns the oracle, the elapsed seconds and the thread count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # noqa: E402
     oracle.build()
@@ -118,7 +121,7 @@ def cpu_baseline(case, sample_steps: int):
     t0 = time.perf_counter()
     o.step(sample_steps, case.integrator)
     dt = time.perf_counter() - t0
-    return o, dt
+    return o, dt, int(oracle.lib().orc_threads())
 
 
 def main():
@@ -127,7 +130,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="shell6")
-    ap.add_argument("--cpu-sample-steps", type=int, default=4)
+    ap.add_argument("--cpu-sample-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-ops", action="store_true", help="print per-op timings of the fused forms to stderr")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -164,10 +167,10 @@ def main():
     # CPU oracle sample: the baseline timing and the reference state for the error of each form
     cpu, oref = None, None
     if not args.no_cpu:
-        oref, tcpu = cpu_baseline(case, args.cpu_sample_steps)
-        cpu = {"value": ncells * args.cpu_sample_steps / tcpu, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_sample_steps} full steps of {case.name} ({ncells} cells), CPU oracle, 1 thread, "
-                         f"{tcpu:.1f} s"}
+        oref, tcpu, nthr = cpu_baseline(case, args.cpu_sample_steps)
+        cpu = {"value": ncells * args.cpu_sample_steps / tcpu, "unit": "cell-updates/s", "cores": nthr, "kind": "port",
+               "sample": f"{args.cpu_sample_steps} full steps of {case.name} ({ncells} cells), CPU oracle, "
+                         f"{nthr} OpenMP threads, {tcpu:.1f} s"}
         print(f"[bench] cpu sample {tcpu:.1f}s -> {cpu['value']:.3e} cell-updates/s", file=sys.stderr, flush=True)
 
     forms = {}

@@ -26,6 +26,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 
 /* ------------------------------------------------------------------------- */
 /* physics constants, same layout as MercuryPhysics in include/mercury_hip.h   */
@@ -112,6 +113,16 @@ typedef struct {
 #define XN(b, i, j, k) ((b)->x[NX(b, i, j, k)])
 #define YN(b, i, j, k) ((b)->y[NX(b, i, j, k)])
 #define ZN(b, i, j, k) ((b)->z[NX(b, i, j, k)])
+
+/* Threading: every per-point loop marked PAR2 / PAR1 writes each output element
+ * from inputs held in other slots, so its iterations are independent and a
+ * threaded run gives the same bits as a serial one.  PAR2 splits the two outer
+ * levels (i, j) of a perfectly nested i/j/k loop, PAR1 a flat loop over the
+ * box.  Halo / boundary fills, alias means and the resistive CG (whose dot
+ * products would change with the summation order) stay serial.  The thread
+ * count is OpenMP's (OMP_NUM_THREADS). */
+#define PAR2 _Pragma("omp parallel for collapse(2) schedule(static)")
+#define PAR1 _Pragma("omp parallel for schedule(static)")
 
 static inline double dmax(double a, double b) { return (a < b) ? b : a; } /* std::max */
 static inline double dmin(double a, double b) { return (b < a) ? b : a; } /* std::min */
@@ -773,6 +784,8 @@ static void corner_default(OBlock *b, const GField *gf) {
     owned_hi(b, gf->loc, ohi);
     int lo[3], hi[3];
     alloc_box(b, gf->loc, b->g, lo, hi);
+    /* writes corner ghosts only, reads owned points only */
+    PAR2
     for (int i = lo[0]; i < hi[0]; ++i)
         for (int j = lo[1]; j < hi[1]; ++j)
             for (int k = lo[2]; k < hi[2]; ++k) {
@@ -800,6 +813,10 @@ static void periodic_halo(OBlock *b, const GField *gf) {
     int ohi[3], lo[3], hi[3];
     owned_hi(b, gf->loc, ohi);
     alloc_box(b, gf->loc, b->g, lo, hi);
+    /* with more cells than ghost layers on every axis each source is an owned
+     * point, so the ghost writes do not feed each other and may run in parallel */
+    const int par = b->m[0] > b->g && b->m[1] > b->g && b->m[2] > b->g;
+#pragma omp parallel for collapse(2) schedule(static) if (par)
     for (int i = lo[0]; i < hi[0]; ++i)
         for (int j = lo[1]; j < hi[1]; ++j)
             for (int k = lo[2]; k < hi[2]; ++k) {
@@ -824,6 +841,8 @@ static void periodic_halo(OBlock *b, const GField *gf) {
 static void owner_sync(OBlock *b, const GField *gf) {
     int ohi[3];
     owned_hi(b, gf->loc, ohi);
+    /* writes seam aliases (index m) from their owners (index 0) only */
+    PAR2
     for (int i = 0; i < ohi[0]; ++i)
         for (int j = 0; j < ohi[1]; ++j)
             for (int k = 0; k < ohi[2]; ++k) {
@@ -936,6 +955,7 @@ static void calc_pv(Orc *o) {
         for (int s = 0; s < 2; ++s) {
             int su = s ? S_UN : S_UH, sp = s ? S_PVN : S_PVH;
             double coeff = s ? o->P.state_coeff_Na : o->P.state_coeff_H;
+            PAR1
             for (long q = 0; q < b->V; ++q) {
                 const double rho = dmax(b->f[su][q], 1e-12);
                 const double u = b->f[su + 1][q] / rho, v = b->f[su + 2][q] / rho, w = b->f[su + 3][q] / rho;
@@ -959,6 +979,7 @@ static void calc_pv(Orc *o) {
 static void calc_uplus(Orc *o) {
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
+        PAR1
         for (long q = 0; q < b->V; ++q) {
             const double rH = dmax(b->f[S_UH][q], 0.0), rN = dmax(b->f[S_UN][q], 0.0);
             double uH[3], uN[3];
@@ -976,6 +997,7 @@ static void calc_uplus(Orc *o) {
 static void calc_bcell(Orc *o) {
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
+        PAR2
         for (int i = 0; i < b->m[0]; ++i)
             for (int j = 0; j < b->m[1]; ++j)
                 for (int k = 0; k < b->m[2]; ++k) {
@@ -1017,6 +1039,7 @@ static void adj_curl(Orc *o, int sf, int se) {
         int hi[3];
 #define BB(a, i, j, k) (F(b, S_BETA + (a), i, j, k) * F(b, sf + (a), i, j, k))
         owned_hi(b, L_EX, hi);
+        PAR2
         for (int i = 0; i < hi[0]; ++i)
             for (int j = 0; j < hi[1]; ++j)
                 for (int k = 0; k < hi[2]; ++k) {
@@ -1025,6 +1048,7 @@ static void adj_curl(Orc *o, int sf, int se) {
                     F(b, se + 0, i, j, k) *= F(b, S_ALPHA + 0, i, j, k);
                 }
         owned_hi(b, L_EY, hi);
+        PAR2
         for (int i = 0; i < hi[0]; ++i)
             for (int j = 0; j < hi[1]; ++j)
                 for (int k = 0; k < hi[2]; ++k) {
@@ -1033,6 +1057,7 @@ static void adj_curl(Orc *o, int sf, int se) {
                     F(b, se + 1, i, j, k) *= F(b, S_ALPHA + 1, i, j, k);
                 }
         owned_hi(b, L_EZ, hi);
+        PAR2
         for (int i = 0; i < hi[0]; ++i)
             for (int j = 0; j < hi[1]; ++j)
                 for (int k = 0; k < hi[2]; ++k) {
@@ -1051,6 +1076,7 @@ static void calc_jedge(Orc *o) { adj_curl(o, S_B, S_J); }
 static void calc_jcell(Orc *o) {
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
+        PAR2
         for (int i = 0; i < b->m[0]; ++i)
             for (int j = 0; j < b->m[1]; ++j)
                 for (int k = 0; k < b->m[2]; ++k) {
@@ -1087,6 +1113,8 @@ static double compute_timestep(Orc *o) {
         for (int ib = 0; ib < o->nb; ++ib) {
             OBlock *b = &o->b[ib];
             int su = s ? S_UN : S_UH;
+            /* the minimum is exact, so the reduction order does not matter */
+#pragma omp parallel for collapse(2) schedule(static) reduction(min : dt_local)
             for (int i = 0; i < b->m[0]; ++i)
                 for (int j = 0; j < b->m[1]; ++j)
                     for (int k = 0; k < b->m[2]; ++k) {
@@ -1143,6 +1171,7 @@ static void scheme_u(Orc *o) {
             for (int a = 0; a < 3; ++a) {
                 int hi[3];
                 owned_hi(b, L_FX + a, hi);
+                PAR2
                 for (int i = 0; i < hi[0]; ++i)
                     for (int j = 0; j < hi[1]; ++j)
                         for (int k = 0; k < hi[2]; ++k) {
@@ -1178,6 +1207,7 @@ static void scheme_u(Orc *o) {
                                 F(b, S_F + 5 * a + m, i, j, k) = 0.5 * (Fl[0][m] + Fl[1][m]) - 0.5 * r * (Uc[1][m] - Uc[0][m]);
                         }
             }
+            PAR2
             for (int i = 0; i < b->m[0]; ++i)
                 for (int j = 0; j < b->m[1]; ++j)
                     for (int k = 0; k < b->m[2]; ++k) {
@@ -1198,6 +1228,7 @@ static void add_sources(Orc *o) {
     const OrcPhys *P = &o->P;
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
+        PAR2
         for (int i = 0; i < b->m[0]; ++i)
             for (int j = 0; j < b->m[1]; ++j)
                 for (int k = 0; k < b->m[2]; ++k) {
@@ -1315,6 +1346,7 @@ static void ideal_emf(Orc *o) {
                 lo[a] = dmax(0 - 1, alo[a] + 1);
                 hi[a] = dmin(ohi[a] + 1, ahi[a] - 1);
             }
+            PAR2
             for (int i = lo[0]; i < hi[0]; ++i)
                 for (int j = lo[1]; j < hi[1]; ++j)
                     for (int k = lo[2]; k < hi[2]; ++k) {
@@ -1346,6 +1378,7 @@ static void ideal_emf(Orc *o) {
     dmax(dmax(dmax(dmax(0.0, fabs(u_contra(b, ax, a0, a1, a2))), fabs(u_contra(b, ax, b0, b1, b2))), fabs(u_contra(b, ax, c0, c1, c2))), \
          fabs(u_contra(b, ax, d0, d1, d2)))
         owned_hi(b, L_EX, hi);
+        PAR2
         for (int i = 0; i < hi[0]; i++)
             for (int j = 0; j < hi[1]; j++)
                 for (int k = 0; k < hi[2]; k++) {
@@ -1357,6 +1390,7 @@ static void ideal_emf(Orc *o) {
                     F(b, S_E + 0, i, j, k) = Ec + Cuct * (0.5 * ae * dBz - 0.5 * az * dBe);
                 }
         owned_hi(b, L_EY, hi);
+        PAR2
         for (int i = 0; i < hi[0]; i++)
             for (int j = 0; j < hi[1]; j++)
                 for (int k = 0; k < hi[2]; k++) {
@@ -1368,6 +1402,7 @@ static void ideal_emf(Orc *o) {
                     F(b, S_E + 1, i, j, k) = Ec + Cuct * (0.5 * az * dBx - 0.5 * ax * dBz);
                 }
         owned_hi(b, L_EZ, hi);
+        PAR2
         for (int i = 0; i < hi[0]; i++)
             for (int j = 0; j < hi[1]; j++)
                 for (int k = 0; k < hi[2]; k++) {
@@ -1392,6 +1427,7 @@ static void ambipolar_emf(Orc *o) {
         for (int a = 0; a < 3; ++a) {
             int hi[3];
             owned_hi(b, L_EX + a, hi);
+            PAR2
             for (int i = 0; i < hi[0]; ++i)
                 for (int j = 0; j < hi[1]; ++j)
                     for (int k = 0; k < hi[2]; ++k) {
@@ -1431,6 +1467,7 @@ static void arti_emf(Orc *o) {
         for (int a = 0; a < 3; ++a) {
             int hi[3];
             owned_hi(b, L_EX + a, hi);
+            PAR2
             for (int i = 0; i < hi[0]; ++i)
                 for (int j = 0; j < hi[1]; ++j)
                     for (int k = 0; k < hi[2]; ++k) {
@@ -1458,6 +1495,7 @@ static void arti_emf(Orc *o) {
         for (int a = 0; a < 3; ++a) {
             int hi[3];
             owned_hi(b, L_EX + a, hi);
+            PAR2
             for (int i = 0; i < hi[0]; ++i)
                 for (int j = 0; j < hi[1]; ++j)
                     for (int k = 0; k < hi[2]; ++k) F(b, S_E + a, i, j, k) += F(b, S_LETA + a, i, j, k) * F(b, S_J + a, i, j, k);
@@ -1471,6 +1509,7 @@ static void hall_emf(Orc *o) {
     const double eps = 1e-14, Cwh = 0.1;
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
+        PAR1
         for (long q = 0; q < b->V; ++q) {
             NumInfo n = num_limiter(P, b->f[S_UH][q], b->f[S_UN][q]);
             const double alpha = P->hall_coef / n.ne_eff;
@@ -1489,6 +1528,7 @@ static void hall_emf(Orc *o) {
             if (a == 0) { int t[4][3] = {{0, -1, -1}, {0, -1, 0}, {0, 0, -1}, {0, 0, 0}}; memcpy(c4, t, sizeof(t)); }
             else if (a == 1) { int t[4][3] = {{-1, 0, -1}, {-1, 0, 0}, {0, 0, -1}, {0, 0, 0}}; memcpy(c4, t, sizeof(t)); }
             else { int t[4][3] = {{-1, -1, 0}, {-1, 0, 0}, {0, -1, 0}, {0, 0, 0}}; memcpy(c4, t, sizeof(t)); }
+            PAR2
             for (int i = 0; i < hi[0]; ++i)
                 for (int j = 0; j < hi[1]; ++j)
                     for (int k = 0; k < hi[2]; ++k) {
@@ -1578,6 +1618,7 @@ static void induction_ct(Orc *o) {
         for (int a = 0; a < 3; ++a) {
             int hi[3];
             owned_hi(b, L_EX + a, hi);
+            PAR2
             for (int i = 0; i < hi[0]; ++i)
                 for (int j = 0; j < hi[1]; ++j)
                     for (int k = 0; k < hi[2]; ++k) F(b, S_E + a, i, j, k) += F(b, S_EH + a, i, j, k);
@@ -1589,6 +1630,7 @@ static void induction_ct(Orc *o) {
         OBlock *b = &o->b[ib];
         int hi[3];
         owned_hi(b, L_FX, hi);
+        PAR2
         for (int i = 0; i < hi[0]; ++i)
             for (int j = 0; j < hi[1]; ++j)
                 for (int k = 0; k < hi[2]; ++k) {
@@ -1596,6 +1638,7 @@ static void induction_ct(Orc *o) {
                     F(b, S_RHSB + 0, i, j, k) = -1.0 * c;
                 }
         owned_hi(b, L_FY, hi);
+        PAR2
         for (int i = 0; i < hi[0]; ++i)
             for (int j = 0; j < hi[1]; ++j)
                 for (int k = 0; k < hi[2]; ++k) {
@@ -1603,6 +1646,7 @@ static void induction_ct(Orc *o) {
                     F(b, S_RHSB + 1, i, j, k) = -1.0 * c;
                 }
         owned_hi(b, L_FZ, hi);
+        PAR2
         for (int i = 0; i < hi[0]; ++i)
             for (int j = 0; j < hi[1]; ++j)
                 for (int k = 0; k < hi[2]; ++k) {
@@ -1616,6 +1660,7 @@ static void induction_ct(Orc *o) {
 static void apply_update(Orc *o, double dt) {
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
+        PAR2
         for (int i = 0; i < b->m[0]; ++i)
             for (int j = 0; j < b->m[1]; ++j)
                 for (int k = 0; k < b->m[2]; ++k)
@@ -1626,6 +1671,7 @@ static void apply_update(Orc *o, double dt) {
         for (int a = 0; a < 3; ++a) {
             int hi[3];
             owned_hi(b, L_FX + a, hi);
+            PAR2
             for (int i = 0; i < hi[0]; ++i)
                 for (int j = 0; j < hi[1]; ++j)
                     for (int k = 0; k < hi[2]; ++k) F(b, S_B + a, i, j, k) += dt * F(b, S_RHSB + a, i, j, k);
@@ -1638,6 +1684,7 @@ static void apply_update(Orc *o, double dt) {
 static void rk2_average(Orc *o) {
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
+        PAR2
         for (int i = 0; i < b->m[0]; ++i)
             for (int j = 0; j < b->m[1]; ++j)
                 for (int k = 0; k < b->m[2]; ++k)
@@ -1648,6 +1695,7 @@ static void rk2_average(Orc *o) {
         for (int a = 0; a < 3; ++a) {
             int hi[3];
             owned_hi(b, L_FX + a, hi);
+            PAR2
             for (int i = 0; i < hi[0]; ++i)
                 for (int j = 0; j < hi[1]; ++j)
                     for (int k = 0; k < hi[2]; ++k) F(b, S_B + a, i, j, k) = 0.5 * (F(b, S_B0 + a, i, j, k) + F(b, S_B + a, i, j, k));
@@ -1678,14 +1726,17 @@ static void curl_e2f(Orc *o, int se, int sf, double mult) {
         int hi[3];
 #define EE(a, i, j, k) F(b, se + (a), i, j, k)
         owned_hi(b, L_FX, hi);
+        PAR2
         for (int i = 0; i < hi[0]; ++i)
             for (int j = 0; j < hi[1]; ++j)
                 for (int k = 0; k < hi[2]; ++k) F(b, sf, i, j, k) = mult * ((EE(1, i, j, k) - EE(1, i, j, k + 1)) + (EE(2, i, j + 1, k) - EE(2, i, j, k)));
         owned_hi(b, L_FY, hi);
+        PAR2
         for (int i = 0; i < hi[0]; ++i)
             for (int j = 0; j < hi[1]; ++j)
                 for (int k = 0; k < hi[2]; ++k) F(b, sf + 1, i, j, k) = mult * ((EE(0, i, j, k + 1) - EE(0, i, j, k)) + (EE(2, i, j, k) - EE(2, i + 1, j, k)));
         owned_hi(b, L_FZ, hi);
+        PAR2
         for (int i = 0; i < hi[0]; ++i)
             for (int j = 0; j < hi[1]; ++j)
                 for (int k = 0; k < hi[2]; ++k) F(b, sf + 2, i, j, k) = mult * ((EE(0, i, j, k) - EE(0, i, j + 1, k)) + (EE(1, i + 1, j, k) - EE(1, i, j, k)));
@@ -2184,6 +2235,9 @@ int orc_op(void *h, const char *op, double arg) {
 }
 
 double orc_timestep(void *h) { return compute_timestep((Orc *)h); }
+
+/* threads the per-point loops use */
+int orc_threads(void) { return omp_get_max_threads(); }
 
 /* one implicit resistive correction with time step dt on the current state;
  * returns 0, or -3 when the iteration limit is reached */

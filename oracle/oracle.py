@@ -52,6 +52,8 @@ def lib():
         L.orc_divb.restype = ctypes.c_double
         L.orc_divb.argtypes = [vp, dp]
         L.orc_res_solve.argtypes = [vp, ctypes.c_double, ip, dp]
+        L.orc_threads.restype = ctypes.c_int
+        L.orc_threads.argtypes = []
         _lib = L
     return _lib
 
