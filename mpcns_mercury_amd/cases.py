@@ -9,7 +9,8 @@ the HIP path and for the CPU oracle:
 * owned node coordinates per block (ghost nodes are extrapolated by the solver,
   as ``Block::Add_ghostmesh`` does);
 * the face boundary codes of every block (periodic self-connection, far field,
-  or the fluid side of the solid planet surface);
+  the fluid side of the solid planet surface, or a 1:1 interface to a
+  neighbour block given by the neighbour and a node-index translation);
 * the conserved fluid state of both ion species on the whole allocated cell box
   (ghosts included, as a restart file holds them);
 * an edge vector potential ``A`` whose discrete curl is the divergence-free
@@ -27,7 +28,7 @@ import numpy as np
 
 from .params import MercuryPhysics, MercuryUnits, derive, RHO_SMALL
 
-BC_PERIODIC, BC_FARFIELD, BC_WALL = 0, 1, 2
+BC_PERIODIC, BC_FARFIELD, BC_WALL, BC_INTERFACE = 0, 1, 2, 3
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 
@@ -52,6 +53,8 @@ class Block:
     dims: tuple  # cells (mx, my, mz)
     bc: tuple  # 6 face codes: -xi, +xi, -eta, +eta, -zeta, +zeta
     nodes: np.ndarray  # (3, mx+1, my+1, mz+1)
+    nbr: tuple = (-1,) * 6  # BC_INTERFACE: neighbour block of each face
+    off: tuple = ((0, 0, 0),) * 6  # BC_INTERFACE: node (i,j,k) -> neighbour node (i,j,k) + off
 
 
 @dataclass
@@ -83,6 +86,18 @@ class Case:
 
     def bc_array(self) -> np.ndarray:
         return np.array([b.bc for b in self.blocks], dtype=np.int32).reshape(-1)
+
+    def has_interfaces(self) -> bool:
+        return any(c == BC_INTERFACE for b in self.blocks for c in b.bc)
+
+    def nbr_array(self) -> np.ndarray:
+        return np.array([b.nbr for b in self.blocks], dtype=np.int32).reshape(-1)
+
+    def trans_array(self) -> np.ndarray:
+        """IndexTransform per block face (perm[3], sign[3], offset[3]),
+        include/2_topology/TopologyTypes.h:93-98: translations only."""
+        t = [[0, 1, 2, 1, 1, 1, *b.off[f]] for b in self.blocks for f in range(6)]
+        return np.array(t, dtype=np.int32).reshape(-1)
 
     def nodes_flat(self) -> np.ndarray:
         return np.concatenate([np.ascontiguousarray(b.nodes).reshape(-1) for b in self.blocks]).astype(np.float64)
@@ -125,11 +140,7 @@ def make_periodic_cube(n=(48, 48, 48), g: int = 4, seed: int = 1, amp: float = 1
     switches the interior resistivity on with that inver_Rem on the window
     r in [0.5, 0.9] (width 0.05), which crosses the periodic seams."""
     mx, my, mz = n
-    i, j, k = np.meshgrid(np.arange(mx + 1), np.arange(my + 1), np.arange(mz + 1), indexing="ij")
-    y = j / my
-    z = k / mz
-    x = i / mx + 0.05 * np.sin(2 * np.pi * y) * np.sin(2 * np.pi * z)
-    nodes = np.stack([x, y, z]).astype(np.float64)
+    nodes = _cube_nodes(n, (0, 0, 0), n)
     units = MercuryUnits().alfven_units()
     units.CFL = 0.4
     if hall:
@@ -140,8 +151,30 @@ def make_periodic_cube(n=(48, 48, 48), g: int = 4, seed: int = 1, amp: float = 1
     if resist is not None:
         set_resistivity(units, resist, 0.5, 0.9, 0.05)
     P = derive(units)
-    gamma = P.gamma
-    I, J, K = _box_index(n, g)
+    fields, A = _cube_state(n, (0, 0, 0), n, g, seed, amp, P.gamma)
+    blk = Block(dims=(mx, my, mz), bc=(BC_PERIODIC,) * 6, nodes=nodes)
+    return Case(name=f"periodic_cube_{mx}x{my}x{mz}" + ("_resist" if resist is not None else ""), g=g, blocks=[blk],
+                phys=P, fields=[fields], A=[A], Bu=(1.0, 0.0, 0.0), steps=steps, integrator=integrator,
+                meta={"seed": seed, "species": 2, "hall": hall, "resist": resist})
+
+
+def _cube_nodes(n, origin, dims):
+    """Nodes of the sub-box [origin, origin + dims] of the C1 mapping of an n-cell cube."""
+    mx, my, mz = n
+    i, j, k = np.meshgrid(*(np.arange(o, o + d + 1) for o, d in zip(origin, dims)), indexing="ij")
+    y = j / my
+    z = k / mz
+    x = i / mx + 0.05 * np.sin(2 * np.pi * y) * np.sin(2 * np.pi * z)
+    return np.stack([x, y, z]).astype(np.float64)
+
+
+def _cube_state(n, origin, dims, g, seed, amp, gamma):
+    """Seeded state and edge potential on the box of the sub-block at origin,
+    keyed by the wrapped global index (identical values at identical global
+    positions whatever the block split)."""
+    mx, my, mz = n
+    I, J, K = _box_index(dims, g)
+    I, J, K = I + origin[0], J + origin[1], K + origin[2]
     key = ((np.mod(I, mx) * my + np.mod(J, my)) * mz + np.mod(K, mz)).astype(np.uint64)
     du = [amp * uniform(seed, s, key) for s in range(3)]
     shape = I.shape
@@ -151,11 +184,56 @@ def make_periodic_cube(n=(48, 48, 48), g: int = 4, seed: int = 1, amp: float = 1
     UNa = _conserved(RHO_SMALL * rho, du[0], du[1], du[2], RHO_SMALL * p, gamma)
     h = 1.0 / mx
     A = np.stack([amp * h * uniform(seed, 10 + a, key) for a in range(3)])
-    fields = [{"U_H": UH, "U_Na": UNa, "Photo_rate": np.zeros((1,) + shape)}]
-    blk = Block(dims=(mx, my, mz), bc=(BC_PERIODIC,) * 6, nodes=nodes)
-    return Case(name=f"periodic_cube_{mx}x{my}x{mz}" + ("_resist" if resist is not None else ""), g=g, blocks=[blk],
-                phys=P, fields=fields, A=[A], Bu=(1.0, 0.0, 0.0), steps=steps, integrator=integrator,
-                meta={"seed": seed, "species": 2, "hall": hall, "resist": resist})
+    return {"U_H": UH, "U_Na": UNa, "Photo_rate": np.zeros((1,) + shape)}, A
+
+
+def make_split_cube(n=(48, 48, 48), split=(2, 2, 2), g: int = 4, seed: int = 1, amp: float = 1e-3,
+                    integrator: int = 0, steps: int = 100, hall: bool = False, resist: float | None = None,
+                    cuts=None) -> Case:
+    """The C1 cube cut into split[0] x split[1] x split[2] blocks glued by 1:1
+    interfaces (the last block of each row wraps around to the first, which
+    carries the periodicity).  Every input is the same function of the global
+    position as in ``make_periodic_cube``, so the split case advances the same
+    field.  ``cuts`` optionally gives the cell counts per axis (each list sums to
+    n[a]); by default the axis is divided evenly.  Block b = (bx*sy + by)*sz + bz."""
+    base = make_periodic_cube(n, g=g, seed=seed, amp=amp, integrator=integrator, steps=steps, hall=hall, resist=resist)
+    if cuts is None:
+        cuts = []
+        for a in range(3):
+            q, r = divmod(n[a], split[a])
+            cuts.append([q + (1 if t < r else 0) for t in range(split[a])])
+    cuts = [list(c) for c in cuts]
+    for a in range(3):
+        if sum(cuts[a]) != n[a] or min(cuts[a]) < g:
+            raise ValueError("cuts must sum to n and give every block at least g cells")
+    sx, sy, sz = (len(c) for c in cuts)
+    org = [np.concatenate([[0], np.cumsum(c)[:-1]]).astype(int) for c in cuts]
+    bid = lambda bx, by, bz: (bx * sy + by) * sz + bz
+    blocks, fields, A = [], [], []
+    for bx in range(sx):
+        for by in range(sy):
+            for bz in range(sz):
+                pos = (bx, by, bz)
+                dims = tuple(int(cuts[a][pos[a]]) for a in range(3))
+                origin = tuple(int(org[a][pos[a]]) for a in range(3))
+                nbr, off = [], []
+                for f in range(6):
+                    a, side = f // 2, f & 1
+                    q = list(pos)
+                    q[a] = (q[a] + (1 if side else -1)) % len(cuts[a])
+                    nbr.append(bid(*q))
+                    o = [0, 0, 0]
+                    o[a] = -dims[a] if side else int(cuts[a][q[a]])
+                    off.append(tuple(o))
+                blocks.append(Block(dims=dims, bc=(BC_INTERFACE,) * 6, nodes=_cube_nodes(n, origin, dims),
+                                    nbr=tuple(nbr), off=tuple(off)))
+                fl, a_ = _cube_state(n, origin, dims, g, seed, amp, base.phys.gamma)
+                fields.append(fl)
+                A.append(a_)
+    meta = dict(base.meta)
+    meta.update({"split": (sx, sy, sz), "cuts": cuts, "origins": [o.tolist() for o in org]})
+    return Case(name=base.name.replace("periodic_cube", "split_cube") + f"_{sx}x{sy}x{sz}", g=g, blocks=blocks,
+                phys=base.phys, fields=fields, A=A, Bu=base.Bu, steps=steps, integrator=integrator, meta=meta)
 
 
 # ---------------------------------------------------------------------------

@@ -55,7 +55,9 @@ typedef struct {
     double resist_rtol, resist_atol, resist_max_it;
 } OrcPhys;
 
-enum { BC_PERIODIC = 0, BC_FARFIELD = 1, BC_WALL = 2 };
+/* face codes; BC_INTERFACE: a 1:1 inner connection to another (or the same)
+ * block, given by the neighbour and a translation of node indices */
+enum { BC_PERIODIC = 0, BC_FARFIELD = 1, BC_WALL = 2, BC_INTERFACE = 3 };
 
 /* stagger locations */
 enum { L_CELL = 0, L_FX, L_FY, L_FZ, L_EX, L_EY, L_EZ, L_NODE };
@@ -91,6 +93,8 @@ typedef struct {
     int N[3];
     long V;
     int bc[6];
+    int nbr[6];    /* BC_INTERFACE: neighbour block of the face */
+    int off[6][3]; /* BC_INTERFACE: node (i,j,k) -> neighbour node (i,j,k) + off */
     int Nn[3];
     long Vn;
     double *x, *y, *z;
@@ -105,6 +109,13 @@ typedef struct {
     double time;
     int step;
     int integrator; /* 0 Euler (reference), 1 SSP-RK2 */
+    /* equivalence classes of shared DOFs per location (faces, edges, nodes):
+     * members in ascending EntityKey order (block, i, j, k), the first is the
+     * owner; cls_b / cls_q: block and box index of each member */
+    int ncls[8];
+    long *cls_start[8];
+    int *cls_b[8];
+    long *cls_q[8];
 } Orc;
 
 #define IX(b, i, j, k) ((((long)(i) + (b)->g) * (b)->N[1] + ((j) + (b)->g)) * (b)->N[2] + ((k) + (b)->g))
@@ -777,8 +788,29 @@ static void apply_handler(OBlock *b, const OrcPhys *P, const GField *gf, int bc,
     }
 }
 
+/* connection of block face f: 1 and (*nb, off) for a periodic self-connection
+ * (translation by the block size) or an interface, 0 for a physical face */
+static int face_conn(const Orc *o, int ib, int f, int *nb, int off[3]) {
+    const OBlock *b = &o->b[ib];
+    const int a = f / 2;
+    off[0] = off[1] = off[2] = 0;
+    if (b->bc[f] == BC_PERIODIC) {
+        *nb = ib;
+        off[a] = (f & 1) ? -b->m[a] : b->m[a];
+        return 1;
+    }
+    if (b->bc[f] == BC_INTERFACE) {
+        *nb = b->nbr[f];
+        for (int c = 0; c < 3; ++c) off[c] = b->off[f][c];
+        return 1;
+    }
+    return 0;
+}
+
 /* fill_physical_corner_ghost (src/5_boundary/Boundary_Physical.cpp:39-59) on the
- * corner ghost boxes of physical edges (2 physical faces) and vertices (3). */
+ * corner ghost boxes of physical edges (2 physical faces) and vertices (3):
+ * an edge or vertex with a periodic or interface face is an inner patch
+ * (src/2_topology/TopologyEdge.cpp:161-166) and gets no corner default. */
 static void corner_default(OBlock *b, const GField *gf) {
     int ohi[3];
     owned_hi(b, gf->loc, ohi);
@@ -794,7 +826,8 @@ static void corner_default(OBlock *b, const GField *gf) {
                     int side = idx[a] < 0 ? 0 : (idx[a] >= ohi[a] ? 1 : -1);
                     if (side < 0) continue;
                     ++nout;
-                    if (b->bc[2 * a + side] == BC_PERIODIC) ok = 0;
+                    const int c = b->bc[2 * a + side];
+                    if (c != BC_FARFIELD && c != BC_WALL) ok = 0;
                 }
                 if (nout < 2 || !ok) continue;
                 int ii = i < 0 ? 0 : (i > ohi[0] - 1 ? ohi[0] - 1 : i);
@@ -804,70 +837,71 @@ static void corner_default(OBlock *b, const GField *gf) {
             }
 }
 
-/* Periodic self-connection halo (Halo::sync_group, HaloSyncFramework.cpp:969-1035,
- * with the strip rules of LayoutTraits.cpp:93-130): a ghost DOF whose
- * out-of-range axes are all periodic and number at most `level` takes the value
- * of its translate by whole periods.  Along an axis with delta = 0 the shared
- * boundary DOF is excluded from the source strip, i.e. i<0 -> i+m and i>m -> i-m. */
-static void periodic_halo(OBlock *b, const GField *gf) {
+/* One stage of the staged halo (MercuryBoundary::Sync_,
+ * Z4_Mercury/src/boundary/Boundary_Sync_.cpp:11-99; Halo::sync_group,
+ * src/4_halo/core/HaloSyncFramework.cpp:969-1035) for the ghost DOFs with
+ * exactly `stage` out-of-range axes: stage 1 the face strips
+ * (build_inner_1DCorner_pattern, src/4_halo/build/HaloBuildFace.cpp:6-57; strips
+ * of LayoutTraits.cpp:93-130, the shared boundary DOF excluded on the source
+ * side), stage 2 the edge and stage 3 the vertex corner boxes.  A ghost is
+ * filled when one of its out-of-range faces is a periodic or interface
+ * connection (the inner patch wins over physical ones,
+ * src/2_topology/TopologyEdge.cpp:161-166): its value is read one hop away,
+ * at the translated index in the neighbour across the first such face (axis
+ * order), copied as in exchange_inner / exchange_inner_edge / _vertex
+ * (src/4_halo/component_copy/HaloExchangeInner.cpp:90-247).  The hop removes
+ * one out-of-range axis, so a stage reads owned values or ghosts that an earlier
+ * stage, a physical handler or the corner default has written, never its own
+ * destinations. */
+static void halo_stage(Orc *o, int ib, const GField *gf, int stage) {
+    OBlock *b = &o->b[ib];
     int ohi[3], lo[3], hi[3];
     owned_hi(b, gf->loc, ohi);
     alloc_box(b, gf->loc, b->g, lo, hi);
-    /* with more cells than ghost layers on every axis each source is an owned
-     * point, so the ghost writes do not feed each other and may run in parallel */
-    const int par = b->m[0] > b->g && b->m[1] > b->g && b->m[2] > b->g;
-#pragma omp parallel for collapse(2) schedule(static) if (par)
+    PAR2
     for (int i = lo[0]; i < hi[0]; ++i)
         for (int j = lo[1]; j < hi[1]; ++j)
             for (int k = lo[2]; k < hi[2]; ++k) {
-                int idx[3] = {i, j, k}, src[3], nout = 0, ok = 1;
+                int idx[3] = {i, j, k}, nout = 0, hop = -1;
                 for (int a = 0; a < 3; ++a) {
-                    src[a] = idx[a];
                     if (idx[a] >= 0 && idx[a] < ohi[a]) continue;
                     ++nout;
-                    int side = idx[a] < 0 ? 0 : 1;
-                    if (b->bc[2 * a + side] != BC_PERIODIC) { ok = 0; continue; }
-                    src[a] = idx[a] < 0 ? idx[a] + b->m[a] : idx[a] - b->m[a];
+                    const int c = b->bc[2 * a + (idx[a] < 0 ? 0 : 1)];
+                    if (hop < 0 && (c == BC_PERIODIC || c == BC_INTERFACE)) hop = a;
                 }
-                if (nout == 0 || nout > gf->level || !ok) continue;
-                for (int m = 0; m < gf->ncomp; ++m) F(b, gf->slot + m, i, j, k) = F(b, gf->slot + m, src[0], src[1], src[2]);
+                if (nout != stage || hop < 0) continue;
+                int nb, off[3];
+                face_conn(o, ib, 2 * hop + (idx[hop] < 0 ? 0 : 1), &nb, off);
+                OBlock *s = &o->b[nb];
+                for (int m = 0; m < gf->ncomp; ++m)
+                    F(b, gf->slot + m, i, j, k) = F(s, gf->slot + m, i + off[0], j + off[1], k + off[2]);
             }
 }
 
 /* owner -> alias (Halo::sync_owner_alias_group, HaloSyncFramework.cpp:695-712):
- * on a periodic axis the DOF at index m (delta = 0) aliases the one at 0; the
- * owner of an equivalence class is its smallest EntityKey, i.e. every seam index
- * replaced by 0; orientation signs are +1 for translations. */
-static void owner_sync(OBlock *b, const GField *gf) {
-    int ohi[3];
-    owned_hi(b, gf->loc, ohi);
-    /* writes seam aliases (index m) from their owners (index 0) only */
-    PAR2
-    for (int i = 0; i < ohi[0]; ++i)
-        for (int j = 0; j < ohi[1]; ++j)
-            for (int k = 0; k < ohi[2]; ++k) {
-                int idx[3] = {i, j, k}, o[3], alias = 0;
-                for (int a = 0; a < 3; ++a) {
-                    o[a] = idx[a];
-                    if (DELTA[gf->loc][a] == 0 && b->bc[2 * a] == BC_PERIODIC && idx[a] == b->m[a]) {
-                        o[a] = 0;
-                        alias = 1;
-                    }
-                }
-                if (alias) F(b, gf->slot, i, j, k) = F(b, gf->slot, o[0], o[1], o[2]);
-            }
+ * every member of a class of shared DOFs takes the owner's value; orientation
+ * signs are +1 for translations. */
+static void owner_sync(Orc *o, const GField *gf) {
+    const int L = gf->loc;
+    PAR1
+    for (int c = 0; c < o->ncls[L]; ++c) {
+        const long s0 = o->cls_start[L][c], s1 = o->cls_start[L][c + 1];
+        const double v = o->b[o->cls_b[L][s0]].f[gf->slot][o->cls_q[L][s0]];
+        for (long t = s0 + 1; t < s1; ++t) o->b[o->cls_b[L][t]].f[gf->slot][o->cls_q[L][t]] = v;
+    }
 }
 
-/* MercuryBoundary::Sync_ (Z4_Mercury/src/boundary/Boundary_Sync_.cpp:4-110):
- * physical handlers region by region (Farfield regions before the wall, the
- * order of the sorted physical patch list), corner defaults, staged halo, then
- * owner -> alias for the face/edge form groups. */
+/* MercuryBoundary::Sync_ (Z4_Mercury/src/boundary/Boundary_Sync_.cpp:4-110),
+ * over all blocks phase by phase: physical handlers region by region (Farfield
+ * regions before the wall, the order of the sorted physical patch list), corner
+ * defaults, the halo stages up to the field's level, then owner -> alias for
+ * the face/edge form groups. */
 static void sync_group(Orc *o, int gid) {
     GField gf[6];
     int nf = group_fields(gid, gf);
-    for (int ib = 0; ib < o->nb; ++ib) {
-        OBlock *b = &o->b[ib];
-        for (int n = 0; n < nf; ++n) {
+    for (int n = 0; n < nf; ++n)
+        for (int ib = 0; ib < o->nb; ++ib) {
+            OBlock *b = &o->b[ib];
             for (int pass = 0; pass < 2; ++pass) {
                 int want = pass == 0 ? BC_FARFIELD : BC_WALL;
                 for (int f = 0; f < 6; ++f) {
@@ -876,48 +910,129 @@ static void sync_group(Orc *o, int gid) {
                     apply_handler(b, &o->P, &gf[n], want, dir);
                 }
             }
-            corner_default(b, &gf[n]);
-            periodic_halo(b, &gf[n]);
-            if (gf[n].owner_sync) owner_sync(b, &gf[n]);
+        }
+    for (int n = 0; n < nf; ++n)
+        for (int ib = 0; ib < o->nb; ++ib) corner_default(&o->b[ib], &gf[n]);
+    for (int stage = 1; stage <= 3; ++stage)
+        for (int n = 0; n < nf; ++n) {
+            if (stage > gf[n].level) continue;
+            for (int ib = 0; ib < o->nb; ++ib) halo_stage(o, ib, &gf[n], stage);
+        }
+    for (int n = 0; n < nf; ++n)
+        if (gf[n].owner_sync) owner_sync(o, &gf[n]);
+}
+
+/* ReduceEdgeAliasCandidatesToOwners_ (MercurySolver_Current.cpp:3-57): the owner
+ * of each shared edge class takes the signed mean of all members, summed in
+ * ascending EntityKey order. */
+static void alias_reduce_edges(Orc *o, int slot0) {
+    for (int a = 0; a < 3; ++a) {
+        const int L = L_EX + a;
+        PAR1
+        for (int c = 0; c < o->ncls[L]; ++c) {
+            const long s0 = o->cls_start[L][c], s1 = o->cls_start[L][c + 1];
+            double sum = 0.0;
+            for (long t = s0; t < s1; ++t) sum += 1 * o->b[o->cls_b[L][t]].f[slot0 + a][o->cls_q[L][t]];
+            o->b[o->cls_b[L][s0]].f[slot0 + a][o->cls_q[L][s0]] = 1 * sum / (double)(s1 - s0);
         }
     }
 }
 
-/* ReduceEdgeAliasCandidatesToOwners_ (MercurySolver_Current.cpp:3-57): the owner
- * of each shared periodic-seam edge class takes the signed mean of all members,
- * summed in ascending EntityKey order. */
-static void alias_reduce_edges(Orc *o, int slot0) {
-    for (int ib = 0; ib < o->nb; ++ib) {
-        OBlock *b = &o->b[ib];
-        for (int a = 0; a < 3; ++a) {
-            int loc = L_EX + a, ohi[3];
-            owned_hi(b, loc, ohi);
-            int per[3];
-            for (int c = 0; c < 3; ++c) per[c] = (DELTA[loc][c] == 0 && b->bc[2 * c] == BC_PERIODIC);
-            for (int i = 0; i < ohi[0]; ++i)
-                for (int j = 0; j < ohi[1]; ++j)
-                    for (int k = 0; k < ohi[2]; ++k) {
-                        int idx[3] = {i, j, k}, nseam = 0, ax[3];
-                        int isowner = 1;
-                        for (int c = 0; c < 3; ++c) {
-                            if (!per[c]) continue;
-                            if (idx[c] == b->m[c]) isowner = 0;
-                            if (idx[c] == 0) ax[nseam++] = c;
-                        }
-                        if (!isowner || nseam == 0) continue;
-                        /* members: each seam axis index in {0, m}; ascending key order */
-                        int nm = 1 << nseam;
-                        double sum = 0.0;
-                        for (int mask = 0; mask < nm; ++mask) {
-                            int q[3] = {i, j, k};
-                            for (int t = 0; t < nseam; ++t)
-                                if (mask & (1 << (nseam - 1 - t))) q[ax[t]] = b->m[ax[t]];
-                            sum += 1 * F(b, slot0 + a, q[0], q[1], q[2]);
-                        }
-                        F(b, slot0 + a, i, j, k) = 1 * sum / (double)nm;
-                    }
-        }
+/* Equivalence classes of the DOFs shared through periodic and interface faces
+ * (Topology equivalence, src/2_topology/TopologyFaceEquiv.cpp,
+ * TopologyEdgeEquiv.cpp; owner = smallest EntityKey): every owned DOF on a
+ * connected block face is united with its translate in the neighbour.  Only
+ * locations with delta = 0 along the face normal lie on the face. */
+static long uf_find(long *p, long x) {
+    while (p[x] != x) {
+        p[x] = p[p[x]];
+        x = p[x];
     }
+    return x;
+}
+static int cmp_pair(const void *a, const void *b) {
+    const long *x = (const long *)a, *y = (const long *)b;
+    if (x[0] != y[0]) return (x[0] > y[0]) - (x[0] < y[0]);
+    return (x[1] > y[1]) - (x[1] < y[1]);
+}
+static void build_classes(Orc *o) {
+    int any = 0;
+    for (int ib = 0; ib < o->nb; ++ib)
+        for (int f = 0; f < 6; ++f) any |= o->b[ib].bc[f] == BC_PERIODIC || o->b[ib].bc[f] == BC_INTERFACE;
+    for (int L = 0; L < 8; ++L) {
+        o->ncls[L] = 0;
+        o->cls_start[L] = (long *)calloc(1, sizeof(long));
+        o->cls_b[L] = NULL;
+        o->cls_q[L] = NULL;
+    }
+    if (!any) return;
+    long *base = (long *)calloc(o->nb + 1, sizeof(long));
+    for (int ib = 0; ib < o->nb; ++ib) base[ib + 1] = base[ib] + o->b[ib].V;
+    const long tot = base[o->nb];
+    long *par = (long *)malloc(sizeof(long) * tot);
+    char *hit = (char *)malloc(tot);
+    for (int L = L_FX; L <= L_NODE; ++L) {
+        for (long x = 0; x < tot; ++x) par[x] = x;
+        memset(hit, 0, tot);
+        for (int ib = 0; ib < o->nb; ++ib) {
+            OBlock *b = &o->b[ib];
+            int ohi[3];
+            owned_hi(b, L, ohi);
+            for (int f = 0; f < 6; ++f) {
+                const int a = f / 2;
+                int nb, off[3];
+                if (DELTA[L][a] != 0 || !face_conn(o, ib, f, &nb, off)) continue;
+                OBlock *nbk = &o->b[nb];
+                int lo[3] = {0, 0, 0}, hi[3] = {ohi[0], ohi[1], ohi[2]};
+                lo[a] = (f & 1) ? b->m[a] : 0;
+                hi[a] = lo[a] + 1;
+                for (int i = lo[0]; i < hi[0]; ++i)
+                    for (int j = lo[1]; j < hi[1]; ++j)
+                        for (int k = lo[2]; k < hi[2]; ++k) {
+                            const long x = base[ib] + IX(b, i, j, k);
+                            const long y = base[nb] + IX(nbk, i + off[0], j + off[1], k + off[2]);
+                            const long rx = uf_find(par, x), ry = uf_find(par, y);
+                            if (rx != ry) par[rx < ry ? ry : rx] = rx < ry ? rx : ry;
+                            hit[x] = hit[y] = 1;
+                        }
+            }
+        }
+        /* members grouped by class root (the smallest member, since the union
+         * keeps the smaller root); the global index orders (block, i, j, k) */
+        long nmem = 0;
+        for (long x = 0; x < tot; ++x) nmem += hit[x];
+        long *key = (long *)malloc(sizeof(long) * 2 * (nmem ? nmem : 1));
+        long t = 0;
+        for (long x = 0; x < tot; ++x)
+            if (hit[x]) {
+                key[2 * t] = uf_find(par, x);
+                key[2 * t + 1] = x;
+                ++t;
+            }
+        qsort(key, nmem, 2 * sizeof(long), cmp_pair);
+        long *st = (long *)malloc(sizeof(long) * (nmem + 1));
+        int *cb = (int *)malloc(sizeof(int) * (nmem ? nmem : 1));
+        long *cq = (long *)malloc(sizeof(long) * (nmem ? nmem : 1));
+        int nc = 0;
+        for (long u = 0; u < nmem; ++u) {
+            if (u == 0 || key[2 * u] != key[2 * (u - 1)]) st[nc++] = u;
+            const long x = key[2 * u + 1];
+            int ib = 0;
+            while (base[ib + 1] <= x) ++ib;
+            cb[u] = ib;
+            cq[u] = x - base[ib];
+        }
+        st[nc] = nmem;
+        free(o->cls_start[L]);
+        o->ncls[L] = nc;
+        o->cls_start[L] = st;
+        o->cls_b[L] = cb;
+        o->cls_q[L] = cq;
+        free(key);
+    }
+    free(hit);
+    free(par);
+    free(base);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1761,6 +1876,17 @@ static double resist_shape(const OrcPhys *P, double r) {
  * they reach any face).  Writes eta into S_ETAR (0 off the unknowns). */
 static void resist_dofs(Orc *o) {
     const OrcPhys *P = &o->P;
+    /* non-owner members of the edge classes */
+    char **alias[3];
+    for (int a = 0; a < 3; ++a) {
+        alias[a] = (char **)calloc(o->nb, sizeof(char *));
+        for (int ib = 0; ib < o->nb; ++ib) alias[a][ib] = (char *)calloc(o->b[ib].V, 1);
+    }
+    for (int a = 0; a < 3; ++a) {
+        const int L = L_EX + a;
+        for (int c = 0; c < o->ncls[L]; ++c)
+            for (long t = o->cls_start[L][c] + 1; t < o->cls_start[L][c + 1]; ++t) alias[a][o->cls_b[L][t]][o->cls_q[L][t]] = 1;
+    }
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
         for (int a = 0; a < 3; ++a) {
@@ -1771,10 +1897,7 @@ static void resist_dofs(Orc *o) {
             for (int i = 0; i < hi[0]; ++i)
                 for (int j = 0; j < hi[1]; ++j)
                     for (int k = 0; k < hi[2]; ++k) {
-                        int idx[3] = {i, j, k}, skip = 0;
-                        for (int c = 0; c < 3; ++c)
-                            if (DELTA[loc][c] == 0 && b->bc[2 * c] == BC_PERIODIC && idx[c] == b->m[c]) skip = 1;
-                        if (skip) continue;
+                        if (alias[a][ib][IX(b, i, j, k)]) continue;
                         if (fabs(F(b, S_DL + a, i, j, k)) <= 1.0e-12) continue;
                         const int ir = i + (a == 0), jr = j + (a == 1), kr = k + (a == 2);
                         const double xm = 0.5 * (XN(b, i, j, k) + XN(b, ir, jr, kr));
@@ -1794,6 +1917,10 @@ static void resist_dofs(Orc *o) {
                         for (int k = lo[2]; k < sh[2]; ++k) F(b, S_ETAR + a, i, j, k) = 0.0;
             }
         }
+    }
+    for (int a = 0; a < 3; ++a) {
+        for (int ib = 0; ib < o->nb; ++ib) free(alias[a][ib]);
+        free(alias[a]);
     }
 }
 
@@ -1989,7 +2116,44 @@ static int stage(Orc *o, double dt, int resist) {
 /* C ABI for the tests                                                        */
 /* ------------------------------------------------------------------------- */
 
+/* Blocks with 1:1 interface connections: for every face with code
+ * BC_INTERFACE, nbr[6*ib+f] names the neighbour block and trans[9*(6*ib+f)..]
+ * its IndexTransform (perm[3], sign[3], offset[3]: neighbour node
+ * index[perm[a]] = sign[a] * node index[a] + offset[a],
+ * include/2_topology/TopologyTypes.h:93-98).  Supported: translations
+ * (perm = identity, sign = +1) that glue the face to the opposite face of the
+ * neighbour over its whole extent, declared from both sides, with at least
+ * as many cells as ghost layers along the connection.  Returns NULL on
+ * anything else. */
+void *orc_create_conn(int nb, const int *dims, int g, const int *bc, const int *nbr, const int *trans, const double *nodes,
+                      const OrcPhys *P);
 void *orc_create(int nb, const int *dims, int g, const int *bc, const double *nodes, const OrcPhys *P) {
+    return orc_create_conn(nb, dims, g, bc, NULL, NULL, nodes, P);
+}
+static int conn_valid(int nb, const int *dims, int g, const int *bc, const int *nbr, const int *trans) {
+    if (nb <= 0) return 0;
+    for (int ib = 0; ib < nb; ++ib)
+        for (int f = 0; f < 6; ++f) {
+            const int c = bc[6 * ib + f];
+            if (c < 0 || c > 3) return 0;
+            if (c == BC_PERIODIC && bc[6 * ib + (f ^ 1)] != BC_PERIODIC) return 0;
+            if (c != BC_INTERFACE) continue;
+            if (!nbr || !trans) return 0;
+            const int n = nbr[6 * ib + f], a = f / 2, fo = f ^ 1;
+            if (n < 0 || n >= nb || bc[6 * n + fo] != BC_INTERFACE || nbr[6 * n + fo] != ib) return 0;
+            const int *t = trans + 9 * (6 * ib + f), *u = trans + 9 * (6 * n + fo);
+            for (int d = 0; d < 3; ++d) {
+                if (t[d] != d || t[3 + d] != 1 || t[6 + d] != -u[6 + d]) return 0;
+                if (d != a && (t[6 + d] != 0 || dims[3 * ib + d] != dims[3 * n + d])) return 0;
+            }
+            if (t[6 + a] != ((f & 1) ? -dims[3 * ib + a] : dims[3 * n + a])) return 0;
+            if (dims[3 * ib + a] < g || dims[3 * n + a] < g) return 0;
+        }
+    return 1;
+}
+void *orc_create_conn(int nb, const int *dims, int g, const int *bc, const int *nbr, const int *trans, const double *nodes,
+                      const OrcPhys *P) {
+    if (!conn_valid(nb, dims, g, bc, nbr, trans)) return NULL;
     Orc *o = (Orc *)calloc(1, sizeof(Orc));
     o->nb = nb;
     o->b = (OBlock *)calloc(nb, sizeof(OBlock));
@@ -2003,7 +2167,14 @@ void *orc_create(int nb, const int *dims, int g, const int *bc, const double *no
             b->N[a] = b->m[a] + 1 + 2 * g;
             b->Nn[a] = b->m[a] + 1 + 2 * (g + 1);
         }
-        for (int f = 0; f < 6; ++f) b->bc[f] = bc[6 * ib + f];
+        for (int f = 0; f < 6; ++f) {
+            b->bc[f] = bc[6 * ib + f];
+            b->nbr[f] = -1;
+            if (b->bc[f] == BC_INTERFACE) {
+                b->nbr[f] = nbr[6 * ib + f];
+                for (int c = 0; c < 3; ++c) b->off[f][c] = trans[9 * (6 * ib + f) + 6 + c];
+            }
+        }
         b->V = (long)b->N[0] * b->N[1] * b->N[2];
         b->Vn = (long)b->Nn[0] * b->Nn[1] * b->Nn[2];
         b->x = (double *)calloc(b->Vn, sizeof(double));
@@ -2027,6 +2198,7 @@ void *orc_create(int nb, const int *dims, int g, const int *bc, const double *no
         recon_weights(b);
         edge_statics(b, &o->P);
     }
+    build_classes(o);
     resist_dofs(o);
     return o;
 }
@@ -2038,6 +2210,11 @@ void orc_destroy(void *h) {
         OBlock *b = &o->b[ib];
         free(b->x); free(b->y); free(b->z);
         for (int s = 0; s < S_NSLOT; ++s) free(b->f[s]);
+    }
+    for (int L = 0; L < 8; ++L) {
+        free(o->cls_start[L]);
+        free(o->cls_b[L]);
+        free(o->cls_q[L]);
     }
     free(o->b);
     free(o);

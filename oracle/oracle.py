@@ -36,6 +36,8 @@ def lib():
         vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
         L.orc_create.restype = vp
         L.orc_create.argtypes = [ctypes.c_int, ip, ctypes.c_int, ip, dp, ctypes.POINTER(MercuryPhysics)]
+        L.orc_create_conn.restype = vp
+        L.orc_create_conn.argtypes = [ctypes.c_int, ip, ctypes.c_int, ip, ip, ip, dp, ctypes.POINTER(MercuryPhysics)]
         L.orc_destroy.argtypes = [vp]
         L.orc_slot.argtypes = [ctypes.c_char_p]
         L.orc_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
@@ -72,8 +74,12 @@ class Oracle:
         dims = case.dims_array()
         bc = case.bc_array()
         nodes = case.nodes_flat()
-        self.h = L.orc_create(case.nb, dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), case.g,
-                              bc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(nodes), ctypes.byref(self.phys))
+        ip = ctypes.POINTER(ctypes.c_int)
+        nbr, trans = case.nbr_array(), case.trans_array()
+        self.h = L.orc_create_conn(case.nb, dims.ctypes.data_as(ip), case.g, bc.ctypes.data_as(ip), nbr.ctypes.data_as(ip),
+                                   trans.ctypes.data_as(ip), _dp(nodes), ctypes.byref(self.phys))
+        if not self.h:
+            raise ValueError("oracle: unsupported block connections")
         for ib, fl in enumerate(case.fields):
             for name, arr in fl.items():
                 self.set(name, ib, arr)
