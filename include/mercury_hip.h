@@ -29,8 +29,11 @@ extern "C" {
  * FARFIELD: the "Farfield" physical patch (Z4_Mercury/src/boundary/Boundary_Build.cpp:45-203).
  * WALL:     the fluid side of the solid planet surface, i.e. the "Coupled-Solid"
  *           handlers (Boundary_Build.cpp:59-71, 111-119) and the stationary-wall EMF
- *           (Z4_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:5-89). */
-enum { MERCURY_BC_PERIODIC = 0, MERCURY_BC_FARFIELD = 1, MERCURY_BC_WALL = 2 };
+ *           (Z4_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:5-89).
+ * INTERFACE: a 1:1 inner connection to a neighbour block (the reference's Inner
+ *           InterfacePatch, include/2_topology/TopologyTypes.h:104-139), given to
+ *           mercury_create_connected. */
+enum { MERCURY_BC_PERIODIC = 0, MERCURY_BC_FARFIELD = 1, MERCURY_BC_WALL = 2, MERCURY_BC_INTERFACE = 3 };
 
 /* Integrators: forward Euler (the reference StepOnce) or SSP-RK2 (two Euler
  * stages and the average; no reference counterpart). */
@@ -78,6 +81,23 @@ int mercury_version(void);
  * x, y, z planes of the (mx+1)(my+1)(mz+1) owned nodes, k fastest. */
 int mercury_create(MercuryContext **ctx, int nblocks, const int *dims, int nghost, const int *bc,
                    const double *owned_nodes, const MercuryPhysics *phys);
+
+/* mercury_create for blocks glued by 1:1 interfaces: the topology that
+ * TOPO::build_topology (src/2_topology/TopologyBuilder.cpp:436) derives from the
+ * grid's inner-connection files.  For every face with code MERCURY_BC_INTERFACE,
+ * nbr[6*b+f] is the neighbour block and transform[9*(6*b+f) ...] its
+ * IndexTransform (perm[3], sign[3], offset[3]: neighbour node index[perm[a]] =
+ * sign[a] * node index[a] + offset[a], include/2_topology/TopologyTypes.h:93-98);
+ * other faces ignore both.  Supported are translations (perm identity, sign +1)
+ * that glue a face to the opposite face of the neighbour over its whole extent,
+ * declared from both sides, with at least nghost cells along the connection;
+ * anything else fails with a message.  A block may be its own neighbour (a
+ * periodic axis).  Halo copies, owner -> alias copies and the edge alias means
+ * then run across blocks as in the reference's staged Sync_
+ * (Z4_Mercury/src/boundary/Boundary_Sync_.cpp:4-110).  nbr and transform may be
+ * NULL when no face is an interface. */
+int mercury_create_connected(MercuryContext **ctx, int nblocks, const int *dims, int nghost, const int *bc, const int *nbr,
+                             const int *transform, const double *owned_nodes, const MercuryPhysics *phys);
 /* Replaces the deletes at the end of Z4_Mercury/main.cpp:86-89. */
 int mercury_destroy(MercuryContext *ctx);
 

@@ -21,6 +21,7 @@ Layout of every per-block array: the uniform box [-g, m+g] per axis, k fastest.
 """
 from __future__ import annotations
 
+import itertools
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
 
@@ -187,53 +188,76 @@ def _cube_state(n, origin, dims, g, seed, amp, gamma):
     return {"U_H": UH, "U_Na": UNa, "Photo_rate": np.zeros((1,) + shape)}, A
 
 
+def split_block_case(case: Case, cuts, name: str | None = None) -> Case:
+    """Cut the single block of ``case`` into blocks glued by 1:1 interfaces.
+
+    ``cuts[a]`` lists the cell counts along axis a (summing to the block's size,
+    each at least g).  Cut faces become interfaces to the neighbouring piece; a
+    periodic axis wraps from the last piece to the first through an interface;
+    the other outer faces keep their codes.  Nodes, the state on the box and the
+    edge potential are the original arrays sliced, so every piece holds the
+    original values at its positions.  Block b = (bx*sy + by)*sz + bz."""
+    if case.nb != 1:
+        raise ValueError("split_block_case takes a one-block case")
+    blk, g = case.blocks[0], case.g
+    m = blk.dims
+    cuts = [list(int(x) for x in c) for c in cuts]
+    for a in range(3):
+        if sum(cuts[a]) != m[a] or min(cuts[a]) < g:
+            raise ValueError("cuts must sum to the block size and give every piece at least g cells")
+    sx, sy, sz = (len(c) for c in cuts)
+    org = [np.concatenate([[0], np.cumsum(c)[:-1]]).astype(int) for c in cuts]
+    bid = lambda p: (p[0] * sy + p[1]) * sz + p[2]
+    blocks, fields, A = [], [], []
+    for pos in itertools.product(range(sx), range(sy), range(sz)):
+        dims = tuple(int(cuts[a][pos[a]]) for a in range(3))
+        o = tuple(int(org[a][pos[a]]) for a in range(3))
+        bc, nbr, off = [], [], []
+        for f in range(6):
+            a, side = f // 2, f & 1
+            q = list(pos)
+            q[a] += 1 if side else -1
+            inner = 0 <= q[a] < len(cuts[a])
+            if not inner and blk.bc[f] == BC_PERIODIC:
+                q[a] %= len(cuts[a])
+                inner = True
+            if inner:
+                bc.append(BC_INTERFACE)
+                nbr.append(bid(q))
+                t = [0, 0, 0]
+                t[a] = -dims[a] if side else int(cuts[a][q[a]])
+                off.append(tuple(t))
+            else:
+                bc.append(blk.bc[f])
+                nbr.append(-1)
+                off.append((0, 0, 0))
+        nodes = blk.nodes[:, o[0]:o[0] + dims[0] + 1, o[1]:o[1] + dims[1] + 1, o[2]:o[2] + dims[2] + 1]
+        blocks.append(Block(dims=dims, bc=tuple(bc), nodes=np.ascontiguousarray(nodes), nbr=tuple(nbr), off=tuple(off)))
+        box = tuple(slice(o[a], o[a] + dims[a] + 1 + 2 * g) for a in range(3))
+        fields.append({k: np.ascontiguousarray(v[(slice(None),) + box]) for k, v in case.fields[0].items()})
+        if case.A is not None:
+            A.append(np.ascontiguousarray(case.A[0][(slice(None),) + box]))
+    meta = dict(case.meta)
+    meta.update({"split": (sx, sy, sz), "cuts": cuts, "origins": [o.tolist() for o in org]})
+    return Case(name=name or f"{case.name}_split{sx}x{sy}x{sz}", g=g, blocks=blocks, phys=case.phys, fields=fields,
+                A=A if case.A is not None else None, Bu=case.Bu, steps=case.steps, integrator=case.integrator, meta=meta)
+
+
 def make_split_cube(n=(48, 48, 48), split=(2, 2, 2), g: int = 4, seed: int = 1, amp: float = 1e-3,
                     integrator: int = 0, steps: int = 100, hall: bool = False, resist: float | None = None,
                     cuts=None) -> Case:
     """The C1 cube cut into split[0] x split[1] x split[2] blocks glued by 1:1
     interfaces (the last block of each row wraps around to the first, which
-    carries the periodicity).  Every input is the same function of the global
-    position as in ``make_periodic_cube``, so the split case advances the same
-    field.  ``cuts`` optionally gives the cell counts per axis (each list sums to
-    n[a]); by default the axis is divided evenly.  Block b = (bx*sy + by)*sz + bz."""
+    carries the periodicity).  ``cuts`` optionally gives the cell counts per axis;
+    by default each axis is divided evenly."""
     base = make_periodic_cube(n, g=g, seed=seed, amp=amp, integrator=integrator, steps=steps, hall=hall, resist=resist)
     if cuts is None:
         cuts = []
         for a in range(3):
             q, r = divmod(n[a], split[a])
             cuts.append([q + (1 if t < r else 0) for t in range(split[a])])
-    cuts = [list(c) for c in cuts]
-    for a in range(3):
-        if sum(cuts[a]) != n[a] or min(cuts[a]) < g:
-            raise ValueError("cuts must sum to n and give every block at least g cells")
-    sx, sy, sz = (len(c) for c in cuts)
-    org = [np.concatenate([[0], np.cumsum(c)[:-1]]).astype(int) for c in cuts]
-    bid = lambda bx, by, bz: (bx * sy + by) * sz + bz
-    blocks, fields, A = [], [], []
-    for bx in range(sx):
-        for by in range(sy):
-            for bz in range(sz):
-                pos = (bx, by, bz)
-                dims = tuple(int(cuts[a][pos[a]]) for a in range(3))
-                origin = tuple(int(org[a][pos[a]]) for a in range(3))
-                nbr, off = [], []
-                for f in range(6):
-                    a, side = f // 2, f & 1
-                    q = list(pos)
-                    q[a] = (q[a] + (1 if side else -1)) % len(cuts[a])
-                    nbr.append(bid(*q))
-                    o = [0, 0, 0]
-                    o[a] = -dims[a] if side else int(cuts[a][q[a]])
-                    off.append(tuple(o))
-                blocks.append(Block(dims=dims, bc=(BC_INTERFACE,) * 6, nodes=_cube_nodes(n, origin, dims),
-                                    nbr=tuple(nbr), off=tuple(off)))
-                fl, a_ = _cube_state(n, origin, dims, g, seed, amp, base.phys.gamma)
-                fields.append(fl)
-                A.append(a_)
-    meta = dict(base.meta)
-    meta.update({"split": (sx, sy, sz), "cuts": cuts, "origins": [o.tolist() for o in org]})
-    return Case(name=base.name.replace("periodic_cube", "split_cube") + f"_{sx}x{sy}x{sz}", g=g, blocks=blocks,
-                phys=base.phys, fields=fields, A=A, Bu=base.Bu, steps=steps, integrator=integrator, meta=meta)
+    return split_block_case(base, cuts, name=base.name.replace("periodic_cube", "split_cube") +
+                            "_{}x{}x{}".format(*(len(c) for c in cuts)))
 
 
 # ---------------------------------------------------------------------------
@@ -365,3 +389,14 @@ def make_ensemble_run(r: int, scale: float = 1.0, g: int = 4, steps: int = 100) 
     return make_shell(blocks=tuple(dims), g=g, seed=seed, steps=steps, scale=1.0, hall_coef=hall, inflow=speed,
                       imf=(0.0, 0.2 * np.sin(ang), -0.2 * np.cos(ang)), na_fraction=RHO_SMALL, panels=panels,
                       name=f"ensemble_run{r}")
+
+
+def make_split_sector(dims=(24, 20, 16), cuts=((24,), (10, 10), (16,)), **kw) -> Case:
+    """One shell sector of the two-fluid Hall configuration (planet surface on
+    xi-, far field on the other faces) cut into blocks glued by interfaces: the
+    cut faces meet the wall and far-field faces, so the corner ghosts mix inner
+    and physical patches."""
+    kw.setdefault("name", "sector_{}x{}x{}".format(*dims))
+    base = make_shell(blocks=(tuple(dims),), panels=[0], **kw)
+    return split_block_case(base, cuts)
+

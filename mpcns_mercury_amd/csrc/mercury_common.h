@@ -87,7 +87,8 @@ struct MercRange {
     const int *n;            // device, 3*nb
 };
 
-enum MercBC { MBC_PERIODIC = 0, MBC_FARFIELD = 1, MBC_WALL = 2 };
+// face codes; MBC_INTERFACE: 1:1 connection to a neighbour block (node-index translation)
+enum MercBC { MBC_PERIODIC = 0, MBC_FARFIELD = 1, MBC_WALL = 2, MBC_INTERFACE = 3 };
 
 // slot sets of a merged sync op (passed by value as a kernel argument)
 #define MERC_MAX_SETS 12

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -125,8 +126,15 @@ public:
     std::vector<double *> hptr;
     MercBlk *dblk = nullptr;
 
-    Field(int nb, const int *dims, int g, const int *bc) {
+    // per block face: neighbour block and node-index translation of a periodic
+    // self-connection or an interface (cnb = -1 on physical faces)
+    std::vector<int> cnb;
+    std::vector<std::array<int, 3>> coff;
+
+    Field(int nb, const int *dims, int g, const int *bc, const int *nbr, const int *trans) {
         blk.resize(nb);
+        cnb.assign(6 * nb, -1);
+        coff.assign(6 * nb, {0, 0, 0});
         long long off = 0;
         for (int b = 0; b < nb; ++b) {
             MercBlk &k = blk[b];
@@ -138,7 +146,17 @@ public:
             }
             k.sJ = k.N[2];
             k.sI = k.N[1] * k.N[2];
-            for (int f = 0; f < 6; ++f) k.bc[f] = bc[6 * b + f];
+            for (int f = 0; f < 6; ++f) {
+                k.bc[f] = bc[6 * b + f];
+                const int a = f / 2;
+                if (k.bc[f] == MBC_PERIODIC) {
+                    cnb[6 * b + f] = b;
+                    coff[6 * b + f][a] = (f & 1) ? -k.m[a] : k.m[a];
+                } else if (k.bc[f] == MBC_INTERFACE) {
+                    cnb[6 * b + f] = nbr[6 * b + f];
+                    for (int c = 0; c < 3; ++c) coff[6 * b + f][c] = trans[9 * (6 * b + f) + 6 + c];
+                }
+            }
             off += (long long)k.N[0] * k.N[1] * k.N[2];
         }
         Vtot = (off + 63) / 64 * 64;
@@ -165,6 +183,17 @@ public:
     }
     void owned_hi(int b, int loc, int hi[3]) const {
         for (int a = 0; a < 3; ++a) hi[a] = blk[b].m[a] + 1 - DELTA[loc][a];
+    }
+    bool connected() const {
+        for (int x : cnb)
+            if (x >= 0) return true;
+        return false;
+    }
+    bool has_interface() const {
+        for (const auto &k : blk)
+            for (int f = 0; f < 6; ++f)
+                if (k.bc[f] == MBC_INTERFACE) return true;
+        return false;
     }
     void upload_block(int slot, int b, const double *h) {
         HIPCHK(hipMemcpy(hptr[slot] + blk[b].off, h, sizeof(double) * vol(b), hipMemcpyHostToDevice));
@@ -378,7 +407,9 @@ public:
                 pairs.push_back(F.gidx(b, ii, jj, k2));
             });
         };
-        auto phys = [&](int f) { return k.bc[f] != MBC_PERIODIC; };
+        // an edge or vertex with a periodic or interface face is an inner patch
+        // (src/2_topology/TopologyEdge.cpp:161-166): no corner default there
+        auto phys = [&](int f) { return k.bc[f] == MBC_FARFIELD || k.bc[f] == MBC_WALL; };
         for (int a = 0; a < 3; ++a)
             for (int c = a + 1; c < 3; ++c)
                 for (int sa = 0; sa < 2; ++sa)
@@ -406,80 +437,120 @@ public:
 };
 
 // ---------------------------------------------------------------------------
-// Halo: periodic self-connections with the reference's staged strip rules
-// (Halo::build_inner_1DCorner_pattern, src/4_halo/build/HaloBuildFace.cpp:6-57;
-// ghost / inner strip boxes, src/0_basic/LayoutTraits.cpp:93-130; copies with the
-// inverse index transform, src/4_halo/component_copy/HaloExchangeInner.cpp:90-117)
+// Halo: periodic self-connections and 1:1 block interfaces (translations).
+// The reference fills ghosts stage by stage (MercuryBoundary::Sync_,
+// Z4_Mercury/src/boundary/Boundary_Sync_.cpp:11-99): face strips
+// (build_inner_1DCorner_pattern, src/4_halo/build/HaloBuildFace.cpp:6-57; strips
+// of src/0_basic/LayoutTraits.cpp:93-130), then edge and vertex corner boxes
+// through the face connection of the patch (src/2_topology/TopologyEdge.cpp:
+// 132-217), each read at the translated index of the neighbour
+// (src/4_halo/component_copy/HaloExchangeInner.cpp:90-247).  A stage-s ghost is
+// read one hop away, where it has s-1 out-of-range axes and was written by the
+// previous stage, a physical handler or the corner default.  The plan follows
+// those hops on the host and copies straight from where the chain ends: an owned
+// DOF, or a ghost of physical faces only (written by the handlers and corner
+// defaults, which run in earlier phases).
 // ---------------------------------------------------------------------------
 class Halo {
 public:
-    static bool periodic(const MercBlk &k, int a) { return k.bc[2 * a] == MBC_PERIODIC && k.bc[2 * a + 1] == MBC_PERIODIC; }
-
-    static void periodic_pairs(const Field &F, int b, int loc, int level, std::vector<int> &pairs) {
+    static void connected_pairs(const Field &F, int b, int loc, int level, std::vector<int> &pairs) {
         const MercBlk &k = F.blk[b];
         int ohi[3];
         F.owned_hi(b, loc, ohi);
         const int G = k.g;
-        std::vector<int> axes;
-        for (int a = 0; a < 3; ++a)
-            if (periodic(k, a)) axes.push_back(a);
-        const int na = (int)axes.size();
-        // stage s: every subset of s periodic axes, every side combination
-        for (int s = 1; s <= std::min(level, na); ++s)
-            for (int mask = 0; mask < (1 << na); ++mask) {
-                if (__builtin_popcount(mask) != s) continue;
-                std::vector<int> sel;
-                for (int t = 0; t < na; ++t)
-                    if (mask & (1 << t)) sel.push_back(axes[t]);
-                for (int sides = 0; sides < (1 << s); ++sides) {
-                    Box recv;
-                    int shift[3] = {0, 0, 0};
-                    for (int a = 0; a < 3; ++a) {
-                        recv.lo[a] = 0;
-                        recv.hi[a] = ohi[a];
-                    }
-                    for (int t = 0; t < s; ++t) {
-                        const int a = sel[t], side = (sides >> t) & 1;
-                        if (side == 0) {
-                            recv.lo[a] = -G;
-                            recv.hi[a] = 0;
-                            shift[a] = +k.m[a];
-                        } else {
-                            recv.lo[a] = ohi[a];
-                            recv.hi[a] = ohi[a] + G;
-                            shift[a] = -k.m[a];
+        for (int i = -G; i < ohi[0] + G; ++i)
+            for (int j = -G; j < ohi[1] + G; ++j)
+                for (int kk = -G; kk < ohi[2] + G; ++kk) {
+                    int idx[3] = {i, j, kk}, nout = 0;
+                    for (int a = 0; a < 3; ++a) nout += (idx[a] < 0 || idx[a] >= ohi[a]);
+                    if (nout == 0 || nout > level) continue;
+                    int cb = b, q[3] = {i, j, kk}, hops = 0;
+                    for (;;) {
+                        int ch[3];
+                        F.owned_hi(cb, loc, ch);
+                        int f = -1;
+                        for (int a = 0; a < 3 && f < 0; ++a) {
+                            const int side = q[a] < 0 ? 0 : (q[a] >= ch[a] ? 1 : -1);
+                            if (side >= 0 && F.cnb[6 * cb + 2 * a + side] >= 0) f = 2 * a + side;
                         }
+                        if (f < 0) break;
+                        const auto &o = F.coff[6 * cb + f];
+                        for (int a = 0; a < 3; ++a) q[a] += o[a];
+                        cb = F.cnb[6 * cb + f];
+                        ++hops;
                     }
-                    for_box(recv, [&](int i, int j, int kk) {
-                        pairs.push_back(F.gidx(b, i, j, kk));
-                        pairs.push_back(F.gidx(b, i + shift[0], j + shift[1], kk + shift[2]));
+                    if (hops == 0) continue;
+                    pairs.push_back(F.gidx(b, i, j, kk));
+                    pairs.push_back(F.gidx(cb, q[0], q[1], q[2]));
+                }
+    }
+};
+
+// Equivalence classes of the DOFs shared through periodic and interface faces
+// (topology equivalence, src/2_topology/TopologyFaceEquiv.cpp,
+// TopologyEdgeEquiv.cpp): every owned DOF on a connected face is united with its
+// translate in the neighbour; members sorted by arena index, i.e. by EntityKey
+// (block, i, j, k); the first member is the owner.
+struct Classes {
+    std::vector<int> start[8], member[8];
+    void build(const Field &F) {
+        for (int L = 0; L < 8; ++L) {
+            start[L].assign(1, 0);
+            member[L].clear();
+        }
+        if (!F.connected()) return;
+        std::vector<int> par(F.Vtot);
+        std::vector<char> hit(F.Vtot);
+        auto find = [&](int x) {
+            while (par[x] != x) x = par[x] = par[par[x]];
+            return x;
+        };
+        for (int L = LOC_FX; L <= LOC_NODE; ++L) {
+            for (long long x = 0; x < F.Vtot; ++x) par[x] = (int)x;
+            std::fill(hit.begin(), hit.end(), 0);
+            for (int b = 0; b < F.nb(); ++b) {
+                const MercBlk &k = F.blk[b];
+                int ohi[3];
+                F.owned_hi(b, L, ohi);
+                for (int f = 0; f < 6; ++f) {
+                    const int a = f / 2, nb = F.cnb[6 * b + f];
+                    if (DELTA[L][a] != 0 || nb < 0) continue;
+                    const auto &o = F.coff[6 * b + f];
+                    Box x;
+                    for (int c = 0; c < 3; ++c) {
+                        x.lo[c] = 0;
+                        x.hi[c] = ohi[c];
+                    }
+                    x.lo[a] = (f & 1) ? k.m[a] : 0;
+                    x.hi[a] = x.lo[a] + 1;
+                    for_box(x, [&](int i, int j, int kk) {
+                        const int u = F.gidx(b, i, j, kk), v = F.gidx(nb, i + o[0], j + o[1], kk + o[2]);
+                        const int ru = find(u), rv = find(v);
+                        if (ru != rv) par[std::max(ru, rv)] = std::min(ru, rv);
+                        hit[u] = hit[v] = 1;
                     });
                 }
             }
+            std::vector<std::pair<int, int>> mem;
+            for (long long x = 0; x < F.Vtot; ++x)
+                if (hit[x]) mem.push_back({find((int)x), (int)x});
+            std::sort(mem.begin(), mem.end());
+            for (size_t t = 0; t < mem.size(); ++t) {
+                if (t > 0 && mem[t].first != mem[t - 1].first) start[L].push_back((int)t);
+                member[L].push_back(mem[t].second);
+            }
+            if (!mem.empty()) start[L].push_back((int)mem.size());
+        }
     }
-
-    // owner -> alias (Halo::sync_owner_alias_group, src/4_halo/core/HaloSyncFramework.cpp:695-712):
-    // the seam DOF at index m on a periodic axis (delta = 0) aliases the owner,
-    // the smallest EntityKey of its class, i.e. every seam index set to 0
-    static void owner_pairs(const Field &F, int b, int loc, std::vector<int> &pairs) {
-        const MercBlk &k = F.blk[b];
-        int ohi[3];
-        F.owned_hi(b, loc, ohi);
-        for (int i = 0; i < ohi[0]; ++i)
-            for (int j = 0; j < ohi[1]; ++j)
-                for (int kk = 0; kk < ohi[2]; ++kk) {
-                    int idx[3] = {i, j, kk}, o[3] = {i, j, kk};
-                    bool alias = false;
-                    for (int a = 0; a < 3; ++a)
-                        if (DELTA[loc][a] == 0 && periodic(k, a) && idx[a] == k.m[a]) {
-                            o[a] = 0;
-                            alias = true;
-                        }
-                    if (alias) {
-                        pairs.push_back(F.gidx(b, i, j, kk));
-                        pairs.push_back(F.gidx(b, o[0], o[1], o[2]));
-                    }
-                }
+    int ncls(int L) const { return (int)start[L].size() - 1; }
+    // owner -> alias copy pairs (Halo::sync_owner_alias_group,
+    // src/4_halo/core/HaloSyncFramework.cpp:695-712), orientation sign +1
+    void owner_pairs(int L, std::vector<int> &pairs) const {
+        for (int c = 0; c < ncls(L); ++c)
+            for (int t = start[L][c] + 1; t < start[L][c + 1]; ++t) {
+                pairs.push_back(member[L][t]);
+                pairs.push_back(member[L][start[L][c]]);
+            }
     }
 };
 
@@ -487,40 +558,19 @@ public:
 struct AliasClasses {
     DBuf<int> start, member, msign, owner, osign, slot_of;
     long long n = 0;
-    void build(const Field &F) {
+    void build(const Classes &C) {
         std::vector<int> st{0}, mem, ms, ow, os, sl;
-        for (int b = 0; b < F.nb(); ++b) {
-            const MercBlk &k = F.blk[b];
-            for (int a = 0; a < 3; ++a) {
-                const int loc = LOC_EX + a;
-                int ohi[3];
-                F.owned_hi(b, loc, ohi);
-                bool per[3];
-                for (int c = 0; c < 3; ++c) per[c] = DELTA[loc][c] == 0 && Halo::periodic(k, c);
-                for (int i = 0; i < ohi[0]; ++i)
-                    for (int j = 0; j < ohi[1]; ++j)
-                        for (int kk = 0; kk < ohi[2]; ++kk) {
-                            int idx[3] = {i, j, kk}, seam[3], ns = 0;
-                            bool owner = true;
-                            for (int c = 0; c < 3; ++c) {
-                                if (!per[c]) continue;
-                                if (idx[c] == k.m[c]) owner = false;
-                                if (idx[c] == 0) seam[ns++] = c;
-                            }
-                            if (!owner || ns == 0) continue;
-                            // members in ascending (i, j, k) order
-                            for (int mask = 0; mask < (1 << ns); ++mask) {
-                                int q[3] = {i, j, kk};
-                                for (int t = 0; t < ns; ++t)
-                                    if (mask & (1 << (ns - 1 - t))) q[seam[t]] = k.m[seam[t]];
-                                mem.push_back(F.gidx(b, q[0], q[1], q[2]));
-                                ms.push_back(1);
-                            }
-                            st.push_back((int)mem.size());
-                            ow.push_back(F.gidx(b, i, j, kk));
-                            os.push_back(1);
-                            sl.push_back(a);
-                        }
+        for (int a = 0; a < 3; ++a) {
+            const int L = LOC_EX + a;
+            for (int c = 0; c < C.ncls(L); ++c) {
+                for (int t = C.start[L][c]; t < C.start[L][c + 1]; ++t) {
+                    mem.push_back(C.member[L][t]);
+                    ms.push_back(1);
+                }
+                st.push_back((int)mem.size());
+                ow.push_back(C.member[L][C.start[L][c]]);
+                os.push_back(1);
+                sl.push_back(a);
             }
         }
         n = (long long)ow.size();
@@ -547,6 +597,7 @@ public:
     int march = 4;       // owned cells per thread along i in the flux pass
     GroupProg prog[G_NGROUP];
     std::map<unsigned, MergedProg> mprog;  // keyed by the bit set of groups
+    Classes cls;          // shared-DOF classes (owner / aliases) of every location
     AliasClasses alias;
     hipStream_t st = nullptr;
     unsigned long long *d_dt = nullptr;       // dt word (double bits)
@@ -571,27 +622,47 @@ public:
     double res_rel = 0.0;
     static constexpr int RES_CHECK = 4;  // the residual is read back every RES_CHECK iterations
 
-    MercurySolver(int nb, const int *dims, int g, const int *bc, const double *nodes, const MercPhys &phys) : P(phys) {
-        if (g < 2) throw std::runtime_error("nghost must be >= 2");
-        bool any_per = false, any_phys = false;
+    // Interfaces: the face's IndexTransform (neighbour node index[perm[a]] =
+    // sign[a] * index[a] + offset[a], include/2_topology/TopologyTypes.h:93-98)
+    // must be a translation gluing the face to the opposite face of the
+    // neighbour over its whole extent, declared from both sides, with at least
+    // as many cells as ghost layers along the connection.
+    static void check_topology(int nb, const int *dims, int g, const int *bc, const int *nbr, const int *trans) {
         for (int b = 0; b < nb; ++b) {
-            bool bp = false, bq = false;
-            for (int f = 0; f < 6; ++f) {
-                const int c = bc[6 * b + f];
-                if (c < 0 || c > 2) throw std::runtime_error("unknown boundary code");
-                if (c == MBC_PERIODIC) bp = true; else bq = true;
-                if (c == MBC_PERIODIC && bc[6 * b + (f ^ 1)] != MBC_PERIODIC) throw std::runtime_error("periodic faces must be paired");
-            }
-            if (bp && bq) throw std::runtime_error("blocks mixing periodic and physical faces are not supported");
-            any_per |= bp;
-            any_phys |= bq;
             for (int a = 0; a < 3; ++a)
                 if (dims[3 * b + a] < 2) throw std::runtime_error("every block needs at least 2 cells per axis");
+            for (int f = 0; f < 6; ++f) {
+                const int c = bc[6 * b + f];
+                if (c < 0 || c > 3) throw std::runtime_error("unknown boundary code");
+                if (c == MBC_PERIODIC && bc[6 * b + (f ^ 1)] != MBC_PERIODIC) throw std::runtime_error("periodic faces must be paired");
+                if (c == MBC_PERIODIC && dims[3 * b + f / 2] < g) throw std::runtime_error("a periodic axis needs at least nghost cells");
+                if (c != MBC_INTERFACE) continue;
+                if (!nbr || !trans) throw std::runtime_error("interface faces need neighbour blocks and index transforms");
+                const int n = nbr[6 * b + f], a = f / 2, fo = f ^ 1;
+                if (n < 0 || n >= nb || bc[6 * n + fo] != MBC_INTERFACE || nbr[6 * n + fo] != b)
+                    throw std::runtime_error("an interface must join opposite faces and be declared from both sides");
+                const int *t = trans + 9 * (6 * b + f), *u = trans + 9 * (6 * n + fo);
+                for (int d = 0; d < 3; ++d) {
+                    if (t[d] != d || t[3 + d] != 1)
+                        throw std::runtime_error("only translation interfaces (identity permutation, sign +1) are supported");
+                    if (t[6 + d] != -u[6 + d]) throw std::runtime_error("interface transforms of the two sides are not inverse");
+                    if (d != a && (t[6 + d] != 0 || dims[3 * b + d] != dims[3 * n + d]))
+                        throw std::runtime_error("an interface must match the whole face of the neighbour");
+                }
+                if (t[6 + a] != ((f & 1) ? -dims[3 * b + a] : dims[3 * n + a]))
+                    throw std::runtime_error("interface offset does not glue the face to the neighbour's opposite face");
+                if (dims[3 * b + a] < g || dims[3 * n + a] < g) throw std::runtime_error("an interface axis needs at least nghost cells");
+            }
         }
-        (void)any_per;
-        (void)any_phys;
+    }
+
+    MercurySolver(int nb, const int *dims, int g, const int *bc, const int *nbr, const int *trans, const double *nodes,
+                  const MercPhys &phys)
+        : P(phys) {
+        if (g < 2) throw std::runtime_error("nghost must be >= 2");
+        check_topology(nb, dims, g, bc, nbr, trans);
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        F.reset(new Field(nb, dims, g, bc));
+        F.reset(new Field(nb, dims, g, bc, nbr, trans));
         HIPCHK(hipMalloc(&d_dt, sizeof(unsigned long long)));
         HIPCHK(hipMalloc(&d_dt_next, sizeof(unsigned long long)));
         HIPCHK(hipMalloc(&d_red, 2 * sizeof(unsigned long long)));
@@ -650,8 +721,9 @@ public:
         make_range(*F, -1, 2, R_eface);
         if (const char *e = std::getenv("MERCURY_RHS_MARCH")) march = std::max(1, std::atoi(e));
         set_march(march);
+        cls.build(*F);
         build_programs();
-        alias.build(*F);
+        alias.build(cls);
         build_resist();
     }
     ~MercurySolver() {
@@ -725,7 +797,7 @@ public:
     void build_programs() {
         const Field &Fl = *F;
         for (int gid = 0; gid < G_NGROUP; ++gid) {
-            std::vector<std::unique_ptr<SyncOp>> A, Bc, C, D, E;
+            std::vector<std::unique_ptr<SyncOp>> A, Bc, C, D, H, E;
             auto add = [](std::vector<std::unique_ptr<SyncOp>> &ph, OpType t, const std::vector<int> &slots, std::vector<int> &&idx,
                           std::vector<double> &&val = {}, const double *fill = nullptr, int is_na = 0) {
                 if (idx.empty()) return;
@@ -746,7 +818,7 @@ public:
             };
             for (const GField &gf : group_fields(gid)) {
                 const auto sl = slots_of(gf);
-                std::vector<int> fillA, copyB, setd, wallC, zeroC, copyD, ownE;
+                std::vector<int> fillA, copyB, setd, wallC, zeroC, copyD, haloH, ownE;
                 std::vector<double> setv;
                 const double *fill = nullptr;
                 static const double zero3[3] = {0, 0, 0};
@@ -815,9 +887,9 @@ public:
                         }
                     }
                     BoundaryCore::corner_default(Fl, b, gf.loc, copyD);
-                    Halo::periodic_pairs(Fl, b, gf.loc, gf.level, copyD);
-                    if (gf.owner_sync) Halo::owner_pairs(Fl, b, gf.loc, ownE);
+                    Halo::connected_pairs(Fl, b, gf.loc, gf.level, haloH);
                 }
+                if (gf.owner_sync) cls.owner_pairs(gf.loc, ownE);
                 if (!fillA.empty()) add(A, OP_FILL, sl, std::move(fillA), {}, fill);
                 if (!setd.empty()) add(A, OP_SETV, sl, std::move(setd), std::move(setv));
                 add(Bc, OP_COPY, sl, std::move(copyB));
@@ -827,10 +899,11 @@ public:
                     add(C, OP_FILL, sl, std::move(zeroC), {}, z);
                 }
                 add(D, OP_COPY, sl, std::move(copyD));
+                add(H, OP_COPY, sl, std::move(haloH));
                 add(E, OP_COPY, sl, std::move(ownE));
             }
             int phase = 0;
-            for (auto *ph : {&A, &Bc, &C, &D, &E}) {
+            for (auto *ph : {&A, &Bc, &C, &D, &H, &E}) {
                 for (auto &o : *ph) {
                     o->phase = phase;
                     prog[gid].ops.push_back(std::move(o));
@@ -855,7 +928,7 @@ public:
     void build_merged(unsigned gset) {
         MergedProg &mp = mprog[gset];
         const int g = F->blk[0].g;
-        for (int phase = 0; phase < 5; ++phase)
+        for (int phase = 0; phase < 6; ++phase)
             for (OpType t : {OP_FILL, OP_SETV, OP_WALLF, OP_COPY}) {
                 std::unique_ptr<MergedOp> m(new MergedOp());
                 m->type = t;
@@ -934,6 +1007,12 @@ public:
         std::vector<int> q, ax, si, sj;
         std::vector<double> eta;
         const Field &Fl = *F;
+        std::vector<char> alias_ax[3];  // non-owner members of the edge classes, per axis
+        for (int a = 0; a < 3; ++a) {
+            alias_ax[a].assign(Fl.Vtot, 0);
+            for (int c = 0; c < cls.ncls(LOC_EX + a); ++c)
+                for (int t = cls.start[LOC_EX + a][c] + 1; t < cls.start[LOC_EX + a][c + 1]; ++t) alias_ax[a][cls.member[LOC_EX + a][t]] = 1;
+        }
         for (int b = 0; b < Fl.nb(); ++b) {
             const MercBlk &k = Fl.blk[b];
             const BlockGeom &G = *geom[b];
@@ -952,11 +1031,7 @@ public:
                 for (int i = 0; i < hi[0]; ++i)
                     for (int j = 0; j < hi[1]; ++j)
                         for (int kk = 0; kk < hi[2]; ++kk) {
-                            const int idx[3] = {i, j, kk};
-                            bool alias_e = false;
-                            for (int c = 0; c < 3; ++c)
-                                if (DELTA[loc][c] == 0 && k.bc[2 * c] == MBC_PERIODIC && idx[c] == k.m[c]) alias_e = true;
-                            if (alias_e) continue;
+                            if (alias_ax[a][Fl.gidx(b, i, j, kk)]) continue;
                             const long long l = Fl.gidx(b, i, j, kk) - k.off;
                             if (std::fabs(dl[l]) <= 1.0e-12 || al[l] == 0.0 || ff[l]) continue;
                             const auto r0 = G.node(i, j, kk), r1 = G.node(i + (a == 0), j + (a == 1), kk + (a == 2));
@@ -979,8 +1054,12 @@ public:
         }
         // the CG applies the linear operator, whose faces and edges lie within two
         // points of an unknown (and of its periodic images): per block the bounding
-        // box of the unknowns grown by 2, the whole axis where the block is periodic
-        {
+        // box of the unknowns grown by 2, the whole axis where the block is periodic.
+        // With interfaces, owner -> alias copies reach into neighbour blocks, so
+        // every block takes its whole owned range.
+        if (Fl.has_interface()) {
+            make_range(Fl, 0, 1, R_res);
+        } else {
             std::vector<int> lo(3 * Fl.nb(), 0), nn(3 * Fl.nb(), 0);
             std::vector<int> bmin(3 * Fl.nb(), 1 << 30), bmax(3 * Fl.nb(), -1);
             for (size_t e = 0; e < q.size(); ++e) {
@@ -1436,8 +1515,8 @@ const char *mercury_last_error(void) { return merc::g_last_error.c_str(); }
 
 int mercury_version(void) { return MERCURY_HIP_ABI_VERSION; }
 
-int mercury_create(MercuryContext **out, int nblocks, const int *dims, int nghost, const int *bc, const double *owned_nodes,
-                   const MercuryPhysics *phys) {
+int mercury_create_connected(MercuryContext **out, int nblocks, const int *dims, int nghost, const int *bc, const int *nbr,
+                             const int *transform, const double *owned_nodes, const MercuryPhysics *phys) {
     if (!out || nblocks <= 0 || !dims || !bc || !owned_nodes || !phys) {
         merc::g_last_error = "mercury_create: invalid argument";
         return -1;
@@ -1446,9 +1525,14 @@ int mercury_create(MercuryContext **out, int nblocks, const int *dims, int nghos
         MercPhys P;
         std::memcpy(&P, phys, sizeof(P));
         std::unique_ptr<MercuryContext> c(new MercuryContext());
-        c->s.reset(new merc::MercurySolver(nblocks, dims, nghost, bc, owned_nodes, P));
+        c->s.reset(new merc::MercurySolver(nblocks, dims, nghost, bc, nbr, transform, owned_nodes, P));
         *out = c.release();
     })
+}
+
+int mercury_create(MercuryContext **out, int nblocks, const int *dims, int nghost, const int *bc, const double *owned_nodes,
+                   const MercuryPhysics *phys) {
+    return mercury_create_connected(out, nblocks, dims, nghost, bc, nullptr, nullptr, owned_nodes, phys);
 }
 
 int mercury_destroy(MercuryContext *ctx) {

@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(_HERE, "libmercury_hip.so")
 _lib = None
 
 # every symbol declared in include/mercury_hip.h
-ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercury_destroy", "mercury_field_info",
+ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercury_create_connected", "mercury_destroy",
+               "mercury_field_info",
                "mercury_block_volume", "mercury_total_volume", "mercury_upload", "mercury_download",
                "mercury_init_bface", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
                "mercury_divb", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
@@ -40,6 +41,8 @@ def load():
     vp, ip, dp, cp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), ctypes.c_char_p
     L.mercury_last_error.restype = ctypes.c_char_p
     L.mercury_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ip, ctypes.c_int, ip, dp, ctypes.POINTER(MercuryPhysics)]
+    L.mercury_create_connected.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ip, ctypes.c_int, ip, ip, ip, dp,
+                                           ctypes.POINTER(MercuryPhysics)]
     L.mercury_destroy.argtypes = [vp]
     L.mercury_field_info.argtypes = [cp, ip, ip, ip]
     L.mercury_block_volume.restype = ctypes.c_longlong
@@ -111,8 +114,15 @@ class Mercury:
         dims = case.dims_array()
         bc = case.bc_array()
         nodes = case.nodes_flat()
-        _chk(L.mercury_create(ctypes.byref(self._h), case.nb, dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), case.g,
-                              bc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(nodes), ctypes.byref(self.phys)))
+        ip = ctypes.POINTER(ctypes.c_int)
+        if case.has_interfaces():
+            nbr, trans = case.nbr_array(), case.trans_array()
+            _chk(L.mercury_create_connected(ctypes.byref(self._h), case.nb, dims.ctypes.data_as(ip), case.g, bc.ctypes.data_as(ip),
+                                            nbr.ctypes.data_as(ip), trans.ctypes.data_as(ip), _dp(nodes),
+                                            ctypes.byref(self.phys)))
+        else:
+            _chk(L.mercury_create(ctypes.byref(self._h), case.nb, dims.ctypes.data_as(ip), case.g, bc.ctypes.data_as(ip),
+                                  _dp(nodes), ctypes.byref(self.phys)))
         for ib, fl in enumerate(case.fields):
             for name, arr in fl.items():
                 self.set(name, ib, arr)
