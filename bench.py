@@ -137,6 +137,9 @@ def main():
                     help="first run three device-to-device copies of MIB MiB (a kernel of known traffic for HBM counters)")
     ap.add_argument("--resist-steps", type=int, default=20,
                     help="steps of the shell with the implicit resistive correction (0: skip that measurement)")
+    ap.add_argument("--rshell-steps", type=int, default=20,
+                    help="steps of the resistive shell (4 blocks glued by interfaces, exponential eta) per form; 0 skips it")
+    ap.add_argument("--rshell-scale", type=float, default=1.0, help="size factor of the resistive shell (1: 18.9 M cells)")
     ap.add_argument("--ensemble-runs", type=int, default=8,
                     help="members of the batched-ensemble measurement (0: skip; the full configuration has 64)")
     ap.add_argument("--forms", default="fp64_plain,mixed,fp64_fused",
@@ -304,6 +307,37 @@ def main():
         for m in members:
             m.close()
         print(f"[bench] ensemble setup+run {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    # resistive shell: 256 x 192 x 384 cells in 2 x 2 blocks glued by interfaces, two-fluid
+    # Hall MHD with the implicit resistive correction on every edge (eta = 1e-3 exp(-(r-1)/0.1))
+    rshell = None
+    if args.rshell_steps > 0:
+        t0 = time.perf_counter()
+        rc = cases.make_resistive_shell(scale=args.rshell_scale)
+        rcells = rc.ncells()
+        rg = Mercury(rc)
+        rg.prepare()
+        rshell = {"cells": rcells, "blocks": [list(b.dims) for b in rc.blocks], "steps": args.rshell_steps,
+                  "setup_s": time.perf_counter() - t0}
+        for form, label in ((1, "fp64_fused"), (2, "mixed")):
+            rg.set_form(form)
+            rg.step(2, 0)
+            rg.synchronize()
+            its = []
+            t1 = time.perf_counter()
+            for _ in range(args.rshell_steps):
+                rg.step(1, 0)
+                its.append(rg.resist_stats()[1])
+            rg.synchronize()
+            el = time.perf_counter() - t1
+            n_unk, _, rel = rg.resist_stats()
+            rshell[label] = {"ms_per_step": el * 1e3 / args.rshell_steps, "cell_updates_per_s": rcells * args.rshell_steps / el,
+                             "unknown_edges": n_unk, "cg_iterations_mean": float(np.mean(its)), "last_rel_residual": rel,
+                             "divB_normalised": rg.divb()[0]}
+            print(f"[bench] resistive shell {label}: {rcells} cells, {el * 1e3 / args.rshell_steps:.3f} ms/step, "
+                  f"{n_unk} unknown edges, {np.mean(its):.1f} CG iterations/step", file=sys.stderr, flush=True)
+        rshell["mixed_over_fp64"] = rshell["mixed"]["cell_updates_per_s"] / rshell["fp64_fused"]["cell_updates_per_s"]
+        rg.close()
+        print(f"[bench] resistive shell setup+run {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     times = op_times["fp64_fused"]
     sync_times = {op: g.time_op(op, 20) for op in ("sync_ucell", "sync_bface", "sync_eedge", "sync_ehall", "sync_jedge",
                                                    "sync_bcell", "sync_jcell")}
@@ -350,6 +384,7 @@ def main():
         "kernel_ms": op_times,
         "resistive": resist,
         "ensemble": ensemble,
+        "resistive_shell": rshell,
     }
     print(json.dumps(out), flush=True)
 

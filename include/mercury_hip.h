@@ -66,6 +66,10 @@ typedef struct MercuryPhysics {
      * and the CG stopping rule |b - A x| <= max(rtol |b|, atol), at most max_it iterations */
     double resist_enabled, inver_Rem, resist_r_inner, resist_r_outer, resist_width;
     double resist_rtol, resist_atol, resist_max_it;
+    /* eta profile: 0 the reference's window inver_Rem * 0.5 (tanh((r - r_inner) / width)
+     * - tanh((r - r_outer) / width)) (MercurySolver_Tools.cpp:50-69); 1 an exponential
+     * inver_Rem * exp(-(r - r_inner) / width) (no reference counterpart) */
+    double resist_profile;
 } MercuryPhysics;
 
 typedef struct MercuryContext MercuryContext;

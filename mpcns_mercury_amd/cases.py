@@ -287,7 +287,7 @@ def sector_nodes(dims, panel, r0=1.0, r1=4.0, half_angle=np.deg2rad(40.0)):
 def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, integrator: int = 0,
                scale: float = 1.0, hall_coef: float | None = None, resist: float | None = None,
                inflow: float = 1.0, imf=(0.0, 0.0, -0.2), na_fraction: float = 0.01, panels=None,
-               name: str | None = None) -> Case:
+               name: str | None = None, resist_profile: str = "window") -> Case:
     """Two-fluid Hall MHD: H+ and Na+ (mass ratio 1:23, density ratio 1:0.01) plus
     electron pressure (ambipolar term), inflow u = (-1,0,0), IMF B = (0,0,-0.2),
     the fluid side of a perfectly conducting sphere at r = 1 on every block's
@@ -309,7 +309,12 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
     units.is_local_artificial_resistivity = True
     units.hall_coef_override = hall_coef
     if resist is not None:
-        set_resistivity(units, resist, 0.84, 1.04, 0.02)
+        if resist_profile == "exp":
+            # eta(r) = inver_Rem * exp(-(r - 1) / 0.1): surface-peaked resistivity
+            set_resistivity(units, resist, 1.0, 1.0, 0.1)
+        else:
+            set_resistivity(units, resist, 0.84, 1.04, 0.02)
+        units.mercury_eta_profile = resist_profile
     P = derive(units)
     gamma = P.gamma
     blks, fields = [], []
@@ -399,4 +404,23 @@ def make_split_sector(dims=(24, 20, 16), cuts=((24,), (10, 10), (16,)), **kw) ->
     kw.setdefault("name", "sector_{}x{}x{}".format(*dims))
     base = make_shell(blocks=(tuple(dims),), panels=[0], **kw)
     return split_block_case(base, cuts)
+
+
+C5_DIMS = (256, 192, 384)
+
+
+def make_resistive_shell(scale: float = 1.0, g: int = 4, steps: int = 500, integrator: int = 0,
+                         inver_Rem: float = 1e-3) -> Case:
+    """The resistive shell configuration: one shell sector of 256 x 192 x 384 cells
+    (times ``scale``) on r in [1, 4], cut into 2 x 2 blocks across the two angular
+    axes and glued by interfaces, two-fluid Hall MHD (the reference carries two ion
+    species), with the implicit resistive correction on every edge of the shell
+    for eta(r) = inver_Rem * exp(-(r - 1) / 0.1) (``resist_profile`` "exp").  At
+    scale 1: 18 874 368 cells in four blocks of 256 x 96 x 192."""
+    dims = tuple(max(2 * g, 2 * int(round(d * scale / 2))) for d in C5_DIMS)
+    dims = (max(g, int(round(C5_DIMS[0] * scale))),) + dims[1:]
+    base = make_shell(blocks=(dims,), panels=[0], g=g, steps=steps, integrator=integrator, resist=inver_Rem,
+                      resist_profile="exp", name="resistive_shell_{}x{}x{}".format(*dims))
+    cuts = ((dims[0],), (dims[1] // 2, dims[1] - dims[1] // 2), (dims[2] // 2, dims[2] - dims[2] // 2))
+    return split_block_case(base, cuts, name=base.name + "_2x2")
 

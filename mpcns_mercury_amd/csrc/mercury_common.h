@@ -66,6 +66,7 @@ struct MercPhys {
     // implicit resistive correction (MercurySolver.cpp:173-184, MercurySolver_Tools.cpp:44-69)
     double resist_enabled, inver_Rem, resist_r_inner, resist_r_outer, resist_width;
     double resist_rtol, resist_atol, resist_max_it;
+    double resist_profile;  // 0: the reference's tanh window, 1: exponential
 };
 
 // per-block geometry of the box

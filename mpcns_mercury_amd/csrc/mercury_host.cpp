@@ -1037,9 +1037,14 @@ public:
                             const auto r0 = G.node(i, j, kk), r1 = G.node(i + (a == 0), j + (a == 1), kk + (a == 2));
                             const double xm = 0.5 * (r0[0] + r1[0]), ym = 0.5 * (r0[1] + r1[1]), zm = 0.5 * (r0[2] + r1[2]);
                             const double r = std::sqrt(xm * xm + ym * ym + zm * zm);
-                            const double shape = 0.5 * (std::tanh((r - P.resist_r_inner) / P.resist_width) -
-                                                        std::tanh((r - P.resist_r_outer) / P.resist_width));
-                            const double sh = (!std::isfinite(shape) || shape <= 1.0e-10) ? 0.0 : std::min(1.0, shape);
+                            double sh;
+                            if (P.resist_profile == 1.0) {  // exponential profile (no reference counterpart)
+                                sh = std::exp(-(r - P.resist_r_inner) / P.resist_width);
+                            } else {
+                                const double shape = 0.5 * (std::tanh((r - P.resist_r_inner) / P.resist_width) -
+                                                            std::tanh((r - P.resist_r_outer) / P.resist_width));
+                                sh = (!std::isfinite(shape) || shape <= 1.0e-10) ? 0.0 : std::min(1.0, shape);
+                            }
                             const double e = P.inver_Rem * sh;
                             if (!(e > 0.0)) continue;
                             q.push_back(Fl.gidx(b, i, j, kk));

@@ -50,6 +50,7 @@ class MercuryPhysics(ctypes.Structure):
         ("resist_enabled", ctypes.c_double), ("inver_Rem", ctypes.c_double),
         ("resist_r_inner", ctypes.c_double), ("resist_r_outer", ctypes.c_double), ("resist_width", ctypes.c_double),
         ("resist_rtol", ctypes.c_double), ("resist_atol", ctypes.c_double), ("resist_max_it", ctypes.c_double),
+        ("resist_profile", ctypes.c_double),
     ]
 
 
@@ -95,6 +96,9 @@ class MercuryUnits:
     implicit_resistive_rtol: float = 1.0e-10
     implicit_resistive_atol: float = 1.0e-30
     implicit_resistive_max_it: int = 500
+    # eta profile: "window" (the reference's, MercurySolver_Tools.cpp:50-69) or "exp",
+    # inver_Rem * exp(-(r - r_inner) / width) (an extension for the resistive shell case)
+    mercury_eta_profile: str = "window"
 
     def alfven_units(self) -> "MercuryUnits":
         """Choose B_ref so that inver_MA2 = c_hall = 1 (B in Alfven units)."""
@@ -169,4 +173,7 @@ def derive(u: MercuryUnits) -> MercuryPhysics:
     P.resist_r_inner, P.resist_r_outer, P.resist_width = u.mercury_eta_r_inner, u.mercury_eta_r_outer, u.mercury_eta_width
     P.resist_rtol, P.resist_atol = u.implicit_resistive_rtol, u.implicit_resistive_atol
     P.resist_max_it = float(u.implicit_resistive_max_it)
+    if u.mercury_eta_profile not in ("window", "exp"):
+        raise ValueError(f"unknown eta profile {u.mercury_eta_profile!r}")
+    P.resist_profile = 1.0 if u.mercury_eta_profile == "exp" else 0.0
     return P

@@ -53,6 +53,7 @@ typedef struct {
     /* implicit resistive correction (MercurySolver.cpp:173-184, MercurySolver_Tools.cpp:44) */
     double resist_enabled, inver_Rem, resist_r_inner, resist_r_outer, resist_width;
     double resist_rtol, resist_atol, resist_max_it;
+    double resist_profile; /* 0: the reference's window, 1: exponential (no reference counterpart) */
 } OrcPhys;
 
 /* face codes; BC_INTERFACE: a 1:1 inner connection to another (or the same)
@@ -1859,8 +1860,10 @@ static void curl_e2f(Orc *o, int se, int sf, double mult) {
     }
 }
 
-/* MercuryResistivityShape_ (Z4_Mercury/src/physics/MercurySolver_Tools.cpp:50-69) */
+/* MercuryResistivityShape_ (Z4_Mercury/src/physics/MercurySolver_Tools.cpp:50-69);
+ * profile 1: exp(-(r - r_inner) / width), an extension with no reference counterpart */
 static double resist_shape(const OrcPhys *P, double r) {
+    if (P->resist_profile == 1.0) return exp(-(r - P->resist_r_inner) / P->resist_width);
     const double shape = 0.5 * (tanh((r - P->resist_r_inner) / P->resist_width) - tanh((r - P->resist_r_outer) / P->resist_width));
     if (!isfinite(shape) || shape <= 1.0e-10) return 0.0;
     return dmin(1.0, shape);

@@ -138,3 +138,32 @@ def test_eta_window_and_unknown_set(oracle_mod):
         np.testing.assert_allclose(own[I, J, K], expect, rtol=1e-13, atol=1e-15 * P.inver_Rem)
         # inside the window the amplitude reaches inver_Rem
         assert own.max() <= P.inver_Rem and own.max() > 0.9 * P.inver_Rem
+
+
+def test_eta_exponential_profile_and_unknown_set(oracle_mod):
+    """Resistive shell: eta = inver_Rem * exp(-(r - 1) / 0.1) at the edge midpoint on
+    every unknown; unknowns are the owned edges that own their class (one copy of
+    each edge shared by the 2 x 2 blocks), off the far-field boundary layers."""
+    case = cases.make_resistive_shell(scale=0.06)
+    o = oracle_mod.Oracle(case)
+    P = case.phys
+    assert P.resist_profile == 1.0 and P.resist_r_inner == 1.0 and P.resist_width == 0.1
+    g = case.g
+    n_unknown = 0
+    for ib, blk in enumerate(case.blocks):
+        m = blk.dims
+        for a, s in enumerate(AX):
+            eta = o.get("eta_res" + s, ib)[0]
+            hi = [m[0] + (a != 0), m[1] + (a != 1), m[2] + (a != 2)]
+            own = eta[g:g + hi[0], g:g + hi[1], g:g + hi[2]]
+            I, J, K = np.nonzero(own > 0)
+            n_unknown += len(I)
+            e = np.eye(3, dtype=int)[a]
+            mid = 0.5 * (blk.nodes[:, I, J, K] + blk.nodes[:, I + e[0], J + e[1], K + e[2]])
+            r = np.sqrt((mid * mid).sum(axis=0))
+            np.testing.assert_allclose(own[I, J, K], P.inver_Rem * np.exp(-(r - 1.0) / 0.1), rtol=1e-14)
+            # the interface planes: block 0 owns the eta-cut edges, so the +eta face of
+            # block 0 carries unknowns and the -eta face of block 2 (its neighbour) none
+            if ib == 2 and a != 1:
+                assert not np.any(own[:, 0, :] > 0)
+    assert n_unknown > 1000
