@@ -616,7 +616,7 @@ public:
     ResDofs rd;
     DBuf<int> r_q, r_ax, r_sI, r_sJ;
     DBuf<double> r_eta, r_vec, r_part, r_S;
-    double *rx = nullptr, *rr = nullptr, *rz = nullptr, *rp = nullptr, *rAp = nullptr, *rb = nullptr, *rdg = nullptr, *rw = nullptr,
+    double *rx = nullptr, *rr = nullptr, *rp = nullptr, *rAp = nullptr, *rb = nullptr, *rdg = nullptr, *rw = nullptr,
            *rt = nullptr;
     int res_its = 0;
     double res_rel = 0.0;
@@ -1103,10 +1103,10 @@ public:
         rd.sI = r_sI.p;
         rd.sJ = r_sJ.p;
         rd.eta = r_eta.p;
-        r_vec.upload(std::vector<double>(9 * (size_t)std::max(n, 1), 0.0));
+        r_vec.upload(std::vector<double>(8 * (size_t)std::max(n, 1), 0.0));
         double *v = r_vec.p;
         const size_t m = (size_t)std::max(n, 1);
-        rx = v; rr = v + m; rz = v + 2 * m; rp = v + 3 * m; rAp = v + 4 * m; rb = v + 5 * m; rdg = v + 6 * m; rw = v + 7 * m; rt = v + 8 * m;
+        rx = v; rr = v + m; rp = v + 2 * m; rAp = v + 3 * m; rb = v + 4 * m; rdg = v + 5 * m; rw = v + 6 * m; rt = v + 7 * m;
         r_part.upload(std::vector<double>(3 * (size_t)std::max(res_blocks(n), 1), 0.0));
         r_S.upload(std::vector<double>(CG_NSCALAR, 0.0));
     }
@@ -1144,7 +1144,7 @@ public:
         launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
         launch_res_rhs(st, rd, Fd(), dt, rb, rdg, rw);
         res_apply(rx, rAp, dt, false, nullptr);
-        launch_cg_init(st, n, rb, rAp, rdg, rw, rr, rz, rp, r_part.p, r_S.p);
+        launch_cg_init(st, n, rb, rAp, rdg, rw, rr, rp, r_part.p, r_S.p);
         int its = 0;
         double S[CG_NSCALAR];
         for (;;) {
@@ -1160,8 +1160,8 @@ public:
             }
             res_apply(rp, rAp, dt, true, rw);
             launch_cg_pap(st, n, r_part.p, r_S.p);
-            launch_cg_xr(st, n, r_S.p, rp, rAp, rdg, rw, rx, rr, rz, r_part.p, r_S.p);
-            launch_cg_p(st, n, r_S.p, rz, rp);
+            launch_cg_xr(st, n, r_S.p, rp, rAp, rdg, rw, rx, rr, r_part.p, r_S.p);
+            launch_cg_p(st, n, r_S.p, rr, rdg, rp);
             ++its;
         }
         res_its = its;

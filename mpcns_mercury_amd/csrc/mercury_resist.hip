@@ -4,9 +4,13 @@
 // axis); the field parts of the operator (curls, synchronisation, alias mean)
 // reuse the generic kernels on the Eres / dB / dJ slots.
 //
-// Reductions are deterministic: every 256-thread block writes its partial sum
-// (fixed tree order) and one block adds the partials in block order.
+// Reductions are deterministic: the reducing kernels run a capped grid (at most
+// RGRID blocks) with grid-stride loops, so each thread sums a fixed set of
+// unknowns in a fixed order; every block writes its tree sum, and one block adds
+// the (at most RGRID) partials in block order.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "mercury_device.h"
 #include "mercury_resist.h"
@@ -15,7 +19,8 @@ namespace {
 using namespace merc_dev;
 using namespace merc;
 
-constexpr int RB = 256;  // threads per block of the unknown-vector kernels
+constexpr int RB = 256;     // threads per block of the unknown-vector kernels
+constexpr int RGRID = 2048; // blocks of the reducing kernels (partials per reduced quantity)
 
 // block tree sum of v into part[blockIdx.x]
 DEVI void block_sum(double v, double *__restrict__ part) {
@@ -49,12 +54,11 @@ __global__ void k_pick(int n, const int *__restrict__ q, const int *__restrict__
 __global__ void k_gather(int n, const int *__restrict__ q, const int *__restrict__ ax, const double *__restrict__ eta,
                          double *const *__restrict__ F, double *__restrict__ y, const double *__restrict__ p,
                          const double *__restrict__ w, double *__restrict__ part) {
-    const int t = blockIdx.x * RB + threadIdx.x;
     double v = 0.0;
-    if (t < n) {
+    for (int t = blockIdx.x * RB + threadIdx.x; t < n; t += gridDim.x * RB) {
         const double yy = F[SL_ERES + ax[t]][q[t]] - eta[t] * F[SL_DJ + ax[t]][q[t]];
         y[t] = yy;
-        if (w) v = p[t] * yy * w[t];
+        if (w) v += p[t] * yy * w[t];
     }
     if (w) block_sum(v, part);
 }
@@ -80,20 +84,18 @@ __global__ void k_rhs(int n, const int *__restrict__ q, const int *__restrict__ 
     w[t] = 1.0 / ea;
 }
 
-// r = b - Ax, z = r / dg, p = z; partials of b.b, r.r, r.z.w
+// r = b - Ax, p = z = r / dg (z is not stored: it is recomputed from r and dg);
+// partials of b.b, r.r, r.z.w
 __global__ void k_cg_init(int n, const double *__restrict__ b, const double *__restrict__ Ax, const double *__restrict__ dg,
-                          const double *__restrict__ w, double *__restrict__ r, double *__restrict__ z, double *__restrict__ p,
-                          double *__restrict__ part) {
-    const int t = blockIdx.x * RB + threadIdx.x;
+                          const double *__restrict__ w, double *__restrict__ r, double *__restrict__ p, double *__restrict__ part) {
     double bb = 0.0, rr = 0.0, rz = 0.0;
-    if (t < n) {
+    for (int t = blockIdx.x * RB + threadIdx.x; t < n; t += gridDim.x * RB) {
         const double rv = b[t] - Ax[t], zv = rv / dg[t];
         r[t] = rv;
-        z[t] = zv;
         p[t] = zv;
-        bb = b[t] * b[t];
-        rr = rv * rv;
-        rz = rv * zv * w[t];
+        bb += b[t] * b[t];
+        rr += rv * rv;
+        rz += rv * zv * w[t];
     }
     block_sum(bb, part);
     __syncthreads();
@@ -102,30 +104,29 @@ __global__ void k_cg_init(int n, const double *__restrict__ b, const double *__r
     block_sum(rz, part + 2 * gridDim.x);
 }
 
-// x += alpha p, r -= alpha Ap, z = r / dg; partials of r.r and r.z.w
+// x += alpha p, r -= alpha Ap; partials of r.r and r.z.w with z = r / dg
 __global__ void k_cg_xr(int n, const double *__restrict__ S, const double *__restrict__ p, const double *__restrict__ Ap,
                         const double *__restrict__ dg, const double *__restrict__ w, double *__restrict__ x,
-                        double *__restrict__ r, double *__restrict__ z, double *__restrict__ part) {
-    const int t = blockIdx.x * RB + threadIdx.x;
+                        double *__restrict__ r, double *__restrict__ part) {
     const double al = S[CG_ALPHA];
     double rr = 0.0, rz = 0.0;
-    if (t < n) {
+    for (int t = blockIdx.x * RB + threadIdx.x; t < n; t += gridDim.x * RB) {
         x[t] += al * p[t];
         const double rv = r[t] - al * Ap[t], zv = rv / dg[t];
         r[t] = rv;
-        z[t] = zv;
-        rr = rv * rv;
-        rz = rv * zv * w[t];
+        rr += rv * rv;
+        rz += rv * zv * w[t];
     }
     block_sum(rr, part);
     __syncthreads();
     block_sum(rz, part + gridDim.x);
 }
 
-// p = z + beta p
-__global__ void k_cg_p(int n, const double *__restrict__ S, const double *__restrict__ z, double *__restrict__ p) {
+// p = z + beta p, z = r / dg
+__global__ void k_cg_p(int n, const double *__restrict__ S, const double *__restrict__ r, const double *__restrict__ dg,
+                       double *__restrict__ p) {
     const int t = blockIdx.x * RB + threadIdx.x;
-    if (t < n) p[t] = z[t] + S[CG_BETA] * p[t];
+    if (t < n) p[t] = r[t] / dg[t] + S[CG_BETA] * p[t];
 }
 
 // sum of nq partial arrays of length nb (in block order) and the scalar update
@@ -176,8 +177,9 @@ __global__ void k_bres_update(const MercRange R, const MercBlk *__restrict__ B, 
 namespace merc {
 
 static inline unsigned rblk(int n) { return (unsigned)((n + RB - 1) / RB); }
+static inline unsigned rgrid(int n) { return std::min(rblk(n), (unsigned)RGRID); }
 
-int res_blocks(int n) { return (int)rblk(n); }
+int res_blocks(int n) { return (int)rgrid(n); }
 
 void launch_res_scatter(hipStream_t st, const ResDofs &d, const double *x, double *const *F, int slot) {
     if (d.n > 0) hipLaunchKernelGGL(k_scatter, dim3(rblk(d.n)), dim3(RB), 0, st, d.n, d.q, d.ax, x, F, slot);
@@ -186,28 +188,28 @@ void launch_res_pick(hipStream_t st, const ResDofs &d, double *const *F, int slo
     if (d.n > 0) hipLaunchKernelGGL(k_pick, dim3(rblk(d.n)), dim3(RB), 0, st, d.n, d.q, d.ax, F, slot, x);
 }
 void launch_res_gather(hipStream_t st, const ResDofs &d, double *const *F, double *y, const double *p, const double *w, double *part) {
-    if (d.n > 0) hipLaunchKernelGGL(k_gather, dim3(rblk(d.n)), dim3(RB), 0, st, d.n, d.q, d.ax, d.eta, F, y, p, w, part);
+    if (d.n > 0) hipLaunchKernelGGL(k_gather, dim3(rgrid(d.n)), dim3(RB), 0, st, d.n, d.q, d.ax, d.eta, F, y, p, w, part);
 }
 void launch_res_rhs(hipStream_t st, const ResDofs &d, double *const *F, double dt, double *b, double *dg, double *w) {
     if (d.n > 0) hipLaunchKernelGGL(k_rhs, dim3(rblk(d.n)), dim3(RB), 0, st, d.n, d.q, d.ax, d.sI, d.sJ, d.eta, F, dt, b, dg, w);
 }
-void launch_cg_init(hipStream_t st, int n, const double *b, const double *Ax, const double *dg, const double *w, double *r, double *z,
-                    double *p, double *part, double *S) {
+void launch_cg_init(hipStream_t st, int n, const double *b, const double *Ax, const double *dg, const double *w, double *r, double *p,
+                    double *part, double *S) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cg_init, dim3(rblk(n)), dim3(RB), 0, st, n, b, Ax, dg, w, r, z, p, part);
-    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(RB), 0, st, (int)rblk(n), 3, part, S, 0);
+    hipLaunchKernelGGL(k_cg_init, dim3(rgrid(n)), dim3(RB), 0, st, n, b, Ax, dg, w, r, p, part);
+    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(RB), 0, st, (int)rgrid(n), 3, part, S, 0);
 }
 void launch_cg_pap(hipStream_t st, int n, const double *part, double *S) {
-    if (n > 0) hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(RB), 0, st, (int)rblk(n), 1, part, S, 1);
+    if (n > 0) hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(RB), 0, st, (int)rgrid(n), 1, part, S, 1);
 }
 void launch_cg_xr(hipStream_t st, int n, const double *S, const double *p, const double *Ap, const double *dg, const double *w, double *x,
-                  double *r, double *z, double *part, double *Sw) {
+                  double *r, double *part, double *Sw) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cg_xr, dim3(rblk(n)), dim3(RB), 0, st, n, S, p, Ap, dg, w, x, r, z, part);
-    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(RB), 0, st, (int)rblk(n), 2, part, Sw, 2);
+    hipLaunchKernelGGL(k_cg_xr, dim3(rgrid(n)), dim3(RB), 0, st, n, S, p, Ap, dg, w, x, r, part);
+    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(RB), 0, st, (int)rgrid(n), 2, part, Sw, 2);
 }
-void launch_cg_p(hipStream_t st, int n, const double *S, const double *z, double *p) {
-    if (n > 0) hipLaunchKernelGGL(k_cg_p, dim3(rblk(n)), dim3(RB), 0, st, n, S, z, p);
+void launch_cg_p(hipStream_t st, int n, const double *S, const double *r, const double *dg, double *p) {
+    if (n > 0) hipLaunchKernelGGL(k_cg_p, dim3(rblk(n)), dim3(RB), 0, st, n, S, r, dg, p);
 }
 void launch_bres_update(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, double dt) {
     if (R.total > 0) hipLaunchKernelGGL(k_bres_update, dim3((unsigned)((R.total + 255) / 256)), dim3(256), 0, st, R.dev, B, F, dt);
