@@ -600,6 +600,12 @@ public:
     Classes cls;          // shared-DOF classes (owner / aliases) of every location
     AliasClasses alias;
     hipStream_t st = nullptr;
+    // side stream of the fused stage: the fluid RHS runs there while the
+    // induction EMF chain runs on st (they read the same state and write
+    // disjoint fields); both streams keep the default priority
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool overlap = true;
     unsigned long long *d_dt = nullptr;       // dt word (double bits)
     unsigned long long *d_dt_next = nullptr;  // next step's dt, reduced at the end of the derived fields
     int form = 1;                             // 0: one kernel per reference function, 1: fused FP64, 2: mixed
@@ -720,6 +726,10 @@ public:
         make_range(*F, 0, 1, R_node);
         make_range(*F, -1, 2, R_eface);
         if (const char *e = std::getenv("MERCURY_RHS_MARCH")) march = std::max(1, std::atoi(e));
+        if (const char *e = std::getenv("MERCURY_OVERLAP")) overlap = std::atoi(e) != 0;
+        HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
         set_march(march);
         cls.build(*F);
         build_programs();
@@ -735,6 +745,9 @@ public:
         if (fptr) (void)hipFree(fptr);
         if (d_red) (void)hipFree(d_red);
         F.reset();
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
 
@@ -1188,11 +1201,23 @@ public:
 
     // StepOnce without the time step, fused passes
     void stage_fused(bool final) {
-        launch_rhs_cell(st, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
+        // Scheme_U_ + sources (writes RHS_H/RHS_Na) and the induction EMF chain
+        // (writes face EMFs, E, E_hall) only read what the previous stage left:
+        // with overlap the RHS pass runs on st2, forked from and joined to st by
+        // events (also under graph capture)
+        if (overlap) {
+            HIPCHK(hipEventRecord(ev_fork, st));
+            HIPCHK(hipStreamWaitEvent(st2, ev_fork, 0));
+            launch_rhs_cell(st2, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
+            HIPCHK(hipEventRecord(ev_join, st2));
+        } else {
+            launch_rhs_cell(st, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
+        }
         launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix());
         if (alias.n == 0) {
             launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, 1, mix());
             run_merged(MG_EDGE_PAIR);
+            if (overlap) HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
             launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix());
             run_merged(MG_STATE_PAIR);
             if (res_now) res_solve(res_dt);
@@ -1207,6 +1232,7 @@ public:
             launch_add_hall_wall(st, R_node.h, Bd(), Fd());
         }
         sync(G_EEDGE);
+        if (overlap) HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
         launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix());
         run_merged(MG_STATE_PAIR);
         if (res_now) res_solve(res_dt);
