@@ -178,6 +178,32 @@ int mercury_graph_launch(MercuryContext *ctx, int nsteps);
  * back; mercury_ensemble_launch replays it nsteps times and waits.  Contexts
  * keep their own physics, dt and state and must outlive the ensemble; contexts
  * with the implicit resistive correction are refused. */
+/* Restart of device-resident fields in the reference's MPCNSRST version-1 file
+ * (IOModule::WriteRestartBinFile, src/6_io/IOModule_Bin.cpp:168-228): the
+ * reference's whitelist U_H, U_Na, B_xi, B_eta, B_zeta (Z4_Mercury/src/MercurySolver.cpp:31-44),
+ * plus J_xi, J_eta, J_zeta when with_jedge is nonzero (its OUTPUT_DEC_JEDGE
+ * option); every block active, values over the allocated box of the location in
+ * (i, j, k, component) order. */
+int mercury_write_restart(MercuryContext *ctx, const char *path, int step, double time, int with_jedge);
+
+/* IOModule::ReadRestartBinFile (src/6_io/IOModule_Bin.cpp:230-420): magic,
+ * version and block count are checked; every field the library knows by name is
+ * validated (location, ncomp, nghost, per-block lo/hi) and uploaded, others are
+ * stepped over; a partial J edge triplet or an inactive block of a known field is
+ * an error.  step and time return the file's values.  Call mercury_prepare
+ * afterwards (the reference syncs Badd and Bface and rebuilds the derived fields
+ * after a restart, Z4_Mercury/src/MercurySolver.cpp:135-145). */
+int mercury_read_restart(MercuryContext *ctx, const char *path, int *step, double *time);
+
+/* RunData::WriteBinary / ReadBinary (include/6_io/RunData.h:98-205), the
+ * MPCNSRUN version-1 file.  Monitors are written empty; on reading they are
+ * parsed and dropped.  mercury_read_rundata stores at most max_ref residual
+ * references (an error if the file holds more) and returns their count in nref. */
+int mercury_write_rundata(const char *path, int step, double time, double dt, int nref, const double *residual_ref,
+                          double residual_max);
+int mercury_read_rundata(const char *path, int *step, double *time, double *dt, int max_ref, int *nref, double *residual_ref,
+                         double *residual_max);
+
 typedef struct MercuryEnsemble MercuryEnsemble;
 int mercury_ensemble_create(MercuryContext *const *ctxs, int n, int integrator, MercuryEnsemble **out);
 int mercury_ensemble_launch(MercuryEnsemble *ens, int nsteps);

@@ -27,6 +27,7 @@
 #include "mercury_geometry.h"
 #include "mercury_kernels.h"
 #include "mercury_fused.h"
+#include "mercury_io.h"
 #include "mercury_resist.h"
 
 static_assert(sizeof(MercuryPhysics) == sizeof(MercPhys), "MercuryPhysics / MercPhys layout mismatch");
@@ -1395,11 +1396,16 @@ public:
     }
 
     // MercurySolver ctor tail + Advance head (MercurySolver.cpp:141-145, MercurySolver_Driver.cpp:6-19)
+    // the derived fields (and the dt candidate) of the current form are a
+    // function of U and B only: a fused form recomputes the whole derived chain
+    // with its own passes, so a state read from a restart continues exactly as
+    // the run that wrote it
     void prepare() {
         run_merged(MG_PREPARE_PAIR);
         derived();
         refresh_float_statics();
-        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix(), hall64());
+        if (form >= 1) derived_fused(true);
+        else launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, false, 0);
         HIPCHK(hipStreamSynchronize(st));
     }
 
@@ -1408,7 +1414,7 @@ public:
         form = f;
         if (f >= 1) {
             refresh_float_statics();
-            launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix(), hall64());
+            derived_fused(true);
         }
         if (gexec) {
             (void)hipGraphExecDestroy(gexec);
@@ -1522,9 +1528,9 @@ struct MercuryContext {
     std::unique_ptr<merc::MercurySolver> s;
 };
 
-#define ABI_TRY(body)                                  \
+#define ABI_TRY(...)                                   \
     try {                                              \
-        body;                                          \
+        __VA_ARGS__;                                   \
         return 0;                                      \
     } catch (const std::exception &e) {                \
         merc::g_last_error = e.what();                 \
@@ -1611,6 +1617,160 @@ int mercury_download(MercuryContext *ctx, const char *name, int block, int comp,
         const int slot = field_comp_slot(name, comp);
         ctx->s->materialize(slot);
         ctx->s->F->download_block(slot, block, host);
+    })
+}
+
+// ---------------------------------------------------------------------------
+// restart files (MPCNSRST v1) and run data (MPCNSRUN v1) on device-resident fields
+// ---------------------------------------------------------------------------
+extern "C++" {
+namespace {
+// the reference's restart whitelist (Z4_Mercury/src/MercurySolver.cpp:31-44)
+const char *const kRestartFields[] = {"U_H", "U_Na", "B_xi", "B_eta", "B_zeta"};
+const char *const kJedgeFields[] = {"J_xi", "J_eta", "J_zeta"};
+
+// allocated box of a location (FieldBlock get_lo / get_hi): [-g, m + 1 - delta + g)
+std::array<int, 6> alloc_box(const MercBlk &k, int loc) {
+    std::array<int, 6> b;
+    for (int a = 0; a < 3; ++a) {
+        b[a] = -k.g;
+        b[3 + a] = k.m[a] + 1 - merc::DELTA[loc][a] + k.g;
+    }
+    return b;
+}
+}  // namespace
+}
+
+int mercury_write_restart(MercuryContext *ctx, const char *path, int step, double time, int with_jedge) {
+    CHECK_CTX(ctx);
+    if (!path) {
+        merc::g_last_error = "mercury_write_restart: path is null";
+        return -1;
+    }
+    ABI_TRY({
+        merc::MercurySolver &S = *ctx->s;
+        HIPCHK(hipStreamSynchronize(S.st));
+        const merc::Field &F = *S.F;
+        std::vector<std::string> names(std::begin(kRestartFields), std::end(kRestartFields));
+        if (with_jedge) names.insert(names.end(), std::begin(kJedgeFields), std::end(kJedgeFields));
+        std::vector<merc::io::RestartField> out;
+        for (const std::string &nm : names) {
+            const merc::FieldSpec *f = merc::find_field(nm.c_str());
+            merc::io::RestartField rf;
+            rf.meta.name = nm;
+            rf.meta.loc = f->loc;
+            rf.meta.ncomp = f->ncomp;
+            rf.meta.nghost = F.blk[0].g;
+            std::vector<std::vector<double>> comp(f->ncomp);
+            for (int b = 0; b < F.nb(); ++b) {
+                const MercBlk &k = F.blk[b];
+                const auto box = alloc_box(k, f->loc);
+                for (int c = 0; c < f->ncomp; ++c) {
+                    S.materialize(f->slot + c);
+                    comp[c].resize(F.vol(b));
+                    F.download_block(f->slot + c, b, comp[c].data());
+                }
+                std::vector<double> v;
+                v.reserve((size_t)(box[3] - box[0]) * (box[4] - box[1]) * (box[5] - box[2]) * f->ncomp);
+                for (int i = box[0]; i < box[3]; ++i)
+                    for (int j = box[1]; j < box[4]; ++j)
+                        for (int kk = box[2]; kk < box[5]; ++kk) {
+                            const long long q = (long long)(i + k.g) * k.sI + (long long)(j + k.g) * k.sJ + (kk + k.g);
+                            for (int c = 0; c < f->ncomp; ++c) v.push_back(comp[c][q]);
+                        }
+                rf.box.push_back(box);
+                rf.data.push_back(std::move(v));
+            }
+            out.push_back(std::move(rf));
+        }
+        merc::io::write_restart(path, step, time, F.nb(), out);
+    })
+}
+
+int mercury_read_restart(MercuryContext *ctx, const char *path, int *step, double *time) {
+    CHECK_CTX(ctx);
+    if (!path) {
+        merc::g_last_error = "mercury_read_restart: path is null";
+        return -1;
+    }
+    ABI_TRY({
+        merc::MercurySolver &S = *ctx->s;
+        HIPCHK(hipStreamSynchronize(S.st));
+        merc::Field &F = *S.F;
+        std::map<std::string, std::vector<char>> delivered;
+        int njedge = 0;
+        auto want = [&](const merc::io::FieldMeta &m) {
+            const merc::FieldSpec *f = merc::find_field(m.name.c_str());
+            if (!f) return false;  // fields this build does not hold are stepped over
+            if (m.loc != f->loc) throw std::runtime_error("restart: location mismatch for " + m.name);
+            if (m.ncomp != f->ncomp) throw std::runtime_error("restart: ncomp mismatch for " + m.name);
+            if (m.nghost != F.blk[0].g) throw std::runtime_error("restart: nghost mismatch for " + m.name);
+            if (delivered.count(m.name)) throw std::runtime_error("restart: duplicate field " + m.name);
+            for (const char *j : kJedgeFields)
+                if (m.name == j) ++njedge;
+            delivered[m.name].assign(F.nb(), 0);
+            return true;
+        };
+        std::vector<double> comp;
+        auto put = [&](const merc::io::FieldMeta &m, int b, const std::array<int, 6> &box, const double *vals) {
+            const merc::FieldSpec *f = merc::find_field(m.name.c_str());
+            const MercBlk &k = F.blk[b];
+            if (box != alloc_box(k, f->loc)) throw std::runtime_error("restart: lo/hi mismatch for " + m.name + " block " + std::to_string(b));
+            comp.resize(F.vol(b));
+            for (int c = 0; c < f->ncomp; ++c) {
+                F.download_block(f->slot + c, b, comp.data());  // keeps the box points outside the allocation
+                size_t t = (size_t)c;
+                for (int i = box[0]; i < box[3]; ++i)
+                    for (int j = box[1]; j < box[4]; ++j)
+                        for (int kk = box[2]; kk < box[5]; ++kk, t += f->ncomp)
+                            comp[(long long)(i + k.g) * k.sI + (long long)(j + k.g) * k.sJ + (kk + k.g)] = vals[t];
+                F.upload_block(f->slot + c, b, comp.data());
+            }
+            delivered[m.name][b] = 1;
+        };
+        merc::io::read_restart(path, F.nb(), want, put, step, time);
+        if (njedge != 0 && njedge != 3) throw std::runtime_error("restart: incomplete J edge triplet (J_xi, J_eta, J_zeta go together)");
+        for (const auto &kv : delivered)
+            for (int b = 0; b < F.nb(); ++b)
+                if (!kv.second[b])
+                    throw std::runtime_error("restart: block " + std::to_string(b) + " of " + kv.first + " is inactive in the file");
+    })
+}
+
+int mercury_write_rundata(const char *path, int step, double time, double dt, int nref, const double *residual_ref,
+                          double residual_max) {
+    if (!path || nref < 0 || (nref > 0 && !residual_ref)) {
+        merc::g_last_error = "mercury_write_rundata: invalid argument";
+        return -1;
+    }
+    ABI_TRY({
+        merc::io::RunData r;
+        r.step = step;
+        r.time = time;
+        r.dt = dt;
+        r.residual_ref.assign(residual_ref, residual_ref + nref);
+        r.residual_max = residual_max;
+        merc::io::write_rundata(path, r);
+    })
+}
+
+int mercury_read_rundata(const char *path, int *step, double *time, double *dt, int max_ref, int *nref, double *residual_ref,
+                         double *residual_max) {
+    if (!path) {
+        merc::g_last_error = "mercury_read_rundata: path is null";
+        return -1;
+    }
+    ABI_TRY({
+        const merc::io::RunData r = merc::io::read_rundata(path);
+        if (step) *step = r.step;
+        if (time) *time = r.time;
+        if (dt) *dt = r.dt;
+        if (nref) *nref = (int)r.residual_ref.size();
+        if ((int)r.residual_ref.size() > max_ref && residual_ref)
+            throw std::runtime_error("mercury_read_rundata: residual_ref holds " + std::to_string(r.residual_ref.size()) + " values");
+        if (residual_ref)
+            for (size_t i = 0; i < r.residual_ref.size(); ++i) residual_ref[i] = r.residual_ref[i];
+        if (residual_max) *residual_max = r.residual_max;
     })
 }
 

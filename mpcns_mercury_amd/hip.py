@@ -23,7 +23,8 @@ ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercu
                "mercury_init_bface", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
                "mercury_divb", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
                "mercury_time_op", "mercury_geometry_block", "mercury_set_form", "mercury_resist_stats",
-               "mercury_ensemble_create", "mercury_ensemble_launch", "mercury_ensemble_destroy", "mercury_autotune")
+               "mercury_ensemble_create", "mercury_ensemble_launch", "mercury_ensemble_destroy", "mercury_autotune",
+               "mercury_write_restart", "mercury_read_restart", "mercury_write_rundata", "mercury_read_rundata")
 
 
 class MercuryError(RuntimeError):
@@ -44,6 +45,10 @@ def load():
     L.mercury_create_connected.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ip, ctypes.c_int, ip, ip, ip, dp,
                                            ctypes.POINTER(MercuryPhysics)]
     L.mercury_destroy.argtypes = [vp]
+    L.mercury_write_restart.argtypes = [vp, cp, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    L.mercury_read_restart.argtypes = [vp, cp, ip, dp]
+    L.mercury_write_rundata.argtypes = [cp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, ctypes.c_double]
+    L.mercury_read_rundata.argtypes = [cp, ip, dp, dp, ctypes.c_int, ip, dp, dp]
     L.mercury_field_info.argtypes = [cp, ip, ip, ip]
     L.mercury_block_volume.restype = ctypes.c_longlong
     L.mercury_block_volume.argtypes = [vp, ctypes.c_int]
@@ -79,6 +84,22 @@ def _chk(rc):
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def write_rundata(path, step, time, dt, residual_ref=(), residual_max=0.0):
+    """MPCNSRUN v1 run-data file (monitors empty)."""
+    ref = np.ascontiguousarray(residual_ref, dtype=np.float64)
+    _chk(load().mercury_write_rundata(str(path).encode(), int(step), float(time), float(dt), len(ref), _dp(ref),
+                                      float(residual_max)))
+
+
+def read_rundata(path, max_ref=64):
+    """(step, time, dt, residual_ref, residual_max) of an MPCNSRUN v1 file."""
+    st, tm, dt, n, rmax = ctypes.c_int(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+    ref = np.zeros(max_ref)
+    _chk(load().mercury_read_rundata(str(path).encode(), ctypes.byref(st), ctypes.byref(tm), ctypes.byref(dt), max_ref,
+                                     ctypes.byref(n), _dp(ref), ctypes.byref(rmax)))
+    return st.value, tm.value, dt.value, ref[:n.value].copy(), rmax.value
 
 
 def field_info(name):
@@ -199,6 +220,17 @@ class Mercury:
 
     def graph_launch(self, n=1):
         _chk(load().mercury_graph_launch(self._h, n))
+
+    def write_restart(self, path, step=0, time=0.0, with_jedge=False):
+        """MPCNSRST v1 restart of U_H, U_Na, B_xi/eta/zeta (and the J edges)."""
+        _chk(load().mercury_write_restart(self._h, str(path).encode(), int(step), float(time), 1 if with_jedge else 0))
+
+    def read_restart(self, path):
+        """Load a restart file into the device fields; returns (step, time).
+        Call prepare() afterwards."""
+        st, tm = ctypes.c_int(), ctypes.c_double()
+        _chk(load().mercury_read_restart(self._h, str(path).encode(), ctypes.byref(st), ctypes.byref(tm)))
+        return st.value, tm.value
 
     def resist_stats(self):
         """(unknown edges, iterations of the last solve, its relative residual)."""
