@@ -605,7 +605,7 @@ public:
     // induction EMF chain runs on st (they read the same state and write
     // disjoint fields); both streams keep the default priority
     hipStream_t st2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
     bool overlap = true;
     unsigned long long *d_dt = nullptr;       // dt word (double bits)
     unsigned long long *d_dt_next = nullptr;  // next step's dt, reduced at the end of the derived fields
@@ -731,6 +731,8 @@ public:
         HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_fork2, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_join2, hipEventDisableTiming));
         set_march(march);
         cls.build(*F);
         build_programs();
@@ -748,6 +750,8 @@ public:
         F.reset();
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_fork2) (void)hipEventDestroy(ev_fork2);
+        if (ev_join2) (void)hipEventDestroy(ev_join2);
         if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -990,16 +994,17 @@ public:
             }
     }
 
-    void run_merged(unsigned gset) {
+    void run_merged(unsigned gset) { run_merged(gset, st); }
+    void run_merged(unsigned gset, hipStream_t s) {
         auto it = mprog.find(gset);
         if (it == mprog.end()) throw std::runtime_error("merged sync program missing");
         const int g = F->blk[0].g;
         for (auto &m : it->second.ops) {
             switch (m->type) {
-            case OP_COPY: launch_copy_multi(st, m->ent.p, m->n, Fd(), m->S); break;
-            case OP_FILL: launch_fill_multi(st, m->ent.p, m->n, Fd(), m->S); break;
-            case OP_SETV: launch_setv_multi(st, m->ent.p, m->val.p, m->n, Fd()); break;
-            case OP_WALLF: launch_wall_multi(st, m->ent.p, m->n, g, Fd(), P); break;
+            case OP_COPY: launch_copy_multi(s, m->ent.p, m->n, Fd(), m->S); break;
+            case OP_FILL: launch_fill_multi(s, m->ent.p, m->n, Fd(), m->S); break;
+            case OP_SETV: launch_setv_multi(s, m->ent.p, m->val.p, m->n, Fd()); break;
+            case OP_WALLF: launch_wall_multi(s, m->ent.p, m->n, g, Fd(), P); break;
             default: break;
             }
         }
@@ -1200,6 +1205,39 @@ public:
         launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, final ? 1 : 0, mix(), hall64());
     }
 
+    // After an update: Sync(U), Sync(Bface), the implicit resistive correction
+    // when it is due, and the derived passes.  With overlap (and no correction)
+    // the U sync runs on st2 beside the B sync and the B-cell / J-edge pass, and,
+    // when no DOF is shared between blocks or across periodic seams (so no
+    // owner -> alias copy writes an owned J edge), the B_cell / J-edge sync also
+    // runs on st2 beside the J_cell pass; the last derived pass joins both.
+    void after_update(bool final) {
+        if (!overlap || res_now) {
+            run_merged(MG_STATE_PAIR);
+            if (res_now) res_solve(res_dt);
+            derived_fused(final);
+            return;
+        }
+        HIPCHK(hipEventRecord(ev_fork2, st));
+        HIPCHK(hipStreamWaitEvent(st2, ev_fork2, 0));
+        run_merged(1u << G_UCELL, st2);
+        run_merged(1u << G_BFACE, st);
+        launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix());
+        launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
+        if (alias.n == 0 && !F->connected()) {
+            HIPCHK(hipEventRecord(ev_fork2, st));
+            HIPCHK(hipStreamWaitEvent(st2, ev_fork2, 0));
+            run_merged(MG_DERIVED_PAIR, st2);
+        } else {
+            run_merged(MG_DERIVED_PAIR, st);
+        }
+        launch_jcell_t(st, R_cell.h, Bd(), Fd(), Gd(), mix());
+        sync(G_JCELL);
+        HIPCHK(hipEventRecord(ev_join2, st2));
+        HIPCHK(hipStreamWaitEvent(st, ev_join2, 0));
+        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, final ? 1 : 0, mix(), hall64());
+    }
+
     // StepOnce without the time step, fused passes
     void stage_fused(bool final) {
         // Scheme_U_ + sources (writes RHS_H/RHS_Na) and the induction EMF chain
@@ -1220,9 +1258,7 @@ public:
             run_merged(MG_EDGE_PAIR);
             if (overlap) HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
             launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix());
-            run_merged(MG_STATE_PAIR);
-            if (res_now) res_solve(res_dt);
-            derived_fused(final);
+            after_update(final);
             return;
         } else {
             // periodic seams: alias mean of the non-Hall EMF before the Hall part
@@ -1235,9 +1271,7 @@ public:
         sync(G_EEDGE);
         if (overlap) HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
         launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix());
-        run_merged(MG_STATE_PAIR);
-        if (res_now) res_solve(res_dt);
-        derived_fused(final);
+        after_update(final);
     }
 
     // UpdateDerivedFields_ (MercurySolver_Tools.cpp:303-321)
@@ -1300,9 +1334,8 @@ public:
                 stage_fused(false);
                 stage_fused(false);
                 launch_rk2(st, R_node.h, Bd(), Fd(), 0);
-                run_merged(MG_STATE_PAIR);
-                if (resist) res_solve(res_dt);
-                derived_fused(true);
+                res_now = resist;
+                after_update(true);
             }
             res_now = false;
             return;
