@@ -40,7 +40,10 @@ enum MercSlot {
     // implicit resistive correction: Eres, dJ and eta on the unknown edges;
     // dB and RHS_Bres faces
     SL_ERES = SL_UCON + 3, SL_DJ = SL_ERES + 3, SL_ETAR = SL_DJ + 3, SL_DB = SL_ETAR + 3, SL_RHSBR = SL_DB + 3,
-    SL_NSLOT = SL_RHSBR + 3
+    // node: electron pressure (mean of p_H + p_Na over the 8 cells around the
+    // node) of the ambipolar EMF (fused path)
+    SL_PEN = SL_RHSBR + 3,
+    SL_NSLOT = SL_PEN + 1
 };
 
 // physics constants (same layout as MercuryPhysics in include/mercury_hip.h)

@@ -9,7 +9,8 @@ void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *
                      int L, bool mix);
 void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                      unsigned long long *dt_next, int compute_dt, bool mix, int hall64 = 0);
-void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);
+// ambi: also the node electron pressure of the ambipolar EMF (read by k_edge_total)
+void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix, int ambi);
 void launch_edge_total(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                        int with_hall, bool mix);
 void launch_update_all(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const double *dt,

@@ -393,13 +393,29 @@ DEVI typename Prec<MIX>::T emf_flux_u(double *const *__restrict__ F, const float
     return T(0.5) * (GL + GR) - T(0.5) * radius * (biR - biL);
 }
 
+// face EMF candidates (AssembleOneDirectionEMF_ x3) and, with the ambipolar
+// term, the node electron pressure its edges need: the mean of p_H + p_Na over
+// the 8 cells around the node, summed in the order of AddAmbipolarEdgeEMF_
+// (MercurySolver_MagExplicit_Ambipo.cpp:3-143), once per node instead of once
+// per edge end
 template <bool MIX>
 __global__ void k_eface_u(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
-                          const float *const *__restrict__ G) {
+                          const float *const *__restrict__ G, int ambi) {
     using T = typename Prec<MIX>::T;
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const int idx[3] = {p.i, p.j, p.k};
+    if (ambi && p.i >= 0 && p.j >= 0 && p.k >= 0 && p.i <= bk.m[0] && p.j <= bk.m[1] && p.k <= bk.m[2]) {
+        const long long di = OI(p), dj = OJ(p);
+        T sum = T(0);
+        for (int ddi = -1; ddi <= 0; ++ddi)
+            for (int ddj = -1; ddj <= 0; ++ddj)
+                for (int ddk = -1; ddk <= 0; ++ddk) {
+                    const long long c = p.q + ddi * di + ddj * dj + ddk;
+                    sum += (T)F[SL_PVH + 3][c] + (T)F[SL_PVN + 3][c];
+                }
+        wr<MIX>(F, G, SL_PEN, p.q, T(0.125) * sum);
+    }
     for (int fa = 0; fa < 3; ++fa) {
         bool ok = true;
         for (int a = 0; a < 3; ++a) {
@@ -491,18 +507,7 @@ __global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, d
         }
         if (ambi) {
             const long long ea = AX(p, a);
-            T pe[2];
-            for (int e = 0; e < 2; ++e) {
-                const long long n0 = q + (e ? ea : 0);
-                T sum = T(0);
-                for (int ddi = -1; ddi <= 0; ++ddi)
-                    for (int ddj = -1; ddj <= 0; ++ddj)
-                        for (int ddk = -1; ddk <= 0; ++ddk) {
-                            const long long c = n0 + ddi * di + ddj * dj + ddk;
-                            sum += (T)F[SL_PVH + 3][c] + (T)F[SL_PVN + 3][c];
-                        }
-                pe[e] = T(0.125) * sum;
-            }
+            const T pe[2] = {st<MIX>(F, G, SL_PEN, q), st<MIX>(F, G, SL_PEN, q + ea)};  // node electron pressure (k_eface_u)
             double rH = 0.0, rN = 0.0;
             const long long t1 = (a == 0) ? dj : di, t2 = (a == 2) ? dj : 1;
             for (int d1 = -1; d1 <= 0; ++d1)
@@ -738,9 +743,9 @@ void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *
     if (mix) LAUNCH_R(k_cell_end<true>, R, B, F, G, P, dt_next, compute_dt, hall64);
     else LAUNCH_R(k_cell_end<false>, R, B, F, G, P, dt_next, compute_dt, hall64);
 }
-void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix) {
-    if (mix) LAUNCH_R(k_eface_u<true>, R, B, F, G);
-    else LAUNCH_R(k_eface_u<false>, R, B, F, G);
+void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix, int ambi) {
+    if (mix) LAUNCH_R(k_eface_u<true>, R, B, F, G, ambi);
+    else LAUNCH_R(k_eface_u<false>, R, B, F, G, ambi);
 }
 void launch_edge_total(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                        int with_hall, bool mix) {

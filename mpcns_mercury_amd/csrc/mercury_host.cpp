@@ -713,6 +713,7 @@ public:
         for (int c = 0; c < 9; ++c) islots.push_back(SL_EF + c);
         for (int c = 0; c < 5; ++c) islots.push_back(SL_RH + c);
         for (int c = 0; c < 5; ++c) islots.push_back(SL_RN + c);
+        islots.push_back(SL_PEN);
         const size_t nf = fslots.size() + islots.size();
         HIPCHK(hipMalloc(&farena, sizeof(float) * F->Vtot * nf));
         HIPCHK(hipMemset(farena, 0, sizeof(float) * F->Vtot * nf));
@@ -1252,7 +1253,7 @@ public:
         } else {
             launch_rhs_cell(st, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
         }
-        launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix());
+        launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix(), P.ambipolar_enabled != 0.0);
         if (alias.n == 0) {
             launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, 1, mix());
             run_merged(MG_EDGE_PAIR);
@@ -1480,7 +1481,7 @@ public:
         else if (op == "stage") stage();
         else if (op == "rhs_cell") launch_rhs_cell(st, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
         else if (op == "cell_end") launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix(), hall64());
-        else if (op == "eface_u") launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix());
+        else if (op == "eface_u") launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix(), P.ambipolar_enabled != 0.0);
         else if (op == "edge_total") launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, alias.n == 0 ? 1 : 0, mix());
         else if (op == "update_all") launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix());
         else if (op == "bcell_jedge") launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix());
