@@ -41,11 +41,11 @@ ALGO_BYTES = {
     # flux + sources: U_s (10) + PV_s (10) + JD (9) + Jac + U+ (3) + B_cell (3) + J_cell (3) + Photo + centre (3) read,
     # RHS_s (10) written
     "rhs_cell": 8 * (10 + 10 + 9 + 1 + 3 + 3 + 3 + 1 + 3 + 10),
-    # B (3) + Badd (3) + U_con (3) read, Eface (9) written
-    "eface_u": 8 * (3 + 3 + 3 + 9),
-    # Eface (9) + U_con (3) + B (3) + PV p (2) + U rho (2) + J (3) + eta_local (3) + E_hc (3) + beta (1) + dr (9)
-    # + Area (3) + dl (3) + taper (3) read, E (3) + Ehall (3) written
-    "edge_total": 8 * (9 + 3 + 3 + 2 + 2 + 3 + 3 + 3 + 1 + 9 + 3 + 3 + 3 + 6),
+    # B (3) + Badd (3) + U_con (3) + PV p (2) read, Eface (9) + node electron pressure (1) written
+    "eface_u": 8 * (3 + 3 + 3 + 2 + 9 + 1),
+    # Eface (9) + U_con (3) + B (3) + node electron pressure (1) + U rho (2) + J (3) + eta_local (3) + E_hc (3)
+    # + beta (1) + dr (9) + Area (3) + dl (3) + taper (3) read, E (3) + Ehall (3) written
+    "edge_total": 8 * (9 + 3 + 3 + 1 + 2 + 3 + 3 + 3 + 1 + 9 + 3 + 3 + 3 + 6),
     # U_s (10) + RHS_s (10) + E (3) + B (3) read, U_s (10) + B (3) + RHS_B (3) written
     "update_all": 8 * (10 + 10 + 3 + 3 + 10 + 3 + 3),
     # B (3) + Badd (3) + W_b (18) + beta (3) + alpha (3) read, B_cell + Bind (6) + J (3) written

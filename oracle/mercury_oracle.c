@@ -2389,6 +2389,8 @@ int orc_op(void *h, const char *op, double arg) {
     else if (!strcmp(op, "sources")) add_sources(o);
     else if (!strcmp(op, "ideal_emf")) ideal_emf(o);
     else if (!strcmp(op, "hall_emf")) hall_emf(o);
+    else if (!strcmp(op, "ambipolar_emf")) ambipolar_emf(o);
+    else if (!strcmp(op, "arti_emf")) arti_emf(o);
     else if (!strcmp(op, "induction")) induction_ct(o);
     else if (!strcmp(op, "alias_e")) alias_reduce_edges(o, S_E);
     else if (!strcmp(op, "alias_j")) alias_reduce_edges(o, S_J);

@@ -1,0 +1,312 @@
+"""Pins of the oracle's non-ideal edge EMFs (CPU only): ambipolar, artificial
+and local artificial resistivity, Hall (centred term, hyper-diffusion, radial
+taper).
+
+The reference holds no test for these terms.  Each test builds a Cartesian block
+with distinct spacings per axis and a manufactured field for which the edge EMF
+has a closed form, so that a wrong stencil, a transposed axis, a dropped term or
+a wrong sign changes the result:
+
+* ambipolar (Z4_Mercury/src/physics/MercurySolver_MagExplicit_Ambipo.cpp:3-143):
+  E_a -= coef * (pe_node(end) - pe_node(start)) / ne_eff, pe_node the mean of the
+  8 cells around a node, ne_eff from the mean density of the 4 cells around the
+  edge.  A quadratic pe with cross terms separates every possible stencil shift;
+  the fluid source -chi_s grad(pe) is the exact gradient of such a pe on an
+  affine grid;
+* artificial resistivity (MercurySolver_MagExplicit_Resis.cpp:6-63): the
+  smoothstep q^2 (3 - 2q) at its fixed points q = 0, 1/2, 1, the thresholds and
+  the |dr| normalisation;
+* local artificial resistivity (MercurySolver_MagExplicit_Resis.cpp:64-185):
+  eta_max at the edge whose midpoint is the centre, exp(-d / r_decay) along the
+  axes, zero from the cutoff on;
+* Hall (MercurySolver_MagExplicit_Hall.cpp:6-433): for a linear face field
+  B(x) = B0 + G x on a Cartesian block the edge EMF is
+  alpha (J x B_cell) . dr + 0.5 C_wh be h_a (curl B)_a, with alpha = hall_coef / ne_eff,
+  be = |alpha| |B_cell|, C_wh = 0.1 and h_a the edge length;
+  the radial taper is 1/2 at the mid radius and odd about it.
+"""
+import numpy as np
+import pytest
+
+from mpcns_mercury_amd import cases
+from mpcns_mercury_amd.params import MercuryUnits
+
+N = (6, 5, 4)
+H = (0.1, 0.2, 0.15)
+G = 4
+EDGES = ("E_xi", "E_eta", "E_zeta")
+
+
+@pytest.fixture(scope="module")
+def O(oracle_mod):
+    return oracle_mod
+
+
+def _block(units=None, **kw):
+    return cases.make_uniform_block(cases.cartesian_nodes(N, H, origin=(1.0, 0.0, 0.0)), (0,) * 6, g=G, units=units, **kw)
+
+
+def _ne_eff(P, rhoH, rhoN):
+    nH = np.maximum(rhoH, 0.0) / P.M_H * P.M_ref
+    nN = np.maximum(rhoN, 0.0) / P.M_Na * P.M_ref
+    return np.sqrt((nH + nN) ** 2 + P.ne_floor ** 2)
+
+
+def _edge_box(a):
+    """Box indices (I, J, K) of the owned edges of axis a (arrays of the owned shape)."""
+    hi = [G + m + 1 - (1 if b == a else 0) for b, m in enumerate(N)]
+    return np.meshgrid(*[np.arange(G, h) for h in hi], indexing="ij")
+
+
+def _owned_edges(arr, a):
+    hi = [G + m + 1 - (1 if b == a else 0) for b, m in enumerate(N)]
+    return arr[G:hi[0], G:hi[1], G:hi[2]]
+
+
+def _edge_length(o, a):
+    dr = o.get(("dr_xi", "dr_eta", "dr_zeta")[a])
+    return np.sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2])
+
+
+# --------------------------------------------------------------------------
+# ambipolar
+# --------------------------------------------------------------------------
+def test_ambipolar_quadratic_pressure(O):
+    c = _block()
+    P = c.phys
+    P.ambipolar_enabled = 1.0
+    assert P.pressure_coef > 0.0
+    o = O.Oracle(c)
+    shape = c.box(0)
+    I, J, K = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij")
+    # integer coefficients: every node mean and difference below is exact
+    a, b, cc, e, f, h = 3.0, -2.0, 5.0, 7.0, -1.0, 4.0
+    pe = a * I * I + b * J * J + cc * K * K + e * I * J + f * J * K + h * I * K
+    pN = 2.0 * I + 1.0  # split between the species: only the sum enters
+    PVH, PVN = o.get("PV_H"), o.get("PV_Na")
+    PVH[3], PVN[3] = pe - pN, pN
+    o.set("PV_H", 0, PVH)
+    o.set("PV_Na", 0, PVN)
+    rhoH = 1.0 + 0.125 * I + 0.25 * J + 0.0625 * K
+    UH, UN = o.get("U_H"), o.get("U_Na")
+    UH[0] = rhoH
+    UN[0] = 0.5
+    o.set("U_H", 0, UH)
+    o.set("U_Na", 0, UN)
+    for nm in EDGES:
+        o.set(nm, 0, np.zeros(shape))
+    o.op("ambipolar_emf")
+    for ax, nm in enumerate(EDGES):
+        Ie, Je, Ke = _edge_box(ax)
+        # node mean over cells (n-1, n): pe_node(n) differences along the edge axis
+        if ax == 0:
+            dpe = 2 * a * Ie + e * (Je - 0.5) + h * (Ke - 0.5)
+            rho = 1.0 + 0.125 * Ie + 0.25 * (Je - 0.5) + 0.0625 * (Ke - 0.5)
+        elif ax == 1:
+            dpe = 2 * b * Je + e * (Ie - 0.5) + f * (Ke - 0.5)
+            rho = 1.0 + 0.125 * (Ie - 0.5) + 0.25 * Je + 0.0625 * (Ke - 0.5)
+        else:
+            dpe = 2 * cc * Ke + f * (Je - 0.5) + h * (Ie - 0.5)
+            rho = 1.0 + 0.125 * (Ie - 0.5) + 0.25 * (Je - 0.5) + 0.0625 * Ke
+        want = -P.pressure_coef * dpe / _ne_eff(P, rho, 0.5)
+        got = _owned_edges(o.get(nm)[0], ax)
+        np.testing.assert_allclose(got, want, rtol=1e-13, err_msg=nm)
+
+
+def test_ambipolar_disabled_and_uniform(O):
+    c = _block()
+    P = c.phys
+    o = O.Oracle(c)
+    shape = c.box(0)
+    for nm in EDGES:
+        o.set(nm, 0, np.full(shape, 0.25))
+    o.op("ambipolar_emf")  # disabled: nothing changes
+    for nm in EDGES:
+        assert np.all(o.get(nm)[0] == 0.25)
+    P.ambipolar_enabled = 1.0
+    o = O.Oracle(c)  # uniform pressure: no gradient, no EMF
+    for nm in EDGES:
+        o.set(nm, 0, np.full(shape, 0.25))
+    o.op("ambipolar_emf")
+    for ax, nm in enumerate(EDGES):
+        assert np.all(_owned_edges(o.get(nm)[0], ax) == 0.25)
+
+
+def test_ambipolar_source_is_the_pressure_gradient(O):
+    """AddSourceToRHS_Fluid's ambipolar term (MercurySolver_FluidRHS.cpp:206-527):
+    momentum gains -chi_s grad(pe), with grad(pe) = sum_a 0.5 (pe(+a) - pe(-a)) JD_a / V.
+    On an affine grid x = o + A (i, j, k) and a quadratic pe, central differences
+    are exact and JD_a / V are the rows of A^-1, so grad(pe) = A^-T g, g the
+    index-space gradient; a sheared A separates every transposition."""
+    A = np.array([[0.1, 0.02, -0.01], [0.015, 0.2, 0.03], [-0.02, 0.01, 0.15]])
+    i, j, k = np.meshgrid(*[np.arange(n + 1, dtype=np.float64) for n in N], indexing="ij")
+    nodes = np.einsum("ab,bijk->aijk", A, np.stack([i, j, k])) + np.array([1.0, 0.0, 0.0])[:, None, None, None]
+    c = cases.make_uniform_block(nodes, (0,) * 6, g=G)
+    P = c.phys
+    P.ambipolar_enabled = 1.0
+    o = O.Oracle(c)
+    o.prepare()
+    shape = c.box(0)
+    I, J, K = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij")
+    a, b, cc, e, f, h = 3.0, -2.0, 5.0, 7.0, -1.0, 4.0
+    PVH = o.get("PV_H")
+    PVH[3] = 0.25 + 1e-3 * (a * I * I + b * J * J + cc * K * K + e * I * J + f * J * K + h * I * K)
+    o.set("PV_H", 0, PVH)
+    PVN = o.get("PV_Na")
+    pN = PVN[3][G, G, G]
+    o.set("RHS_H", 0, np.zeros((5,) + shape))
+    o.op("sources")
+    R = o.get("RHS_H")[:, G:G + N[0], G:G + N[1], G:G + N[2]]
+    Ic, Jc, Kc = [x[G:G + N[0], G:G + N[1], G:G + N[2]] for x in (I, J, K)]
+    gidx = 1e-3 * np.stack([2 * a * Ic + e * Jc + h * Kc, 2 * b * Jc + e * Ic + f * Kc, 2 * cc * Kc + f * Jc + h * Ic])
+    grad = np.einsum("ab,bijk->aijk", np.linalg.inv(A).T, gidx)
+    assert np.all(PVN[3] == pN)  # the Na pressure is uniform: only the H part varies
+    rhoH, rhoN = o.get("U_H")[0][G, G, G], o.get("U_Na")[0][G, G, G]
+    chi = (rhoH / P.M_H * P.M_ref) / _ne_eff(P, rhoH, rhoN)
+    for q in range(3):
+        np.testing.assert_allclose(R[1 + q], -chi * grad[q], rtol=1e-9, atol=1e-15, err_msg=f"momentum {q}")
+
+
+# --------------------------------------------------------------------------
+# artificial and local artificial resistivity
+# --------------------------------------------------------------------------
+def test_artificial_resistivity_smoothstep(O):
+    c = _block()
+    P = c.phys
+    P.arti_eta_max, P.J_start, P.J_on = 0.3, 1.0, 3.0
+    o = O.Oracle(c)
+    shape = c.box(0)
+    # |J| / |dr| per edge cycles through the thresholds and the smoothstep's fixed points
+    jm = np.array([0.5, 1.0, 2.0, 3.0, 5.0])
+    eta = np.array([0.0, 0.0, 0.15, 0.3, 0.3])
+    rng = np.random.default_rng(5)
+    picks = {}
+    for ax, nm in enumerate(EDGES):
+        sel = rng.integers(0, len(jm), size=shape)
+        sign = np.where(rng.random(shape) < 0.5, -1.0, 1.0)
+        Jv = sign * jm[sel] * _edge_length(o, ax)  # |J| / |dr| is jm exactly
+        o.set("J_" + ("xi", "eta", "zeta")[ax], 0, Jv)
+        o.set(nm, 0, np.full(shape, 0.5))
+        picks[ax] = (sel, Jv)
+    o.op("arti_emf")
+    for ax, nm in enumerate(EDGES):
+        sel, Jv = picks[ax]
+        np.testing.assert_allclose(_owned_edges(_edge_length(o, ax), ax), H[ax], rtol=1e-14)
+        want = 0.5 + eta[sel] * Jv
+        np.testing.assert_allclose(_owned_edges(o.get(nm)[0], ax), _owned_edges(want, ax), rtol=1e-14, atol=1e-16, err_msg=nm)
+
+
+def test_artificial_resistivity_step_when_range_is_empty(O):
+    c = _block()
+    P = c.phys
+    P.arti_eta_max, P.J_start, P.J_on = 0.3, 2.0, 2.0
+    o = O.Oracle(c)
+    dl = _edge_length(o, 0)
+    Jv = 2.0 * dl  # exactly at the threshold: not above it
+    Jv[G + 1, G + 1, G + 1] = 2.5 * H[0]
+    shape = c.box(0)
+    o.set("J_xi", 0, Jv)
+    o.set("E_xi", 0, np.zeros(shape))
+    o.op("arti_emf")
+    E = o.get("E_xi")[0]
+    assert E[G + 1, G + 1, G + 1] == pytest.approx(0.3 * 2.5 * H[0], rel=1e-15)
+    E[G + 1, G + 1, G + 1] = 0.0
+    assert np.all(_owned_edges(E, 0) == 0.0)
+
+
+def test_local_artificial_resistivity_profile(O):
+    c = _block()
+    P = c.phys
+    P.local_enabled, P.local_eta_max, P.local_r_decay, P.local_r_cutoff = 1.0, 0.2, 0.05, 0.25
+    # centre on the midpoint of the owned xi edge (2, 2, 1): x = 1 + 2.5 hx, y = 2 hy, z = hz
+    ic = (2, 2, 1)
+    ctr = (1.0 + (ic[0] + 0.5) * H[0], ic[1] * H[1], ic[2] * H[2])
+    for q in range(3):
+        P.local_center[q] = ctr[q]
+    o = O.Oracle(c)
+    eta = {ax: o.get("eta_local_" + s)[0] for ax, s in enumerate(("xi", "eta", "zeta"))}
+    e0 = eta[0][G + ic[0], G + ic[1], G + ic[2]]
+    assert e0 == 0.2
+    # the next xi edges along x lie hx away, along y hy away
+    assert eta[0][G + ic[0] + 1, G + ic[1], G + ic[2]] == pytest.approx(0.2 * np.exp(-H[0] / 0.05), rel=1e-14)
+    assert eta[0][G + ic[0], G + ic[1] + 1, G + ic[2]] == pytest.approx(0.2 * np.exp(-H[1] / 0.05), rel=1e-14)
+    # everything at the cutoff distance or beyond is zero, everything inside is positive
+    for ax in range(3):
+        Ie, Je, Ke = _edge_box(ax)
+        mid = [1.0 + (Ie - G) * H[0], (Je - G) * H[1], (Ke - G) * H[2]]
+        mid[ax] = mid[ax] + 0.5 * H[ax]
+        r = np.sqrt(sum((mid[q] - ctr[q]) ** 2 for q in range(3)))
+        got = _owned_edges(eta[ax], ax)
+        assert np.all(got[r >= 0.25 + 1e-12] == 0.0)
+        np.testing.assert_allclose(got[r < 0.25 - 1e-12], 0.2 * np.exp(-r[r < 0.25 - 1e-12] / 0.05), rtol=1e-13)
+    # and the EMF gains eta * J
+    shape = c.box(0)
+    o.set("J_xi", 0, np.full(shape, 3.0))
+    o.set("E_xi", 0, np.zeros(shape))
+    o.op("arti_emf")
+    assert o.get("E_xi")[0][G + ic[0], G + ic[1], G + ic[2]] == 0.2 * 3.0
+
+
+# --------------------------------------------------------------------------
+# Hall
+# --------------------------------------------------------------------------
+def _units_no_taper():
+    u = MercuryUnits().alfven_units()
+    u.r_min, u.r_max = 0.0, 0.0  # radial taper = 1 everywhere
+    return u
+
+
+def test_hall_edge_emf_linear_field(O):
+    c = _block(units=_units_no_taper())
+    P = c.phys
+    o = O.Oracle(c)
+    o.prepare()
+    shape = c.box(0)
+    Bc, Jc = np.array([0.2, -0.4, 0.7]), np.array([1.5, 0.3, -0.2])
+    o.set("B_cell", 0, np.stack([np.full(shape, v) for v in Bc]))
+    o.set("J_cell", 0, np.stack([np.full(shape, v) for v in Jc]))
+    # face fluxes of B(x) = B0 + Gm x (box coordinates), Gm[i][j] = dB_i / dx_j
+    Gm = np.array([[0.3, -1.1, 0.7], [0.9, -0.2, 1.3], [-0.6, 0.5, 0.1]])
+    B0 = np.array([0.4, -0.3, 0.2])
+    I, J, K = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij")
+    centre = [(I + 0.5) * H[0], (J + 0.5) * H[1], (K + 0.5) * H[2]]
+    area = (H[1] * H[2], H[0] * H[2], H[0] * H[1])
+    for ax, nm in enumerate(("B_xi", "B_eta", "B_zeta")):
+        x = list(centre)
+        x[ax] = x[ax] - 0.5 * H[ax]  # the face at the low side of the cell
+        o.set(nm, 0, area[ax] * (B0[ax] + sum(Gm[ax, q] * x[q] for q in range(3))))
+    assert np.all(o.get("taper_xi")[0][G:G + N[0], G:G + N[1] + 1, G:G + N[2] + 1] == 1.0)
+    o.op("hall_emf")
+    rhoH = float(o.get("U_H")[0][G, G, G])
+    rhoN = float(o.get("U_Na")[0][G, G, G])
+    alpha = P.hall_coef / _ne_eff(P, rhoH, rhoN)
+    be = abs(alpha) * np.linalg.norm(Bc)
+    curl = np.array([Gm[2, 1] - Gm[1, 2], Gm[0, 2] - Gm[2, 0], Gm[1, 0] - Gm[0, 1]])
+    ecen = alpha * np.cross(Jc, Bc)
+    for ax, nm in enumerate(("Ehall_xi", "Ehall_eta", "Ehall_zeta")):
+        want = ecen[ax] * H[ax] + 0.5 * 0.1 * be * H[ax] * curl[ax]
+        got = _owned_edges(o.get(nm)[0], ax)
+        # edges whose four faces lie in the owned region (ghost geometry is extrapolated)
+        inner = got[1:-1, 1:-1, 1:-1]
+        np.testing.assert_allclose(inner, want, rtol=1e-11, err_msg=nm)
+
+
+def test_hall_radial_taper(O):
+    u = MercuryUnits().alfven_units()
+    u.r_min, u.r_max = 1.15, 1.55
+    c = _block(units=u)
+    o = O.Oracle(c)
+    mid, width = 1.35, 0.4
+    for ax, s in enumerate(("xi", "eta", "zeta")):
+        Ie, Je, Ke = _edge_box(ax)
+        m = [1.0 + (Ie - G) * H[0], (Je - G) * H[1], (Ke - G) * H[2]]
+        m[ax] = m[ax] + 0.5 * H[ax]
+        r = np.sqrt(m[0] ** 2 + m[1] ** 2 + m[2] ** 2)
+        tp = _owned_edges(o.get("taper_" + s)[0], ax)
+        # monotone in r, within (0, 1), 0.5 (1 + tanh(2 (r - mid) / width))
+        order = np.argsort(r, axis=None)
+        assert np.all(np.diff(tp.ravel()[order]) >= -1e-15)  # equal radii may differ in the last bit
+        assert np.all((tp > 0.0) & (tp < 1.0))
+        np.testing.assert_allclose(tp, 0.5 * (1.0 + np.tanh(2.0 * (r - mid) / width)), rtol=1e-14)
+    # the xi edge whose midpoint sits on the mid radius (y = z = 0): x = 1.35
+    assert o.get("taper_xi")[0][G + 3, G, G] == pytest.approx(0.5, abs=1e-14)
