@@ -38,20 +38,21 @@ HBM_PEAK_GBS = 8000.0
 # (edge/face kernels, one point carries up to 3 edges or faces).
 ALGO_BYTES = {
     # fused passes
-    # flux + sources: U_s (10) + PV_s (10) + JD (9) + Jac + U+ (3) + B_cell (3) + J_cell (3) + Photo + centre (3) read,
-    # RHS_s (10) written
-    "rhs_cell": 8 * (10 + 10 + 9 + 1 + 3 + 3 + 3 + 1 + 3 + 10),
-    # B (3) + Badd (3) + U_con (3) + PV p (2) read, Eface (9) + node electron pressure (1) written
-    "eface_u": 8 * (3 + 3 + 3 + 2 + 9 + 1),
-    # Eface (9) + U_con (3) + B (3) + node electron pressure (1) + U rho (2) + J (3) + eta_local (3) + E_hc (3)
-    # + beta (1) + dr (9) + Area (3) + dl (3) + taper (3) read, E (3) + Ehall (3) written
-    "edge_total": 8 * (9 + 3 + 3 + 1 + 2 + 3 + 3 + 3 + 1 + 9 + 3 + 3 + 3 + 6),
+    # flux + sources: U_s density (2) + PV_s (10) + JD (9) + Jac + U+ (3) + B_cell (3) + J_cell (3) + Photo
+    # + centre (3) read, RHS_s (10) written
+    "rhs_cell": 8 * (2 + 10 + 9 + 1 + 3 + 3 + 3 + 1 + 3 + 10),
+    # B (3) + Badd (3) + U_con (3) + PV p (2) read, tangential Eface (6) + node electron pressure (1) written
+    "eface_u": 8 * (3 + 3 + 3 + 2 + 6 + 1),
+    # tangential Eface (6) + U_con (3) + B (3) + node electron pressure (1) + U rho (2) + J (3) + eta_local (3)
+    # + E_hc (3) + beta (1) + dr (9) + Area (3) + dl (3) + taper (3) read, E (3) + Ehall (3) written
+    "edge_total": 8 * (6 + 3 + 3 + 1 + 2 + 3 + 3 + 3 + 1 + 9 + 3 + 3 + 3 + 6),
     # U_s (10) + RHS_s (10) + E (3) + B (3) read, U_s (10) + B (3) + RHS_B (3) written
     "update_all": 8 * (10 + 10 + 3 + 3 + 10 + 3 + 3),
     # B (3) + Badd (3) + W_b (18) + beta (3) + alpha (3) read, B_cell + Bind (6) + J (3) written
     "bcell_jedge": 8 * (3 + 3 + 18 + 3 + 3 + 6 + 3),
-    # U_s (10) + J_cell (3) + B_cell (3) + JD (9) + Jac read, PV_s (10) + U+ (3) + E_hc (3) + beta + U_con (3) written
-    "cell_end": 8 * (10 + 3 + 3 + 9 + 1 + 10 + 3 + 3 + 1 + 3),
+    # U_s (10) + J_cell (3) + B_cell (3) + JD (9) + Jac + Area (3, the dt candidate) read,
+    # PV_s (10) + U+ (3) + E_hc (3) + beta + U_con (3) written
+    "cell_end": 8 * (10 + 3 + 3 + 9 + 1 + 3 + 10 + 3 + 3 + 1 + 3),
     # U_s (2x5) + PV_s u,v,w,p (2x4) + JD (9) + Jac (1) read, RHS_s (2x5) written
     "flux": 8 * (10 + 8 + 9 + 1 + 10),
     # U rho (2) + PV (2x5) + U+ (3) + B_cell (3) + J_cell (3) + Photo + centre (3) + Jac + JD (9) + RHS r/w (2x10)

@@ -427,26 +427,25 @@ __global__ void k_eface_u(const MercRange R, const MercBlk *__restrict__ B, doub
         if (!ok) continue;
         const long long Rq = p.q, Lq = p.q - AX(p, fa);
         const T bff = (T)F[SL_B + fa][Rq] + (T)F[SL_BADD + fa][Rq];
-        T e0 = T(0), e1 = T(0), e2 = T(0);
+        // only the two tangential components are written: the normal one is
+        // zero by construction (the per-function pass stores that zero) and
+        // its slot keeps the zero of the arena's initialisation
         if (fa == 0) {
             const T Gex = emf_flux_u<MIX>(F, G, p, 1, 0, Lq, Rq, bff);
             const T Gzx = emf_flux_u<MIX>(F, G, p, 2, 0, Lq, Rq, bff);
-            e1 = Gzx;
-            e2 = -Gex;
+            wr<MIX>(F, G, SL_EF + 1, Rq, Gzx);
+            wr<MIX>(F, G, SL_EF + 2, Rq, -Gex);
         } else if (fa == 1) {
             const T Gxe = emf_flux_u<MIX>(F, G, p, 0, 1, Lq, Rq, bff);
             const T Gze = emf_flux_u<MIX>(F, G, p, 2, 1, Lq, Rq, bff);
-            e0 = -Gze;
-            e2 = Gxe;
+            wr<MIX>(F, G, SL_EF + 3, Rq, -Gze);
+            wr<MIX>(F, G, SL_EF + 5, Rq, Gxe);
         } else {
             const T Gxz = emf_flux_u<MIX>(F, G, p, 0, 2, Lq, Rq, bff);
             const T Gez = emf_flux_u<MIX>(F, G, p, 1, 2, Lq, Rq, bff);
-            e0 = Gez;
-            e1 = -Gxz;
+            wr<MIX>(F, G, SL_EF + 6, Rq, Gez);
+            wr<MIX>(F, G, SL_EF + 7, Rq, -Gxz);
         }
-        wr<MIX>(F, G, SL_EF + 3 * fa, Rq, e0);
-        wr<MIX>(F, G, SL_EF + 3 * fa + 1, Rq, e1);
-        wr<MIX>(F, G, SL_EF + 3 * fa + 2, Rq, e2);
     }
 }
 
