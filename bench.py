@@ -138,6 +138,7 @@ def main():
                     help="first run three device-to-device copies of MIB MiB (a kernel of known traffic for HBM counters)")
     ap.add_argument("--resist-steps", type=int, default=20,
                     help="steps of the shell with the implicit resistive correction (0: skip that measurement)")
+    ap.add_argument("--tecplot", type=int, default=1, help="time one Tecplot output of the stepped state (0: skip)")
     ap.add_argument("--rshell-steps", type=int, default=20,
                     help="steps of the resistive shell (4 blocks glued by interfaces, exponential eta) per form; 0 skips it")
     ap.add_argument("--rshell-scale", type=float, default=1.0, help="size factor of the resistive shell (1: 18.9 M cells)")
@@ -232,6 +233,18 @@ def main():
                     t = op_times[label][op]
                     print(f"[bench] {label} op {op:12s} {t * 1e3:9.1f} us  {ALGO_BYTES[op] * ncells / (t * 1e-3) / 1e9:8.1f} GB/s",
                           file=sys.stderr)
+    # visualisation output of the stepped state (Tecplot, the Mercury variable list at the
+    # nodes): node means on the device, file written by the host; outside the timed steps
+    output = None
+    if args.tecplot and g is not None:
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "flow_field.plt")
+            t0 = time.perf_counter()
+            g.write_tecplot(path, time=0.0)
+            dt_out = time.perf_counter() - t0
+            output = {"tecplot_s": dt_out, "tecplot_bytes": os.path.getsize(path)}
+        print(f"[bench] tecplot output: {dt_out:.3f} s, {output['tecplot_bytes'] / 1e6:.1f} MB", file=sys.stderr, flush=True)
     ms_step = forms["fp64_fused"]["ms_per_step"]
     value = forms["fp64_fused"]["cell_updates_per_s"]
     divb_norm = forms["fp64_fused"]["divB_normalised"]
@@ -386,6 +399,7 @@ def main():
         "resistive": resist,
         "ensemble": ensemble,
         "resistive_shell": rshell,
+        "output": output,
     }
     print(json.dumps(out), flush=True)
 

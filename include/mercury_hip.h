@@ -204,6 +204,21 @@ int mercury_write_rundata(const char *path, int step, double time, double dt, in
 int mercury_read_rundata(const char *path, int *step, double *time, double *dt, int max_ref, int *nref, double *residual_ref,
                          double *residual_max);
 
+/* IOModule::WriteTecplotBinFile in the all-node mode the Mercury solver selects
+ * (src/6_io/IOModule_Tecplot.cpp:691-1017; Z4_Mercury/src/MercurySolver.cpp:53-86):
+ * a Tecplot binary (#!TDV112) file titled "flow_field" with one ordered zone
+ * "Zone<b>_<name>" per block over its (m0+1) x (m1+1) x (m2+1) owned nodes,
+ * variables x, y, z and the components of the listed cell or node fields, all
+ * written at the nodes as FP32 with FP64 min/max.  A cell value reaches a node
+ * as the |Jac|-weighted mean of the 8 cells around it, ghosts included
+ * (EvalValue_CellToNode_, :322-418), computed on the device.  fields = NULL
+ * (nfields = 0) writes the Mercury list PV_H, PV_Na, B_cell, Bind_cell, Na,
+ * J_cell with its component names (u_H ... Jz); unknown names and fields at
+ * faces or edges are skipped with a message, as in the reference.
+ * block_names = NULL names every zone's block "Fluid". */
+int mercury_write_tecplot(MercuryContext *ctx, const char *path, double time, const char *const *fields, int nfields,
+                          const char *const *block_names);
+
 typedef struct MercuryEnsemble MercuryEnsemble;
 int mercury_ensemble_create(MercuryContext *const *ctxs, int n, int integrator, MercuryEnsemble **out);
 int mercury_ensemble_launch(MercuryEnsemble *ens, int nsteps);

@@ -43,7 +43,10 @@ enum MercSlot {
     // node: electron pressure (mean of p_H + p_Na over the 8 cells around the
     // node) of the ambipolar EMF (fused path)
     SL_PEN = SL_RHSBR + 3,
-    SL_NSLOT = SL_PEN + 1
+    // cell: neutral Na density, a static input that the step does not read
+    // (the reference's "Na" field, written by the output)
+    SL_NANEU = SL_PEN + 1,
+    SL_NSLOT = SL_NANEU + 1
 };
 
 // physics constants (same layout as MercuryPhysics in include/mercury_hip.h)

@@ -28,6 +28,7 @@
 #include "mercury_kernels.h"
 #include "mercury_fused.h"
 #include "mercury_io.h"
+#include "mercury_output.h"
 #include "mercury_resist.h"
 
 static_assert(sizeof(MercuryPhysics) == sizeof(MercPhys), "MercuryPhysics / MercPhys layout mismatch");
@@ -58,6 +59,7 @@ static const FieldSpec kFields[] = {
     {"RHS_H", SL_RH, 5, LOC_CELL}, {"RHS_Na", SL_RN, 5, LOC_CELL}, {"Ehc", SL_EHC, 3, LOC_CELL}, {"hall_beta", SL_HB, 1, LOC_CELL},
     {"Jac", SL_JAC, 1, LOC_CELL}, {"Bcell_from_Bface_w", SL_WB, 18, LOC_CELL}, {"Jcell_from_Jedge_w", SL_WJ, 36, LOC_CELL},
     {"Photo_rate", SL_PHOTO, 1, LOC_CELL}, {"cell_center", SL_CC, 3, LOC_CELL}, {"divB", SL_DIVB, 1, LOC_CELL},
+    {"Na", SL_NANEU, 1, LOC_CELL},
     {"cell_scalar", SL_CELLS, 1, LOC_CELL},
     {"Eres_xi", SL_ERES, 1, LOC_EX}, {"Eres_eta", SL_ERES + 1, 1, LOC_EY}, {"Eres_zeta", SL_ERES + 2, 1, LOC_EZ},
     {"dJ_xi", SL_DJ, 1, LOC_EX}, {"dJ_eta", SL_DJ + 1, 1, LOC_EY}, {"dJ_zeta", SL_DJ + 2, 1, LOC_EZ},
@@ -1805,6 +1807,138 @@ int mercury_read_rundata(const char *path, int *step, double *time, double *dt, 
         if (residual_ref)
             for (size_t i = 0; i < r.residual_ref.size(); ++i) residual_ref[i] = r.residual_ref[i];
         if (residual_max) *residual_max = r.residual_max;
+    })
+}
+
+// ---------------------------------------------------------------------------
+// visualisation output: Tecplot binary with every variable at the nodes
+// ---------------------------------------------------------------------------
+extern "C++" {
+namespace {
+// the Mercury output list and its component names (Z4_Mercury/src/MercurySolver.cpp:53-86)
+struct TecOutput {
+    const char *field;
+    std::vector<std::string> comps;
+};
+const TecOutput kTecOutputs[] = {
+    {"PV_H", {"u_H", "v_H", "w_H", "p_H", "T_H"}},
+    {"PV_Na", {"u_Na", "v_Na", "w_Na", "p_Na", "T_Na"}},
+    {"B_cell", {"Bx", "By", "Bz"}},
+    {"Bind_cell", {"bx_ind", "by_ind", "bz_ind"}},
+    {"Na", {"n_Na"}},
+    {"J_cell", {"Jx", "Jy", "Jz"}},
+};
+
+// names of a field's components: the Mercury names, else name_c
+// (IOModule::AddTecFieldOrGroup_, src/6_io/IOModule_Tecplot.cpp:119-226)
+std::string tec_component_name(const std::string &field, int c) {
+    for (const TecOutput &t : kTecOutputs)
+        if (field == t.field && c < (int)t.comps.size()) return t.comps[c];
+    return field + "_" + std::to_string(c);
+}
+
+struct DeviceScratch {
+    double *p = nullptr;
+    explicit DeviceScratch(size_t n) { HIPCHK(hipMalloc(&p, sizeof(double) * (n ? n : 1))); }
+    ~DeviceScratch() {
+        if (p) (void)hipFree(p);
+    }
+};
+}  // namespace
+}
+
+int mercury_write_tecplot(MercuryContext *ctx, const char *path, double time, const char *const *fields, int nfields,
+                          const char *const *block_names) {
+    CHECK_CTX(ctx);
+    if (!path) {
+        merc::g_last_error = "mercury_write_tecplot: path is null";
+        return -1;
+    }
+    if (nfields < 0 || (nfields > 0 && !fields)) {
+        merc::g_last_error = "mercury_write_tecplot: bad field list";
+        return -1;
+    }
+    ABI_TRY({
+        merc::MercurySolver &S = *ctx->s;
+        HIPCHK(hipStreamSynchronize(S.st));
+        const merc::Field &F = *S.F;
+        std::vector<std::string> names;
+        if (nfields > 0)
+            for (int t = 0; t < nfields; ++t) names.push_back(fields[t] ? fields[t] : "");
+        else
+            for (const auto &t : kTecOutputs) names.push_back(t.field);
+        // variables: coordinates, then the components of every cell or node field
+        struct Var {
+            int slot, loc;
+        };
+        std::vector<std::string> vnames = {"x", "y", "z"};
+        std::vector<Var> vars;
+        std::vector<std::string> seen;
+        for (const std::string &nm : names) {
+            if (nm.empty()) continue;
+            const merc::FieldSpec *f = merc::find_field(nm.c_str());
+            if (!f || (f->loc != LOC_CELL && f->loc != LOC_NODE)) {
+                std::fprintf(stderr, "[mercury_write_tecplot] skip missing or non-cell/node field: %s\n", nm.c_str());
+                continue;
+            }
+            if (std::find(seen.begin(), seen.end(), nm) != seen.end()) continue;
+            seen.push_back(nm);
+            for (int c = 0; c < f->ncomp; ++c) {
+                vnames.push_back(tec_component_name(nm, c));
+                vars.push_back({f->slot + c, f->loc});
+            }
+        }
+        for (const Var &v : vars) S.materialize(v.slot);
+        std::vector<merc::io::TecplotWriter::Zone> zones;
+        long long nmax = 0;
+        for (int b = 0; b < F.nb(); ++b) {
+            const MercBlk &k = F.blk[b];
+            merc::io::TecplotWriter::Zone z;
+            z.name = "Zone" + std::to_string(b) + "_" + ((block_names && block_names[b]) ? block_names[b] : "Fluid");
+            z.imax = k.m[0] + 1;
+            z.jmax = k.m[1] + 1;
+            z.kmax = k.m[2] + 1;
+            zones.push_back(z);
+            nmax = std::max(nmax, (long long)z.imax * z.jmax * z.kmax);
+        }
+        merc::io::TecplotWriter w(path, "flow_field", vnames, zones, time);
+        DeviceScratch dnode((size_t)nmax);
+        std::vector<double> tmp, box;
+        for (int b = 0; b < F.nb(); ++b) {
+            const MercBlk &k = F.blk[b];
+            const int ni = k.m[0] + 1, nj = k.m[1] + 1, nk = k.m[2] + 1;
+            const size_t n = (size_t)ni * nj * nk;
+            auto file_index = [&](int i, int j, int kk) { return ((size_t)kk * nj + j) * ni + i; };
+            std::vector<std::vector<double>> values(vnames.size(), std::vector<double>(n));
+            const merc::BlockGeom &G = *S.geom[b];
+            for (int i = 0; i < ni; ++i)
+                for (int j = 0; j < nj; ++j)
+                    for (int kk = 0; kk < nk; ++kk) {
+                        const auto x = G.node(i, j, kk);
+                        for (int c = 0; c < 3; ++c) values[c][file_index(i, j, kk)] = x[c];
+                    }
+            for (size_t v = 0; v < vars.size(); ++v) {
+                std::vector<double> &dst = values[3 + v];
+                if (vars[v].loc == LOC_CELL) {
+                    merc::launch_node_average(S.st, F.hptr[vars[v].slot] + k.off, F.hptr[SL_JAC] + k.off, k, dnode.p);
+                    tmp.resize(n);
+                    HIPCHK(hipMemcpyAsync(tmp.data(), dnode.p, sizeof(double) * n, hipMemcpyDeviceToHost, S.st));
+                    HIPCHK(hipStreamSynchronize(S.st));
+                    for (int i = 0; i < ni; ++i)
+                        for (int j = 0; j < nj; ++j)
+                            for (int kk = 0; kk < nk; ++kk) dst[file_index(i, j, kk)] = tmp[((size_t)i * nj + j) * nk + kk];
+                } else {
+                    box.resize(F.vol(b));
+                    F.download_block(vars[v].slot, b, box.data());
+                    for (int i = 0; i < ni; ++i)
+                        for (int j = 0; j < nj; ++j)
+                            for (int kk = 0; kk < nk; ++kk)
+                                dst[file_index(i, j, kk)] = box[(size_t)(i + k.g) * k.sI + (size_t)(j + k.g) * k.sJ + (kk + k.g)];
+                }
+            }
+            w.zone_data(values);
+        }
+        w.close();
     })
 }
 

@@ -165,5 +165,88 @@ RunData read_rundata(const std::string &path) {
     return r;
 }
 
+namespace {
+void tec_i32(std::FILE *fp, int32_t v) { std::fwrite(&v, 4, 1, fp); }
+void tec_f32(std::FILE *fp, float v) { std::fwrite(&v, 4, 1, fp); }
+void tec_f64(std::FILE *fp, double v) { std::fwrite(&v, 8, 1, fp); }
+void tec_str(std::FILE *fp, const std::string &s) {
+    for (char c : s) tec_i32(fp, (int32_t)c);
+    tec_i32(fp, 0);
+}
+}  // namespace
+
+TecplotWriter::TecplotWriter(const std::string &path, const std::string &title, const std::vector<std::string> &vars,
+                             const std::vector<Zone> &zones, double time)
+    : path_(path), zones_(zones), nvar_(vars.size()) {
+    fp_ = std::fopen(path.c_str(), "wb");
+    if (!fp_) fail("cannot open " + path + " for writing");
+    std::fwrite("#!TDV112", 8, 1, fp_);
+    tec_i32(fp_, 1);  // byte order: little endian
+    tec_i32(fp_, 0);  // full file
+    tec_str(fp_, title);
+    tec_i32(fp_, (int32_t)vars.size());
+    for (const auto &v : vars) tec_str(fp_, v);
+    for (const Zone &z : zones_) {
+        tec_f32(fp_, 299.0f);
+        tec_str(fp_, z.name);
+        tec_i32(fp_, -1);  // parent zone
+        tec_i32(fp_, -2);  // strand
+        tec_f64(fp_, time);
+        tec_i32(fp_, -1);  // colour
+        tec_i32(fp_, 0);   // ordered
+        tec_i32(fp_, 1);   // variable locations given
+        for (size_t v = 0; v < nvar_; ++v) tec_i32(fp_, 0);  // all nodal
+        tec_i32(fp_, 0);   // raw face neighbours
+        tec_i32(fp_, 0);   // miscellaneous face neighbours
+        tec_i32(fp_, z.imax);
+        tec_i32(fp_, z.jmax);
+        tec_i32(fp_, z.kmax);
+        tec_i32(fp_, 0);  // auxiliary data
+    }
+    tec_f32(fp_, 357.0f);  // end of header
+}
+
+TecplotWriter::~TecplotWriter() {
+    if (fp_) std::fclose(fp_);
+}
+
+void TecplotWriter::zone_data(const std::vector<std::vector<double>> &values) {
+    if (!fp_) fail("tecplot writer is closed");
+    if (next_ >= zones_.size()) fail("more zone data than zones");
+    if (values.size() != nvar_) fail("zone data: wrong variable count");
+    const Zone &z = zones_[next_++];
+    const size_t n = (size_t)z.imax * z.jmax * z.kmax;
+    tec_f32(fp_, 299.0f);
+    for (size_t v = 0; v < nvar_; ++v) tec_i32(fp_, 1);  // float
+    tec_i32(fp_, 0);   // passive variables
+    tec_i32(fp_, 0);   // shared variables
+    tec_i32(fp_, -1);  // shared connectivity
+    for (const auto &a : values) {
+        if (a.size() != n) fail("zone data: wrong value count");
+        double lo = n ? a[0] : 0.0, hi = lo;
+        for (double x : a) {
+            if (x < lo) lo = x;
+            if (x > hi) hi = x;
+        }
+        tec_f64(fp_, lo);
+        tec_f64(fp_, hi);
+    }
+    std::vector<float> buf;
+    for (const auto &a : values) {
+        buf.resize(n);
+        for (size_t t = 0; t < n; ++t) buf[t] = (float)a[t];
+        std::fwrite(buf.data(), sizeof(float), n, fp_);
+    }
+}
+
+void TecplotWriter::close() {
+    if (!fp_) return;
+    if (next_ != zones_.size()) fail("tecplot file closed before every zone was written");
+    const bool bad = std::ferror(fp_) != 0;
+    std::fclose(fp_);
+    fp_ = nullptr;
+    if (bad) fail("write failed: " + path_);
+}
+
 }  // namespace io
 }  // namespace merc

@@ -14,6 +14,7 @@
 //   | i32 nd | (str key, f64 value) * nd | i32 ni | (str key, i32 value) * ni
 #pragma once
 #include <array>
+#include <cstdio>
 #include <functional>
 #include <map>
 #include <string>
@@ -56,6 +57,37 @@ struct RunData {
 
 void write_rundata(const std::string &path, const RunData &r);
 RunData read_rundata(const std::string &path);
+
+// Tecplot binary (#!TDV112) of ordered zones with every variable at the nodes
+// (IOModule::WriteTecplotBinFile in the all-node mode,
+// src/6_io/IOModule_Tecplot.cpp:691-1017):
+//   "#!TDV112" | i32 1 | i32 0 | str title | i32 nvar | str name * nvar
+//   per zone: f32 299 | str zone | i32 -1 | i32 -2 | f64 time | i32 -1 | i32 0
+//             | i32 1 | i32 0 * nvar | i32 0 | i32 0 | i32 I | i32 J | i32 K | i32 0
+//   f32 357
+//   per zone: f32 299 | i32 1 * nvar | i32 0 | i32 0 | i32 -1
+//             | (f64 min, f64 max) * nvar | f32 values, variable-major, i fastest
+//   str = one i32 per character, then i32 0.
+class TecplotWriter {
+public:
+    struct Zone {
+        std::string name;
+        int imax = 0, jmax = 0, kmax = 0;
+    };
+    // writes the header and every zone header
+    TecplotWriter(const std::string &path, const std::string &title, const std::vector<std::string> &vars, const std::vector<Zone> &zones,
+                  double time);
+    ~TecplotWriter();
+    // the data section of the next zone: one array per variable, i fastest
+    void zone_data(const std::vector<std::vector<double>> &values);
+    void close();
+
+private:
+    std::FILE *fp_ = nullptr;
+    std::string path_;
+    std::vector<Zone> zones_;
+    size_t nvar_ = 0, next_ = 0;
+};
 
 }  // namespace io
 }  // namespace merc

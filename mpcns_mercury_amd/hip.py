@@ -24,7 +24,8 @@ ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercu
                "mercury_divb", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
                "mercury_time_op", "mercury_geometry_block", "mercury_set_form", "mercury_resist_stats",
                "mercury_ensemble_create", "mercury_ensemble_launch", "mercury_ensemble_destroy", "mercury_autotune",
-               "mercury_write_restart", "mercury_read_restart", "mercury_write_rundata", "mercury_read_rundata")
+               "mercury_write_restart", "mercury_read_restart", "mercury_write_rundata", "mercury_read_rundata",
+               "mercury_write_tecplot")
 
 
 class MercuryError(RuntimeError):
@@ -49,6 +50,7 @@ def load():
     L.mercury_read_restart.argtypes = [vp, cp, ip, dp]
     L.mercury_write_rundata.argtypes = [cp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, ctypes.c_double]
     L.mercury_read_rundata.argtypes = [cp, ip, dp, dp, ctypes.c_int, ip, dp, dp]
+    L.mercury_write_tecplot.argtypes = [vp, cp, ctypes.c_double, ctypes.POINTER(cp), ctypes.c_int, ctypes.POINTER(cp)]
     L.mercury_field_info.argtypes = [cp, ip, ip, ip]
     L.mercury_block_volume.restype = ctypes.c_longlong
     L.mercury_block_volume.argtypes = [vp, ctypes.c_int]
@@ -231,6 +233,20 @@ class Mercury:
         st, tm = ctypes.c_int(), ctypes.c_double()
         _chk(load().mercury_read_restart(self._h, str(path).encode(), ctypes.byref(st), ctypes.byref(tm)))
         return st.value, tm.value
+
+    def write_tecplot(self, path, time=0.0, fields=None, block_names=None):
+        """Tecplot binary of the listed cell/node fields at the nodes (default: the
+        Mercury list PV_H, PV_Na, B_cell, Bind_cell, Na, J_cell)."""
+        def _strs(xs):
+            if xs is None:
+                return None, 0
+            arr = (ctypes.c_char_p * len(xs))(*[str(x).encode() for x in xs])
+            return arr, len(xs)
+        fl, nf = _strs(fields)
+        bn, nb = _strs(block_names)
+        if bn is not None and nb != self.case.nb:
+            raise ValueError("block_names needs one name per block")
+        _chk(load().mercury_write_tecplot(self._h, str(path).encode(), float(time), fl, nf, bn))
 
     def resist_stats(self):
         """(unknown edges, iterations of the last solve, its relative residual)."""

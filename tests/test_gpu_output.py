@@ -1,0 +1,113 @@
+"""Tecplot output of device-resident fields (mercury_write_tecplot) through the C ABI.
+
+* the file parses with an independent reader of the reference's layout
+  (tecplot_format.py) and holds one ordered zone per block over its owned nodes;
+* every variable equals, in FP32, the numpy restatement of the reference's
+  cell -> node mean applied to the downloaded fields (bit for bit), and the
+  stored min/max are the FP64 extremes of those node values;
+* closed form: a field linear in the cell centres on a Cartesian block comes out
+  as the same linear function at the nodes;
+* the mixed form writes its FP32 intermediates widened, as downloads do;
+* unknown names and face fields are skipped, as in the reference.
+"""
+import numpy as np
+import pytest
+
+from mpcns_mercury_amd import cases
+from mpcns_mercury_amd.hip import Mercury
+
+import tecplot_format as tf
+
+pytestmark = pytest.mark.gpu
+
+MERCURY = [("PV_H", ["u_H", "v_H", "w_H", "p_H", "T_H"]), ("PV_Na", ["u_Na", "v_Na", "w_Na", "p_Na", "T_Na"]),
+           ("B_cell", ["Bx", "By", "Bz"]), ("Bind_cell", ["bx_ind", "by_ind", "bz_ind"]), ("Na", ["n_Na"]),
+           ("J_cell", ["Jx", "Jy", "Jz"])]
+
+
+def _check_file(g, c, path, time, fields=MERCURY):
+    f = tf.read_plt(path)
+    names = ["x", "y", "z"] + [n for _, comps in fields for n in comps]
+    assert f["title"] == "flow_field" and f["names"] == names
+    assert len(f["zones"]) == c.nb
+    for ib, (z, d) in enumerate(zip(f["zones"], f["data"])):
+        m = c.blocks[ib].dims
+        assert z["name"] == f"Zone{ib}_Fluid" and z["time"] == time
+        assert z["dims"] == tuple(x + 1 for x in m) and z["locs"] == [0] * len(names)
+        want = [c.blocks[ib].nodes[a] for a in range(3)]
+        jac = g.get("Jac", ib)[0]
+        for fld, comps in fields:
+            arr = g.get(fld, ib)
+            for cc in range(len(comps)):
+                want.append(tf.node_average(arr[cc], jac, c.g, m))
+        for v, (w, got, mm) in enumerate(zip(want, d["values"], d["minmax"])):
+            # the file is i fastest: (K, J, I)
+            np.testing.assert_array_equal(got, w.astype(np.float32).transpose(2, 1, 0), err_msg=f"{names[v]}[{ib}]")
+            assert mm == (float(w.min()), float(w.max())), names[v]
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_tecplot_holds_the_node_means(tmp_path, form):
+    c = cases.make_shell(scale=0.1)
+    print(f"[case] {c.name} tecplot form={form}", flush=True)
+    g = Mercury(c)
+    try:
+        rng = np.random.default_rng(3)
+        for ib in range(c.nb):
+            g.set("Na", ib, rng.random(c.box(ib)))
+        g.prepare()
+        g.set_form(form)
+        g.step(3, 0)
+        p = tmp_path / "flow_field.plt"
+        g.write_tecplot(p, time=0.375)
+        _check_file(g, c, p, 0.375)
+    finally:
+        g.close()
+
+
+def test_tecplot_of_a_split_case_names_its_zones(tmp_path):
+    c = cases.make_split_cube((12, 10, 8), (2, 2, 1))
+    g = Mercury(c)
+    try:
+        g.prepare()
+        g.step(2, 0)
+        p = tmp_path / "f.plt"
+        g.write_tecplot(p, time=1.0, fields=["PV_H", "B_xi", "no_such_field", "PV_H", "node_scalar"],
+                        block_names=[f"blk{b}" for b in range(c.nb)])
+        f = tf.read_plt(p)
+        assert f["names"] == ["x", "y", "z", "u_H", "v_H", "w_H", "p_H", "T_H", "node_scalar_0"]
+        assert [z["name"] for z in f["zones"]] == [f"Zone{b}_blk{b}" for b in range(c.nb)]
+        for ib, d in enumerate(f["data"]):
+            m = c.blocks[ib].dims
+            node = g.get("node_scalar", ib)[0][c.g:c.g + m[0] + 1, c.g:c.g + m[1] + 1, c.g:c.g + m[2] + 1]
+            np.testing.assert_array_equal(d["values"][-1], node.astype(np.float32).transpose(2, 1, 0))
+            pv = g.get("PV_H", ib)
+            jac = g.get("Jac", ib)[0]
+            np.testing.assert_array_equal(d["values"][3], tf.node_average(pv[0], jac, c.g, m).astype(np.float32).transpose(2, 1, 0))
+    finally:
+        g.close()
+
+
+def test_tecplot_linear_field_closed_form(tmp_path):
+    n, h, g0 = (6, 5, 4), (0.1, 0.2, 0.15), 4
+    origin = (1.0, -0.3, 0.2)
+    c = cases.make_uniform_block(cases.cartesian_nodes(n, h, origin=origin), (0,) * 6, g=g0)
+    g = Mercury(c)
+    try:
+        g.prepare()
+        shape = c.box(0)
+        I, J, K = np.meshgrid(*[np.arange(s, dtype=np.float64) for s in shape], indexing="ij")
+        xc = [origin[a] + (X - g0 + 0.5) * h[a] for a, X in enumerate((I, J, K))]
+        coef = np.array([[0.5, 1.0, -2.0, 3.0], [1.5, -0.25, 0.75, 0.5], [-1.0, 2.0, 0.5, -1.5], [2.0, 0.0, 1.0, 1.0],
+                         [0.25, 1.0, 1.0, 1.0]])
+        pv = np.stack([co[0] + co[1] * xc[0] + co[2] * xc[1] + co[3] * xc[2] for co in coef])
+        g.set("PV_H", 0, pv)
+        p = tmp_path / "lin.plt"
+        g.write_tecplot(p, fields=["PV_H"])
+        d = tf.read_plt(p)["data"][0]
+        x = [c.blocks[0].nodes[a] for a in range(3)]
+        for q, co in enumerate(coef):
+            want = co[0] + co[1] * x[0] + co[2] * x[1] + co[3] * x[2]
+            np.testing.assert_allclose(d["values"][3 + q], want.transpose(2, 1, 0), rtol=2e-7, atol=1e-6)
+    finally:
+        g.close()
