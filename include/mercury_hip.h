@@ -219,6 +219,37 @@ int mercury_read_rundata(const char *path, int *step, double *time, double *dt, 
 int mercury_write_tecplot(MercuryContext *ctx, const char *path, double time, const char *const *fields, int nfields,
                           const char *const *block_names);
 
+/* CASE geometry directories: Grid::ReadGeometry (src/1_grid/1_MPCNS_Grid.cpp:10-590)
+ * over the files of every rank ("grid   r.grd", boundary_condition/"boundary   r.txt",
+ * boundary_name.fvbnd, "inner   r.txt", "parallel   r.txt"; r padded to 4
+ * characters), merged into one block list in rank-major order, and the patches
+ * of TOPO::build_topology (src/2_topology/TopologyBuilder.cpp:436-462): inner
+ * patches as written, parallel patches paired by their send/receive flags in
+ * the target rank's file, each with the IndexTransform of
+ * make_index_transform_from_boundary_arrays (src/2_topology/TopologyOps.cpp:118-219).
+ * nrank <= 0 counts the grid files present; binary selects Read_Grid_Binary
+ * (Read_Method != 0) or the text Read_Grid. */
+typedef struct MercuryCase MercuryCase;
+int mercury_case_read(MercuryCase **c, const char *geometry_dir, int nrank, int binary);
+void mercury_case_destroy(MercuryCase *c);
+/* block count and, when dims is not NULL, cells per axis (nodes - 1) per block */
+int mercury_case_blocks(const MercuryCase *c, int *nblocks, int *dims);
+int mercury_case_block_name(const MercuryCase *c, int block, char *buf, int len);
+/* owned node coordinates in the layout mercury_create takes */
+int mercury_case_nodes(const MercuryCase *c, double *owned_nodes);
+/* bc / nbr / transform for mercury_create_connected: every face must be one
+ * patch over the whole face, a Farfield or Solid_Surface boundary (codes
+ * MERCURY_BC_FARFIELD, MERCURY_BC_WALL) or a connection (MERCURY_BC_INTERFACE,
+ * periodic ones included); anything else fails with a message naming the face. */
+int mercury_case_topology(const MercuryCase *c, int *bc, int *nbr, int *transform);
+/* patch listings (records may be NULL to query n).  Connection patch, 27 ints:
+ * kind (0 inner, 1 parallel), periodic, block, nb_block, direction,
+ * nb_direction (+-(axis+1)), node box lo[3] hi[3], neighbour node box lo[3]
+ * hi[3] (half-open), perm[3], sign[3], offset[3].  Physical patch, 9 ints:
+ * block, direction, bc id (boundary_name.fvbnd), node box lo[3] hi[3]. */
+int mercury_case_interfaces(const MercuryCase *c, int *n, int *records);
+int mercury_case_physical(const MercuryCase *c, int *n, int *records);
+
 typedef struct MercuryEnsemble MercuryEnsemble;
 int mercury_ensemble_create(MercuryContext *const *ctxs, int n, int integrator, MercuryEnsemble **out);
 int mercury_ensemble_launch(MercuryEnsemble *ens, int nsteps);

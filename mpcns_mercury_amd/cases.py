@@ -243,6 +243,29 @@ def split_block_case(case: Case, cuts, name: str | None = None) -> Case:
                 A=A if case.A is not None else None, Bu=case.Bu, steps=case.steps, integrator=case.integrator, meta=meta)
 
 
+def case_from_geometry(geo, template: Case, name: str | None = None) -> Case:
+    """A case on the blocks of a CASE geometry directory (``hip.CaseGeometry``):
+    dimensions, node coordinates and face topology come from the files; the
+    physics, the fields on every block box, the edge potential and the uniform
+    field come from ``template``, whose blocks must have the same sizes in the
+    same order.  Connections must be translations (the solver's restriction)."""
+    if geo.nb != template.nb or any(tuple(geo.dims[b]) != tuple(template.blocks[b].dims) for b in range(geo.nb)):
+        raise ValueError("the template's blocks differ from the geometry's")
+    bc, nbr, tr = geo.topology()
+    blocks = []
+    for b in range(geo.nb):
+        offs = []
+        for f in range(6):
+            t = tr[9 * (6 * b + f):9 * (6 * b + f + 1)]
+            if bc[6 * b + f] == BC_INTERFACE and (tuple(t[:3]) != (0, 1, 2) or tuple(t[3:6]) != (1, 1, 1)):
+                raise ValueError(f"block {b} face {f}: rotated or mirrored connection (perm {tuple(t[:3])}, sign {tuple(t[3:6])})")
+            offs.append(tuple(int(x) for x in t[6:9]))
+        blocks.append(Block(dims=tuple(geo.dims[b]), bc=tuple(int(x) for x in bc[6 * b:6 * b + 6]), nodes=geo.nodes[b],
+                            nbr=tuple(int(x) for x in nbr[6 * b:6 * b + 6]), off=tuple(offs)))
+    return Case(name=name or f"{template.name}_case", g=template.g, blocks=blocks, phys=template.phys, fields=template.fields,
+                A=template.A, Bu=template.Bu, steps=template.steps, integrator=template.integrator, meta=dict(template.meta))
+
+
 def make_split_cube(n=(48, 48, 48), split=(2, 2, 2), g: int = 4, seed: int = 1, amp: float = 1e-3,
                     integrator: int = 0, steps: int = 100, hall: bool = False, resist: float | None = None,
                     cuts=None) -> Case:

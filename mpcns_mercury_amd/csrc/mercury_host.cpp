@@ -27,6 +27,7 @@
 #include "mercury_geometry.h"
 #include "mercury_kernels.h"
 #include "mercury_fused.h"
+#include "mercury_case.h"
 #include "mercury_io.h"
 #include "mercury_output.h"
 #include "mercury_resist.h"
@@ -1940,6 +1941,111 @@ int mercury_write_tecplot(MercuryContext *ctx, const char *path, double time, co
         }
         w.close();
     })
+}
+
+// ---------------------------------------------------------------------------
+// CASE geometry directories (Grid::ReadGeometry + TOPO::build_topology)
+// ---------------------------------------------------------------------------
+struct MercuryCase {
+    merc::casefile::Geometry G;
+};
+
+int mercury_case_read(MercuryCase **out, const char *geometry_dir, int nrank, int binary) {
+    if (!out || !geometry_dir) {
+        merc::g_last_error = "mercury_case_read: null argument";
+        return -1;
+    }
+    *out = nullptr;
+    ABI_TRY({
+        std::unique_ptr<MercuryCase> c(new MercuryCase);
+        c->G = merc::casefile::Geometry::read(geometry_dir, nrank, binary != 0);
+        *out = c.release();
+    })
+}
+
+void mercury_case_destroy(MercuryCase *c) { delete c; }
+
+int mercury_case_blocks(const MercuryCase *c, int *nblocks, int *dims) {
+    if (!c || !nblocks) {
+        merc::g_last_error = "mercury_case_blocks: null argument";
+        return -1;
+    }
+    *nblocks = (int)c->G.blocks.size();
+    if (dims)
+        for (size_t b = 0; b < c->G.blocks.size(); ++b)
+            for (int a = 0; a < 3; ++a) dims[3 * b + a] = c->G.blocks[b].nn[a] - 1;
+    return 0;
+}
+
+int mercury_case_block_name(const MercuryCase *c, int block, char *buf, int len) {
+    if (!c || !buf || len <= 0 || block < 0 || block >= (int)c->G.blocks.size()) {
+        merc::g_last_error = "mercury_case_block_name: bad argument";
+        return -1;
+    }
+    const std::string &n = c->G.blocks[block].name;
+    if ((int)n.size() >= len) {
+        merc::g_last_error = "mercury_case_block_name: buffer too small";
+        return -1;
+    }
+    std::memcpy(buf, n.c_str(), n.size() + 1);
+    return 0;
+}
+
+int mercury_case_nodes(const MercuryCase *c, double *owned_nodes) {
+    if (!c || !owned_nodes) {
+        merc::g_last_error = "mercury_case_nodes: null argument";
+        return -1;
+    }
+    double *o = owned_nodes;
+    for (const auto &B : c->G.blocks) {
+        const int ni = B.nn[0], nj = B.nn[1], nk = B.nn[2];
+        for (int a = 0; a < 3; ++a)
+            for (int i = 0; i < ni; ++i)
+                for (int j = 0; j < nj; ++j)
+                    for (int k = 0; k < nk; ++k) *o++ = B.xyz[a][((size_t)k * nj + j) * ni + i];
+    }
+    return 0;
+}
+
+int mercury_case_topology(const MercuryCase *c, int *bc, int *nbr, int *transform) {
+    if (!c || !bc || !nbr || !transform) {
+        merc::g_last_error = "mercury_case_topology: null argument";
+        return -1;
+    }
+    ABI_TRY({ c->G.face_topology(bc, nbr, transform); })
+}
+
+int mercury_case_interfaces(const MercuryCase *c, int *n, int *records) {
+    if (!c || !n) {
+        merc::g_last_error = "mercury_case_interfaces: null argument";
+        return -1;
+    }
+    *n = (int)c->G.interfaces.size();
+    if (records)
+        for (const auto &p : c->G.interfaces) {
+            const int head[6] = {p.kind, p.periodic ? 1 : 0, p.block, p.nb_block, p.direction, p.nb_direction};
+            for (int v : head) *records++ = v;
+            for (const int *a : {p.lo, p.hi, p.nb_lo, p.nb_hi, p.perm, p.sign, p.offset})
+                for (int d = 0; d < 3; ++d) *records++ = a[d];
+        }
+    return 0;
+}
+
+int mercury_case_physical(const MercuryCase *c, int *n, int *records) {
+    if (!c || !n) {
+        merc::g_last_error = "mercury_case_physical: null argument";
+        return -1;
+    }
+    *n = (int)c->G.physical.size();
+    if (records)
+        for (const auto &p : c->G.physical) {
+            *records++ = p.block;
+            *records++ = p.direction;
+            *records++ = p.bc_id;
+            for (int d = 0; d < 3; ++d) *records++ = p.sub[d];
+            for (int d = 0; d < 3; ++d) *records++ = p.sup[d] + 1;
+        }
+    return 0;
 }
 
 int mercury_init_bface(MercuryContext *ctx, const double *Bu, const double *const *A) {

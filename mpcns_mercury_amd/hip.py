@@ -25,7 +25,9 @@ ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercu
                "mercury_time_op", "mercury_geometry_block", "mercury_set_form", "mercury_resist_stats",
                "mercury_ensemble_create", "mercury_ensemble_launch", "mercury_ensemble_destroy", "mercury_autotune",
                "mercury_write_restart", "mercury_read_restart", "mercury_write_rundata", "mercury_read_rundata",
-               "mercury_write_tecplot")
+               "mercury_write_tecplot", "mercury_case_read", "mercury_case_destroy", "mercury_case_blocks",
+               "mercury_case_block_name", "mercury_case_nodes", "mercury_case_topology", "mercury_case_interfaces",
+               "mercury_case_physical")
 
 
 class MercuryError(RuntimeError):
@@ -51,6 +53,15 @@ def load():
     L.mercury_write_rundata.argtypes = [cp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, ctypes.c_double]
     L.mercury_read_rundata.argtypes = [cp, ip, dp, dp, ctypes.c_int, ip, dp, dp]
     L.mercury_write_tecplot.argtypes = [vp, cp, ctypes.c_double, ctypes.POINTER(cp), ctypes.c_int, ctypes.POINTER(cp)]
+    L.mercury_case_read.argtypes = [ctypes.POINTER(vp), cp, ctypes.c_int, ctypes.c_int]
+    L.mercury_case_destroy.argtypes = [vp]
+    L.mercury_case_destroy.restype = None
+    L.mercury_case_blocks.argtypes = [vp, ip, ip]
+    L.mercury_case_block_name.argtypes = [vp, ctypes.c_int, cp, ctypes.c_int]
+    L.mercury_case_nodes.argtypes = [vp, dp]
+    L.mercury_case_topology.argtypes = [vp, ip, ip, ip]
+    L.mercury_case_interfaces.argtypes = [vp, ip, ip]
+    L.mercury_case_physical.argtypes = [vp, ip, ip]
     L.mercury_field_info.argtypes = [cp, ip, ip, ip]
     L.mercury_block_volume.restype = ctypes.c_longlong
     L.mercury_block_volume.argtypes = [vp, ctypes.c_int]
@@ -282,6 +293,81 @@ class Ensemble:
     def close(self):
         if getattr(self, "_h", None):
             load().mercury_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CaseGeometry:
+    """A CASE geometry directory read by the library's C++ reader
+    (mercury_case_read): blocks of every rank in rank-major order, their
+    physical and connection patches, and the per-face topology for a solver."""
+
+    def __init__(self, geometry_dir, nrank=0, binary=True):
+        L = load()
+        h = ctypes.c_void_p()
+        _chk(L.mercury_case_read(ctypes.byref(h), str(geometry_dir).encode(), int(nrank), 1 if binary else 0))
+        self._h = h
+        nb = ctypes.c_int()
+        _chk(L.mercury_case_blocks(self._h, ctypes.byref(nb), None))
+        self.nb = nb.value
+        dims = np.zeros(3 * self.nb, dtype=np.int32)
+        _chk(L.mercury_case_blocks(self._h, ctypes.byref(nb), dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        self.dims = [tuple(int(x) for x in dims[3 * b:3 * b + 3]) for b in range(self.nb)]
+        self.names = []
+        buf = ctypes.create_string_buffer(256)
+        for b in range(self.nb):
+            _chk(L.mercury_case_block_name(self._h, b, buf, 256))
+            self.names.append(buf.value.decode())
+        flat = np.zeros(sum(3 * (m[0] + 1) * (m[1] + 1) * (m[2] + 1) for m in self.dims))
+        _chk(L.mercury_case_nodes(self._h, flat.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        self.nodes, o = [], 0
+        for m in self.dims:
+            n = 3 * (m[0] + 1) * (m[1] + 1) * (m[2] + 1)
+            self.nodes.append(flat[o:o + n].reshape(3, m[0] + 1, m[1] + 1, m[2] + 1).copy())
+            o += n
+
+    def _records(self, fn, width):
+        n = ctypes.c_int()
+        _chk(fn(self._h, ctypes.byref(n), None))
+        rec = np.zeros(max(n.value, 1) * width, dtype=np.int32)
+        _chk(fn(self._h, ctypes.byref(n), rec.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return rec[:n.value * width].reshape(n.value, width)
+
+    def interfaces(self):
+        """Connection patches: dicts with kind ('inner' / 'parallel'), periodic, block,
+        nb_block, direction, nb_direction, lo, hi, nb_lo, nb_hi, perm, sign, offset."""
+        out = []
+        for r in self._records(load().mercury_case_interfaces, 27):
+            r = [int(x) for x in r]
+            out.append({"kind": "parallel" if r[0] else "inner", "periodic": bool(r[1]), "block": r[2], "nb_block": r[3],
+                        "direction": r[4], "nb_direction": r[5], "lo": tuple(r[6:9]), "hi": tuple(r[9:12]),
+                        "nb_lo": tuple(r[12:15]), "nb_hi": tuple(r[15:18]), "perm": tuple(r[18:21]),
+                        "sign": tuple(r[21:24]), "offset": tuple(r[24:27])})
+        return out
+
+    def physical(self):
+        """Physical patches: dicts with block, direction, bc_id, lo, hi (half-open node box)."""
+        return [{"block": int(r[0]), "direction": int(r[1]), "bc_id": int(r[2]), "lo": tuple(int(x) for x in r[3:6]),
+                 "hi": tuple(int(x) for x in r[6:9])} for r in self._records(load().mercury_case_physical, 9)]
+
+    def topology(self):
+        """(bc, nbr, transform) per block face for mercury_create_connected; raises
+        MercuryError for faces that are not one whole-face patch."""
+        bc = np.zeros(6 * self.nb, dtype=np.int32)
+        nbr = np.zeros(6 * self.nb, dtype=np.int32)
+        tr = np.zeros(54 * self.nb, dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        _chk(load().mercury_case_topology(self._h, bc.ctypes.data_as(ip), nbr.ctypes.data_as(ip), tr.ctypes.data_as(ip)))
+        return bc, nbr, tr
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().mercury_case_destroy(self._h)
             self._h = None
 
     def __del__(self):

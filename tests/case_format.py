@@ -1,0 +1,114 @@
+"""Writer of CASE geometry directories in the reference's formats, for the tests
+(not a test module): the inverse of Grid::ReadGeometry
+(src/1_grid/1_MPCNS_Grid.cpp:10-590) for cases whose faces are whole-face
+boundaries or translation connections.
+
+Connection ranges are 1-based node indices; the first in-face axis is written
+positive and the second negative, on both sides, which is how the reader tells
+the in-face axes apart (Grid_Boundary.h Pre_process).  Connections between
+blocks of one rank go to "inner", across ranks to "parallel" with matching
+send/receive flags.  A connection whose two faces do not hold the same node
+coordinates is written as periodic (negative block numbers).
+"""
+import os
+import struct
+
+import numpy as np
+
+from mpcns_mercury_amd import cases
+
+BC_NAMES = ["No_Boundary_Condition", "Solid_Surface", "Symmetry", "Farfield", "Inflow", "Outflow", "Pole"]
+BC_ID = {name: i + 1 for i, name in enumerate(BC_NAMES)}  # line 2 is id 0, the next names get 2, 3, ...
+BC_ID["No_Boundary_Condition"] = 0
+
+
+def _suffix(r):
+    return f"{r:4d}"
+
+
+def _face_nodes(nodes, f):
+    a, side = f // 2, f & 1
+    sl = [slice(None)] * 3
+    sl[a] = -1 if side else 0
+    return nodes[(slice(None),) + tuple(sl)]
+
+
+def _ranges(dims, f):
+    """(sub, sup) of face f as 1-based signed node ranges: normal, then in-face
+    axes, the second of them negative."""
+    a, side = f // 2, f & 1
+    sub, sup = [0, 0, 0], [0, 0, 0]
+    inface = [d for d in range(3) if d != a]
+    v = (dims[a] if side else 0) + 1
+    sub[a] = sup[a] = v
+    sub[inface[0]], sup[inface[0]] = 1, dims[inface[0]] + 1
+    sub[inface[1]], sup[inface[1]] = -1, -(dims[inface[1]] + 1)
+    return sub, sup
+
+
+def _fmt(sub, sup):
+    return "\t".join(str(x) for d in range(3) for x in (sub[d], sup[d]))
+
+
+def write_case(path, case, ranks):
+    """Write ``case`` (blocks in rank-major order, ``ranks[b]`` the rank of block b)."""
+    nr = max(ranks) + 1
+    assert list(ranks) == sorted(ranks), "blocks must be in rank-major order"
+    local = [sum(1 for q in ranks[:b] if q == ranks[b]) for b in range(case.nb)]
+    os.makedirs(os.path.join(path, "boundary_condition"), exist_ok=True)
+    with open(os.path.join(path, "boundary_condition", "boundary_name.fvbnd"), "w") as f:
+        f.write("FVBND 1 3\n" + "\n".join(BC_NAMES) + "\nBOUNDARIES\n")
+    inner = [[[] for _ in range(ranks.count(r))] for r in range(nr)]
+    par = [[[] for _ in range(ranks.count(r))] for r in range(nr)]
+    phys = [[[] for _ in range(ranks.count(r))] for r in range(nr)]
+    flag = {}
+    for b, blk in enumerate(case.blocks):
+        r = ranks[b]
+        for f in range(6):
+            code = blk.bc[f]
+            if code in (cases.BC_FARFIELD, cases.BC_WALL):
+                a, side = f // 2, f & 1
+                lo = [0, 0, 0]
+                hi = list(blk.dims)
+                lo[a] = hi[a] = blk.dims[a] if side else 0
+                name = "Farfield" if code == cases.BC_FARFIELD else "Solid_Surface"
+                phys[r][local[b]].append("\t".join(str(x) for d in range(3) for x in (lo[d], hi[d])) + f"\t{BC_ID[name]}")
+                continue
+            assert code == cases.BC_INTERFACE, "periodic faces must be written as connections"
+            n, fo = blk.nbr[f], f ^ 1
+            sub, sup = _ranges(blk.dims, f)
+            tsub, tsup = _ranges(case.blocks[n].dims, fo)
+            same = np.array_equal(_face_nodes(blk.nodes, f), _face_nodes(case.blocks[n].nodes, fo))
+            if ranks[n] == r:
+                sgn = 1 if same else -1
+                inner[r][local[b]].append(f"{_fmt(sub, sup)}\t{sgn * (local[b] + 1)}\tFluid\t{_fmt(tsub, tsup)}\t{sgn * (local[n] + 1)}\tFluid")
+            else:
+                key, rkey = (b, f), (n, fo)
+                if key not in flag:
+                    flag[key], flag[rkey] = len(flag) + 1, len(flag) + 2
+                par[r][local[b]].append(f"{_fmt(sub, sup)}\t{ranks[n]}\t{flag[key]}\t{flag[rkey]}\tFluid")
+    for r in range(nr):
+        mine = [b for b in range(case.nb) if ranks[b] == r]
+        with open(os.path.join(path, f"grid{_suffix(r)}.grd"), "wb") as f:
+            f.write(struct.pack("<i", len(mine)))
+            for _ in mine:
+                name = b"Fluid\0"
+                f.write(struct.pack("<i", len(name)) + name)
+            for b in mine:
+                f.write(struct.pack("<3i", *(m + 1 for m in case.blocks[b].dims)))
+            for b in mine:
+                for c in range(3):  # k outer, j, i inner
+                    f.write(np.ascontiguousarray(case.blocks[b].nodes[c].transpose(2, 1, 0), dtype="<f8").tobytes())
+        with open(os.path.join(path, "boundary_condition", f"boundary{_suffix(r)}.txt"), "w") as f:
+            for rows in phys[r]:
+                f.write(f"\t{len(rows)}\n" + "".join(row + "\n" for row in rows))
+        with open(os.path.join(path, "boundary_condition", f"inner{_suffix(r)}.txt"), "w") as f:
+            f.write(f"\t{sum(len(x) for x in inner[r])}\n" + "".join(f"\t{len(x)}" for x in inner[r]) + "\n")
+            f.write("imin\timax\tjmin\tjmax\tkmin\tkmax\tid\tPhysic_Name\n")
+            for rows in inner[r]:
+                f.write("".join(row + "\n" for row in rows))
+        with open(os.path.join(path, "boundary_condition", f"parallel{_suffix(r)}.txt"), "w") as f:
+            f.write(f"\t{len(mine)}\t{sum(len(x) for x in par[r])}\n" + "".join(f"\t{len(x)}" for x in par[r]) + "\n")
+            f.write("imin\timax\tjmin\tjmax\tkmin\tkmax\ts_r_id\ts_flag\t\tr_flag\t\tPhysic_Name\n")
+            for rows in par[r]:
+                f.write("".join(row + "\n" for row in rows))
