@@ -57,6 +57,15 @@ DEVI long long xcd_tid() { return (long long)xcd_block() * blockDim.x + threadId
     Pt p;                    \
     if (!locate(R, B, TID, p)) return;
 
+// Field slot tables as the kernels use them: every slot pointer addresses
+// global memory (a global address is the same 64-bit value as its flat
+// address), and saying so makes the compiler emit global rather than flat
+// loads and stores, which wait on the vector-memory counter alone.
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) float gfloat;
+DEVI gdouble *const *gslots(double *const *F) { return static_cast<gdouble *const *>(static_cast<const void *>(F)); }
+DEVI const gfloat *const *gslots(const float *const *G) { return static_cast<const gfloat *const *>(static_cast<const void *>(G)); }
+
 // neighbour offsets
 #define OI(p) ((long long)(p).sI)
 #define OJ(p) ((long long)(p).sJ)
@@ -86,7 +95,7 @@ DEVI Num num_limiter(const MercPhys &P, double rhoH, double rhoNa) {
     return n;
 }
 
-DEVI void rusanov_face(const double *__restrict__ const *F, int su, int sp, long long qL, long long qR, long long qf, int a,
+DEVI void rusanov_face(const gdouble *__restrict__ const *F, int su, int sp, long long qL, long long qR, long long qf, int a,
                        double gm, double out[5]) {
     const double k1 = F[SL_JD + 3 * a][qf], k2 = F[SL_JD + 3 * a + 1][qf], k3 = F[SL_JD + 3 * a + 2][qf];
     double Uc[2][5], Fl[2][5], rad[2];
@@ -115,7 +124,7 @@ DEVI void rusanov_face(const double *__restrict__ const *F, int su, int sp, long
     for (int m = 0; m < 5; ++m) out[m] = 0.5 * (Fl[0][m] + Fl[1][m]) - 0.5 * r * (Uc[1][m] - Uc[0][m]);
 }
 
-DEVI double u_contra(double *const *__restrict__ F, const Pt &p, int axis, long long c) {
+DEVI double u_contra(gdouble *const *__restrict__ F, const Pt &p, int axis, long long c) {
     const long long cp = c + AX(p, axis);
     const int s0 = SL_JD + 3 * axis;
     const double kx = 0.5 * (F[s0][c] + F[s0][cp]);
@@ -124,14 +133,14 @@ DEVI double u_contra(double *const *__restrict__ F, const Pt &p, int axis, long 
     const double inv_jac = 1.0 / (fabs(F[SL_JAC][c]) + 1.0e-30);
     return (kx * F[SL_UP][c] + ky * F[SL_UP + 1][c] + kz * F[SL_UP + 2][c]) * inv_jac;
 }
-DEVI double beta_ind(double *const *__restrict__ F, const Pt &p, int axis, long long c) {
+DEVI double beta_ind(gdouble *const *__restrict__ F, const Pt &p, int axis, long long c) {
     return 0.5 * (F[SL_B + axis][c] + F[SL_B + axis][c + AX(p, axis)]);
 }
-DEVI double beta_tot(double *const *__restrict__ F, const Pt &p, int axis, long long c) {
+DEVI double beta_tot(gdouble *const *__restrict__ F, const Pt &p, int axis, long long c) {
     const long long cp = c + AX(p, axis);
     return 0.5 * ((F[SL_B + axis][c] + F[SL_BADD + axis][c]) + (F[SL_B + axis][cp] + F[SL_BADD + axis][cp]));
 }
-DEVI double emf_flux(double *const *__restrict__ F, const Pt &p, int ba, int fa, long long L, long long Rr, double bff) {
+DEVI double emf_flux(gdouble *const *__restrict__ F, const Pt &p, int ba, int fa, long long L, long long Rr, double bff) {
     const double btL = beta_tot(F, p, ba, L), btR = beta_tot(F, p, ba, Rr);
     const double biL = beta_ind(F, p, ba, L), biR = beta_ind(F, p, ba, Rr);
     const double ufL = u_contra(F, p, fa, L), ufR = u_contra(F, p, fa, Rr);
@@ -141,7 +150,7 @@ DEVI double emf_flux(double *const *__restrict__ F, const Pt &p, int ba, int fa,
     return 0.5 * (GL + GR) - 0.5 * radius * (biR - biL);
 }
 
-DEVI double maxu4(double *const *__restrict__ F, const Pt &p, int ax, long long c0, long long c1, long long c2, long long c3) {
+DEVI double maxu4(gdouble *const *__restrict__ F, const Pt &p, int ax, long long c0, long long c1, long long c2, long long c3) {
     double a = 0.0;
     a = dmax(a, fabs(u_contra(F, p, ax, c0)));
     a = dmax(a, fabs(u_contra(F, p, ax, c1)));

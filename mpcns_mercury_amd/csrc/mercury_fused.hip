@@ -41,13 +41,13 @@ template <> struct Prec<true> {
 // kept in FP32 by the mixed form are PV u, v, w, T (pressure stays FP64), U+,
 // the Hall cell terms, the contravariant U+, the face EMFs and the fluid RHS.
 template <bool MIX>
-DEVI typename Prec<MIX>::T st(double *const *__restrict__ F, const float *const *__restrict__ G, int s, long long q) {
+DEVI typename Prec<MIX>::T st(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, int s, long long q) {
     if constexpr (MIX) return G[s][q];
     else return F[s][q];
 }
 template <bool MIX, typename V>
-DEVI void wr(double *const *__restrict__ F, const float *const *__restrict__ G, int s, long long q, V v) {
-    if constexpr (MIX) const_cast<float *>(G[s])[q] = (float)v;
+DEVI void wr(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, int s, long long q, V v) {
+    if constexpr (MIX) const_cast<gfloat *>(G[s])[q] = (float)v;
     else F[s][q] = (double)v;
 }
 template <typename T> DEVI T tmax(T a, T b) { return (a < b) ? b : a; }
@@ -72,7 +72,7 @@ template <bool MIX> struct Acc {
 };
 
 template <bool MIX>
-DEVI void rusanov_face_t(double *const *__restrict__ F, const float *const *__restrict__ G, int su, int sp, long long qL, long long qR,
+DEVI void rusanov_face_t(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, int su, int sp, long long qL, long long qR,
                          long long qf, int a, typename Prec<MIX>::T gm, typename Prec<MIX>::T out[5]) {
     using T = typename Prec<MIX>::T;
     using PR = Prec<MIX>;
@@ -111,8 +111,10 @@ DEVI void rusanov_face_t(double *const *__restrict__ F, const float *const *__re
 // i-neighbour planes are read while they are still in cache.  The launch range
 // has ceil(m0 / L) points along i per block.
 template <bool MIX>
-__global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
-                           const float *const *__restrict__ G, const MercPhys P, int L) {
+__global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
+                           const float *const *__restrict__ G_, const MercPhys P, int L) {
+    gdouble *const *__restrict__ F = gslots(F_);
+    const gfloat *const *__restrict__ G = gslots(G_);
     using T = typename Prec<MIX>::T;
     LOCATE(R);
     const MercBlk &bk = B[p.b];
@@ -239,9 +241,11 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
 // contravariant velocity of U+ per axis (cell_u_contra, Ideal.cpp:513-526) and,
 // on owned cells when requested, the CFL candidate of Compute_Timestep.
 template <bool MIX>
-__global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
-                           const float *const *__restrict__ G, const MercPhys P, unsigned long long *__restrict__ dt_bits,
+__global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
+                           const float *const *__restrict__ G_, const MercPhys P, unsigned long long *__restrict__ dt_bits,
                            int compute_dt, int hall64) {
+    gdouble *const *__restrict__ F = gslots(F_);
+    const gfloat *const *__restrict__ G = gslots(G_);
     using T = typename Prec<MIX>::T;
     using PR = Prec<MIX>;
     double best = 1e100;
@@ -378,7 +382,7 @@ __global__ void k_cell_end(const MercRange R, const MercBlk *__restrict__ B, dou
 
 // ---------------------------------------------------------------------------
 template <bool MIX>
-DEVI typename Prec<MIX>::T emf_flux_u(double *const *__restrict__ F, const float *const *__restrict__ G, const Pt &p, int ba, int fa,
+DEVI typename Prec<MIX>::T emf_flux_u(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, const Pt &p, int ba, int fa,
                                       long long L, long long Rr, typename Prec<MIX>::T bff) {
     using T = typename Prec<MIX>::T;
     const long long dL = AX(p, ba);
@@ -399,8 +403,10 @@ DEVI typename Prec<MIX>::T emf_flux_u(double *const *__restrict__ F, const float
 // (MercurySolver_MagExplicit_Ambipo.cpp:3-143), once per node instead of once
 // per edge end
 template <bool MIX>
-__global__ void k_eface_u(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
-                          const float *const *__restrict__ G, int ambi) {
+__global__ void k_eface_u(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
+                          const float *const *__restrict__ G_, int ambi) {
+    gdouble *const *__restrict__ F = gslots(F_);
+    const gfloat *const *__restrict__ G = gslots(G_);
     using T = typename Prec<MIX>::T;
     LOCATE(R);
     const MercBlk &bk = B[p.b];
@@ -450,7 +456,7 @@ __global__ void k_eface_u(const MercRange R, const MercBlk *__restrict__ B, doub
 }
 
 template <bool MIX>
-DEVI typename Prec<MIX>::T maxu4_p(double *const *__restrict__ F, const float *const *__restrict__ G, int ax, long long c0, long long c1,
+DEVI typename Prec<MIX>::T maxu4_p(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, int ax, long long c0, long long c1,
                                    long long c2, long long c3) {
     using T = typename Prec<MIX>::T;
     T a = T(0);
@@ -465,8 +471,10 @@ DEVI typename Prec<MIX>::T maxu4_p(double *const *__restrict__ F, const float *c
 // applies the wall condition (used when no periodic seam needs the alias mean
 // of the non-Hall part first)
 template <bool MIX>
-__global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
-                             const float *const *__restrict__ G, const MercPhys P, int with_hall) {
+__global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
+                             const float *const *__restrict__ G_, const MercPhys P, int with_hall) {
+    gdouble *const *__restrict__ F = gslots(F_);
+    const gfloat *const *__restrict__ G = gslots(G_);
     using T = typename Prec<MIX>::T;
     using PR = Prec<MIX>;
     LOCATE(R);
@@ -603,8 +611,10 @@ __global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, d
 // ---------------------------------------------------------------------------
 // (the RHS comes from the FP32 table in the mixed form; the accumulation is FP64)
 template <bool MIX>
-__global__ void k_update_all(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
-                             const float *const *__restrict__ G, const double *__restrict__ dtp) {
+__global__ void k_update_all(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
+                             const float *const *__restrict__ G_, const double *__restrict__ dtp) {
+    gdouble *const *__restrict__ F = gslots(F_);
+    const gfloat *const *__restrict__ G = gslots(G_);
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
@@ -636,8 +646,10 @@ __global__ void k_update_all(const MercRange R, const MercBlk *__restrict__ B, d
 
 // ---------------------------------------------------------------------------
 template <bool MIX>
-__global__ void k_bcell_jedge(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
-                              const float *const *__restrict__ G) {
+__global__ void k_bcell_jedge(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
+                              const float *const *__restrict__ G_) {
+    gdouble *const *__restrict__ F = gslots(F_);
+    const gfloat *const *__restrict__ G = gslots(G_);
     using T = typename Prec<MIX>::T;
     LOCATE(R);
     const MercBlk &bk = B[p.b];
@@ -688,8 +700,10 @@ __global__ void k_bcell_jedge(const MercRange R, const MercBlk *__restrict__ B, 
 
 // calc_Jcell (MercurySolver_Current.cpp:203-259), owned cells
 template <bool MIX>
-__global__ void k_jcell_t(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
-                          const float *const *__restrict__ G) {
+__global__ void k_jcell_t(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
+                          const float *const *__restrict__ G_) {
+    gdouble *const *__restrict__ F = gslots(F_);
+    const gfloat *const *__restrict__ G = gslots(G_);
     using T = typename Prec<MIX>::T;
     LOCATE(R);
     const long long q = p.q, di = OI(p), dj = OJ(p);

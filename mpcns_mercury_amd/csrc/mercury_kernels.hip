@@ -25,7 +25,8 @@ using namespace merc_dev;
 // ---------------------------------------------------------------------------
 
 // calc_PV + calc_Uplus (MercurySolver_Tools.cpp:159-301), whole box
-__global__ void k_pv_uplus(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, const MercPhys P) {
+__global__ void k_pv_uplus(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, const MercPhys P) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const long long q = p.q;
     double pv[2][5];
@@ -54,7 +55,8 @@ __global__ void k_pv_uplus(const MercRange R, const MercBlk *__restrict__ B, dou
 }
 
 // calc_Bcell (MercurySolver_MagneticCell.cpp:3-81), owned cells
-__global__ void k_bcell(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F) {
+__global__ void k_bcell(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const long long q = p.q, di = OI(p), dj = OJ(p);
     double pt[6], pi[6];
@@ -89,7 +91,8 @@ __global__ void k_bcell(const MercRange R, const MercBlk *__restrict__ B, double
 // followed by J *= alpha (MercurySolver_Current.cpp:140-201); owned edges of the
 // three axes, one thread per node-box point
 // (face slots sf.., edge slots se..: B -> J, or dB -> dJ of the implicit correction)
-__global__ void k_jedge(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, int sf, int se) {
+__global__ void k_jedge(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, int sf, int se) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
@@ -110,7 +113,8 @@ __global__ void k_jedge(const MercRange R, const MercBlk *__restrict__ B, double
 }
 
 // calc_Jcell (MercurySolver_Current.cpp:203-259), owned cells
-__global__ void k_jcell(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F) {
+__global__ void k_jcell(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const long long q = p.q, di = OI(p), dj = OJ(p);
     const double s[12] = {F[SL_J + 0][q], F[SL_J + 0][q + dj], F[SL_J + 0][q + 1], F[SL_J + 0][q + dj + 1],
@@ -130,8 +134,9 @@ __global__ void k_jcell(const MercRange R, const MercBlk *__restrict__ B, double
 // the IEEE bit patterns order like the values and an unsigned 64-bit atomic
 // minimum gives the exact global minimum independent of the order.
 // ---------------------------------------------------------------------------
-__global__ void k_timestep(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, const MercPhys P,
+__global__ void k_timestep(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, const MercPhys P,
                            unsigned long long *__restrict__ dt_bits) {
+    gdouble *const *__restrict__ F = gslots(F_);
     double best = 1e100;
     Pt p;
     if (locate(R, B, TID, p)) {
@@ -194,7 +199,8 @@ __global__ void k_timestep(const MercRange R, const MercBlk *__restrict__ B, dou
 // accumulates -dF/Jac in the reference's xi, eta, zeta order; no face flux
 // array round trip.
 // ---------------------------------------------------------------------------
-__global__ void k_flux_div(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, const MercPhys P) {
+__global__ void k_flux_div(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, const MercPhys P) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const long long q = p.q;
     const double gm = P.gamma;
@@ -205,8 +211,8 @@ __global__ void k_flux_div(const MercRange R, const MercBlk *__restrict__ B, dou
         for (int a = 0; a < 3; ++a) {
             const long long d = AX(p, a);
             double Fm[5], Fp[5];
-            rusanov_face((const double *const *)F, su, sp, q - d, q, q, a, gm, Fm);
-            rusanov_face((const double *const *)F, su, sp, q, q + d, q + d, a, gm, Fp);
+            rusanov_face(F, su, sp, q - d, q, q, a, gm, Fm);
+            rusanov_face(F, su, sp, q, q + d, q + d, a, gm, Fp);
 #pragma unroll
             for (int m = 0; m < 5; ++m) r[m] -= (Fp[m] - Fm[m]) * invJ;
         }
@@ -215,7 +221,8 @@ __global__ void k_flux_div(const MercRange R, const MercBlk *__restrict__ B, dou
 }
 
 // AddSourceToRHS_Fluid (MercurySolver_FluidRHS.cpp:206-527), owned cells
-__global__ void k_sources(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, const MercPhys P) {
+__global__ void k_sources(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, const MercPhys P) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const long long q = p.q, di = OI(p), dj = OJ(p);
     const double upx = F[SL_UP][q], upy = F[SL_UP + 1][q], upz = F[SL_UP + 2][q];
@@ -297,7 +304,8 @@ __global__ void k_sources(const MercRange R, const MercBlk *__restrict__ B, doub
 // ---------------------------------------------------------------------------
 // face candidates on owned faces +-1 (clamped to the allocation +-1); range is
 // the box [-1, m+2)^3 and each axis keeps its own valid sub-range
-__global__ void k_eface(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F) {
+__global__ void k_eface(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const int idx[3] = {p.i, p.j, p.k};
@@ -339,7 +347,8 @@ __global__ void k_eface(const MercRange R, const MercBlk *__restrict__ B, double
 // C_uct = 0.5), then in the reference's order AddAmbipolarEdgeEMF_
 // (Ambipo.cpp:3-143), AddArtificialResistivityToEdgeEMF_ (Resis.cpp:6-65) and
 // AddLocalArtificialResistivityToEdgeEMF_ (Resis.cpp:66-185)
-__global__ void k_edge_emf(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, const MercPhys P) {
+__global__ void k_edge_emf(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, const MercPhys P) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
@@ -420,7 +429,8 @@ __global__ void k_edge_emf(const MercRange R, const MercBlk *__restrict__ B, dou
 // ---------------------------------------------------------------------------
 // Hall EMF: BuildHallFaceEMF_Rusanov_diff_ (MercurySolver_MagExplicit_Hall.cpp:58-433)
 // ---------------------------------------------------------------------------
-__global__ void k_hall_cell(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, const MercPhys P) {
+__global__ void k_hall_cell(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, const MercPhys P) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const long long q = p.q;
     const Num n = num_limiter(P, F[SL_UH][q], F[SL_UN][q]);
@@ -433,7 +443,8 @@ __global__ void k_hall_cell(const MercRange R, const MercBlk *__restrict__ B, do
     F[SL_HB][q] = fabs(alpha) * sqrt(Bx * Bx + By * By + Bz * Bz);
 }
 
-__global__ void k_hall_edge(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F) {
+__global__ void k_hall_edge(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
@@ -488,7 +499,8 @@ __global__ void k_hall_edge(const MercRange R, const MercBlk *__restrict__ B, do
 // E += Ehall on owned edges (MercurySolver_MagneticRHS.cpp:134-147), then the
 // stationary-wall condition zeroes tangential E on owned wall edges
 // (ApplyStationaryWallNonResistiveEMF_, MercurySolver_MagneticRHS.cpp:5-89)
-__global__ void k_add_hall_wall(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F) {
+__global__ void k_add_hall_wall(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q;
@@ -507,8 +519,9 @@ __global__ void k_add_hall_wall(const MercRange R, const MercBlk *__restrict__ B
 
 // RHS_B = -d1(E) (CTOperators::CurlEdgeToFace, CTOperators.cpp:6-57) and the
 // face part of ApplyUpdate_Euler_ (MercurySolver_Advance.cpp:122-146): B += dt*RHS_B
-__global__ void k_curl_update(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F,
+__global__ void k_curl_update(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
                               const double *__restrict__ dtp, int do_update) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
@@ -534,7 +547,8 @@ __global__ void k_curl_update(const MercRange R, const MercBlk *__restrict__ B, 
 }
 
 // cell part of ApplyUpdate_Euler_ (MercurySolver_Advance.cpp:73-121)
-__global__ void k_update_u(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, const double *__restrict__ dtp) {
+__global__ void k_update_u(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, const double *__restrict__ dtp) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const long long q = p.q;
     const double dt = *dtp;
@@ -545,7 +559,8 @@ __global__ void k_update_u(const MercRange R, const MercBlk *__restrict__ B, dou
 }
 
 // SSP-RK2: save the step-start state / average 0.5*(U0 + U) on owned DOFs
-__global__ void k_rk2(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, int save) {
+__global__ void k_rk2(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, int save) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q;
@@ -577,28 +592,32 @@ __global__ void k_rk2(const MercRange R, const MercBlk *__restrict__ B, double *
 // fields (and of independent sync groups). Within one launch no destination is
 // the source of another entry.
 // COPY: entry (dst, src, set), dst <- src for every slot of the set
-__global__ void k_copy_multi(const int *__restrict__ ent, long long n, double *const *__restrict__ F, const MultiSets S) {
+__global__ void k_copy_multi(const int *__restrict__ ent, long long n, double *const *__restrict__ F_, const MultiSets S) {
+    gdouble *const *__restrict__ F = gslots(F_);
     const long long t = TID;
     if (t >= n) return;
     const int d = ent[3 * t], s = ent[3 * t + 1], k = ent[3 * t + 2];
     for (int c = 0; c < S.n[k]; ++c) F[S.s[k][c]][d] = F[S.s[k][c]][s];
 }
 // FILL: entry (dst, set), dst <- the set's constant per slot
-__global__ void k_fill_multi(const int *__restrict__ ent, long long n, double *const *__restrict__ F, const MultiSets S) {
+__global__ void k_fill_multi(const int *__restrict__ ent, long long n, double *const *__restrict__ F_, const MultiSets S) {
+    gdouble *const *__restrict__ F = gslots(F_);
     const long long t = TID;
     if (t >= n) return;
     const int d = ent[2 * t], k = ent[2 * t + 1];
     for (int c = 0; c < S.n[k]; ++c) F[S.s[k][c]][d] = S.v[k][c];
 }
 // SETV: entry (dst, slot) with its own value
-__global__ void k_setv_multi(const int *__restrict__ ent, const double *__restrict__ val, long long n, double *const *__restrict__ F) {
+__global__ void k_setv_multi(const int *__restrict__ ent, const double *__restrict__ val, long long n, double *const *__restrict__ F_) {
+    gdouble *const *__restrict__ F = gslots(F_);
     const long long t = TID;
     if (t >= n) return;
     F[ent[2 * t + 1]][ent[2 * t]] = val[t];
 }
 // BC_Solid_Surface_ (Boundary_Condition_Fluid.cpp:36-260): per wall cell the
 // entry holds (species base slot, is_na, first layer, one deeper, g ghosts)
-__global__ void k_wall_multi(const int *__restrict__ ent, long long n, int g, double *const *__restrict__ F, const MercPhys P) {
+__global__ void k_wall_multi(const int *__restrict__ ent, long long n, int g, double *const *__restrict__ F_, const MercPhys P) {
+    gdouble *const *__restrict__ F = gslots(F_);
     const long long t = TID;
     if (t >= n) return;
     const int *e = ent + (long long)t * (4 + g);
@@ -642,11 +661,12 @@ __global__ void k_wall_multi(const int *__restrict__ ent, long long n, int g, do
 // MercurySolver_Current.cpp:3-57); members summed in the listed order
 __global__ void k_alias_reduce(const int *__restrict__ cls_start, const int *__restrict__ member, const int *__restrict__ msign,
                                const int *__restrict__ owner, const int *__restrict__ osign, const int *__restrict__ slot_of,
-                               long long ncls, double *const *__restrict__ F, int slot0) {
+                               long long ncls, double *const *__restrict__ F_, int slot0) {
+    gdouble *const *__restrict__ F = gslots(F_);
     const long long t = TID;
     if (t >= ncls) return;
     const int b = cls_start[t], e = cls_start[t + 1];
-    double *f = F[slot0 + slot_of[t]];
+    gdouble *f = F[slot0 + slot_of[t]];
     double sum = 0.0;
     for (int m = b; m < e; ++m) sum += msign[m] * f[member[m]];
     f[owner[t]] = osign[t] * sum / (double)(e - b);
@@ -656,7 +676,8 @@ __global__ void k_alias_reduce(const int *__restrict__ cls_start, const int *__r
 // DEC operators (src/8_dec/DecExteriorDerivative.cpp) on generic slots
 // ---------------------------------------------------------------------------
 // d0: node scalar -> edge 1-form, e = phi(end) - phi(start), owned edges
-__global__ void k_dec_d0(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, int s_node, int s_edge) {
+__global__ void k_dec_d0(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, int s_node, int s_edge) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q;
@@ -665,7 +686,8 @@ __global__ void k_dec_d0(const MercRange R, const MercBlk *__restrict__ B, doubl
     if (p.k < bk.m[2]) F[s_edge + 2][q] = F[s_node][q + 1] - F[s_node][q];
 }
 // d1: edge 1-form -> face 2-form (circulation), multiplier as CurlEdgeToFace
-__global__ void k_dec_d1(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, int s_edge, int s_face, double mult) {
+__global__ void k_dec_d1(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, int s_edge, int s_face, double mult) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
@@ -677,14 +699,16 @@ __global__ void k_dec_d1(const MercRange R, const MercBlk *__restrict__ B, doubl
         F[s_face + 2][q] = mult * ((F[s_edge + 0][q] - F[s_edge + 0][q + dj]) + (F[s_edge + 1][q + di] - F[s_edge + 1][q]));
 }
 // d2: face 2-form -> cell 3-form (outward flux sum), owned cells
-__global__ void k_dec_d2(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, int s_face, int s_cell) {
+__global__ void k_dec_d2(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, int s_face, int s_cell) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const long long q = p.q;
     F[s_cell][q] = F[s_face][q + OI(p)] - F[s_face][q] + F[s_face + 1][q + OJ(p)] - F[s_face + 1][q] + F[s_face + 2][q + 1] - F[s_face + 2][q];
 }
 
 // calc_divB (MercurySolver_MagneticCell.cpp:151-241) plus max |d2 B| and max |B_face|
-__global__ void k_divb(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, unsigned long long *__restrict__ out) {
+__global__ void k_divb(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, unsigned long long *__restrict__ out) {
+    gdouble *const *__restrict__ F = gslots(F_);
     double dmx = 0.0, bmx = 0.0;
     Pt p;
     if (locate(R, B, TID, p)) {
@@ -714,7 +738,8 @@ struct SlotSet {
     int s[MERC_MAX_SYNC_SLOTS];
 };
 // zero a set of slots over whole blocks (replaces memset inside a captured step)
-__global__ void k_zero_span(double *const *__restrict__ F, const SlotSet S, long long n) {
+__global__ void k_zero_span(double *const *__restrict__ F_, const SlotSet S, long long n) {
+    gdouble *const *__restrict__ F = gslots(F_);
     const long long t = TID;
     if (t >= n) return;
     for (int c = 0; c < S.n; ++c) F[S.s[c]][t] = 0.0;

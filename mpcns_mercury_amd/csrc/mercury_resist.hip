@@ -36,14 +36,16 @@ DEVI void block_sum(double v, double *__restrict__ part) {
 
 // edge slots s.. <- x on the unknowns (x == nullptr: 0)
 __global__ void k_scatter(int n, const int *__restrict__ q, const int *__restrict__ ax, const double *__restrict__ x,
-                          double *const *__restrict__ F, int s) {
+                          double *const *__restrict__ F_, int s) {
+    gdouble *const *__restrict__ F = gslots(F_);
     const int t = blockIdx.x * RB + threadIdx.x;
     if (t >= n) return;
     F[s + ax[t]][q[t]] = x ? x[t] : 0.0;
 }
 // x <- edge slots s.. on the unknowns
-__global__ void k_pick(int n, const int *__restrict__ q, const int *__restrict__ ax, double *const *__restrict__ F, int s,
+__global__ void k_pick(int n, const int *__restrict__ q, const int *__restrict__ ax, double *const *__restrict__ F_, int s,
                        double *__restrict__ x) {
+    gdouble *const *__restrict__ F = gslots(F_);
     const int t = blockIdx.x * RB + threadIdx.x;
     if (t >= n) return;
     x[t] = F[s + ax[t]][q[t]];
@@ -52,8 +54,9 @@ __global__ void k_pick(int n, const int *__restrict__ q, const int *__restrict__
 // y = Eres - eta * dJ on the unknowns (MatMultImplicitResistive_ :449-459);
 // with w != nullptr also the block partials of sum(p * y * w)
 __global__ void k_gather(int n, const int *__restrict__ q, const int *__restrict__ ax, const double *__restrict__ eta,
-                         double *const *__restrict__ F, double *__restrict__ y, const double *__restrict__ p,
+                         double *const *__restrict__ F_, double *__restrict__ y, const double *__restrict__ p,
                          const double *__restrict__ w, double *__restrict__ part) {
+    gdouble *const *__restrict__ F = gslots(F_);
     double v = 0.0;
     for (int t = blockIdx.x * RB + threadIdx.x; t < n; t += gridDim.x * RB) {
         const double yy = F[SL_ERES + ax[t]][q[t]] - eta[t] * F[SL_DJ + ax[t]][q[t]];
@@ -67,13 +70,14 @@ __global__ void k_gather(int n, const int *__restrict__ q, const int *__restrict
 // Jacobi diagonal 1 + dt*eta*alpha*sum(beta of the 4 faces around the edge)
 // and the inner-product weight 1/(eta*alpha)
 __global__ void k_rhs(int n, const int *__restrict__ q, const int *__restrict__ ax, const int *__restrict__ sI,
-                      const int *__restrict__ sJ, const double *__restrict__ eta, double *const *__restrict__ F, double dt,
+                      const int *__restrict__ sJ, const double *__restrict__ eta, double *const *__restrict__ F_, double dt,
                       double *__restrict__ b, double *__restrict__ dg, double *__restrict__ w) {
+    gdouble *const *__restrict__ F = gslots(F_);
     const int t = blockIdx.x * RB + threadIdx.x;
     if (t >= n) return;
     const long long e = q[t], di = sI[t], dj = sJ[t];
     const int a = ax[t];
-    const double *be = F[SL_BETA], *bh = F[SL_BETA + 1], *bz = F[SL_BETA + 2];
+    const gdouble *be = F[SL_BETA], *bh = F[SL_BETA + 1], *bz = F[SL_BETA + 2];
     double s;
     if (a == 0) s = bh[e - 1] + bh[e] + bz[e] + bz[e - dj];
     else if (a == 1) s = be[e] + be[e - 1] + bz[e - di] + bz[e];
@@ -163,7 +167,8 @@ __global__ void k_cg_scalars(int nb, int nq, const double *__restrict__ part, do
 
 // B += dt * RHS_Bres on owned faces (ApplyUpdate_Euler_BfaceOnly_,
 // MercurySolver_MagExplicit_Hall.cpp:435-473)
-__global__ void k_bres_update(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F, double dt) {
+__global__ void k_bres_update(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, double dt) {
+    gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q;
