@@ -50,8 +50,10 @@ def _fmt(sub, sup):
     return "\t".join(str(x) for d in range(3) for x in (sub[d], sup[d]))
 
 
-def write_case(path, case, ranks):
-    """Write ``case`` (blocks in rank-major order, ``ranks[b]`` the rank of block b)."""
+def write_case(path, case, ranks, binary=True):
+    """Write ``case`` (blocks in rank-major order, ``ranks[b]`` the rank of block b);
+    binary=False writes text grids (Grid::Read_Grid: block count, names, node
+    counts, then x, y, z of every block, i fastest)."""
     nr = max(ranks) + 1
     assert list(ranks) == sorted(ranks), "blocks must be in rank-major order"
     local = [sum(1 for q in ranks[:b] if q == ranks[b]) for b in range(case.nb)]
@@ -89,16 +91,25 @@ def write_case(path, case, ranks):
                 par[r][local[b]].append(f"{_fmt(sub, sup)}\t{ranks[n]}\t{flag[key]}\t{flag[rkey]}\tFluid")
     for r in range(nr):
         mine = [b for b in range(case.nb) if ranks[b] == r]
-        with open(os.path.join(path, f"grid{_suffix(r)}.grd"), "wb") as f:
-            f.write(struct.pack("<i", len(mine)))
-            for _ in mine:
-                name = b"Fluid\0"
-                f.write(struct.pack("<i", len(name)) + name)
-            for b in mine:
-                f.write(struct.pack("<3i", *(m + 1 for m in case.blocks[b].dims)))
-            for b in mine:
-                for c in range(3):  # k outer, j, i inner
-                    f.write(np.ascontiguousarray(case.blocks[b].nodes[c].transpose(2, 1, 0), dtype="<f8").tobytes())
+        if binary:
+            with open(os.path.join(path, f"grid{_suffix(r)}.grd"), "wb") as f:
+                f.write(struct.pack("<i", len(mine)))
+                for _ in mine:
+                    name = b"Fluid\0"
+                    f.write(struct.pack("<i", len(name)) + name)
+                for b in mine:
+                    f.write(struct.pack("<3i", *(m + 1 for m in case.blocks[b].dims)))
+                for b in mine:
+                    for c in range(3):  # k outer, j, i inner
+                        f.write(np.ascontiguousarray(case.blocks[b].nodes[c].transpose(2, 1, 0), dtype="<f8").tobytes())
+        else:
+            with open(os.path.join(path, f"grid{_suffix(r)}.grd"), "w") as f:
+                f.write(f"{len(mine)}\n" + " ".join("Fluid" for _ in mine) + "\n")
+                for b in mine:
+                    f.write(" ".join(str(m + 1) for m in case.blocks[b].dims) + "\n")
+                for b in mine:
+                    for c in range(3):
+                        f.write("\n".join(repr(float(x)) for x in case.blocks[b].nodes[c].transpose(2, 1, 0).ravel()) + "\n")
         with open(os.path.join(path, "boundary_condition", f"boundary{_suffix(r)}.txt"), "w") as f:
             for rows in phys[r]:
                 f.write(f"\t{len(rows)}\n" + "".join(row + "\n" for row in rows))

@@ -145,6 +145,24 @@ def test_round_trip_through_the_case_files(tmp_path, which, ranks):
         geo.close()
 
 
+def test_text_grids(tmp_path):
+    """Read_Method 0: text grids hold the same nodes (written with repr, so they
+    read back exactly)."""
+    c = cases.make_split_sector(dims=(16, 16, 12), cuts=((8, 8), (16,), (12,)), scale=1.0)
+    cf.write_case(tmp_path, c, [0, 1], binary=False)
+    geo = hip.CaseGeometry(tmp_path, binary=False)
+    try:
+        assert geo.nb == 2 and geo.names == ["Fluid", "Fluid"]
+        for b in range(c.nb):
+            np.testing.assert_array_equal(geo.nodes[b], c.blocks[b].nodes)
+        for got, want in zip(geo.topology(), _topology_of(c)):
+            np.testing.assert_array_equal(got, want)
+    finally:
+        geo.close()
+    with pytest.raises(hip.MercuryError):  # a text grid is not a binary one
+        hip.CaseGeometry(tmp_path, binary=True)
+
+
 def test_unsupported_boundaries_are_named(tmp_path):
     c = cases.make_split_sector(dims=(16, 16, 12), cuts=((16,), (8, 8), (12,)), scale=1.0)
     cf.write_case(tmp_path, c, [0, 0])
