@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from mpcns_mercury_amd import cases
-from mpcns_mercury_amd.hip import Mercury
+from mpcns_mercury_amd.hip import Mercury, MercuryError
 
 pytestmark = pytest.mark.gpu
 
@@ -224,3 +224,30 @@ def test_mixed_precision_accuracy_100_steps(pair):
     print("[mixed 100] rel-L2:", errs, flush=True)
     assert all(e <= 1e-5 for e in errs.values()), errs
     assert g.divb()[0] <= 1e-12
+
+
+@pytest.mark.parametrize("form", [0, 1, 2])
+def test_dt_underflow_is_an_error(oracle_mod, form):
+    """Compute_Timestep's abort rule (MercurySolver_Timestep.cpp:129-140): a cell
+    moving at 1e10 drives dt below 1e-7, and the step fails through the C ABI
+    with the reason, as the oracle refuses the same step; a healthy state steps."""
+    case = small_cube()
+    print(f"[case] {case.name} dt underflow form={form}", flush=True)
+    g = Mercury(case)
+    o = oracle_mod.Oracle(case)
+    try:
+        U = g.get("U_H", 0)
+        i = case.g + 2
+        U[1, i, i, i] = 1e10 * U[0, i, i, i]
+        g.set("U_H", 0, U)
+        o.set("U_H", 0, U)
+        g.prepare()
+        g.set_form(form)
+        o.prepare()
+        with pytest.raises(MercuryError, match="dt"):
+            g.step(1, 0)
+        with pytest.raises(RuntimeError):
+            o.step(1, 0)
+    finally:
+        g.close()
+
