@@ -177,3 +177,144 @@ def derive(u: MercuryUnits) -> MercuryPhysics:
         raise ValueError(f"unknown eta profile {u.mercury_eta_profile!r}")
     P.resist_profile = 1.0 if u.mercury_eta_profile == "exp" else 0.0
     return P
+
+
+# ---------------------------------------------------------------------------
+# the reference's setup files (Param, src/0_basic/1_MPCNS_Parameter.cpp:4-434)
+# ---------------------------------------------------------------------------
+_SEP = set("=\r\n\t#$,;\" ()")
+_TYPES = ("string", "int", "double", "bool")
+
+
+def _words(line: str):
+    """Split on the reference's separators: any of = \\r \\n \\t # $ , ; " ( ) and space."""
+    out, cur = [], []
+    for ch in line:
+        if ch in _SEP:
+            if cur:
+                out.append("".join(cur))
+                cur = []
+        else:
+            cur.append(ch)
+    if cur:
+        out.append("".join(cur))
+    return out
+
+
+def _value(kind: str, text: str):
+    if kind == "int":
+        return int(text)
+    if kind == "double":
+        return float(text)
+    if kind == "bool":
+        return text != "0"
+    return text
+
+
+def read_setup(setup_dir) -> Dict[str, object]:
+    """All parameters of a CASE setup directory: the files listed in ``filenames``
+    (up to ###END###), read in order.  Lines are ``type name value`` (string,
+    int, double, bool; a bool is true unless its value is "0"), LIST blocks
+    (name and count, (int) or (double), the names, the values, the closing LIST
+    line) become dicts, IF/ELSEIF/ELSE/ENDIF blocks keep the branch whose
+    condition (type, an earlier parameter, a value) holds, END_OF_FILE ends a
+    file, and lines starting with // or # are comments.  A later definition
+    replaces an earlier one."""
+    import os
+    params: Dict[str, object] = {}
+    with open(os.path.join(setup_dir, "filenames")) as f:
+        names = []
+        for tok in f.read().split():
+            if tok == "###END###":
+                break
+            names.append(tok)
+    for name in names:
+        path = os.path.join(setup_dir, name)
+        with open(path) as f:
+            lines = f.read().split("\n")
+        n = 0
+        # stack of IF states: [taking this branch, some branch already taken]
+        ifs = []
+
+        def active():
+            return all(t for t, _ in ifs)
+
+        def cond(ws):
+            kind, key, val = ws[1], ws[2], ws[3]
+            if key not in params:
+                raise KeyError(f"{path}: IF on undefined parameter {key}")
+            return params[key] == _value(kind, val)
+
+        while n < len(lines):
+            line = lines[n]
+            n += 1
+            if not any(ch not in _SEP for ch in line) or line.startswith("//") or line.startswith("#"):
+                continue
+            ws = _words(line)
+            kw = ws[0]
+            if kw == "END_OF_FILE":
+                break
+            if kw == "IF":
+                take = active() and cond(ws)
+                ifs.append([take, take])
+                continue
+            if kw == "ELSEIF":
+                taken = ifs[-1][1]
+                ifs[-1][0] = (not taken) and all(t for t, _ in ifs[:-1]) and cond(ws)
+                ifs[-1][1] = taken or ifs[-1][0]
+                continue
+            if kw == "ELSE":
+                ifs[-1][0] = not ifs[-1][1] and all(t for t, _ in ifs[:-1])
+                ifs[-1][1] = True
+                continue
+            if kw == "ENDIF":
+                ifs.pop()
+                continue
+            if kw == "LIST":
+                head = _words(lines[n])
+                kind = _words(lines[n + 1])[0]
+                keys = _words(lines[n + 2])
+                vals = _words(lines[n + 3])
+                n += 5  # the closing LIST line too
+                count = int(head[1])
+                if kind not in ("int", "double") or len(keys) < count or len(vals) < count:
+                    raise ValueError(f"{path}: malformed LIST {head[0]}")
+                if active():
+                    params[head[0]] = {keys[t]: _value(kind, vals[t]) for t in range(count)}
+                continue
+            if kw not in _TYPES:
+                raise ValueError(f"{path}: unknown keyword {kw}")
+            if active():
+                params[ws[1]] = _value(kw, ws[2])
+    return params
+
+
+def units_from_setup(params: Dict[str, object]) -> MercuryUnits:
+    """MercuryUnits from the keys the Mercury build reads (Param::GetX aborts on a
+    missing key: so does this, with the key's name)."""
+    def get(key):
+        if key not in params:
+            raise KeyError(f"setup key {key} is missing")
+        return params[key]
+
+    u = MercuryUnits()
+    u.constant = dict(get("constant"))
+    u.REF = dict(get("REF"))
+    u.INITIAL = dict(get("INITIAL"))
+    u.mole_mass1, u.mole_mass2 = get("mole_mass1"), get("mole_mass2")
+    u.CFL = get("CFL")
+    u.r_min, u.r_max = get("r_min"), get("r_max")
+    u.arti_eta_max, u.J_range_start, u.J_range_on = get("arti_eta_max"), get("J_range_start"), get("J_range_on")
+    u.is_local_artificial_resistivity = bool(get("is_local_artificial_resistivity"))
+    u.local_arti_eta_max = get("local_arti_eta_max")
+    u.local_arti_center = (get("local_arti_x0"), get("local_arti_y0"), get("local_arti_z0"))
+    u.local_arti_r_decay, u.local_arti_r_cutoff = get("local_arti_r_decay"), get("local_arti_r_cutoff")
+    u.is_Ambipolar_Efield = bool(get("is_Ambipolar_Efield"))
+    u.is_Mercury_resistance = bool(get("is_Mercury_resistance"))
+    u.eta_max_mercury = get("eta_max_mercury")
+    u.mercury_eta_r_inner, u.mercury_eta_r_outer = get("mercury_eta_r_inner"), get("mercury_eta_r_outer")
+    u.mercury_eta_width = get("mercury_eta_width")
+    u.implicit_resistive_rtol = get("implicit_resistive_ksp_rtol")
+    u.implicit_resistive_atol = get("implicit_resistive_ksp_atol")
+    u.implicit_resistive_max_it = get("implicit_resistive_ksp_max_it")
+    return u

@@ -1,0 +1,66 @@
+"""The reference's setup files (Param, src/0_basic/1_MPCNS_Parameter.cpp:4-434)
+read by params.read_setup, and the Mercury keys mapped onto the physics (CPU)."""
+import math
+import pathlib
+
+import pytest
+
+from mpcns_mercury_amd import params
+
+SETUP = pathlib.Path(__file__).parent / "golden" / "mercury_setup"
+
+
+def test_mercury_example_setup():
+    p = params.read_setup(SETUP)
+    # typed scalars, as written in Z4_Mercury/9999setup
+    assert p["CFL"] == 0.8 and p["ngg"] == 4 and p["dimension"] == 3 and p["Read_Method"] == 2
+    assert p["continue_calc"] is True and p["output_dec_jedge"] is False
+    assert p["eta_max_mercury"] == 1.25e7 and p["implicit_resistive_ksp_max_it"] == 200
+    # LIST blocks
+    assert p["constant"] == {"gamma": 1.6667, "NA": 6.02e23, "R_uni": 8.314, "mu_mag": 1.256637061e-6, "q_e": 1.602e-19}
+    assert p["Boundary_Priority"] == {"Pole": 0, "Outflow": 4, "Absorbing": 2, "Solid_Surface": 3, "Farfield": 1}
+    assert p["INITIAL"]["Bx"] == -1.52e-8 and p["Dipole1"]["mz"] == -2.83e19
+    assert p["scale"] == 1.0 and len(p) == 48
+
+
+def test_mercury_keys_reach_the_physics():
+    u = params.units_from_setup(params.read_setup(SETUP))
+    P = params.derive(u)
+    assert P.cfl == 0.8 and P.gamma == 1.6667
+    assert P.ambipolar_enabled == 1.0 and P.local_enabled == 1.0
+    assert tuple(P.local_center) == (1.53, 0.0, 0.25) and P.local_r_decay == 0.05
+    assert (P.taper_rmin, P.taper_rmax) == (1.1, 1.2)
+    assert (P.J_start, P.J_on) == (800.0, 3000.0)
+    # IMF of INITIAL in units of B_ref = 1 nT
+    assert math.isclose(P.B_imf[0], -15.2, rel_tol=1e-12)
+    assert u.is_Mercury_resistance and u.implicit_resistive_rtol == 5.0e-3
+
+
+def test_parser_rules(tmp_path):
+    (tmp_path / "filenames").write_text("a.txt\nb.txt\n###END###\nnot_read.txt\n")
+    (tmp_path / "a.txt").write_text(
+        "// a comment\n# another\n\n"
+        "int\tmode = 2\n"
+        "string name \"shell\"\n"
+        "bool on 0\nbool off2 1\n"
+        "IF int mode 1\n double x 1.0\nELSEIF int mode 2\n double x 2.0\nELSE\n double x 3.0\nENDIF\n"
+        "IF bool on 1\n double y 5.0\nELSE\n double y 6.0\nENDIF\n"
+        "LIST====\nL 2\n(int)\nu v\n7 8\nLIST====\n"
+        "END_OF_FILE\n"
+        "double after 1.0\n")
+    (tmp_path / "b.txt").write_text("double x 9.5\n")
+    p = params.read_setup(tmp_path)
+    assert p["mode"] == 2 and p["name"] == "shell" and p["on"] is False and p["off2"] is True
+    assert p["y"] == 6.0 and p["L"] == {"u": 7, "v": 8}
+    assert p["x"] == 9.5  # the ELSEIF branch gave 2.0; b.txt, read later, replaced it
+    assert "after" not in p
+    (tmp_path / "b.txt").write_text("float z 1\n")
+    with pytest.raises(ValueError, match="unknown keyword"):
+        params.read_setup(tmp_path)
+
+
+def test_missing_keys_are_named():
+    p = params.read_setup(SETUP)
+    del p["r_max"]
+    with pytest.raises(KeyError, match="r_max"):
+        params.units_from_setup(p)
