@@ -624,25 +624,32 @@ __global__ void k_update_all(const MercRange R, const MercBlk *__restrict__ B, d
     const double dt = *dtp;
     if (p.i < bk.m[0] && p.j < bk.m[1] && p.k < bk.m[2])
         for (int m = 0; m < 5; ++m) {
-            F[SL_UH + m][q] += dt * (double)st<MIX>(F, G, SL_RH + m, q);
-            F[SL_UN + m][q] += dt * (double)st<MIX>(F, G, SL_RN + m, q);
+            // the fluid RHS is read here only, RHS_B written here and only downloaded:
+            // non-temporal, so they do not displace reused data from the caches
+            if constexpr (MIX) {
+                F[SL_UH + m][q] += dt * (double)__builtin_nontemporal_load(&G[SL_RH + m][q]);
+                F[SL_UN + m][q] += dt * (double)__builtin_nontemporal_load(&G[SL_RN + m][q]);
+            } else {
+                F[SL_UH + m][q] += dt * __builtin_nontemporal_load(&F[SL_RH + m][q]);
+                F[SL_UN + m][q] += dt * __builtin_nontemporal_load(&F[SL_RN + m][q]);
+            }
         }
     if (p.j < bk.m[1] && p.k < bk.m[2]) {
         const double c = (F[SL_E + 1][q] - F[SL_E + 1][q + dk]) + (F[SL_E + 2][q + dj] - F[SL_E + 2][q]);
         const double r = -1.0 * c;
-        F[SL_RHSB + 0][q] = r;
+        __builtin_nontemporal_store(r, &F[SL_RHSB + 0][q]);
         F[SL_B + 0][q] += dt * r;
     }
     if (p.i < bk.m[0] && p.k < bk.m[2]) {
         const double c = (F[SL_E + 0][q + dk] - F[SL_E + 0][q]) + (F[SL_E + 2][q] - F[SL_E + 2][q + di]);
         const double r = -1.0 * c;
-        F[SL_RHSB + 1][q] = r;
+        __builtin_nontemporal_store(r, &F[SL_RHSB + 1][q]);
         F[SL_B + 1][q] += dt * r;
     }
     if (p.i < bk.m[0] && p.j < bk.m[1]) {
         const double c = (F[SL_E + 0][q] - F[SL_E + 0][q + dj]) + (F[SL_E + 1][q + di] - F[SL_E + 1][q]);
         const double r = -1.0 * c;
-        F[SL_RHSB + 2][q] = r;
+        __builtin_nontemporal_store(r, &F[SL_RHSB + 2][q]);
         F[SL_B + 2][q] += dt * r;
     }
 }
