@@ -88,6 +88,22 @@ def test_tecplot_of_a_split_case_names_its_zones(tmp_path):
         g.close()
 
 
+def test_tecplot_errors(tmp_path):
+    """An unwritable path is an error through the ABI; a block-name list of the wrong
+    length is refused before the call."""
+    from mpcns_mercury_amd.hip import MercuryError
+    c = cases.make_periodic_cube(n=(6, 5, 4), g=4)
+    g = Mercury(c)
+    try:
+        g.prepare()
+        with pytest.raises(MercuryError, match="cannot open"):
+            g.write_tecplot(tmp_path / "no_such_dir" / "f.plt")
+        with pytest.raises(ValueError):
+            g.write_tecplot(tmp_path / "f.plt", block_names=["a", "b"])
+    finally:
+        g.close()
+
+
 def test_tecplot_linear_field_closed_form(tmp_path):
     n, h, g0 = (6, 5, 4), (0.1, 0.2, 0.15), 4
     origin = (1.0, -0.3, 0.2)
