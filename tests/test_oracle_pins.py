@@ -1,8 +1,8 @@
 """Pins of the CPU oracle (CPU only).
 
 The reference's own tests for this path hold invariants, not numeric vectors
-(SURVEY of Z0_NULL/test: DEC chain d1.d0 = 0 and d2.d1 = 0, halo NaN-overwrite
-with unique integer codes, manufactured J cases).  These tests restate them and
+(Z0_NULL/test: DEC chain d1.d0 = 0 and d2.d1 = 0, halo NaN-overwrite with
+unique integer codes, manufactured J cases).  These tests restate them and
 add closed forms chosen so that a plausible mistake in the oracle (a dropped
 term, a wrong sign or index, a transposed operand) fails one of them.
 """
