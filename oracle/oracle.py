@@ -14,7 +14,9 @@ import numpy as np
 from mpcns_mercury_amd.params import MercuryPhysics
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle.so")
+# MERCURY_ORACLE_LIB names another build of the same source (the planted-mistake
+# builds of tests/test_oracle_mutants.py); the default is the in-tree build.
+_LIB = os.environ.get("MERCURY_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 _NCOMP = {"U_H": 5, "U_Na": 5, "PV_H": 5, "PV_Na": 5, "U_plus": 3, "B_cell": 3, "Bind_cell": 3, "J_cell": 3,
@@ -24,6 +26,8 @@ _NCOMP = {"U_H": 5, "U_Na": 5, "PV_H": 5, "PV_Na": 5, "U_plus": 3, "B_cell": 3, 
 
 
 def build():
+    if os.environ.get("MERCURY_ORACLE_LIB"):
+        return
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 

@@ -176,9 +176,10 @@ def test_artificial_resistivity_smoothstep(O):
     P.arti_eta_max, P.J_start, P.J_on = 0.3, 1.0, 3.0
     o = O.Oracle(c)
     shape = c.box(0)
-    # |J| / |dr| per edge cycles through the thresholds and the smoothstep's fixed points
-    jm = np.array([0.5, 1.0, 2.0, 3.0, 5.0])
-    eta = np.array([0.0, 0.0, 0.15, 0.3, 0.3])
+    # |J| / |dr| per edge cycles through the thresholds, the smoothstep's fixed points
+    # and q = 1/4, 3/4, where q^2 (3 - 2q) = 5/32, 27/32 differs from a linear ramp
+    jm = np.array([0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 5.0])
+    eta = 0.3 * np.array([0.0, 0.0, 5.0 / 32.0, 0.5, 27.0 / 32.0, 1.0, 1.0])
     rng = np.random.default_rng(5)
     picks = {}
     for ax, nm in enumerate(EDGES):
