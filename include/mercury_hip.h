@@ -123,13 +123,20 @@ int mercury_download(MercuryContext *ctx, const char *name, int block, int comp,
 int mercury_init_bface(MercuryContext *ctx, const double *Bu, const double *const *A);
 
 /* Replaces the constructor tail Sync("Badd"), Sync("Bface") (MercurySolver.cpp:141-145)
- * and the UpdateDerivedFields_ call at the head of Advance (MercurySolver_Driver.cpp:6-19). */
+ * and the UpdateDerivedFields_ call at the head of Advance (MercurySolver_Driver.cpp:6-19).
+ * Also clears a stop of the dt abort rule (below). */
 int mercury_prepare(MercuryContext *ctx);
 
 /* Replaces the while(!if_stop) StepOnce() loop of MercurySolver::Advance
  * (Z4_Mercury/src/MercurySolver_Driver.cpp:6-61) for nsteps steps; dt_out (nsteps
- * doubles, may be NULL) receives each step's dt.  Fails on the reference's dt
- * abort rule (MercurySolver_Timestep.cpp:129-140). */
+ * doubles, may be NULL) receives each step's dt.
+ * The reference's dt abort rule (MercurySolver_Timestep.cpp:129-140: dt not
+ * finite or below 1e-7 ends the run before the update) is applied on the device
+ * at the start of every step, in mercury_step, mercury_graph_launch and
+ * mercury_ensemble_launch alike: the refused step and every later one leave the
+ * state untouched, and the call returns -1 with a message naming the refused
+ * step (counted since mercury_prepare).  The context refuses further steps until
+ * mercury_prepare is called again. */
 int mercury_step(MercuryContext *ctx, int nsteps, int integrator, double *dt_out);
 
 /* Individual operators (one reference function each), for parity tests and
@@ -168,16 +175,13 @@ int mercury_resist_stats(MercuryContext *ctx, int *n_unknowns, int *iterations, 
 
 /* One explicit step captured as a HIP graph and replayed (no reference
  * counterpart: the reference loops on the host).  Not available with the
- * implicit resistive correction (its iteration count is data dependent). */
+ * implicit resistive correction (its iteration count is data dependent).
+ * mercury_graph_launch replays nsteps times, waits, and applies the dt abort
+ * rule as mercury_step does.  mercury_set_form and mercury_autotune drop the
+ * captured graph. */
 int mercury_graph_capture(MercuryContext *ctx, int integrator);
 int mercury_graph_launch(MercuryContext *ctx, int nsteps);
 
-/* Ensemble of independent runs (no reference counterpart: the reference runs
- * one case per MPI job).  One HIP graph forks into every context's stream,
- * captures one step of each (integrator as in mercury_graph_capture) and joins
- * back; mercury_ensemble_launch replays it nsteps times and waits.  Contexts
- * keep their own physics, dt and state and must outlive the ensemble; contexts
- * with the implicit resistive correction are refused. */
 /* Restart of device-resident fields in the reference's MPCNSRST version-1 file
  * (IOModule::WriteRestartBinFile, src/6_io/IOModule_Bin.cpp:168-228): the
  * reference's whitelist U_H, U_Na, B_xi, B_eta, B_zeta (Z4_Mercury/src/MercurySolver.cpp:31-44),
@@ -250,6 +254,15 @@ int mercury_case_topology(const MercuryCase *c, int *bc, int *nbr, int *transfor
 int mercury_case_interfaces(const MercuryCase *c, int *n, int *records);
 int mercury_case_physical(const MercuryCase *c, int *n, int *records);
 
+/* Ensemble of independent runs (no reference counterpart: the reference runs
+ * one case per MPI job).  One HIP graph forks into every context's stream,
+ * captures one step of each (integrator as in mercury_graph_capture) and joins
+ * back; mercury_ensemble_launch replays it nsteps times, waits, and reports the
+ * first member stopped by the dt abort rule.  Contexts keep their own physics,
+ * dt and state and must outlive the ensemble; contexts with the implicit
+ * resistive correction are refused.  A member whose form or launch shape
+ * changed after the capture (mercury_set_form, mercury_autotune) makes
+ * mercury_ensemble_launch fail before anything is launched. */
 typedef struct MercuryEnsemble MercuryEnsemble;
 int mercury_ensemble_create(MercuryContext *const *ctxs, int n, int integrator, MercuryEnsemble **out);
 int mercury_ensemble_launch(MercuryEnsemble *ens, int nsteps);

@@ -19,5 +19,5 @@ void launch_to_double(hipStream_t st, const float *src, double *dst, long long n
 void launch_bcell_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);
 void launch_jcell_t(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);
 void launch_to_float(hipStream_t st, const double *src, float *dst, long long n);
-void launch_dt_commit(hipStream_t st, unsigned long long *dt, unsigned long long *dt_next);
+void launch_dt_commit(hipStream_t st, unsigned long long *dt, const unsigned long long *dt_next, unsigned long long *status);
 }  // namespace merc

@@ -622,6 +622,7 @@ __global__ void k_update_all(const MercRange R, const MercBlk *__restrict__ B, d
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
     const double dt = *dtp;
+    if (dt == 0.0) return;  // the run is stopped (k_dt_commit)
     if (p.i < bk.m[0] && p.j < bk.m[1] && p.k < bk.m[2])
         for (int m = 0; m < 5; ++m) {
             // the fluid RHS is read here only, RHS_B written here and only downloaded:
@@ -738,8 +739,28 @@ __global__ void k_to_float(const double *__restrict__ src, float *__restrict__ d
     if (t < n) dst[t] = (float)src[t];
 }
 
-__global__ void k_dt_commit(unsigned long long *dt, unsigned long long *dt_next) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *dt = *dt_next;
+// Commit the reduced dt of the step about to run, with Compute_Timestep's abort
+// rule (MercurySolver_Timestep.cpp:129-140): a dt that is not finite or is below
+// 1e-7 stops the run.  The stop is sticky in status[0] (1 = dt underflow, with
+// the offending bits in status[2]); status[1] counts the steps committed since
+// the status was cleared.  A stopped context commits dt = 0, and the update
+// passes skip their writes when dt is 0 (no valid step has dt = 0), so the
+// state stays that of the last completed step, as the reference exits before
+// its update.
+__global__ void k_dt_commit(unsigned long long *dt, const unsigned long long *dt_next, unsigned long long *status) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const unsigned long long bits = *dt_next;
+    const double d = __longlong_as_double((long long)bits);
+    if (status[0] != 0ULL) {
+        *dt = 0ULL;
+    } else if (!isfinite(d) || d < 1e-7) {
+        status[0] = 1ULL;
+        status[2] = bits;
+        *dt = 0ULL;
+    } else {
+        *dt = bits;
+        status[1] += 1ULL;
+    }
 }
 __global__ void k_dt_reset(unsigned long long *dt_next) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *dt_next = (unsigned long long)__double_as_longlong(1e100);
@@ -794,8 +815,8 @@ void launch_jcell_t(hipStream_t st, const RangeH &R, const MercBlk *B, double *c
 void launch_to_float(hipStream_t st, const double *src, float *dst, long long n) {
     if (n > 0) hipLaunchKernelGGL(k_to_float, dim3(nblk(n, 256)), dim3(256), 0, st, src, dst, n);
 }
-void launch_dt_commit(hipStream_t st, unsigned long long *dt, unsigned long long *dt_next) {
-    hipLaunchKernelGGL(k_dt_commit, dim3(1), dim3(64), 0, st, dt, dt_next);
+void launch_dt_commit(hipStream_t st, unsigned long long *dt, const unsigned long long *dt_next, unsigned long long *status) {
+    hipLaunchKernelGGL(k_dt_commit, dim3(1), dim3(64), 0, st, dt, dt_next, status);
 }
 
 }  // namespace merc

@@ -587,6 +587,26 @@ struct AliasClasses {
     }
 };
 
+// Ends a stream capture that an exception interrupted: the capture is ended on
+// every stream that is still capturing (the origin and any stream forked into
+// it) and the partial graph is dropped, so the streams are usable again.
+struct CaptureGuard {
+    std::vector<hipStream_t> streams;
+    bool active = false;
+    explicit CaptureGuard(std::vector<hipStream_t> s) : streams(std::move(s)) {}
+    ~CaptureGuard() {
+        if (!active) return;
+        for (hipStream_t s : streams) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs == hipStreamCaptureStatusNone) continue;
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();
+    }
+};
+
 // ---------------------------------------------------------------------------
 // MercurySolver
 // ---------------------------------------------------------------------------
@@ -612,12 +632,19 @@ public:
     bool overlap = true;
     unsigned long long *d_dt = nullptr;       // dt word (double bits)
     unsigned long long *d_dt_next = nullptr;  // next step's dt, reduced at the end of the derived fields
+    // run status written by k_dt_commit: [0] stop code (1 = dt underflow, sticky),
+    // [1] steps committed since the status was cleared, [2] bits of the refused dt
+    unsigned long long *d_status = nullptr;
+    // bumped whenever something a captured graph has baked in changes (form,
+    // launch shape): graphs captured at an older generation must not replay
+    unsigned long long generation = 0;
     int form = 1;                             // 0: one kernel per reference function, 1: fused FP64, 2: mixed
     float *farena = nullptr;                  // FP32 copies of the static arrays (mixed form)
     float **fptr = nullptr;                   // device table indexed like the FP64 slots (null where absent)
     std::vector<int> fslots;                  // FP32 static copies
     std::vector<int> islots;                  // FP32 derived intermediates of the mixed form
-    unsigned long long *d_red = nullptr;      // reduction scratch (2 words)
+    unsigned long long *d_red = nullptr;      // reduction result (2 words)
+    double *d_part = nullptr;                 // per-workgroup partial maxima of the div B diagnostic
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_integrator = -1;
@@ -675,7 +702,10 @@ public:
         F.reset(new Field(nb, dims, g, bc, nbr, trans));
         HIPCHK(hipMalloc(&d_dt, sizeof(unsigned long long)));
         HIPCHK(hipMalloc(&d_dt_next, sizeof(unsigned long long)));
+        HIPCHK(hipMalloc(&d_status, 4 * sizeof(unsigned long long)));
+        HIPCHK(hipMemset(d_status, 0, 4 * sizeof(unsigned long long)));
         HIPCHK(hipMalloc(&d_red, 2 * sizeof(unsigned long long)));
+        HIPCHK(hipMalloc(&d_part, 2 * sizeof(double) * divb_partials()));
         // static geometry per block
         long long noff = 0;
         for (int b = 0; b < nb; ++b) {
@@ -748,9 +778,11 @@ public:
         if (graph) (void)hipGraphDestroy(graph);
         if (d_dt) (void)hipFree(d_dt);
         if (d_dt_next) (void)hipFree(d_dt_next);
+        if (d_status) (void)hipFree(d_status);
         if (farena) (void)hipFree(farena);
         if (fptr) (void)hipFree(fptr);
         if (d_red) (void)hipFree(d_red);
+        if (d_part) (void)hipFree(d_part);
         F.reset();
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
@@ -797,6 +829,7 @@ public:
 
     void set_march(int L) {
         march = L;
+        ++generation;
         std::vector<int> lo(3 * F->nb(), 0), n(3 * F->nb());
         for (int b = 0; b < F->nb(); ++b) {
             n[3 * b] = (F->blk[b].m[0] + L - 1) / L;
@@ -1323,11 +1356,13 @@ public:
     double res_dt = 0.0;
 
     void enqueue_step(int integrator) {
-        if (form >= 1) launch_dt_commit(st, d_dt, d_dt_next);
-        else launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt);
+        // the fused forms reduced this step's dt at the end of the previous one;
+        // the plain form reduces it here.  The commit applies the abort rule.
+        if (form == 0) launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt_next);
+        launch_dt_commit(st, d_dt, d_dt_next, d_status);
         if (resist) {
             res_dt = read_dt();
-            check_dt(res_dt);
+            check_status();
         }
         res_now = resist && integrator == 0;
         if (form >= 1) {
@@ -1367,21 +1402,35 @@ public:
         return dt;
     }
 
-    void check_dt(double dt) {
-        // Compute_Timestep abort rule (MercurySolver_Timestep.cpp:129-140)
-        if (!std::isfinite(dt) || dt < 1e-7) throw std::runtime_error("dt too small/NaN (dt underflow)");
+    // Compute_Timestep abort rule (MercurySolver_Timestep.cpp:129-140), applied on
+    // the device by k_dt_commit: after a stop the state is that of the last
+    // completed step and every further step is refused until prepare().
+    void check_status() {
+        unsigned long long h[4];
+        HIPCHK(hipMemcpyAsync(h, d_status, sizeof(h), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (h[0] == 0ULL) return;
+        double bad;
+        std::memcpy(&bad, &h[2], sizeof(bad));
+        char msg[256];
+        std::snprintf(msg, sizeof(msg),
+                      "dt too small/NaN (dt underflow): step %llu since prepare refused dt = %.17g; the state is that of "
+                      "step %llu and the context refuses further steps until prepare",
+                      h[1] + 1ULL, bad, h[1]);
+        throw std::runtime_error(msg);
     }
+    void clear_status() { HIPCHK(hipMemsetAsync(d_status, 0, 4 * sizeof(unsigned long long), st)); }
 
     void step(int n, int integrator, double *dt_out) {
         for (int s = 0; s < n; ++s) {
             enqueue_step(integrator);
             if (dt_out) {
                 dt_out[s] = read_dt();
-                check_dt(dt_out[s]);
+                check_status();
             }
         }
         HIPCHK(hipGetLastError());
-        if (!dt_out && n > 0) check_dt(read_dt());
+        if (n > 0) check_status();
     }
 
     void capture(int integrator) {
@@ -1390,15 +1439,29 @@ public:
         if (graph) (void)hipGraphDestroy(graph);
         gexec = nullptr;
         graph = nullptr;
-        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        enqueue_step(integrator);
-        HIPCHK(hipStreamEndCapture(st, &graph));
+        {
+            CaptureGuard guard({st, st2});
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            guard.active = true;
+            enqueue_step(integrator);
+            guard.active = false;
+            HIPCHK(hipStreamEndCapture(st, &graph));
+        }
         HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
         graph_integrator = integrator;
+        graph_generation = generation;
+    }
+    unsigned long long graph_generation = 0;
+    // n replays of the captured step, then the abort rule's status (one
+    // synchronisation per call)
+    void graph_replay(int n) {
+        if (!gexec) throw std::runtime_error("no captured step graph");
+        if (graph_generation != generation) throw std::runtime_error("the captured step graph is stale (form or launch shape changed)");
+        for (int s = 0; s < n; ++s) HIPCHK(hipGraphLaunch(gexec, st));
     }
     void graph_launch(int n) {
-        if (!gexec) throw std::runtime_error("no captured step graph");
-        for (int s = 0; s < n; ++s) HIPCHK(hipGraphLaunch(gexec, st));
+        graph_replay(n);
+        if (n > 0) check_status();
     }
 
     void init_bface(const double Bu[3], const double *const *A) {
@@ -1438,6 +1501,7 @@ public:
     // with its own passes, so a state read from a restart continues exactly as
     // the run that wrote it
     void prepare() {
+        clear_status();
         run_merged(MG_PREPARE_PAIR);
         derived();
         refresh_float_statics();
@@ -1449,6 +1513,7 @@ public:
     void set_form(int f) {
         if (f < 0 || f > 2) throw std::runtime_error("form must be 0 (per-function FP64), 1 (fused FP64) or 2 (mixed)");
         form = f;
+        ++generation;
         if (f >= 1) {
             refresh_float_statics();
             derived_fused(true);
@@ -1522,8 +1587,7 @@ public:
     }
 
     void divb(double *norm, double *maxabs) {
-        HIPCHK(hipMemsetAsync(d_red, 0, 2 * sizeof(unsigned long long), st));
-        launch_divb(st, R_node.h, Bd(), Fd(), d_red);
+        launch_divb(st, R_node.h, Bd(), Fd(), d_part, d_red, 1);
         unsigned long long h[2];
         HIPCHK(hipMemcpyAsync(h, d_red, sizeof(h), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -1539,11 +1603,11 @@ public:
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        if (op == "step_graph") graph_launch(1);
+        if (op == "step_graph") graph_replay(1);
         else run_op(op, 0.0);
         HIPCHK(hipEventRecord(e0, st));
         for (int r = 0; r < reps; ++r) {
-            if (op == "step_graph") graph_launch(1);
+            if (op == "step_graph") graph_replay(1);
             else run_op(op, 0.0);
         }
         HIPCHK(hipEventRecord(e1, st));
@@ -2112,6 +2176,7 @@ int mercury_set_form(MercuryContext *ctx, int form) {
 // ---------------------------------------------------------------------------
 struct MercuryEnsemble {
     std::vector<MercuryContext *> ctx;
+    std::vector<unsigned long long> gen;  // each member's generation at capture
     hipStream_t root = nullptr;
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;
@@ -2143,16 +2208,27 @@ int mercury_ensemble_create(MercuryContext *const *ctxs, int n, int integrator, 
         e->join.assign(n, nullptr);
         for (int i = 0; i < n; ++i) HIPCHK(hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming));
         // fork from the root stream into every context's stream, join back
-        HIPCHK(hipStreamBeginCapture(e->root, hipStreamCaptureModeThreadLocal));
-        HIPCHK(hipEventRecord(e->fork, e->root));
+        std::vector<hipStream_t> all{e->root};
         for (int i = 0; i < n; ++i) {
-            hipStream_t cs = e->ctx[i]->s->st;
-            HIPCHK(hipStreamWaitEvent(cs, e->fork, 0));
-            e->ctx[i]->s->enqueue_step(integrator);
-            HIPCHK(hipEventRecord(e->join[i], cs));
-            HIPCHK(hipStreamWaitEvent(e->root, e->join[i], 0));
+            all.push_back(e->ctx[i]->s->st);
+            all.push_back(e->ctx[i]->s->st2);
+            e->gen.push_back(e->ctx[i]->s->generation);
         }
-        HIPCHK(hipStreamEndCapture(e->root, &e->graph));
+        {
+            merc::CaptureGuard guard(all);
+            HIPCHK(hipStreamBeginCapture(e->root, hipStreamCaptureModeThreadLocal));
+            guard.active = true;
+            HIPCHK(hipEventRecord(e->fork, e->root));
+            for (int i = 0; i < n; ++i) {
+                hipStream_t cs = e->ctx[i]->s->st;
+                HIPCHK(hipStreamWaitEvent(cs, e->fork, 0));
+                e->ctx[i]->s->enqueue_step(integrator);
+                HIPCHK(hipEventRecord(e->join[i], cs));
+                HIPCHK(hipStreamWaitEvent(e->root, e->join[i], 0));
+            }
+            guard.active = false;
+            HIPCHK(hipStreamEndCapture(e->root, &e->graph));
+        }
         HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
         *out = e.release();
     })
@@ -2164,8 +2240,21 @@ int mercury_ensemble_launch(MercuryEnsemble *e, int nsteps) {
         return -1;
     }
     ABI_TRY({
+        // a member whose form or launch shape changed since the capture would
+        // replay kernels with arguments that no longer hold: refuse before launching
+        for (size_t i = 0; i < e->ctx.size(); ++i)
+            if (e->ctx[i]->s->generation != e->gen[i])
+                throw std::runtime_error("ensemble member " + std::to_string(i) +
+                                         " changed form or launch shape after the ensemble was captured; create the ensemble again");
         for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(e->exec, e->root));
         HIPCHK(hipStreamSynchronize(e->root));
+        for (size_t i = 0; i < e->ctx.size(); ++i) {
+            try {
+                e->ctx[i]->s->check_status();
+            } catch (const std::exception &x) {
+                throw std::runtime_error("ensemble member " + std::to_string(i) + ": " + x.what());
+            }
+        }
     })
 }
 

@@ -30,7 +30,10 @@ void launch_rk2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const
 void launch_dec_d0(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int s_node, int s_edge);
 void launch_dec_d1(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int s_edge, int s_face, double mult);
 void launch_dec_d2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int s_face, int s_cell);
-void launch_divb(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, unsigned long long *out);
+// d2 B diagnostic: partial holds 2 * divb_partials() doubles, out 2 words (max|d2 B|, max|B_face| bits)
+int divb_partials();
+void launch_divb(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, double *partial, unsigned long long *out,
+                 int write_field);
 
 void launch_alias_reduce(hipStream_t st, const int *cls_start, const int *member, const int *msign, const int *owner, const int *osign,
                          const int *slot_of, long long ncls, double *const *F, int slot0);

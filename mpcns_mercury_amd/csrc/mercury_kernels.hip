@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "mercury_common.h"
 #include "mercury_kernels.h"
 
@@ -526,6 +528,7 @@ __global__ void k_curl_update(const MercRange R, const MercBlk *__restrict__ B, 
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
     const double dt = do_update ? *dtp : 0.0;
+    if (do_update && dt == 0.0) return;  // the run is stopped (k_dt_commit)
     if (p.j < bk.m[1] && p.k < bk.m[2]) {
         const double c = (F[SL_E + 1][q] - F[SL_E + 1][q + dk]) + (F[SL_E + 2][q + dj] - F[SL_E + 2][q]);
         const double r = -1.0 * c;
@@ -552,6 +555,7 @@ __global__ void k_update_u(const MercRange R, const MercBlk *__restrict__ B, dou
     LOCATE(R);
     const long long q = p.q;
     const double dt = *dtp;
+    if (dt == 0.0) return;  // the run is stopped (k_dt_commit)
     for (int m = 0; m < 5; ++m) {
         F[SL_UH + m][q] += dt * F[SL_RH + m][q];
         F[SL_UN + m][q] += dt * F[SL_RN + m][q];
@@ -707,17 +711,28 @@ __global__ void k_dec_d2(const MercRange R, const MercBlk *__restrict__ B, doubl
 }
 
 // calc_divB (MercurySolver_MagneticCell.cpp:151-241) plus max |d2 B| and max |B_face|
-__global__ void k_divb(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, unsigned long long *__restrict__ out) {
+// calc_divB (MercurySolver_MagneticCell.cpp:151-241): d2 B per owned cell
+// (optionally stored as divB = d2 B / Jac) and, for the normalised diagnostic,
+// max|d2 B| and max|B_face| over owned cells / faces.  A capped grid walks the
+// range with a stride; each workgroup reduces its waves in LDS and writes one
+// pair of partial maxima (no atomics); k_divb_final folds the partials.  Maxima
+// are exact, so the result does not depend on the order.
+constexpr int DIVB_MAX_GROUPS = 2048;
+__global__ void __launch_bounds__(256) k_divb(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
+                                              double *__restrict__ partial, int write_field) {
     gdouble *const *__restrict__ F = gslots(F_);
     double dmx = 0.0, bmx = 0.0;
-    Pt p;
-    if (locate(R, B, TID, p)) {
+    const long long total = R.start[R.nb];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+        Pt p;
+        locate(R, B, t, p);
         const MercBlk &bk = B[p.b];
         const long long q = p.q;
         if (p.i < bk.m[0] && p.j < bk.m[1] && p.k < bk.m[2]) {
             const double d = (F[SL_B][q + OI(p)] - F[SL_B][q] + F[SL_B + 1][q + OJ(p)] - F[SL_B + 1][q] + F[SL_B + 2][q + 1] - F[SL_B + 2][q]);
-            F[SL_DIVB][q] = d / F[SL_JAC][q];
-            dmx = fabs(d);
+            if (write_field) F[SL_DIVB][q] = d / F[SL_JAC][q];
+            dmx = dmax(dmx, fabs(d));
         }
         if (p.j < bk.m[1] && p.k < bk.m[2]) bmx = dmax(bmx, fabs(F[SL_B][q]));
         if (p.i < bk.m[0] && p.k < bk.m[2]) bmx = dmax(bmx, fabs(F[SL_B + 1][q]));
@@ -727,9 +742,46 @@ __global__ void k_divb(const MercRange R, const MercBlk *__restrict__ B, double 
         dmx = dmax(dmx, __shfl_xor(dmx, off, 64));
         bmx = dmax(bmx, __shfl_xor(bmx, off, 64));
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(out, (unsigned long long)__double_as_longlong(dmx));
-        atomicMax(out + 1, (unsigned long long)__double_as_longlong(bmx));
+    __shared__ double wd[4], wb[4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        wd[wid] = dmx;
+        wb[wid] = bmx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            dmx = dmax(dmx, wd[w]);
+            bmx = dmax(bmx, wb[w]);
+        }
+        partial[2 * blockIdx.x] = dmx;
+        partial[2 * blockIdx.x + 1] = bmx;
+    }
+}
+__global__ void __launch_bounds__(256) k_divb_final(const double *__restrict__ partial, int ngroups, unsigned long long *__restrict__ out) {
+    double dmx = 0.0, bmx = 0.0;
+    for (int g = threadIdx.x; g < ngroups; g += blockDim.x) {
+        dmx = dmax(dmx, partial[2 * g]);
+        bmx = dmax(bmx, partial[2 * g + 1]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        dmx = dmax(dmx, __shfl_xor(dmx, off, 64));
+        bmx = dmax(bmx, __shfl_xor(bmx, off, 64));
+    }
+    __shared__ double wd[4], wb[4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        wd[wid] = dmx;
+        wb[wid] = bmx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            dmx = dmax(dmx, wd[w]);
+            bmx = dmax(bmx, wb[w]);
+        }
+        out[0] = (unsigned long long)__double_as_longlong(dmx);
+        out[1] = (unsigned long long)__double_as_longlong(bmx);
     }
 }
 
@@ -786,7 +838,14 @@ void launch_rk2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const
 void launch_dec_d0(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int sn, int se) { LAUNCH_R(k_dec_d0, R, B, F, sn, se); }
 void launch_dec_d1(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int se, int sf, double mult) { LAUNCH_R(k_dec_d1, R, B, F, se, sf, mult); }
 void launch_dec_d2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int sf, int sc) { LAUNCH_R(k_dec_d2, R, B, F, sf, sc); }
-void launch_divb(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, unsigned long long *out) { LAUNCH_R(k_divb, R, B, F, out); }
+int divb_partials() { return DIVB_MAX_GROUPS; }
+void launch_divb(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, double *partial, unsigned long long *out,
+                 int write_field) {
+    if (R.total <= 0) return;
+    const int groups = (int)std::min<long long>(DIVB_MAX_GROUPS, (R.total + 255) / 256);
+    hipLaunchKernelGGL(k_divb, dim3(groups), dim3(256), 0, st, R.dev, B, F, partial, write_field);
+    hipLaunchKernelGGL(k_divb_final, dim3(1), dim3(256), 0, st, partial, groups, out);
+}
 
 void launch_alias_reduce(hipStream_t st, const int *cls_start, const int *member, const int *msign, const int *owner, const int *osign,
                          const int *slot_of, long long ncls, double *const *F, int slot0) {

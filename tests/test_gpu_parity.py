@@ -226,28 +226,102 @@ def test_mixed_precision_accuracy_100_steps(pair):
     assert g.divb()[0] <= 1e-12
 
 
+def _runaway(case, g, o=None):
+    """A cell moving at 1e10 drives the CFL time step below 1e-7."""
+    U = g.get("U_H", 0)
+    i = case.g + 2
+    U[1, i, i, i] = 1e10 * U[0, i, i, i]
+    g.set("U_H", 0, U)
+    if o is not None:
+        o.set("U_H", 0, U)
+
+
+@pytest.mark.parametrize("path", ["step", "step_dt_out", "graph"])
 @pytest.mark.parametrize("form", [0, 1, 2])
-def test_dt_underflow_is_an_error(oracle_mod, form):
-    """Compute_Timestep's abort rule (MercurySolver_Timestep.cpp:129-140): a cell
-    moving at 1e10 drives dt below 1e-7, and the step fails through the C ABI
-    with the reason, as the oracle refuses the same step; a healthy state steps."""
+def test_dt_underflow_is_an_error(oracle_mod, form, path):
+    """Compute_Timestep's abort rule (MercurySolver_Timestep.cpp:129-140), applied
+    on the device before the update in every path: the refused step leaves the
+    state of the last completed step, the call fails through the C ABI naming
+    the step, later calls are refused until prepare(), and the oracle refuses
+    the same step."""
     case = small_cube()
-    print(f"[case] {case.name} dt underflow form={form}", flush=True)
+    print(f"[case] {case.name} dt underflow form={form} path={path}", flush=True)
     g = Mercury(case)
     o = oracle_mod.Oracle(case)
     try:
-        U = g.get("U_H", 0)
-        i = case.g + 2
-        U[1, i, i, i] = 1e10 * U[0, i, i, i]
-        g.set("U_H", 0, U)
-        o.set("U_H", 0, U)
+        healthy = {n: g.get(n, 0) for n in STATE}
+        _runaway(case, g, o)
         g.prepare()
         g.set_form(form)
         o.prepare()
-        with pytest.raises(MercuryError, match="dt"):
+        before = {n: g.get(n, 0) for n in STATE}
+        if path == "graph":
+            g.graph_capture(0)
+            with pytest.raises(MercuryError, match="dt underflow.*step 1 "):
+                g.graph_launch(3)
+        else:
+            with pytest.raises(MercuryError, match="dt underflow.*step 1 "):
+                g.step(3, 0, record_dt=(path == "step_dt_out"))
+        for n in STATE:
+            np.testing.assert_array_equal(g.get(n, 0), before[n], err_msg=f"{n} changed by a refused step")
+        with pytest.raises(MercuryError, match="dt underflow"):
             g.step(1, 0)
         with pytest.raises(RuntimeError):
             o.step(1, 0)
+        # a healthy state after prepare() steps again
+        for n in STATE:
+            g.set(n, 0, healthy[n])
+        g.prepare()
+        g.step(2, 0)
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_dt_underflow_in_an_ensemble(form):
+    """An ensemble replay reports the member that hit the abort rule; that
+    member keeps its last completed state while the healthy one advances."""
+    from mpcns_mercury_amd.hip import Ensemble
+    c0, c1 = small_cube(), small_cube(seed=2)
+    print(f"[case] ensemble dt underflow form={form}", flush=True)
+    g0, g1 = Mercury(c0), Mercury(c1)
+    try:
+        _runaway(c1, g1)
+        for g in (g0, g1):
+            g.prepare()
+            g.set_form(form)
+        b0, b1 = g0.get("U_H", 0), g1.get("U_H", 0)
+        ens = Ensemble([g0, g1])
+        with pytest.raises(MercuryError, match="member 1: dt underflow"):
+            ens.launch(2)
+        np.testing.assert_array_equal(g1.get("U_H", 0), b1)
+        assert not np.array_equal(g0.get("U_H", 0), b0)
+        ens.close()
+    finally:
+        g0.close()
+        g1.close()
+
+
+def test_stale_ensemble_is_refused():
+    """A member whose form changes after the ensemble was captured makes the
+    ensemble refuse to launch (checked on the host, nothing replays)."""
+    from mpcns_mercury_amd.hip import Ensemble
+    print("[case] stale ensemble", flush=True)
+    g0, g1 = Mercury(small_cube()), Mercury(small_cube(seed=2))
+    try:
+        for g in (g0, g1):
+            g.prepare()
+        ens = Ensemble([g0, g1])
+        ens.launch(1)
+        g1.set_form(2)
+        with pytest.raises(MercuryError, match="member 1 changed form"):
+            ens.launch(1)
+        ens.close()
+        g0.graph_capture(0)
+        g0.autotune()
+        with pytest.raises(MercuryError, match="no captured step graph|stale"):
+            g0.graph_launch(1)
+    finally:
+        g0.close()
+        g1.close()
 
