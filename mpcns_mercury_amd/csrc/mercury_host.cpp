@@ -1812,29 +1812,37 @@ int mercury_read_restart(MercuryContext *ctx, const char *path, int *step, doubl
             delivered[m.name].assign(F.nb(), 0);
             return true;
         };
-        std::vector<double> comp;
+        auto check = [&](const merc::io::FieldMeta &m, int b, const std::array<int, 6> &box) {
+            const merc::FieldSpec *f = merc::find_field(m.name.c_str());
+            if (box != alloc_box(F.blk[b], f->loc)) throw std::runtime_error("restart: lo/hi mismatch for " + m.name + " block " + std::to_string(b));
+        };
+        // every value is staged on the host first; the device fields change only
+        // after the whole file has been read and validated
+        std::map<std::pair<int, int>, std::vector<double>> staged;  // (slot, block) -> box array
         auto put = [&](const merc::io::FieldMeta &m, int b, const std::array<int, 6> &box, const double *vals) {
             const merc::FieldSpec *f = merc::find_field(m.name.c_str());
             const MercBlk &k = F.blk[b];
-            if (box != alloc_box(k, f->loc)) throw std::runtime_error("restart: lo/hi mismatch for " + m.name + " block " + std::to_string(b));
-            comp.resize(F.vol(b));
             for (int c = 0; c < f->ncomp; ++c) {
-                F.download_block(f->slot + c, b, comp.data());  // keeps the box points outside the allocation
+                std::vector<double> &comp = staged[{f->slot + c, b}];
+                if (comp.empty()) {
+                    comp.resize(F.vol(b));
+                    F.download_block(f->slot + c, b, comp.data());  // keeps the box points outside the allocation
+                }
                 size_t t = (size_t)c;
                 for (int i = box[0]; i < box[3]; ++i)
                     for (int j = box[1]; j < box[4]; ++j)
                         for (int kk = box[2]; kk < box[5]; ++kk, t += f->ncomp)
                             comp[(long long)(i + k.g) * k.sI + (long long)(j + k.g) * k.sJ + (kk + k.g)] = vals[t];
-                F.upload_block(f->slot + c, b, comp.data());
             }
             delivered[m.name][b] = 1;
         };
-        merc::io::read_restart(path, F.nb(), want, put, step, time);
+        merc::io::read_restart(path, F.nb(), want, check, put, step, time);
         if (njedge != 0 && njedge != 3) throw std::runtime_error("restart: incomplete J edge triplet (J_xi, J_eta, J_zeta go together)");
         for (const auto &kv : delivered)
             for (int b = 0; b < F.nb(); ++b)
                 if (!kv.second[b])
                     throw std::runtime_error("restart: block " + std::to_string(b) + " of " + kv.first + " is inactive in the file");
+        for (const auto &kv : staged) F.upload_block(kv.first.first, kv.first.second, kv.second.data());
     })
 }
 

@@ -66,10 +66,13 @@ void write_restart(const std::string &path, int step, double time, int nblock, c
 }
 
 void read_restart(const std::string &path, int nblock, const std::function<bool(const FieldMeta &)> &want,
+                  const std::function<void(const FieldMeta &, int, const std::array<int, 6> &)> &check,
                   const std::function<void(const FieldMeta &, int, const std::array<int, 6> &, const double *)> &put, int *step,
                   double *time) {
-    std::ifstream in(path, std::ios::binary);
+    std::ifstream in(path, std::ios::binary | std::ios::ate);
     if (!in) fail("cannot open " + path);
+    const long long fsize = (long long)in.tellg();
+    in.seekg(0, std::ios::beg);
     char magic[8];
     in.read(magic, 8);
     if (!in || std::memcmp(magic, "MPCNSRST", 8) != 0) fail("not an MPCNSRST file: " + path);
@@ -97,6 +100,11 @@ void read_restart(const std::string &path, int nblock, const std::function<bool(
             const int32_t active = get_i32(in, "an active flag");
             if (!active) continue;
             const long long nval = ni * nj * nk * m.ncomp;
+            // a block's values must fit in what is left of the file, and a field
+            // the caller takes must match its allocation, before anything is
+            // allocated or read
+            if (nval > (fsize - (long long)in.tellg()) / (long long)sizeof(double)) fail("truncated data of " + m.name);
+            if (take) check(m, b, box);
             if (!take) {
                 in.seekg((std::streamoff)(nval * (long long)sizeof(double)), std::ios::cur);
                 if (!in) fail("truncated data of " + m.name);

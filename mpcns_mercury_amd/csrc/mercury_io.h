@@ -39,10 +39,14 @@ struct RestartField {
 void write_restart(const std::string &path, int step, double time, int nblock, const std::vector<RestartField> &fields);
 
 // Reads a restart file whose block count must equal nblock.  For every field,
-// want(meta) decides whether its blocks are delivered; put(meta, block, box,
-// values) receives an active block in file order (skipped fields are stepped
-// over, as the reference does).  Returns step and time through the pointers.
+// want(meta) decides whether its blocks are delivered; check(meta, block, box)
+// validates an active block's box before its values are allocated or read (it
+// throws to refuse); put(meta, block, box, values) receives the block in file
+// order (skipped fields are stepped over, as the reference does).  A block whose
+// values do not fit in the rest of the file is an error.  Returns step and time
+// through the pointers.
 void read_restart(const std::string &path, int nblock, const std::function<bool(const FieldMeta &)> &want,
+                  const std::function<void(const FieldMeta &, int, const std::array<int, 6> &)> &check,
                   const std::function<void(const FieldMeta &, int, const std::array<int, 6> &, const double *)> &put, int *step,
                   double *time);
 

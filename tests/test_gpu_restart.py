@@ -12,6 +12,8 @@
 * the oracle, loaded from the parsed file, continues identically;
 * mismatching files are rejected.
 """
+import struct
+
 import numpy as np
 import pytest
 
@@ -177,3 +179,38 @@ def test_mismatching_restarts_are_rejected(tmp_path):
     finally:
         g.close()
         h.close()
+
+
+def test_truncated_or_oversized_restart_leaves_the_state(tmp_path):
+    """A restart whose data stop early, or whose box claims more values than the
+    file holds, fails before anything reaches the device: every field keeps the
+    values it had, including the fields the file delivered before the fault."""
+    shell = _case("shell")
+    print(f"[case] {shell.name} truncated restart", flush=True)
+    g = Mercury(shell)
+    try:
+        g.prepare()
+        p = tmp_path / "full.bin"
+        g.write_restart(p)
+        g.step(2, 0)
+        before = {n: [g.get(n, ib) for ib in range(shell.nb)] for n in STATE}
+        data = p.read_bytes()
+        q = tmp_path / "cut.bin"
+        q.write_bytes(data[: len(data) - 1000])
+        with pytest.raises(MercuryError, match="truncated"):
+            g.read_restart(q)
+        # the first block box of U_H patched in place: hi[0] grows by 1e6, so the
+        # box claims far more values than the file holds
+        raw = bytearray(data)
+        at = raw.index(b"U_H") + len(b"U_H")
+        box_at = at + 12  # location, ncomp, nghost
+        hi0 = struct.unpack_from("<i", raw, box_at + 12)[0]
+        struct.pack_into("<i", raw, box_at + 12, hi0 + 10**6)
+        q.write_bytes(bytes(raw))
+        with pytest.raises(MercuryError):
+            g.read_restart(q)
+        for n in STATE:
+            for ib in range(shell.nb):
+                np.testing.assert_array_equal(g.get(n, ib), before[n][ib], err_msg=f"{n}[{ib}] changed by a refused restart")
+    finally:
+        g.close()
