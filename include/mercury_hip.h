@@ -111,6 +111,21 @@ int mercury_field_info(const char *name, int *slot, int *ncomp, int *location);
 long long mercury_block_volume(MercuryContext *ctx, int block);
 long long mercury_total_volume(MercuryContext *ctx);
 
+/* Cells per axis and ghost layers of one block (FieldBlock::get_block / the
+ * Block extents, include/3_field/FieldBlock.h:27-35). */
+int mercury_block_shape(MercuryContext *ctx, int block, int *dims, int *nghost);
+/* Device address of component comp of a field on one block: the block box of
+ * the storage convention above, mercury_block_volume(ctx, block) doubles,
+ * owned by the context and valid until mercury_destroy.  It is the device
+ * counterpart of FieldBlock::raw_data (include/3_field/FieldBlock.h:52) and
+ * lets a caller run its own kernels or the mercury_dec_* operators on solver
+ * fields in place.  Waits for the context's stream first; in the mixed form a
+ * field the step holds in FP32 is first widened into its FP64 array. */
+int mercury_field_device(MercuryContext *ctx, const char *name, int block, int comp, double **ptr);
+/* The context's HIP stream (a hipStream_t): work a caller orders against the
+ * step goes on it. */
+int mercury_stream(MercuryContext *ctx, void **stream);
+
 /* Replace direct FieldBlock(i,j,k,m) access on host storage
  * (include/3_field/FieldBlock.h:38-49) and the restart injection of
  * IOModule::ReadRestartBinFile (src/6_io/IOModule_Bin.cpp:234-420). */
@@ -118,9 +133,19 @@ int mercury_upload(MercuryContext *ctx, const char *name, int block, int comp, c
 int mercury_download(MercuryContext *ctx, const char *name, int block, int comp, double *host);
 
 /* Initial face fluxes: B = Bu . JD + d1(A) and Badd = B_imf . JD on the allocated
- * face range; A holds 3 edge arrays per block (or NULL).  The IMF part stands in
- * for DipoleField::Build_Badd_FaceFlux (Z4_Mercury/include/operators/Dipole.h:89-374). */
+ * face range; A holds 3 edge arrays per block (or NULL).  Badd is the IMF part of
+ * DipoleField::Build_Badd_FaceFlux (Z4_Mercury/include/operators/Dipole.h:326-372). */
 int mercury_init_bface(MercuryContext *ctx, const double *Bu, const double *const *A);
+/* mercury_init_bface plus the intrinsic dipoles of DipoleField::Build_Badd_FaceFlux
+ * (Dipole.h:89-374, called from Mercury_Initial::Initial_Badd, Z4_Mercury/include/2_Initial.h:219-224):
+ * Badd += d1(A_dip . dr) on the face boxes of ghost depth 1, where A_dip is the
+ * sum over dipoles of (m x r) / (|r|^2 + a^2)^(3/2), r = x - r0, taken at each
+ * edge midpoint (DipoleField::A_of_dipole, :25-39).  dipoles holds 7
+ * nondimensional doubles per dipole, x0 y0 z0 mx my mz a, as
+ * DipoleField::load_from_param (:44-87) derives them from the Dipole<i> lists
+ * (r0 and a over L_ref, m over 4 pi B_ref L_ref^3 / mu_mag).  Needs nghost >= 1. */
+int mercury_init_bface_dipoles(MercuryContext *ctx, const double *Bu, const double *const *A, int ndipoles,
+                               const double *dipoles);
 
 /* Replaces the constructor tail Sync("Badd"), Sync("Bface") (MercurySolver.cpp:141-145)
  * and the UpdateDerivedFields_ call at the head of Advance (MercurySolver_Driver.cpp:6-19).
@@ -285,6 +310,34 @@ int mercury_time_op(MercuryContext *ctx, const char *op, int reps, double *ms_av
  * SetupCellReconstructionWeights_ (Z4_Mercury/src/MercurySolver_ReconstructionSetup.cpp:29-193). */
 int mercury_geometry_block(const int *dims, int nghost, const double *owned_nodes, const MercuryPhysics *phys,
                            const char *name, int comp, double *out);
+
+/* Discrete exterior derivatives on one block held in CALLER device arrays
+ * (no context): dims = cells per axis, nghost = ghost layers, every array one
+ * component over the block box of the storage convention above.  Each
+ * operator writes the owned range of its output location ([0, m + 1 - delta)
+ * per axis, FieldBlock::inner_lo/inner_hi, include/3_field/FieldBlock.h:34-35)
+ * and reads its inputs there and one index above; ghost layers are left
+ * untouched.  stream is a hipStream_t (NULL: the null stream); the calls are
+ * asynchronous on it.  edge and face are arrays of 3 pointers (xi, eta, zeta). */
+/* DEC::d0_node_to_edge(FieldBlock&...) (include/8_dec/DecOps.h:11-16,
+ * src/8_dec/DecExteriorDerivative.cpp:41-77) */
+int mercury_dec_d0(const int *dims, int nghost, const double *node, double *const *edge, void *stream);
+/* DEC::d1_edge_to_face(FieldBlock&...) (DecOps.h:24-31, DecExteriorDerivative.cpp:99-137) */
+int mercury_dec_d1(const int *dims, int nghost, const double *const *edge, double *const *face, void *stream);
+/* DEC::d2_face_to_cell(FieldBlock&...) (DecOps.h:39-44, DecExteriorDerivative.cpp:164-180) */
+int mercury_dec_d2(const int *dims, int nghost, const double *const *face, double *cell, void *stream);
+/* DEC::faraday_update_face_2form(FieldBlock&...): face -= dt * d1(edge)
+ * (DecOps.h:52-60, DecExteriorDerivative.cpp:202-241) */
+int mercury_dec_faraday_update(const int *dims, int nghost, const double *const *edge, double *const *face, double dt,
+                               void *stream);
+/* DEC::max_abs_div_curl(FieldBlock&...) (include/8_dec/DecDiagnostics.h:17-20,
+ * src/8_dec/DecDiagnostics.cpp:110-135): max over owned cells of |d2 d1 edge|,
+ * written to result, ONE DEVICE double.  The per-workgroup partial maxima live
+ * in a stream-ordered allocation (hipMallocAsync / hipFreeAsync on stream). */
+int mercury_dec_max_abs_div_curl(const int *dims, int nghost, const double *const *edge, double *result, void *stream);
+/* DEC::max_abs_curl_grad(FieldBlock&...) (DecDiagnostics.h:11, DecDiagnostics.cpp:71-93):
+ * max over owned faces of |d1 d0 node|, to one device double, as above. */
+int mercury_dec_max_abs_curl_grad(const int *dims, int nghost, const double *node, double *result, void *stream);
 
 #ifdef __cplusplus
 }

@@ -15,7 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmercury_hip.so")
-SOURCES = ["mercury_kernels.hip", "mercury_fused.hip", "mercury_resist.hip", "mercury_output.hip", "mercury_geometry.cpp",
+SOURCES = ["mercury_kernels.hip", "mercury_fused.hip", "mercury_resist.hip", "mercury_output.hip", "mercury_dec.hip", "mercury_geometry.cpp",
            "mercury_io.cpp", "mercury_case.cpp", "mercury_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -40,6 +40,25 @@ def build(verbose: bool = False) -> str:
     for o in objs:
         os.remove(o)
     return LIB
+
+
+ROOT = os.path.dirname(HERE)
+CALLER_SRC = os.path.join(ROOT, "tests", "cpp", "mercury_caller.cpp")
+CALLER = os.path.join(ROOT, "tests", "cpp", "mercury_caller")
+
+
+def build_cpp_caller(verbose: bool = False) -> str:
+    """The C++ program of tests/cpp, written against the drop-in headers of
+    include/mercury and linked with the in-tree library (host code only: g++)."""
+    tmp = CALLER + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include", "mercury"),
+           "-I", "/opt/rocm/include", CALLER_SRC, "-o", tmp, "-L", HERE, "-lmercury_hip", "-L", "/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/../../mpcns_mercury_amd", "-Wl,-rpath,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, CALLER)
+    return CALLER
 
 
 if __name__ == "__main__":

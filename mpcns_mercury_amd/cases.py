@@ -27,7 +27,7 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
-from .params import MercuryPhysics, MercuryUnits, derive, RHO_SMALL
+from .params import MercuryPhysics, MercuryUnits, derive, dipole_array, RHO_SMALL
 
 BC_PERIODIC, BC_FARFIELD, BC_WALL, BC_INTERFACE = 0, 1, 2, 3
 
@@ -70,6 +70,12 @@ class Case:
     steps: int = 100
     integrator: int = 0  # 0 forward Euler (reference), 1 SSP-RK2
     meta: dict = field(default_factory=dict)
+    # intrinsic dipoles of Badd: (n, 7) nondimensional x0 y0 z0 mx my mz a, or None
+    dipoles: Optional[np.ndarray] = None
+
+    def dipole_array(self) -> np.ndarray:
+        d = np.zeros((0, 7)) if self.dipoles is None else np.asarray(self.dipoles, dtype=np.float64).reshape(-1, 7)
+        return np.ascontiguousarray(d)
 
     @property
     def nb(self) -> int:
@@ -240,7 +246,8 @@ def split_block_case(case: Case, cuts, name: str | None = None) -> Case:
     meta = dict(case.meta)
     meta.update({"split": (sx, sy, sz), "cuts": cuts, "origins": [o.tolist() for o in org]})
     return Case(name=name or f"{case.name}_split{sx}x{sy}x{sz}", g=g, blocks=blocks, phys=case.phys, fields=fields,
-                A=A if case.A is not None else None, Bu=case.Bu, steps=case.steps, integrator=case.integrator, meta=meta)
+                A=A if case.A is not None else None, Bu=case.Bu, steps=case.steps, integrator=case.integrator, meta=meta,
+                dipoles=case.dipoles)
 
 
 def case_from_geometry(geo, template: Case, name: str | None = None) -> Case:
@@ -263,7 +270,8 @@ def case_from_geometry(geo, template: Case, name: str | None = None) -> Case:
         blocks.append(Block(dims=tuple(geo.dims[b]), bc=tuple(int(x) for x in bc[6 * b:6 * b + 6]), nodes=geo.nodes[b],
                             nbr=tuple(int(x) for x in nbr[6 * b:6 * b + 6]), off=tuple(offs)))
     return Case(name=name or f"{template.name}_case", g=template.g, blocks=blocks, phys=template.phys, fields=template.fields,
-                A=template.A, Bu=template.Bu, steps=template.steps, integrator=template.integrator, meta=dict(template.meta))
+                A=template.A, Bu=template.Bu, steps=template.steps, integrator=template.integrator, meta=dict(template.meta),
+                dipoles=template.dipoles)
 
 
 def make_split_cube(n=(48, 48, 48), split=(2, 2, 2), g: int = 4, seed: int = 1, amp: float = 1e-3,
@@ -294,6 +302,10 @@ _PANELS = (((1, 0, 0), (0, 1, 0), (0, 0, 1)), ((-1, 0, 0), (0, 0, 1), (0, 1, 0))
            ((0, 0, 1), (1, 0, 0), (0, 1, 0)), ((0, 0, -1), (0, 1, 0), (1, 0, 0)))
 
 
+# the intrinsic dipole of the reference's example setup (Z4_Mercury/9999setup/3_MHD_Control.txt:4-9)
+MERCURY_DIPOLE = {"x0": 0.0, "y0": 0.0, "z0": 4.88e5, "mx": 0.0, "my": 0.0, "mz": -2.83e19, "a": 5.0e2}
+
+
 def sector_nodes(dims, panel, r0=1.0, r1=4.0, half_angle=np.deg2rad(40.0)):
     """Gnomonic sector of the shell r in [r0, r1]: xi radial, eta/zeta angular."""
     mx, my, mz = dims
@@ -310,7 +322,7 @@ def sector_nodes(dims, panel, r0=1.0, r1=4.0, half_angle=np.deg2rad(40.0)):
 def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, integrator: int = 0,
                scale: float = 1.0, hall_coef: float | None = None, resist: float | None = None,
                inflow: float = 1.0, imf=(0.0, 0.0, -0.2), na_fraction: float = 0.01, panels=None,
-               name: str | None = None, resist_profile: str = "window") -> Case:
+               name: str | None = None, resist_profile: str = "window", dipole: bool = False) -> Case:
     """Two-fluid Hall MHD: H+ and Na+ (mass ratio 1:23, density ratio 1:0.01) plus
     electron pressure (ambipolar term), inflow u = (-1,0,0), IMF B = (0,0,-0.2),
     the fluid side of a perfectly conducting sphere at r = 1 on every block's
@@ -318,6 +330,10 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
     ``resist`` switches the interior resistivity on with that inver_Rem on the
     reference's window (r_inner 0.84, r_outer 1.04, width 0.02,
     Z4_Mercury/9999setup/3_MHD_Control.txt:12-22).
+
+    ``dipole`` adds Mercury's intrinsic dipole of the reference's example setup
+    (0, 0, 4.88e5 m; m = (0, 0, -2.83e19) A m^2; a = 500 m;
+    Z4_Mercury/9999setup/3_MHD_Control.txt:1-9) to Badd.
 
     The listed block sizes admit no conformal 1:1 tiling of the shell, so every
     block is a separate gnomonic sector (no inter-block interfaces)."""
@@ -338,6 +354,8 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
         else:
             set_resistivity(units, resist, 0.84, 1.04, 0.02)
         units.mercury_eta_profile = resist_profile
+    if dipole:
+        units.dipoles = [dict(MERCURY_DIPOLE)]
     P = derive(units)
     gamma = P.gamma
     blks, fields = [], []
@@ -367,7 +385,8 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
         name = "shell6" if scale == 1.0 else f"shell6_x{scale:g}"
     return Case(name=name + ("_resist" if resist is not None else ""), g=g, blocks=blks, phys=P, fields=fields,
                 A=None, Bu=(0.0, 0.0, 0.0), steps=steps, integrator=integrator,
-                meta={"seed": seed, "species": 2, "hall": True, "resist": resist})
+                meta={"seed": seed, "species": 2, "hall": True, "resist": resist, "dipole": dipole},
+                dipoles=dipole_array(units) if dipole else None)
 
 
 def make_uniform_block(nodes: np.ndarray, bc, g: int = 4, rho: float = 1.0, u=(0.0, 0.0, 0.0), p: float = 0.1,

@@ -403,4 +403,48 @@ void jacobi_eigen3(const double Min[3][3], double V[3][3], double d[3]) {
     d[2] = A[2][2];
 }
 
+// DipoleField::Build_Badd_FaceFlux steps 1-2 (Z4_Mercury/include/operators/Dipole.h:128-324)
+// with A_of_dipole (:25-39): the edge 1-form A.dr of the dipoles' vector
+// potential (midpoint value times the edge vector) on the edge boxes of ghost
+// depth 1, and its curl (d1) on the face boxes of ghost depth 1, added to the
+// three face arrays in badd (one rounded addition each).  dip: 7 values per
+// dipole, x0 y0 z0 mx my mz a, nondimensional.
+void BlockGeom::add_dipole_flux(int nd, const double *dip, std::array<std::vector<double>, 3> &badd) const {
+    if (nd <= 0) return;
+    static const int DE[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    std::array<std::vector<double>, 3> Ae;
+    for (int a = 0; a < 3; ++a) {
+        Ae[a].assign(V, 0.0);
+        for (int i = -1; i < m[0] + 2 - DE[a][0]; ++i)
+            for (int j = -1; j < m[1] + 2 - DE[a][1]; ++j)
+                for (int k = -1; k < m[2] + 2 - DE[a][2]; ++k) {
+                    const V3 x1 = node(i, j, k), x2 = node(i + DE[a][0], j + DE[a][1], k + DE[a][2]);
+                    const V3 xm = {(x1[0] + x2[0]) * 0.5, (x1[1] + x2[1]) * 0.5, (x1[2] + x2[2]) * 0.5};
+                    const V3 dr = sub(x2, x1);
+                    V3 A = {0.0, 0.0, 0.0};
+                    for (int d = 0; d < nd; ++d) {
+                        const double *D = dip + 7 * d;
+                        const V3 r = {xm[0] - D[0], xm[1] - D[1], xm[2] - D[2]};
+                        double f = (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) + D[6] * D[6];
+                        f = std::sqrt(f) * f;
+                        const V3 c = crs({D[3], D[4], D[5]}, r);
+                        for (int t = 0; t < 3; ++t) A[t] += c[t] / f;
+                    }
+                    Ae[a][idx(i, j, k)] = dot(A, dr);
+                }
+    }
+    auto E = [&](int c, int i, int j, int k) { return Ae[c][idx(i, j, k)]; };
+    for (int a = 0; a < 3; ++a)
+        for (int i = -1; i < m[0] + 1 + (a == 0); ++i)
+            for (int j = -1; j < m[1] + 1 + (a == 1); ++j)
+                for (int k = -1; k < m[2] + 1 + (a == 2); ++k) {
+                    double c;
+                    if (a == 0) c = (E(1, i, j, k) - E(1, i, j, k + 1)) + (E(2, i, j + 1, k) - E(2, i, j, k));
+                    else if (a == 1) c = (E(0, i, j, k + 1) - E(0, i, j, k)) + (E(2, i, j, k) - E(2, i + 1, j, k));
+                    else c = (E(0, i, j, k) - E(0, i, j + 1, k)) + (E(1, i + 1, j, k) - E(1, i, j, k));
+                    double &v = badd[a][idx(i, j, k)];
+                    v = c + v;
+                }
+}
+
 }  // namespace merc

@@ -27,6 +27,8 @@ struct BlockGeom {
     std::array<double, 3> node(int i, int j, int k) const;
     void add_ghost_mesh();
     void compute(const MercPhys &P, SlotWriter &out);
+    // dipoles' face flux curl(A.dr) on the face boxes of ghost depth 1, added to badd (3 face arrays)
+    void add_dipole_flux(int nd, const double *dip, std::array<std::vector<double>, 3> &badd) const;
 };
 
 }  // namespace merc

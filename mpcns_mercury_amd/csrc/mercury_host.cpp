@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -31,6 +32,7 @@
 #include "mercury_io.h"
 #include "mercury_output.h"
 #include "mercury_resist.h"
+#include "mercury_dec.h"
 
 static_assert(sizeof(MercuryPhysics) == sizeof(MercPhys), "MercuryPhysics / MercPhys layout mismatch");
 
@@ -1464,14 +1466,18 @@ public:
         if (n > 0) check_status();
     }
 
-    void init_bface(const double Bu[3], const double *const *A) {
+    void init_bface(const double Bu[3], const double *const *A, int ndip = 0, const double *dip = nullptr) {
+        if (ndip < 0 || (ndip > 0 && !dip)) throw std::runtime_error("init_bface: invalid dipole list");
+        if (ndip > 0 && F->nb() > 0 && geom[0]->g < 1) throw std::runtime_error("init_bface: dipoles need at least one ghost layer");
         for (int b = 0; b < F->nb(); ++b) {
             const BlockGeom &G = *geom[b];
             const long long V = G.V;
             const double *Ax = A ? A[3 * b] : nullptr, *Ay = A ? A[3 * b + 1] : nullptr, *Az = A ? A[3 * b + 2] : nullptr;
             auto AA = [&](const double *p, int i, int j, int k) { return p ? p[G.idx(i, j, k)] : 0.0; };
+            std::array<std::vector<double>, 3> Badd;
             for (int a = 0; a < 3; ++a) {
-                std::vector<double> Bf(V, 0.0), Ba(V, 0.0);
+                std::vector<double> Bf(V, 0.0), &Ba = Badd[a];
+                Ba.assign(V, 0.0);
                 int lo[3], hi[3];
                 for (int c = 0; c < 3; ++c) {
                     lo[c] = -G.g;
@@ -1490,8 +1496,11 @@ public:
                             Ba[q] = P.B_imf[0] * j0 + P.B_imf[1] * j1 + P.B_imf[2] * j2;
                         }
                 F->upload_block(SL_B + a, b, Bf.data());
-                F->upload_block(SL_BADD + a, b, Ba.data());
             }
+            // the dipoles' part of Badd on the face boxes of ghost depth 1
+            // (DipoleField::Build_Badd_FaceFlux, Z4_Mercury/include/operators/Dipole.h:89-374)
+            G.add_dipole_flux(ndip, dip, Badd);
+            for (int a = 0; a < 3; ++a) F->upload_block(SL_BADD + a, b, Badd[a].data());
         }
     }
 
@@ -1699,6 +1708,38 @@ static int field_comp_slot(const char *name, int comp) {
     if (!f) throw std::runtime_error(std::string("unknown field: ") + (name ? name : "(null)"));
     if (comp < 0 || comp >= f->ncomp) throw std::runtime_error("component out of range");
     return f->slot + comp;
+}
+
+int mercury_block_shape(MercuryContext *ctx, int block, int *dims, int *nghost) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        if (block < 0 || block >= ctx->s->F->nb()) throw std::runtime_error("invalid block");
+        const MercBlk &k = ctx->s->F->blk[block];
+        if (dims)
+            for (int a = 0; a < 3; ++a) dims[a] = k.m[a];
+        if (nghost) *nghost = k.g;
+    })
+}
+
+int mercury_field_device(MercuryContext *ctx, const char *name, int block, int comp, double **ptr) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        if (block < 0 || block >= ctx->s->F->nb() || !ptr) throw std::runtime_error("invalid block or output pointer");
+        const int slot = field_comp_slot(name, comp);
+        HIPCHK(hipStreamSynchronize(ctx->s->st));
+        ctx->s->materialize(slot);
+        *ptr = ctx->s->F->hptr[slot] + ctx->s->F->blk[block].off;
+    })
+}
+
+int mercury_stream(MercuryContext *ctx, void **stream) {
+    CHECK_CTX(ctx);
+    if (!stream) {
+        merc::g_last_error = "mercury_stream: stream is NULL";
+        return -1;
+    }
+    *stream = (void *)ctx->s->st;
+    return 0;
 }
 
 int mercury_upload(MercuryContext *ctx, const char *name, int block, int comp, const double *host) {
@@ -2120,7 +2161,7 @@ int mercury_case_physical(const MercuryCase *c, int *n, int *records) {
     return 0;
 }
 
-int mercury_init_bface(MercuryContext *ctx, const double *Bu, const double *const *A) {
+int mercury_init_bface_dipoles(MercuryContext *ctx, const double *Bu, const double *const *A, int ndipoles, const double *dipoles) {
     CHECK_CTX(ctx);
     if (!Bu) {
         merc::g_last_error = "mercury_init_bface: Bu is null";
@@ -2128,8 +2169,12 @@ int mercury_init_bface(MercuryContext *ctx, const double *Bu, const double *cons
     }
     ABI_TRY({
         HIPCHK(hipStreamSynchronize(ctx->s->st));
-        ctx->s->init_bface(Bu, A);
+        ctx->s->init_bface(Bu, A, ndipoles, dipoles);
     })
+}
+
+int mercury_init_bface(MercuryContext *ctx, const double *Bu, const double *const *A) {
+    return mercury_init_bface_dipoles(ctx, Bu, A, 0, nullptr);
 }
 
 int mercury_prepare(MercuryContext *ctx) {
@@ -2167,6 +2212,71 @@ int mercury_timestep(MercuryContext *ctx, double *dt) {
 int mercury_divb(MercuryContext *ctx, double *normalised, double *maxabs) {
     CHECK_CTX(ctx);
     ABI_TRY({ ctx->s->divb(normalised, maxabs); })
+}
+
+// ---------------------------------------------------------------------------
+// DEC operators on caller device arrays (one block)
+// ---------------------------------------------------------------------------
+static void dec_args(const int *dims, int nghost, std::initializer_list<const void *> ptrs) {
+    merc::dec_check_box(dims, nghost);
+    for (const void *p : ptrs)
+        if (!p) throw std::runtime_error("DEC: a device array is NULL");
+}
+
+int mercury_dec_d0(const int *dims, int nghost, const double *node, double *const *edge, void *stream) {
+    ABI_TRY({
+        if (!edge) throw std::runtime_error("DEC: edge is NULL");
+        dec_args(dims, nghost, {node, edge[0], edge[1], edge[2]});
+        merc::dec_d0(dims, nghost, node, edge, (hipStream_t)stream);
+    })
+}
+
+int mercury_dec_d1(const int *dims, int nghost, const double *const *edge, double *const *face, void *stream) {
+    ABI_TRY({
+        if (!edge || !face) throw std::runtime_error("DEC: edge or face is NULL");
+        dec_args(dims, nghost, {edge[0], edge[1], edge[2], face[0], face[1], face[2]});
+        merc::dec_d1(dims, nghost, edge, face, 0, 0.0, (hipStream_t)stream);
+    })
+}
+
+int mercury_dec_d2(const int *dims, int nghost, const double *const *face, double *cell, void *stream) {
+    ABI_TRY({
+        if (!face) throw std::runtime_error("DEC: face is NULL");
+        dec_args(dims, nghost, {face[0], face[1], face[2], cell});
+        merc::dec_d2(dims, nghost, face, cell, (hipStream_t)stream);
+    })
+}
+
+int mercury_dec_faraday_update(const int *dims, int nghost, const double *const *edge, double *const *face, double dt, void *stream) {
+    ABI_TRY({
+        if (!edge || !face) throw std::runtime_error("DEC: edge or face is NULL");
+        dec_args(dims, nghost, {edge[0], edge[1], edge[2], face[0], face[1], face[2]});
+        merc::dec_d1(dims, nghost, edge, face, 1, dt, (hipStream_t)stream);
+    })
+}
+
+// partial maxima in a stream-ordered allocation released on the same stream
+int mercury_dec_max_abs_div_curl(const int *dims, int nghost, const double *const *edge, double *result, void *stream) {
+    ABI_TRY({
+        if (!edge) throw std::runtime_error("DEC: edge is NULL");
+        dec_args(dims, nghost, {edge[0], edge[1], edge[2], result});
+        const hipStream_t st = (hipStream_t)stream;
+        double *part = nullptr;
+        HIPCHK(hipMallocAsync((void **)&part, sizeof(double) * merc::dec_div_curl_groups(dims), st));
+        merc::dec_div_curl(dims, nghost, edge, part, result, st);
+        HIPCHK(hipFreeAsync(part, st));
+    })
+}
+
+int mercury_dec_max_abs_curl_grad(const int *dims, int nghost, const double *node, double *result, void *stream) {
+    ABI_TRY({
+        dec_args(dims, nghost, {node, result});
+        const hipStream_t st = (hipStream_t)stream;
+        double *part = nullptr;
+        HIPCHK(hipMallocAsync((void **)&part, sizeof(double) * merc::dec_curl_grad_groups(dims), st));
+        merc::dec_curl_grad(dims, nghost, node, part, result, st);
+        HIPCHK(hipFreeAsync(part, st));
+    })
 }
 
 int mercury_synchronize(MercuryContext *ctx) {

@@ -19,15 +19,16 @@ _lib = None
 # every symbol declared in include/mercury_hip.h
 ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercury_create_connected", "mercury_destroy",
                "mercury_field_info",
-               "mercury_block_volume", "mercury_total_volume", "mercury_upload", "mercury_download",
-               "mercury_init_bface", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
+               "mercury_block_volume", "mercury_block_shape", "mercury_field_device", "mercury_stream", "mercury_total_volume", "mercury_upload", "mercury_download",
+               "mercury_init_bface", "mercury_init_bface_dipoles", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
                "mercury_divb", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
                "mercury_time_op", "mercury_geometry_block", "mercury_set_form", "mercury_resist_stats",
                "mercury_ensemble_create", "mercury_ensemble_launch", "mercury_ensemble_destroy", "mercury_autotune",
                "mercury_write_restart", "mercury_read_restart", "mercury_write_rundata", "mercury_read_rundata",
                "mercury_write_tecplot", "mercury_case_read", "mercury_case_destroy", "mercury_case_blocks",
                "mercury_case_block_name", "mercury_case_nodes", "mercury_case_topology", "mercury_case_interfaces",
-               "mercury_case_physical")
+               "mercury_case_physical", "mercury_dec_d0", "mercury_dec_d1", "mercury_dec_d2", "mercury_dec_faraday_update",
+               "mercury_dec_max_abs_div_curl", "mercury_dec_max_abs_curl_grad")
 
 
 class MercuryError(RuntimeError):
@@ -68,8 +69,12 @@ def load():
     L.mercury_total_volume.restype = ctypes.c_longlong
     L.mercury_total_volume.argtypes = [vp]
     L.mercury_upload.argtypes = [vp, cp, ctypes.c_int, ctypes.c_int, dp]
+    L.mercury_block_shape.argtypes = [vp, ctypes.c_int, ip, ip]
+    L.mercury_field_device.argtypes = [vp, cp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+    L.mercury_stream.argtypes = [vp, ctypes.POINTER(vp)]
     L.mercury_download.argtypes = [vp, cp, ctypes.c_int, ctypes.c_int, dp]
     L.mercury_init_bface.argtypes = [vp, dp, ctypes.POINTER(dp)]
+    L.mercury_init_bface_dipoles.argtypes = [vp, dp, ctypes.POINTER(dp), ctypes.c_int, dp]
     L.mercury_prepare.argtypes = [vp]
     L.mercury_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     L.mercury_op.argtypes = [vp, cp, ctypes.c_double]
@@ -86,6 +91,14 @@ def load():
     L.mercury_ensemble_destroy.argtypes = [vp]
     L.mercury_autotune.argtypes = [vp, ip]
     L.mercury_geometry_block.argtypes = [ip, ctypes.c_int, dp, ctypes.POINTER(MercuryPhysics), cp, ctypes.c_int, dp]
+    # DEC operators on caller device arrays: pointers travel as void *
+    pp = ctypes.POINTER(vp)
+    L.mercury_dec_d0.argtypes = [ip, ctypes.c_int, vp, pp, vp]
+    L.mercury_dec_d1.argtypes = [ip, ctypes.c_int, pp, pp, vp]
+    L.mercury_dec_d2.argtypes = [ip, ctypes.c_int, pp, vp, vp]
+    L.mercury_dec_faraday_update.argtypes = [ip, ctypes.c_int, pp, pp, ctypes.c_double, vp]
+    L.mercury_dec_max_abs_div_curl.argtypes = [ip, ctypes.c_int, pp, vp, vp]
+    L.mercury_dec_max_abs_curl_grad.argtypes = [ip, ctypes.c_int, vp, vp, vp]
     _lib = L
     return L
 
@@ -161,13 +174,13 @@ class Mercury:
             for name, arr in fl.items():
                 self.set(name, ib, arr)
         Bu = np.array(case.Bu, dtype=np.float64)
+        dip = case.dipole_array()
+        ptrs = None
         if case.A is not None:
             flat = [np.ascontiguousarray(case.A[ib][a], dtype=np.float64) for ib in range(case.nb) for a in range(3)]
             self._A = flat
             ptrs = (ctypes.POINTER(ctypes.c_double) * len(flat))(*[_dp(x) for x in flat])
-            _chk(L.mercury_init_bface(self._h, _dp(Bu), ptrs))
-        else:
-            _chk(L.mercury_init_bface(self._h, _dp(Bu), None))
+        _chk(L.mercury_init_bface_dipoles(self._h, _dp(Bu), ptrs, len(dip), _dp(dip)))
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -15,6 +15,8 @@ from __future__ import annotations
 
 import ctypes
 import math
+
+import numpy as np
 from dataclasses import dataclass, field
 from typing import Dict
 
@@ -99,6 +101,9 @@ class MercuryUnits:
     # eta profile: "window" (the reference's, MercurySolver_Tools.cpp:50-69) or "exp",
     # inver_Rem * exp(-(r - r_inner) / width) (an extension for the resistive shell case)
     mercury_eta_profile: str = "window"
+    # intrinsic dipoles (num_of_dipole + LIST Dipole<i>, Z4_Mercury/9999setup/3_MHD_Control.txt:1-9):
+    # dimensional x0 y0 z0 [m], mx my mz [A m^2], a [m]
+    dipoles: list = field(default_factory=list)
 
     def alfven_units(self) -> "MercuryUnits":
         """Choose B_ref so that inver_MA2 = c_hall = 1 (B in Alfven units)."""
@@ -317,4 +322,19 @@ def units_from_setup(params: Dict[str, object]) -> MercuryUnits:
     u.implicit_resistive_rtol = get("implicit_resistive_ksp_rtol")
     u.implicit_resistive_atol = get("implicit_resistive_ksp_atol")
     u.implicit_resistive_max_it = get("implicit_resistive_ksp_max_it")
+    u.dipoles = [dict(get(f"Dipole{i}")) for i in range(1, int(get("num_of_dipole")) + 1)]
     return u
+
+
+def dipole_array(u: MercuryUnits) -> np.ndarray:
+    """Nondimensional dipoles, 7 values each (x0 y0 z0 mx my mz a), as
+    DipoleField::load_from_param derives them (Z4_Mercury/include/operators/Dipole.h:44-87):
+    positions and softening length over L_ref, moments over
+    4 * 3.1415926535 * B_ref * L_ref^3 / mu_mag (the reference's literal pi)."""
+    L_ref, B_ref, mu = u.REF["L_ref"], u.REF["B_ref"], u.constant["mu_mag"]
+    inv = 1.0 / (4 * 3.1415926535 * B_ref * L_ref * L_ref * L_ref / mu)
+    out = np.zeros((len(u.dipoles), 7))
+    for n, d in enumerate(u.dipoles):
+        out[n] = (d["x0"] / L_ref, d["y0"] / L_ref, d["z0"] / L_ref, d["mx"] * inv, d["my"] * inv, d["mz"] * inv,
+                  d["a"] / L_ref)
+    return out

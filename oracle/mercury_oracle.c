@@ -2307,6 +2307,74 @@ int orc_init_bface(void *h, const double *Bu, const double *const *A) {
     return 0;
 }
 
+/* DipoleField::A_of_dipole (Z4_Mercury/include/operators/Dipole.h:25-39): the
+ * vector potential sum_d (m_d x r) / (|r|^2 + a_d^2)^(3/2), r = p - r0_d; dip
+ * holds 7 nondimensional values per dipole: x0 y0 z0 mx my mz a.  Pinned by
+ * the closed-form curl (3 r (r.m) - m (r^2 - 2 a^2)) / (r^2 + a^2)^(5/2) of the
+ * face fluxes and by Mercury's equatorial surface field (tests). */
+static void dipole_A(int nd, const double *dip, const double p[3], double A[3]) {
+    A[0] = A[1] = A[2] = 0.0;
+    for (int d = 0; d < nd; ++d) {
+        const double *D = dip + 7 * d;
+        const double r[3] = {p[0] - D[0], p[1] - D[1], p[2] - D[2]};
+        double f = (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) + D[6] * D[6];
+        f = sqrt(f) * f;
+        A[0] += (D[4] * r[2] - D[5] * r[1]) / f;
+        A[1] += (D[5] * r[0] - D[3] * r[2]) / f;
+        A[2] += (D[3] * r[1] - D[4] * r[0]) / f;
+    }
+}
+
+/* DipoleField::Build_Badd_FaceFlux steps 1-2 (Dipole.h:128-324): the edge
+ * 1-form A.dr (midpoint value of A times the edge vector) on the edge boxes of
+ * ghost depth 1, then its curl on the face boxes of ghost depth 1, added to the
+ * IMF flux already in Badd (step 3, :326-372; one rounded addition, so the
+ * order of the two parts does not matter).  Faces beyond depth 1 keep the IMF
+ * flux alone until Sync(Badd) fills them. */
+int orc_add_dipole_badd(void *h, int nd, const double *dip) {
+    Orc *o = (Orc *)h;
+    if (nd <= 0) return 0;
+    for (int ib = 0; ib < o->nb; ++ib) {
+        OBlock *b = &o->b[ib];
+        double *Ae[3];
+        for (int a = 0; a < 3; ++a) {
+            Ae[a] = (double *)calloc((size_t)b->V, sizeof(double));
+            int lo[3], hi[3];
+            alloc_box(b, L_EX + a, 1, lo, hi);
+            for (int i = lo[0]; i < hi[0]; ++i)
+                for (int j = lo[1]; j < hi[1]; ++j)
+                    for (int k = lo[2]; k < hi[2]; ++k) {
+                        double x1[3], x2[3], xm[3], dr[3], A[3];
+                        node(b, i, j, k, x1);
+                        node(b, i + (a == 0), j + (a == 1), k + (a == 2), x2);
+                        for (int c = 0; c < 3; ++c) {
+                            xm[c] = (x1[c] + x2[c]) * 0.5;
+                            dr[c] = x2[c] - x1[c];
+                        }
+                        dipole_A(nd, dip, xm, A);
+                        Ae[a][IX(b, i, j, k)] = A[0] * dr[0] + A[1] * dr[1] + A[2] * dr[2];
+                    }
+        }
+#define AE(c, i, j, k) Ae[c][IX(b, i, j, k)]
+        for (int a = 0; a < 3; ++a) {
+            int lo[3], hi[3];
+            alloc_box(b, L_FX + a, 1, lo, hi);
+            for (int i = lo[0]; i < hi[0]; ++i)
+                for (int j = lo[1]; j < hi[1]; ++j)
+                    for (int k = lo[2]; k < hi[2]; ++k) {
+                        double c;
+                        if (a == 0) c = (AE(1, i, j, k) - AE(1, i, j, k + 1)) + (AE(2, i, j + 1, k) - AE(2, i, j, k));
+                        else if (a == 1) c = (AE(0, i, j, k + 1) - AE(0, i, j, k)) + (AE(2, i, j, k) - AE(2, i + 1, j, k));
+                        else c = (AE(0, i, j, k) - AE(0, i, j + 1, k)) + (AE(1, i + 1, j, k) - AE(1, i, j, k));
+                        F(b, S_BADD + a, i, j, k) = c + F(b, S_BADD + a, i, j, k);
+                    }
+        }
+#undef AE
+        for (int a = 0; a < 3; ++a) free(Ae[a]);
+    }
+    return 0;
+}
+
 /* MercurySolver ctor tail + Advance head: Sync(Badd), Sync(Bface),
  * UpdateDerivedFields_ (MercurySolver.cpp:141-145, MercurySolver_Driver.cpp:6-19) */
 int orc_prepare(void *h) {
@@ -2450,4 +2518,137 @@ double orc_divb(void *h, double *maxabs_out) {
     }
     if (maxabs_out) *maxabs_out = dmaxv;
     return bmax > 0.0 ? dmaxv / bmax : dmaxv;
+}
+
+/* ------------------------------------------------------------------------- */
+/* DEC operators of one block on caller arrays (src/8_dec)                     */
+/* ------------------------------------------------------------------------- */
+/* Array layout as above: N_a = m_a + 1 + 2g, k fastest.  Loops run over the
+ * owned range of the output location, [0, m + 1 - delta) per axis
+ * (FieldBlock::inner_lo / inner_hi, include/3_field/FieldBlock.h:34-35).
+ * Pinned by the reference's own DEC chain test (d1 d0 = 0 and d2 d1 = 0 on its
+ * polynomial fills, Z0_NULL/test/dec_chain/Z0_DecChainTest.cpp:13-51,85-100),
+ * by single-point known answers (one unit value gives the signed incidence
+ * pattern of the operator) and by max|.| of a single planted value. */
+typedef struct {
+    int m[3], g;
+    long long sI, sJ;
+} DecBox;
+static DecBox dec_box(const int *dims, int g) {
+    DecBox b;
+    for (int a = 0; a < 3; ++a) b.m[a] = dims[a];
+    b.g = g;
+    b.sJ = dims[2] + 1 + 2LL * g;
+    b.sI = (dims[1] + 1 + 2LL * g) * b.sJ;
+    return b;
+}
+#define DQ(b, i, j, k) ((long long)((i) + (b).g) * (b).sI + (long long)((j) + (b).g) * (b).sJ + ((k) + (b).g))
+
+/* DEC::d0_node_to_edge (src/8_dec/DecExteriorDerivative.cpp:41-77) */
+void orc_dec_d0(const int *dims, int g, const double *n, double *ex, double *ey, double *ez) {
+    const DecBox b = dec_box(dims, g);
+    for (int i = 0; i < b.m[0]; ++i)
+        for (int j = 0; j <= b.m[1]; ++j)
+            for (int k = 0; k <= b.m[2]; ++k) ex[DQ(b, i, j, k)] = n[DQ(b, i + 1, j, k)] - n[DQ(b, i, j, k)];
+    for (int i = 0; i <= b.m[0]; ++i)
+        for (int j = 0; j < b.m[1]; ++j)
+            for (int k = 0; k <= b.m[2]; ++k) ey[DQ(b, i, j, k)] = n[DQ(b, i, j + 1, k)] - n[DQ(b, i, j, k)];
+    for (int i = 0; i <= b.m[0]; ++i)
+        for (int j = 0; j <= b.m[1]; ++j)
+            for (int k = 0; k < b.m[2]; ++k) ez[DQ(b, i, j, k)] = n[DQ(b, i, j, k + 1)] - n[DQ(b, i, j, k)];
+}
+
+/* d1_xi_at / d1_eta_at / d1_zeta_at (DecExteriorDerivative.cpp:7-39) */
+static double dec_d1_xi(DecBox b, const double *ey, const double *ez, int i, int j, int k) {
+    return (ey[DQ(b, i, j, k)] - ey[DQ(b, i, j, k + 1)]) + (ez[DQ(b, i, j + 1, k)] - ez[DQ(b, i, j, k)]);
+}
+static double dec_d1_eta(DecBox b, const double *ex, const double *ez, int i, int j, int k) {
+    return (ex[DQ(b, i, j, k + 1)] - ex[DQ(b, i, j, k)]) + (ez[DQ(b, i, j, k)] - ez[DQ(b, i + 1, j, k)]);
+}
+static double dec_d1_zeta(DecBox b, const double *ex, const double *ey, int i, int j, int k) {
+    return (ex[DQ(b, i, j, k)] - ex[DQ(b, i, j + 1, k)]) + (ey[DQ(b, i + 1, j, k)] - ey[DQ(b, i, j, k)]);
+}
+
+/* DEC::d1_edge_to_face (:99-137); update != 0: faraday_update_face_2form,
+ * face -= dt * d1(edge) (:202-241) */
+void orc_dec_d1(const int *dims, int g, const double *ex, const double *ey, const double *ez, double *fx, double *fy, double *fz,
+                int update, double dt) {
+    const DecBox b = dec_box(dims, g);
+    for (int i = 0; i <= b.m[0]; ++i)
+        for (int j = 0; j < b.m[1]; ++j)
+            for (int k = 0; k < b.m[2]; ++k) {
+                const double v = dec_d1_xi(b, ey, ez, i, j, k);
+                if (update) fx[DQ(b, i, j, k)] -= dt * v;
+                else fx[DQ(b, i, j, k)] = v;
+            }
+    for (int i = 0; i < b.m[0]; ++i)
+        for (int j = 0; j <= b.m[1]; ++j)
+            for (int k = 0; k < b.m[2]; ++k) {
+                const double v = dec_d1_eta(b, ex, ez, i, j, k);
+                if (update) fy[DQ(b, i, j, k)] -= dt * v;
+                else fy[DQ(b, i, j, k)] = v;
+            }
+    for (int i = 0; i < b.m[0]; ++i)
+        for (int j = 0; j < b.m[1]; ++j)
+            for (int k = 0; k <= b.m[2]; ++k) {
+                const double v = dec_d1_zeta(b, ex, ey, i, j, k);
+                if (update) fz[DQ(b, i, j, k)] -= dt * v;
+                else fz[DQ(b, i, j, k)] = v;
+            }
+}
+
+/* DEC::d2_face_to_cell (:164-180) */
+void orc_dec_d2(const int *dims, int g, const double *fx, const double *fy, const double *fz, double *c) {
+    const DecBox b = dec_box(dims, g);
+    for (int i = 0; i < b.m[0]; ++i)
+        for (int j = 0; j < b.m[1]; ++j)
+            for (int k = 0; k < b.m[2]; ++k)
+                c[DQ(b, i, j, k)] = (fx[DQ(b, i + 1, j, k)] - fx[DQ(b, i, j, k)]) + (fy[DQ(b, i, j + 1, k)] - fy[DQ(b, i, j, k)]) +
+                                    (fz[DQ(b, i, j, k + 1)] - fz[DQ(b, i, j, k)]);
+}
+
+/* DEC::max_abs_div_curl (src/8_dec/DecDiagnostics.cpp:110-135, curl_*_at :37-69):
+ * sequential max from 0.0, std::max(acc, |x|) (a NaN value leaves acc unchanged) */
+double orc_dec_max_abs_div_curl(const int *dims, int g, const double *ex, const double *ey, const double *ez) {
+    const DecBox b = dec_box(dims, g);
+    double acc = 0.0;
+    for (int i = 0; i < b.m[0]; ++i)
+        for (int j = 0; j < b.m[1]; ++j)
+            for (int k = 0; k < b.m[2]; ++k) {
+                const double v = (dec_d1_xi(b, ey, ez, i + 1, j, k) - dec_d1_xi(b, ey, ez, i, j, k)) +
+                                 (dec_d1_eta(b, ex, ez, i, j + 1, k) - dec_d1_eta(b, ex, ez, i, j, k)) +
+                                 (dec_d1_zeta(b, ex, ey, i, j, k + 1) - dec_d1_zeta(b, ex, ey, i, j, k));
+                acc = dmax(acc, fabs(v));
+            }
+    return acc;
+}
+
+/* DEC::max_abs_curl_grad (DecDiagnostics.cpp:71-93, abs_curl_grad_* :10-35) */
+double orc_dec_max_abs_curl_grad(const int *dims, int g, const double *n) {
+    const DecBox b = dec_box(dims, g);
+#define NV(i, j, k) n[DQ(b, i, j, k)]
+    double acc = 0.0;
+    for (int i = 0; i <= b.m[0]; ++i)
+        for (int j = 0; j < b.m[1]; ++j)
+            for (int k = 0; k < b.m[2]; ++k) {
+                const double e0 = NV(i, j + 1, k) - NV(i, j, k), e1 = NV(i, j + 1, k + 1) - NV(i, j, k + 1);
+                const double z0 = NV(i, j, k + 1) - NV(i, j, k), z1 = NV(i, j + 1, k + 1) - NV(i, j + 1, k);
+                acc = dmax(acc, fabs((e0 - e1) + (z1 - z0)));
+            }
+    for (int i = 0; i < b.m[0]; ++i)
+        for (int j = 0; j <= b.m[1]; ++j)
+            for (int k = 0; k < b.m[2]; ++k) {
+                const double x0 = NV(i + 1, j, k) - NV(i, j, k), x1 = NV(i + 1, j, k + 1) - NV(i, j, k + 1);
+                const double z0 = NV(i, j, k + 1) - NV(i, j, k), z1 = NV(i + 1, j, k + 1) - NV(i + 1, j, k);
+                acc = dmax(acc, fabs((x1 - x0) + (z0 - z1)));
+            }
+    for (int i = 0; i < b.m[0]; ++i)
+        for (int j = 0; j < b.m[1]; ++j)
+            for (int k = 0; k <= b.m[2]; ++k) {
+                const double x0 = NV(i + 1, j, k) - NV(i, j, k), x1 = NV(i + 1, j + 1, k) - NV(i, j + 1, k);
+                const double e0 = NV(i, j + 1, k) - NV(i, j, k), e1 = NV(i + 1, j + 1, k) - NV(i + 1, j, k);
+                acc = dmax(acc, fabs((x0 - x1) + (e1 - e0)));
+            }
+#undef NV
+    return acc;
 }

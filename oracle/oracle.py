@@ -50,6 +50,7 @@ def lib():
         L.orc_block_volume.restype = ctypes.c_long
         L.orc_block_volume.argtypes = [vp, ctypes.c_int]
         L.orc_init_bface.argtypes = [vp, dp, ctypes.POINTER(dp)]
+        L.orc_add_dipole_badd.argtypes = [vp, ctypes.c_int, dp]
         L.orc_prepare.argtypes = [vp]
         L.orc_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.orc_op.argtypes = [vp, ctypes.c_char_p, ctypes.c_double]
@@ -95,6 +96,9 @@ class Oracle:
             L.orc_init_bface(self.h, _dp(np.array(case.Bu, dtype=np.float64)), ptrs)
         else:
             L.orc_init_bface(self.h, _dp(np.array(case.Bu, dtype=np.float64)), None)
+        dip = case.dipole_array()
+        if len(dip):
+            L.orc_add_dipole_badd(self.h, len(dip), _dp(dip))
 
     def __del__(self):
         try:
@@ -161,3 +165,61 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"oracle resistive solve failed ({rc})")
         return its.value, rel.value
+
+
+# --- DEC operators of one block on plain arrays (src/8_dec) -------------------
+def _box_shape(dims, g):
+    return tuple(int(m) + 1 + 2 * g for m in dims)
+
+
+def _dec_args(dims, g):
+    L = lib()
+    for name in ("orc_dec_d0", "orc_dec_d1", "orc_dec_d2", "orc_dec_max_abs_div_curl", "orc_dec_max_abs_curl_grad"):
+        getattr(L, name).restype = ctypes.c_double if name.startswith("orc_dec_max") else None
+    return L, (ctypes.c_int * 3)(*[int(m) for m in dims]), int(g)
+
+
+def _c(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    assert a.shape == shape, (a.shape, shape)
+    return a
+
+
+def dec_d0(dims, g, node, out=None):
+    """Edge 1-form d0(node) over the owned edges; out (3 arrays) keeps its other points."""
+    L, d, g = _dec_args(dims, g)
+    sh = _box_shape(dims, g)
+    e = [np.full(sh, np.nan) for _ in range(3)] if out is None else [_c(x, sh).copy() for x in out]
+    L.orc_dec_d0(d, g, _dp(_c(node, sh)), _dp(e[0]), _dp(e[1]), _dp(e[2]))
+    return e
+
+
+def dec_d1(dims, g, edge, out=None, update=False, dt=0.0):
+    """Face 2-form d1(edge), or out - dt * d1(edge) with update=True."""
+    L, d, g = _dec_args(dims, g)
+    sh = _box_shape(dims, g)
+    e = [_c(x, sh) for x in edge]
+    f = [np.full(sh, np.nan) for _ in range(3)] if out is None else [_c(x, sh).copy() for x in out]
+    L.orc_dec_d1(d, g, _dp(e[0]), _dp(e[1]), _dp(e[2]), _dp(f[0]), _dp(f[1]), _dp(f[2]), int(update), ctypes.c_double(dt))
+    return f
+
+
+def dec_d2(dims, g, face, out=None):
+    L, d, g = _dec_args(dims, g)
+    sh = _box_shape(dims, g)
+    f = [_c(x, sh) for x in face]
+    c = np.full(sh, np.nan) if out is None else _c(out, sh).copy()
+    L.orc_dec_d2(d, g, _dp(f[0]), _dp(f[1]), _dp(f[2]), _dp(c))
+    return c
+
+
+def dec_max_abs_div_curl(dims, g, edge):
+    L, d, g = _dec_args(dims, g)
+    sh = _box_shape(dims, g)
+    e = [_c(x, sh) for x in edge]
+    return L.orc_dec_max_abs_div_curl(d, g, _dp(e[0]), _dp(e[1]), _dp(e[2]))
+
+
+def dec_max_abs_curl_grad(dims, g, node):
+    L, d, g = _dec_args(dims, g)
+    return L.orc_dec_max_abs_curl_grad(d, g, _dp(_c(node, _box_shape(dims, g))))

@@ -2,6 +2,7 @@
 include/mercury_hip.h declares, its host-side geometry equals the oracle's bit
 for bit, and invalid calls fail with a message instead of crashing."""
 import ctypes
+import subprocess
 import os
 import re
 
@@ -74,3 +75,29 @@ def test_host_geometry_bit_exact_vs_oracle(oracle_mod, which):
             a = hip.geometry_block(c, ib, name)
             b = o.get(name, ib)
             assert np.array_equal(a, b), f"{which} block {ib} {name}: max diff {np.abs(a - b).max():.3e}"
+
+
+DROPIN = os.path.join(ROOT, "include", "mercury")
+DROPIN_HEADERS = ["1_grid/1_MPCNS_Grid.h", "2_topology/Topology.h", "3_field/Field.h", "4_halo/Halo.h", "8_dec/DecOps.h",
+                  "8_dec/DecDiagnostics.h", "MercurySolver.h"]
+
+
+@pytest.mark.parametrize("header", DROPIN_HEADERS)
+def test_dropin_header_is_self_contained(header, tmp_path):
+    """Each C++ drop-in header compiles on its own (g++, host only)."""
+    src = tmp_path / "inc.cpp"
+    src.write_text(f'#include "{header}"\nint main() {{ return 0; }}\n')
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-D__HIP_PLATFORM_AMD__",
+                        "-I", DROPIN, "-I", "/opt/rocm/include", str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_cpp_caller_links_against_the_library():
+    """The C++ caller of tests/cpp (built by __graft_entry__.build) resolves every
+    symbol it takes from libmercury_hip.so."""
+    exe = os.path.join(ROOT, "tests", "cpp", "mercury_caller")
+    if not os.path.exists(exe):
+        from mpcns_mercury_amd import build
+        build.build_cpp_caller()
+    r = subprocess.run(["ldd", "-r", exe], capture_output=True, text=True)
+    assert "libmercury_hip.so" in r.stdout and "undefined symbol" not in (r.stdout + r.stderr), r.stdout + r.stderr

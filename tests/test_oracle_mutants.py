@@ -18,7 +18,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "mercury_oracle.c")
-PINS = ["tests/test_oracle_pins.py", "tests/test_oracle_term_pins.py", "tests/test_emf_pins.py"]
+PINS = ["tests/test_oracle_pins.py", "tests/test_oracle_term_pins.py", "tests/test_emf_pins.py", "tests/test_dipole_oracle.py",
+        "tests/test_dec_oracle.py"]
 
 # (what the mistake is, exact fragment, replacement)
 MUTANTS = [
@@ -97,6 +98,25 @@ MUTANTS = [
     # A4 CurlAdjFaceToEdge, A16 CurlEdgeToFace
     ("adjoint curl: beta dropped", "#define BB(a, i, j, k) (F(b, S_BETA + (a), i, j, k) * F(b, sf + (a), i, j, k))",
      "#define BB(a, i, j, k) (F(b, sf + (a), i, j, k))"),
+    # A20 DipoleField::Build_Badd_FaceFlux (Z4_Mercury/include/operators/Dipole.h:25-324)
+    ("dipole: softening dropped", "double f = (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) + D[6] * D[6];",
+     "double f = (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);"),
+    ("dipole: one cross-product component flipped", "A[1] += (D[5] * r[0] - D[3] * r[2]) / f;",
+     "A[1] += (D[3] * r[2] - D[5] * r[0]) / f;"),
+    ("dipole: potential at the edge start", "xm[c] = (x1[c] + x2[c]) * 0.5;", "xm[c] = x1[c];"),
+    ("dipole: power 1/2", "f = sqrt(f) * f;", "f = sqrt(f);"),
+    ("dipole: curl of the eta face with the wrong sign",
+     "else if (a == 1) c = (AE(0, i, j, k + 1) - AE(0, i, j, k)) + (AE(2, i, j, k) - AE(2, i + 1, j, k));",
+     "else if (a == 1) c = (AE(0, i, j, k) - AE(0, i, j, k + 1)) + (AE(2, i, j, k) - AE(2, i + 1, j, k));"),
+    # src/8_dec DecExteriorDerivative.cpp / DecDiagnostics.cpp
+    ("DEC d0: reversed difference", "for (int k = 0; k <= b.m[2]; ++k) ey[DQ(b, i, j, k)] = n[DQ(b, i, j + 1, k)] - n[DQ(b, i, j, k)];",
+     "for (int k = 0; k <= b.m[2]; ++k) ey[DQ(b, i, j, k)] = n[DQ(b, i, j, k)] - n[DQ(b, i, j + 1, k)];"),
+    ("DEC d1: eta face sign", "return (ex[DQ(b, i, j, k + 1)] - ex[DQ(b, i, j, k)]) + (ez[DQ(b, i, j, k)] - ez[DQ(b, i + 1, j, k)]);",
+     "return (ex[DQ(b, i, j, k + 1)] - ex[DQ(b, i, j, k)]) + (ez[DQ(b, i + 1, j, k)] - ez[DQ(b, i, j, k)]);"),
+    ("DEC d1: owned range of xi faces", "    for (int i = 0; i <= b.m[0]; ++i)\n        for (int j = 0; j < b.m[1]; ++j)\n            for (int k = 0; k < b.m[2]; ++k) {\n                const double v = dec_d1_xi",
+     "    for (int i = 0; i < b.m[0]; ++i)\n        for (int j = 0; j < b.m[1]; ++j)\n            for (int k = 0; k < b.m[2]; ++k) {\n                const double v = dec_d1_xi"),
+    ("DEC faraday: plus sign", "if (update) fz[DQ(b, i, j, k)] -= dt * v;", "if (update) fz[DQ(b, i, j, k)] += dt * v;"),
+    ("DEC d2: zeta term dropped", "(fz[DQ(b, i, j, k + 1)] - fz[DQ(b, i, j, k)]);\n}", "0.0;\n}"),
 ]
 
 

@@ -34,6 +34,12 @@ def test_mercury_keys_reach_the_physics():
     # IMF of INITIAL in units of B_ref = 1 nT
     assert math.isclose(P.B_imf[0], -15.2, rel_tol=1e-12)
     assert u.is_Mercury_resistance and u.implicit_resistive_rtol == 5.0e-3
+    # the intrinsic dipole (num_of_dipole 1, LIST Dipole1): r0 and a over L_ref,
+    # the moment over 4 pi B_ref L_ref^3 / mu0 (B_ref = 1 nT here)
+    D = params.dipole_array(u)
+    assert D.shape == (1, 7)
+    assert D[0, 2] == 4.88e5 / 2.44e6 and D[0, 6] == 5.0e2 / 2.44e6
+    assert math.isclose(D[0, 5], -2.83e19 * 1.256637061e-6 / (4 * math.pi * 1e-9 * 2.44e6 ** 3), rel_tol=1e-9)
 
 
 def test_parser_rules(tmp_path):
