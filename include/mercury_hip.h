@@ -338,6 +338,16 @@ int mercury_dec_max_abs_div_curl(const int *dims, int nghost, const double *cons
 /* DEC::max_abs_curl_grad(FieldBlock&...) (DecDiagnostics.h:11, DecDiagnostics.cpp:71-93):
  * max over owned faces of |d1 d0 node|, to one device double, as above. */
 int mercury_dec_max_abs_curl_grad(const int *dims, int nghost, const double *node, double *result, void *stream);
+/* mercury_dec_max_abs_div_curl without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The partial maxima go to a static device array of the library (1 MiB per
+ * GPU, held until the process ends), so a block may have at most 33 554 432
+ * owned cells (one partial per 256 cells); larger blocks fail with a message.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (two
+ * streams, two threads) race and must be serialised by the caller. */
+int mercury_dec_max_abs_div_curl_noalloc(const int *dims, int nghost, const double *const *edge, double *result,
+                                         void *stream);
 
 #ifdef __cplusplus
 }

@@ -2,6 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// partial maxima the no-allocation diagnostics can hold: one per workgroup of
+// 256 points, so at most 33 554 432 points (1 MiB of scratch)
+#define MERC_DEC_SCRATCH_DOUBLES 131072
+
 namespace merc {
 
 void dec_check_box(const int *dims, int nghost);
@@ -15,5 +19,7 @@ long long dec_curl_grad_groups(const int *dims);
 // partial: dec_*_groups(dims) doubles; out: one device double
 void dec_div_curl(const int *dims, int g, const double *const *edge, double *partial, double *out, hipStream_t st);
 void dec_curl_grad(const int *dims, int g, const double *node, double *partial, double *out, hipStream_t st);
+// this GPU's copy of the static scratch array (set up on the first call per device)
+double *dec_noalloc_scratch();
 
 }  // namespace merc

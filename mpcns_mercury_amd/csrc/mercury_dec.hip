@@ -18,11 +18,19 @@
 // partial starts from 0.0 and drops NaN the way std::max(acc, NaN) does).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
 #include "mercury_dec.h"
 #include "mercury_device.h"
+
+// Scratch of the no-allocation diagnostics: the partial maxima of up to
+// MERC_DEC_SCRATCH_DOUBLES workgroups (1 MiB), one copy per GPU for the life
+// of the process.  Host code never passes the array's name to a kernel: it
+// uses the device address hipGetSymbolAddress returns for the current device
+// (dec_noalloc_scratch below).
+static __device__ __align__(256) double scratch[MERC_DEC_SCRATCH_DOUBLES];
 
 namespace {
 using merc_dev::dmax;
@@ -166,6 +174,10 @@ __global__ void __launch_bounds__(256) k_fold_max(const double *__restrict__ par
     if (threadIdx.x == 0) out[0] = dmax(dmax(lds[0], lds[1]), dmax(lds[2], lds[3]));
 }
 
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
 DBox make_box(const int *dims, int g, int n0, int n1, int n2) {
     DBox b;
     b.m0 = dims[0];
@@ -238,6 +250,23 @@ void dec_curl_grad(const int *dims, int g, const double *node, double *partial, 
     hipLaunchKernelGGL(k_curl_grad, dim3(ng), dim3(256), 0, st, b, node, partial);
     hipLaunchKernelGGL(k_fold_max, dim3(1), dim3(256), 0, st, partial, (int)ng, out);
     DEC_HIPCHK(hipGetLastError());
+}
+
+// Device address of this GPU's copy of the scratch array, looked up once per
+// device with that device current.
+double *dec_noalloc_scratch() {
+    int dev = -1;
+    DEC_HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("DEC: device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("DEC: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    return g_scratch_ptr[dev];
 }
 
 }  // namespace merc

@@ -2279,6 +2279,21 @@ int mercury_dec_max_abs_curl_grad(const int *dims, int nghost, const double *nod
     })
 }
 
+// the same diagnostic with its partial maxima in the library's static scratch
+// array: no allocation, at most MERC_DEC_SCRATCH_DOUBLES workgroups
+int mercury_dec_max_abs_div_curl_noalloc(const int *dims, int nghost, const double *const *edge, double *result, void *stream) {
+    ABI_TRY({
+        if (!edge) throw std::runtime_error("DEC: edge is NULL");
+        dec_args(dims, nghost, {edge[0], edge[1], edge[2], result});
+        const long long ng = merc::dec_div_curl_groups(dims);
+        if (ng > MERC_DEC_SCRATCH_DOUBLES)
+            throw std::runtime_error("mercury_dec_max_abs_div_curl_noalloc: " + std::to_string((long long)dims[0] * dims[1] * dims[2]) +
+                                     " cells exceed the scratch limit of " + std::to_string(256LL * MERC_DEC_SCRATCH_DOUBLES) +
+                                     " cells per block");
+        merc::dec_div_curl(dims, nghost, edge, merc::dec_noalloc_scratch(), result, (hipStream_t)stream);
+    })
+}
+
 int mercury_synchronize(MercuryContext *ctx) {
     CHECK_CTX(ctx);
     ABI_TRY({ HIPCHK(hipStreamSynchronize(ctx->s->st)); })

@@ -28,7 +28,7 @@ ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercu
                "mercury_write_tecplot", "mercury_case_read", "mercury_case_destroy", "mercury_case_blocks",
                "mercury_case_block_name", "mercury_case_nodes", "mercury_case_topology", "mercury_case_interfaces",
                "mercury_case_physical", "mercury_dec_d0", "mercury_dec_d1", "mercury_dec_d2", "mercury_dec_faraday_update",
-               "mercury_dec_max_abs_div_curl", "mercury_dec_max_abs_curl_grad")
+               "mercury_dec_max_abs_div_curl", "mercury_dec_max_abs_curl_grad", "mercury_dec_max_abs_div_curl_noalloc")
 
 
 class MercuryError(RuntimeError):
@@ -99,6 +99,7 @@ def load():
     L.mercury_dec_faraday_update.argtypes = [ip, ctypes.c_int, pp, pp, ctypes.c_double, vp]
     L.mercury_dec_max_abs_div_curl.argtypes = [ip, ctypes.c_int, pp, vp, vp]
     L.mercury_dec_max_abs_curl_grad.argtypes = [ip, ctypes.c_int, vp, vp, vp]
+    L.mercury_dec_max_abs_div_curl_noalloc.argtypes = [ip, ctypes.c_int, pp, vp, vp]
     _lib = L
     return L
 
