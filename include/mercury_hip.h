@@ -70,6 +70,13 @@ typedef struct MercuryPhysics {
      * - tanh((r - r_outer) / width)) (MercurySolver_Tools.cpp:50-69); 1 an exponential
      * inver_Rem * exp(-(r - r_inner) / width) (no reference counterpart) */
     double resist_profile;
+    /* ion species: 2 = H+ and Na+, the reference's two fluids
+     * (Z4_Mercury/include/0_MercuryFieldCatalog.h:77-82); 3 adds He++ (no reference
+     * counterpart) with its molar mass, state coefficient R T_ref / (M U_ref^2),
+     * charge number (its charge density Z n joins n_e, the U+ weights, the Hall and
+     * induction forces), far-field state and wall density floor.  Its state is the
+     * field U_He (PV_He, RHS_He); sums over species add the He++ term last. */
+    double n_species, M_He, state_coeff_He, Z_He, qinf_He[5], wall_rho_floor_He;
 } MercuryPhysics;
 
 typedef struct MercuryContext MercuryContext;

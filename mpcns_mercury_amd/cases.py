@@ -139,13 +139,15 @@ def set_resistivity(units: MercuryUnits, inver_Rem: float, r_inner: float, r_out
 
 def make_periodic_cube(n=(48, 48, 48), g: int = 4, seed: int = 1, amp: float = 1e-3,
                        integrator: int = 0, steps: int = 100, hall: bool = False,
-                       resist: float | None = None) -> Case:
+                       resist: float | None = None, he: bool = False) -> Case:
     """One block mapped from the unit cube by x += 0.05 sin(2 pi y) sin(2 pi z),
     periodic on every axis; rho = 1, p = 0.1, B = (1,0,0) + curl of a random edge
     potential, seeded velocity perturbations; the Na+ fluid sits at rho_small.
     Hall is off (taper radii beyond the domain) unless ``hall``.  ``resist``
     switches the interior resistivity on with that inver_Rem on the window
-    r in [0.5, 0.9] (width 0.05), which crosses the periodic seams."""
+    r in [0.5, 0.9] (width 0.05), which crosses the periodic seams.  ``he`` adds
+    a He++ fluid at 4 % of the H+ number density with its own seeded velocity
+    perturbations."""
     mx, my, mz = n
     nodes = _cube_nodes(n, (0, 0, 0), n)
     units = MercuryUnits().alfven_units()
@@ -157,12 +159,14 @@ def make_periodic_cube(n=(48, 48, 48), g: int = 4, seed: int = 1, amp: float = 1
         units.r_min, units.r_max = 100.0, 100.0
     if resist is not None:
         set_resistivity(units, resist, 0.5, 0.9, 0.05)
+    units.he_species = he
     P = derive(units)
-    fields, A = _cube_state(n, (0, 0, 0), n, g, seed, amp, P.gamma)
+    fields, A = _cube_state(n, (0, 0, 0), n, g, seed, amp, P.gamma, he_frac=units.he_number_fraction if he else None,
+                            he_mass_ratio=units.mole_mass_he / units.mole_mass1)
     blk = Block(dims=(mx, my, mz), bc=(BC_PERIODIC,) * 6, nodes=nodes)
     return Case(name=f"periodic_cube_{mx}x{my}x{mz}" + ("_resist" if resist is not None else ""), g=g, blocks=[blk],
                 phys=P, fields=[fields], A=[A], Bu=(1.0, 0.0, 0.0), steps=steps, integrator=integrator,
-                meta={"seed": seed, "species": 2, "hall": hall, "resist": resist})
+                meta={"seed": seed, "species": 3 if he else 2, "hall": hall, "resist": resist})
 
 
 def _cube_nodes(n, origin, dims):
@@ -175,7 +179,7 @@ def _cube_nodes(n, origin, dims):
     return np.stack([x, y, z]).astype(np.float64)
 
 
-def _cube_state(n, origin, dims, g, seed, amp, gamma):
+def _cube_state(n, origin, dims, g, seed, amp, gamma, he_frac=None, he_mass_ratio=4.0):
     """Seeded state and edge potential on the box of the sub-block at origin,
     keyed by the wrapped global index (identical values at identical global
     positions whatever the block split)."""
@@ -191,7 +195,11 @@ def _cube_state(n, origin, dims, g, seed, amp, gamma):
     UNa = _conserved(RHO_SMALL * rho, du[0], du[1], du[2], RHO_SMALL * p, gamma)
     h = 1.0 / mx
     A = np.stack([amp * h * uniform(seed, 10 + a, key) for a in range(3)])
-    return {"U_H": UH, "U_Na": UNa, "Photo_rate": np.zeros((1,) + shape)}, A
+    out = {"U_H": UH, "U_Na": UNa, "Photo_rate": np.zeros((1,) + shape)}
+    if he_frac is not None:
+        dv = [amp * uniform(seed, 30 + s, key) for s in range(3)]
+        out["U_He"] = _conserved(he_frac * he_mass_ratio * rho, dv[0], dv[1], dv[2], he_frac * p, gamma)
+    return out, A
 
 
 def split_block_case(case: Case, cuts, name: str | None = None) -> Case:
@@ -322,7 +330,7 @@ def sector_nodes(dims, panel, r0=1.0, r1=4.0, half_angle=np.deg2rad(40.0)):
 def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, integrator: int = 0,
                scale: float = 1.0, hall_coef: float | None = None, resist: float | None = None,
                inflow: float = 1.0, imf=(0.0, 0.0, -0.2), na_fraction: float = 0.01, panels=None,
-               name: str | None = None, resist_profile: str = "window", dipole: bool = False) -> Case:
+               name: str | None = None, resist_profile: str = "window", dipole: bool = False, he: bool = False) -> Case:
     """Two-fluid Hall MHD: H+ and Na+ (mass ratio 1:23, density ratio 1:0.01) plus
     electron pressure (ambipolar term), inflow u = (-1,0,0), IMF B = (0,0,-0.2),
     the fluid side of a perfectly conducting sphere at r = 1 on every block's
@@ -331,6 +339,8 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
     reference's window (r_inner 0.84, r_outer 1.04, width 0.02,
     Z4_Mercury/9999setup/3_MHD_Control.txt:12-22).
 
+    ``he`` adds a He++ fluid (solar-wind alphas, 4 % of the H+ number density,
+    the far-field state of ``derive``) with the same seeded density noise.
     ``dipole`` adds Mercury's intrinsic dipole of the reference's example setup
     (0, 0, 4.88e5 m; m = (0, 0, -2.83e19) A m^2; a = 500 m;
     Z4_Mercury/9999setup/3_MHD_Control.txt:1-9) to Badd.
@@ -356,6 +366,7 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
         units.mercury_eta_profile = resist_profile
     if dipole:
         units.dipoles = [dict(MERCURY_DIPOLE)]
+    units.he_species = he
     P = derive(units)
     gamma = P.gamma
     blks, fields = [], []
@@ -380,12 +391,19 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
         cc[:, g:g + dims[0], g:g + dims[1], g:g + dims[2]] = ccore
         r = np.sqrt((cc * cc).sum(axis=0))
         photo = np.where(r > 1.0, 1e-2 * np.exp(-(r - 1.0) / 0.2), 0.0)
-        fields.append({"U_H": UH, "U_Na": UNa, "Photo_rate": photo[None]})
+        fl = {"U_H": UH, "U_Na": UNa, "Photo_rate": photo[None]}
+        if he:
+            q = P.qinf_He
+            rhoHe = q[0] * (1.0 + 1e-3 * uniform(seed, 21, key))
+            pHe = (q[4] - 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0]) * (P.gamma - 1.0)
+            fl["U_He"] = _conserved(rhoHe, np.full(shape, q[1] / q[0]), np.full(shape, q[2] / q[0]), np.full(shape, q[3] / q[0]),
+                                    np.full(shape, pHe), gamma)
+        fields.append(fl)
     if name is None:
         name = "shell6" if scale == 1.0 else f"shell6_x{scale:g}"
     return Case(name=name + ("_resist" if resist is not None else ""), g=g, blocks=blks, phys=P, fields=fields,
                 A=None, Bu=(0.0, 0.0, 0.0), steps=steps, integrator=integrator,
-                meta={"seed": seed, "species": 2, "hall": True, "resist": resist, "dipole": dipole},
+                meta={"seed": seed, "species": 3 if he else 2, "hall": True, "resist": resist, "dipole": dipole},
                 dipoles=dipole_array(units) if dipole else None)
 
 
@@ -452,17 +470,20 @@ C5_DIMS = (256, 192, 384)
 
 
 def make_resistive_shell(scale: float = 1.0, g: int = 4, steps: int = 500, integrator: int = 0,
-                         inver_Rem: float = 1e-3) -> Case:
+                         inver_Rem: float = 1e-3, species: int = 3) -> Case:
     """The resistive shell configuration: one shell sector of 256 x 192 x 384 cells
     (times ``scale``) on r in [1, 4], cut into 2 x 2 blocks across the two angular
-    axes and glued by interfaces, two-fluid Hall MHD (the reference carries two ion
-    species), with the implicit resistive correction on every edge of the shell
+    axes and glued by interfaces, Hall MHD with H+, Na+ and (``species`` = 3, the
+    default) He++, with the implicit resistive correction on every edge of the shell
     for eta(r) = inver_Rem * exp(-(r - 1) / 0.1) (``resist_profile`` "exp").  At
     scale 1: 18 874 368 cells in four blocks of 256 x 96 x 192."""
     dims = tuple(max(2 * g, 2 * int(round(d * scale / 2))) for d in C5_DIMS)
     dims = (max(g, int(round(C5_DIMS[0] * scale))),) + dims[1:]
+    if species not in (2, 3):
+        raise ValueError("species must be 2 or 3")
     base = make_shell(blocks=(dims,), panels=[0], g=g, steps=steps, integrator=integrator, resist=inver_Rem,
-                      resist_profile="exp", name="resistive_shell_{}x{}x{}".format(*dims))
+                      resist_profile="exp", name="resistive_shell_{}x{}x{}".format(*dims) + ("_he" if species == 3 else ""),
+                      he=species == 3)
     cuts = ((dims[0],), (dims[1] // 2, dims[1] - dims[1] // 2), (dims[2] // 2, dims[2] - dims[2] // 2))
     return split_block_case(base, cuts, name=base.name + "_2x2")
 

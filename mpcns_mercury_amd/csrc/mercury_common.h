@@ -46,8 +46,18 @@ enum MercSlot {
     // cell: neutral Na density, a static input that the step does not read
     // (the reference's "Na" field, written by the output)
     SL_NANEU = SL_PEN + 1,
-    SL_NSLOT = SL_NANEU + 1
+    // the third ion species (He++): state, PV, RHS and step-start copy
+    SL_UHE = SL_NANEU + 1, SL_PVHE = SL_UHE + 5, SL_RHE = SL_PVHE + 5, SL_U0HE = SL_RHE + 5,
+    SL_NSLOT = SL_U0HE + 5
 };
+
+// species order H+, Na+, He++ (the reference's two first: every sum over
+// species adds the He++ term last, so two-species runs are unchanged)
+#define MERC_NSMAX 3
+#define SPEC_U(s) ((s) == 0 ? SL_UH : ((s) == 1 ? SL_UN : SL_UHE))
+#define SPEC_PV(s) ((s) == 0 ? SL_PVH : ((s) == 1 ? SL_PVN : SL_PVHE))
+#define SPEC_R(s) ((s) == 0 ? SL_RH : ((s) == 1 ? SL_RN : SL_RHE))
+#define SPEC_U0(s) ((s) == 0 ? SL_U0H : ((s) == 1 ? SL_U0N : SL_U0HE))
 
 // physics constants (same layout as MercuryPhysics in include/mercury_hip.h)
 struct MercPhys {
@@ -73,6 +83,8 @@ struct MercPhys {
     double resist_enabled, inver_Rem, resist_r_inner, resist_r_outer, resist_width;
     double resist_rtol, resist_atol, resist_max_it;
     double resist_profile;  // 0: the reference's tanh window, 1: exponential
+    // ion species: 2 (H+, Na+) or 3 with He++ (no reference counterpart)
+    double n_species, M_He, state_coeff_He, Z_He, qinf_He[5], wall_rho_floor_He;
 };
 
 // per-block geometry of the box

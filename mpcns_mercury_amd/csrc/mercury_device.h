@@ -71,29 +71,58 @@ DEVI const gfloat *const *gslots(const float *const *G) { return static_cast<con
 #define OJ(p) ((long long)(p).sJ)
 #define AX(p, a) ((a) == 0 ? OI(p) : ((a) == 1 ? OJ(p) : 1LL))
 
+// n: number densities, q: charge densities Z n (q = n for the singly charged
+// H+ and Na+), chi = q / n_e (the weights of U+), w = q / ne_eff
 struct Num {
-    double nH, nNa, ne, ne_eff, chiH, chiNa, wH, wNa;
+    double n[MERC_NSMAX], q[MERC_NSMAX], ne, ne_eff, chi[MERC_NSMAX], w[MERC_NSMAX];
 };
 
-// Hall_Num_Limiter (Z4_Mercury/src/physics/MercurySolver_Tools.cpp:111-157)
-DEVI Num num_limiter(const MercPhys &P, double rhoH, double rhoNa) {
+// ion species of a run: 2 (H+, Na+) or 3 (with He++)
+DEVI int nspec(const MercPhys &P) { return P.n_species >= 3.0 ? 3 : 2; }
+
+// Hall_Num_Limiter (Z4_Mercury/src/physics/MercurySolver_Tools.cpp:111-157); with
+// He++ its charge density Z_He n_He joins n_e last (no reference counterpart)
+DEVI Num num_limiter(const MercPhys &P, double rhoH, double rhoNa, double rhoHe = 0.0, int ns = 2) {
     Num n;
-    const double rp = dmax(rhoH, 0.0), rn = dmax(rhoNa, 0.0);
-    n.nH = rp / P.M_H * P.M_ref;
-    n.nNa = rn / P.M_Na * P.M_ref;
-    n.ne = n.nH + n.nNa;
-    n.ne_eff = sqrt(n.ne * n.ne + P.ne_floor * P.ne_floor);
-    if (n.ne > 1.0e-300) {
-        n.chiH = n.nH / n.ne;
-        n.chiNa = n.nNa / n.ne;
-    } else {
-        n.chiH = 1.0;
-        n.chiNa = 0.0;
+    n.n[0] = dmax(rhoH, 0.0) / P.M_H * P.M_ref;
+    n.n[1] = dmax(rhoNa, 0.0) / P.M_Na * P.M_ref;
+    n.q[0] = n.n[0];
+    n.q[1] = n.n[1];
+    n.ne = n.q[0] + n.q[1];
+    n.n[2] = n.q[2] = 0.0;
+    if (ns > 2) {
+        n.n[2] = dmax(rhoHe, 0.0) / P.M_He * P.M_ref;
+        n.q[2] = P.Z_He * n.n[2];
+        n.ne = n.ne + n.q[2];
     }
-    n.wH = n.nH / n.ne_eff;
-    n.wNa = n.nNa / n.ne_eff;
+    n.ne_eff = sqrt(n.ne * n.ne + P.ne_floor * P.ne_floor);
+    n.chi[2] = n.w[2] = 0.0;
+    if (n.ne > 1.0e-300) {
+        for (int s = 0; s < ns; ++s) n.chi[s] = n.q[s] / n.ne;
+    } else {
+        n.chi[0] = 1.0;
+        for (int s = 1; s < ns; ++s) n.chi[s] = 0.0;
+    }
+    for (int s = 0; s < ns; ++s) n.w[s] = n.q[s] / n.ne_eff;
     return n;
 }
+
+// the limiter at one point from the state slots
+template <typename FT> DEVI Num num_at(const MercPhys &P, const FT &F, long long q, int ns) {
+    return num_limiter(P, F[SL_UH][q], F[SL_UN][q], ns > 2 ? (double)F[SL_UHE][q] : 0.0, ns);
+}
+
+// electron pressure stand-in of the ambipolar terms: the species pressures'
+// sum at a cell (p_H + p_Na in the reference, + p_He with He++)
+template <typename FT> DEVI double pe_sum(const FT &F, long long q, int ns) {
+    double pe = F[SL_PVH + 3][q] + F[SL_PVN + 3][q];
+    if (ns > 2) pe = pe + F[SL_PVHE + 3][q];
+    return pe;
+}
+
+// molar mass and state coefficient of species s
+DEVI double species_mass(const MercPhys &P, int s) { return s == 0 ? P.M_H : (s == 1 ? P.M_Na : P.M_He); }
+DEVI double species_coeff(const MercPhys &P, int s) { return s == 0 ? P.state_coeff_H : (s == 1 ? P.state_coeff_Na : P.state_coeff_He); }
 
 DEVI void rusanov_face(const gdouble *__restrict__ const *F, int su, int sp, long long qL, long long qR, long long qf, int a,
                        double gm, double out[5]) {

@@ -10,11 +10,13 @@ void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *
 void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                      unsigned long long *dt_next, int compute_dt, bool mix, int hall64 = 0);
 // ambi: also the node electron pressure of the ambipolar EMF (read by k_edge_total)
-void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix, int ambi);
+// ns: ion species of the run (2 or 3)
+void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix, int ambi,
+                    int ns);
 void launch_edge_total(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                        int with_hall, bool mix);
 void launch_update_all(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const double *dt,
-                       bool mix);
+                       bool mix, int ns);
 void launch_to_double(hipStream_t st, const float *src, double *dst, long long n);
 void launch_bcell_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);
 void launch_jcell_t(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);

@@ -110,7 +110,7 @@ DEVI void rusanov_face_t(gdouble *const *__restrict__ F, const gfloat *const *__
 // next one (the same face evaluated once, with the same operands), and the
 // i-neighbour planes are read while they are still in cache.  The launch range
 // has ceil(m0 / L) points along i per block.
-template <bool MIX>
+template <bool MIX, int NS>
 __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
                            const float *const *__restrict__ G_, const MercPhys P, int L) {
     gdouble *const *__restrict__ F = gslots(F_);
@@ -123,14 +123,14 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
     const long long q0 = p.q + (long long)(i0 - p.i) * di;
     const T gm = (T)P.gamma;
     // --- Scheme_U_ (MercurySolver_FluidRHS.cpp:3-202)
-    T Fx[2][5];  // i-1/2 flux of the current cell, per species
-    for (int s = 0; s < 2; ++s) rusanov_face_t<MIX>(F, G, s ? SL_UN : SL_UH, s ? SL_PVN : SL_PVH, q0 - di, q0, q0, 0, gm, Fx[s]);
+    T Fx[NS][5];  // i-1/2 flux of the current cell, per species
+    for (int s = 0; s < NS; ++s) rusanov_face_t<MIX>(F, G, SPEC_U(s), SPEC_PV(s), q0 - di, q0, q0, 0, gm, Fx[s]);
     for (int i = i0; i < i1; ++i) {
         const long long q = q0 + (long long)(i - i0) * di;
         const T invJ = (T)(1.0 / F[SL_JAC][q]);
-        double rr[2][5];
-        for (int s = 0; s < 2; ++s) {
-            const int su = s ? SL_UN : SL_UH, sp = s ? SL_PVN : SL_PVH;
+        double rr[NS][5];
+        for (int s = 0; s < NS; ++s) {
+            const int su = SPEC_U(s), sp = SPEC_PV(s);
             Acc<MIX> r[5];
             {
                 T Fp[5];
@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
         const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
         const double Jx = F[SL_JC][q], Jy = F[SL_JC + 1][q], Jz = F[SL_JC + 2][q];
         const double sjbx = Jy * Bz - Jz * By, sjby = Jz * Bx - Jx * Bz, sjbz = Jx * By - Jy * Bx;
-        const Num num = num_limiter(P, F[SL_UH][q], F[SL_UN][q]);
+        const Num num = num_at(P, F, q, NS);
         double dpex = 0.0, dpey = 0.0, dpez = 0.0;
         const bool ambi = P.ambipolar_enabled != 0.0;
         if (ambi) {
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
                     d[6 + c] = 0.5 * (F[SL_JD + 6 + c][q - 1] + F[SL_JD + 6 + c][q]) / V;
                 }
             }
-    #define PE(o) (F[SL_PVH + 3][q + (o)] + F[SL_PVN + 3][q + (o)])
+    #define PE(o) pe_sum(F, q + (o), NS)
             const double pip = PE(di), pim = PE(-di), pjp = PE(dj), pjm = PE(-dj), pkp = PE(1), pkm = PE(-1);
     #undef PE
             dpex = 0.5 * (d[0] * (pip - pim) + d[3] * (pjp - pjm) + d[6] * (pkp - pkm));
@@ -190,8 +190,8 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
         const double rho_cyl = sqrt(y * y + z * z);
         const bool in_shadow = (x < 0.0 && rho_cyl < 1.0);
         const double vst = in_shadow ? P.nu_shadow : P.nu_lit;
-        for (int s = 0; s < 2; ++s) {
-            const int su = s ? SL_UN : SL_UH, sp = s ? SL_PVN : SL_PVH, sr = s ? SL_RN : SL_RH;
+        for (int s = 0; s < NS; ++s) {
+            const int su = SPEC_U(s), sp = SPEC_PV(s), sr = SPEC_R(s);
             const double rho = dmax(F[su][q], 0.0);
             const double u = st<MIX>(F, G, sp, q), v = st<MIX>(F, G, sp + 1, q), w = st<MIX>(F, G, sp + 2, q);
             const double u2 = u * u + v * v + w * w;
@@ -202,24 +202,25 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
             const double sjbu = sjbx * u + sjby * v + sjbz * w;
             const double dpeu = dpex * u + dpey * v + dpez * w;
             const double Ts0 = (double)st<MIX>(F, G, sp + 4, q) * P.T_ref;
-            const double Ms = s ? P.M_Na : P.M_H;
+            const double Ms = species_mass(P, s);
             const double Mo = P.M_Na;
             const double b1 = vst * (P.Tn0 - Ts0) * (Ms / (Ms + Mo));
             const double b2 = (Mo / (Ms + Mo)) * vst;
-            const double ns = s ? num.nNa : num.nH;
-            const double chi = s ? num.wNa : num.wH;
+            // number density in the thermal exchange, charge density in the induction force
+            const double nsp = num.n[s], qs = num.q[s];
+            const double chi = num.w[s];
             double r0 = rr[s][0], r1 = rr[s][1], r2 = rr[s][2], r3 = rr[s][3], r4 = rr[s][4];
-            r1 += P.c_ind * ns * subx + P.c_hall * chi * sjbx;
-            r2 += P.c_ind * ns * suby + P.c_hall * chi * sjby;
-            r3 += P.c_ind * ns * subz + P.c_hall * chi * sjbz;
-            r4 += P.c_ind * ns * subu + P.c_hall * chi * sjbu;
+            r1 += P.c_ind * qs * subx + P.c_hall * chi * sjbx;
+            r2 += P.c_ind * qs * suby + P.c_hall * chi * sjby;
+            r3 += P.c_ind * qs * subz + P.c_hall * chi * sjbz;
+            r4 += P.c_ind * qs * subu + P.c_hall * chi * sjbu;
             if (ambi) {
                 r1 -= chi * dpex;
                 r2 -= chi * dpey;
                 r3 -= chi * dpez;
                 r4 -= chi * dpeu;
             }
-            if (s) {
+            if (s == 1) {
                 r0 += P.a1_Na * sss;
                 r4 += P.a6 * sss * P.Tn0;
             }
@@ -227,7 +228,7 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
             r2 -= P.a4 * rho * v * vst;
             r3 -= P.a4 * rho * w * vst;
             r4 -= P.a4 * rho * u2 * vst;
-            r4 += P.a5 * ns * b1;
+            r4 += P.a5 * nsp * b1;
             r4 += P.a4 * rho * u2 * b2;
             wr<MIX>(F, G, sr, q, r0); wr<MIX>(F, G, sr + 1, q, r1); wr<MIX>(F, G, sr + 2, q, r2);
             wr<MIX>(F, G, sr + 3, q, r3); wr<MIX>(F, G, sr + 4, q, r4);
@@ -243,7 +244,7 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
 // Six waves per SIMD: the FP64 body fits in 80 VGPRs without spilling (102
 // unconstrained, four waves), and the latency-bound pass runs 10 % faster
 // (398 -> 359 us on shell6); seven waves would spill.
-template <bool MIX>
+template <bool MIX, int NS>
 __global__ void __attribute__((amdgpu_waves_per_eu(6))) k_cell_end(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
                            const float *const *__restrict__ G_, const MercPhys P, unsigned long long *__restrict__ dt_bits,
                            int compute_dt, int hall64) {
@@ -256,10 +257,10 @@ __global__ void __attribute__((amdgpu_waves_per_eu(6))) k_cell_end(const MercRan
     if (locate(R, B, TID, p)) {
         const MercBlk &bk = B[p.b];
         const long long q = p.q;
-        double pv[2][5];
-        for (int s = 0; s < 2; ++s) {
-            const int su = s ? SL_UN : SL_UH;
-            const double coeff = s ? P.state_coeff_Na : P.state_coeff_H;
+        double pv[NS][5];
+        for (int s = 0; s < NS; ++s) {
+            const int su = SPEC_U(s);
+            const double coeff = species_coeff(P, s);
             const double rho = dmax(F[su][q], 1e-12);
             const double u = F[su + 1][q] / rho, v = F[su + 2][q] / rho, w = F[su + 3][q] / rho;
             const double E = F[su + 4][q];
@@ -269,21 +270,23 @@ __global__ void __attribute__((amdgpu_waves_per_eu(6))) k_cell_end(const MercRan
             double pr = (P.gamma - 1.0) * eint;
             pr = dmax(pr, 1e-12);
             pv[s][0] = u; pv[s][1] = v; pv[s][2] = w; pv[s][3] = pr; pv[s][4] = pr / (rho * coeff);
-            const int sp = s ? SL_PVN : SL_PVH;
+            const int sp = SPEC_PV(s);
             for (int c = 0; c < 5; ++c) {
                 if (c == 3) F[sp + 3][q] = pv[s][3];
                 else wr<MIX>(F, G, sp + c, q, pv[s][c]);
             }
         }
-        const double rH = dmax(F[SL_UH][q], 0.0), rN = dmax(F[SL_UN][q], 0.0);
+        double r[NS];
+        for (int s = 0; s < NS; ++s) r[s] = dmax(F[SPEC_U(s)][q], 0.0);
         // one limiter serves U+ (floored densities) and the Hall terms (raw
         // densities, Hall.cpp:92-95): it floors its inputs at 0 itself
-        const Num n = num_limiter(P, rH, rN);
+        const Num n = num_limiter(P, r[0], r[1], NS > 2 ? r[NS - 1] : 0.0, NS);
         double up[3];
         for (int c = 0; c < 3; ++c) {
-            const double uH = (rH > 1e-20) ? pv[0][c] : 0.0;
-            const double uN = (rN > 1e-20) ? pv[1][c] : 0.0;
-            up[c] = n.chiH * uH + n.chiNa * uN;
+            const double uH = (r[0] > 1e-20) ? pv[0][c] : 0.0;
+            const double uN = (r[1] > 1e-20) ? pv[1][c] : 0.0;
+            up[c] = n.chi[0] * uH + n.chi[1] * uN;
+            if constexpr (NS > 2) up[c] = up[c] + n.chi[2] * ((r[2] > 1e-20) ? pv[2][c] : 0.0);
             wr<MIX>(F, G, SL_UP + c, q, up[c]);
         }
         const Num &nh = n;
@@ -329,9 +332,10 @@ __global__ void __attribute__((amdgpu_waves_per_eu(6))) k_cell_end(const MercRan
                 // stays an exact reduction of FP64 values)
                 const T bx = (T)Bx, by = (T)By, bz = (T)Bz, gam = (T)P.gamma, mi = (T)P.inver_MA2;
                 const T B2 = bx * bx + by * by + bz * bz;
-                T rho[2], ux[2], uy[2], uz[2], cs2[2], vA2[2], denom[2] = {T(0), T(0)};
-                for (int s = 0; s < 2; ++s) {
-                    const int su = s ? SL_UN : SL_UH;
+                T rho[NS], ux[NS], uy[NS], uz[NS], cs2[NS], vA2[NS], denom[NS];
+                for (int s = 0; s < NS; ++s) {
+                    const int su = SPEC_U(s);
+                    denom[s] = T(0);
                     rho[s] = tmax((T)F[su][q], T(1e-2));
                     ux[s] = PR::div((T)F[su + 1][q], rho[s]);
                     uy[s] = PR::div((T)F[su + 2][q], rho[s]);
@@ -348,14 +352,13 @@ __global__ void __attribute__((amdgpu_waves_per_eu(6))) k_cell_end(const MercRan
                         const T ax = (T)F[SL_JD + 3 * a][fq], ay = (T)F[SL_JD + 3 * a + 1][fq], az = (T)F[SL_JD + 3 * a + 2][fq];
                         const T A = st<MIX>(F, G, SL_AREA + a, fq);
                         if (A <= T(0)) {
-                            denom[0] += T(0);
-                            denom[1] += T(0);
+                            for (int s = 0; s < NS; ++s) denom[s] += T(0);
                             continue;
                         }
                         T nx = PR::div(ax, A), ny = PR::div(ay, A), nz = PR::div(az, A);
                         if (!sd) { nx = -nx; ny = -ny; nz = -nz; }
                         const T Bn = bx * nx + by * ny + bz * nz;
-                        for (int s = 0; s < 2; ++s) {
+                        for (int s = 0; s < NS; ++s) {
                             const T un = ux[s] * nx + uy[s] * ny + uz[s] * nz;
                             const T vAn2 = PR::div(mi * (Bn * Bn), rho[s]);
                             const T t = cs2[s] + vA2[s];
@@ -365,7 +368,7 @@ __global__ void __attribute__((amdgpu_waves_per_eu(6))) k_cell_end(const MercRan
                             denom[s] += (fabs(un) + cf) * A;
                         }
                     }
-                for (int s = 0; s < 2; ++s)
+                for (int s = 0; s < NS; ++s)
                     if (denom[s] > T(0)) best = dmin(best, P.cfl * V / (double)denom[s]);
             }
         }
@@ -407,7 +410,7 @@ DEVI typename Prec<MIX>::T emf_flux_u(gdouble *const *__restrict__ F, const gflo
 // per edge end
 template <bool MIX>
 __global__ void k_eface_u(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
-                          const float *const *__restrict__ G_, int ambi) {
+                          const float *const *__restrict__ G_, int ambi, int ns) {
     gdouble *const *__restrict__ F = gslots(F_);
     const gfloat *const *__restrict__ G = gslots(G_);
     using T = typename Prec<MIX>::T;
@@ -421,7 +424,9 @@ __global__ void k_eface_u(const MercRange R, const MercBlk *__restrict__ B, doub
             for (int ddj = -1; ddj <= 0; ++ddj)
                 for (int ddk = -1; ddk <= 0; ++ddk) {
                     const long long c = p.q + ddi * di + ddj * dj + ddk;
-                    sum += (T)F[SL_PVH + 3][c] + (T)F[SL_PVN + 3][c];
+                    T pe = (T)F[SL_PVH + 3][c] + (T)F[SL_PVN + 3][c];
+                    if (ns > 2) pe = pe + (T)F[SL_PVHE + 3][c];
+                    sum += pe;
                 }
         wr<MIX>(F, G, SL_PEN, p.q, T(0.125) * sum);
     }
@@ -518,15 +523,17 @@ __global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, d
         if (ambi) {
             const long long ea = AX(p, a);
             const T pe[2] = {st<MIX>(F, G, SL_PEN, q), st<MIX>(F, G, SL_PEN, q + ea)};  // node electron pressure (k_eface_u)
-            double rH = 0.0, rN = 0.0;
+            const int ns = nspec(P);
+            double rH = 0.0, rN = 0.0, rHe = 0.0;
             const long long t1 = (a == 0) ? dj : di, t2 = (a == 2) ? dj : 1;
             for (int d1 = -1; d1 <= 0; ++d1)
                 for (int d2 = -1; d2 <= 0; ++d2) {
                     const long long c = q + d1 * t1 + d2 * t2;
                     rH += F[SL_UH][c];
                     rN += F[SL_UN][c];
+                    if (ns > 2) rHe += F[SL_UHE][c];
                 }
-            const Num n = num_limiter(P, 0.25 * rH, 0.25 * rN);
+            const Num n = num_limiter(P, 0.25 * rH, 0.25 * rN, 0.25 * rHe, ns);
             E -= PR::div((T)P.pressure_coef * (pe[1] - pe[0]), (T)n.ne_eff);
         }
         if (arti) {
@@ -615,7 +622,7 @@ __global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, d
 // (the RHS comes from the FP32 table in the mixed form; the accumulation is FP64)
 template <bool MIX>
 __global__ void k_update_all(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
-                             const float *const *__restrict__ G_, const double *__restrict__ dtp) {
+                             const float *const *__restrict__ G_, const double *__restrict__ dtp, int ns) {
     gdouble *const *__restrict__ F = gslots(F_);
     const gfloat *const *__restrict__ G = gslots(G_);
     LOCATE(R);
@@ -627,12 +634,9 @@ __global__ void k_update_all(const MercRange R, const MercBlk *__restrict__ B, d
         for (int m = 0; m < 5; ++m) {
             // the fluid RHS is read here only, RHS_B written here and only downloaded:
             // non-temporal, so they do not displace reused data from the caches
-            if constexpr (MIX) {
-                F[SL_UH + m][q] += dt * (double)__builtin_nontemporal_load(&G[SL_RH + m][q]);
-                F[SL_UN + m][q] += dt * (double)__builtin_nontemporal_load(&G[SL_RN + m][q]);
-            } else {
-                F[SL_UH + m][q] += dt * __builtin_nontemporal_load(&F[SL_RH + m][q]);
-                F[SL_UN + m][q] += dt * __builtin_nontemporal_load(&F[SL_RN + m][q]);
+            for (int s = 0; s < ns; ++s) {
+                if constexpr (MIX) F[SPEC_U(s) + m][q] += dt * (double)__builtin_nontemporal_load(&G[SPEC_R(s) + m][q]);
+                else F[SPEC_U(s) + m][q] += dt * __builtin_nontemporal_load(&F[SPEC_R(s) + m][q]);
             }
         }
     if (p.j < bk.m[1] && p.k < bk.m[2]) {
@@ -778,18 +782,31 @@ static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1
 
 void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                      int L, bool mix) {
-    if (mix) LAUNCH_R(k_rhs_cell<true>, R, B, F, G, P, L);
-    else LAUNCH_R(k_rhs_cell<false>, R, B, F, G, P, L);
+    const bool three = P.n_species >= 3.0;
+    if (mix) {
+        if (three) LAUNCH_R((k_rhs_cell<true, 3>), R, B, F, G, P, L);
+        else LAUNCH_R((k_rhs_cell<true, 2>), R, B, F, G, P, L);
+    } else {
+        if (three) LAUNCH_R((k_rhs_cell<false, 3>), R, B, F, G, P, L);
+        else LAUNCH_R((k_rhs_cell<false, 2>), R, B, F, G, P, L);
+    }
 }
 void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                      unsigned long long *dt_next, int compute_dt, bool mix, int hall64) {
     if (compute_dt) hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, st, dt_next);
-    if (mix) LAUNCH_R(k_cell_end<true>, R, B, F, G, P, dt_next, compute_dt, hall64);
-    else LAUNCH_R(k_cell_end<false>, R, B, F, G, P, dt_next, compute_dt, hall64);
+    const bool three = P.n_species >= 3.0;
+    if (mix) {
+        if (three) LAUNCH_R((k_cell_end<true, 3>), R, B, F, G, P, dt_next, compute_dt, hall64);
+        else LAUNCH_R((k_cell_end<true, 2>), R, B, F, G, P, dt_next, compute_dt, hall64);
+    } else {
+        if (three) LAUNCH_R((k_cell_end<false, 3>), R, B, F, G, P, dt_next, compute_dt, hall64);
+        else LAUNCH_R((k_cell_end<false, 2>), R, B, F, G, P, dt_next, compute_dt, hall64);
+    }
 }
-void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix, int ambi) {
-    if (mix) LAUNCH_R(k_eface_u<true>, R, B, F, G, ambi);
-    else LAUNCH_R(k_eface_u<false>, R, B, F, G, ambi);
+void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix, int ambi,
+                    int ns) {
+    if (mix) LAUNCH_R(k_eface_u<true>, R, B, F, G, ambi, ns);
+    else LAUNCH_R(k_eface_u<false>, R, B, F, G, ambi, ns);
 }
 void launch_edge_total(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                        int with_hall, bool mix) {
@@ -797,9 +814,9 @@ void launch_edge_total(hipStream_t st, const RangeH &R, const MercBlk *B, double
     else LAUNCH_R(k_edge_total<false>, R, B, F, G, P, with_hall);
 }
 void launch_update_all(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const double *dt,
-                       bool mix) {
-    if (mix) LAUNCH_R(k_update_all<true>, R, B, F, G, dt);
-    else LAUNCH_R(k_update_all<false>, R, B, F, G, dt);
+                       bool mix, int ns) {
+    if (mix) LAUNCH_R(k_update_all<true>, R, B, F, G, dt, ns);
+    else LAUNCH_R(k_update_all<false>, R, B, F, G, dt, ns);
 }
 void launch_to_double(hipStream_t st, const float *src, double *dst, long long n) {
     if (n > 0) hipLaunchKernelGGL(k_to_double, dim3(nblk(n, 256)), dim3(256), 0, st, src, dst, n);

@@ -63,6 +63,8 @@ static const FieldSpec kFields[] = {
     {"Jac", SL_JAC, 1, LOC_CELL}, {"Bcell_from_Bface_w", SL_WB, 18, LOC_CELL}, {"Jcell_from_Jedge_w", SL_WJ, 36, LOC_CELL},
     {"Photo_rate", SL_PHOTO, 1, LOC_CELL}, {"cell_center", SL_CC, 3, LOC_CELL}, {"divB", SL_DIVB, 1, LOC_CELL},
     {"Na", SL_NANEU, 1, LOC_CELL},
+    // the third ion species (He++), no reference counterpart
+    {"U_He", SL_UHE, 5, LOC_CELL}, {"PV_He", SL_PVHE, 5, LOC_CELL}, {"RHS_He", SL_RHE, 5, LOC_CELL},
     {"cell_scalar", SL_CELLS, 1, LOC_CELL},
     {"Eres_xi", SL_ERES, 1, LOC_EX}, {"Eres_eta", SL_ERES + 1, 1, LOC_EY}, {"Eres_zeta", SL_ERES + 2, 1, LOC_EZ},
     {"dJ_xi", SL_DJ, 1, LOC_EX}, {"dJ_eta", SL_DJ + 1, 1, LOC_EY}, {"dJ_zeta", SL_DJ + 2, 1, LOC_EZ},
@@ -267,7 +269,7 @@ struct SyncOp {
     int phase = 0;
     std::vector<int> slots;
     std::vector<double> fillv;
-    int is_na = 0;
+    int is_na = 0;  // WALLF: species (0 H+, 1 Na+, 2 He++)
     std::vector<int> hidx;     // pairs for COPY, (first, deeper, ghosts) for WALLF
     std::vector<double> hval;  // SETV values
 };
@@ -278,7 +280,7 @@ struct GroupProg {
 
 // A merged launch: every op of one type in one phase of a set of independent
 // groups. Entries name their slot set: COPY (dst, src, set), FILL (dst, set),
-// SETV (dst, slot) + value, WALLF (slot, is_na, first, deeper, g ghosts).
+// SETV (dst, slot) + value, WALLF (slot, species, first, deeper, g ghosts).
 struct MergedOp {
     OpType type;
     MultiSets S{};
@@ -292,7 +294,7 @@ struct MergedProg {
 
 enum GroupId { G_UCELL = 0, G_BFACE, G_EEDGE, G_EHALL, G_JEDGE, G_BCELL, G_JCELL, G_BADD, G_ERES, G_ERES0, G_DB, G_NGROUP };
 // K_ERES0: Eres with a zero far field (the linear part of the resistive operator)
-enum Kind { K_UH, K_UNA, K_BFACE, K_EEDGE, K_EHALL, K_JEDGE, K_BCELL, K_BIND, K_JCELL, K_BADD, K_ERES, K_ERES0, K_DB };
+enum Kind { K_UH, K_UNA, K_BFACE, K_EEDGE, K_EHALL, K_JEDGE, K_BCELL, K_BIND, K_JCELL, K_BADD, K_ERES, K_ERES0, K_DB, K_UHE };
 
 struct GField {
     int kind, slot, ncomp, loc, level, owner_sync;
@@ -300,10 +302,13 @@ struct GField {
 
 // sync groups (Z4_Mercury/src/boundary/Boundary_Build.cpp:564-703) with each
 // field's halo level and owner policy (0_MercuryFieldCatalog.h:77-178)
-static std::vector<GField> group_fields(int gid) {
+static std::vector<GField> group_fields(int gid, int ns) {
     std::vector<GField> v;
     switch (gid) {
-    case G_UCELL: v = {{K_UH, SL_UH, 5, LOC_CELL, 1, 0}, {K_UNA, SL_UN, 5, LOC_CELL, 1, 0}}; break;
+    case G_UCELL:
+        v = {{K_UH, SL_UH, 5, LOC_CELL, 1, 0}, {K_UNA, SL_UN, 5, LOC_CELL, 1, 0}};
+        if (ns > 2) v.push_back({K_UHE, SL_UHE, 5, LOC_CELL, 1, 0});  // He++ (no reference counterpart)
+        break;
     case G_BFACE: for (int a = 0; a < 3; ++a) v.push_back({K_BFACE, SL_B + a, 1, LOC_FX + a, 3, 1}); break;
     case G_BADD: for (int a = 0; a < 3; ++a) v.push_back({K_BADD, SL_BADD + a, 1, LOC_FX + a, 3, 1}); break;
     case G_EEDGE: for (int a = 0; a < 3; ++a) v.push_back({K_EEDGE, SL_E + a, 1, LOC_EX + a, 3, 1}); break;
@@ -616,6 +621,8 @@ class MercurySolver {
 public:
     std::unique_ptr<Field> F;
     MercPhys P{};
+    // ion species of the run: 2 (H+, Na+) or 3 (He++ added)
+    int nspecies() const { return P.n_species >= 3.0 ? 3 : 2; }
     std::vector<std::unique_ptr<BlockGeom>> geom;
     RangeOwner R_cell, R_box, R_node, R_eface;
     RangeOwner R_res;  // node-box points around the resistive unknowns (the CG's curls)
@@ -739,8 +746,8 @@ public:
         // FP32 derived intermediates of the mixed form (written by its passes):
         // PV u, v, w, T (the pressure stays FP64), U+, Hall cell terms,
         // contravariant U+, face EMFs, fluid RHS
-        for (int sp : {SL_PVH, SL_PVN})
-            for (int c : {0, 1, 2, 4}) islots.push_back(sp + c);
+        for (int s = 0; s < nspecies(); ++s)
+            for (int c : {0, 1, 2, 4}) islots.push_back(SPEC_PV(s) + c);
         for (int c = 0; c < 3; ++c) islots.push_back(SL_UP + c);
         for (int c = 0; c < 3; ++c) islots.push_back(SL_EHC + c);
         islots.push_back(SL_HB);
@@ -748,6 +755,8 @@ public:
         for (int c = 0; c < 9; ++c) islots.push_back(SL_EF + c);
         for (int c = 0; c < 5; ++c) islots.push_back(SL_RH + c);
         for (int c = 0; c < 5; ++c) islots.push_back(SL_RN + c);
+        if (nspecies() > 2)
+            for (int c = 0; c < 5; ++c) islots.push_back(SL_RHE + c);
         islots.push_back(SL_PEN);
         const size_t nf = fslots.size() + islots.size();
         HIPCHK(hipMalloc(&farena, sizeof(float) * F->Vtot * nf));
@@ -873,7 +882,7 @@ public:
                 if (fill) o->fillv.assign(fill, fill + slots.size());
                 ph.push_back(std::move(o));
             };
-            for (const GField &gf : group_fields(gid)) {
+            for (const GField &gf : group_fields(gid, nspecies())) {
                 const auto sl = slots_of(gf);
                 std::vector<int> fillA, copyB, setd, wallC, zeroC, copyD, haloH, ownE;
                 std::vector<double> setv;
@@ -887,6 +896,7 @@ public:
                         switch (gf.kind) {
                         case K_UH: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = P.qinf; break;
                         case K_UNA: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = P.qinfs; break;
+                        case K_UHE: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = P.qinf_He; break;
                         case K_BFACE: break;
                         case K_BCELL: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = P.B_imf; break;
                         case K_BIND: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = zero3; break;
@@ -915,7 +925,7 @@ public:
                         if (k.bc[f] != MBC_WALL) continue;
                         const int dir = face_dir(f);
                         const int ax = std::abs(dir) - 1, sgn = dir > 0 ? 1 : -1;
-                        if (gf.kind == K_UH || gf.kind == K_UNA) {
+                        if (gf.kind == K_UH || gf.kind == K_UNA || gf.kind == K_UHE) {
                             // BC_Solid_Surface_ (Boundary_Condition_Fluid.cpp:36-260): (first, deeper, ghosts)
                             int lo[3], hi[3];
                             for (int a = 0; a < 3; ++a) { lo[a] = -k.g; hi[a] = k.m[a] + k.g; }
@@ -950,7 +960,7 @@ public:
                 if (!fillA.empty()) add(A, OP_FILL, sl, std::move(fillA), {}, fill);
                 if (!setd.empty()) add(A, OP_SETV, sl, std::move(setd), std::move(setv));
                 add(Bc, OP_COPY, sl, std::move(copyB));
-                if (!wallC.empty()) add(C, OP_WALLF, sl, std::move(wallC), {}, nullptr, gf.kind == K_UNA);
+                if (!wallC.empty()) add(C, OP_WALLF, sl, std::move(wallC), {}, nullptr, gf.kind == K_UNA ? 1 : (gf.kind == K_UHE ? 2 : 0));
                 if (!zeroC.empty()) {
                     static const double z[10] = {0};
                     add(C, OP_FILL, sl, std::move(zeroC), {}, z);
@@ -1291,12 +1301,12 @@ public:
         } else {
             launch_rhs_cell(st, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
         }
-        launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix(), P.ambipolar_enabled != 0.0);
+        launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix(), P.ambipolar_enabled != 0.0, nspecies());
         if (alias.n == 0) {
             launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, 1, mix());
             run_merged(MG_EDGE_PAIR);
             if (overlap) HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
-            launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix());
+            launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix(), nspecies());
             after_update(final);
             return;
         } else {
@@ -1309,7 +1319,7 @@ public:
         }
         sync(G_EEDGE);
         if (overlap) HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
-        launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix());
+        launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix(), nspecies());
         after_update(final);
     }
 
@@ -1345,7 +1355,7 @@ public:
         launch_sources(st, R_cell.h, Bd(), Fd(), P);
         induction_emf();
         launch_curl_update(st, R_node.h, Bd(), Fd(), dtp(), 1);
-        launch_update_u(st, R_cell.h, Bd(), Fd(), dtp());
+        launch_update_u(st, R_cell.h, Bd(), Fd(), dtp(), nspecies());
         run_merged(MG_STATE_PAIR);
         if (res_now) res_solve(res_dt);
         derived();
@@ -1371,10 +1381,10 @@ public:
             if (integrator == 0) {
                 stage_fused(true);
             } else {
-                launch_rk2(st, R_node.h, Bd(), Fd(), 1);
+                launch_rk2(st, R_node.h, Bd(), Fd(), 1, nspecies());
                 stage_fused(false);
                 stage_fused(false);
-                launch_rk2(st, R_node.h, Bd(), Fd(), 0);
+                launch_rk2(st, R_node.h, Bd(), Fd(), 0, nspecies());
                 res_now = resist;
                 after_update(true);
             }
@@ -1384,10 +1394,10 @@ public:
         if (integrator == 0) {
             stage();
         } else {
-            launch_rk2(st, R_node.h, Bd(), Fd(), 1);
+            launch_rk2(st, R_node.h, Bd(), Fd(), 1, nspecies());
             stage();
             stage();
-            launch_rk2(st, R_node.h, Bd(), Fd(), 0);
+            launch_rk2(st, R_node.h, Bd(), Fd(), 0, nspecies());
             run_merged(MG_STATE_PAIR);
             if (resist) res_solve(res_dt);
             derived();
@@ -1558,9 +1568,9 @@ public:
         else if (op == "stage") stage();
         else if (op == "rhs_cell") launch_rhs_cell(st, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
         else if (op == "cell_end") launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix(), hall64());
-        else if (op == "eface_u") launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix(), P.ambipolar_enabled != 0.0);
+        else if (op == "eface_u") launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix(), P.ambipolar_enabled != 0.0, nspecies());
         else if (op == "edge_total") launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, alias.n == 0 ? 1 : 0, mix());
-        else if (op == "update_all") launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix());
+        else if (op == "update_all") launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix(), nspecies());
         else if (op == "bcell_jedge") launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix());
         else if (op == "jcell_t") launch_jcell_t(st, R_cell.h, Bd(), Fd(), Gd(), mix());
         else if (op == "sync_all") {
@@ -1794,6 +1804,9 @@ int mercury_write_restart(MercuryContext *ctx, const char *path, int step, doubl
         HIPCHK(hipStreamSynchronize(S.st));
         const merc::Field &F = *S.F;
         std::vector<std::string> names(std::begin(kRestartFields), std::end(kRestartFields));
+        // a three-species run adds its He++ state after the whitelist (the reader
+        // takes any field it knows by name)
+        if (S.nspecies() > 2) names.insert(names.begin() + 2, "U_He");
         if (with_jedge) names.insert(names.end(), std::begin(kJedgeFields), std::end(kJedgeFields));
         std::vector<merc::io::RestartField> out;
         for (const std::string &nm : names) {

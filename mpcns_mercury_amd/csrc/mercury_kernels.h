@@ -25,8 +25,9 @@ void launch_hall_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double 
 void launch_hall_edge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
 void launch_add_hall_wall(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
 void launch_curl_update(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const double *dt, int do_update);
-void launch_update_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const double *dt);
-void launch_rk2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int save);
+// ns: ion species of the run (2 or 3)
+void launch_update_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const double *dt, int ns);
+void launch_rk2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int save, int ns);
 void launch_dec_d0(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int s_node, int s_edge);
 void launch_dec_d1(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int s_edge, int s_face, double mult);
 void launch_dec_d2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int s_face, int s_cell);

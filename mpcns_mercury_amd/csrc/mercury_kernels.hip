@@ -31,10 +31,11 @@ __global__ void k_pv_uplus(const MercRange R, const MercBlk *__restrict__ B, dou
     gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const long long q = p.q;
-    double pv[2][5];
-    for (int s = 0; s < 2; ++s) {
-        const int su = s ? SL_UN : SL_UH;
-        const double coeff = s ? P.state_coeff_Na : P.state_coeff_H;
+    const int ns = nspec(P);
+    double pv[MERC_NSMAX][5];
+    for (int s = 0; s < ns; ++s) {
+        const int su = SPEC_U(s);
+        const double coeff = species_coeff(P, s);
         const double rho = dmax(F[su][q], 1e-12);
         const double u = F[su + 1][q] / rho, v = F[su + 2][q] / rho, w = F[su + 3][q] / rho;
         const double E = F[su + 4][q];
@@ -44,15 +45,18 @@ __global__ void k_pv_uplus(const MercRange R, const MercBlk *__restrict__ B, dou
         double pr = (P.gamma - 1.0) * eint;
         pr = dmax(pr, 1e-12);
         pv[s][0] = u; pv[s][1] = v; pv[s][2] = w; pv[s][3] = pr; pv[s][4] = pr / (rho * coeff);
-        const int sp = s ? SL_PVN : SL_PVH;
+        const int sp = SPEC_PV(s);
         for (int c = 0; c < 5; ++c) F[sp + c][q] = pv[s][c];
     }
-    const double rH = dmax(F[SL_UH][q], 0.0), rN = dmax(F[SL_UN][q], 0.0);
-    const Num n = num_limiter(P, rH, rN);
+    double r[MERC_NSMAX] = {0.0, 0.0, 0.0};
+    for (int s = 0; s < ns; ++s) r[s] = dmax(F[SPEC_U(s)][q], 0.0);
+    const Num n = num_limiter(P, r[0], r[1], r[2], ns);
     for (int c = 0; c < 3; ++c) {
-        const double uH = (rH > 1e-20) ? pv[0][c] : 0.0;
-        const double uN = (rN > 1e-20) ? pv[1][c] : 0.0;
-        F[SL_UP + c][q] = n.chiH * uH + n.chiNa * uN;
+        const double uH = (r[0] > 1e-20) ? pv[0][c] : 0.0;
+        const double uN = (r[1] > 1e-20) ? pv[1][c] : 0.0;
+        double up = n.chi[0] * uH + n.chi[1] * uN;
+        if (ns > 2) up = up + n.chi[2] * ((r[2] > 1e-20) ? pv[2][c] : 0.0);
+        F[SL_UP + c][q] = up;
     }
 }
 
@@ -147,8 +151,8 @@ __global__ void k_timestep(const MercRange R, const MercBlk *__restrict__ B, dou
         if (!(V <= 0.0)) {
             const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
             const double B2 = Bx * Bx + By * By + Bz * Bz;
-            for (int s = 0; s < 2; ++s) {
-                const int su = s ? SL_UN : SL_UH;
+            for (int s = 0; s < nspec(P); ++s) {
+                const int su = SPEC_U(s);
                 const double rho = dmax(F[su][q], 1e-2);
                 const double ux = F[su + 1][q] / rho, uy = F[su + 2][q] / rho, uz = F[su + 3][q] / rho;
                 const double E = F[su + 4][q];
@@ -207,8 +211,8 @@ __global__ void k_flux_div(const MercRange R, const MercBlk *__restrict__ B, dou
     const long long q = p.q;
     const double gm = P.gamma;
     const double invJ = 1.0 / F[SL_JAC][q];
-    for (int s = 0; s < 2; ++s) {
-        const int su = s ? SL_UN : SL_UH, sp = s ? SL_PVN : SL_PVH, sr = s ? SL_RN : SL_RH;
+    for (int s = 0; s < nspec(P); ++s) {
+        const int su = SPEC_U(s), sp = SPEC_PV(s), sr = SPEC_R(s);
         double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         for (int a = 0; a < 3; ++a) {
             const long long d = AX(p, a);
@@ -231,7 +235,8 @@ __global__ void k_sources(const MercRange R, const MercBlk *__restrict__ B, doub
     const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
     const double Jx = F[SL_JC][q], Jy = F[SL_JC + 1][q], Jz = F[SL_JC + 2][q];
     const double sjbx = Jy * Bz - Jz * By, sjby = Jz * Bx - Jx * Bz, sjbz = Jx * By - Jy * Bx;
-    const Num num = num_limiter(P, F[SL_UH][q], F[SL_UN][q]);
+    const int ns = nspec(P);
+    const Num num = num_at(P, F, q, ns);
     double dpex = 0.0, dpey = 0.0, dpez = 0.0;
     const bool ambi = P.ambipolar_enabled != 0.0;
     if (ambi) {
@@ -246,7 +251,7 @@ __global__ void k_sources(const MercRange R, const MercBlk *__restrict__ B, doub
                 d[6 + c] = 0.5 * (F[SL_JD + 6 + c][q - 1] + F[SL_JD + 6 + c][q]) / V;
             }
         }
-#define PE(o) (F[SL_PVH + 3][q + (o)] + F[SL_PVN + 3][q + (o)])
+#define PE(o) pe_sum(F, q + (o), ns)
         const double pip = PE(di), pim = PE(-di), pjp = PE(dj), pjm = PE(-dj), pkp = PE(1), pkm = PE(-1);
 #undef PE
         dpex = 0.5 * (d[0] * (pip - pim) + d[3] * (pjp - pjm) + d[6] * (pkp - pkm));
@@ -258,8 +263,8 @@ __global__ void k_sources(const MercRange R, const MercBlk *__restrict__ B, doub
     const double rho_cyl = sqrt(y * y + z * z);
     const bool in_shadow = (x < 0.0 && rho_cyl < 1.0);
     const double vst = in_shadow ? P.nu_shadow : P.nu_lit;
-    for (int s = 0; s < 2; ++s) {
-        const int su = s ? SL_UN : SL_UH, sp = s ? SL_PVN : SL_PVH, sr = s ? SL_RN : SL_RH;
+    for (int s = 0; s < ns; ++s) {
+        const int su = SPEC_U(s), sp = SPEC_PV(s), sr = SPEC_R(s);
         const double rho = dmax(F[su][q], 0.0);
         const double u = F[sp][q], v = F[sp + 1][q], w = F[sp + 2][q];
         const double u2 = u * u + v * v + w * w;
@@ -270,24 +275,25 @@ __global__ void k_sources(const MercRange R, const MercBlk *__restrict__ B, doub
         const double sjbu = sjbx * u + sjby * v + sjbz * w;
         const double dpeu = dpex * u + dpey * v + dpez * w;
         const double Ts0 = F[sp + 4][q] * P.T_ref;
-        const double Ms = s ? P.M_Na : P.M_H;
+        const double Ms = species_mass(P, s);
         const double Mo = P.M_Na;
         const double b1 = vst * (P.Tn0 - Ts0) * (Ms / (Ms + Mo));
         const double b2 = (Mo / (Ms + Mo)) * vst;
-        const double ns = s ? num.nNa : num.nH;
-        const double chi = s ? num.wNa : num.wH;
+        // number density in the thermal exchange, charge density in the induction force
+        const double nsp = num.n[s], qs = num.q[s];
+        const double chi = num.w[s];
         double r0 = F[sr][q], r1 = F[sr + 1][q], r2 = F[sr + 2][q], r3 = F[sr + 3][q], r4 = F[sr + 4][q];
-        r1 += P.c_ind * ns * subx + P.c_hall * chi * sjbx;
-        r2 += P.c_ind * ns * suby + P.c_hall * chi * sjby;
-        r3 += P.c_ind * ns * subz + P.c_hall * chi * sjbz;
-        r4 += P.c_ind * ns * subu + P.c_hall * chi * sjbu;
+        r1 += P.c_ind * qs * subx + P.c_hall * chi * sjbx;
+        r2 += P.c_ind * qs * suby + P.c_hall * chi * sjby;
+        r3 += P.c_ind * qs * subz + P.c_hall * chi * sjbz;
+        r4 += P.c_ind * qs * subu + P.c_hall * chi * sjbu;
         if (ambi) {
             r1 -= chi * dpex;
             r2 -= chi * dpey;
             r3 -= chi * dpez;
             r4 -= chi * dpeu;
         }
-        if (s) {
+        if (s == 1) {
             r0 += P.a1_Na * sss;
             r4 += P.a6 * sss * P.Tn0;
         }
@@ -295,7 +301,7 @@ __global__ void k_sources(const MercRange R, const MercBlk *__restrict__ B, doub
         r2 -= P.a4 * rho * v * vst;
         r3 -= P.a4 * rho * w * vst;
         r4 -= P.a4 * rho * u2 * vst;
-        r4 += P.a5 * ns * b1;
+        r4 += P.a5 * nsp * b1;
         r4 += P.a4 * rho * u2 * b2;
         F[sr][q] = r0; F[sr + 1][q] = r1; F[sr + 2][q] = r2; F[sr + 3][q] = r3; F[sr + 4][q] = r4;
     }
@@ -357,6 +363,7 @@ __global__ void k_edge_emf(const MercRange R, const MercBlk *__restrict__ B, dou
     const double Cuct = 0.5;
     const bool ambi = P.ambipolar_enabled != 0.0;
     const bool arti = P.arti_eta_max > 0.0;
+    const int ns = nspec(P);
     const bool local = P.local_enabled != 0.0 && P.local_eta_max > 0.0 && P.local_r_decay > 0.0;
     for (int a = 0; a < 3; ++a) {
         if ((a == 0 && p.i >= bk.m[0]) || (a == 1 && p.j >= bk.m[1]) || (a == 2 && p.k >= bk.m[2])) continue;
@@ -393,19 +400,20 @@ __global__ void k_edge_emf(const MercRange R, const MercBlk *__restrict__ B, dou
                     for (int ddj = -1; ddj <= 0; ++ddj)
                         for (int ddk = -1; ddk <= 0; ++ddk) {
                             const long long c = n0 + ddi * di + ddj * dj + ddk;
-                            sum += F[SL_PVH + 3][c] + F[SL_PVN + 3][c];
+                            sum += pe_sum(F, c, ns);
                         }
                 pe[e] = 0.125 * sum;
             }
-            double rH = 0.0, rN = 0.0;
+            double rH = 0.0, rN = 0.0, rHe = 0.0;
             const long long t1 = (a == 0) ? dj : di, t2 = (a == 2) ? dj : 1;
             for (int d1 = -1; d1 <= 0; ++d1)
                 for (int d2 = -1; d2 <= 0; ++d2) {
                     const long long c = q + d1 * t1 + d2 * t2;
                     rH += F[SL_UH][c];
                     rN += F[SL_UN][c];
+                    if (ns > 2) rHe += F[SL_UHE][c];
                 }
-            const Num n = num_limiter(P, 0.25 * rH, 0.25 * rN);
+            const Num n = num_limiter(P, 0.25 * rH, 0.25 * rN, 0.25 * rHe, ns);
             E -= P.pressure_coef * (pe[1] - pe[0]) / n.ne_eff;
         }
         if (arti) {
@@ -435,7 +443,7 @@ __global__ void k_hall_cell(const MercRange R, const MercBlk *__restrict__ B, do
     gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const long long q = p.q;
-    const Num n = num_limiter(P, F[SL_UH][q], F[SL_UN][q]);
+    const Num n = num_at(P, F, q, nspec(P));
     const double alpha = P.hall_coef / n.ne_eff;
     const double Jx = F[SL_JC][q], Jy = F[SL_JC + 1][q], Jz = F[SL_JC + 2][q];
     const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
@@ -550,20 +558,19 @@ __global__ void k_curl_update(const MercRange R, const MercBlk *__restrict__ B, 
 }
 
 // cell part of ApplyUpdate_Euler_ (MercurySolver_Advance.cpp:73-121)
-__global__ void k_update_u(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, const double *__restrict__ dtp) {
+__global__ void k_update_u(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, const double *__restrict__ dtp,
+                           int ns) {
     gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const long long q = p.q;
     const double dt = *dtp;
     if (dt == 0.0) return;  // the run is stopped (k_dt_commit)
-    for (int m = 0; m < 5; ++m) {
-        F[SL_UH + m][q] += dt * F[SL_RH + m][q];
-        F[SL_UN + m][q] += dt * F[SL_RN + m][q];
-    }
+    for (int m = 0; m < 5; ++m)
+        for (int s = 0; s < ns; ++s) F[SPEC_U(s) + m][q] += dt * F[SPEC_R(s) + m][q];
 }
 
 // SSP-RK2: save the step-start state / average 0.5*(U0 + U) on owned DOFs
-__global__ void k_rk2(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, int save) {
+__global__ void k_rk2(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_, int save, int ns) {
     gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const MercBlk &bk = B[p.b];
@@ -572,18 +579,14 @@ __global__ void k_rk2(const MercRange R, const MercBlk *__restrict__ B, double *
     const bool face[3] = {p.j < bk.m[1] && p.k < bk.m[2], p.i < bk.m[0] && p.k < bk.m[2], p.i < bk.m[0] && p.j < bk.m[1]};
     if (save) {
         if (cell)
-            for (int m = 0; m < 5; ++m) {
-                F[SL_U0H + m][q] = F[SL_UH + m][q];
-                F[SL_U0N + m][q] = F[SL_UN + m][q];
-            }
+            for (int m = 0; m < 5; ++m)
+                for (int s = 0; s < ns; ++s) F[SPEC_U0(s) + m][q] = F[SPEC_U(s) + m][q];
         for (int a = 0; a < 3; ++a)
             if (face[a]) F[SL_B0 + a][q] = F[SL_B + a][q];
     } else {
         if (cell)
-            for (int m = 0; m < 5; ++m) {
-                F[SL_UH + m][q] = 0.5 * (F[SL_U0H + m][q] + F[SL_UH + m][q]);
-                F[SL_UN + m][q] = 0.5 * (F[SL_U0N + m][q] + F[SL_UN + m][q]);
-            }
+            for (int m = 0; m < 5; ++m)
+                for (int s = 0; s < ns; ++s) F[SPEC_U(s) + m][q] = 0.5 * (F[SPEC_U0(s) + m][q] + F[SPEC_U(s) + m][q]);
         for (int a = 0; a < 3; ++a)
             if (face[a]) F[SL_B + a][q] = 0.5 * (F[SL_B0 + a][q] + F[SL_B + a][q]);
     }
@@ -619,16 +622,17 @@ __global__ void k_setv_multi(const int *__restrict__ ent, const double *__restri
     F[ent[2 * t + 1]][ent[2 * t]] = val[t];
 }
 // BC_Solid_Surface_ (Boundary_Condition_Fluid.cpp:36-260): per wall cell the
-// entry holds (species base slot, is_na, first layer, one deeper, g ghosts)
+// entry holds (species base slot, species (0 H+, 1 Na+, 2 He++), first layer,
+// one deeper, g ghosts); He++ takes the H+ rule with its own density floor
 __global__ void k_wall_multi(const int *__restrict__ ent, long long n, int g, double *const *__restrict__ F_, const MercPhys P) {
     gdouble *const *__restrict__ F = gslots(F_);
     const long long t = TID;
     if (t >= n) return;
     const int *e = ent + (long long)t * (4 + g);
-    const int slot = e[0], is_na = e[1];
+    const int slot = e[0], species = e[1], is_na = species == 1;
     e += 2;
     const double gm1 = P.gamma - 1.0;
-    const double rho_floor = is_na ? P.wall_rho_floor_Na : P.wall_rho_floor_H;
+    const double rho_floor = species == 1 ? P.wall_rho_floor_Na : (species == 2 ? P.wall_rho_floor_He : P.wall_rho_floor_H);
     const double p_floor = P.wall_p_floor;
     if (is_na) {
         for (int l = 0; l < g; ++l) {
@@ -833,8 +837,10 @@ void launch_add_hall_wall(hipStream_t st, const RangeH &R, const MercBlk *B, dou
 void launch_curl_update(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const double *dt, int do_update) {
     LAUNCH_R(k_curl_update, R, B, F, dt, do_update);
 }
-void launch_update_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const double *dt) { LAUNCH_R(k_update_u, R, B, F, dt); }
-void launch_rk2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int save) { LAUNCH_R(k_rk2, R, B, F, save); }
+void launch_update_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const double *dt, int ns) {
+    LAUNCH_R(k_update_u, R, B, F, dt, ns);
+}
+void launch_rk2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int save, int ns) { LAUNCH_R(k_rk2, R, B, F, save, ns); }
 void launch_dec_d0(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int sn, int se) { LAUNCH_R(k_dec_d0, R, B, F, sn, se); }
 void launch_dec_d1(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int se, int sf, double mult) { LAUNCH_R(k_dec_d1, R, B, F, se, sf, mult); }
 void launch_dec_d2(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int sf, int sc) { LAUNCH_R(k_dec_d2, R, B, F, sf, sc); }

@@ -53,6 +53,9 @@ class MercuryPhysics(ctypes.Structure):
         ("resist_r_inner", ctypes.c_double), ("resist_r_outer", ctypes.c_double), ("resist_width", ctypes.c_double),
         ("resist_rtol", ctypes.c_double), ("resist_atol", ctypes.c_double), ("resist_max_it", ctypes.c_double),
         ("resist_profile", ctypes.c_double),
+        # ion species: 2 (H+, Na+) or 3 with He++ (no reference counterpart)
+        ("n_species", ctypes.c_double), ("M_He", ctypes.c_double), ("state_coeff_He", ctypes.c_double),
+        ("Z_He", ctypes.c_double), ("qinf_He", ctypes.c_double * 5), ("wall_rho_floor_He", ctypes.c_double),
     ]
 
 
@@ -98,6 +101,11 @@ class MercuryUnits:
     implicit_resistive_rtol: float = 1.0e-10
     implicit_resistive_atol: float = 1.0e-30
     implicit_resistive_max_it: int = 500
+    # a third ion species, He++ (an extension; the reference has H+ and Na+): its molar
+    # mass and its share of the inflow's ions by number
+    he_species: bool = False
+    mole_mass_he: float = 0.004
+    he_number_fraction: float = 0.04
     # eta profile: "window" (the reference's, MercurySolver_Tools.cpp:50-69) or "exp",
     # inver_Rem * exp(-(r - r_inner) / width) (an extension for the resistive shell case)
     mercury_eta_profile: str = "window"
@@ -181,6 +189,20 @@ def derive(u: MercuryUnits) -> MercuryPhysics:
     if u.mercury_eta_profile not in ("window", "exp"):
         raise ValueError(f"unknown eta profile {u.mercury_eta_profile!r}")
     P.resist_profile = 1.0 if u.mercury_eta_profile == "exp" else 0.0
+    # He++: state coefficient as for the other species, charge 2; far field at the
+    # inflow velocity and temperature with he_number_fraction of the H+ number density
+    P.n_species = 3.0 if u.he_species else 2.0
+    M_He = u.mole_mass_he
+    P.M_He, P.Z_He = M_He, 2.0
+    P.state_coeff_He = (R_uni * T_ref) / (M_He * U_ref * U_ref)
+    P.wall_rho_floor_He = P.wall_rho_floor_H * (M_He / M_H) * u.he_number_fraction
+    if ini:
+        f = u.he_number_fraction
+        rho_he = f * rho0 * (M_He / ini["Molecular_mass"])
+        p_he = f * p0
+        q = (rho_he, rho_he * u0, rho_he * v0, rho_he * w0, 0.5 * rho_he * (u0 * u0 + v0 * v0 + w0 * w0) + p_he / (gamma - 1.0))
+        for m, v in enumerate(q):
+            P.qinf_He[m] = v
     return P
 
 

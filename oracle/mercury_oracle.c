@@ -54,6 +54,10 @@ typedef struct {
     double resist_enabled, inver_Rem, resist_r_inner, resist_r_outer, resist_width;
     double resist_rtol, resist_atol, resist_max_it;
     double resist_profile; /* 0: the reference's window, 1: exponential (no reference counterpart) */
+    /* ion species: 2 (H+, Na+), the reference's two fluids (Z4_Mercury/include/0_MercuryFieldCatalog.h:77-82),
+     * or 3 with He++ (no reference counterpart): its mass, state coefficient, charge
+     * number, far-field state and wall density floor */
+    double n_species, M_He, state_coeff_He, Z_He, qinf_He[5], wall_rho_floor_He;
 } OrcPhys;
 
 /* face codes; BC_INTERFACE: a 1:1 inner connection to another (or the same)
@@ -86,8 +90,19 @@ enum {
      * solution kept as the next initial guess (edges); dB, RHS_Bres (faces) */
     S_ERES = S_CELLS + 1, S_DJ = S_ERES + 3, S_ETAR = S_DJ + 3, S_XRES = S_ETAR + 3,
     S_DB = S_XRES + 3, S_RHSBR = S_DB + 3,
-    S_NSLOT = S_RHSBR + 3
+    /* the third ion species (He++): state, PV, RHS and step-start copy */
+    S_UHE = S_RHSBR + 3, S_PVHE = S_UHE + 5, S_RHE = S_PVHE + 5, S_U0HE = S_RHE + 5,
+    S_NSLOT = S_U0HE + 5
 };
+
+/* species order H+, Na+, He++: the reference's two species first, so every sum
+ * over species adds the He++ term last and two-species runs are unchanged */
+#define NSMAX 3
+static const int SP_U[NSMAX] = {S_UH, S_UN, S_UHE}, SP_PV[NSMAX] = {S_PVH, S_PVN, S_PVHE}, SP_R[NSMAX] = {S_RH, S_RN, S_RHE},
+                 SP_U0[NSMAX] = {S_U0H, S_U0N, S_U0HE};
+static int n_species(const OrcPhys *P) { return P->n_species >= 3.0 ? 3 : 2; }
+static double species_mass(const OrcPhys *P, int s) { return s == 0 ? P->M_H : (s == 1 ? P->M_Na : P->M_He); }
+static double species_coeff(const OrcPhys *P, int s) { return s == 0 ? P->state_coeff_H : (s == 1 ? P->state_coeff_Na : P->state_coeff_He); }
 
 typedef struct {
     int m[3], g;
@@ -662,10 +677,12 @@ static void farfield_eedge_zero(OBlock *b, const FDesc *fd, int dir) {
 /* MercuryBoundary::BC_Solid_Surface_ (Boundary_Condition_Fluid.cpp:36-260):
  * non-Na species: first active layer <- (rho, p) one cell deeper, zero velocity;
  * ghosts <- first layer.  Na species: ghosts <- floor state. */
-static void wall_fluid(OBlock *b, const OrcPhys *P, int slot, int is_na, int dir) {
+static void wall_fluid(OBlock *b, const OrcPhys *P, int slot, int species, int dir) {
     int ax = abs(dir), sgn = dir > 0 ? 1 : -1;
     int cyc[3] = {ax == 1 ? sgn : 0, ax == 2 ? sgn : 0, ax == 3 ? sgn : 0};
-    const double rho_floor = is_na ? P->wall_rho_floor_Na : P->wall_rho_floor_H;
+    /* He++ takes the H+ rule with its own density floor (no reference counterpart) */
+    const int is_na = species == 1;
+    const double rho_floor = species == 1 ? P->wall_rho_floor_Na : (species == 2 ? P->wall_rho_floor_He : P->wall_rho_floor_H);
     const double p_floor = P->wall_p_floor;
     const double gm1 = P->gamma - 1.0;
     int ilo[3], ihi[3];
@@ -734,18 +751,22 @@ static void wall_jcell(OBlock *b, const FDesc *fd, int dir) {
 enum { G_UCELL, G_BFACE, G_EEDGE, G_EHALL, G_JEDGE, G_BCELL, G_JCELL, G_BADD, G_ERES, G_ERES0, G_DB, G_DJ, G_NGROUP };
 /* field kinds for handler dispatch (K_ERES0: Eres with a zero far-field
  * field, the linear part of the resistive operator) */
-enum { K_UH, K_UNA, K_BFACE, K_EEDGE, K_EHALL, K_JEDGE, K_BCELL, K_BIND, K_JCELL, K_BADD, K_ERES, K_ERES0, K_DB, K_DJ };
+enum { K_UH, K_UNA, K_BFACE, K_EEDGE, K_EHALL, K_JEDGE, K_BCELL, K_BIND, K_JCELL, K_BADD, K_ERES, K_ERES0, K_DB, K_DJ, K_UHE };
 
 typedef struct {
     int kind, slot, ncomp, loc, level; /* level: 1 = Corner1D, 3 = Corner3D */
     int owner_sync;
 } GField;
 
-static int group_fields(int gid, GField out[6]) {
+static int group_fields(const OrcPhys *P, int gid, GField out[6]) {
     int n = 0;
 #define ADD(K, S, NC, L, LV, OS) out[n].kind = K, out[n].slot = S, out[n].ncomp = NC, out[n].loc = L, out[n].level = LV, out[n].owner_sync = OS, ++n
     switch (gid) {
-    case G_UCELL: ADD(K_UH, S_UH, 5, L_CELL, 1, 0); ADD(K_UNA, S_UN, 5, L_CELL, 1, 0); break;
+    case G_UCELL:
+        ADD(K_UH, S_UH, 5, L_CELL, 1, 0);
+        ADD(K_UNA, S_UN, 5, L_CELL, 1, 0);
+        if (n_species(P) > 2) ADD(K_UHE, S_UHE, 5, L_CELL, 1, 0);
+        break;
     case G_BFACE: for (int a = 0; a < 3; ++a) ADD(K_BFACE, S_B + a, 1, L_FX + a, 3, 1); break;
     case G_BADD: for (int a = 0; a < 3; ++a) ADD(K_BADD, S_BADD + a, 1, L_FX + a, 3, 1); break;
     case G_EEDGE: for (int a = 0; a < 3; ++a) ADD(K_EEDGE, S_E + a, 1, L_EX + a, 3, 1); break;
@@ -771,6 +792,7 @@ static void apply_handler(OBlock *b, const OrcPhys *P, const GField *gf, int bc,
         switch (gf->kind) {
         case K_UH: fill_const(b, &fd, dir, P->qinf); break;
         case K_UNA: fill_const(b, &fd, dir, P->qinfs); break;
+        case K_UHE: fill_const(b, &fd, dir, P->qinf_He); break;
         case K_BFACE: break; /* nop */
         case K_BCELL: fill_const(b, &fd, dir, P->B_imf); break;
         case K_BIND: fill_const(b, &fd, dir, zero3); break;
@@ -783,6 +805,7 @@ static void apply_handler(OBlock *b, const OrcPhys *P, const GField *gf, int bc,
         switch (gf->kind) {
         case K_UH: wall_fluid(b, P, gf->slot, 0, dir); break;
         case K_UNA: wall_fluid(b, P, gf->slot, 1, dir); break;
+        case K_UHE: wall_fluid(b, P, gf->slot, 2, dir); break;
         case K_JCELL: wall_jcell(b, &fd, dir); break;
         default: break; /* nop for every other field on the fluid side of the wall */
         }
@@ -899,7 +922,7 @@ static void owner_sync(Orc *o, const GField *gf) {
  * the face/edge form groups. */
 static void sync_group(Orc *o, int gid) {
     GField gf[6];
-    int nf = group_fields(gid, gf);
+    int nf = group_fields(&o->P, gid, gf);
     for (int n = 0; n < nf; ++n)
         for (int ib = 0; ib < o->nb; ++ib) {
             OBlock *b = &o->b[ib];
@@ -1040,37 +1063,53 @@ static void build_classes(Orc *o) {
 /* physics                                                                    */
 /* ------------------------------------------------------------------------- */
 
+/* n: number densities, q: charge densities Z_s n_s (q = n for the singly
+ * charged H+ and Na+), chi = q / n_e (the weights of U+), w = q / ne_eff */
 typedef struct {
-    double nH, nNa, ne, ne_eff, chiH, chiNa, wH, wNa;
+    double n[NSMAX], q[NSMAX], ne, ne_eff, chi[NSMAX], w[NSMAX];
 } NumInfo;
 
-/* Hall_Num_Limiter (MercurySolver_Tools.cpp:111-157) */
-static NumInfo num_limiter(const OrcPhys *P, double rhoH, double rhoNa) {
+/* Hall_Num_Limiter (MercurySolver_Tools.cpp:111-157); with He++ its charge
+ * density Z_He n_He joins n_e (no reference counterpart) */
+static NumInfo num_limiter(const OrcPhys *P, const double *rho, int ns) {
     NumInfo n;
-    const double rhoH_pos = dmax(rhoH, 0.0), rhoNa_pos = dmax(rhoNa, 0.0);
-    n.nH = rhoH_pos / P->M_H * P->M_ref;
-    n.nNa = rhoNa_pos / P->M_Na * P->M_ref;
-    n.ne = n.nH + n.nNa;
+    n.n[0] = dmax(rho[0], 0.0) / P->M_H * P->M_ref;
+    n.n[1] = dmax(rho[1], 0.0) / P->M_Na * P->M_ref;
+    n.q[0] = n.n[0];
+    n.q[1] = n.n[1];
+    n.ne = n.q[0] + n.q[1];
+    n.n[2] = n.q[2] = 0.0;
+    if (ns > 2) {
+        n.n[2] = dmax(rho[2], 0.0) / P->M_He * P->M_ref;
+        n.q[2] = P->Z_He * n.n[2];
+        n.ne = n.ne + n.q[2];
+    }
     n.ne_eff = sqrt(n.ne * n.ne + P->ne_floor * P->ne_floor);
     if (n.ne > 1.0e-300) {
-        n.chiH = n.nH / n.ne;
-        n.chiNa = n.nNa / n.ne;
+        for (int s = 0; s < ns; ++s) n.chi[s] = n.q[s] / n.ne;
     } else {
-        n.chiH = 1.0;
-        n.chiNa = 0.0;
+        n.chi[0] = 1.0;
+        for (int s = 1; s < ns; ++s) n.chi[s] = 0.0;
     }
-    n.wH = n.nH / n.ne_eff;
-    n.wNa = n.nNa / n.ne_eff;
+    for (int s = 0; s < ns; ++s) n.w[s] = n.q[s] / n.ne_eff;
     return n;
+}
+
+/* the species densities of one point */
+static NumInfo num_at(const OrcPhys *P, const OBlock *b, long q) {
+    const int ns = n_species(P);
+    double rho[NSMAX] = {0.0, 0.0, 0.0};
+    for (int s = 0; s < ns; ++s) rho[s] = b->f[SP_U[s]][q];
+    return num_limiter(P, rho, ns);
 }
 
 /* calc_PV (MercurySolver_Tools.cpp:159-213), full allocated range */
 static void calc_pv(Orc *o) {
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
-        for (int s = 0; s < 2; ++s) {
-            int su = s ? S_UN : S_UH, sp = s ? S_PVN : S_PVH;
-            double coeff = s ? o->P.state_coeff_Na : o->P.state_coeff_H;
+        for (int s = 0; s < n_species(&o->P); ++s) {
+            int su = SP_U[s], sp = SP_PV[s];
+            double coeff = species_coeff(&o->P, s);
             PAR1
             for (long q = 0; q < b->V; ++q) {
                 const double rho = dmax(b->f[su][q], 1e-12);
@@ -1097,14 +1136,18 @@ static void calc_uplus(Orc *o) {
         OBlock *b = &o->b[ib];
         PAR1
         for (long q = 0; q < b->V; ++q) {
-            const double rH = dmax(b->f[S_UH][q], 0.0), rN = dmax(b->f[S_UN][q], 0.0);
-            double uH[3], uN[3];
-            for (int c = 0; c < 3; ++c) {
-                uH[c] = (rH > 1e-20) ? b->f[S_PVH + c][q] : 0.0;
-                uN[c] = (rN > 1e-20) ? b->f[S_PVN + c][q] : 0.0;
+            const int ns = n_species(&o->P);
+            double r[NSMAX], us[NSMAX][3];
+            for (int s = 0; s < ns; ++s) {
+                r[s] = dmax(b->f[SP_U[s]][q], 0.0);
+                for (int c = 0; c < 3; ++c) us[s][c] = (r[s] > 1e-20) ? b->f[SP_PV[s] + c][q] : 0.0;
             }
-            NumInfo n = num_limiter(&o->P, rH, rN);
-            for (int c = 0; c < 3; ++c) b->f[S_UP + c][q] = n.chiH * uH[c] + n.chiNa * uN[c];
+            NumInfo n = num_limiter(&o->P, r, ns);
+            for (int c = 0; c < 3; ++c) {
+                double up = n.chi[0] * us[0][c] + n.chi[1] * us[1][c];
+                if (ns > 2) up = up + n.chi[2] * us[2][c];
+                b->f[S_UP + c][q] = up;
+            }
         }
     }
 }
@@ -1225,10 +1268,10 @@ static void update_derived(Orc *o) {
 static double compute_timestep(Orc *o) {
     const OrcPhys *P = &o->P;
     double dt_local = 1e100;
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < n_species(P); ++s)
         for (int ib = 0; ib < o->nb; ++ib) {
             OBlock *b = &o->b[ib];
-            int su = s ? S_UN : S_UH;
+            int su = SP_U[s];
             /* the minimum is exact, so the reduction order does not matter */
 #pragma omp parallel for collapse(2) schedule(static) reduction(min : dt_local)
             for (int i = 0; i < b->m[0]; ++i)
@@ -1280,10 +1323,10 @@ static double compute_timestep(Orc *o) {
  * (ZeroRHS_, MercurySolver_Advance.cpp:4-71). */
 static void scheme_u(Orc *o) {
     const double gm = o->P.gamma;
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < n_species(&o->P); ++s)
         for (int ib = 0; ib < o->nb; ++ib) {
             OBlock *b = &o->b[ib];
-            int su = s ? S_UN : S_UH, sp = s ? S_PVN : S_PVH, sr = s ? S_RN : S_RH;
+            int su = SP_U[s], sp = SP_PV[s], sr = SP_R[s];
             for (int a = 0; a < 3; ++a) {
                 int hi[3];
                 owned_hi(b, L_FX + a, hi);
@@ -1339,6 +1382,14 @@ static void scheme_u(Orc *o) {
         }
 }
 
+/* electron pressure stand-in of the ambipolar terms: the sum of the species
+ * pressures at a cell (p_H + p_Na in the reference, + p_He with He++) */
+static double pe_sum(const OrcPhys *P, const OBlock *b, int i, int j, int k) {
+    double pe = F(b, S_PVH + 3, i, j, k) + F(b, S_PVN + 3, i, j, k);
+    if (n_species(P) > 2) pe = pe + F(b, S_PVHE + 3, i, j, k);
+    return pe;
+}
+
 /* AddSourceToRHS_Fluid (MercurySolver_FluidRHS.cpp:206-527) */
 static void add_sources(Orc *o) {
     const OrcPhys *P = &o->P;
@@ -1352,7 +1403,8 @@ static void add_sources(Orc *o) {
                     const double Bx = F(b, S_BC, i, j, k), By = F(b, S_BC + 1, i, j, k), Bz = F(b, S_BC + 2, i, j, k);
                     const double Jx = F(b, S_JC, i, j, k), Jy = F(b, S_JC + 1, i, j, k), Jz = F(b, S_JC + 2, i, j, k);
                     const double sjbx = Jy * Bz - Jz * By, sjby = Jz * Bx - Jx * Bz, sjbz = Jx * By - Jy * Bx;
-                    NumInfo num = num_limiter(P, F(b, S_UH, i, j, k), F(b, S_UN, i, j, k));
+                    const int ns = n_species(P);
+                    NumInfo num = num_at(P, b, IX(b, i, j, k));
                     double dpex = 0.0, dpey = 0.0, dpez = 0.0;
                     if (P->ambipolar_enabled != 0.0) {
                         double V = fabs(F(b, S_JAC, i, j, k));
@@ -1366,7 +1418,7 @@ static void add_sources(Orc *o) {
                                 d[6 + c] = 0.5 * (F(b, S_JD + 6 + c, i, j, k - 1) + F(b, S_JD + 6 + c, i, j, k)) / V;
                             }
                         }
-#define PE(a1, a2, a3) (F(b, S_PVH + 3, a1, a2, a3) + F(b, S_PVN + 3, a1, a2, a3))
+#define PE(a1, a2, a3) pe_sum(P, b, a1, a2, a3)
                         const double pip = PE(i + 1, j, k), pim = PE(i - 1, j, k), pjp = PE(i, j + 1, k), pjm = PE(i, j - 1, k), pkp = PE(i, j, k + 1), pkm = PE(i, j, k - 1);
 #undef PE
                         dpex = 0.5 * (d[0] * (pip - pim) + d[3] * (pjp - pjm) + d[6] * (pkp - pkm));
@@ -1378,8 +1430,8 @@ static void add_sources(Orc *o) {
                     const double rho_cyl = sqrt(y * y + z * z);
                     const int in_shadow = (x < 0.0 && rho_cyl < 1.0);
                     const double vst = in_shadow ? P->nu_shadow : P->nu_lit;
-                    for (int s = 0; s < 2; ++s) {
-                        int su = s ? S_UN : S_UH, sp = s ? S_PVN : S_PVH, sr = s ? S_RN : S_RH;
+                    for (int s = 0; s < ns; ++s) {
+                        int su = SP_U[s], sp = SP_PV[s], sr = SP_R[s];
                         const double rho = dmax(F(b, su, i, j, k), 0.0);
                         const double u = F(b, sp, i, j, k), v = F(b, sp + 1, i, j, k), w = F(b, sp + 2, i, j, k);
                         const double u2 = u * u + v * v + w * w;
@@ -1390,23 +1442,25 @@ static void add_sources(Orc *o) {
                         const double sjbu = sjbx * u + sjby * v + sjbz * w;
                         const double dpeu = dpex * u + dpey * v + dpez * w;
                         const double Ts0 = F(b, sp + 4, i, j, k) * P->T_ref;
-                        const double Ms = s ? P->M_Na : P->M_H;
-                        const double Mo = P->M_Na; /* H+ pairs with (M_H, M_Na), Na+ with (M_Na, M_Na) */
+                        const double Ms = species_mass(P, s);
+                        const double Mo = P->M_Na; /* H+ pairs with (M_H, M_Na), Na+ with (M_Na, M_Na), He++ with (M_He, M_Na) */
                         const double b1 = vst * (P->Tn0 - Ts0) * (Ms / (Ms + Mo));
                         const double b2 = (Mo / (Ms + Mo)) * vst;
-                        const double ns = s ? num.nNa : num.nH;
-                        const double chi = s ? num.wNa : num.wH;
-                        F(b, sr + 1, i, j, k) += P->c_ind * ns * subx + P->c_hall * chi * sjbx;
-                        F(b, sr + 2, i, j, k) += P->c_ind * ns * suby + P->c_hall * chi * sjby;
-                        F(b, sr + 3, i, j, k) += P->c_ind * ns * subz + P->c_hall * chi * sjbz;
-                        F(b, sr + 4, i, j, k) += P->c_ind * ns * subu + P->c_hall * chi * sjbu;
+                        /* number density in the thermal exchange, charge density in the
+                         * induction and Hall forces (equal for the singly charged ions) */
+                        const double nsp = num.n[s], qs = num.q[s];
+                        const double chi = num.w[s];
+                        F(b, sr + 1, i, j, k) += P->c_ind * qs * subx + P->c_hall * chi * sjbx;
+                        F(b, sr + 2, i, j, k) += P->c_ind * qs * suby + P->c_hall * chi * sjby;
+                        F(b, sr + 3, i, j, k) += P->c_ind * qs * subz + P->c_hall * chi * sjbz;
+                        F(b, sr + 4, i, j, k) += P->c_ind * qs * subu + P->c_hall * chi * sjbu;
                         if (P->ambipolar_enabled != 0.0) {
                             F(b, sr + 1, i, j, k) -= chi * dpex;
                             F(b, sr + 2, i, j, k) -= chi * dpey;
                             F(b, sr + 3, i, j, k) -= chi * dpez;
                             F(b, sr + 4, i, j, k) -= chi * dpeu;
                         }
-                        if (s) {
+                        if (s == 1) {
                             F(b, sr + 0, i, j, k) += P->a1_Na * sss;
                             F(b, sr + 4, i, j, k) += P->a6 * sss * P->Tn0;
                         }
@@ -1414,7 +1468,7 @@ static void add_sources(Orc *o) {
                         F(b, sr + 2, i, j, k) -= P->a4 * rho * v * vst;
                         F(b, sr + 3, i, j, k) -= P->a4 * rho * w * vst;
                         F(b, sr + 4, i, j, k) -= P->a4 * rho * u2 * vst;
-                        F(b, sr + 4, i, j, k) += P->a5 * ns * b1;
+                        F(b, sr + 4, i, j, k) += P->a5 * nsp * b1;
                         F(b, sr + 4, i, j, k) += P->a4 * rho * u2 * b2;
                     }
                 }
@@ -1539,7 +1593,7 @@ static void ambipolar_emf(Orc *o) {
     if (P->ambipolar_enabled == 0.0) return;
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
-#define PEC(a1, a2, a3) (F(b, S_PVH + 3, a1, a2, a3) + F(b, S_PVN + 3, a1, a2, a3))
+#define PEC(a1, a2, a3) pe_sum(P, b, a1, a2, a3)
         for (int a = 0; a < 3; ++a) {
             int hi[3];
             owned_hi(b, L_EX + a, hi);
@@ -1556,17 +1610,18 @@ static void ambipolar_emf(Orc *o) {
                                     for (int dk = -1; dk <= 0; ++dk) sum += PEC(ni + di, nj + dj, nk + dk);
                             pe[e] = 0.125 * sum;
                         }
-                        double rH = 0.0, rN = 0.0;
+                        const int ns = n_species(P);
+                        double rs[NSMAX] = {0.0, 0.0, 0.0};
                         for (int d1 = -1; d1 <= 0; ++d1)
                             for (int d2 = -1; d2 <= 0; ++d2) {
                                 int ii = i, jj = j, kk = k;
                                 if (a == 0) { jj += d1; kk += d2; }
                                 else if (a == 1) { ii += d1; kk += d2; }
                                 else { ii += d1; jj += d2; }
-                                rH += F(b, S_UH, ii, jj, kk);
-                                rN += F(b, S_UN, ii, jj, kk);
+                                for (int s = 0; s < ns; ++s) rs[s] += F(b, SP_U[s], ii, jj, kk);
                             }
-                        NumInfo n = num_limiter(P, 0.25 * rH, 0.25 * rN);
+                        for (int s = 0; s < ns; ++s) rs[s] = 0.25 * rs[s];
+                        NumInfo n = num_limiter(P, rs, ns);
                         F(b, S_E + a, i, j, k) -= P->pressure_coef * (pe[1] - pe[0]) / n.ne_eff;
                     }
         }
@@ -1627,7 +1682,7 @@ static void hall_emf(Orc *o) {
         OBlock *b = &o->b[ib];
         PAR1
         for (long q = 0; q < b->V; ++q) {
-            NumInfo n = num_limiter(P, b->f[S_UH][q], b->f[S_UN][q]);
+            NumInfo n = num_at(P, b, q);
             const double alpha = P->hall_coef / n.ne_eff;
             const double Jx = b->f[S_JC][q], Jy = b->f[S_JC + 1][q], Jz = b->f[S_JC + 2][q];
             const double Bx = b->f[S_BC][q], By = b->f[S_BC + 1][q], Bz = b->f[S_BC + 2][q];
@@ -1780,10 +1835,8 @@ static void apply_update(Orc *o, double dt) {
         for (int i = 0; i < b->m[0]; ++i)
             for (int j = 0; j < b->m[1]; ++j)
                 for (int k = 0; k < b->m[2]; ++k)
-                    for (int m = 0; m < 5; ++m) {
-                        F(b, S_UH + m, i, j, k) += dt * F(b, S_RH + m, i, j, k);
-                        F(b, S_UN + m, i, j, k) += dt * F(b, S_RN + m, i, j, k);
-                    }
+                    for (int m = 0; m < 5; ++m)
+                        for (int s = 0; s < n_species(&o->P); ++s) F(b, SP_U[s] + m, i, j, k) += dt * F(b, SP_R[s] + m, i, j, k);
         for (int a = 0; a < 3; ++a) {
             int hi[3];
             owned_hi(b, L_FX + a, hi);
@@ -1804,10 +1857,9 @@ static void rk2_average(Orc *o) {
         for (int i = 0; i < b->m[0]; ++i)
             for (int j = 0; j < b->m[1]; ++j)
                 for (int k = 0; k < b->m[2]; ++k)
-                    for (int m = 0; m < 5; ++m) {
-                        F(b, S_UH + m, i, j, k) = 0.5 * (F(b, S_U0H + m, i, j, k) + F(b, S_UH + m, i, j, k));
-                        F(b, S_UN + m, i, j, k) = 0.5 * (F(b, S_U0N + m, i, j, k) + F(b, S_UN + m, i, j, k));
-                    }
+                    for (int m = 0; m < 5; ++m)
+                        for (int s = 0; s < n_species(&o->P); ++s)
+                            F(b, SP_U[s] + m, i, j, k) = 0.5 * (F(b, SP_U0[s] + m, i, j, k) + F(b, SP_U[s] + m, i, j, k));
         for (int a = 0; a < 3; ++a) {
             int hi[3];
             owned_hi(b, L_FX + a, hi);
@@ -1822,10 +1874,8 @@ static void rk2_average(Orc *o) {
 static void save_state(Orc *o) {
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
-        for (int m = 0; m < 5; ++m) {
-            memcpy(b->f[S_U0H + m], b->f[S_UH + m], sizeof(double) * b->V);
-            memcpy(b->f[S_U0N + m], b->f[S_UN + m], sizeof(double) * b->V);
-        }
+        for (int m = 0; m < 5; ++m)
+            for (int s = 0; s < n_species(&o->P); ++s) memcpy(b->f[SP_U0[s] + m], b->f[SP_U[s] + m], sizeof(double) * b->V);
         for (int a = 0; a < 3; ++a) memcpy(b->f[S_B0 + a], b->f[S_B + a], sizeof(double) * b->V);
     }
 }
@@ -2231,6 +2281,7 @@ long orc_block_volume(void *h, int ib) { return ((Orc *)h)->b[ib].V; }
 int orc_slot(const char *name) {
     static const struct { const char *n; int s; } T[] = {
         {"U_H", S_UH}, {"U_Na", S_UN}, {"PV_H", S_PVH}, {"PV_Na", S_PVN}, {"U_plus", S_UP}, {"B_cell", S_BC},
+        {"U_He", S_UHE}, {"PV_He", S_PVHE}, {"RHS_He", S_RHE},
         {"Bind_cell", S_BI}, {"J_cell", S_JC}, {"RHS_H", S_RH}, {"RHS_Na", S_RN}, {"Ehc", S_EHC}, {"hall_beta", S_HB},
         {"Jac", S_JAC}, {"Bcell_from_Bface_w", S_WB}, {"Jcell_from_Jedge_w", S_WJ}, {"Photo_rate", S_PHOTO},
         {"cell_center", S_CC}, {"divB", S_DIVB},

@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.environ.get("MERCURY_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
-_NCOMP = {"U_H": 5, "U_Na": 5, "PV_H": 5, "PV_Na": 5, "U_plus": 3, "B_cell": 3, "Bind_cell": 3, "J_cell": 3,
+_NCOMP = {"U_H": 5, "U_Na": 5, "PV_H": 5, "PV_Na": 5, "U_He": 5, "PV_He": 5, "RHS_He": 5, "U_plus": 3, "B_cell": 3, "Bind_cell": 3, "J_cell": 3,
           "RHS_H": 5, "RHS_Na": 5, "Ehc": 3, "Bcell_from_Bface_w": 18, "Jcell_from_Jedge_w": 36,
           "cell_center": 3, "JDxi": 3, "JDet": 3, "JDze": 3, "Eface_xi": 3, "Eface_eta": 3, "Eface_zeta": 3,
           "dr_xi": 3, "dr_eta": 3, "dr_zeta": 3}

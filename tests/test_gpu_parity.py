@@ -330,3 +330,56 @@ def test_stale_ensemble_is_refused():
         g0.close()
         g1.close()
 
+
+
+STATE3 = STATE + ["U_He"]
+DERIVED3 = DERIVED + ["PV_He"]
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("shape", ["cube_he", "shell_he"])
+def test_three_species_steps_bit_exact(oracle_mod, shape, form, integrator):
+    """H+, Na+ and He++ (the third species has no reference counterpart; the
+    oracle's is pinned by reductions to two species): 5 steps of the plain and
+    fused FP64 forms equal the oracle bit for bit, div B at roundoff."""
+    case = cases.make_periodic_cube(n=(10, 8, 6), g=4, he=True, hall=True) if shape == "cube_he" else small_shell(he=True)
+    print(f"[case] {case.name} three species form={form} integrator={integrator}", flush=True)
+    g = Mercury(case)
+    o = oracle_mod.Oracle(case)
+    try:
+        g.prepare()
+        o.prepare()
+        g.set_form(form)
+        np.testing.assert_array_equal(g.step(5, integrator), o.step(5, integrator))
+        _cmp(g, o, STATE3 + DERIVED3, f"three species, 5 steps form={form}")
+        assert g.divb()[0] <= 1e-12
+    finally:
+        g.close()
+
+
+def test_three_species_graph_and_mixed(oracle_mod):
+    """The three-species shell replayed from a graph (fused FP64: bit-exact) and
+    in the mixed form (rel-L2 <= 1e-5 per conserved variable after 20 steps)."""
+    case = small_shell(he=True)
+    print(f"[case] {case.name} three species graph + mixed", flush=True)
+    o = oracle_mod.Oracle(case)
+    o.prepare()
+    o.step(20, 0)
+    for form in (1, 2):
+        g = Mercury(case)
+        try:
+            g.prepare()
+            g.set_form(form)
+            g.graph_capture(0)
+            g.graph_launch(20)
+            g.synchronize()
+            if form == 1:
+                _cmp(g, o, STATE3, "three species graph")
+            else:
+                errs = {n: _rel_l2(g, o, n, case.nb) for n in STATE3}
+                print("[mixed, three species] rel-L2:", errs, flush=True)
+                assert all(e <= 1e-5 for e in errs.values()), errs
+            assert g.divb()[0] <= 1e-12
+        finally:
+            g.close()

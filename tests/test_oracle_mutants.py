@@ -19,7 +19,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "mercury_oracle.c")
 PINS = ["tests/test_oracle_pins.py", "tests/test_oracle_term_pins.py", "tests/test_emf_pins.py", "tests/test_dipole_oracle.py",
-        "tests/test_dec_oracle.py"]
+        "tests/test_dec_oracle.py", "tests/test_species_oracle.py"]
 
 # (what the mistake is, exact fragment, replacement)
 MUTANTS = [
@@ -68,12 +68,12 @@ MUTANTS = [
     ("sources: photo energy without Tn0", "F(b, sr + 4, i, j, k) += P->a6 * sss * P->Tn0;",
      "F(b, sr + 4, i, j, k) += P->a6 * sss;"),
     ("sources: photo mass coefficient", "F(b, sr + 0, i, j, k) += P->a1_Na * sss;", "F(b, sr + 0, i, j, k) += P->a6 * sss;"),
-    ("sources: Hall weight n_s/n_e", "const double chi = s ? num.wNa : num.wH;", "const double chi = s ? num.chiNa : num.chiH;"),
+    ("sources: Hall weight n_s/n_e", "const double chi = num.w[s];", "const double chi = num.chi[s];"),
     ("sources: drag on the energy with u", "F(b, sr + 4, i, j, k) -= P->a4 * rho * u2 * vst;",
      "F(b, sr + 4, i, j, k) -= P->a4 * rho * u * vst;"),
     # A1 / A2 calc_PV, calc_Uplus, Hall_Num_Limiter (MercurySolver_Tools.cpp:111-301)
-    ("U+: species weight", "b->f[S_UP + c][q] = n.chiH * uH[c] + n.chiNa * uN[c];",
-     "b->f[S_UP + c][q] = n.chiH * uH[c] + n.chiH * uN[c];"),
+    ("U+: species weight", "double up = n.chi[0] * us[0][c] + n.chi[1] * us[1][c];",
+     "double up = n.chi[0] * us[0][c] + n.chi[0] * us[1][c];"),
     ("limiter: floor dropped", "n.ne_eff = sqrt(n.ne * n.ne + P->ne_floor * P->ne_floor);", "n.ne_eff = n.ne;"),
     # A6 Compute_Timestep (MercurySolver_Timestep.cpp:8-141)
     ("time step: slow magnetosonic speed", "double cf = sqrt(0.5 * (t + sqrt(disc)));", "double cf = sqrt(0.5 * (t - sqrt(disc)));"),
@@ -89,8 +89,8 @@ MUTANTS = [
     ("Hall: C_wh = 0.2", "const double eps = 1e-14, Cwh = 0.1;", "const double eps = 1e-14, Cwh = 0.2;"),
     ("Hall: diffusion sign", "val = Ecen + 0.5 * mue * d1 - 0.5 * muz * d2;", "val = Ecen - 0.5 * mue * d1 - 0.5 * muz * d2;"),
     ("Hall: taper width", "tp = 0.5 * (1.0 + tanh(2.0 * (r - mid) / width));", "tp = 0.5 * (1.0 + tanh((r - mid) / width));"),
-    ("ambipolar: density sum instead of mean", "NumInfo n = num_limiter(P, 0.25 * rH, 0.25 * rN);",
-     "NumInfo n = num_limiter(P, rH, rN);"),
+    ("ambipolar: density sum instead of mean", "for (int s = 0; s < ns; ++s) rs[s] = 0.25 * rs[s];",
+     "for (int s = 0; s < ns; ++s) rs[s] = 1.0 * rs[s];"),
     ("ambipolar: node mean over 4", "pe[e] = 0.125 * sum;", "pe[e] = 0.25 * sum;"),
     ("artificial: linear ramp", "eta = P->arti_eta_max * (q * q * (3.0 - 2.0 * q));", "eta = P->arti_eta_max * q;"),
     ("local eta: squared distance", "eta = P->local_eta_max * exp(-rr / P->local_r_decay);",
@@ -108,6 +108,13 @@ MUTANTS = [
     ("dipole: curl of the eta face with the wrong sign",
      "else if (a == 1) c = (AE(0, i, j, k + 1) - AE(0, i, j, k)) + (AE(2, i, j, k) - AE(2, i + 1, j, k));",
      "else if (a == 1) c = (AE(0, i, j, k) - AE(0, i, j, k + 1)) + (AE(2, i, j, k) - AE(2, i + 1, j, k));"),
+    # the third ion species, He++ (no reference counterpart; pinned by reductions to two species)
+    ("He++: charge dropped from its charge density", "n.q[2] = P->Z_He * n.n[2];", "n.q[2] = n.n[2];"),
+    ("He++: left out of n_e", "n.ne = n.ne + n.q[2];", "n.ne = n.ne;"),
+    ("He++: pressure left out of p_e", "if (n_species(P) > 2) pe = pe + F(b, S_PVHE + 3, i, j, k);", "if (n_species(P) > 2) pe = pe;"),
+    ("He++: dropped from U+", "if (ns > 2) up = up + n.chi[2] * us[2][c];", "if (ns > 2) up = up;"),
+    ("induction force with the number density", "const double nsp = num.n[s], qs = num.q[s];",
+     "const double nsp = num.n[s], qs = num.n[s];"),
     # src/8_dec DecExteriorDerivative.cpp / DecDiagnostics.cpp
     ("DEC d0: reversed difference", "for (int k = 0; k <= b.m[2]; ++k) ey[DQ(b, i, j, k)] = n[DQ(b, i, j + 1, k)] - n[DQ(b, i, j, k)];",
      "for (int k = 0; k <= b.m[2]; ++k) ey[DQ(b, i, j, k)] = n[DQ(b, i, j, k)] - n[DQ(b, i, j + 1, k)];"),
