@@ -142,8 +142,12 @@ def main():
     ap.add_argument("--rshell-steps", type=int, default=20,
                     help="steps of the resistive shell (4 blocks glued by interfaces, exponential eta) per form; 0 skips it")
     ap.add_argument("--rshell-scale", type=float, default=1.0, help="size factor of the resistive shell (1: 18.9 M cells)")
-    ap.add_argument("--ensemble-runs", type=int, default=8,
+    ap.add_argument("--ensemble-runs", type=int, default=64,
                     help="members of the batched-ensemble measurement (0: skip; the full configuration has 64)")
+    ap.add_argument("--mixed-horizon", type=int, default=200,
+                    help="steps of the mixed form against the FP64 form with div B after every step (0: skip)")
+    ap.add_argument("--c5-steps", type=int, default=500,
+                    help="steps of the three-species resistive shell in the mixed form (0: skip)")
     ap.add_argument("--forms", default="fp64_plain,mixed,fp64_fused",
                     help="comma list of forms to run (fp64_fused is always run: it gives the value)")
     args = ap.parse_args()
@@ -245,6 +249,48 @@ def main():
             dt_out = time.perf_counter() - t0
             output = {"tecplot_s": dt_out, "tecplot_bytes": os.path.getsize(path)}
         print(f"[bench] tecplot output: {dt_out:.3f} s, {output['tecplot_bytes'] / 1e6:.1f} MB", file=sys.stderr, flush=True)
+    # the mixed form over the full horizon of the configuration against the FP64 form
+    # (bit-identical to the CPU oracle: checked above after the sample steps), with the
+    # normalised div B after every step
+    horizon = None
+    if args.mixed_horizon > 0:
+        g.close()
+        g = None
+        t0 = time.perf_counter()
+        gf, gm = Mercury(case), Mercury(case)
+        gf.prepare()
+        gm.prepare()
+        gf.set_form(1)
+        gm.set_form(2)
+        gf.graph_capture(case.integrator)
+        gf.graph_launch(args.mixed_horizon)
+        divb_max = 0.0
+        for _ in range(args.mixed_horizon):
+            gm.step(1, case.integrator, record_dt=False)
+            divb_max = max(divb_max, gm.divb()[0])
+        gf.synchronize()
+        err, emax = {}, {}
+        for name in ("U_H", "U_Na", "B_xi", "B_eta", "B_zeta"):
+            num = den = mx = scale = 0.0
+            for ib in range(case.nb):
+                a_, b_ = gm.get(name, ib), gf.get(name, ib)
+                num += float(np.sum((a_ - b_) ** 2))
+                den += float(np.sum(b_ ** 2))
+                mx = max(mx, float(np.abs(a_ - b_).max()))
+                scale = max(scale, float(np.abs(b_).max()))
+            err[name] = (num / den) ** 0.5 if den > 0 else num ** 0.5
+            emax[name] = mx / scale if scale > 0 else mx
+        horizon = {"steps": args.mixed_horizon, "rel_l2_vs_fp64": err, "rel_max_vs_fp64": emax,
+                   "divB_normalised_max_over_steps": divb_max, "divB_per_step_us": gm.time_op("divb", 20) * 1e3,
+                   "reference": "FP64 fused GPU form, bit-identical to the CPU oracle"}
+        print(f"[bench] mixed over {args.mixed_horizon} steps: rel-L2 {err}, max div B {divb_max:.2e} "
+              f"({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+        gm.close()
+        gf.close()
+        g = Mercury(case)
+        g.prepare()
+        g.set_form(1)
+        g.graph_capture(case.integrator)
     ms_step = forms["fp64_fused"]["ms_per_step"]
     value = forms["fp64_fused"]["cell_updates_per_s"]
     divb_norm = forms["fp64_fused"]["divB_normalised"]
@@ -350,6 +396,23 @@ def main():
             print(f"[bench] resistive shell {label}: {rcells} cells, {el * 1e3 / args.rshell_steps:.3f} ms/step, "
                   f"{n_unk} unknown edges, {np.mean(its):.1f} CG iterations/step", file=sys.stderr, flush=True)
         rshell["mixed_over_fp64"] = rshell["mixed"]["cell_updates_per_s"] / rshell["fp64_fused"]["cell_updates_per_s"]
+        rshell["species"] = int(rc.phys.n_species)
+        if args.c5_steps > 0:
+            # the configuration's full run in the mixed form (steps already taken count)
+            t1 = time.perf_counter()
+            rest = args.c5_steps - (args.rshell_steps + 2) * 2
+            its = []
+            for _ in range(max(rest, 0)):
+                rg.step(1, 0)
+                its.append(rg.resist_stats()[1])
+            rg.synchronize()
+            el = time.perf_counter() - t1
+            rshell["mixed_full_run"] = {"steps_total": args.c5_steps, "timed_steps": max(rest, 0),
+                                        "ms_per_step": el * 1e3 / max(rest, 1), "cg_iterations_mean": float(np.mean(its)) if its else None,
+                                        "divB_normalised": rg.divb()[0],
+                                        "he_density_min": float(min(rg.get("U_He", ib)[0].min() for ib in range(rc.nb)))}
+            print(f"[bench] resistive shell mixed full run: {args.c5_steps} steps, {el * 1e3 / max(rest, 1):.3f} ms/step, "
+                  f"div B {rshell['mixed_full_run']['divB_normalised']:.2e}", file=sys.stderr, flush=True)
         rg.close()
         print(f"[bench] resistive shell setup+run {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     times = op_times["fp64_fused"]
@@ -399,6 +462,7 @@ def main():
         "resistive": resist,
         "ensemble": ensemble,
         "resistive_shell": rshell,
+        "mixed_horizon": horizon,
         "output": output,
     }
     print(json.dumps(out), flush=True)

@@ -109,6 +109,25 @@ int mercury_create(MercuryContext **ctx, int nblocks, const int *dims, int nghos
  * NULL when no face is an interface. */
 int mercury_create_connected(MercuryContext **ctx, int nblocks, const int *dims, int nghost, const int *bc, const int *nbr,
                              const int *transform, const double *owned_nodes, const MercuryPhysics *phys);
+/* An ensemble of independent runs in ONE context (no reference counterpart: the
+ * reference runs one case per MPI job): run r owns the next run_nblocks[r] blocks
+ * of the concatenated block list (dims, bc, owned_nodes as for
+ * mercury_create_connected; nbr holds indices into the concatenated list and an
+ * interface may not join two runs) and phys[r] is its physics.  The runs may
+ * differ only in hall_coef, the far-field states (qinf, qinfs, q_pv_inf,
+ * qinf_He) and B_imf; everything else must be equal, and the implicit resistive
+ * correction is refused.  Every pass of a step is one launch over every block
+ * of every run; each run has its own CFL time step and its own dt abort status
+ * (a refused run stays at its last completed step, mercury_step and
+ * mercury_graph_launch fail naming the run).  Fields are addressed by the
+ * block's index in the concatenated list. */
+int mercury_create_ensemble(MercuryContext **ctx, int nruns, const int *run_nblocks, const int *dims, int nghost, const int *bc,
+                            const int *nbr, const int *transform, const double *owned_nodes, const MercuryPhysics *phys);
+/* Number of runs of a context (1 unless made by mercury_create_ensemble) and,
+ * when dts is not NULL, the dt each run committed for its last step (nruns
+ * doubles; 0 for a run stopped by the abort rule).  mercury_step's dt_out and
+ * mercury_timestep report run 0. */
+int mercury_run_dt(MercuryContext *ctx, int *nruns, double *dts);
 /* Replaces the deletes at the end of Z4_Mercury/main.cpp:86-89. */
 int mercury_destroy(MercuryContext *ctx);
 
@@ -175,7 +194,8 @@ int mercury_step(MercuryContext *ctx, int nsteps, int integrator, double *dt_out
  * drop-in use: pv_uplus, bcell, jedge, jcell, derived, flux, sources, eface,
  * edge_emf, hall_cell, hall_edge, induction, alias_e, alias_j, timestep, stage,
  * dec_d0, dec_d1 (arg = multiplier), dec_d2, sync_ucell, sync_bface, sync_eedge,
- * sync_ehall, sync_jedge, sync_bcell, sync_jcell, sync_badd. */
+ * sync_ehall, sync_jedge, sync_bcell, sync_jcell, sync_badd, divb (calc_divB's
+ * reduction left on the device; mercury_divb reads it). */
 int mercury_op(MercuryContext *ctx, const char *op, double arg);
 
 /* Replaces MercurySolver::Compute_Timestep (MercurySolver_Timestep.cpp:8-141). */

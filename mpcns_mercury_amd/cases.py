@@ -456,6 +456,35 @@ def make_ensemble_run(r: int, scale: float = 1.0, g: int = 4, steps: int = 100) 
                       name=f"ensemble_run{r}")
 
 
+def concat_runs(runs, name: str = "ensemble") -> Case:
+    """Independent runs as one case of concatenated blocks for an ensemble
+    context (``hip.Mercury`` of the result calls mercury_create_ensemble): block
+    interfaces are renumbered into the concatenated list; ``meta`` keeps the
+    block count and the physics of every run.  The runs must share the ghost
+    depth, the uniform field Bu and the dipoles."""
+    if not runs:
+        raise ValueError("an ensemble needs at least one run")
+    g = runs[0].g
+    blocks, fields, A, base = [], [], [], 0
+    for c in runs:
+        if c.g != g or tuple(c.Bu) != tuple(runs[0].Bu):
+            raise ValueError("the runs of an ensemble share nghost and Bu")
+        if not np.array_equal(c.dipole_array(), runs[0].dipole_array()):
+            raise ValueError("the runs of an ensemble share the dipoles")
+        for b in c.blocks:
+            nbr = tuple(x + base if x >= 0 else x for x in b.nbr)
+            blocks.append(Block(dims=b.dims, bc=b.bc, nodes=b.nodes, nbr=nbr, off=b.off))
+        fields.extend(c.fields)
+        if c.A is not None:
+            A.extend(c.A)
+        base += c.nb
+    if any(c.A is not None for c in runs) and not all(c.A is not None for c in runs):
+        raise ValueError("either every run has an edge potential or none")
+    meta = {"run_nblocks": [c.nb for c in runs], "run_phys": [c.phys for c in runs], "run_names": [c.name for c in runs]}
+    return Case(name=name, g=g, blocks=blocks, phys=runs[0].phys, fields=fields, A=A if runs[0].A is not None else None,
+                Bu=runs[0].Bu, steps=runs[0].steps, integrator=runs[0].integrator, meta=meta, dipoles=runs[0].dipoles)
+
+
 def make_split_sector(dims=(24, 20, 16), cuts=((24,), (10, 10), (16,)), **kw) -> Case:
     """One shell sector of the two-fluid Hall configuration (planet surface on
     xi-, far field on the other faces) cut into blocks glued by interfaces: the

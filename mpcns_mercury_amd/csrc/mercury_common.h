@@ -95,6 +95,8 @@ struct MercBlk {
     int N[3];        // box extent m+1+2g
     int sI, sJ;      // strides of i and j (k stride is 1)
     int bc[6];       // face codes, -xi,+xi,-eta,+eta,-zeta,+zeta
+    int run;         // run of the block in an ensemble context (0 otherwise): its dt and status index
+    double hall_coef;  // the run's Hall coefficient (the one physics constant the kernels read per run)
 };
 
 // a launch range: per block a box [lo, lo+n) of (i,j,k), linearised k fastest;

@@ -66,6 +66,55 @@ typedef __attribute__((address_space(1))) float gfloat;
 DEVI gdouble *const *gslots(double *const *F) { return static_cast<gdouble *const *>(static_cast<const void *>(F)); }
 DEVI const gfloat *const *gslots(const float *const *G) { return static_cast<const gfloat *const *>(static_cast<const void *>(G)); }
 
+// Minimum of the threads' time-step candidates per run, folded into dt_bits[run]
+// (FP64 bits, atomicMin: positive doubles order like their bit patterns).  A
+// workgroup whose located threads all belong to one run issues one atomic;
+// otherwise (a workgroup across a block boundary between runs) each wave whose
+// threads share a run issues one, and a wave spanning two runs falls back to one
+// atomic per thread.  run < 0 marks a thread with no candidate.  Every thread of
+// the workgroup must call this (it synchronises the workgroup).
+DEVI void dt_min_runs(double best, int run, unsigned long long *__restrict__ dt_bits) {
+    __shared__ double wmin[16];
+    __shared__ int wrun[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+    // the wave's run: the largest (any valid one); uniform if min == max over valid lanes
+    int rmax = run, rmin = run < 0 ? 0x7fffffff : run;
+    for (int off = 32; off > 0; off >>= 1) {
+        rmax = max(rmax, __shfl_xor(rmax, off, 64));
+        rmin = min(rmin, __shfl_xor(rmin, off, 64));
+    }
+    const bool wave_uniform = rmax < 0 || rmin == rmax;
+    if (!wave_uniform) {
+        if (run >= 0 && best < 1e100) atomicMin(dt_bits + run, (unsigned long long)__double_as_longlong(best));
+        best = 1e100;
+    }
+    for (int off = 32; off > 0; off >>= 1) best = dmin(best, __shfl_xor(best, off, 64));
+    if (lane == 0) {
+        wmin[wid] = best;
+        wrun[wid] = wave_uniform ? rmax : -1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int r0 = -1;
+        bool same = true;
+        for (int w = 0; w < nw; ++w) {
+            if (wrun[w] < 0) continue;
+            if (r0 < 0) r0 = wrun[w];
+            else if (wrun[w] != r0) same = false;
+        }
+        if (same) {
+            if (r0 >= 0) {
+                double m = wmin[0];
+                for (int w = 1; w < nw; ++w) m = dmin(m, wmin[w]);
+                atomicMin(dt_bits + r0, (unsigned long long)__double_as_longlong(m));
+            }
+        } else {
+            for (int w = 0; w < nw; ++w)
+                if (wrun[w] >= 0) atomicMin(dt_bits + wrun[w], (unsigned long long)__double_as_longlong(wmin[w]));
+        }
+    }
+}
+
 // neighbour offsets
 #define OI(p) ((long long)(p).sI)
 #define OJ(p) ((long long)(p).sJ)
@@ -96,14 +145,23 @@ DEVI Num num_limiter(const MercPhys &P, double rhoH, double rhoNa, double rhoHe 
         n.ne = n.ne + n.q[2];
     }
     n.ne_eff = sqrt(n.ne * n.ne + P.ne_floor * P.ne_floor);
+    // loops over the fixed maximum with a guard: unrolled, every index is a
+    // constant and the arrays stay in registers (a loop bounded by ns would
+    // index them dynamically, through scratch memory)
     n.chi[2] = n.w[2] = 0.0;
     if (n.ne > 1.0e-300) {
-        for (int s = 0; s < ns; ++s) n.chi[s] = n.q[s] / n.ne;
+#pragma unroll
+        for (int s = 0; s < MERC_NSMAX; ++s)
+            if (s < ns) n.chi[s] = n.q[s] / n.ne;
     } else {
         n.chi[0] = 1.0;
-        for (int s = 1; s < ns; ++s) n.chi[s] = 0.0;
+#pragma unroll
+        for (int s = 1; s < MERC_NSMAX; ++s)
+            if (s < ns) n.chi[s] = 0.0;
     }
-    for (int s = 0; s < ns; ++s) n.w[s] = n.q[s] / n.ne_eff;
+#pragma unroll
+    for (int s = 0; s < MERC_NSMAX; ++s)
+        if (s < ns) n.w[s] = n.q[s] / n.ne_eff;
     return n;
 }
 

@@ -7,8 +7,9 @@ namespace merc {
 // R: per block ceil(m0 / L) x m1 x m2 points; each thread marches over L cells along i
 void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                      int L, bool mix);
+// dt_next: one word per run (nruns of them), reduced per run
 void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
-                     unsigned long long *dt_next, int compute_dt, bool mix, int hall64 = 0);
+                     unsigned long long *dt_next, int compute_dt, bool mix, int hall64, int nruns);
 // ambi: also the node electron pressure of the ambipolar EMF (read by k_edge_total)
 // ns: ion species of the run (2 or 3)
 void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix, int ambi,
@@ -21,5 +22,6 @@ void launch_to_double(hipStream_t st, const float *src, double *dst, long long n
 void launch_bcell_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);
 void launch_jcell_t(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);
 void launch_to_float(hipStream_t st, const double *src, float *dst, long long n);
-void launch_dt_commit(hipStream_t st, unsigned long long *dt, const unsigned long long *dt_next, unsigned long long *status);
+// per run: the abort rule on dt_next[r], dt[r] committed, status 4 words per run
+void launch_dt_commit(hipStream_t st, unsigned long long *dt, const unsigned long long *dt_next, unsigned long long *status, int nruns);
 }  // namespace merc

@@ -244,8 +244,11 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
 // Six waves per SIMD: the FP64 body fits in 80 VGPRs without spilling (102
 // unconstrained, four waves), and the latency-bound pass runs 10 % faster
 // (398 -> 359 us on shell6); seven waves would spill.
-template <bool MIX, int NS>
-__global__ void __attribute__((amdgpu_waves_per_eu(6))) k_cell_end(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
+// ENS: an ensemble context (per-block Hall coefficient and per-run dt); a
+// one-run context takes both from the kernel arguments and keeps every
+// register of the six-wave budget for the body.
+template <bool MIX, int NS, bool ENS>
+__global__ void __attribute__((amdgpu_waves_per_eu(NS == 2 ? 6 : 5))) k_cell_end(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
                            const float *const *__restrict__ G_, const MercPhys P, unsigned long long *__restrict__ dt_bits,
                            int compute_dt, int hall64) {
     gdouble *const *__restrict__ F = gslots(F_);
@@ -254,7 +257,8 @@ __global__ void __attribute__((amdgpu_waves_per_eu(6))) k_cell_end(const MercRan
     using PR = Prec<MIX>;
     double best = 1e100;
     Pt p;
-    if (locate(R, B, TID, p)) {
+    const bool located = locate(R, B, TID, p);
+    if (located) {
         const MercBlk &bk = B[p.b];
         const long long q = p.q;
         double pv[NS][5];
@@ -292,7 +296,7 @@ __global__ void __attribute__((amdgpu_waves_per_eu(6))) k_cell_end(const MercRan
         const Num &nh = n;
         const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
         {
-            const T alpha = (T)(P.hall_coef / nh.ne_eff);
+            const T alpha = (T)((ENS ? bk.hall_coef : P.hall_coef) / nh.ne_eff);
             const T Jx = (T)F[SL_JC][q], Jy = (T)F[SL_JC + 1][q], Jz = (T)F[SL_JC + 2][q];
             const T bx = (T)Bx, by = (T)By, bz = (T)Bz;
             const T e0 = alpha * (Jy * bz - Jz * by), e1 = alpha * (Jz * bx - Jx * bz), e2 = alpha * (Jx * by - Jy * bx);
@@ -374,16 +378,8 @@ __global__ void __attribute__((amdgpu_waves_per_eu(6))) k_cell_end(const MercRan
         }
     }
     if (!compute_dt) return;
-    for (int off = 32; off > 0; off >>= 1) best = dmin(best, __shfl_xor(best, off, 64));
-    __shared__ double wmin[16];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) wmin[wid] = best;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = wmin[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = dmin(m, wmin[w]);
-        atomicMin(dt_bits, (unsigned long long)__double_as_longlong(m));
-    }
+    // the run is read here, not before the body: its register is not held across it
+    dt_min_runs(best, located ? (ENS ? B[p.b].run : 0) : -1, dt_bits);
 }
 
 // ---------------------------------------------------------------------------
@@ -628,13 +624,14 @@ __global__ void k_update_all(const MercRange R, const MercBlk *__restrict__ B, d
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
-    const double dt = *dtp;
+    const double dt = dtp[bk.run];
     if (dt == 0.0) return;  // the run is stopped (k_dt_commit)
     if (p.i < bk.m[0] && p.j < bk.m[1] && p.k < bk.m[2])
         for (int m = 0; m < 5; ++m) {
             // the fluid RHS is read here only, RHS_B written here and only downloaded:
             // non-temporal, so they do not displace reused data from the caches
-            for (int s = 0; s < ns; ++s) {
+#pragma unroll
+            for (int s = 0; s < MERC_NSMAX && s < ns; ++s) {
                 if constexpr (MIX) F[SPEC_U(s) + m][q] += dt * (double)__builtin_nontemporal_load(&G[SPEC_R(s) + m][q]);
                 else F[SPEC_U(s) + m][q] += dt * __builtin_nontemporal_load(&F[SPEC_R(s) + m][q]);
             }
@@ -751,23 +748,27 @@ __global__ void k_to_float(const double *__restrict__ src, float *__restrict__ d
 // passes skip their writes when dt is 0 (no valid step has dt = 0), so the
 // state stays that of the last completed step, as the reference exits before
 // its update.
-__global__ void k_dt_commit(unsigned long long *dt, const unsigned long long *dt_next, unsigned long long *status) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const unsigned long long bits = *dt_next;
+// One thread per run of the context (status: 4 words per run).
+__global__ void k_dt_commit(unsigned long long *dt, const unsigned long long *dt_next, unsigned long long *status, int nruns) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nruns) return;
+    unsigned long long *st = status + 4 * r;
+    const unsigned long long bits = dt_next[r];
     const double d = __longlong_as_double((long long)bits);
-    if (status[0] != 0ULL) {
-        *dt = 0ULL;
+    if (st[0] != 0ULL) {
+        dt[r] = 0ULL;
     } else if (!isfinite(d) || d < 1e-7) {
-        status[0] = 1ULL;
-        status[2] = bits;
-        *dt = 0ULL;
+        st[0] = 1ULL;
+        st[2] = bits;
+        dt[r] = 0ULL;
     } else {
-        *dt = bits;
-        status[1] += 1ULL;
+        dt[r] = bits;
+        st[1] += 1ULL;
     }
 }
-__global__ void k_dt_reset(unsigned long long *dt_next) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *dt_next = (unsigned long long)__double_as_longlong(1e100);
+__global__ void k_dt_reset(unsigned long long *dt_next, int nruns) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nruns) dt_next[r] = (unsigned long long)__double_as_longlong(1e100);
 }
 
 }  // namespace
@@ -792,16 +793,18 @@ void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *
     }
 }
 void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
-                     unsigned long long *dt_next, int compute_dt, bool mix, int hall64) {
-    if (compute_dt) hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(64), 0, st, dt_next);
-    const bool three = P.n_species >= 3.0;
+                     unsigned long long *dt_next, int compute_dt, bool mix, int hall64, int nruns) {
+    if (compute_dt) hipLaunchKernelGGL(k_dt_reset, dim3(nblk(nruns, 64)), dim3(64), 0, st, dt_next, nruns);
+    const bool three = P.n_species >= 3.0, ens = nruns > 1;
+#define CELL_END(M, N, E) LAUNCH_R((k_cell_end<M, N, E>), R, B, F, G, P, dt_next, compute_dt, hall64)
     if (mix) {
-        if (three) LAUNCH_R((k_cell_end<true, 3>), R, B, F, G, P, dt_next, compute_dt, hall64);
-        else LAUNCH_R((k_cell_end<true, 2>), R, B, F, G, P, dt_next, compute_dt, hall64);
+        if (three) { if (ens) CELL_END(true, 3, true); else CELL_END(true, 3, false); }
+        else { if (ens) CELL_END(true, 2, true); else CELL_END(true, 2, false); }
     } else {
-        if (three) LAUNCH_R((k_cell_end<false, 3>), R, B, F, G, P, dt_next, compute_dt, hall64);
-        else LAUNCH_R((k_cell_end<false, 2>), R, B, F, G, P, dt_next, compute_dt, hall64);
+        if (three) { if (ens) CELL_END(false, 3, true); else CELL_END(false, 3, false); }
+        else { if (ens) CELL_END(false, 2, true); else CELL_END(false, 2, false); }
     }
+#undef CELL_END
 }
 void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix, int ambi,
                     int ns) {
@@ -832,8 +835,8 @@ void launch_jcell_t(hipStream_t st, const RangeH &R, const MercBlk *B, double *c
 void launch_to_float(hipStream_t st, const double *src, float *dst, long long n) {
     if (n > 0) hipLaunchKernelGGL(k_to_float, dim3(nblk(n, 256)), dim3(256), 0, st, src, dst, n);
 }
-void launch_dt_commit(hipStream_t st, unsigned long long *dt, const unsigned long long *dt_next, unsigned long long *status) {
-    hipLaunchKernelGGL(k_dt_commit, dim3(1), dim3(64), 0, st, dt, dt_next, status);
+void launch_dt_commit(hipStream_t st, unsigned long long *dt, const unsigned long long *dt_next, unsigned long long *status, int nruns) {
+    hipLaunchKernelGGL(k_dt_commit, dim3(nblk(nruns, 64)), dim3(64), 0, st, dt, dt_next, status, nruns);
 }
 
 }  // namespace merc

@@ -184,6 +184,7 @@ public:
         if (dblk) (void)hipFree(dblk);
     }
     int nb() const { return (int)blk.size(); }
+    void upload_blocks() { HIPCHK(hipMemcpy(dblk, blk.data(), sizeof(MercBlk) * blk.size(), hipMemcpyHostToDevice)); }
     long long vol(int b) const { return (long long)blk[b].N[0] * blk[b].N[1] * blk[b].N[2]; }
     int gidx(int b, int i, int j, int k) const {
         const MercBlk &q = blk[b];
@@ -620,7 +621,14 @@ struct CaptureGuard {
 class MercurySolver {
 public:
     std::unique_ptr<Field> F;
-    MercPhys P{};
+    MercPhys P{};  // the physics of run 0; every run shares all of it but the per-run fields below
+    // Ensemble context: independent runs, each a set of blocks with its own
+    // physics (hall_coef, inflow state, IMF), dt and abort status; one launch
+    // per pass covers every block of every run.  A plain context has one run.
+    int nruns = 1;
+    std::vector<int> brun;        // run of each block
+    std::vector<MercPhys> Prun;   // physics of each run
+    const MercPhys &Pb(int b) const { return Prun[brun[b]]; }
     // ion species of the run: 2 (H+, Na+) or 3 (He++ added)
     int nspecies() const { return P.n_species >= 3.0 ? 3 : 2; }
     std::vector<std::unique_ptr<BlockGeom>> geom;
@@ -702,17 +710,61 @@ public:
         }
     }
 
+    // The physics fields an ensemble's runs may differ in: the Hall coefficient,
+    // the far-field (inflow) states and the IMF.  Everything else is shared.
+    static void check_run_physics(int nruns, const MercPhys *prun) {
+        MercPhys a = prun[0];
+        auto blank = [](MercPhys &q) {
+            q.hall_coef = 0.0;
+            for (int c = 0; c < 5; ++c) q.qinf[c] = q.qinfs[c] = q.q_pv_inf[c] = q.qinf_He[c] = 0.0;
+            for (int c = 0; c < 3; ++c) q.B_imf[c] = 0.0;
+        };
+        blank(a);
+        for (int r = 1; r < nruns; ++r) {
+            MercPhys b = prun[r];
+            blank(b);
+            if (std::memcmp(&a, &b, sizeof(MercPhys)) != 0)
+                throw std::runtime_error("ensemble run " + std::to_string(r) +
+                                         ": the runs of one context may differ only in hall_coef, the far-field states and B_imf");
+        }
+        if (nruns > 1 && prun[0].resist_enabled != 0.0)
+            throw std::runtime_error("an ensemble context cannot run the implicit resistive correction");
+    }
+
     MercurySolver(int nb, const int *dims, int g, const int *bc, const int *nbr, const int *trans, const double *nodes,
                   const MercPhys &phys)
-        : P(phys) {
+        : MercurySolver(nb, dims, g, bc, nbr, trans, nodes, 1, nullptr, &phys) {}
+
+    // block_run: run of each block (NULL with one run); prun: physics of each run
+    MercurySolver(int nb, const int *dims, int g, const int *bc, const int *nbr, const int *trans, const double *nodes, int nruns_,
+                  const int *block_run, const MercPhys *prun)
+        : P(prun[0]), nruns(nruns_) {
         if (g < 2) throw std::runtime_error("nghost must be >= 2");
+        if (nruns < 1) throw std::runtime_error("an ensemble needs at least one run");
         check_topology(nb, dims, g, bc, nbr, trans);
+        brun.assign(nb, 0);
+        if (block_run)
+            for (int b = 0; b < nb; ++b) {
+                if (block_run[b] < 0 || block_run[b] >= nruns) throw std::runtime_error("block run id out of range");
+                brun[b] = block_run[b];
+            }
+        for (int b = 0; b < nb; ++b)
+            for (int f = 0; f < 6; ++f)
+                if (bc[6 * b + f] == MBC_INTERFACE && brun[nbr[6 * b + f]] != brun[b])
+                    throw std::runtime_error("block " + std::to_string(b) + ": an interface joins two runs of the ensemble");
+        Prun.assign(prun, prun + nruns);
+        check_run_physics(nruns, prun);
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         F.reset(new Field(nb, dims, g, bc, nbr, trans));
-        HIPCHK(hipMalloc(&d_dt, sizeof(unsigned long long)));
-        HIPCHK(hipMalloc(&d_dt_next, sizeof(unsigned long long)));
-        HIPCHK(hipMalloc(&d_status, 4 * sizeof(unsigned long long)));
-        HIPCHK(hipMemset(d_status, 0, 4 * sizeof(unsigned long long)));
+        for (int b = 0; b < nb; ++b) {
+            F->blk[b].run = brun[b];
+            F->blk[b].hall_coef = Pb(b).hall_coef;
+        }
+        F->upload_blocks();
+        HIPCHK(hipMalloc(&d_dt, sizeof(unsigned long long) * nruns));
+        HIPCHK(hipMalloc(&d_dt_next, sizeof(unsigned long long) * nruns));
+        HIPCHK(hipMalloc(&d_status, 4 * sizeof(unsigned long long) * nruns));
+        HIPCHK(hipMemset(d_status, 0, 4 * sizeof(unsigned long long) * nruns));
         HIPCHK(hipMalloc(&d_red, 2 * sizeof(unsigned long long)));
         HIPCHK(hipMalloc(&d_part, 2 * sizeof(double) * divb_partials()));
         // static geometry per block
@@ -728,7 +780,7 @@ public:
             } w;
             w.F = F.get();
             w.b = b;
-            geom[b]->compute(P, w);
+            geom[b]->compute(Pb(b), w);
         }
         // FP32 static copies: reconstruction weights, Hodge stars, edge geometry, taper, eta, photo rate
         for (int c = 0; c < 18; ++c) fslots.push_back(SL_WB + c);
@@ -888,24 +940,43 @@ public:
                 std::vector<double> setv;
                 const double *fill = nullptr;
                 static const double zero3[3] = {0, 0, 0};
+                // per-run far-field values: with one run a constant fill, in an
+                // ensemble explicit per-entry values (one SETV list per component)
+                std::vector<std::vector<int>> fill_d(gf.ncomp);
+                std::vector<std::vector<double>> fill_v(gf.ncomp);
+                auto farfill = [&](int b, int dir, const double *val) {
+                    if (nruns == 1) {
+                        BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA);
+                        fill = val;
+                        return;
+                    }
+                    std::vector<int> d;
+                    BoundaryCore::ghost_list(Fl, b, gf.loc, dir, d);
+                    for (int c = 0; c < gf.ncomp; ++c)
+                        for (int x : d) {
+                            fill_d[c].push_back(x);
+                            fill_v[c].push_back(val[c]);
+                        }
+                };
                 for (int b = 0; b < Fl.nb(); ++b) {
                     const MercBlk &k = Fl.blk[b];
+                    const MercPhys &Q = Pb(b);
                     for (int f = 0; f < 6; ++f) {
                         if (k.bc[f] != MBC_FARFIELD) continue;
                         const int dir = face_dir(f);
                         switch (gf.kind) {
-                        case K_UH: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = P.qinf; break;
-                        case K_UNA: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = P.qinfs; break;
-                        case K_UHE: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = P.qinf_He; break;
+                        case K_UH: farfill(b, dir, Q.qinf); break;
+                        case K_UNA: farfill(b, dir, Q.qinfs); break;
+                        case K_UHE: farfill(b, dir, Q.qinf_He); break;
                         case K_BFACE: break;
-                        case K_BCELL: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = P.B_imf; break;
-                        case K_BIND: BoundaryCore::ghost_list(Fl, b, gf.loc, dir, fillA); fill = zero3; break;
+                        case K_BCELL: farfill(b, dir, Q.B_imf); break;
+                        case K_BIND: farfill(b, dir, zero3); break;
                         case K_EEDGE:
                         case K_ERES:
                         case K_ERES0: {
                             // BC_Farfield_Eedge_set_zerocurl (Boundary_Condition.cpp:573-623), also the
                             // Eres far-field handler (Boundary_Build.cpp:198-200); K_ERES0 sets 0
-                            const double *u = P.q_pv_inf, *Bi = P.B_imf;
+                            const double *u = Q.q_pv_inf, *Bi = Q.B_imf;
                             const bool zero = gf.kind == K_ERES0;
                             const double Ei[3] = {Bi[1] * u[2] - Bi[2] * u[1], Bi[2] * u[0] - Bi[0] * u[2], Bi[0] * u[1] - Bi[1] * u[0]};
                             const int a = gf.loc - LOC_EX;
@@ -958,6 +1029,8 @@ public:
                 }
                 if (gf.owner_sync) cls.owner_pairs(gf.loc, ownE);
                 if (!fillA.empty()) add(A, OP_FILL, sl, std::move(fillA), {}, fill);
+                for (int c = 0; c < gf.ncomp; ++c)
+                    if (!fill_d[c].empty()) add(A, OP_SETV, {sl[c]}, std::move(fill_d[c]), std::move(fill_v[c]));
                 if (!setd.empty()) add(A, OP_SETV, sl, std::move(setd), std::move(setv));
                 add(Bc, OP_COPY, sl, std::move(copyB));
                 if (!wallC.empty()) add(C, OP_WALLF, sl, std::move(wallC), {}, nullptr, gf.kind == K_UNA ? 1 : (gf.kind == K_UHE ? 2 : 0));
@@ -1251,7 +1324,7 @@ public:
         run_merged(MG_DERIVED_PAIR);
         launch_jcell_t(st, R_cell.h, Bd(), Fd(), Gd(), mix());
         sync(G_JCELL);
-        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, final ? 1 : 0, mix(), hall64());
+        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, final ? 1 : 0, mix(), hall64(), nruns);
     }
 
     // After an update: Sync(U), Sync(Bface), the implicit resistive correction
@@ -1284,7 +1357,7 @@ public:
         sync(G_JCELL);
         HIPCHK(hipEventRecord(ev_join2, st2));
         HIPCHK(hipStreamWaitEvent(st, ev_join2, 0));
-        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, final ? 1 : 0, mix(), hall64());
+        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, final ? 1 : 0, mix(), hall64(), nruns);
     }
 
     // StepOnce without the time step, fused passes
@@ -1370,8 +1443,8 @@ public:
     void enqueue_step(int integrator) {
         // the fused forms reduced this step's dt at the end of the previous one;
         // the plain form reduces it here.  The commit applies the abort rule.
-        if (form == 0) launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt_next);
-        launch_dt_commit(st, d_dt, d_dt_next, d_status);
+        if (form == 0) launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt_next, nruns);
+        launch_dt_commit(st, d_dt, d_dt_next, d_status, nruns);
         if (resist) {
             res_dt = read_dt();
             check_status();
@@ -1418,20 +1491,30 @@ public:
     // the device by k_dt_commit: after a stop the state is that of the last
     // completed step and every further step is refused until prepare().
     void check_status() {
-        unsigned long long h[4];
-        HIPCHK(hipMemcpyAsync(h, d_status, sizeof(h), hipMemcpyDeviceToHost, st));
+        std::vector<unsigned long long> h(4 * (size_t)nruns);
+        HIPCHK(hipMemcpyAsync(h.data(), d_status, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (h[0] == 0ULL) return;
-        double bad;
-        std::memcpy(&bad, &h[2], sizeof(bad));
-        char msg[256];
-        std::snprintf(msg, sizeof(msg),
-                      "dt too small/NaN (dt underflow): step %llu since prepare refused dt = %.17g; the state is that of "
-                      "step %llu and the context refuses further steps until prepare",
-                      h[1] + 1ULL, bad, h[1]);
-        throw std::runtime_error(msg);
+        for (int r = 0; r < nruns; ++r) {
+            const unsigned long long *w = &h[4 * (size_t)r];
+            if (w[0] == 0ULL) continue;
+            double bad;
+            std::memcpy(&bad, &w[2], sizeof(bad));
+            char msg[320];
+            std::snprintf(msg, sizeof(msg),
+                          "%sdt too small/NaN (dt underflow): step %llu since prepare refused dt = %.17g; the state is that of "
+                          "step %llu and the context refuses further steps until prepare",
+                          nruns > 1 ? ("run " + std::to_string(r) + ": ").c_str() : "", w[1] + 1ULL, bad, w[1]);
+            throw std::runtime_error(msg);
+        }
     }
-    void clear_status() { HIPCHK(hipMemsetAsync(d_status, 0, 4 * sizeof(unsigned long long), st)); }
+    // the dt each run committed for its last step (0 for a stopped run)
+    void run_dts(double *out) {
+        std::vector<unsigned long long> bits(nruns);
+        HIPCHK(hipMemcpyAsync(bits.data(), d_dt, sizeof(unsigned long long) * nruns, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        std::memcpy(out, bits.data(), sizeof(double) * nruns);
+    }
+    void clear_status() { HIPCHK(hipMemsetAsync(d_status, 0, 4 * sizeof(unsigned long long) * nruns, st)); }
 
     void step(int n, int integrator, double *dt_out) {
         for (int s = 0; s < n; ++s) {
@@ -1503,7 +1586,7 @@ public:
                             else if (a == 1) c = (AA(Ax, i, j, k + 1) - AA(Ax, i, j, k)) + (AA(Az, i, j, k) - AA(Az, i + 1, j, k));
                             else c = (AA(Ax, i, j, k) - AA(Ax, i, j + 1, k)) + (AA(Ay, i + 1, j, k) - AA(Ay, i, j, k));
                             Bf[q] = (Bu[0] * j0 + Bu[1] * j1 + Bu[2] * j2) + c;
-                            Ba[q] = P.B_imf[0] * j0 + P.B_imf[1] * j1 + P.B_imf[2] * j2;
+                            Ba[q] = Pb(b).B_imf[0] * j0 + Pb(b).B_imf[1] * j1 + Pb(b).B_imf[2] * j2;
                         }
                 F->upload_block(SL_B + a, b, Bf.data());
             }
@@ -1525,7 +1608,7 @@ public:
         derived();
         refresh_float_statics();
         if (form >= 1) derived_fused(true);
-        else launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, false, 0);
+        else launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, false, 0, nruns);
         HIPCHK(hipStreamSynchronize(st));
     }
 
@@ -1564,10 +1647,10 @@ public:
             launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_E);
         else if (op == "alias_j")
             launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
-        else if (op == "timestep") launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt);
+        else if (op == "timestep") launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt, nruns);
         else if (op == "stage") stage();
         else if (op == "rhs_cell") launch_rhs_cell(st, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
-        else if (op == "cell_end") launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix(), hall64());
+        else if (op == "cell_end") launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix(), hall64(), nruns);
         else if (op == "eface_u") launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix(), P.ambipolar_enabled != 0.0, nspecies());
         else if (op == "edge_total") launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, alias.n == 0 ? 1 : 0, mix());
         else if (op == "update_all") launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix(), nspecies());
@@ -1601,6 +1684,7 @@ public:
         else if (op == "sync_bcell") sync(G_BCELL);
         else if (op == "sync_jcell") sync(G_JCELL);
         else if (op == "sync_badd") sync(G_BADD);
+        else if (op == "divb") launch_divb(st, R_node.h, Bd(), Fd(), d_part, d_red, 1);  // calc_divB on the device, no read-back
         else throw std::runtime_error("unknown op: " + op);
         HIPCHK(hipGetLastError());
     }
@@ -1690,6 +1774,35 @@ int mercury_create_connected(MercuryContext **out, int nblocks, const int *dims,
 int mercury_create(MercuryContext **out, int nblocks, const int *dims, int nghost, const int *bc, const double *owned_nodes,
                    const MercuryPhysics *phys) {
     return mercury_create_connected(out, nblocks, dims, nghost, bc, nullptr, nullptr, owned_nodes, phys);
+}
+
+int mercury_create_ensemble(MercuryContext **out, int nruns, const int *run_nblocks, const int *dims, int nghost, const int *bc,
+                            const int *nbr, const int *transform, const double *owned_nodes, const MercuryPhysics *phys) {
+    if (!out || nruns <= 0 || !run_nblocks || !dims || !bc || !owned_nodes || !phys) {
+        merc::g_last_error = "mercury_create_ensemble: invalid argument";
+        return -1;
+    }
+    ABI_TRY({
+        std::vector<int> block_run;
+        for (int r = 0; r < nruns; ++r) {
+            if (run_nblocks[r] <= 0) throw std::runtime_error("mercury_create_ensemble: every run needs at least one block");
+            block_run.insert(block_run.end(), run_nblocks[r], r);
+        }
+        std::vector<MercPhys> P(nruns);
+        std::memcpy(P.data(), phys, sizeof(MercPhys) * nruns);
+        std::unique_ptr<MercuryContext> c(new MercuryContext());
+        c->s.reset(new merc::MercurySolver((int)block_run.size(), dims, nghost, bc, nbr, transform, owned_nodes, nruns, block_run.data(),
+                                           P.data()));
+        *out = c.release();
+    })
+}
+
+int mercury_run_dt(MercuryContext *ctx, int *nruns, double *dts) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        if (nruns) *nruns = ctx->s->nruns;
+        if (dts) ctx->s->run_dts(dts);
+    })
 }
 
 int mercury_destroy(MercuryContext *ctx) {

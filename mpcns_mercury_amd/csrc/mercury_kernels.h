@@ -16,7 +16,9 @@ void launch_pv_uplus(hipStream_t st, const RangeH &R, const MercBlk *B, double *
 void launch_bcell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
 void launch_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int s_face = SL_B, int s_edge = SL_J);
 void launch_jcell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);
-void launch_timestep(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P, unsigned long long *dt_bits);
+// dt_bits: one word per run (nruns of them), each the minimum over its run's cells
+void launch_timestep(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P, unsigned long long *dt_bits,
+                     int nruns);
 void launch_flux_div(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P);
 void launch_sources(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P);
 void launch_eface(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F);

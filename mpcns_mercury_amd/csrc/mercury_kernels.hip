@@ -33,7 +33,8 @@ __global__ void k_pv_uplus(const MercRange R, const MercBlk *__restrict__ B, dou
     const long long q = p.q;
     const int ns = nspec(P);
     double pv[MERC_NSMAX][5];
-    for (int s = 0; s < ns; ++s) {
+#pragma unroll
+    for (int s = 0; s < MERC_NSMAX && s < ns; ++s) {
         const int su = SPEC_U(s);
         const double coeff = species_coeff(P, s);
         const double rho = dmax(F[su][q], 1e-12);
@@ -49,7 +50,8 @@ __global__ void k_pv_uplus(const MercRange R, const MercBlk *__restrict__ B, dou
         for (int c = 0; c < 5; ++c) F[sp + c][q] = pv[s][c];
     }
     double r[MERC_NSMAX] = {0.0, 0.0, 0.0};
-    for (int s = 0; s < ns; ++s) r[s] = dmax(F[SPEC_U(s)][q], 0.0);
+#pragma unroll
+    for (int s = 0; s < MERC_NSMAX && s < ns; ++s) r[s] = dmax(F[SPEC_U(s)][q], 0.0);
     const Num n = num_limiter(P, r[0], r[1], r[2], ns);
     for (int c = 0; c < 3; ++c) {
         const double uH = (r[0] > 1e-20) ? pv[0][c] : 0.0;
@@ -144,14 +146,17 @@ __global__ void k_timestep(const MercRange R, const MercBlk *__restrict__ B, dou
                            unsigned long long *__restrict__ dt_bits) {
     gdouble *const *__restrict__ F = gslots(F_);
     double best = 1e100;
+    int run = -1;
     Pt p;
     if (locate(R, B, TID, p)) {
         const long long q = p.q;
+        run = B[p.b].run;
         const double V = fabs(F[SL_JAC][q]);
         if (!(V <= 0.0)) {
             const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
             const double B2 = Bx * Bx + By * By + Bz * Bz;
-            for (int s = 0; s < nspec(P); ++s) {
+#pragma unroll
+            for (int s = 0, ns_ = nspec(P); s < MERC_NSMAX && s < ns_; ++s) {
                 const int su = SPEC_U(s);
                 const double rho = dmax(F[su][q], 1e-2);
                 const double ux = F[su + 1][q] / rho, uy = F[su + 2][q] / rho, uz = F[su + 3][q] / rho;
@@ -185,17 +190,8 @@ __global__ void k_timestep(const MercRange R, const MercBlk *__restrict__ B, dou
             }
         }
     }
-    // wave64 then workgroup minimum
-    for (int off = 32; off > 0; off >>= 1) best = dmin(best, __shfl_xor(best, off, 64));
-    __shared__ double wmin[16];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) wmin[wid] = best;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = wmin[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = dmin(m, wmin[w]);
-        atomicMin(dt_bits, (unsigned long long)__double_as_longlong(m));
-    }
+    // wave64 then workgroup minimum, per run
+    dt_min_runs(best, run, dt_bits);
 }
 
 // ---------------------------------------------------------------------------
@@ -211,7 +207,8 @@ __global__ void k_flux_div(const MercRange R, const MercBlk *__restrict__ B, dou
     const long long q = p.q;
     const double gm = P.gamma;
     const double invJ = 1.0 / F[SL_JAC][q];
-    for (int s = 0; s < nspec(P); ++s) {
+#pragma unroll
+    for (int s = 0, ns_ = nspec(P); s < MERC_NSMAX && s < ns_; ++s) {
         const int su = SPEC_U(s), sp = SPEC_PV(s), sr = SPEC_R(s);
         double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         for (int a = 0; a < 3; ++a) {
@@ -263,7 +260,8 @@ __global__ void k_sources(const MercRange R, const MercBlk *__restrict__ B, doub
     const double rho_cyl = sqrt(y * y + z * z);
     const bool in_shadow = (x < 0.0 && rho_cyl < 1.0);
     const double vst = in_shadow ? P.nu_shadow : P.nu_lit;
-    for (int s = 0; s < ns; ++s) {
+#pragma unroll
+    for (int s = 0; s < MERC_NSMAX && s < ns; ++s) {
         const int su = SPEC_U(s), sp = SPEC_PV(s), sr = SPEC_R(s);
         const double rho = dmax(F[su][q], 0.0);
         const double u = F[sp][q], v = F[sp + 1][q], w = F[sp + 2][q];
@@ -444,7 +442,7 @@ __global__ void k_hall_cell(const MercRange R, const MercBlk *__restrict__ B, do
     LOCATE(R);
     const long long q = p.q;
     const Num n = num_at(P, F, q, nspec(P));
-    const double alpha = P.hall_coef / n.ne_eff;
+    const double alpha = B[p.b].hall_coef / n.ne_eff;
     const double Jx = F[SL_JC][q], Jy = F[SL_JC + 1][q], Jz = F[SL_JC + 2][q];
     const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
     F[SL_EHC][q] = alpha * (Jy * Bz - Jz * By);
@@ -535,7 +533,7 @@ __global__ void k_curl_update(const MercRange R, const MercBlk *__restrict__ B, 
     LOCATE(R);
     const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
-    const double dt = do_update ? *dtp : 0.0;
+    const double dt = do_update ? dtp[bk.run] : 0.0;
     if (do_update && dt == 0.0) return;  // the run is stopped (k_dt_commit)
     if (p.j < bk.m[1] && p.k < bk.m[2]) {
         const double c = (F[SL_E + 1][q] - F[SL_E + 1][q + dk]) + (F[SL_E + 2][q + dj] - F[SL_E + 2][q]);
@@ -563,10 +561,11 @@ __global__ void k_update_u(const MercRange R, const MercBlk *__restrict__ B, dou
     gdouble *const *__restrict__ F = gslots(F_);
     LOCATE(R);
     const long long q = p.q;
-    const double dt = *dtp;
+    const double dt = dtp[B[p.b].run];
     if (dt == 0.0) return;  // the run is stopped (k_dt_commit)
     for (int m = 0; m < 5; ++m)
-        for (int s = 0; s < ns; ++s) F[SPEC_U(s) + m][q] += dt * F[SPEC_R(s) + m][q];
+#pragma unroll
+        for (int s = 0; s < MERC_NSMAX && s < ns; ++s) F[SPEC_U(s) + m][q] += dt * F[SPEC_R(s) + m][q];
 }
 
 // SSP-RK2: save the step-start state / average 0.5*(U0 + U) on owned DOFs
@@ -580,13 +579,15 @@ __global__ void k_rk2(const MercRange R, const MercBlk *__restrict__ B, double *
     if (save) {
         if (cell)
             for (int m = 0; m < 5; ++m)
-                for (int s = 0; s < ns; ++s) F[SPEC_U0(s) + m][q] = F[SPEC_U(s) + m][q];
+#pragma unroll
+                for (int s = 0; s < MERC_NSMAX && s < ns; ++s) F[SPEC_U0(s) + m][q] = F[SPEC_U(s) + m][q];
         for (int a = 0; a < 3; ++a)
             if (face[a]) F[SL_B0 + a][q] = F[SL_B + a][q];
     } else {
         if (cell)
             for (int m = 0; m < 5; ++m)
-                for (int s = 0; s < ns; ++s) F[SPEC_U(s) + m][q] = 0.5 * (F[SPEC_U0(s) + m][q] + F[SPEC_U(s) + m][q]);
+#pragma unroll
+                for (int s = 0; s < MERC_NSMAX && s < ns; ++s) F[SPEC_U(s) + m][q] = 0.5 * (F[SPEC_U0(s) + m][q] + F[SPEC_U(s) + m][q]);
         for (int a = 0; a < 3; ++a)
             if (face[a]) F[SL_B + a][q] = 0.5 * (F[SL_B0 + a][q] + F[SL_B + a][q]);
     }
@@ -802,8 +803,9 @@ __global__ void k_zero_span(double *const *__restrict__ F_, const SlotSet S, lon
 }
 
 // dt word initialisation inside the step
-__global__ void k_dt_init(unsigned long long *dt_bits) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *dt_bits = (unsigned long long)__double_as_longlong(1e100);
+__global__ void k_dt_init(unsigned long long *dt_bits, int nruns) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nruns) dt_bits[r] = (unsigned long long)__double_as_longlong(1e100);
 }
 
 }  // namespace
@@ -823,8 +825,9 @@ void launch_pv_uplus(hipStream_t st, const RangeH &R, const MercBlk *B, double *
 void launch_bcell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F) { LAUNCH_R(k_bcell, R, B, F); }
 void launch_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, int sf, int se) { LAUNCH_R(k_jedge, R, B, F, sf, se); }
 void launch_jcell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F) { LAUNCH_R(k_jcell, R, B, F); }
-void launch_timestep(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P, unsigned long long *dt_bits) {
-    hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(64), 0, st, dt_bits);
+void launch_timestep(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P, unsigned long long *dt_bits,
+                     int nruns) {
+    hipLaunchKernelGGL(k_dt_init, dim3(nblk(nruns, 64)), dim3(64), 0, st, dt_bits, nruns);
     LAUNCH_R(k_timestep, R, B, F, P, dt_bits);
 }
 void launch_flux_div(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const MercPhys &P) { LAUNCH_R(k_flux_div, R, B, F, P); }

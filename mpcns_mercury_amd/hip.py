@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libmercury_hip.so")
 _lib = None
 
 # every symbol declared in include/mercury_hip.h
-ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercury_create_connected", "mercury_destroy",
+ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercury_create_connected", "mercury_create_ensemble", "mercury_run_dt", "mercury_destroy",
                "mercury_field_info",
                "mercury_block_volume", "mercury_block_shape", "mercury_field_device", "mercury_stream", "mercury_total_volume", "mercury_upload", "mercury_download",
                "mercury_init_bface", "mercury_init_bface_dipoles", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
@@ -49,6 +49,9 @@ def load():
     L.mercury_create_connected.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ip, ctypes.c_int, ip, ip, ip, dp,
                                            ctypes.POINTER(MercuryPhysics)]
     L.mercury_destroy.argtypes = [vp]
+    L.mercury_create_ensemble.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ip, ip, ctypes.c_int, ip, ip, ip, dp,
+                                          ctypes.POINTER(MercuryPhysics)]
+    L.mercury_run_dt.argtypes = [vp, ip, dp]
     L.mercury_write_restart.argtypes = [vp, cp, ctypes.c_int, ctypes.c_double, ctypes.c_int]
     L.mercury_read_restart.argtypes = [vp, cp, ip, dp]
     L.mercury_write_rundata.argtypes = [cp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, ctypes.c_double]
@@ -163,7 +166,16 @@ class Mercury:
         bc = case.bc_array()
         nodes = case.nodes_flat()
         ip = ctypes.POINTER(ctypes.c_int)
-        if case.has_interfaces():
+        if "run_nblocks" in case.meta:
+            # an ensemble of independent runs in one context (cases.concat_runs)
+            runs = case.meta["run_nblocks"]
+            nb_arr = np.array(runs, dtype=np.int32)
+            phys = (MercuryPhysics * len(runs))(*case.meta["run_phys"])
+            nbr, trans = case.nbr_array(), case.trans_array()
+            _chk(L.mercury_create_ensemble(ctypes.byref(self._h), len(runs), nb_arr.ctypes.data_as(ip), dims.ctypes.data_as(ip),
+                                           case.g, bc.ctypes.data_as(ip), nbr.ctypes.data_as(ip), trans.ctypes.data_as(ip),
+                                           _dp(nodes), phys))
+        elif case.has_interfaces():
             nbr, trans = case.nbr_array(), case.trans_array()
             _chk(L.mercury_create_connected(ctypes.byref(self._h), case.nb, dims.ctypes.data_as(ip), case.g, bc.ctypes.data_as(ip),
                                             nbr.ctypes.data_as(ip), trans.ctypes.data_as(ip), _dp(nodes),
@@ -224,6 +236,14 @@ class Mercury:
 
     def op(self, name, arg=0.0):
         _chk(load().mercury_op(self._h, name.encode(), float(arg)))
+
+    def run_dts(self):
+        """The dt each run of an ensemble context committed for its last step."""
+        n = ctypes.c_int(0)
+        _chk(load().mercury_run_dt(self._h, ctypes.byref(n), None))
+        out = np.zeros(n.value)
+        _chk(load().mercury_run_dt(self._h, ctypes.byref(n), _dp(out)))
+        return out
 
     def timestep(self):
         v = ctypes.c_double()

@@ -101,3 +101,40 @@ def test_cpp_caller_links_against_the_library():
         build.build_cpp_caller()
     r = subprocess.run(["ldd", "-r", exe], capture_output=True, text=True)
     assert "libmercury_hip.so" in r.stdout and "undefined symbol" not in (r.stdout + r.stderr), r.stdout + r.stderr
+
+
+def _ensemble_args(runs):
+    c = cases.concat_runs(runs)
+    phys = (MercuryPhysics * len(runs))(*c.meta["run_phys"])
+    return c, np.array(c.meta["run_nblocks"], dtype=np.int32), phys
+
+
+def test_ensemble_refuses_runs_with_other_physics():
+    """An ensemble context's runs may differ in hall_coef, inflow and IMF only
+    (checked before anything touches the GPU)."""
+    L = hip.load()
+    runs = [cases.make_ensemble_run(r, scale=0.25) for r in range(2)]
+    runs[1].phys.gamma = 1.4
+    c, nbl, phys = _ensemble_args(runs)
+    ip = ctypes.POINTER(ctypes.c_int)
+    h = ctypes.c_void_p()
+    dims, bc, nbr, tr, nodes = c.dims_array(), c.bc_array(), c.nbr_array(), c.trans_array(), c.nodes_flat()
+    rc = L.mercury_create_ensemble(ctypes.byref(h), 2, nbl.ctypes.data_as(ip), dims.ctypes.data_as(ip), c.g, bc.ctypes.data_as(ip),
+                                   nbr.ctypes.data_as(ip), tr.ctypes.data_as(ip), nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   phys)
+    assert rc == -1 and b"may differ only in hall_coef" in L.mercury_last_error()
+
+
+def test_ensemble_refuses_an_interface_between_runs():
+    L = hip.load()
+    a = cases.make_split_cube(n=(8, 8, 8), split=(2, 1, 1), g=4)
+    b = cases.make_periodic_cube(n=(8, 8, 8), g=4)
+    c, nbl, phys = _ensemble_args([a, b])
+    nbl[:] = (1, 2)  # declare the first run as one block: its interface now reaches into run 1
+    ip = ctypes.POINTER(ctypes.c_int)
+    h = ctypes.c_void_p()
+    dims, bc, nbr, tr, nodes = c.dims_array(), c.bc_array(), c.nbr_array(), c.trans_array(), c.nodes_flat()
+    rc = L.mercury_create_ensemble(ctypes.byref(h), 2, nbl.ctypes.data_as(ip), dims.ctypes.data_as(ip), c.g, bc.ctypes.data_as(ip),
+                                   nbr.ctypes.data_as(ip), tr.ctypes.data_as(ip), nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   phys)
+    assert rc == -1 and b"joins two runs" in L.mercury_last_error()
