@@ -326,46 +326,89 @@ def main():
         g.set_form(1)
         g.graph_capture(case.integrator)
     # batched ensemble of independent single-fluid + Hall runs (seeded sizes in
-    # [32,96]^3): one graph steps every member; serial member graphs for comparison
+    # [32,96]^3): every run's blocks in one context, one launch per pass (per-run
+    # physics, dt and status); at half scale also against separate member
+    # contexts stepped by one forked graph and one after another
     ensemble = None
     if args.ensemble_runs > 0:
         from mpcns_mercury_amd.hip import Ensemble  # noqa: E402
         t0 = time.perf_counter()
         runs = [cases.make_ensemble_run(r) for r in range(args.ensemble_runs)]
         ecells = sum(c.ncells() for c in runs)
-        members = [Mercury(c) for c in runs]
+        eg = Mercury(cases.concat_runs(runs))
+        eg.prepare()
+        ensemble = {"runs": args.ensemble_runs, "cells": ecells, "blocks": sum(c.nb for c in runs),
+                    "setup_s": time.perf_counter() - t0, "form": "one context, one launch per pass over every run"}
+        for form, label in ((1, "fp64_fused"), (2, "mixed")):
+            eg.set_form(form)
+            eg.graph_capture(0)
+            eg.graph_launch(3)
+            eg.synchronize()
+            t1 = time.perf_counter()
+            eg.graph_launch(20)
+            eg.synchronize()
+            el = time.perf_counter() - t1
+            ensemble[label] = {"ms_per_step": el * 1e3 / 20, "cell_updates_per_s": ecells * 20 / el}
+            print(f"[bench] ensemble {label}: {args.ensemble_runs} runs, {ecells} cells in one context: {el * 1e3 / 20:.3f} ms/step "
+                  f"({ecells * 20 / el:.3e} cell-updates/s)", file=sys.stderr, flush=True)
+        ensemble["mixed_over_fp64"] = ensemble["mixed"]["cell_updates_per_s"] / ensemble["fp64_fused"]["cell_updates_per_s"]
+        eg.close()
+        del runs
+        # half scale: one context vs member contexts (forked graph, and member graphs one after another)
+        hruns = [cases.make_ensemble_run(r, scale=0.5) for r in range(args.ensemble_runs)]
+        hcells = sum(c.ncells() for c in hruns)
+        eg = Mercury(cases.concat_runs(hruns))
+        eg.prepare()
+        members = [Mercury(c) for c in hruns]
         for m in members:
             m.prepare()
-        ensemble = {"runs": args.ensemble_runs, "cells": ecells, "blocks": sum(c.nb for c in runs)}
-        for form, label in ((1, "fp64_fused"), (2, "mixed")):
+        half = {"runs": args.ensemble_runs, "cells": hcells, "form": "fp64_fused"}
+        eg.set_form(1)
+        eg.graph_capture(0)
+        eg.graph_launch(5)
+        for m in members:
+            m.set_form(1)
+            m.graph_capture(0)
+            m.graph_launch(5)
+        eg.synchronize()
+        for m in members:
+            m.synchronize()
+        # the same five steps: every run of the one context equals its own member context
+        base = 0
+        same = True
+        for c, m in zip(hruns, members):
+            for ib in range(c.nb):
+                for name in ("U_H", "B_xi"):
+                    same = same and np.array_equal(eg.get(name, base + ib), m.get(name, ib))
+            base += c.nb
+        half["one_context_equals_members"] = bool(same)
+        t1 = time.perf_counter()
+        eg.graph_launch(20)
+        eg.synchronize()
+        half["one_context_ms_per_step"] = (time.perf_counter() - t1) * 1e3 / 20
+        ens = Ensemble(members, 0)
+        ens.launch(2)
+        t1 = time.perf_counter()
+        ens.launch(20)
+        half["forked_graph_ms_per_step"] = (time.perf_counter() - t1) * 1e3 / 20
+        ens.close()
+        for m in members:
+            m.graph_capture(0)
+        t1 = time.perf_counter()
+        for _ in range(10):
             for m in members:
-                m.set_form(form)
-            ens = Ensemble(members, 0)
-            ens.launch(3)
-            t1 = time.perf_counter()
-            ens.launch(20)
-            el = time.perf_counter() - t1
-            ens.close()
-            for m in members:
-                m.graph_capture(0)
-                m.graph_launch(2)
-            for m in members:
-                m.synchronize()
-            t1 = time.perf_counter()
-            for _ in range(5):
-                for m in members:
-                    m.graph_launch(1)
-            for m in members:
-                m.synchronize()
-            el_serial = (time.perf_counter() - t1) / 5
-            ensemble[label] = {"ms_per_step": el * 1e3 / 20, "cell_updates_per_s": ecells * 20 / el,
-                               "serial_member_graphs_ms_per_step": el_serial * 1e3}
-            print(f"[bench] ensemble {label}: {args.ensemble_runs} runs, {ecells} cells: {el * 1e3 / 20:.3f} ms/step "
-                  f"({ecells * 20 / el:.3e} cell-updates/s), member graphs one after another {el_serial * 1e3:.3f} ms",
-                  file=sys.stderr, flush=True)
-        ensemble["mixed_over_fp64"] = ensemble["mixed"]["cell_updates_per_s"] / ensemble["fp64_fused"]["cell_updates_per_s"]
+                m.graph_launch(1)
+        for m in members:
+            m.synchronize()
+        half["serial_member_graphs_ms_per_step"] = (time.perf_counter() - t1) * 1e3 / 10
+        half["one_context_over_serial"] = half["serial_member_graphs_ms_per_step"] / half["one_context_ms_per_step"]
+        ensemble["half_scale"] = half
+        print(f"[bench] ensemble half scale ({hcells} cells): one context {half['one_context_ms_per_step']:.3f} ms/step, forked graph "
+              f"{half['forked_graph_ms_per_step']:.3f}, member graphs one after another {half['serial_member_graphs_ms_per_step']:.3f}; "
+              f"runs equal their members: {same}", file=sys.stderr, flush=True)
         for m in members:
             m.close()
+        eg.close()
         print(f"[bench] ensemble setup+run {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     # resistive shell: 256 x 192 x 384 cells in 2 x 2 blocks glued by interfaces, two-fluid
     # Hall MHD with the implicit resistive correction on every edge (eta = 1e-3 exp(-(r-1)/0.1))
