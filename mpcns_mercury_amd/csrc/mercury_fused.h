@@ -7,6 +7,10 @@ namespace merc {
 // R: per block ceil(m0 / L) x m1 x m2 points; each thread marches over L cells along i
 void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                      int L, bool mix);
+// R: per block ceil(m0 / C) x ceil(m1 / 16) x ceil(m2 / 16) tiles, one workgroup
+// each, marching over C planes along i (the LDS-tiled flux pass)
+void launch_rhs_tile(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
+                     int C, bool mix);
 // dt_next: one word per run (nruns of them), reduced per run
 void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
                      unsigned long long *dt_next, int compute_dt, bool mix, int hall64, int nruns);

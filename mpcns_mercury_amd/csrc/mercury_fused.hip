@@ -71,18 +71,18 @@ template <bool MIX> struct Acc {
     DEVI double value() const { return MIX ? (double)s - (double)c : (double)s; }
 };
 
+// Rusanov flux of one face from the two sides' primitive values
+// v = (rho, u, v, w, p) and the face metric (k1, k2, k3)
 template <bool MIX>
-DEVI void rusanov_face_t(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, int su, int sp, long long qL, long long qR,
-                         long long qf, int a, typename Prec<MIX>::T gm, typename Prec<MIX>::T out[5]) {
+DEVI void rusanov_vals(typename Prec<MIX>::T k1, typename Prec<MIX>::T k2, typename Prec<MIX>::T k3, const typename Prec<MIX>::T vL[5],
+                       const typename Prec<MIX>::T vR[5], typename Prec<MIX>::T gm, typename Prec<MIX>::T out[5]) {
     using T = typename Prec<MIX>::T;
     using PR = Prec<MIX>;
-    const T k1 = (T)F[SL_JD + 3 * a][qf], k2 = (T)F[SL_JD + 3 * a + 1][qf], k3 = (T)F[SL_JD + 3 * a + 2][qf];
     T Uc[2][5], Fl[2][5], rad[2];
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
-        const long long qq = side ? qR : qL;
-        const T rho = (T)F[su][qq], u = st<MIX>(F, G, sp, qq), v = st<MIX>(F, G, sp + 1, qq), w = st<MIX>(F, G, sp + 2, qq);
-        const T pr = (T)F[sp + 3][qq];
+        const T *v5 = side ? vR : vL;
+        const T rho = v5[0], u = v5[1], v = v5[2], w = v5[3], pr = v5[4];
         Uc[side][0] = rho;
         Uc[side][1] = rho * u;
         Uc[side][2] = rho * v;
@@ -102,6 +102,110 @@ DEVI void rusanov_face_t(gdouble *const *__restrict__ F, const gfloat *const *__
     const T r = tmax(rad[0], rad[1]);
 #pragma unroll
     for (int m = 0; m < 5; ++m) out[m] = T(0.5) * (Fl[0][m] + Fl[1][m]) - T(0.5) * r * (Uc[1][m] - Uc[0][m]);
+}
+
+// the primitive values of species (su, sp) at point qq as the flux reads them
+template <bool MIX>
+DEVI void face_side(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, int su, int sp, long long qq,
+                    typename Prec<MIX>::T v5[5]) {
+    using T = typename Prec<MIX>::T;
+    v5[0] = (T)F[su][qq];
+    v5[1] = st<MIX>(F, G, sp, qq);
+    v5[2] = st<MIX>(F, G, sp + 1, qq);
+    v5[3] = st<MIX>(F, G, sp + 2, qq);
+    v5[4] = (T)F[sp + 3][qq];
+}
+
+template <bool MIX>
+DEVI void rusanov_face_t(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, int su, int sp, long long qL, long long qR,
+                         long long qf, int a, typename Prec<MIX>::T gm, typename Prec<MIX>::T out[5]) {
+    using T = typename Prec<MIX>::T;
+    const T k1 = (T)F[SL_JD + 3 * a][qf], k2 = (T)F[SL_JD + 3 * a + 1][qf], k3 = (T)F[SL_JD + 3 * a + 2][qf];
+    T vL[5], vR[5];
+    face_side<MIX>(F, G, su, sp, qL, vL);
+    face_side<MIX>(F, G, su, sp, qR, vR);
+    rusanov_vals<MIX>(k1, k2, k3, vL, vR, gm, out);
+}
+
+// AddSourceToRHS_Fluid (MercurySolver_FluidRHS.cpp:206-527) at owned cell q:
+// adds the sources to the flux divergence rr and writes the fluid RHS.  pe(o)
+// is the electron pressure stand-in at the neighbour o = 0..5 (+i, -i, +j, -j,
+// +k, -k) of the ambipolar gradient.
+template <bool MIX, int NS, typename PE>
+DEVI void rhs_sources(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, const MercPhys &P, long long q, long long di,
+                      long long dj, double (&rr)[NS][5], PE pe) {
+    const double upx = st<MIX>(F, G, SL_UP, q), upy = st<MIX>(F, G, SL_UP + 1, q), upz = st<MIX>(F, G, SL_UP + 2, q);
+    const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
+    const double Jx = F[SL_JC][q], Jy = F[SL_JC + 1][q], Jz = F[SL_JC + 2][q];
+    const double sjbx = Jy * Bz - Jz * By, sjby = Jz * Bx - Jx * Bz, sjbz = Jx * By - Jy * Bx;
+    const Num num = num_at(P, F, q, NS);
+    double dpex = 0.0, dpey = 0.0, dpez = 0.0;
+    const bool ambi = P.ambipolar_enabled != 0.0;
+    if (ambi) {
+        const double V = fabs(F[SL_JAC][q]);
+        double d[9];
+        if (V <= 0.0) {
+            for (int t = 0; t < 9; ++t) d[t] = 0.0;
+        } else {
+            for (int c = 0; c < 3; ++c) {
+                d[c] = 0.5 * (F[SL_JD + c][q - di] + F[SL_JD + c][q]) / V;
+                d[3 + c] = 0.5 * (F[SL_JD + 3 + c][q - dj] + F[SL_JD + 3 + c][q]) / V;
+                d[6 + c] = 0.5 * (F[SL_JD + 6 + c][q - 1] + F[SL_JD + 6 + c][q]) / V;
+            }
+        }
+        const double pip = pe(0), pim = pe(1), pjp = pe(2), pjm = pe(3), pkp = pe(4), pkm = pe(5);
+        dpex = 0.5 * (d[0] * (pip - pim) + d[3] * (pjp - pjm) + d[6] * (pkp - pkm));
+        dpey = 0.5 * (d[1] * (pip - pim) + d[4] * (pjp - pjm) + d[7] * (pkp - pkm));
+        dpez = 0.5 * (d[2] * (pip - pim) + d[5] * (pjp - pjm) + d[8] * (pkp - pkm));
+    }
+    const double sss = (double)st<MIX>(F, G, SL_PHOTO, q);
+    const double x = F[SL_CC][q], y = F[SL_CC + 1][q], z = F[SL_CC + 2][q];
+    const double rho_cyl = sqrt(y * y + z * z);
+    const bool in_shadow = (x < 0.0 && rho_cyl < 1.0);
+    const double vst = in_shadow ? P.nu_shadow : P.nu_lit;
+    for (int s = 0; s < NS; ++s) {
+        const int su = SPEC_U(s), sp = SPEC_PV(s), sr = SPEC_R(s);
+        const double rho = dmax(F[su][q], 0.0);
+        const double u = st<MIX>(F, G, sp, q), v = st<MIX>(F, G, sp + 1, q), w = st<MIX>(F, G, sp + 2, q);
+        const double u2 = u * u + v * v + w * w;
+        const double subx = (v - upy) * Bz - (w - upz) * By;
+        const double suby = (w - upz) * Bx - (u - upx) * Bz;
+        const double subz = (u - upx) * By - (v - upy) * Bx;
+        const double subu = subx * u + suby * v + subz * w;
+        const double sjbu = sjbx * u + sjby * v + sjbz * w;
+        const double dpeu = dpex * u + dpey * v + dpez * w;
+        const double Ts0 = (double)st<MIX>(F, G, sp + 4, q) * P.T_ref;
+        const double Ms = species_mass(P, s);
+        const double Mo = P.M_Na;
+        const double b1 = vst * (P.Tn0 - Ts0) * (Ms / (Ms + Mo));
+        const double b2 = (Mo / (Ms + Mo)) * vst;
+        // number density in the thermal exchange, charge density in the induction force
+        const double nsp = num.n[s], qs = num.q[s];
+        const double chi = num.w[s];
+        double r0 = rr[s][0], r1 = rr[s][1], r2 = rr[s][2], r3 = rr[s][3], r4 = rr[s][4];
+        r1 += P.c_ind * qs * subx + P.c_hall * chi * sjbx;
+        r2 += P.c_ind * qs * suby + P.c_hall * chi * sjby;
+        r3 += P.c_ind * qs * subz + P.c_hall * chi * sjbz;
+        r4 += P.c_ind * qs * subu + P.c_hall * chi * sjbu;
+        if (ambi) {
+            r1 -= chi * dpex;
+            r2 -= chi * dpey;
+            r3 -= chi * dpez;
+            r4 -= chi * dpeu;
+        }
+        if (s == 1) {
+            r0 += P.a1_Na * sss;
+            r4 += P.a6 * sss * P.Tn0;
+        }
+        r1 -= P.a4 * rho * u * vst;
+        r2 -= P.a4 * rho * v * vst;
+        r3 -= P.a4 * rho * w * vst;
+        r4 -= P.a4 * rho * u2 * vst;
+        r4 += P.a5 * nsp * b1;
+        r4 += P.a4 * rho * u2 * b2;
+        wr<MIX>(F, G, sr, q, r0); wr<MIX>(F, G, sr + 1, q, r1); wr<MIX>(F, G, sr + 2, q, r2);
+        wr<MIX>(F, G, sr + 3, q, r3); wr<MIX>(F, G, sr + 4, q, r4);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -158,81 +262,221 @@ __global__ void __launch_bounds__(256) k_rhs_cell(const MercRange R, const MercB
             }
             for (int m = 0; m < 5; ++m) rr[s][m] = r[m].value();
         }
-        // --- AddSourceToRHS_Fluid (MercurySolver_FluidRHS.cpp:206-527)
-        const double upx = st<MIX>(F, G, SL_UP, q), upy = st<MIX>(F, G, SL_UP + 1, q), upz = st<MIX>(F, G, SL_UP + 2, q);
-        const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
-        const double Jx = F[SL_JC][q], Jy = F[SL_JC + 1][q], Jz = F[SL_JC + 2][q];
-        const double sjbx = Jy * Bz - Jz * By, sjby = Jz * Bx - Jx * Bz, sjbz = Jx * By - Jy * Bx;
-        const Num num = num_at(P, F, q, NS);
-        double dpex = 0.0, dpey = 0.0, dpez = 0.0;
-        const bool ambi = P.ambipolar_enabled != 0.0;
-        if (ambi) {
-            const double V = fabs(F[SL_JAC][q]);
-            double d[9];
-            if (V <= 0.0) {
-                for (int t = 0; t < 9; ++t) d[t] = 0.0;
-            } else {
-                for (int c = 0; c < 3; ++c) {
-                    d[c] = 0.5 * (F[SL_JD + c][q - di] + F[SL_JD + c][q]) / V;
-                    d[3 + c] = 0.5 * (F[SL_JD + 3 + c][q - dj] + F[SL_JD + 3 + c][q]) / V;
-                    d[6 + c] = 0.5 * (F[SL_JD + 6 + c][q - 1] + F[SL_JD + 6 + c][q]) / V;
+        rhs_sources<MIX, NS>(F, G, P, q, di, dj, rr, [&](int o) {
+            const long long d = o == 0 ? di : o == 1 ? -di : o == 2 ? dj : o == 3 ? -dj : o == 4 ? 1LL : -1LL;
+            return pe_sum(F, q + d, NS);
+        });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Scheme_U_ + AddSourceToRHS_Fluid, LDS-tiled.  A workgroup owns a 16 x 16
+// tile of (j, k) columns of one block, one column per thread, and marches over
+// up to C planes along i.  Each plane's flux operands (rho, u, v, w, p per
+// species, as the flux reads them) are staged in LDS with the tile's one-cell
+// j/k halo ring; every j and k face of the tile is evaluated once, by one
+// thread, and reaches both of its cells through LDS (the column kernel
+// evaluates each such face twice, from global memory).  The i face is carried
+// along the column as in k_rhs_cell, and the next plane is loaded while the
+// current one is in use.  Same operands, arithmetic and summation order as
+// k_rhs_cell, so the two kernels agree bit for bit.  The launch range has
+// ceil(m0 / C) x ceil(m1 / 16) x ceil(m2 / 16) points per block, one
+// workgroup each.
+constexpr int RT_J = 16, RT_K = 16;
+constexpr int RT_W = RT_K + 2;                   // staged row: the tile's k range and its two halo cells
+constexpr int RT_S = (RT_J + 2) * RT_W;          // staged plane
+constexpr int RT_FJ = (RT_J + 1) * RT_K;         // j faces of a tile plane
+constexpr int RT_F = RT_FJ + RT_J * (RT_K + 1);  // and the k faces
+static_assert(RT_J * RT_K == 256 && RT_J == 16 && RT_K == 16, "one thread per column, 64 halo cells");
+static_assert(RT_F > 512 && RT_F <= 768, "three face slots per thread");
+
+template <bool MIX, int NS>
+__global__ void __launch_bounds__(256) k_rhs_tile(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
+                                                  const float *const *__restrict__ G_, const MercPhys P, int C) {
+    gdouble *const *__restrict__ F = gslots(F_);
+    const gfloat *const *__restrict__ G = gslots(G_);
+    using T = typename Prec<MIX>::T;
+    __shared__ T s_v[NS * 5][RT_S];  // flux operands of the staged plane
+    __shared__ double s_pe[RT_S];     // electron pressure stand-in of the staged plane
+    // face fluxes of SB species at a time: all of them where the LDS allows
+    // (FP64 three-species: one, 63 KB with the staged plane)
+    constexpr int SB = (MIX || NS == 2) ? NS : 1;
+    __shared__ T s_f[SB][5][RT_F];
+    Pt p;
+    if (!locate(R, B, xcd_block(), p)) return;  // the tile: uniform over the workgroup
+    const MercBlk &bk = B[p.b];
+    const int tid = threadIdx.x, tj = tid / RT_K, tk = tid % RT_K;
+    const int j0 = p.j * RT_J, k0 = p.k * RT_K, i0 = p.i * C;
+    const int i1 = (bk.m[0] < i0 + C) ? bk.m[0] : i0 + C;
+    const int m1 = bk.m[1], m2 = bk.m[2], g = bk.g;
+    const long long di = bk.sI, dj = bk.sJ;
+    // staged cell (r, c) of plane i is the point (i, j0 - 1 + r, k0 - 1 + c);
+    // base0 addresses (i0, j0 - 1, k0 - 1)
+    const long long base0 = bk.off + (long long)(i0 + g) * di + (long long)(j0 - 1 + g) * dj + (k0 - 1 + g);
+    const int c_own = (tj + 1) * RT_W + (tk + 1);
+    const long long own_off = (long long)(tj + 1) * dj + (tk + 1);
+    const bool own = (j0 + tj < m1) && (k0 + tk < m2);
+    // the staged points are the tile's cells up to index m (the upper halo of
+    // a partial tile) and the halo ring; one in four threads stages a ring cell
+    const bool ld = (j0 + tj <= m1) && (k0 + tk <= m2);
+    int c_h = -1;
+    long long h_off = 0;
+    if ((tid & 3) == 0) {
+        const int h = tid >> 2, e = h & 15, side = h >> 4;
+        const int r = side == 0 ? 0 : (side == 1 ? RT_J + 1 : e + 1);
+        const int c = side < 2 ? e + 1 : (side == 2 ? 0 : RT_K + 1);
+        if (j0 - 1 + r <= m1 && k0 - 1 + c <= m2) {
+            c_h = r * RT_W + c;
+            h_off = (long long)r * dj + c;
+        }
+    }
+    // the faces this thread evaluates: f = tid, tid + 256 and tid + 512 of the
+    // tile plane's RT_F faces (j faces first, then k faces); face f of axis a
+    // lies between the staged cells fl and fr and is the lower face of fr
+    int fl[3], fr[3], fa[3];
+    long long fo[3];
+    bool fv[3];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        const int f = tid + 256 * u;
+        int r0, c0;
+        if (f < RT_FJ) {
+            r0 = f / RT_K;
+            c0 = f % RT_K + 1;
+            fl[u] = r0 * RT_W + c0;
+            fr[u] = fl[u] + RT_W;
+            fo[u] = (long long)(r0 + 1) * dj + c0;
+            fa[u] = 1;
+            fv[u] = (j0 + r0 <= m1) && (k0 + c0 - 1 < m2);
+        } else {
+            const int e = f - RT_FJ;
+            r0 = e / (RT_K + 1) + 1;
+            c0 = e % (RT_K + 1);
+            fl[u] = r0 * RT_W + c0;
+            fr[u] = fl[u] + 1;
+            fo[u] = (long long)r0 * dj + c0 + 1;
+            fa[u] = 2;
+            fv[u] = (f < RT_F) && (j0 + r0 - 1 < m1) && (k0 + c0 <= m2);
+        }
+    }
+    const int fj_lo = tj * RT_K + tk, fk_lo = RT_FJ + tj * (RT_K + 1) + tk;
+    const T gm = (T)P.gamma;
+    auto load = [&](long long qq, T v[NS * 5], double &pe) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) face_side<MIX>(F, G, SPEC_U(s), SPEC_PV(s), qq, v + 5 * s);
+        pe = pe_sum(F, qq, NS);
+    };
+    auto stage = [&](int c, const T v[NS * 5], double pe) {
+#pragma unroll
+        for (int t = 0; t < NS * 5; ++t) s_v[t][c] = v[t];
+        s_pe[c] = pe;
+    };
+    // --- Scheme_U_ (MercurySolver_FluidRHS.cpp:3-202)
+    T vn[NS * 5], vh[NS * 5];  // the next plane's own and ring cells
+    double pen = 0.0, peh = 0.0, pe_m = 0.0;
+    T Fx[NS][5];  // i-1/2 flux of the current cell, per species
+    if (own) {
+        T vm[NS * 5];
+        load(base0 - di + own_off, vm, pe_m);
+        load(base0 + own_off, vn, pen);
+        const long long q = base0 + own_off;
+        const T k1 = (T)F[SL_JD][q], k2 = (T)F[SL_JD + 1][q], k3 = (T)F[SL_JD + 2][q];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) rusanov_vals<MIX>(k1, k2, k3, vm + 5 * s, vn + 5 * s, gm, Fx[s]);
+    } else if (ld) {
+        load(base0 + own_off, vn, pen);
+    }
+    if (c_h >= 0) load(base0 + h_off, vh, peh);
+    if (ld) stage(c_own, vn, pen);
+    if (c_h >= 0) stage(c_h, vh, peh);
+    __syncthreads();
+    for (int i = i0; i < i1; ++i) {
+        const long long base = base0 + (long long)(i - i0) * di;
+        const long long q = base + own_off;
+        const double pe_c = pen;
+        // plane i + 1 (a plane of the box: i + 1 <= m0)
+        if (ld) load(q + di, vn, pen);
+        if (c_h >= 0) load(base + di + h_off, vh, peh);
+        T km[3][3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            if (!fv[u]) continue;
+            // the two candidate arrays are uniform: a per-lane select of the
+            // pointer, not a per-lane load from the slot table
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const gdouble *pj = F[SL_JD + 3 + c], *pk = F[SL_JD + 6 + c];
+                km[u][c] = (T)(fa[u] == 2 ? pk : pj)[base + fo[u]];
+            }
+        }
+        T ki[3] = {T(0), T(0), T(0)};
+        T invJ = T(0);
+        double pjp = 0.0, pjm = 0.0, pkp = 0.0, pkm = 0.0;
+        if (own) {
+            ki[0] = (T)F[SL_JD][q + di];
+            ki[1] = (T)F[SL_JD + 1][q + di];
+            ki[2] = (T)F[SL_JD + 2][q + di];
+            invJ = (T)(1.0 / F[SL_JAC][q]);
+            pjp = s_pe[c_own + RT_W];
+            pjm = s_pe[c_own - RT_W];
+            pkp = s_pe[c_own + 1];
+            pkm = s_pe[c_own - 1];
+        }
+        double rr[NS][5];
+#pragma unroll
+        for (int s0 = 0; s0 < NS; s0 += SB) {
+            if (s0 > 0) __syncthreads();  // the previous batch's fluxes are consumed
+#pragma unroll
+            for (int s = s0; s < s0 + SB; ++s) {
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    if (!fv[u]) continue;
+                    T vL[5], vR[5], out[5];
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) {
+                        vL[m] = s_v[5 * s + m][fl[u]];
+                        vR[m] = s_v[5 * s + m][fr[u]];
+                    }
+                    rusanov_vals<MIX>(km[u][0], km[u][1], km[u][2], vL, vR, gm, out);
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) s_f[s - s0][m][tid + 256 * u] = out[m];
                 }
             }
-    #define PE(o) pe_sum(F, q + (o), NS)
-            const double pip = PE(di), pim = PE(-di), pjp = PE(dj), pjm = PE(-dj), pkp = PE(1), pkm = PE(-1);
-    #undef PE
-            dpex = 0.5 * (d[0] * (pip - pim) + d[3] * (pjp - pjm) + d[6] * (pkp - pkm));
-            dpey = 0.5 * (d[1] * (pip - pim) + d[4] * (pjp - pjm) + d[7] * (pkp - pkm));
-            dpez = 0.5 * (d[2] * (pip - pim) + d[5] * (pjp - pjm) + d[8] * (pkp - pkm));
-        }
-        const double sss = (double)st<MIX>(F, G, SL_PHOTO, q);
-        const double x = F[SL_CC][q], y = F[SL_CC + 1][q], z = F[SL_CC + 2][q];
-        const double rho_cyl = sqrt(y * y + z * z);
-        const bool in_shadow = (x < 0.0 && rho_cyl < 1.0);
-        const double vst = in_shadow ? P.nu_shadow : P.nu_lit;
-        for (int s = 0; s < NS; ++s) {
-            const int su = SPEC_U(s), sp = SPEC_PV(s), sr = SPEC_R(s);
-            const double rho = dmax(F[su][q], 0.0);
-            const double u = st<MIX>(F, G, sp, q), v = st<MIX>(F, G, sp + 1, q), w = st<MIX>(F, G, sp + 2, q);
-            const double u2 = u * u + v * v + w * w;
-            const double subx = (v - upy) * Bz - (w - upz) * By;
-            const double suby = (w - upz) * Bx - (u - upx) * Bz;
-            const double subz = (u - upx) * By - (v - upy) * Bx;
-            const double subu = subx * u + suby * v + subz * w;
-            const double sjbu = sjbx * u + sjby * v + sjbz * w;
-            const double dpeu = dpex * u + dpey * v + dpez * w;
-            const double Ts0 = (double)st<MIX>(F, G, sp + 4, q) * P.T_ref;
-            const double Ms = species_mass(P, s);
-            const double Mo = P.M_Na;
-            const double b1 = vst * (P.Tn0 - Ts0) * (Ms / (Ms + Mo));
-            const double b2 = (Mo / (Ms + Mo)) * vst;
-            // number density in the thermal exchange, charge density in the induction force
-            const double nsp = num.n[s], qs = num.q[s];
-            const double chi = num.w[s];
-            double r0 = rr[s][0], r1 = rr[s][1], r2 = rr[s][2], r3 = rr[s][3], r4 = rr[s][4];
-            r1 += P.c_ind * qs * subx + P.c_hall * chi * sjbx;
-            r2 += P.c_ind * qs * suby + P.c_hall * chi * sjby;
-            r3 += P.c_ind * qs * subz + P.c_hall * chi * sjbz;
-            r4 += P.c_ind * qs * subu + P.c_hall * chi * sjbu;
-            if (ambi) {
-                r1 -= chi * dpex;
-                r2 -= chi * dpey;
-                r3 -= chi * dpez;
-                r4 -= chi * dpeu;
+            __syncthreads();
+            if (own) {
+#pragma unroll
+                for (int s = s0; s < s0 + SB; ++s) {
+                    Acc<MIX> r[5];
+                    T vc[5], Fp[5];
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) vc[m] = s_v[5 * s + m][c_own];
+                    rusanov_vals<MIX>(ki[0], ki[1], ki[2], vc, vn + 5 * s, gm, Fp);
+                    const T(&f)[5][RT_F] = s_f[s - s0];
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) {
+                        r[m].sub((Fp[m] - Fx[s][m]) * invJ);
+                        Fx[s][m] = Fp[m];
+                    }
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) r[m].sub((f[m][fj_lo + RT_K] - f[m][fj_lo]) * invJ);
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) r[m].sub((f[m][fk_lo + 1] - f[m][fk_lo]) * invJ);
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) rr[s][m] = r[m].value();
+                }
             }
-            if (s == 1) {
-                r0 += P.a1_Na * sss;
-                r4 += P.a6 * sss * P.Tn0;
-            }
-            r1 -= P.a4 * rho * u * vst;
-            r2 -= P.a4 * rho * v * vst;
-            r3 -= P.a4 * rho * w * vst;
-            r4 -= P.a4 * rho * u2 * vst;
-            r4 += P.a5 * nsp * b1;
-            r4 += P.a4 * rho * u2 * b2;
-            wr<MIX>(F, G, sr, q, r0); wr<MIX>(F, G, sr + 1, q, r1); wr<MIX>(F, G, sr + 2, q, r2);
-            wr<MIX>(F, G, sr + 3, q, r3); wr<MIX>(F, G, sr + 4, q, r4);
         }
+        // after the last barrier each thread reads only its own staged cell,
+        // which only it rewrites below
+        if (own) {
+            rhs_sources<MIX, NS>(F, G, P, q, di, dj, rr, [&](int o) {
+                return o == 0 ? pen : o == 1 ? pe_m : o == 2 ? pjp : o == 3 ? pjm : o == 4 ? pkp : pkm;
+            });
+        }
+        pe_m = pe_c;
+        if (ld) stage(c_own, vn, pen);
+        if (c_h >= 0) stage(c_h, vh, peh);
+        __syncthreads();
     }
 }
 
@@ -790,6 +1034,19 @@ void launch_rhs_cell(hipStream_t st, const RangeH &R, const MercBlk *B, double *
     } else {
         if (three) LAUNCH_R((k_rhs_cell<false, 3>), R, B, F, G, P, L);
         else LAUNCH_R((k_rhs_cell<false, 2>), R, B, F, G, P, L);
+    }
+}
+void launch_rhs_tile(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
+                     int C, bool mix) {
+    if (R.total <= 0) return;
+    const dim3 grid((unsigned)R.total), blk(256);
+    const bool three = P.n_species >= 3.0;
+    if (mix) {
+        if (three) hipLaunchKernelGGL((k_rhs_tile<true, 3>), grid, blk, 0, st, R.dev, B, F, G, P, C);
+        else hipLaunchKernelGGL((k_rhs_tile<true, 2>), grid, blk, 0, st, R.dev, B, F, G, P, C);
+    } else {
+        if (three) hipLaunchKernelGGL((k_rhs_tile<false, 3>), grid, blk, 0, st, R.dev, B, F, G, P, C);
+        else hipLaunchKernelGGL((k_rhs_tile<false, 2>), grid, blk, 0, st, R.dev, B, F, G, P, C);
     }
 }
 void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,

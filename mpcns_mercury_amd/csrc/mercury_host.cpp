@@ -634,8 +634,10 @@ public:
     std::vector<std::unique_ptr<BlockGeom>> geom;
     RangeOwner R_cell, R_box, R_node, R_eface;
     RangeOwner R_res;  // node-box points around the resistive unknowns (the CG's curls)
-    RangeOwner R_march;  // flux pass: ceil(m0 / march) x m1 x m2 per block
+    RangeOwner R_march;  // flux pass: ceil(m0 / march) x m1 x m2 per block (column kernel)
     int march = 4;       // owned cells per thread along i in the flux pass
+    RangeOwner R_tile;   // tiled flux pass: ceil(m0 / tile_c) x ceil(m1 / 16) x ceil(m2 / 16) tiles per block
+    int tile_c = 0;      // planes per tile along i; 0: the column kernel runs
     GroupProg prog[G_NGROUP];
     std::map<unsigned, MergedProg> mprog;  // keyed by the bit set of groups
     Classes cls;          // shared-DOF classes (owner / aliases) of every location
@@ -824,13 +826,14 @@ public:
         make_range(*F, 0, 1, R_node);
         make_range(*F, -1, 2, R_eface);
         if (const char *e = std::getenv("MERCURY_RHS_MARCH")) march = std::max(1, std::atoi(e));
+        if (const char *e = std::getenv("MERCURY_RHS_TILE")) tile_c = std::max(0, std::atoi(e));
         if (const char *e = std::getenv("MERCURY_OVERLAP")) overlap = std::atoi(e) != 0;
         HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&ev_fork2, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&ev_join2, hipEventDisableTiming));
-        set_march(march);
+        set_flux_shape(tile_c > 0, tile_c > 0 ? tile_c : march);
         cls.build(*F);
         build_programs();
         alias.build(cls);
@@ -870,36 +873,59 @@ public:
     std::vector<float *> fptr_h;
     bool mix() const { return form == 2; }
     // Launch-shape autotuner: times the flux pass (HIP events on the solver
-    // stream) for march lengths 1, 2, 4, 8 in the current form and keeps the
-    // fastest; the choice never changes a result.  Returns the chosen length.
+    // stream) as the column kernel with march lengths 1, 2, 4, 8 and as the
+    // tiled kernel with 4, 8, 16 planes per tile, in the current form, and
+    // keeps the fastest; the choice never changes a result.  Returns the
+    // chosen length (cells per thread, or planes per tile when tiled).
     int autotune() {
-        int best = march;
+        int best_len = march;
+        bool best_tiled = tile_c > 0;
         double tbest = 1e300;
-        for (int L : {1, 2, 4, 8}) {
-            set_march(L);
-            const double t = time_op("rhs_cell", 5);
-            if (t < tbest) tbest = t, best = L;
-        }
-        set_march(best);
-        if (gexec) {  // a captured step keeps the old launch shape
+        for (int tiled = 0; tiled < 2; ++tiled)
+            for (int L : tiled ? std::vector<int>{4, 8, 16} : std::vector<int>{1, 2, 4, 8}) {
+                set_flux_shape(tiled != 0, L);
+                const double t = time_op("rhs_cell", 5);
+                if (t < tbest) tbest = t, best_len = L, best_tiled = tiled != 0;
+            }
+        set_flux_shape(best_tiled, best_len);
+        return best_len;
+    }
+
+    // the flux pass's launch shape: the column kernel with L cells per thread
+    // along i, or the tiled kernel with L planes per 16 x 16 tile; drops a
+    // captured step graph (it holds the old shape)
+    void set_flux_shape(bool tiled, int L) {
+        if (L < 1) throw std::runtime_error("mercury_set_flux_shape: length must be >= 1");
+        ++generation;
+        if (gexec) {
             (void)hipGraphExecDestroy(gexec);
             (void)hipGraphDestroy(graph);
             gexec = nullptr;
             graph = nullptr;
         }
-        return best;
-    }
-
-    void set_march(int L) {
-        march = L;
-        ++generation;
         std::vector<int> lo(3 * F->nb(), 0), n(3 * F->nb());
-        for (int b = 0; b < F->nb(); ++b) {
-            n[3 * b] = (F->blk[b].m[0] + L - 1) / L;
-            n[3 * b + 1] = F->blk[b].m[1];
-            n[3 * b + 2] = F->blk[b].m[2];
+        if (tiled) {
+            tile_c = L;
+            for (int b = 0; b < F->nb(); ++b) {
+                n[3 * b] = (F->blk[b].m[0] + L - 1) / L;
+                n[3 * b + 1] = (F->blk[b].m[1] + 15) / 16;
+                n[3 * b + 2] = (F->blk[b].m[2] + 15) / 16;
+            }
+            make_range_boxes(*F, lo, n, R_tile);
+        } else {
+            tile_c = 0;
+            march = L;
+            for (int b = 0; b < F->nb(); ++b) {
+                n[3 * b] = (F->blk[b].m[0] + L - 1) / L;
+                n[3 * b + 1] = F->blk[b].m[1];
+                n[3 * b + 2] = F->blk[b].m[2];
+            }
+            make_range_boxes(*F, lo, n, R_march);
         }
-        make_range_boxes(*F, lo, n, R_march);
+    }
+    void launch_flux(hipStream_t s) {
+        if (tile_c > 0) launch_rhs_tile(s, R_tile.h, Bd(), Fd(), Gd(), P, tile_c, mix());
+        else launch_rhs_cell(s, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
     }
 
     // refresh the FP32 static copies from the FP64 arrays
@@ -1369,10 +1395,10 @@ public:
         if (overlap) {
             HIPCHK(hipEventRecord(ev_fork, st));
             HIPCHK(hipStreamWaitEvent(st2, ev_fork, 0));
-            launch_rhs_cell(st2, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
+            launch_flux(st2);
             HIPCHK(hipEventRecord(ev_join, st2));
         } else {
-            launch_rhs_cell(st, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
+            launch_flux(st);
         }
         launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix(), P.ambipolar_enabled != 0.0, nspecies());
         if (alias.n == 0) {
@@ -1649,7 +1675,7 @@ public:
             launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
         else if (op == "timestep") launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt, nruns);
         else if (op == "stage") stage();
-        else if (op == "rhs_cell") launch_rhs_cell(st, R_march.h, Bd(), Fd(), Gd(), P, march, mix());
+        else if (op == "rhs_cell") launch_flux(st);
         else if (op == "cell_end") launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix(), hall64(), nruns);
         else if (op == "eface_u") launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix(), P.ambipolar_enabled != 0.0, nspecies());
         else if (op == "edge_total") launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, alias.n == 0 ? 1 : 0, mix());
@@ -2549,6 +2575,23 @@ int mercury_autotune(MercuryContext *ctx, int *march_out) {
     ABI_TRY({
         const int L = ctx->s->autotune();
         if (march_out) *march_out = L;
+    })
+}
+
+int mercury_set_flux_shape(MercuryContext *ctx, int tiled, int length) {
+    CHECK_CTX(ctx);
+    if (length < 1) {
+        merc::g_last_error = "mercury_set_flux_shape: length must be >= 1";
+        return -1;
+    }
+    ABI_TRY({ ctx->s->set_flux_shape(tiled != 0, length); })
+}
+
+int mercury_flux_shape(MercuryContext *ctx, int *tiled, int *length) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        if (tiled) *tiled = ctx->s->tile_c > 0 ? 1 : 0;
+        if (length) *length = ctx->s->tile_c > 0 ? ctx->s->tile_c : ctx->s->march;
     })
 }
 

@@ -24,6 +24,7 @@ ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercu
                "mercury_divb", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
                "mercury_time_op", "mercury_geometry_block", "mercury_set_form", "mercury_resist_stats",
                "mercury_ensemble_create", "mercury_ensemble_launch", "mercury_ensemble_destroy", "mercury_autotune",
+               "mercury_set_flux_shape", "mercury_flux_shape",
                "mercury_write_restart", "mercury_read_restart", "mercury_write_rundata", "mercury_read_rundata",
                "mercury_write_tecplot", "mercury_case_read", "mercury_case_destroy", "mercury_case_blocks",
                "mercury_case_block_name", "mercury_case_nodes", "mercury_case_topology", "mercury_case_interfaces",
@@ -93,6 +94,8 @@ def load():
     L.mercury_ensemble_launch.argtypes = [vp, ctypes.c_int]
     L.mercury_ensemble_destroy.argtypes = [vp]
     L.mercury_autotune.argtypes = [vp, ip]
+    L.mercury_set_flux_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.mercury_flux_shape.argtypes = [vp, ip, ip]
     L.mercury_geometry_block.argtypes = [ip, ctypes.c_int, dp, ctypes.POINTER(MercuryPhysics), cp, ctypes.c_int, dp]
     # DEC operators on caller device arrays: pointers travel as void *
     pp = ctypes.POINTER(vp)
@@ -300,10 +303,22 @@ class Mercury:
         return n.value, it.value, rel.value
 
     def autotune(self):
-        """Pick the fastest launch shape of the flux pass; returns the march length."""
+        """Pick the fastest launch shape of the flux pass; returns its length
+        (cells per thread, or planes per tile: see flux_shape)."""
         v = ctypes.c_int()
         _chk(load().mercury_autotune(self._h, ctypes.byref(v)))
         return v.value
+
+    def set_flux_shape(self, tiled, length):
+        """Run the flux pass as the column kernel (tiled False, length cells per
+        thread along i) or the LDS-tiled kernel (length planes per tile)."""
+        _chk(load().mercury_set_flux_shape(self._h, 1 if tiled else 0, int(length)))
+
+    def flux_shape(self):
+        """(tiled, length) of the flux pass's launch shape."""
+        t, n = ctypes.c_int(), ctypes.c_int()
+        _chk(load().mercury_flux_shape(self._h, ctypes.byref(t), ctypes.byref(n)))
+        return bool(t.value), n.value
 
     def time_op(self, op, reps=10):
         v = ctypes.c_double()

@@ -108,6 +108,77 @@ def test_fused_rhs_bit_exact(oracle_mod, shape):
         g.close()
 
 
+TILE_SHAPES = {
+    "cube": small_cube,  # every tile partial (m < 16)
+    "cube_tiles": lambda: cases.make_periodic_cube(n=(6, 32, 17), g=4),  # full j tiles; a one-column k tile
+    "cube_wide": lambda: cases.make_periodic_cube(n=(12, 24, 30), g=4),
+    "shell": lambda: cases.make_shell(scale=0.25),
+    "cube_he": lambda: cases.make_periodic_cube(n=(10, 20, 18), g=4, he=True),
+}
+
+
+@pytest.mark.parametrize("length", [1, 3, 16])
+@pytest.mark.parametrize("shape", list(TILE_SHAPES))
+def test_tiled_rhs_bit_exact(oracle_mod, shape, length):
+    """The LDS-tiled flux + sources pass (16 x 16 column tiles marching `length`
+    planes along i) equals the oracle's Scheme_U_ followed by
+    AddSourceToRHS_Fluid in FP64, and the column kernel's result in the mixed
+    form, bit for bit: on partial tiles, full tiles whose upper faces lie on the
+    block edge, a one-column tile, chunks that end inside and at the block end,
+    and three species."""
+    case = TILE_SHAPES[shape]()
+    names = ["RHS_H", "RHS_Na"] + (["RHS_He"] if case.phys.n_species >= 3 else [])
+    print(f"[case] {case.name} dims={[b.dims for b in case.blocks]} length={length}", flush=True)
+    g = Mercury(case)
+    o = oracle_mod.Oracle(case)
+    try:
+        g.prepare()
+        o.prepare()
+        g.set_form(1)
+        g.set_flux_shape(True, length)
+        assert g.flux_shape() == (True, length)
+        g.op("rhs_cell")
+        o.op("flux")
+        o.op("sources")
+        _cmp(g, o, names, "tiled rhs")
+        g.set_form(2)
+        g.prepare()
+        g.set_flux_shape(False, 4)
+        g.op("rhs_cell")
+        col = {n: [g.get(n, ib) for ib in range(case.nb)] for n in names}
+        g.set_flux_shape(True, length)
+        g.op("rhs_cell")
+        for n in names:
+            for ib in range(case.nb):
+                np.testing.assert_array_equal(g.get(n, ib), col[n][ib], err_msg=f"mixed tiled {n}[{ib}]")
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_tiled_steps_and_graph_bit_exact(pair, integrator):
+    """Steps with the tiled flux pass, then a captured step replayed from a HIP
+    graph, equal the oracle."""
+    case, g, o = pair
+    g.prepare()
+    o.prepare()
+    g.set_form(1)
+    g.set_flux_shape(True, 5)
+    np.testing.assert_array_equal(g.step(5, integrator), o.step(5, integrator))
+    _cmp(g, o, STATE + DERIVED, f"5 tiled steps integrator={integrator}")
+    g.graph_capture(integrator)
+    g.graph_launch(3)
+    g.synchronize()
+    o.step(3, integrator)
+    _cmp(g, o, STATE + DERIVED, f"tiled graph replay integrator={integrator}")
+
+
+def test_flux_shape_refuses_zero_length(pair):
+    case, g, o = pair
+    with pytest.raises(MercuryError, match="length must be >= 1"):
+        g.set_flux_shape(True, 0)
+
+
 def test_induction_bit_exact(pair):
     case, g, o = pair
     g.prepare()
@@ -163,9 +234,10 @@ def test_autotune_keeps_results(pair):
     o.prepare()
     g.set_form(1)
     L = g.autotune()
-    assert L in (1, 2, 4, 8)
+    tiled, length = g.flux_shape()
+    assert length == L and L in ((4, 8, 16) if tiled else (1, 2, 4, 8))
     np.testing.assert_array_equal(g.step(3, 0), o.step(3, 0))
-    _cmp(g, o, STATE + DERIVED, f"3 steps after autotune (march {L})")
+    _cmp(g, o, STATE + DERIVED, f"3 steps after autotune (tiled {tiled}, length {L})")
 
 
 @pytest.mark.parametrize("integrator", [0, 1])
