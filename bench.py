@@ -82,16 +82,20 @@ ALGO_BYTES = {
 STAGE_BYTES = 8 * (2 * (5 * 2 + 3) + 10 + 60 + 2 + 3 + 18)
 
 
-def measured_traffic(kernel: str):
+def measured_traffic(kernel: str, shape=None):
     """HBM bytes per launch of a fused pass from the committed rocprofv3 counter
-    summary (2*FETCH_SIZE + WRITE_SIZE, see profiles/shell6_hbm_traffic.md), or None."""
+    summary (2*FETCH_SIZE + WRITE_SIZE, see profiles/shell6_hbm_traffic.md), or None.
+    The flux pass is keyed by its launch shape (tiled, length)."""
     path = os.path.join(ROOT, "profiles", "shell6_hbm_traffic.md")
     try:
         text = open(path).read()
         blob = text.split("```json", 1)[1].split("```", 1)[0]
-        return json.loads(blob).get(kernel)
+        table = json.loads(blob)
     except (OSError, IndexError, ValueError):
         return None
+    if kernel == "rhs_cell" and shape is not None:
+        return table.get(f"rhs_cell {'tiled' if shape[0] else 'column'} {shape[1]}")
+    return table.get(kernel)
 
 
 def make_case(name: str, resist=None):
@@ -197,6 +201,9 @@ def main():
         g.prepare()
         g.set_form(form)
         info = {"march": g.autotune() if form >= 1 else None}
+        if form >= 1:
+            tiled, length = g.flux_shape()
+            info["flux_shape"] = {"kernel": "k_rhs_tile" if tiled else "k_rhs_cell", "tiled": tiled, "length": length}
         if oref is not None:
             g.step(args.cpu_sample_steps, case.integrator)
             err, emax = {}, {}
@@ -490,7 +497,9 @@ def main():
                    "cells": ncells, "blocks": case.nb, "form": "fp64_fused", "integrator": "euler"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(dom) if args.workload == "shell6" else None, "kernel": dom,
+                     "traffic": (measured_traffic(dom, (forms["fp64_fused"]["flux_shape"]["tiled"],
+                                                        forms["fp64_fused"]["flux_shape"]["length"]))
+                                 if args.workload == "shell6" else None), "kernel": dom,
                      "algo_bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": units,
                      "avg_launch_ms": times[dom]},
         "step_roofline": {"stage_bytes_per_cell": STAGE_BYTES, "ms_per_step": step_graph_ms, "achieved_GBs": step_gbs,
