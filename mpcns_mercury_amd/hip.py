@@ -13,7 +13,9 @@ import numpy as np
 from .params import MercuryPhysics
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmercury_hip.so")
+# MERCURY_HIP_LIB selects another build of the same ABI, e.g. the host-checked
+# libmercury_hip_checked.so (mpcns_mercury_amd/build.py)
+LIB_PATH = os.environ.get("MERCURY_HIP_LIB") or os.path.join(_HERE, "libmercury_hip.so")
 _lib = None
 
 # every symbol declared in include/mercury_hip.h
