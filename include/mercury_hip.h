@@ -151,6 +151,15 @@ int mercury_field_device(MercuryContext *ctx, const char *name, int block, int c
 /* The context's HIP stream (a hipStream_t): work a caller orders against the
  * step goes on it. */
 int mercury_stream(MercuryContext *ctx, void **stream);
+/* Run the context's work on a caller's HIP stream (a hipStream_t; NULL returns
+ * to the context's own stream).  No reference counterpart: the reference's
+ * solver is synchronous host code; this is the stream of the drop-in's context
+ * (steps, graph launches, syncs, uploads/downloads, DEC diagnostics).  Work
+ * already queued on the previous stream completes before the call returns.
+ * The caller keeps the stream alive while the context uses it and gives it the
+ * default priority; the fork of the fused step still uses an internal side
+ * stream, joined back by events. */
+int mercury_set_stream(MercuryContext *ctx, void *stream);
 
 /* Replace direct FieldBlock(i,j,k,m) access on host storage
  * (include/3_field/FieldBlock.h:38-49) and the restart injection of

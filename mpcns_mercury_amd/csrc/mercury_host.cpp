@@ -642,7 +642,8 @@ public:
     std::map<unsigned, MergedProg> mprog;  // keyed by the bit set of groups
     Classes cls;          // shared-DOF classes (owner / aliases) of every location
     AliasClasses alias;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;      // the stream all of the context's work goes to
+    hipStream_t own_st = nullptr;  // the context's own stream (st unless a caller's stream is set)
     // side stream of the fused stage: the fluid RHS runs there while the
     // induction EMF chain runs on st (they read the same state and write
     // disjoint fields); both streams keep the default priority
@@ -757,6 +758,7 @@ public:
         Prun.assign(prun, prun + nruns);
         check_run_physics(nruns, prun);
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        own_st = st;
         F.reset(new Field(nb, dims, g, bc, nbr, trans));
         for (int b = 0; b < nb; ++b) {
             F->blk[b].run = brun[b];
@@ -855,7 +857,16 @@ public:
         if (ev_fork2) (void)hipEventDestroy(ev_fork2);
         if (ev_join2) (void)hipEventDestroy(ev_join2);
         if (st2) (void)hipStreamDestroy(st2);
-        if (st) (void)hipStreamDestroy(st);
+        if (own_st) (void)hipStreamDestroy(own_st);
+    }
+
+    // Moves the context's work to a caller's stream (nullptr: back to its own).
+    // Work already queued on the current stream and on the side stream
+    // completes first, so the order of the caller's calls is kept.
+    void set_stream(hipStream_t s) {
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipStreamSynchronize(st2));
+        st = s ? s : own_st;
     }
 
     double *const *Fd() const { return F->dptr; }
@@ -1889,6 +1900,11 @@ int mercury_stream(MercuryContext *ctx, void **stream) {
     }
     *stream = (void *)ctx->s->st;
     return 0;
+}
+
+int mercury_set_stream(MercuryContext *ctx, void *stream) {
+    CHECK_CTX(ctx);
+    ABI_TRY({ ctx->s->set_stream((hipStream_t)stream); })
 }
 
 int mercury_upload(MercuryContext *ctx, const char *name, int block, int comp, const double *host) {

@@ -21,7 +21,7 @@ _lib = None
 # every symbol declared in include/mercury_hip.h
 ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercury_create_connected", "mercury_create_ensemble", "mercury_run_dt", "mercury_destroy",
                "mercury_field_info",
-               "mercury_block_volume", "mercury_block_shape", "mercury_field_device", "mercury_stream", "mercury_total_volume", "mercury_upload", "mercury_download",
+               "mercury_block_volume", "mercury_block_shape", "mercury_field_device", "mercury_stream", "mercury_set_stream", "mercury_total_volume", "mercury_upload", "mercury_download",
                "mercury_init_bface", "mercury_init_bface_dipoles", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
                "mercury_divb", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
                "mercury_time_op", "mercury_geometry_block", "mercury_set_form", "mercury_resist_stats",
@@ -78,6 +78,7 @@ def load():
     L.mercury_block_shape.argtypes = [vp, ctypes.c_int, ip, ip]
     L.mercury_field_device.argtypes = [vp, cp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
     L.mercury_stream.argtypes = [vp, ctypes.POINTER(vp)]
+    L.mercury_set_stream.argtypes = [vp, vp]
     L.mercury_download.argtypes = [vp, cp, ctypes.c_int, ctypes.c_int, dp]
     L.mercury_init_bface.argtypes = [vp, dp, ctypes.POINTER(dp)]
     L.mercury_init_bface_dipoles.argtypes = [vp, dp, ctypes.POINTER(dp), ctypes.c_int, dp]
@@ -309,6 +310,22 @@ class Mercury:
         (cells per thread, or planes per tile: see flux_shape)."""
         v = ctypes.c_int()
         _chk(load().mercury_autotune(self._h, ctypes.byref(v)))
+        return v.value
+
+    def stream(self):
+        """The context's HIP stream handle (an integer hipStream_t)."""
+        v = ctypes.c_void_p()
+        _chk(load().mercury_stream(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_stream(self, handle):
+        """Run the context's work on a caller's HIP stream (None: its own)."""
+        _chk(load().mercury_set_stream(self._h, ctypes.c_void_p(handle) if handle else None))
+
+    def field_device(self, name, block=0, comp=0):
+        """Device address of one component of a field on one block."""
+        v = ctypes.c_void_p()
+        _chk(load().mercury_field_device(self._h, name.encode(), block, comp, ctypes.byref(v)))
         return v.value
 
     def set_flux_shape(self, tiled, length):

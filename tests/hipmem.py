@@ -18,6 +18,10 @@ def hip():
         L.hipFree.argtypes = [ctypes.c_void_p]
         L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         L.hipDeviceSynchronize.argtypes = []
+        L.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+        L.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+        L.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+        L.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
         _hip = L
     return _hip
 
@@ -71,3 +75,24 @@ def ptrs(arrs):
 
 def synchronize():
     _chk(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class Stream:
+    """A non-blocking HIP stream of default priority, created and destroyed here."""
+
+    def __init__(self):
+        p = ctypes.c_void_p()
+        _chk(hip().hipStreamCreateWithFlags(ctypes.byref(p), 1), "hipStreamCreateWithFlags")
+        self.handle = p.value
+
+    def synchronize(self):
+        _chk(hip().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
+
+    def copy_to_host(self, ptr, out):
+        """Queue a device-to-host copy of out.nbytes bytes from ptr on this stream."""
+        _chk(hip().hipMemcpyAsync(out.ctypes.data, ptr, out.nbytes, D2H, self.handle), "hipMemcpyAsync D2H")
+
+    def destroy(self):
+        if self.handle:
+            hip().hipStreamDestroy(self.handle)
+            self.handle = None
