@@ -673,6 +673,7 @@ public:
     ResDofs rd;
     DBuf<int> r_q, r_ax, r_sI, r_sJ;
     DBuf<double> r_eta, r_vec, r_part, r_S;
+    double *rwd = nullptr;  // w / dg, the weight of r.z
     double *rx = nullptr, *rr = nullptr, *rp = nullptr, *rAp = nullptr, *rb = nullptr, *rdg = nullptr, *rw = nullptr,
            *rt = nullptr;
     int res_its = 0;
@@ -1281,10 +1282,11 @@ public:
         rd.sI = r_sI.p;
         rd.sJ = r_sJ.p;
         rd.eta = r_eta.p;
-        r_vec.upload(std::vector<double>(8 * (size_t)std::max(n, 1), 0.0));
+        r_vec.upload(std::vector<double>(9 * (size_t)std::max(n, 1), 0.0));
         double *v = r_vec.p;
         const size_t m = (size_t)std::max(n, 1);
         rx = v; rr = v + m; rp = v + 2 * m; rAp = v + 3 * m; rb = v + 4 * m; rdg = v + 5 * m; rw = v + 6 * m; rt = v + 7 * m;
+        rwd = v + 8 * m;
         r_part.upload(std::vector<double>(3 * (size_t)std::max(res_blocks(n), 1), 0.0));
         r_S.upload(std::vector<double>(CG_NSCALAR, 0.0));
     }
@@ -1294,9 +1296,9 @@ public:
     // alpha * CurlAdjFaceToEdge(beta * dB), alias mean, y = Eres - eta * dJ.  The
     // reference's Sync(dJ) touches no unknown and is left out.  lin: the far-field
     // Eres handler with a zero field (the linear part).  With w, the partial sums of
-    // x.y.w go to r_part.
-    void res_apply(const double *x, double *y, double dt, bool lin, const double *w) {
-        launch_res_scatter(st, rd, x, Fd());
+    // x.y.w go to r_part.  scattered: x is already in the Eres slots of the unknowns.
+    void res_apply(const double *x, double *y, double dt, bool lin, const double *w, bool scattered = false) {
+        if (!scattered) launch_res_scatter(st, rd, x, Fd());
         run_merged(1u << (lin ? G_ERES0 : G_ERES));
         launch_dec_d1(st, R_res.h, Bd(), Fd(), SL_ERES, SL_DB, -dt);
         run_merged(1u << G_DB);
@@ -1320,9 +1322,10 @@ public:
         // unknown; the derived pass recomputes J everywhere afterwards)
         launch_jedge(st, R_res.h, Bd(), Fd());
         launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
-        launch_res_rhs(st, rd, Fd(), dt, rb, rdg, rw);
+        launch_res_rhs(st, rd, Fd(), dt, rb, rdg, rw, rwd);
         res_apply(rx, rAp, dt, false, nullptr);
-        launch_cg_init(st, n, rb, rAp, rdg, rw, rr, rp, r_part.p, r_S.p);
+        launch_cg_init(st, n, rb, rAp, rdg, rwd, rr, rp, r_part.p, r_S.p);
+        launch_res_scatter(st, rd, rp, Fd());
         int its = 0;
         double S[CG_NSCALAR];
         for (;;) {
@@ -1336,10 +1339,12 @@ public:
                 if (its >= (int)P.resist_max_it)
                     throw std::runtime_error("implicit resistive CG did not converge within " + std::to_string(its) + " iterations");
             }
-            res_apply(rp, rAp, dt, true, rw);
+            // p is in the Eres slots already (scattered by the init or by the
+            // previous iteration's k_cg_pxs)
+            res_apply(rp, rAp, dt, true, rw, true);
             launch_cg_pap(st, n, r_part.p, r_S.p);
-            launch_cg_xr(st, n, r_S.p, rp, rAp, rdg, rw, rx, rr, r_part.p, r_S.p);
-            launch_cg_p(st, n, r_S.p, rr, rdg, rp);
+            launch_cg_r(st, n, r_S.p, rAp, rwd, rr, r_part.p, r_S.p);
+            launch_cg_pxs(st, rd, r_S.p, rr, rdg, rp, rx, Fd());
             ++its;
         }
         res_its = its;

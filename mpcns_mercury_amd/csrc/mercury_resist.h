@@ -19,13 +19,15 @@ int res_blocks(int n);  // blocks of the reducing kernels (length of one partial
 void launch_res_scatter(hipStream_t st, const ResDofs &d, const double *x, double *const *F, int slot = SL_ERES);
 void launch_res_pick(hipStream_t st, const ResDofs &d, double *const *F, int slot, double *x);
 void launch_res_gather(hipStream_t st, const ResDofs &d, double *const *F, double *y, const double *p, const double *w, double *part);
-void launch_res_rhs(hipStream_t st, const ResDofs &d, double *const *F, double dt, double *b, double *dg, double *w);
-void launch_cg_init(hipStream_t st, int n, const double *b, const double *Ax, const double *dg, const double *w, double *r, double *p,
+void launch_res_rhs(hipStream_t st, const ResDofs &d, double *const *F, double dt, double *b, double *dg, double *w, double *wd);
+void launch_cg_init(hipStream_t st, int n, const double *b, const double *Ax, const double *dg, const double *wd, double *r, double *p,
                     double *part, double *S);
 void launch_cg_pap(hipStream_t st, int n, const double *part, double *S);
-void launch_cg_xr(hipStream_t st, int n, const double *S, const double *p, const double *Ap, const double *dg, const double *w, double *x,
-                  double *r, double *part, double *Sw);
-void launch_cg_p(hipStream_t st, int n, const double *S, const double *r, const double *dg, double *p);
+// r -= alpha Ap with the partials of r.r and r.z.w, then beta
+void launch_cg_r(hipStream_t st, int n, const double *S, const double *Ap, const double *wd, double *r, double *part, double *Sw);
+// x += alpha p; p = z + beta p; p into the edge slots of the unknowns
+void launch_cg_pxs(hipStream_t st, const ResDofs &d, const double *S, const double *r, const double *dg, double *p, double *x,
+                   double *const *F, int slot = SL_ERES);
 void launch_bres_update(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, double dt);
 
 }  // namespace merc

@@ -67,11 +67,12 @@ __global__ void k_gather(int n, const int *__restrict__ q, const int *__restrict
 }
 
 // right-hand side b = eta * J(B*) (PackImplicitJedgeToVec_ :387-404), the
-// Jacobi diagonal 1 + dt*eta*alpha*sum(beta of the 4 faces around the edge)
-// and the inner-product weight 1/(eta*alpha)
+// Jacobi diagonal 1 + dt*eta*alpha*sum(beta of the 4 faces around the edge),
+// the inner-product weight 1/(eta*alpha) and its ratio to the diagonal (the
+// weight of r.z)
 __global__ void k_rhs(int n, const int *__restrict__ q, const int *__restrict__ ax, const int *__restrict__ sI,
                       const int *__restrict__ sJ, const double *__restrict__ eta, double *const *__restrict__ F_, double dt,
-                      double *__restrict__ b, double *__restrict__ dg, double *__restrict__ w) {
+                      double *__restrict__ b, double *__restrict__ dg, double *__restrict__ w, double *__restrict__ wd) {
     gdouble *const *__restrict__ F = gslots(F_);
     const int t = blockIdx.x * RB + threadIdx.x;
     if (t >= n) return;
@@ -86,20 +87,21 @@ __global__ void k_rhs(int n, const int *__restrict__ q, const int *__restrict__ 
     b[t] = 0.0 + eta[t] * F[SL_J + a][e];
     dg[t] = 1.0 + dt * ea * s;
     w[t] = 1.0 / ea;
+    wd[t] = w[t] / dg[t];
 }
 
 // r = b - Ax, p = z = r / dg (z is not stored: it is recomputed from r and dg);
-// partials of b.b, r.r, r.z.w
+// partials of b.b, r.r and r.z.w = r.r.(w / dg)
 __global__ void k_cg_init(int n, const double *__restrict__ b, const double *__restrict__ Ax, const double *__restrict__ dg,
-                          const double *__restrict__ w, double *__restrict__ r, double *__restrict__ p, double *__restrict__ part) {
+                          const double *__restrict__ wd, double *__restrict__ r, double *__restrict__ p, double *__restrict__ part) {
     double bb = 0.0, rr = 0.0, rz = 0.0;
     for (int t = blockIdx.x * RB + threadIdx.x; t < n; t += gridDim.x * RB) {
-        const double rv = b[t] - Ax[t], zv = rv / dg[t];
+        const double rv = b[t] - Ax[t];
         r[t] = rv;
-        p[t] = zv;
+        p[t] = rv / dg[t];
         bb += b[t] * b[t];
         rr += rv * rv;
-        rz += rv * zv * w[t];
+        rz += rv * rv * wd[t];
     }
     block_sum(bb, part);
     __syncthreads();
@@ -108,29 +110,37 @@ __global__ void k_cg_init(int n, const double *__restrict__ b, const double *__r
     block_sum(rz, part + 2 * gridDim.x);
 }
 
-// x += alpha p, r -= alpha Ap; partials of r.r and r.z.w with z = r / dg
-__global__ void k_cg_xr(int n, const double *__restrict__ S, const double *__restrict__ p, const double *__restrict__ Ap,
-                        const double *__restrict__ dg, const double *__restrict__ w, double *__restrict__ x,
-                        double *__restrict__ r, double *__restrict__ part) {
+// r -= alpha Ap; partials of r.r and r.z.w = r.r.(w / dg).  The x update
+// waits for k_cg_pxs, which reads p anyway.
+__global__ void k_cg_r(int n, const double *__restrict__ S, const double *__restrict__ Ap, const double *__restrict__ wd,
+                       double *__restrict__ r, double *__restrict__ part) {
     const double al = S[CG_ALPHA];
     double rr = 0.0, rz = 0.0;
     for (int t = blockIdx.x * RB + threadIdx.x; t < n; t += gridDim.x * RB) {
-        x[t] += al * p[t];
-        const double rv = r[t] - al * Ap[t], zv = rv / dg[t];
+        const double rv = r[t] - al * Ap[t];
         r[t] = rv;
         rr += rv * rv;
-        rz += rv * zv * w[t];
+        rz += rv * rv * wd[t];
     }
     block_sum(rr, part);
     __syncthreads();
     block_sum(rz, part + gridDim.x);
 }
 
-// p = z + beta p, z = r / dg
-__global__ void k_cg_p(int n, const double *__restrict__ S, const double *__restrict__ r, const double *__restrict__ dg,
-                       double *__restrict__ p) {
+// x += alpha p, then p = z + beta p (z = r / dg), and the new p into the edge
+// slots s.. of the unknowns for the next operator application: one pass over
+// p for the end of one iteration and the start of the next
+__global__ void k_cg_pxs(int n, const double *__restrict__ S, const double *__restrict__ r, const double *__restrict__ dg,
+                         double *__restrict__ p, double *__restrict__ x, const int *__restrict__ q, const int *__restrict__ ax,
+                         double *const *__restrict__ F_, int s) {
+    gdouble *const *__restrict__ F = gslots(F_);
     const int t = blockIdx.x * RB + threadIdx.x;
-    if (t < n) p[t] = r[t] / dg[t] + S[CG_BETA] * p[t];
+    if (t >= n) return;
+    const double po = p[t];
+    x[t] += S[CG_ALPHA] * po;
+    const double pn = r[t] / dg[t] + S[CG_BETA] * po;
+    p[t] = pn;
+    F[s + ax[t]][q[t]] = pn;
 }
 
 // sum of nq partial arrays of length nb (in block order) and the scalar update
@@ -195,26 +205,26 @@ void launch_res_pick(hipStream_t st, const ResDofs &d, double *const *F, int slo
 void launch_res_gather(hipStream_t st, const ResDofs &d, double *const *F, double *y, const double *p, const double *w, double *part) {
     if (d.n > 0) hipLaunchKernelGGL(k_gather, dim3(rgrid(d.n)), dim3(RB), 0, st, d.n, d.q, d.ax, d.eta, F, y, p, w, part);
 }
-void launch_res_rhs(hipStream_t st, const ResDofs &d, double *const *F, double dt, double *b, double *dg, double *w) {
-    if (d.n > 0) hipLaunchKernelGGL(k_rhs, dim3(rblk(d.n)), dim3(RB), 0, st, d.n, d.q, d.ax, d.sI, d.sJ, d.eta, F, dt, b, dg, w);
+void launch_res_rhs(hipStream_t st, const ResDofs &d, double *const *F, double dt, double *b, double *dg, double *w, double *wd) {
+    if (d.n > 0) hipLaunchKernelGGL(k_rhs, dim3(rblk(d.n)), dim3(RB), 0, st, d.n, d.q, d.ax, d.sI, d.sJ, d.eta, F, dt, b, dg, w, wd);
 }
-void launch_cg_init(hipStream_t st, int n, const double *b, const double *Ax, const double *dg, const double *w, double *r, double *p,
+void launch_cg_init(hipStream_t st, int n, const double *b, const double *Ax, const double *dg, const double *wd, double *r, double *p,
                     double *part, double *S) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cg_init, dim3(rgrid(n)), dim3(RB), 0, st, n, b, Ax, dg, w, r, p, part);
+    hipLaunchKernelGGL(k_cg_init, dim3(rgrid(n)), dim3(RB), 0, st, n, b, Ax, dg, wd, r, p, part);
     hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(RB), 0, st, (int)rgrid(n), 3, part, S, 0);
 }
 void launch_cg_pap(hipStream_t st, int n, const double *part, double *S) {
     if (n > 0) hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(RB), 0, st, (int)rgrid(n), 1, part, S, 1);
 }
-void launch_cg_xr(hipStream_t st, int n, const double *S, const double *p, const double *Ap, const double *dg, const double *w, double *x,
-                  double *r, double *part, double *Sw) {
+void launch_cg_r(hipStream_t st, int n, const double *S, const double *Ap, const double *wd, double *r, double *part, double *Sw) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cg_xr, dim3(rgrid(n)), dim3(RB), 0, st, n, S, p, Ap, dg, w, x, r, part);
+    hipLaunchKernelGGL(k_cg_r, dim3(rgrid(n)), dim3(RB), 0, st, n, S, Ap, wd, r, part);
     hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(RB), 0, st, (int)rgrid(n), 2, part, Sw, 2);
 }
-void launch_cg_p(hipStream_t st, int n, const double *S, const double *r, const double *dg, double *p) {
-    if (n > 0) hipLaunchKernelGGL(k_cg_p, dim3(rblk(n)), dim3(RB), 0, st, n, S, r, dg, p);
+void launch_cg_pxs(hipStream_t st, const ResDofs &d, const double *S, const double *r, const double *dg, double *p, double *x,
+                   double *const *F, int slot) {
+    if (d.n > 0) hipLaunchKernelGGL(k_cg_pxs, dim3(rblk(d.n)), dim3(RB), 0, st, d.n, S, r, dg, p, x, d.q, d.ax, F, slot);
 }
 void launch_bres_update(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, double dt) {
     if (R.total > 0) hipLaunchKernelGGL(k_bres_update, dim3((unsigned)((R.total + 255) / 256)), dim3(256), 0, st, R.dev, B, F, dt);
