@@ -152,6 +152,8 @@ def main():
                     help="steps of the mixed form against the FP64 form with div B after every step (0: skip)")
     ap.add_argument("--c5-steps", type=int, default=500,
                     help="steps of the three-species resistive shell in the mixed form (0: skip)")
+    ap.add_argument("--cut-shell", type=int, default=1,
+                    help="also time the six sectors cut into 24 interface-glued blocks (0: skip)")
     ap.add_argument("--forms", default="fp64_plain,mixed,fp64_fused",
                     help="comma list of forms to run (fp64_fused is always run: it gives the value)")
     args = ap.parse_args()
@@ -244,6 +246,34 @@ def main():
                     t = op_times[label][op]
                     print(f"[bench] {label} op {op:12s} {t * 1e3:9.1f} us  {ALGO_BYTES[op] * ncells / (t * 1e-3) / 1e9:8.1f} GB/s",
                           file=sys.stderr)
+    # the same six sectors cut into 24 blocks glued by interfaces: the step with
+    # inter-block ghost copies in every sync (the uncut workload has none)
+    cut = None
+    if args.cut_shell and args.workload == "shell6":
+        t0 = time.perf_counter()
+        cc = cases.make_cut_shell()
+        cg = Mercury(cc)
+        cg.prepare()
+        cut = {"blocks": cc.nb, "cells": cc.ncells(), "split": list(cc.meta["split"])}
+        for form, label in ((1, "fp64_fused"), (2, "mixed")):
+            cg.set_form(form)
+            cg.autotune()
+            cg.graph_capture(0)
+            cg.graph_launch(args.warmup)
+            cg.synchronize()
+            t1 = time.perf_counter()
+            cg.graph_launch(args.steps)
+            cg.synchronize()
+            el = time.perf_counter() - t1
+            cut[label] = {"ms_per_step": el * 1e3 / args.steps, "cell_updates_per_s": cc.ncells() * args.steps / el,
+                          "uncut_ms_per_step": forms[label]["ms_per_step"] if label in forms else None,
+                          "divB_normalised": cg.divb()[0]}
+            print(f"[bench] cut shell {label}: {cc.nb} blocks, {el * 1e3 / args.steps:.3f} ms/step "
+                  f"(uncut {cut[label]['uncut_ms_per_step']}), div B {cut[label]['divB_normalised']:.2e}", file=sys.stderr, flush=True)
+        cut["sync_ms"] = {op: cg.time_op(op, 20) for op in ("sync_ucell", "sync_bface", "sync_eedge", "sync_ehall", "sync_jedge",
+                                                             "sync_bcell", "sync_jcell")}
+        cg.close()
+        print(f"[bench] cut shell setup+run {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     # visualisation output of the stepped state (Tecplot, the Mercury variable list at the
     # nodes): node means on the device, file written by the host; outside the timed steps
     output = None
@@ -513,6 +543,7 @@ def main():
         "kernel_ms": op_times,
         "resistive": resist,
         "ensemble": ensemble,
+        "cut_shell": cut,
         "resistive_shell": rshell,
         "mixed_horizon": horizon,
         "output": output,

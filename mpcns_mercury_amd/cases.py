@@ -495,6 +495,24 @@ def make_split_sector(dims=(24, 20, 16), cuts=((24,), (10, 10), (16,)), **kw) ->
     return split_block_case(base, cuts)
 
 
+def make_cut_shell(split=(1, 2, 2), scale: float = 1.0, g: int = 4, **kw) -> Case:
+    """The six C2 sectors, each cut into split[0] x split[1] x split[2] blocks glued
+    by 1:1 interfaces (default: halves across both angular axes, 24 blocks): the
+    C2 problem with inter-block ghost copies in every sync.  One run (one dt)."""
+    pieces = []
+    for ib, dims0 in enumerate(C2_BLOCKS):
+        dims = tuple(max(2 * g, int(round(d * scale))) for d in dims0)
+        base = make_shell(blocks=(dims,), panels=[ib], g=g, scale=1.0, **kw)
+        cuts = []
+        for a in range(3):
+            q, r = divmod(dims[a], split[a])
+            cuts.append([q + (1 if t < r else 0) for t in range(split[a])])
+        pieces.append(split_block_case(base, cuts))
+    c = concat_runs(pieces, name="shell6_cut_{}x{}x{}".format(*split))
+    c.meta = {"split": tuple(split), "sectors": len(pieces)}
+    return c
+
+
 C5_DIMS = (256, 192, 384)
 
 

@@ -13,7 +13,7 @@ void launch_rhs_tile(hipStream_t st, const RangeH &R, const MercBlk *B, double *
                      int C, bool mix);
 // dt_next: one word per run (nruns of them), reduced per run
 void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
-                     unsigned long long *dt_next, int compute_dt, bool mix, int hall64, int nruns);
+                     unsigned long long *dt_next, int compute_dt, bool mix, int nruns);
 // ambi: also the node electron pressure of the ambipolar EMF (read by k_edge_total)
 // ns: ion species of the run (2 or 3)
 void launch_eface_u(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix, int ambi,

@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(256) k_rhs_tile(const MercRange R, const MercB
 template <bool MIX, int NS, bool ENS>
 __global__ void __attribute__((amdgpu_waves_per_eu(NS == 2 ? 6 : 5))) k_cell_end(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
                            const float *const *__restrict__ G_, const MercPhys P, unsigned long long *__restrict__ dt_bits,
-                           int compute_dt, int hall64) {
+                           int compute_dt) {
     gdouble *const *__restrict__ F = gslots(F_);
     const gfloat *const *__restrict__ G = gslots(G_);
     using T = typename Prec<MIX>::T;
@@ -549,13 +549,6 @@ __global__ void __attribute__((amdgpu_waves_per_eu(NS == 2 ? 6 : 5))) k_cell_end
             wr<MIX>(F, G, SL_EHC + 1, q, e1);
             wr<MIX>(F, G, SL_EHC + 2, q, e2);
             wr<MIX>(F, G, SL_HB, q, hb);
-            if (MIX && hall64) {
-                // the per-function Hall edge pass of the periodic-seam path reads FP64
-                F[SL_EHC][q] = e0;
-                F[SL_EHC + 1][q] = e1;
-                F[SL_EHC + 2][q] = e2;
-                F[SL_HB][q] = hb;
-            }
         }
         // contravariant velocities (need the face of the next index on each axis)
         const int idx[3] = {p.i, p.j, p.k};
@@ -715,9 +708,11 @@ DEVI typename Prec<MIX>::T maxu4_p(gdouble *const *__restrict__ F, const gfloat 
     return a;
 }
 
-// edge EMF, all contributions, owned edges; with_hall adds the Hall EMF and
-// applies the wall condition (used when no periodic seam needs the alias mean
-// of the non-Hall part first)
+// edge EMF, all contributions, owned edges.  with_hall 0: the non-Hall EMF
+// only; 1: the Hall EMF (also stored on its own) added and the wall condition
+// applied (when no shared edge needs the alias mean of the non-Hall part
+// first); 2: the non-Hall EMF and the Hall EMF stored apart, for the alias mean
+// of the first before the two are added
 template <bool MIX>
 __global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
                              const float *const *__restrict__ G_, const MercPhys P, int with_hall) {
@@ -846,6 +841,10 @@ __global__ void k_edge_total(const MercRange R, const MercBlk *__restrict__ B, d
 #undef AR
         const T eh = st<MIX>(F, G, SL_TAPER + a, q) * val;
         F[SL_EH + a][q] = (double)eh;
+        if (with_hall == 2) {
+            F[SL_E + a][q] = (double)E;
+            continue;
+        }
         T e = E + eh;
         for (int f = 0; f < 6; ++f) {
             const int nax = f >> 1;
@@ -1050,10 +1049,10 @@ void launch_rhs_tile(hipStream_t st, const RangeH &R, const MercBlk *B, double *
     }
 }
 void launch_cell_end(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const MercPhys &P,
-                     unsigned long long *dt_next, int compute_dt, bool mix, int hall64, int nruns) {
+                     unsigned long long *dt_next, int compute_dt, bool mix, int nruns) {
     if (compute_dt) hipLaunchKernelGGL(k_dt_reset, dim3(nblk(nruns, 64)), dim3(64), 0, st, dt_next, nruns);
     const bool three = P.n_species >= 3.0, ens = nruns > 1;
-#define CELL_END(M, N, E) LAUNCH_R((k_cell_end<M, N, E>), R, B, F, G, P, dt_next, compute_dt, hall64)
+#define CELL_END(M, N, E) LAUNCH_R((k_cell_end<M, N, E>), R, B, F, G, P, dt_next, compute_dt)
     if (mix) {
         if (three) { if (ens) CELL_END(true, 3, true); else CELL_END(true, 3, false); }
         else { if (ens) CELL_END(true, 2, true); else CELL_END(true, 2, false); }

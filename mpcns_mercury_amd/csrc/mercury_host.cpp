@@ -872,7 +872,6 @@ public:
 
     double *const *Fd() const { return F->dptr; }
     const float *const *Gd() const { return fptr; }
-    int hall64() const { return (mix() && alias.n > 0) ? 1 : 0; }
     // the FP64 copy of an intermediate the mixed form keeps in FP32 (downloads)
     void materialize(int slot) {
         if (form != 2) return;
@@ -1103,6 +1102,7 @@ public:
     static constexpr unsigned MG_DERIVED_PAIR = (1u << G_BCELL) | (1u << G_JEDGE);
     static constexpr unsigned MG_PREPARE_PAIR = (1u << G_BADD) | (1u << G_BFACE);
 
+    static constexpr size_t kWave = 64;  // wavefront width of gfx950
     void build_merged(unsigned gset) {
         MergedProg &mp = mprog[gset];
         const int g = F->blk[0].g;
@@ -1125,14 +1125,24 @@ public:
                                 m->S.v[nset][c] = t == OP_FILL ? o->fillv[c] : 0.0;
                             }
                         }
+                        // each op's entries are padded to whole wavefronts with
+                        // entries that do nothing (dst < 0), so that the kernels can
+                        // take the op's slot set as uniform over a wave
+                        const size_t width = t == OP_COPY ? 3 : (t == OP_WALLF ? 4 + g : 2);
                         if (t == OP_COPY) {
                             for (size_t e = 0; e + 1 < o->hidx.size(); e += 2) ent.insert(ent.end(), {o->hidx[e], o->hidx[e + 1], nset});
+                            while ((ent.size() / width) % kWave) ent.insert(ent.end(), {-1, -1, nset});
                         } else if (t == OP_FILL) {
                             for (int d : o->hidx) ent.insert(ent.end(), {d, nset});
+                            while ((ent.size() / width) % kWave) ent.insert(ent.end(), {-1, nset});
                         } else if (t == OP_SETV) {
                             for (size_t e = 0; e < o->hidx.size(); ++e) {
                                 ent.insert(ent.end(), {o->hidx[e], o->slots[0]});
                                 val.push_back(o->hval[e]);
+                            }
+                            while ((ent.size() / width) % kWave) {
+                                ent.insert(ent.end(), {-1, o->slots[0]});
+                                val.push_back(0.0);
                             }
                         } else {
                             const size_t w = 2 + g;
@@ -1140,6 +1150,11 @@ public:
                                 ent.push_back(o->slots[0]);
                                 ent.push_back(o->is_na);
                                 ent.insert(ent.end(), o->hidx.begin() + e, o->hidx.begin() + e + w);
+                            }
+                            while ((ent.size() / width) % kWave) {
+                                ent.push_back(o->slots[0]);
+                                ent.push_back(o->is_na);
+                                ent.insert(ent.end(), w, -1);
                             }
                         }
                         ++nset;
@@ -1366,7 +1381,7 @@ public:
         run_merged(MG_DERIVED_PAIR);
         launch_jcell_t(st, R_cell.h, Bd(), Fd(), Gd(), mix());
         sync(G_JCELL);
-        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, final ? 1 : 0, mix(), hall64(), nruns);
+        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, final ? 1 : 0, mix(), nruns);
     }
 
     // After an update: Sync(U), Sync(Bface), the implicit resistive correction
@@ -1399,7 +1414,7 @@ public:
         sync(G_JCELL);
         HIPCHK(hipEventRecord(ev_join2, st2));
         HIPCHK(hipStreamWaitEvent(st, ev_join2, 0));
-        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, final ? 1 : 0, mix(), hall64(), nruns);
+        launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, final ? 1 : 0, mix(), nruns);
     }
 
     // StepOnce without the time step, fused passes
@@ -1425,10 +1440,10 @@ public:
             after_update(final);
             return;
         } else {
-            // periodic seams: alias mean of the non-Hall EMF before the Hall part
-            launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, 0, mix());
+            // shared edges (periodic seams, interfaces): the alias mean of the
+            // non-Hall EMF before the Hall part is added; one pass writes both
+            launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, 2, mix());
             launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_E);
-            launch_hall_edge(st, R_node.h, Bd(), Fd());
             sync(G_EHALL);
             launch_add_hall_wall(st, R_node.h, Bd(), Fd());
         }
@@ -1650,7 +1665,7 @@ public:
         derived();
         refresh_float_statics();
         if (form >= 1) derived_fused(true);
-        else launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, false, 0, nruns);
+        else launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, false, nruns);
         HIPCHK(hipStreamSynchronize(st));
     }
 
@@ -1692,7 +1707,7 @@ public:
         else if (op == "timestep") launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt, nruns);
         else if (op == "stage") stage();
         else if (op == "rhs_cell") launch_flux(st);
-        else if (op == "cell_end") launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix(), hall64(), nruns);
+        else if (op == "cell_end") launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, mix(), nruns);
         else if (op == "eface_u") launch_eface_u(st, R_eface.h, Bd(), Fd(), Gd(), mix(), P.ambipolar_enabled != 0.0, nspecies());
         else if (op == "edge_total") launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, alias.n == 0 ? 1 : 0, mix());
         else if (op == "update_all") launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix(), nspecies());

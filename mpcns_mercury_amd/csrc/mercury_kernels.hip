@@ -600,11 +600,17 @@ __global__ void k_rk2(const MercRange R, const MercBlk *__restrict__ B, double *
 // fields (and of independent sync groups). Within one launch no destination is
 // the source of another entry.
 // COPY: entry (dst, src, set), dst <- src for every slot of the set
+// The entry lists of the merged programs are padded per slot set to whole
+// wavefronts (entries with dst < 0 do nothing), so the set, and with it every
+// slot and slot pointer, is uniform over a wave: read once into scalar
+// registers instead of a per-lane chain of table loads before each copy.
 __global__ void k_copy_multi(const int *__restrict__ ent, long long n, double *const *__restrict__ F_, const MultiSets S) {
     gdouble *const *__restrict__ F = gslots(F_);
     const long long t = TID;
     if (t >= n) return;
-    const int d = ent[3 * t], s = ent[3 * t + 1], k = ent[3 * t + 2];
+    const int d = ent[3 * t], s = ent[3 * t + 1];
+    const int k = __builtin_amdgcn_readfirstlane(ent[3 * t + 2]);
+    if (d < 0) return;
     for (int c = 0; c < S.n[k]; ++c) F[S.s[k][c]][d] = F[S.s[k][c]][s];
 }
 // FILL: entry (dst, set), dst <- the set's constant per slot
@@ -612,7 +618,9 @@ __global__ void k_fill_multi(const int *__restrict__ ent, long long n, double *c
     gdouble *const *__restrict__ F = gslots(F_);
     const long long t = TID;
     if (t >= n) return;
-    const int d = ent[2 * t], k = ent[2 * t + 1];
+    const int d = ent[2 * t];
+    const int k = __builtin_amdgcn_readfirstlane(ent[2 * t + 1]);
+    if (d < 0) return;
     for (int c = 0; c < S.n[k]; ++c) F[S.s[k][c]][d] = S.v[k][c];
 }
 // SETV: entry (dst, slot) with its own value
@@ -620,7 +628,10 @@ __global__ void k_setv_multi(const int *__restrict__ ent, const double *__restri
     gdouble *const *__restrict__ F = gslots(F_);
     const long long t = TID;
     if (t >= n) return;
-    F[ent[2 * t + 1]][ent[2 * t]] = val[t];
+    const int d = ent[2 * t];
+    const int slot = __builtin_amdgcn_readfirstlane(ent[2 * t + 1]);
+    if (d < 0) return;
+    F[slot][d] = val[t];
 }
 // BC_Solid_Surface_ (Boundary_Condition_Fluid.cpp:36-260): per wall cell the
 // entry holds (species base slot, species (0 H+, 1 Na+, 2 He++), first layer,
@@ -630,8 +641,9 @@ __global__ void k_wall_multi(const int *__restrict__ ent, long long n, int g, do
     const long long t = TID;
     if (t >= n) return;
     const int *e = ent + (long long)t * (4 + g);
-    const int slot = e[0], species = e[1], is_na = species == 1;
+    const int slot = __builtin_amdgcn_readfirstlane(e[0]), species = __builtin_amdgcn_readfirstlane(e[1]), is_na = species == 1;
     e += 2;
+    if (e[0] < 0) return;  // padding of the entry list to whole waves
     const double gm1 = P.gamma - 1.0;
     const double rho_floor = species == 1 ? P.wall_rho_floor_Na : (species == 2 ? P.wall_rho_floor_He : P.wall_rho_floor_H);
     const double p_floor = P.wall_p_floor;
