@@ -85,7 +85,7 @@ STAGE_BYTES = 8 * (2 * (5 * 2 + 3) + 10 + 60 + 2 + 3 + 18)
 def measured_traffic(kernel: str, shape=None):
     """HBM bytes per launch of a fused pass from the committed rocprofv3 counter
     summary (2*FETCH_SIZE + WRITE_SIZE, see profiles/shell6_hbm_traffic.md), or None.
-    The flux pass is keyed by its launch shape (tiled, length)."""
+    The flux pass is keyed by its launch shape (kind, length)."""
     path = os.path.join(ROOT, "profiles", "shell6_hbm_traffic.md")
     try:
         text = open(path).read()
@@ -94,7 +94,7 @@ def measured_traffic(kernel: str, shape=None):
     except (OSError, IndexError, ValueError):
         return None
     if kernel == "rhs_cell" and shape is not None:
-        return table.get(f"rhs_cell {'tiled' if shape[0] else 'column'} {shape[1]}")
+        return table.get(f"rhs_cell {('column', 'tiled')[shape[0]]} {shape[1]}")
     return table.get(kernel)
 
 
@@ -204,8 +204,8 @@ def main():
         g.set_form(form)
         info = {"march": g.autotune() if form >= 1 else None}
         if form >= 1:
-            tiled, length = g.flux_shape()
-            info["flux_shape"] = {"kernel": "k_rhs_tile" if tiled else "k_rhs_cell", "tiled": tiled, "length": length}
+            kind, length = g.flux_shape()
+            info["flux_shape"] = {"kernel": ("k_rhs_cell", "k_rhs_tile")[kind], "kind": kind, "length": length}
         if oref is not None:
             g.step(args.cpu_sample_steps, case.integrator)
             err, emax = {}, {}
@@ -527,7 +527,7 @@ def main():
                    "cells": ncells, "blocks": case.nb, "form": "fp64_fused", "integrator": "euler"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (measured_traffic(dom, (forms["fp64_fused"]["flux_shape"]["tiled"],
+                     "traffic": (measured_traffic(dom, (forms["fp64_fused"]["flux_shape"]["kind"],
                                                         forms["fp64_fused"]["flux_shape"]["length"]))
                                  if args.workload == "shell6" else None), "kernel": dom,
                      "algo_bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": units,

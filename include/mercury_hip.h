@@ -330,20 +330,19 @@ int mercury_ensemble_launch(MercuryEnsemble *ens, int nsteps);
 int mercury_ensemble_destroy(MercuryEnsemble *ens);
 
 /* Launch-shape autotuner (no reference counterpart): times the flux pass
- * (Scheme_U_ + AddSourceToRHS_Fluid) as the column kernel with 1, 2, 4 and 8
- * cells per thread along i and as the LDS-tiled kernel with 4, 8 and 16 planes
- * per 16 x 16 tile, in the current form, and keeps the fastest; results are
- * unchanged by the choice.  Call after mercury_set_form; it drops a captured
- * step graph (capture again afterwards).  Writes the chosen length (query the
- * kernel with mercury_flux_shape). */
+ * (Scheme_U_ + AddSourceToRHS_Fluid) in the current form as the column kernel
+ * with 1, 2, 4 and 8 cells per thread along i and as the LDS-tiled kernel with
+ * 4, 8 and 16 planes per 16 x 16 tile, and keeps the fastest; results are unchanged by the choice.  Call after
+ * mercury_set_form; it drops a captured step graph (capture again afterwards).
+ * Writes the chosen length (query the kernel with mercury_flux_shape). */
 int mercury_autotune(MercuryContext *ctx, int *march_out);
 
-/* The flux pass's launch shape (no reference counterpart): tiled = 0 runs the
- * column kernel with length cells per thread along i, tiled = 1 the LDS-tiled
- * kernel with length planes per tile.  Both give the same result bit for bit.
- * Setting it drops a captured step graph; length < 1 is refused. */
-int mercury_set_flux_shape(MercuryContext *ctx, int tiled, int length);
-int mercury_flux_shape(MercuryContext *ctx, int *tiled, int *length);
+/* The flux pass's launch shape (no reference counterpart): kind 0 runs the
+ * column kernel with length cells per thread along i, kind 1 the LDS-tiled
+ * kernel with length planes per tile.  Both give the same result bit for bit.  Setting it
+ * drops a captured step graph; length < 1 or another kind is refused. */
+int mercury_set_flux_shape(MercuryContext *ctx, int kind, int length);
+int mercury_flux_shape(MercuryContext *ctx, int *kind, int *length);
 
 /* Average device time (ms) of an op over reps launches, HIP events on the
  * solver's stream; op "step_graph" replays the captured step. */

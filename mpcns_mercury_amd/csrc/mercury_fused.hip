@@ -128,84 +128,105 @@ DEVI void rusanov_face_t(gdouble *const *__restrict__ F, const gfloat *const *__
 }
 
 // AddSourceToRHS_Fluid (MercurySolver_FluidRHS.cpp:206-527) at owned cell q:
-// adds the sources to the flux divergence rr and writes the fluid RHS.  pe(o)
-// is the electron pressure stand-in at the neighbour o = 0..5 (+i, -i, +j, -j,
-// +k, -k) of the ambipolar gradient.
+// the quantities every species' sources share (U+, B, J x B, the number and
+// charge densities, the ambipolar electron pressure gradient, the
+// photoionisation rate and the lit/shadow collision rate) ...
+struct SrcShared {
+    double upx, upy, upz, Bx, By, Bz, sjbx, sjby, sjbz, dpex, dpey, dpez, sss, vst;
+    Num num;
+    bool ambi;
+};
+// pe(o) is the electron pressure stand-in at the neighbour o = 0..5 (+i, -i,
+// +j, -j, +k, -k) of the ambipolar gradient
 template <bool MIX, int NS, typename PE>
-DEVI void rhs_sources(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, const MercPhys &P, long long q, long long di,
-                      long long dj, double (&rr)[NS][5], PE pe) {
-    const double upx = st<MIX>(F, G, SL_UP, q), upy = st<MIX>(F, G, SL_UP + 1, q), upz = st<MIX>(F, G, SL_UP + 2, q);
-    const double Bx = F[SL_BC][q], By = F[SL_BC + 1][q], Bz = F[SL_BC + 2][q];
+DEVI SrcShared src_shared(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, const MercPhys &P, long long q, long long di,
+                          long long dj, PE pe) {
+    SrcShared c;
+    c.upx = st<MIX>(F, G, SL_UP, q), c.upy = st<MIX>(F, G, SL_UP + 1, q), c.upz = st<MIX>(F, G, SL_UP + 2, q);
+    c.Bx = F[SL_BC][q], c.By = F[SL_BC + 1][q], c.Bz = F[SL_BC + 2][q];
     const double Jx = F[SL_JC][q], Jy = F[SL_JC + 1][q], Jz = F[SL_JC + 2][q];
-    const double sjbx = Jy * Bz - Jz * By, sjby = Jz * Bx - Jx * Bz, sjbz = Jx * By - Jy * Bx;
-    const Num num = num_at(P, F, q, NS);
-    double dpex = 0.0, dpey = 0.0, dpez = 0.0;
-    const bool ambi = P.ambipolar_enabled != 0.0;
-    if (ambi) {
+    c.sjbx = Jy * c.Bz - Jz * c.By, c.sjby = Jz * c.Bx - Jx * c.Bz, c.sjbz = Jx * c.By - Jy * c.Bx;
+    c.num = num_at(P, F, q, NS);
+    c.dpex = 0.0, c.dpey = 0.0, c.dpez = 0.0;
+    c.ambi = P.ambipolar_enabled != 0.0;
+    if (c.ambi) {
         const double V = fabs(F[SL_JAC][q]);
         double d[9];
         if (V <= 0.0) {
             for (int t = 0; t < 9; ++t) d[t] = 0.0;
         } else {
-            for (int c = 0; c < 3; ++c) {
-                d[c] = 0.5 * (F[SL_JD + c][q - di] + F[SL_JD + c][q]) / V;
-                d[3 + c] = 0.5 * (F[SL_JD + 3 + c][q - dj] + F[SL_JD + 3 + c][q]) / V;
-                d[6 + c] = 0.5 * (F[SL_JD + 6 + c][q - 1] + F[SL_JD + 6 + c][q]) / V;
+            for (int k = 0; k < 3; ++k) {
+                d[k] = 0.5 * (F[SL_JD + k][q - di] + F[SL_JD + k][q]) / V;
+                d[3 + k] = 0.5 * (F[SL_JD + 3 + k][q - dj] + F[SL_JD + 3 + k][q]) / V;
+                d[6 + k] = 0.5 * (F[SL_JD + 6 + k][q - 1] + F[SL_JD + 6 + k][q]) / V;
             }
         }
         const double pip = pe(0), pim = pe(1), pjp = pe(2), pjm = pe(3), pkp = pe(4), pkm = pe(5);
-        dpex = 0.5 * (d[0] * (pip - pim) + d[3] * (pjp - pjm) + d[6] * (pkp - pkm));
-        dpey = 0.5 * (d[1] * (pip - pim) + d[4] * (pjp - pjm) + d[7] * (pkp - pkm));
-        dpez = 0.5 * (d[2] * (pip - pim) + d[5] * (pjp - pjm) + d[8] * (pkp - pkm));
+        c.dpex = 0.5 * (d[0] * (pip - pim) + d[3] * (pjp - pjm) + d[6] * (pkp - pkm));
+        c.dpey = 0.5 * (d[1] * (pip - pim) + d[4] * (pjp - pjm) + d[7] * (pkp - pkm));
+        c.dpez = 0.5 * (d[2] * (pip - pim) + d[5] * (pjp - pjm) + d[8] * (pkp - pkm));
     }
-    const double sss = (double)st<MIX>(F, G, SL_PHOTO, q);
+    c.sss = (double)st<MIX>(F, G, SL_PHOTO, q);
     const double x = F[SL_CC][q], y = F[SL_CC + 1][q], z = F[SL_CC + 2][q];
     const double rho_cyl = sqrt(y * y + z * z);
     const bool in_shadow = (x < 0.0 && rho_cyl < 1.0);
-    const double vst = in_shadow ? P.nu_shadow : P.nu_lit;
-    for (int s = 0; s < NS; ++s) {
-        const int su = SPEC_U(s), sp = SPEC_PV(s), sr = SPEC_R(s);
-        const double rho = dmax(F[su][q], 0.0);
-        const double u = st<MIX>(F, G, sp, q), v = st<MIX>(F, G, sp + 1, q), w = st<MIX>(F, G, sp + 2, q);
-        const double u2 = u * u + v * v + w * w;
-        const double subx = (v - upy) * Bz - (w - upz) * By;
-        const double suby = (w - upz) * Bx - (u - upx) * Bz;
-        const double subz = (u - upx) * By - (v - upy) * Bx;
-        const double subu = subx * u + suby * v + subz * w;
-        const double sjbu = sjbx * u + sjby * v + sjbz * w;
-        const double dpeu = dpex * u + dpey * v + dpez * w;
-        const double Ts0 = (double)st<MIX>(F, G, sp + 4, q) * P.T_ref;
-        const double Ms = species_mass(P, s);
-        const double Mo = P.M_Na;
-        const double b1 = vst * (P.Tn0 - Ts0) * (Ms / (Ms + Mo));
-        const double b2 = (Mo / (Ms + Mo)) * vst;
-        // number density in the thermal exchange, charge density in the induction force
-        const double nsp = num.n[s], qs = num.q[s];
-        const double chi = num.w[s];
-        double r0 = rr[s][0], r1 = rr[s][1], r2 = rr[s][2], r3 = rr[s][3], r4 = rr[s][4];
-        r1 += P.c_ind * qs * subx + P.c_hall * chi * sjbx;
-        r2 += P.c_ind * qs * suby + P.c_hall * chi * sjby;
-        r3 += P.c_ind * qs * subz + P.c_hall * chi * sjbz;
-        r4 += P.c_ind * qs * subu + P.c_hall * chi * sjbu;
-        if (ambi) {
-            r1 -= chi * dpex;
-            r2 -= chi * dpey;
-            r3 -= chi * dpez;
-            r4 -= chi * dpeu;
-        }
-        if (s == 1) {
-            r0 += P.a1_Na * sss;
-            r4 += P.a6 * sss * P.Tn0;
-        }
-        r1 -= P.a4 * rho * u * vst;
-        r2 -= P.a4 * rho * v * vst;
-        r3 -= P.a4 * rho * w * vst;
-        r4 -= P.a4 * rho * u2 * vst;
-        r4 += P.a5 * nsp * b1;
-        r4 += P.a4 * rho * u2 * b2;
-        wr<MIX>(F, G, sr, q, r0); wr<MIX>(F, G, sr + 1, q, r1); wr<MIX>(F, G, sr + 2, q, r2);
-        wr<MIX>(F, G, sr + 3, q, r3); wr<MIX>(F, G, sr + 4, q, r4);
+    c.vst = in_shadow ? P.nu_shadow : P.nu_lit;
+    return c;
+}
+
+// ... and one species' sources added to its flux divergence r, written as its RHS
+template <bool MIX>
+DEVI void src_species(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, const MercPhys &P, long long q, int s,
+                      const SrcShared &c, const double r[5]) {
+    const int su = SPEC_U(s), sp = SPEC_PV(s), sr = SPEC_R(s);
+    const double rho = dmax(F[su][q], 0.0);
+    const double u = st<MIX>(F, G, sp, q), v = st<MIX>(F, G, sp + 1, q), w = st<MIX>(F, G, sp + 2, q);
+    const double u2 = u * u + v * v + w * w;
+    const double subx = (v - c.upy) * c.Bz - (w - c.upz) * c.By;
+    const double suby = (w - c.upz) * c.Bx - (u - c.upx) * c.Bz;
+    const double subz = (u - c.upx) * c.By - (v - c.upy) * c.Bx;
+    const double subu = subx * u + suby * v + subz * w;
+    const double sjbu = c.sjbx * u + c.sjby * v + c.sjbz * w;
+    const double dpeu = c.dpex * u + c.dpey * v + c.dpez * w;
+    const double Ts0 = (double)st<MIX>(F, G, sp + 4, q) * P.T_ref;
+    const double Ms = species_mass(P, s);
+    const double Mo = P.M_Na;
+    const double b1 = c.vst * (P.Tn0 - Ts0) * (Ms / (Ms + Mo));
+    const double b2 = (Mo / (Ms + Mo)) * c.vst;
+    // number density in the thermal exchange, charge density in the induction force
+    const double nsp = c.num.n[s], qs = c.num.q[s];
+    const double chi = c.num.w[s];
+    double r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4];
+    r1 += P.c_ind * qs * subx + P.c_hall * chi * c.sjbx;
+    r2 += P.c_ind * qs * suby + P.c_hall * chi * c.sjby;
+    r3 += P.c_ind * qs * subz + P.c_hall * chi * c.sjbz;
+    r4 += P.c_ind * qs * subu + P.c_hall * chi * sjbu;
+    if (c.ambi) {
+        r1 -= chi * c.dpex;
+        r2 -= chi * c.dpey;
+        r3 -= chi * c.dpez;
+        r4 -= chi * dpeu;
     }
+    if (s == 1) {
+        r0 += P.a1_Na * c.sss;
+        r4 += P.a6 * c.sss * P.Tn0;
+    }
+    r1 -= P.a4 * rho * u * c.vst;
+    r2 -= P.a4 * rho * v * c.vst;
+    r3 -= P.a4 * rho * w * c.vst;
+    r4 -= P.a4 * rho * u2 * c.vst;
+    r4 += P.a5 * nsp * b1;
+    r4 += P.a4 * rho * u2 * b2;
+    wr<MIX>(F, G, sr, q, r0); wr<MIX>(F, G, sr + 1, q, r1); wr<MIX>(F, G, sr + 2, q, r2);
+    wr<MIX>(F, G, sr + 3, q, r3); wr<MIX>(F, G, sr + 4, q, r4);
+}
+
+// the sources of every species of the cell
+template <bool MIX, int NS, typename PE>
+DEVI void rhs_sources(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, const MercPhys &P, long long q, long long di,
+                      long long dj, double (&rr)[NS][5], PE pe) {
+    const SrcShared c = src_shared<MIX, NS>(F, G, P, q, di, dj, pe);
+    for (int s = 0; s < NS; ++s) src_species<MIX>(F, G, P, q, s, c, rr[s]);
 }
 
 // ---------------------------------------------------------------------------

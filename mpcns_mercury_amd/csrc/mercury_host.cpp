@@ -836,7 +836,7 @@ public:
         HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&ev_fork2, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&ev_join2, hipEventDisableTiming));
-        set_flux_shape(tile_c > 0, tile_c > 0 ? tile_c : march);
+        set_flux_shape(tile_c > 0 ? 1 : 0, tile_c > 0 ? tile_c : march);
         cls.build(*F);
         build_programs();
         alias.build(cls);
@@ -889,24 +889,26 @@ public:
     // keeps the fastest; the choice never changes a result.  Returns the
     // chosen length (cells per thread, or planes per tile when tiled).
     int autotune() {
-        int best_len = march;
-        bool best_tiled = tile_c > 0;
+        int best_len = tile_c > 0 ? tile_c : march, best_kind = flux_kind();
         double tbest = 1e300;
-        for (int tiled = 0; tiled < 2; ++tiled)
-            for (int L : tiled ? std::vector<int>{4, 8, 16} : std::vector<int>{1, 2, 4, 8}) {
-                set_flux_shape(tiled != 0, L);
+        for (int kind = 0; kind < 2; ++kind)
+            for (int L : kind == 1 ? std::vector<int>{4, 8, 16} : std::vector<int>{1, 2, 4, 8}) {
+                set_flux_shape(kind, L);
                 const double t = time_op("rhs_cell", 5);
-                if (t < tbest) tbest = t, best_len = L, best_tiled = tiled != 0;
+                if (t < tbest) tbest = t, best_len = L, best_kind = kind;
             }
-        set_flux_shape(best_tiled, best_len);
+        set_flux_shape(best_kind, best_len);
         return best_len;
     }
+    // 0: column kernel, 1: tiled kernel
+    int flux_kind() const { return tile_c > 0 ? 1 : 0; }
 
-    // the flux pass's launch shape: the column kernel with L cells per thread
-    // along i, or the tiled kernel with L planes per 16 x 16 tile; drops a
-    // captured step graph (it holds the old shape)
-    void set_flux_shape(bool tiled, int L) {
+    // the flux pass's launch shape (kind as flux_kind): L cells per thread
+    // along i for the column kernel, L planes per 16 x 16 tile for the tiled
+    // one; drops a captured step graph (it holds the old shape)
+    void set_flux_shape(int kind, int L) {
         if (L < 1) throw std::runtime_error("mercury_set_flux_shape: length must be >= 1");
+        if (kind < 0 || kind > 1) throw std::runtime_error("mercury_set_flux_shape: kind must be 0 (column) or 1 (tiled)");
         ++generation;
         if (gexec) {
             (void)hipGraphExecDestroy(gexec);
@@ -915,7 +917,7 @@ public:
             graph = nullptr;
         }
         std::vector<int> lo(3 * F->nb(), 0), n(3 * F->nb());
-        if (tiled) {
+        if (kind == 1) {
             tile_c = L;
             for (int b = 0; b < F->nb(); ++b) {
                 n[3 * b] = (F->blk[b].m[0] + L - 1) / L;
@@ -2614,19 +2616,19 @@ int mercury_autotune(MercuryContext *ctx, int *march_out) {
     })
 }
 
-int mercury_set_flux_shape(MercuryContext *ctx, int tiled, int length) {
+int mercury_set_flux_shape(MercuryContext *ctx, int kind, int length) {
     CHECK_CTX(ctx);
     if (length < 1) {
         merc::g_last_error = "mercury_set_flux_shape: length must be >= 1";
         return -1;
     }
-    ABI_TRY({ ctx->s->set_flux_shape(tiled != 0, length); })
+    ABI_TRY({ ctx->s->set_flux_shape(kind, length); })
 }
 
-int mercury_flux_shape(MercuryContext *ctx, int *tiled, int *length) {
+int mercury_flux_shape(MercuryContext *ctx, int *kind, int *length) {
     CHECK_CTX(ctx);
     ABI_TRY({
-        if (tiled) *tiled = ctx->s->tile_c > 0 ? 1 : 0;
+        if (kind) *kind = ctx->s->flux_kind();
         if (length) *length = ctx->s->tile_c > 0 ? ctx->s->tile_c : ctx->s->march;
     })
 }

@@ -328,16 +328,17 @@ class Mercury:
         _chk(load().mercury_field_device(self._h, name.encode(), block, comp, ctypes.byref(v)))
         return v.value
 
-    def set_flux_shape(self, tiled, length):
-        """Run the flux pass as the column kernel (tiled False, length cells per
-        thread along i) or the LDS-tiled kernel (length planes per tile)."""
-        _chk(load().mercury_set_flux_shape(self._h, 1 if tiled else 0, int(length)))
+    def set_flux_shape(self, kind, length):
+        """Run the flux pass as the column kernel (kind 0 or False: length cells
+        per thread along i) or the LDS-tiled kernel (1 or True: length planes
+        per tile)."""
+        _chk(load().mercury_set_flux_shape(self._h, int(kind), int(length)))
 
     def flux_shape(self):
-        """(tiled, length) of the flux pass's launch shape."""
+        """(kind, length) of the flux pass's launch shape."""
         t, n = ctypes.c_int(), ctypes.c_int()
         _chk(load().mercury_flux_shape(self._h, ctypes.byref(t), ctypes.byref(n)))
-        return bool(t.value), n.value
+        return t.value, n.value
 
     def time_op(self, op, reps=10):
         v = ctypes.c_double()

@@ -173,10 +173,13 @@ def test_tiled_steps_and_graph_bit_exact(pair, integrator):
     _cmp(g, o, STATE + DERIVED, f"tiled graph replay integrator={integrator}")
 
 
+
 def test_flux_shape_refuses_zero_length(pair):
     case, g, o = pair
     with pytest.raises(MercuryError, match="length must be >= 1"):
         g.set_flux_shape(True, 0)
+    with pytest.raises(MercuryError, match="kind must be"):
+        g.set_flux_shape(2, 4)
 
 
 def test_induction_bit_exact(pair):
@@ -234,10 +237,10 @@ def test_autotune_keeps_results(pair):
     o.prepare()
     g.set_form(1)
     L = g.autotune()
-    tiled, length = g.flux_shape()
-    assert length == L and L in ((4, 8, 16) if tiled else (1, 2, 4, 8))
+    kind, length = g.flux_shape()
+    assert length == L and L in ((4, 8, 16) if kind == 1 else (1, 2, 4, 8)) and kind in (0, 1)
     np.testing.assert_array_equal(g.step(3, 0), o.step(3, 0))
-    _cmp(g, o, STATE + DERIVED, f"3 steps after autotune (tiled {tiled}, length {L})")
+    _cmp(g, o, STATE + DERIVED, f"3 steps after autotune (kind {kind}, length {L})")
 
 
 @pytest.mark.parametrize("integrator", [0, 1])
