@@ -32,8 +32,13 @@ extern "C" {
  *           (Z4_Mercury/src/physics/MercurySolver_MagneticRHS.cpp:5-89).
  * INTERFACE: a 1:1 inner connection to a neighbour block (the reference's Inner
  *           InterfacePatch, include/2_topology/TopologyTypes.h:104-139), given to
- *           mercury_create_connected. */
-enum { MERCURY_BC_PERIODIC = 0, MERCURY_BC_FARFIELD = 1, MERCURY_BC_WALL = 2, MERCURY_BC_INTERFACE = 3 };
+ *           mercury_create_connected.
+ * EXTERNAL: a face whose ghosts and shared DOFs the caller provides: no handler
+ *           and no neighbour in the context; its halo copies come from
+ *           mercury_set_sync_plan and the alias means of its shared edges from
+ *           mercury_set_alias_classes (any connection the caller's own topology
+ *           code can describe, rotated, mirrored or partial-face included). */
+enum { MERCURY_BC_PERIODIC = 0, MERCURY_BC_FARFIELD = 1, MERCURY_BC_WALL = 2, MERCURY_BC_INTERFACE = 3, MERCURY_BC_EXTERNAL = 4 };
 
 /* Integrators: forward Euler (the reference StepOnce) or SSP-RK2 (two Euler
  * stages and the average; no reference counterpart). */
@@ -151,6 +156,40 @@ int mercury_field_device(MercuryContext *ctx, const char *name, int block, int c
 /* The context's HIP stream (a hipStream_t): work a caller orders against the
  * step goes on it. */
 int mercury_stream(MercuryContext *ctx, void **stream);
+/* Halo plans as flat int32 lists (no single reference counterpart: they carry
+ * what Halo's patterns do, include/4_halo/HaloTypes.h:8-35, applied as
+ * Halo::exchange_inner / the triplet copies, src/4_halo/component_copy/
+ * HaloExchangeInner.cpp:90-247, src/4_halo/forms/HaloTriplet.cpp:641-725, and
+ * the owner -> alias copies, src/4_halo/core/HaloSyncFramework.cpp:695-860).
+ * group: a sync group of MercuryBoundary::Sync ("Ucell", "Bface", "Eedge",
+ * "Ehall", "Jedge", "B_cell", "J_cell", "Badd", "Eres", "Eres0" (Eres with a
+ * zero far field), "dB").  stage 0: the halo copies, stage 1: the owner ->
+ * alias copies, which run after them.  An entry is 11 int32:
+ *   dst slot, dst block, di, dj, dk, src slot, src block, si, sj, sk, sign
+ * with slot = the slot of mercury_field_info + the component, points (i, j, k)
+ * in the reference's index space of the slot's location (ghosts negative or
+ * beyond the owned range, inside the block's box), sign +1 or -1; the entry
+ * sets dst = sign * src, and the entries of one stage run concurrently.
+ * Setting a stage replaces the context's own copies of that stage for that
+ * group (the other stage keeps what it had); the physical handlers and corner
+ * defaults of the group still run first.  Drops a captured step graph.
+ * mercury_sync_plan writes the stage's current entries (the context's own
+ * until a caller sets them) and their number; with entries NULL or capacity
+ * below the number, only the number. */
+int mercury_set_sync_plan(MercuryContext *ctx, const char *group, int stage, long long n, const int *entries);
+int mercury_sync_plan(MercuryContext *ctx, const char *group, int stage, long long capacity, long long *n, int *entries);
+/* The classes of ReduceEdgeAliasCandidatesToOwners_ (Z4_Mercury/src/physics/
+ * MercurySolver_Current.cpp:3-57): for class c, edges of axis axis[c] (0 xi,
+ * 1 eta, 2 zeta), members start[c] .. start[c+1]-1 of members (5 int32 each:
+ * block, i, j, k, sign), owner owners[5c .. 5c+4]; the E and J edge passes set
+ * owner = owner sign * (sum of member sign * member) / count.  Setting replaces
+ * the context's own classes; mercury_alias_classes writes the current ones
+ * (counts only when an array is NULL or too small).  Drops a captured step
+ * graph. */
+int mercury_set_alias_classes(MercuryContext *ctx, int ncls, const int *start, const int *axis, const int *members, const int *owners);
+int mercury_alias_classes(MercuryContext *ctx, int cls_capacity, long long member_capacity, int *ncls, long long *nmembers, int *start,
+                          int *axis, int *members, int *owners);
+
 /* Run the context's work on a caller's HIP stream (a hipStream_t; NULL returns
  * to the context's own stream).  No reference counterpart: the reference's
  * solver is synchronous host code; this is the stream of the drop-in's context

@@ -29,7 +29,7 @@ import numpy as np
 
 from .params import MercuryPhysics, MercuryUnits, derive, dipole_array, RHO_SMALL
 
-BC_PERIODIC, BC_FARFIELD, BC_WALL, BC_INTERFACE = 0, 1, 2, 3
+BC_PERIODIC, BC_FARFIELD, BC_WALL, BC_INTERFACE, BC_EXTERNAL = 0, 1, 2, 3, 4
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 

@@ -109,7 +109,9 @@ struct MercRange {
 };
 
 // face codes; MBC_INTERFACE: 1:1 connection to a neighbour block (node-index translation)
-enum MercBC { MBC_PERIODIC = 0, MBC_FARFIELD = 1, MBC_WALL = 2, MBC_INTERFACE = 3 };
+// MBC_EXTERNAL: a face whose ghosts (and shared DOFs) come from a caller-supplied
+// sync plan and alias classes; the context adds no handler and no neighbour there
+enum MercBC { MBC_PERIODIC = 0, MBC_FARFIELD = 1, MBC_WALL = 2, MBC_INTERFACE = 3, MBC_EXTERNAL = 4 };
 
 // slot sets of a merged sync op (passed by value as a kernel argument)
 #define MERC_MAX_SETS 12

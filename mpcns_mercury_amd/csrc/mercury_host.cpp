@@ -198,6 +198,33 @@ public:
             if (x >= 0) return true;
         return false;
     }
+    bool has_external() const {
+        for (const auto &k : blk)
+            for (int f = 0; f < 6; ++f)
+                if (k.bc[f] == MBC_EXTERNAL) return true;
+        return false;
+    }
+    // (block, i, j, k) of an arena index (the inverse of gidx)
+    void point_of(int x, int p[4]) const {
+        int b = 0;
+        while (b + 1 < nb() && blk[b + 1].off <= x) ++b;
+        const MercBlk &q = blk[b];
+        long long r = x - q.off;
+        p[0] = b;
+        p[1] = (int)(r / q.sI) - q.g;
+        r %= q.sI;
+        p[2] = (int)(r / q.sJ) - q.g;
+        p[3] = (int)(r % q.sJ) - q.g;
+    }
+    // arena index of (block, i, j, k) with a check that the point lies in the block's box
+    int gidx_checked(int b, int i, int j, int k, const char *what) const {
+        if (b < 0 || b >= nb()) throw std::runtime_error(std::string(what) + ": block out of range");
+        const MercBlk &q = blk[b];
+        const int ix[3] = {i, j, k};
+        for (int a = 0; a < 3; ++a)
+            if (ix[a] < -q.g || ix[a] > q.m[a] + q.g) throw std::runtime_error(std::string(what) + ": point outside the block's box");
+        return gidx(b, i, j, k);
+    }
     bool has_interface() const {
         for (const auto &k : blk)
             for (int f = 0; f < 6; ++f)
@@ -570,6 +597,24 @@ struct Classes {
 struct AliasClasses {
     DBuf<int> start, member, msign, owner, osign, slot_of;
     long long n = 0;
+    // host copies (arena indices), for the export
+    std::vector<int> hst{0}, hmem, hms, how, hos, hsl;
+    void set(std::vector<int> st, std::vector<int> mem, std::vector<int> ms, std::vector<int> ow, std::vector<int> os,
+             std::vector<int> sl) {
+        n = (long long)ow.size();
+        start.upload(st);
+        member.upload(mem);
+        msign.upload(ms);
+        owner.upload(ow);
+        osign.upload(os);
+        slot_of.upload(sl);
+        hst = std::move(st);
+        hmem = std::move(mem);
+        hms = std::move(ms);
+        how = std::move(ow);
+        hos = std::move(os);
+        hsl = std::move(sl);
+    }
     void build(const Classes &C) {
         std::vector<int> st{0}, mem, ms, ow, os, sl;
         for (int a = 0; a < 3; ++a) {
@@ -585,13 +630,7 @@ struct AliasClasses {
                 sl.push_back(a);
             }
         }
-        n = (long long)ow.size();
-        start.upload(st);
-        member.upload(mem);
-        msign.upload(ms);
-        owner.upload(ow);
-        osign.upload(os);
-        slot_of.upload(sl);
+        set(std::move(st), std::move(mem), std::move(ms), std::move(ow), std::move(os), std::move(sl));
     }
 };
 
@@ -640,6 +679,17 @@ public:
     int tile_c = 0;      // planes per tile along i; 0: the column kernel runs
     GroupProg prog[G_NGROUP];
     std::map<unsigned, MergedProg> mprog;  // keyed by the bit set of groups
+    // caller sync plans: per group, stage 0 stands in for the context's own
+    // halo copies and stage 1 for its owner -> alias copies; entries
+    // (dst slot, dst, src slot, src, sign) on arena indices
+    bool caller_plan[G_NGROUP] = {};
+    std::vector<int> plan_h[G_NGROUP][2];
+    DBuf<int> plan_d[G_NGROUP][2];
+    bool any_caller_plan() const {
+        for (bool b : caller_plan)
+            if (b) return true;
+        return false;
+    }
     Classes cls;          // shared-DOF classes (owner / aliases) of every location
     AliasClasses alias;
     hipStream_t st = nullptr;      // the stream all of the context's work goes to
@@ -691,7 +741,7 @@ public:
                 if (dims[3 * b + a] < 2) throw std::runtime_error("every block needs at least 2 cells per axis");
             for (int f = 0; f < 6; ++f) {
                 const int c = bc[6 * b + f];
-                if (c < 0 || c > 3) throw std::runtime_error("unknown boundary code");
+                if (c < 0 || c > MBC_EXTERNAL) throw std::runtime_error("unknown boundary code");
                 if (c == MBC_PERIODIC && bc[6 * b + (f ^ 1)] != MBC_PERIODIC) throw std::runtime_error("periodic faces must be paired");
                 if (c == MBC_PERIODIC && dims[3 * b + f / 2] < g) throw std::runtime_error("a periodic axis needs at least nghost cells");
                 if (c != MBC_INTERFACE) continue;
@@ -1090,11 +1140,50 @@ public:
                 ++phase;
             }
         }
+        build_all_merged();
+    }
+    void build_all_merged() {
+        mprog.clear();
         for (int gid = 0; gid < G_NGROUP; ++gid) build_merged(1u << gid);
         build_merged(MG_EDGE_PAIR);
         build_merged(MG_STATE_PAIR);
         build_merged(MG_DERIVED_PAIR);
         build_merged(MG_PREPARE_PAIR);
+    }
+
+    // the group's copies of one stage (0: halo, 1: owner -> alias) as
+    // (dst slot, dst, src slot, src, sign): the caller's plan when one is set,
+    // else the context's own
+    std::vector<int> plan_entries(int gid, int stage) const {
+        if (caller_plan[gid]) return plan_h[gid][stage];
+        std::vector<int> out;
+        for (const auto &o : prog[gid].ops) {
+            if (o->type != OP_COPY || o->phase != (stage == 0 ? kPhaseHalo : kPhaseOwner)) continue;
+            for (size_t e = 0; e + 1 < o->hidx.size(); e += 2)
+                for (int sl : o->slots) out.insert(out.end(), {sl, o->hidx[e], sl, o->hidx[e + 1], 1});
+        }
+        return out;
+    }
+    void set_sync_plan(int gid, int stage, std::vector<int> ent) {
+        if (gid < 0 || gid >= G_NGROUP || stage < 0 || stage > 1) throw std::runtime_error("mercury_set_sync_plan: unknown group or stage");
+        if (!caller_plan[gid]) {
+            // the other stage starts as the context's own copies
+            for (int t = 0; t < 2; ++t) plan_h[gid][t] = plan_entries(gid, t);
+            caller_plan[gid] = true;
+        }
+        plan_h[gid][stage] = std::move(ent);
+        for (int t = 0; t < 2; ++t) plan_d[gid][t].upload(plan_h[gid][t]);
+        build_all_merged();
+        drop_graph();
+    }
+    void drop_graph() {
+        ++generation;
+        if (gexec) {
+            (void)hipGraphExecDestroy(gexec);
+            (void)hipGraphDestroy(graph);
+            gexec = nullptr;
+            graph = nullptr;
+        }
     }
 
     // Groups that read and write disjoint slots and that the step syncs back
@@ -1105,6 +1194,7 @@ public:
     static constexpr unsigned MG_PREPARE_PAIR = (1u << G_BADD) | (1u << G_BFACE);
 
     static constexpr size_t kWave = 64;  // wavefront width of gfx950
+    static constexpr int kPhaseHalo = 4, kPhaseOwner = 5;  // phases of a group's program (A, B, C, D, H, E)
     void build_merged(unsigned gset) {
         MergedProg &mp = mprog[gset];
         const int g = F->blk[0].g;
@@ -1117,6 +1207,8 @@ public:
                 std::vector<double> val;
                 for (int gid = 0; gid < G_NGROUP; ++gid) {
                     if (!(gset & (1u << gid))) continue;
+                    // a caller plan stands in for the halo (phase 4) and owner (phase 5) copies
+                    if (caller_plan[gid] && phase >= kPhaseHalo) continue;
                     for (auto &o : prog[gid].ops) {
                         if (o->phase != phase || o->type != t) continue;
                         if (t == OP_COPY || t == OP_FILL) {
@@ -1185,6 +1277,11 @@ public:
             default: break;
             }
         }
+        // caller plans: the halo stage of every group of the set, then the owner stage
+        for (int t = 0; t < 2; ++t)
+            for (int gid = 0; gid < G_NGROUP; ++gid)
+                if ((gset & (1u << gid)) && caller_plan[gid] && plan_d[gid][t].n > 0)
+                    launch_copy_plan(s, plan_d[gid][t].p, plan_d[gid][t].n / 5, Fd());
     }
 
     void sync(int gid) { run_merged(1u << gid); }
@@ -1200,6 +1297,8 @@ public:
     void build_resist() {
         resist = P.resist_enabled != 0.0;
         if (!resist) return;
+        if (F->has_external())
+            throw std::runtime_error("the implicit resistive correction needs the context's own topology: no EXTERNAL faces");
         std::vector<int> q, ax, si, sj;
         std::vector<double> eta;
         const Field &Fl = *F;
@@ -1405,7 +1504,7 @@ public:
         run_merged(1u << G_BFACE, st);
         launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix());
         launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
-        if (alias.n == 0 && !F->connected()) {
+        if (alias.n == 0 && !F->connected() && !any_caller_plan()) {
             HIPCHK(hipEventRecord(ev_fork2, st));
             HIPCHK(hipStreamWaitEvent(st2, ev_fork2, 0));
             run_merged(MG_DERIVED_PAIR, st2);
@@ -1922,6 +2021,118 @@ int mercury_stream(MercuryContext *ctx, void **stream) {
     }
     *stream = (void *)ctx->s->st;
     return 0;
+}
+
+// Sync groups by the names of MercuryBoundary::Sync (Boundary_Build.cpp:564-703),
+// in GroupId order; "Eres0" is Eres with a zero far field, "dB" the resistive dB
+static int sync_group_id(const char *name) {
+    static const char *const names[] = {"Ucell", "Bface", "Eedge", "Ehall", "Jedge", "B_cell", "J_cell", "Badd", "Eres", "Eres0", "dB"};
+    static_assert(sizeof(names) / sizeof(names[0]) == merc::G_NGROUP, "one name per sync group");
+    if (name)
+        for (int g = 0; g < merc::G_NGROUP; ++g)
+            if (!std::strcmp(name, names[g])) return g;
+    throw std::runtime_error(std::string("unknown sync group ") + (name ? name : "(null)"));
+}
+
+int mercury_set_sync_plan(MercuryContext *ctx, const char *group, int stage, long long n, const int *entries) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        if (n < 0 || (n > 0 && !entries)) throw std::runtime_error("mercury_set_sync_plan: invalid entries");
+        merc::MercurySolver &S = *ctx->s;
+        const int gid = sync_group_id(group);
+        std::vector<int> ent;
+        ent.reserve(5 * (size_t)n);
+        for (long long e = 0; e < n; ++e) {
+            const int *x = entries + 11 * e;
+            if (x[0] < 0 || x[0] >= SL_NSLOT || x[5] < 0 || x[5] >= SL_NSLOT)
+                throw std::runtime_error("mercury_set_sync_plan: slot out of range in entry " + std::to_string(e));
+            if (x[10] != 1 && x[10] != -1) throw std::runtime_error("mercury_set_sync_plan: sign must be +1 or -1");
+            const int d = S.F->gidx_checked(x[1], x[2], x[3], x[4], "mercury_set_sync_plan destination");
+            const int src = S.F->gidx_checked(x[6], x[7], x[8], x[9], "mercury_set_sync_plan source");
+            ent.insert(ent.end(), {x[0], d, x[5], src, x[10]});
+        }
+        HIPCHK(hipStreamSynchronize(S.st));
+        S.set_sync_plan(gid, stage, std::move(ent));
+    })
+}
+
+int mercury_sync_plan(MercuryContext *ctx, const char *group, int stage, long long capacity, long long *n, int *entries) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        if (!n || stage < 0 || stage > 1) throw std::runtime_error("mercury_sync_plan: invalid argument");
+        merc::MercurySolver &S = *ctx->s;
+        const std::vector<int> ent = S.plan_entries(sync_group_id(group), stage);
+        *n = (long long)ent.size() / 5;
+        if (!entries || capacity < *n) return 0;
+        for (long long e = 0; e < *n; ++e) {
+            const int *x = ent.data() + 5 * e;
+            int d[4], src[4];
+            S.F->point_of(x[1], d);
+            S.F->point_of(x[3], src);
+            int *y = entries + 11 * e;
+            y[0] = x[0];
+            for (int c = 0; c < 4; ++c) y[1 + c] = d[c];
+            y[5] = x[2];
+            for (int c = 0; c < 4; ++c) y[6 + c] = src[c];
+            y[10] = x[4];
+        }
+    })
+}
+
+int mercury_set_alias_classes(MercuryContext *ctx, int ncls, const int *start, const int *axis, const int *members, const int *owners) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        if (ncls < 0 || (ncls > 0 && (!start || !axis || !members || !owners)))
+            throw std::runtime_error("mercury_set_alias_classes: invalid argument");
+        merc::MercurySolver &S = *ctx->s;
+        std::vector<int> st{0}, mem, ms, ow, os, sl;
+        if (ncls > 0 && start[0] != 0) throw std::runtime_error("mercury_set_alias_classes: start[0] must be 0");
+        for (int c = 0; c < ncls; ++c) {
+            if (start[c + 1] <= start[c]) throw std::runtime_error("mercury_set_alias_classes: empty class " + std::to_string(c));
+            if (axis[c] < 0 || axis[c] > 2) throw std::runtime_error("mercury_set_alias_classes: axis must be 0, 1 or 2");
+            for (int t = start[c]; t < start[c + 1]; ++t) {
+                const int *m = members + 5 * (long long)t;
+                if (m[4] != 1 && m[4] != -1) throw std::runtime_error("mercury_set_alias_classes: sign must be +1 or -1");
+                mem.push_back(S.F->gidx_checked(m[0], m[1], m[2], m[3], "mercury_set_alias_classes member"));
+                ms.push_back(m[4]);
+            }
+            st.push_back((int)mem.size());
+            const int *o = owners + 5 * (long long)c;
+            if (o[4] != 1 && o[4] != -1) throw std::runtime_error("mercury_set_alias_classes: sign must be +1 or -1");
+            ow.push_back(S.F->gidx_checked(o[0], o[1], o[2], o[3], "mercury_set_alias_classes owner"));
+            os.push_back(o[4]);
+            sl.push_back(axis[c]);
+        }
+        HIPCHK(hipStreamSynchronize(S.st));
+        S.alias.set(std::move(st), std::move(mem), std::move(ms), std::move(ow), std::move(os), std::move(sl));
+        S.drop_graph();
+    })
+}
+
+int mercury_alias_classes(MercuryContext *ctx, int cls_capacity, long long member_capacity, int *ncls, long long *nmembers, int *start,
+                          int *axis, int *members, int *owners) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        if (!ncls || !nmembers) throw std::runtime_error("mercury_alias_classes: invalid argument");
+        const merc::AliasClasses &A = ctx->s->alias;
+        *ncls = (int)A.how.size();
+        *nmembers = (long long)A.hmem.size();
+        if (!start || !axis || !members || !owners || cls_capacity < *ncls || member_capacity < *nmembers) return 0;
+        for (int c = 0; c <= *ncls; ++c) start[c] = A.hst[c];
+        for (int c = 0; c < *ncls; ++c) {
+            int p[4];
+            axis[c] = A.hsl[c];
+            ctx->s->F->point_of(A.how[c], p);
+            for (int q = 0; q < 4; ++q) owners[5 * c + q] = p[q];
+            owners[5 * c + 4] = A.hos[c];
+        }
+        for (long long t = 0; t < *nmembers; ++t) {
+            int p[4];
+            ctx->s->F->point_of(A.hmem[t], p);
+            for (int q = 0; q < 4; ++q) members[5 * t + q] = p[q];
+            members[5 * t + 4] = A.hms[t];
+        }
+    })
 }
 
 int mercury_set_stream(MercuryContext *ctx, void *stream) {

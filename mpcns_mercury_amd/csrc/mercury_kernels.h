@@ -45,5 +45,7 @@ void launch_copy_multi(hipStream_t st, const int *ent, long long n, double *cons
 void launch_fill_multi(hipStream_t st, const int *ent, long long n, double *const *F, const MultiSets &S);
 void launch_setv_multi(hipStream_t st, const int *ent, const double *val, long long n, double *const *F);
 void launch_wall_multi(hipStream_t st, const int *ent, long long n, int g, double *const *F, const MercPhys &P);
+// n entries (dst slot, dst, src slot, src, sign): dst <- sign * src
+void launch_copy_plan(hipStream_t st, const int *ent, long long n, double *const *F);
 
 }  // namespace merc

@@ -633,6 +633,14 @@ __global__ void k_setv_multi(const int *__restrict__ ent, const double *__restri
     if (d < 0) return;
     F[slot][d] = val[t];
 }
+// a caller's sync plan: entry (dst slot, dst, src slot, src, sign), dst <- sign * src
+__global__ void k_copy_plan(const int *__restrict__ ent, long long n, double *const *__restrict__ F_) {
+    gdouble *const *__restrict__ F = gslots(F_);
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int *e = ent + 5 * t;
+    F[e[0]][e[1]] = (double)e[4] * F[e[2]][e[3]];
+}
 // BC_Solid_Surface_ (Boundary_Condition_Fluid.cpp:36-260): per wall cell the
 // entry holds (species base slot, species (0 H+, 1 Na+, 2 He++), first layer,
 // one deeper, g ghosts); He++ takes the H+ rule with its own density floor
@@ -881,6 +889,9 @@ void launch_fill_multi(hipStream_t st, const int *ent, long long n, double *cons
 }
 void launch_setv_multi(hipStream_t st, const int *ent, const double *val, long long n, double *const *F) {
     if (n > 0) hipLaunchKernelGGL(k_setv_multi, dim3(nblk(n, 256)), dim3(256), 0, st, ent, val, n, F);
+}
+void launch_copy_plan(hipStream_t st, const int *ent, long long n, double *const *F) {
+    if (n > 0) hipLaunchKernelGGL(k_copy_plan, dim3(nblk(n, 256)), dim3(256), 0, st, ent, n, F);
 }
 void launch_wall_multi(hipStream_t st, const int *ent, long long n, int g, double *const *F, const MercPhys &P) {
     if (n > 0) hipLaunchKernelGGL(k_wall_multi, dim3(nblk(n, 256)), dim3(256), 0, st, ent, n, g, F, P);

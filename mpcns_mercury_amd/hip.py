@@ -26,7 +26,8 @@ ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercu
                "mercury_divb", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
                "mercury_time_op", "mercury_geometry_block", "mercury_set_form", "mercury_resist_stats",
                "mercury_ensemble_create", "mercury_ensemble_launch", "mercury_ensemble_destroy", "mercury_autotune",
-               "mercury_set_flux_shape", "mercury_flux_shape",
+               "mercury_set_flux_shape", "mercury_flux_shape", "mercury_set_sync_plan", "mercury_sync_plan",
+               "mercury_set_alias_classes", "mercury_alias_classes",
                "mercury_write_restart", "mercury_read_restart", "mercury_write_rundata", "mercury_read_rundata",
                "mercury_write_tecplot", "mercury_case_read", "mercury_case_destroy", "mercury_case_blocks",
                "mercury_case_block_name", "mercury_case_nodes", "mercury_case_topology", "mercury_case_interfaces",
@@ -98,6 +99,11 @@ def load():
     L.mercury_ensemble_destroy.argtypes = [vp]
     L.mercury_autotune.argtypes = [vp, ip]
     L.mercury_set_flux_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    L.mercury_set_sync_plan.argtypes = [vp, cp, ctypes.c_int, ctypes.c_longlong, ip]
+    L.mercury_sync_plan.argtypes = [vp, cp, ctypes.c_int, ctypes.c_longlong, llp, ip]
+    L.mercury_set_alias_classes.argtypes = [vp, ctypes.c_int, ip, ip, ip, ip]
+    L.mercury_alias_classes.argtypes = [vp, ctypes.c_int, ctypes.c_longlong, ip, llp, ip, ip, ip, ip]
     L.mercury_flux_shape.argtypes = [vp, ip, ip]
     L.mercury_geometry_block.argtypes = [ip, ctypes.c_int, dp, ctypes.POINTER(MercuryPhysics), cp, ctypes.c_int, dp]
     # DEC operators on caller device arrays: pointers travel as void *
@@ -327,6 +333,38 @@ class Mercury:
         v = ctypes.c_void_p()
         _chk(load().mercury_field_device(self._h, name.encode(), block, comp, ctypes.byref(v)))
         return v.value
+
+    def sync_plan(self, group, stage):
+        """The (n, 11) int32 entries of one stage (0 halo, 1 owner) of a sync group."""
+        n = ctypes.c_longlong()
+        _chk(load().mercury_sync_plan(self._h, group.encode(), stage, 0, ctypes.byref(n), None))
+        out = np.zeros((max(n.value, 1), 11), dtype=np.int32)
+        _chk(load().mercury_sync_plan(self._h, group.encode(), stage, n.value, ctypes.byref(n),
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return out[:n.value]
+
+    def set_sync_plan(self, group, stage, entries):
+        e = np.ascontiguousarray(entries, dtype=np.int32).reshape(-1, 11)
+        _chk(load().mercury_set_sync_plan(self._h, group.encode(), stage, e.shape[0],
+                                          e.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
+    def alias_classes(self):
+        """(start, axis, members (m, 5), owners (c, 5)) of the alias-mean classes."""
+        nc, nm = ctypes.c_int(), ctypes.c_longlong()
+        _chk(load().mercury_alias_classes(self._h, 0, 0, ctypes.byref(nc), ctypes.byref(nm), None, None, None, None))
+        ip = ctypes.POINTER(ctypes.c_int)
+        start = np.zeros(nc.value + 1, np.int32)
+        axis = np.zeros(max(nc.value, 1), np.int32)
+        mem = np.zeros((max(nm.value, 1), 5), np.int32)
+        own = np.zeros((max(nc.value, 1), 5), np.int32)
+        _chk(load().mercury_alias_classes(self._h, nc.value, nm.value, ctypes.byref(nc), ctypes.byref(nm), start.ctypes.data_as(ip),
+                                          axis.ctypes.data_as(ip), mem.ctypes.data_as(ip), own.ctypes.data_as(ip)))
+        return start, axis[:nc.value], mem[:nm.value], own[:nc.value]
+
+    def set_alias_classes(self, start, axis, members, owners):
+        ip = ctypes.POINTER(ctypes.c_int)
+        arrs = [np.ascontiguousarray(a, dtype=np.int32) for a in (start, axis, members, owners)]
+        _chk(load().mercury_set_alias_classes(self._h, len(arrs[1]), *[a.ctypes.data_as(ip) for a in arrs]))
 
     def set_flux_shape(self, kind, length):
         """Run the flux pass as the column kernel (kind 0 or False: length cells
