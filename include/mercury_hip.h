@@ -252,6 +252,19 @@ int mercury_timestep(MercuryContext *ctx, double *dt);
 /* Replaces calc_divB (MercurySolver_MagneticCell.cpp:151-241); returns
  * max|d2 B| / max|B_face| and max|d2 B| over owned cells/faces. */
 int mercury_divb(MercuryContext *ctx, double *normalised, double *maxabs);
+/* The per-step div B monitor (no reference counterpart; it reports calc_divB's
+ * figure without a pass of its own): the fused forms' B-cell pass, which reads
+ * the synchronised face fluxes of every owned cell anyway, folds max|d2 B| and
+ * max|B_face| into per-workgroup maxima, and one small kernel turns them into
+ * the normalised value of mercury_divb.  last: that value for the latest
+ * derived pass (the end of the latest step), max_since_reset: its maximum over
+ * every derived pass since the last reset (Euler: every step; SSP-RK2: every
+ * stage), last_maxabs: the latest max|d2 B|.  reset != 0 clears them after the
+ * read.  Any pointer may be NULL.  The plain form (0) does not update it.
+ * mercury_set_divb_monitor switches it on (off by default: on shell6 it adds
+ * about 12 us to the B-cell pass); switching drops a captured step graph. */
+int mercury_set_divb_monitor(MercuryContext *ctx, int on);
+int mercury_divb_monitor(MercuryContext *ctx, double *last, double *max_since_reset, double *last_maxabs, int reset);
 
 int mercury_synchronize(MercuryContext *ctx);
 

@@ -23,7 +23,11 @@ void launch_edge_total(hipStream_t st, const RangeH &R, const MercBlk *B, double
 void launch_update_all(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, const double *dt,
                        bool mix, int ns);
 void launch_to_double(hipStream_t st, const float *src, double *dst, long long n);
-void launch_bcell_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);
+// divb_part (2 doubles per workgroup of R) and divb_mon (3 doubles), when not
+// null: the div B monitor of the synchronised B faces (k_divb_monitor)
+void launch_bcell_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix,
+                        double *divb_part = nullptr, double *divb_mon = nullptr);
+void launch_divb_monitor_reset(hipStream_t st, double *divb_mon);
 void launch_jcell_t(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix);
 void launch_to_float(hipStream_t st, const double *src, float *dst, long long n);
 // per run: the abort rule on dt_next[r], dt[r] committed, status 4 words per run

@@ -922,13 +922,9 @@ __global__ void k_update_all(const MercRange R, const MercBlk *__restrict__ B, d
 
 // ---------------------------------------------------------------------------
 template <bool MIX>
-__global__ void k_bcell_jedge(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
-                              const float *const *__restrict__ G_) {
-    gdouble *const *__restrict__ F = gslots(F_);
-    const gfloat *const *__restrict__ G = gslots(G_);
+DEVI void bcell_jedge_point(gdouble *const *__restrict__ F, const gfloat *const *__restrict__ G, const MercBlk &bk, const Pt &p,
+                            bool monitor, double &dmx, double &bmx) {
     using T = typename Prec<MIX>::T;
-    LOCATE(R);
-    const MercBlk &bk = B[p.b];
     const long long q = p.q, di = OI(p), dj = OJ(p), dk = 1;
     if (p.i < bk.m[0] && p.j < bk.m[1] && p.k < bk.m[2]) {
         T pt[6], pi[6];
@@ -957,6 +953,16 @@ __global__ void k_bcell_jedge(const MercRange R, const MercBlk *__restrict__ B, 
             F[SL_BC + c][q] = (double)bt[c];
             F[SL_BI + c][q] = (double)bi[c];
         }
+        if (monitor) {
+            // calc_divB's d2 B (the expression and order of k_divb)
+            const double d = (F[SL_B][q + di] - F[SL_B][q] + F[SL_B + 1][q + dj] - F[SL_B + 1][q] + F[SL_B + 2][q + 1] - F[SL_B + 2][q]);
+            dmx = dmax(dmx, fabs(d));
+        }
+    }
+    if (monitor) {
+        if (p.j < bk.m[1] && p.k < bk.m[2]) bmx = dmax(bmx, fabs(F[SL_B][q]));
+        if (p.i < bk.m[0] && p.k < bk.m[2]) bmx = dmax(bmx, fabs(F[SL_B + 1][q]));
+        if (p.i < bk.m[0] && p.j < bk.m[1]) bmx = dmax(bmx, fabs(F[SL_B + 2][q]));
     }
 #define BB(a, o) (st<MIX>(F, G, SL_BETA + (a), q + (o)) * (T)F[SL_B + (a)][q + (o)])
     if (p.i < bk.m[0]) {
@@ -972,6 +978,94 @@ __global__ void k_bcell_jedge(const MercRange R, const MercBlk *__restrict__ B, 
         F[SL_J + 2][q] = (double)((T(1) * v) * st<MIX>(F, G, SL_ALPHA + 2, q));
     }
 #undef BB
+}
+
+// max of two values over the workgroup, written to part[2 * blockIdx.x .. + 1]
+// (every thread of the workgroup calls it)
+DEVI void group_max2(double a, double b, double *__restrict__ part) {
+    __shared__ double wa[16], wb[16];
+    for (int off = 32; off > 0; off >>= 1) {
+        a = dmax(a, __shfl_xor(a, off, 64));
+        b = dmax(b, __shfl_xor(b, off, 64));
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+    if (lane == 0) {
+        wa[wid] = a;
+        wb[wid] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < nw; ++w) {
+            a = dmax(a, wa[w]);
+            b = dmax(b, wb[w]);
+        }
+        part[2 * blockIdx.x] = a;
+        part[2 * blockIdx.x + 1] = b;
+    }
+}
+
+// the monitor's fold (one workgroup): mon[0] = the normalised div B of this
+// pass (max |d2 B| / max |B|, as mercury_divb), mon[1] = its maximum since the
+// last reset, mon[2] = max |d2 B|
+__global__ void __launch_bounds__(1024) k_divb_monitor(const double *__restrict__ part, int ngroups, double *__restrict__ mon) {
+    // eight independent loads in flight per thread: the fold is latency-bound
+    double a = 0.0, b = 0.0;
+    const int step = blockDim.x;
+    int g = threadIdx.x;
+    for (; g + 7 * step < ngroups; g += 8 * step) {
+        double x[16];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            x[2 * u] = part[2 * (g + u * step)];
+            x[2 * u + 1] = part[2 * (g + u * step) + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            a = dmax(a, x[2 * u]);
+            b = dmax(b, x[2 * u + 1]);
+        }
+    }
+    for (; g < ngroups; g += step) {
+        a = dmax(a, part[2 * g]);
+        b = dmax(b, part[2 * g + 1]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a = dmax(a, __shfl_xor(a, off, 64));
+        b = dmax(b, __shfl_xor(b, off, 64));
+    }
+    __shared__ double wa[16], wb[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        wa[wid] = a;
+        wb[wid] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            a = dmax(a, wa[w]);
+            b = dmax(b, wb[w]);
+        }
+        const double n = b > 0.0 ? a / b : a;
+        mon[0] = n;
+        mon[1] = dmax(mon[1], n);
+        mon[2] = a;
+    }
+}
+__global__ void k_divb_monitor_reset(double *mon) {
+    if (threadIdx.x == 0) mon[0] = mon[1] = mon[2] = 0.0;
+}
+
+template <bool MIX>
+__global__ void k_bcell_jedge(const MercRange R, const MercBlk *__restrict__ B, double *const *__restrict__ F_,
+                              const float *const *__restrict__ G_, double *__restrict__ divb_part) {
+    gdouble *const *__restrict__ F = gslots(F_);
+    const gfloat *const *__restrict__ G = gslots(G_);
+    using T = typename Prec<MIX>::T;
+    Pt p;
+    const bool located = locate(R, B, TID, p);
+    double dmx = 0.0, bmx = 0.0;  // the div B monitor: max |d2 B| over owned cells, max |B| over owned faces
+    if (located) bcell_jedge_point<MIX>(F, G, B[p.b], p, divb_part != nullptr, dmx, bmx);
+    if (divb_part) group_max2(dmx, bmx, divb_part);
 }
 
 // calc_Jcell (MercurySolver_Current.cpp:203-259), owned cells
@@ -1101,9 +1195,15 @@ void launch_update_all(hipStream_t st, const RangeH &R, const MercBlk *B, double
 void launch_to_double(hipStream_t st, const float *src, double *dst, long long n) {
     if (n > 0) hipLaunchKernelGGL(k_to_double, dim3(nblk(n, 256)), dim3(256), 0, st, src, dst, n);
 }
-void launch_bcell_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix) {
-    if (mix) LAUNCH_R(k_bcell_jedge<true>, R, B, F, G);
-    else LAUNCH_R(k_bcell_jedge<false>, R, B, F, G);
+void launch_bcell_jedge(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix,
+                        double *divb_part, double *divb_mon) {
+    if (mix) LAUNCH_R(k_bcell_jedge<true>, R, B, F, G, divb_part);
+    else LAUNCH_R(k_bcell_jedge<false>, R, B, F, G, divb_part);
+    if (divb_part && divb_mon && R.total > 0)
+        hipLaunchKernelGGL(k_divb_monitor, dim3(1), dim3(1024), 0, st, divb_part, (int)nblk(R.total, 256), divb_mon);
+}
+void launch_divb_monitor_reset(hipStream_t st, double *divb_mon) {
+    hipLaunchKernelGGL(k_divb_monitor_reset, dim3(1), dim3(64), 0, st, divb_mon);
 }
 void launch_jcell_t(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, const float *const *G, bool mix) {
     if (mix) LAUNCH_R(k_jcell_t<true>, R, B, F, G);

@@ -715,6 +715,11 @@ public:
     std::vector<int> islots;                  // FP32 derived intermediates of the mixed form
     unsigned long long *d_red = nullptr;      // reduction result (2 words)
     double *d_part = nullptr;                 // per-workgroup partial maxima of the div B diagnostic
+    // the div B monitor of the fused derived pass: per-workgroup maxima and
+    // (last normalised div B, its maximum since the last reset, last max |d2 B|)
+    double *d_mpart = nullptr, *d_mon = nullptr;
+    bool divb_mon_on = false;  // off by default: it costs the step about 10 us on shell6
+    double *mon_part() const { return divb_mon_on ? d_mpart : nullptr; }
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_integrator = -1;
@@ -822,6 +827,8 @@ public:
         HIPCHK(hipMemset(d_status, 0, 4 * sizeof(unsigned long long) * nruns));
         HIPCHK(hipMalloc(&d_red, 2 * sizeof(unsigned long long)));
         HIPCHK(hipMalloc(&d_part, 2 * sizeof(double) * divb_partials()));
+        HIPCHK(hipMalloc(&d_mon, 3 * sizeof(double)));
+        HIPCHK(hipMemset(d_mon, 0, 3 * sizeof(double)));
         // static geometry per block
         long long noff = 0;
         for (int b = 0; b < nb; ++b) {
@@ -877,6 +884,7 @@ public:
         make_range(*F, 0, 0, R_cell);
         make_range(*F, -g, g + 1, R_box);
         make_range(*F, 0, 1, R_node);
+        HIPCHK(hipMalloc(&d_mpart, 2 * sizeof(double) * (size_t)std::max<long long>((R_node.h.total + 255) / 256, 1)));
         make_range(*F, -1, 2, R_eface);
         if (const char *e = std::getenv("MERCURY_RHS_MARCH")) march = std::max(1, std::atoi(e));
         if (const char *e = std::getenv("MERCURY_RHS_TILE")) tile_c = std::max(0, std::atoi(e));
@@ -902,6 +910,8 @@ public:
         if (fptr) (void)hipFree(fptr);
         if (d_red) (void)hipFree(d_red);
         if (d_part) (void)hipFree(d_part);
+        if (d_mpart) (void)hipFree(d_mpart);
+        if (d_mon) (void)hipFree(d_mon);
         F.reset();
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
@@ -1477,7 +1487,7 @@ public:
     // UpdateDerivedFields_ (MercurySolver_Tools.cpp:303-321), fused passes; the
     // last derived pass of a step also reduces the next step's dt
     void derived_fused(bool final) {
-        launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix());
+        launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix(), mon_part(), d_mon);
         launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
         run_merged(MG_DERIVED_PAIR);
         launch_jcell_t(st, R_cell.h, Bd(), Fd(), Gd(), mix());
@@ -1502,7 +1512,7 @@ public:
         HIPCHK(hipStreamWaitEvent(st2, ev_fork2, 0));
         run_merged(1u << G_UCELL, st2);
         run_merged(1u << G_BFACE, st);
-        launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix());
+        launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix(), mon_part(), d_mon);
         launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
         if (alias.n == 0 && !F->connected() && !any_caller_plan()) {
             HIPCHK(hipEventRecord(ev_fork2, st));
@@ -1813,6 +1823,7 @@ public:
         else if (op == "edge_total") launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, alias.n == 0 ? 1 : 0, mix());
         else if (op == "update_all") launch_update_all(st, R_node.h, Bd(), Fd(), Gd(), dtp(), mix(), nspecies());
         else if (op == "bcell_jedge") launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix());
+        else if (op == "bcell_jedge_monitor") launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix(), d_mpart, d_mon);
         else if (op == "jcell_t") launch_jcell_t(st, R_cell.h, Bd(), Fd(), Gd(), mix());
         else if (op == "sync_all") {
             for (int gid = 0; gid < G_NGROUP; ++gid) sync(gid);
@@ -1847,6 +1858,17 @@ public:
         HIPCHK(hipGetLastError());
     }
 
+    // the monitor: last normalised div B of the fused derived pass and its
+    // maximum since the last reset (optionally reset now)
+    void divb_monitor(double *last, double *maxn, double *last_abs, bool reset) {
+        double h[3];
+        HIPCHK(hipMemcpyAsync(h, d_mon, sizeof(h), hipMemcpyDeviceToHost, st));
+        if (reset) launch_divb_monitor_reset(st, d_mon);
+        HIPCHK(hipStreamSynchronize(st));
+        if (last) *last = h[0];
+        if (maxn) *maxn = h[1];
+        if (last_abs) *last_abs = h[2];
+    }
     void divb(double *norm, double *maxabs) {
         launch_divb(st, R_node.h, Bd(), Fd(), d_part, d_red, 1);
         unsigned long long h[2];
@@ -2608,6 +2630,22 @@ int mercury_timestep(MercuryContext *ctx, double *dt) {
         const double v = ctx->s->read_dt();
         if (dt) *dt = v;
     })
+}
+
+int mercury_set_divb_monitor(MercuryContext *ctx, int on) {
+    CHECK_CTX(ctx);
+    ABI_TRY({
+        HIPCHK(hipStreamSynchronize(ctx->s->st));
+        if (ctx->s->divb_mon_on != (on != 0)) {
+            ctx->s->divb_mon_on = on != 0;
+            ctx->s->drop_graph();
+        }
+    })
+}
+
+int mercury_divb_monitor(MercuryContext *ctx, double *last, double *max_since_reset, double *last_maxabs, int reset) {
+    CHECK_CTX(ctx);
+    ABI_TRY({ ctx->s->divb_monitor(last, max_since_reset, last_maxabs, reset != 0); })
 }
 
 int mercury_divb(MercuryContext *ctx, double *normalised, double *maxabs) {

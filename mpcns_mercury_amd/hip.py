@@ -23,7 +23,7 @@ ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercu
                "mercury_field_info",
                "mercury_block_volume", "mercury_block_shape", "mercury_field_device", "mercury_stream", "mercury_set_stream", "mercury_total_volume", "mercury_upload", "mercury_download",
                "mercury_init_bface", "mercury_init_bface_dipoles", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
-               "mercury_divb", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
+               "mercury_divb", "mercury_set_divb_monitor", "mercury_divb_monitor", "mercury_synchronize", "mercury_graph_capture", "mercury_graph_launch",
                "mercury_time_op", "mercury_geometry_block", "mercury_set_form", "mercury_resist_stats",
                "mercury_ensemble_create", "mercury_ensemble_launch", "mercury_ensemble_destroy", "mercury_autotune",
                "mercury_set_flux_shape", "mercury_flux_shape", "mercury_set_sync_plan", "mercury_sync_plan",
@@ -99,6 +99,8 @@ def load():
     L.mercury_ensemble_destroy.argtypes = [vp]
     L.mercury_autotune.argtypes = [vp, ip]
     L.mercury_set_flux_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.mercury_divb_monitor.argtypes = [vp, dp, dp, dp, ctypes.c_int]
+    L.mercury_set_divb_monitor.argtypes = [vp, ctypes.c_int]
     llp = ctypes.POINTER(ctypes.c_longlong)
     L.mercury_set_sync_plan.argtypes = [vp, cp, ctypes.c_int, ctypes.c_longlong, ip]
     L.mercury_sync_plan.argtypes = [vp, cp, ctypes.c_int, ctypes.c_longlong, llp, ip]
@@ -266,6 +268,17 @@ class Mercury:
         a, b = ctypes.c_double(), ctypes.c_double()
         _chk(load().mercury_divb(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def set_divb_monitor(self, on=True):
+        """Switch the per-step div B monitor of the fused forms on or off."""
+        _chk(load().mercury_set_divb_monitor(self._h, 1 if on else 0))
+
+    def divb_monitor(self, reset=False):
+        """(last normalised div B of the derived pass, its max since the last reset,
+        last max |d2 B|), from the fused forms' B-cell pass; reset clears them."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _chk(load().mercury_divb_monitor(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), 1 if reset else 0))
+        return a.value, b.value, c.value
 
     def synchronize(self):
         _chk(load().mercury_synchronize(self._h))
