@@ -72,6 +72,12 @@ class Case:
     meta: dict = field(default_factory=dict)
     # intrinsic dipoles of Badd: (n, 7) nondimensional x0 y0 z0 mx my mz a, or None
     dipoles: Optional[np.ndarray] = None
+    # patch connections (n, 18) int32: block, dir, node box lo[3] hi[3], neighbour
+    # block, perm[3], sign[3], offset[3]; None: per-face connections
+    patches: Optional[np.ndarray] = None
+    # staggered-DOF map of patch connections: 0 the reference's node-index map,
+    # 1 positions matched
+    patch_rule: int = 0
 
     def dipole_array(self) -> np.ndarray:
         d = np.zeros((0, 7)) if self.dipoles is None else np.asarray(self.dipoles, dtype=np.float64).reshape(-1, 7)
@@ -534,3 +540,109 @@ def make_resistive_shell(scale: float = 1.0, g: int = 4, steps: int = 500, integ
     cuts = ((dims[0],), (dims[1] // 2, dims[1] - dims[1] // 2), (dims[2] // 2, dims[2] - dims[2] // 2))
     return split_block_case(base, cuts, name=base.name + "_2x2")
 
+
+
+# ---------------------------------------------------------------------------
+# patch connections: rotated, mirrored and partial-face interfaces
+# ---------------------------------------------------------------------------
+
+PATCH_INTS = 18
+
+
+def _patch_row(block, dirc, lo, hi, nb, perm, sign, off):
+    return [int(block), int(dirc), *map(int, lo), *map(int, hi), int(nb), *map(int, perm), *map(int, sign), *map(int, off)]
+
+
+def patches_of_case(case: Case) -> np.ndarray:
+    """The per-face connections of a case (periodic faces and 1:1 translation
+    interfaces) as whole-face patches, in block and face order."""
+    rows = []
+    for b, blk in enumerate(case.blocks):
+        for f in range(6):
+            a, side = f // 2, f & 1
+            if blk.bc[f] == BC_PERIODIC:
+                nb, off = b, [0, 0, 0]
+                off[a] = -blk.dims[a] if side else blk.dims[a]
+            elif blk.bc[f] == BC_INTERFACE:
+                nb, off = blk.nbr[f], list(blk.off[f])
+            else:
+                continue
+            lo, hi = [0, 0, 0], [d + 1 for d in blk.dims]
+            lo[a] = blk.dims[a] if side else 0
+            hi[a] = lo[a] + 1
+            rows.append(_patch_row(b, (a + 1) * (1 if side else -1), lo, hi, nb, (0, 1, 2), (1, 1, 1), off))
+    return np.array(rows, dtype=np.int32).reshape(-1, PATCH_INTS)
+
+
+def patched_case(case: Case, rule: int = 0) -> Case:
+    """The same case with its connections given as patches."""
+    blocks = [Block(dims=b.dims, bc=tuple(BC_INTERFACE if c in (BC_PERIODIC, BC_INTERFACE) else c for c in b.bc), nodes=b.nodes)
+              for b in case.blocks]
+    return Case(name=case.name + "_patched", g=case.g, blocks=blocks, phys=case.phys, fields=case.fields, A=case.A, Bu=case.Bu,
+                steps=case.steps, integrator=case.integrator, meta=dict(case.meta), dipoles=case.dipoles,
+                patches=patches_of_case(case), patch_rule=rule)
+
+
+def patches_of_geometry(geo) -> np.ndarray:
+    """The connection patches of a CASE geometry (``hip.CaseGeometry``), every
+    rank's, in the reader's order."""
+    rows = [_patch_row(p["block"], p["direction"], p["lo"], p["hi"], p["nb_block"], p["perm"], p["sign"], p["offset"])
+            for p in geo.interfaces()]
+    return np.array(rows, dtype=np.int32).reshape(-1, PATCH_INTS)
+
+
+def cartesian_periodic_cube(n: int = 64, g: int = 4, seed: int = 1, amp: float = 1e-3, steps: int = 10, hall: bool = False) -> Case:
+    """The C1 state (seeded velocities, B = (1,0,0) + curl of a seeded edge
+    potential) on the uniform Cartesian unit cube of n^3 cells, periodic."""
+    base = make_periodic_cube((n, n, n), g=g, seed=seed, amp=amp, steps=steps, hall=hall)
+    blk = base.blocks[0]
+    blk.nodes = cartesian_nodes((n, n, n), (1.0 / n,) * 3)
+    base.name = f"cartesian_cube_{n}"
+    return base
+
+
+def _lattice_map(nodes: np.ndarray, n: int):
+    """Affine map of a Cartesian block on the n^3 lattice: global node = g0 + R @ local node."""
+    X = np.rint(nodes * n).astype(np.int64)
+    g0 = X[:, 0, 0, 0]
+    R = np.stack([X[:, 1, 0, 0] - g0, X[:, 0, 1, 0] - g0, X[:, 0, 0, 1] - g0], axis=1)
+    if sorted(np.abs(R).sum(axis=0).tolist()) != [1, 1, 1] or sorted(np.abs(R).sum(axis=1).tolist()) != [1, 1, 1]:
+        raise ValueError("block axes are not lattice directions")
+    return g0, R
+
+
+def cube_on_geometry(geo, cube: Case, rule: int = 0, name: str | None = None) -> Case:
+    """The state of a one-block periodic Cartesian cube (``cartesian_periodic_cube``)
+    cut along a CASE geometry of Cartesian blocks on the same lattice (the
+    reference's sample geometries): nodes and patches from the geometry; every
+    cell value and every edge-potential value taken from the cube at the same
+    physical cell or edge (edge values signed by the block's axis direction),
+    ghosts included, so the patched run starts from the cube's state."""
+    n, g = cube.blocks[0].dims[0], cube.g
+    blocks, fields, A = [], [], []
+    for b in range(geo.nb):
+        dims = tuple(geo.dims[b])
+        nodes = geo.nodes[b]
+        g0, R = _lattice_map(nodes, n)
+        I, J, K = _box_index(dims, g)
+        P = np.stack([I, J, K]).reshape(3, -1)
+        # a cell's global index is its lowest corner: local + (R < 0 along that axis)
+        cell = g0[:, None] + R @ P + np.minimum(R, 0).sum(axis=1)[:, None]
+        ci = np.mod(cell, n) + g
+        fl = {}
+        for key, arr in cube.fields[0].items():
+            fl[key] = np.ascontiguousarray(arr[:, ci[0], ci[1], ci[2]].reshape((arr.shape[0],) + I.shape))
+        fields.append(fl)
+        Ab = np.zeros((3,) + I.shape)
+        for a in range(3):
+            col = R[:, a]
+            c = int(np.nonzero(col)[0][0])
+            s = int(col[c])
+            start = g0[:, None] + R @ P + (np.minimum(col, 0))[:, None]  # lower end of the edge along physical axis c
+            wi = np.mod(start, n) + g
+            Ab[a] = s * cube.A[0][c][wi[0], wi[1], wi[2]].reshape(I.shape)
+        A.append(Ab)
+        blocks.append(Block(dims=dims, bc=(BC_INTERFACE,) * 6, nodes=np.ascontiguousarray(nodes)))
+    return Case(name=name or f"{cube.name}_on_{geo.nb}_blocks", g=g, blocks=blocks, phys=cube.phys, fields=fields, A=A, Bu=cube.Bu,
+                steps=cube.steps, integrator=cube.integrator, meta=dict(cube.meta), patches=patches_of_geometry(geo),
+                patch_rule=rule)

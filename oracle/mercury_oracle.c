@@ -132,6 +132,23 @@ typedef struct {
     long *cls_start[8];
     int *cls_b[8];
     long *cls_q[8];
+    /* each member's axis (the field of its location within a face or edge
+     * triplet) and orientation sign; with patch connections a class may hold
+     * members of different axes, and all classes of a dimension are listed
+     * under its first location (L_FX, L_EX) */
+    int *cls_ax[8];
+    signed char *cls_sg[8];
+    /* patch connections (0 when the faces carry per-face connections): 18 ints
+     * per patch (block, dir, node box lo[3] hi[3], neighbour block, perm[3],
+     * sign[3], offset[3]) and the halo regions of each level built from them */
+    int npatch;
+    int *patch;
+    int rule; /* 0: the reference's node-index map, 1: DOF positions matched */
+    int nreg[4];
+    int *reg[4];
+    /* during a patch sync: per field axis and block, the box points the halo
+     * regions wrote (rule 1 completes the others) */
+    char *wm[3][64];
 } Orc;
 
 #define IX(b, i, j, k) ((((long)(i) + (b)->g) * (b)->N[1] + ((j) + (b)->g)) * (b)->N[2] + ((k) + (b)->g))
@@ -902,16 +919,472 @@ static void halo_stage(Orc *o, int ib, const GField *gf, int stage) {
             }
 }
 
-/* owner -> alias (Halo::sync_owner_alias_group, HaloSyncFramework.cpp:695-712):
- * every member of a class of shared DOFs takes the owner's value; orientation
- * signs are +1 for translations. */
+/* owner -> alias (Halo::sync_owner_alias_group, HaloSyncFramework.cpp:695-712,
+ * pattern built at :501-560): every alias of a class of shared DOFs whose axis
+ * is the field's takes sign(owner) * sign(alias) times the owner's value, read
+ * from the triplet field of the owner's axis.  Classes of one dimension may sit
+ * under any of its locations, so all three are scanned. */
 static void owner_sync(Orc *o, const GField *gf) {
-    const int L = gf->loc;
-    PAR1
-    for (int c = 0; c < o->ncls[L]; ++c) {
-        const long s0 = o->cls_start[L][c], s1 = o->cls_start[L][c + 1];
-        const double v = o->b[o->cls_b[L][s0]].f[gf->slot][o->cls_q[L][s0]];
-        for (long t = s0 + 1; t < s1; ++t) o->b[o->cls_b[L][t]].f[gf->slot][o->cls_q[L][t]] = v;
+    const int Lb = gf->loc >= L_EX && gf->loc <= L_EZ ? L_EX : (gf->loc >= L_FX && gf->loc <= L_FZ ? L_FX : gf->loc);
+    const int a = gf->loc - Lb, base = gf->slot - a, nL = Lb == L_NODE || Lb == L_CELL ? 1 : 3;
+    for (int L = Lb; L < Lb + nL; ++L)
+        for (int c = 0; c < o->ncls[L]; ++c) {
+            const long s0 = o->cls_start[L][c], s1 = o->cls_start[L][c + 1];
+            const double v = o->b[o->cls_b[L][s0]].f[base + o->cls_ax[L][s0]][o->cls_q[L][s0]];
+            for (long t = s0 + 1; t < s1; ++t) {
+                if (o->cls_ax[L][t] != a) continue;
+                const int sg = o->cls_sg[L][s0] * o->cls_sg[L][t];
+                o->b[o->cls_b[L][t]].f[gf->slot][o->cls_q[L][t]] = sg * v;
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------- */
+/* patch connections: rotated, mirrored and partial-face block interfaces     */
+/* ------------------------------------------------------------------------- */
+/* A patch joins a node box on one block face to a node box of a neighbour face
+ * by an IndexTransform (include/2_topology/TopologyTypes.h:93-98): neighbour
+ * node [perm[a]] = sign[a] * node[a] + offset[a].  Record of 18 ints. */
+enum { PT_BLOCK = 0, PT_DIR = 1, PT_LO = 2, PT_HI = 5, PT_NB = 8, PT_PERM = 9, PT_SIGN = 12, PT_OFF = 15, PT_N = 18 };
+/* A halo region of one level: receiving and sending block, node box on the
+ * receiver, the out-of-range directions of its ghost box, the transform, and
+ * whether the source is found by the inverse map (face level: the transform
+ * runs sender -> receiver) or the forward map (edge and vertex levels:
+ * receiver -> sender).  Record of 22 ints. */
+enum { RG_RECV = 0, RG_SEND = 1, RG_LO = 2, RG_HI = 5, RG_ND = 8, RG_DIR = 9, RG_PERM = 12, RG_SIGN = 15, RG_OFF = 18, RG_INV = 21, RG_N = 22 };
+
+/* node index map of a transform (HALO_TOOLS::apply_transform, src/4_halo/build/HaloBuildTools.cpp:296-310) */
+static void tr_map(const int *perm, const int *sign, const int *off, const int p[3], int out[3]) {
+    for (int a = 0; a < 3; ++a) out[perm[a]] = sign[a] * p[a] + off[a];
+}
+/* TOPO::transform_node_box (src/2_topology/TopologyOps.cpp:91-117) */
+static void tr_box(const int *perm, const int *sign, const int *off, const int lo[3], const int hi[3], int olo[3], int ohi[3]) {
+    const int h[3] = {hi[0] - 1, hi[1] - 1, hi[2] - 1};
+    int p0[3], p1[3];
+    tr_map(perm, sign, off, lo, p0);
+    tr_map(perm, sign, off, h, p1);
+    for (int a = 0; a < 3; ++a) {
+        olo[a] = p0[a] < p1[a] ? p0[a] : p1[a];
+        ohi[a] = (p0[a] < p1[a] ? p1[a] : p0[a]) + 1;
+    }
+}
+/* block faces a node box lies on (boundary_dir_count, src/4_halo/build/HaloBuildEdge.cpp:9-25) */
+static int bdir_count(const int lo[3], const int hi[3], const int m[3]) {
+    int c = 0;
+    for (int a = 0; a < 3; ++a) c += (lo[a] == 0 && hi[a] == 1) + (lo[a] == m[a] && hi[a] == m[a] + 1);
+    return c;
+}
+/* HALO_TOOLS::detect_face_direction (HaloBuildTools.cpp:12-37); 0 if the box is on no face */
+static int face_dir_of(const int lo[3], const int hi[3], const int m[3]) {
+    for (int a = 0; a < 3; ++a) {
+        if (lo[a] == 0 && hi[a] == 1) return -(a + 1);
+        if (lo[a] == m[a] && hi[a] == m[a] + 1) return a + 1;
+    }
+    return 0;
+}
+static int inv_perm_axis(const int *perm, int d) {
+    for (int a = 0; a < 3; ++a)
+        if (perm[a] == d) return a;
+    return d;
+}
+static int perm_parity(const int *perm) {
+    int inv = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = a + 1; b < 3; ++b) inv += perm[a] > perm[b];
+    return inv & 1 ? -1 : 1;
+}
+
+static void push_region(Orc *o, int lvl, int recv, int send, const int lo[3], const int hi[3], int nd, const int *dirs,
+                        const int *perm, const int *sign, const int *off, int inv) {
+    o->reg[lvl] = (int *)realloc(o->reg[lvl], sizeof(int) * RG_N * (o->nreg[lvl] + 1));
+    int *r = o->reg[lvl] + RG_N * o->nreg[lvl]++;
+    r[RG_RECV] = recv;
+    r[RG_SEND] = send;
+    for (int a = 0; a < 3; ++a) {
+        r[RG_LO + a] = lo[a];
+        r[RG_HI + a] = hi[a];
+        r[RG_DIR + a] = a < nd ? dirs[a] : 0;
+        r[RG_PERM + a] = perm[a];
+        r[RG_SIGN + a] = sign[a];
+        r[RG_OFF + a] = off[a];
+    }
+    r[RG_ND] = nd;
+    r[RG_INV] = inv;
+}
+
+/* a face patch or a physical face of one block, and an edge patch (two faces
+ * of one block), as the reference's topology lists them */
+typedef struct { int inner, dir, lo[3], hi[3], nb, perm[3], sign[3], off[3]; } OFacePatch;
+typedef struct { int inner, block, lo[3], hi[3], dir1, dir2, nb, perm[3], sign[3], off[3]; } OEdgePatch;
+
+/* Halo regions of the three levels:
+ * level 1 from every patch (Halo::build_inner_1DCorner_pattern,
+ *   src/4_halo/build/HaloBuildFace.cpp:6-57): the receiver is the neighbour,
+ *   its ghost strip outside the neighbour's face box;
+ * level 2 from the edge patches (TOPO::build_edge_patches,
+ *   src/2_topology/TopologyEdge.cpp:45-211): every pair of faces of one block
+ *   (patches in list order, then physical faces) on different axes whose node
+ *   boxes meet; the inner face owns the edge over a physical one, else the
+ *   first of the pair; an inner edge becomes a region when it lies on two faces
+ *   of both blocks (build_inner_2DCorner_pattern, HaloBuildEdge.cpp:188-265);
+ * level 3 from the vertex patches (TOPO::build_vertex_patches,
+ *   TopologyEdge.cpp:213-405; build_inner_3DCorner_pattern,
+ *   src/4_halo/build/HaloBuildVertex.cpp:190-275): pairs of edges of one block
+ *   on different axes meeting in one node, the first pair per node, owned by
+ *   the first inner edge among all edges touching the node. */
+static void build_patch_regions(Orc *o) {
+    for (int l = 0; l < 4; ++l) {
+        free(o->reg[l]);
+        o->reg[l] = NULL;
+        o->nreg[l] = 0;
+    }
+    for (int p = 0; p < o->npatch; ++p) {
+        const int *P = o->patch + PT_N * p;
+        const OBlock *nb = &o->b[P[PT_NB]];
+        int lo[3], hi[3];
+        tr_box(P + PT_PERM, P + PT_SIGN, P + PT_OFF, P + PT_LO, P + PT_HI, lo, hi);
+        const int d = face_dir_of(lo, hi, nb->m);
+        push_region(o, 1, P[PT_NB], P[PT_BLOCK], lo, hi, 1, &d, P + PT_PERM, P + PT_SIGN, P + PT_OFF, 1);
+    }
+    OEdgePatch *ie = NULL, *pe = NULL;
+    int nie = 0, npe = 0;
+    for (int ib = 0; ib < o->nb; ++ib) {
+        const OBlock *b = &o->b[ib];
+        OFacePatch *fl = (OFacePatch *)calloc(o->npatch + 6, sizeof(OFacePatch));
+        int nf = 0;
+        for (int p = 0; p < o->npatch; ++p) {
+            const int *P = o->patch + PT_N * p;
+            if (P[PT_BLOCK] != ib) continue;
+            OFacePatch *f = &fl[nf++];
+            f->inner = 1;
+            f->dir = P[PT_DIR];
+            f->nb = P[PT_NB];
+            for (int a = 0; a < 3; ++a) {
+                f->lo[a] = P[PT_LO + a];
+                f->hi[a] = P[PT_HI + a];
+                f->perm[a] = P[PT_PERM + a];
+                f->sign[a] = P[PT_SIGN + a];
+                f->off[a] = P[PT_OFF + a];
+            }
+        }
+        for (int fc = 0; fc < 6; ++fc) {
+            if (b->bc[fc] != BC_FARFIELD && b->bc[fc] != BC_WALL) continue;
+            OFacePatch *f = &fl[nf++];
+            const int a = fc / 2;
+            f->inner = 0;
+            f->dir = (fc & 1) ? a + 1 : -(a + 1);
+            f->nb = -1;
+            for (int c = 0; c < 3; ++c) {
+                f->lo[c] = 0;
+                f->hi[c] = b->m[c] + 1;
+            }
+            f->lo[a] = (fc & 1) ? b->m[a] : 0;
+            f->hi[a] = f->lo[a] + 1;
+        }
+        for (int i = 0; i < nf; ++i)
+            for (int j = i + 1; j < nf; ++j) {
+                const OFacePatch *f1 = &fl[i], *f2 = &fl[j];
+                if (abs(f1->dir) == abs(f2->dir)) continue;
+                OEdgePatch e;
+                int empty = 0;
+                for (int a = 0; a < 3; ++a) {
+                    e.lo[a] = f1->lo[a] > f2->lo[a] ? f1->lo[a] : f2->lo[a];
+                    e.hi[a] = f1->hi[a] < f2->hi[a] ? f1->hi[a] : f2->hi[a];
+                    empty |= e.hi[a] <= e.lo[a];
+                }
+                if (empty) continue;
+                const OFacePatch *own = f2->inner > f1->inner ? f2 : f1;
+                e.inner = own->inner;
+                e.block = ib;
+                e.nb = own->nb;
+                for (int a = 0; a < 3; ++a) {
+                    e.perm[a] = own->perm[a];
+                    e.sign[a] = own->sign[a];
+                    e.off[a] = own->off[a];
+                }
+                e.dir1 = own == f1 ? f1->dir : f2->dir;
+                e.dir2 = own == f1 ? f2->dir : f1->dir;
+                if (e.inner) {
+                    ie = (OEdgePatch *)realloc(ie, sizeof(OEdgePatch) * (nie + 1));
+                    ie[nie++] = e;
+                } else {
+                    pe = (OEdgePatch *)realloc(pe, sizeof(OEdgePatch) * (npe + 1));
+                    pe[npe++] = e;
+                }
+            }
+        free(fl);
+    }
+    for (int t = 0; t < nie; ++t) {
+        const OEdgePatch *e = &ie[t];
+        int lo[3], hi[3];
+        tr_box(e->perm, e->sign, e->off, e->lo, e->hi, lo, hi);
+        if (bdir_count(e->lo, e->hi, o->b[e->block].m) != 2 || bdir_count(lo, hi, o->b[e->nb].m) != 2) continue;
+        const int dirs[2] = {e->dir1, e->dir2};
+        push_region(o, 2, e->block, e->nb, e->lo, e->hi, 2, dirs, e->perm, e->sign, e->off, 0);
+    }
+    for (int ib = 0; ib < o->nb; ++ib) {
+        const OEdgePatch **el = (const OEdgePatch **)calloc(nie + npe + 1, sizeof(OEdgePatch *));
+        int ne = 0;
+        for (int t = 0; t < nie; ++t)
+            if (ie[t].block == ib) el[ne++] = &ie[t];
+        for (int t = 0; t < npe; ++t)
+            if (pe[t].block == ib) el[ne++] = &pe[t];
+        int *used = (int *)malloc(sizeof(int) * 3 * (ne * ne + 1));
+        int nused = 0;
+        for (int i = 0; i < ne; ++i)
+            for (int j = i + 1; j < ne; ++j) {
+                const OEdgePatch *e1 = el[i], *e2 = el[j];
+                if (6 - abs(e1->dir1) - abs(e1->dir2) == 6 - abs(e2->dir1) - abs(e2->dir2)) continue;
+                int lo[3], hi[3], unit = 1;
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = e1->lo[a] > e2->lo[a] ? e1->lo[a] : e2->lo[a];
+                    hi[a] = e1->hi[a] < e2->hi[a] ? e1->hi[a] : e2->hi[a];
+                    unit &= hi[a] - lo[a] == 1;
+                }
+                if (!unit) continue;
+                int dup = 0;
+                for (int u = 0; u < nused && !dup; ++u) dup = used[3 * u] == lo[0] && used[3 * u + 1] == lo[1] && used[3 * u + 2] == lo[2];
+                if (dup) continue;
+                for (int a = 0; a < 3; ++a) used[3 * nused + a] = lo[a];
+                ++nused;
+                /* candidates: e1, e2, then every other edge touching the node */
+                const OEdgePatch **cand = (const OEdgePatch **)calloc(ne + 1, sizeof(OEdgePatch *));
+                int nc = 0;
+                cand[nc++] = e1;
+                cand[nc++] = e2;
+                for (int k = 0; k < ne; ++k) {
+                    if (k == i || k == j) continue;
+                    int meet = 1;
+                    for (int a = 0; a < 3; ++a) meet &= el[k]->lo[a] <= lo[a] && lo[a] < el[k]->hi[a];
+                    if (meet) cand[nc++] = el[k];
+                }
+                const OEdgePatch *own = cand[0];
+                for (int k = 1; k < nc; ++k)
+                    if (cand[k]->inner > own->inner) own = cand[k];
+                int dba[4] = {0, 0, 0, 0};
+                for (int k = 0; k < nc; ++k) {
+                    const int d[2] = {cand[k]->dir1, cand[k]->dir2};
+                    for (int q = 0; q < 2; ++q)
+                        if (!dba[abs(d[q])]) dba[abs(d[q])] = d[q];
+                }
+                free(cand);
+                if (!own->inner) continue;
+                int nlo[3], nhi[3];
+                tr_box(own->perm, own->sign, own->off, lo, hi, nlo, nhi);
+                if (bdir_count(lo, hi, o->b[ib].m) != 3 || bdir_count(nlo, nhi, o->b[own->nb].m) != 3) continue;
+                push_region(o, 3, ib, own->nb, lo, hi, 3, dba + 1, own->perm, own->sign, own->off, 0);
+            }
+        free(used);
+        free(el);
+    }
+    free(ie);
+    free(pe);
+}
+
+/* ghost box of a region at location L: the DOF box of its node box, moved
+ * outside along each of its directions (LAYOUT::node_box_to_dof_box and
+ * apply_ghost_strip, src/0_basic/LayoutTraits.cpp:104-118, 296-303) */
+static void region_box(const int *r, int L, int G, int lo[3], int hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = r[RG_LO + a];
+        hi[a] = r[RG_HI + a] - DELTA[L][a];
+    }
+    for (int t = 0; t < r[RG_ND]; ++t) {
+        const int d = r[RG_DIR + t], ax = abs(d) - 1;
+        if (d < 0) {
+            lo[ax] = -G;
+            hi[ax] = 0;
+        } else {
+            lo[ax] = hi[ax];
+            hi[ax] = lo[ax] + G;
+        }
+    }
+}
+/* source index of receiver point p (location Lr) of a region.  Rule 0 maps the
+ * DOF's indices as node indices, as the reference does for every location
+ * (HaloExchangeInner.cpp:35-52, HaloTriplet.cpp:53-84); rule 1 maps its
+ * position: along a reversed axis where the DOF sits between nodes, the
+ * mirrored DOF is one index lower. */
+static void region_src(int rule, const int *r, int Lr, const int p[3], int s[3]) {
+    const int *perm = r + RG_PERM, *sign = r + RG_SIGN, *off = r + RG_OFF;
+    if (r[RG_INV]) {
+        for (int a = 0; a < 3; ++a) {
+            s[a] = sign[a] * (p[perm[a]] - off[a]);
+            if (rule && sign[a] < 0) s[a] -= DELTA[Lr][perm[a]];
+        }
+    } else {
+        for (int a = 0; a < 3; ++a) {
+            s[perm[a]] = sign[a] * p[a] + off[a];
+            if (rule && sign[a] < 0) s[perm[a]] -= DELTA[Lr][a];
+        }
+    }
+}
+static int in_alloc(const OBlock *b, int L, const int s[3]) {
+    int lo[3], hi[3];
+    alloc_box(b, L, b->g, lo, hi);
+    for (int a = 0; a < 3; ++a)
+        if (s[a] < lo[a] || s[a] >= hi[a]) return 0;
+    return 1;
+}
+
+/* component copy of one level (Halo::exchange_inner / _edge / _vertex,
+ * src/4_halo/component_copy/HaloExchangeInner.cpp:184-247): every region's
+ * values are read first, then written in region order */
+static void patch_exchange_component(Orc *o, const GField *gf, int lvl) {
+    long n = 0;
+    for (int t = 0; t < o->nreg[lvl]; ++t) {
+        int lo[3], hi[3];
+        region_box(o->reg[lvl] + RG_N * t, gf->loc, o->b[0].g, lo, hi);
+        n += (long)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+    }
+    int *db = (int *)malloc(sizeof(int) * (n + 1));
+    long *dq = (long *)malloc(sizeof(long) * (n + 1));
+    double *v = (double *)malloc(sizeof(double) * gf->ncomp * (n + 1));
+    long u = 0;
+    for (int t = 0; t < o->nreg[lvl]; ++t) {
+        const int *r = o->reg[lvl] + RG_N * t;
+        OBlock *rb = &o->b[r[RG_RECV]], *sb = &o->b[r[RG_SEND]];
+        int lo[3], hi[3], p[3], s[3];
+        region_box(r, gf->loc, rb->g, lo, hi);
+        for (p[0] = lo[0]; p[0] < hi[0]; ++p[0])
+            for (p[1] = lo[1]; p[1] < hi[1]; ++p[1])
+                for (p[2] = lo[2]; p[2] < hi[2]; ++p[2]) {
+                    region_src(o->rule, r, gf->loc, p, s);
+                    if (!in_alloc(sb, gf->loc, s)) continue;
+                    db[u] = r[RG_RECV];
+                    dq[u] = IX(rb, p[0], p[1], p[2]);
+                    for (int m = 0; m < gf->ncomp; ++m) v[gf->ncomp * u + m] = F(sb, gf->slot + m, s[0], s[1], s[2]);
+                    ++u;
+                }
+    }
+    for (long t = 0; t < u; ++t) {
+        for (int m = 0; m < gf->ncomp; ++m) o->b[db[t]].f[gf->slot + m][dq[t]] = v[gf->ncomp * t + m];
+        if (o->wm[0][db[t]]) o->wm[0][db[t]][dq[t]] = 1;
+    }
+    free(db);
+    free(dq);
+    free(v);
+}
+
+/* face 2-form or edge 1-form triplet of one level, written as it is read
+ * (Halo::exchange_inner_{face,edge,vertex}_{face_2form,edge_1form}_triplet_,
+ * src/4_halo/forms/HaloTriplet.cpp:355-489, 641-829): the receiver's axis-d
+ * field takes factor * the sender's field of the corresponding axis.
+ * Face level (transform sender -> receiver): source axis a with perm[a] = d;
+ * factor sign[a] for 1-forms and sign[0]*sign[1]*sign[2]*sign[a] for 2-forms
+ * (face_2form_orientation_sign, HaloBuildTools.cpp:312-327).  Edge and vertex
+ * levels (receiver -> sender): 1-forms read axis perm[d] with factor sign[d];
+ * 2-forms read axis a with perm[a] = d with the same 2-form factor.  Rule 1
+ * instead carries the area vector: factor parity(perm) * sign[0]*sign[1]*
+ * sign[2] * sign of the axis mapped, and 2-forms at the corner levels read axis
+ * perm[d] (sign[d]). */
+static void patch_exchange_triplet(Orc *o, int slot0, int face, int lvl) {
+    const int Lb = face ? L_FX : L_EX;
+    for (int d = 0; d < 3; ++d)
+        for (int t = 0; t < o->nreg[lvl]; ++t) {
+            const int *r = o->reg[lvl] + RG_N * t, *perm = r + RG_PERM, *sign = r + RG_SIGN;
+            const int det = sign[0] * sign[1] * sign[2], par = o->rule ? perm_parity(perm) : 1;
+            int sa, fac;
+            if (lvl == 1) {
+                sa = inv_perm_axis(perm, d);
+                fac = face ? par * det * sign[sa] : sign[sa];
+            } else if (!face) {
+                sa = perm[d];
+                fac = sign[d];
+            } else if (o->rule) {
+                sa = perm[d];
+                fac = par * det * sign[d];
+            } else {
+                sa = inv_perm_axis(perm, d);
+                fac = det * sign[sa];
+            }
+            OBlock *rb = &o->b[r[RG_RECV]], *sb = &o->b[r[RG_SEND]];
+            int lo[3], hi[3], p[3], s[3];
+            region_box(r, Lb + d, rb->g, lo, hi);
+            for (p[0] = lo[0]; p[0] < hi[0]; ++p[0])
+                for (p[1] = lo[1]; p[1] < hi[1]; ++p[1])
+                    for (p[2] = lo[2]; p[2] < hi[2]; ++p[2]) {
+                        region_src(o->rule, r, Lb + d, p, s);
+                        if (!in_alloc(sb, Lb + sa, s)) continue;
+                        F(rb, slot0 + d, p[0], p[1], p[2]) = fac * F(sb, slot0 + sa, s[0], s[1], s[2]);
+                        if (o->wm[d][r[RG_RECV]]) o->wm[d][r[RG_RECV]][IX(rb, p[0], p[1], p[2])] = 1;
+                    }
+        }
+}
+
+/* Rule 1 only (no reference counterpart): a ghost DOF that no region of its
+ * level wrote takes the owned DOF at its position, found by a chain of patch
+ * maps: across the block face of its first out-of-range axis, through the
+ * first patch of that face whose node box holds the DOF's span along the
+ * in-range tangential axes and touches the face's edge along the out-of-range
+ * ones, mapped by position (region_src), until the point is owned (at most 8
+ * hops).  Triplet fields carry the axis and the factor of every hop.  Returns 0
+ * where no chain ends in an owned DOF (a physical face on the way). */
+static int chain_source(const Orc *o, int Lb, int trip, int ib, int d0, const int p0[3], int *ob, int q[3], int *od, int *ofac) {
+    int b = ib, d = d0, fac = 1, p[3] = {p0[0], p0[1], p0[2]};
+    for (int hop = 0; hop < 8; ++hop) {
+        const int L = trip ? Lb + d : Lb;
+        const OBlock *bk = &o->b[b];
+        int hi[3], a = -1;
+        owned_hi(bk, L, hi);
+        for (int t = 0; t < 3 && a < 0; ++t)
+            if (p[t] < 0 || p[t] >= hi[t]) a = t;
+        if (a < 0) {
+            if (hop == 0) return 0;
+            *ob = b, *od = d, *ofac = fac;
+            for (int t = 0; t < 3; ++t) q[t] = p[t];
+            return 1;
+        }
+        const int dir = p[a] < 0 ? -(a + 1) : a + 1;
+        const int *P = NULL;
+        for (int pp = 0; pp < o->npatch && !P; ++pp) {
+            const int *C = o->patch + PT_N * pp;
+            if (C[PT_BLOCK] != b || C[PT_DIR] != dir) continue;
+            int ok = 1;
+            for (int t = 0; t < 3 && ok; ++t) {
+                if (t == a) continue;
+                if (p[t] >= 0 && p[t] < hi[t]) ok = C[PT_LO + t] <= p[t] && p[t] + DELTA[L][t] <= C[PT_HI + t] - 1;
+                else if (p[t] < 0) ok = C[PT_LO + t] == 0;
+                else ok = C[PT_HI + t] - 1 == bk->m[t];
+            }
+            if (ok) P = C;
+        }
+        if (!P) return 0;
+        const int *perm = P + PT_PERM, *sign = P + PT_SIGN, *off = P + PT_OFF;
+        int np[3];
+        for (int t = 0; t < 3; ++t) np[perm[t]] = sign[t] * p[t] + off[t] - (sign[t] < 0 ? DELTA[L][t] : 0);
+        if (trip) {
+            fac *= Lb == L_FX ? perm_parity(perm) * sign[0] * sign[1] * sign[2] * sign[d] : sign[d];
+            d = perm[d];
+        }
+        b = P[PT_NB];
+        for (int t = 0; t < 3; ++t) p[t] = np[t];
+    }
+    return 0;
+}
+static void patch_complete(Orc *o, const GField *gf, int nf, int trip, int stage) {
+    const int Lb = trip ? gf[0].loc : L_CELL;
+    const int nax = trip ? 3 : nf;
+    for (int x = 0; x < nax; ++x) {
+        const GField *f = trip ? &gf[0] : &gf[x];
+        const int d = trip ? x : 0, L = trip ? Lb + x : f->loc, slot = trip ? gf[0].slot + x : f->slot, nc = trip ? 1 : f->ncomp;
+        for (int ib = 0; ib < o->nb; ++ib) {
+            OBlock *b = &o->b[ib];
+            int lo[3], hi[3], ohi[3], p[3];
+            alloc_box(b, L, b->g, lo, hi);
+            owned_hi(b, L, ohi);
+            for (p[0] = lo[0]; p[0] < hi[0]; ++p[0])
+                for (p[1] = lo[1]; p[1] < hi[1]; ++p[1])
+                    for (p[2] = lo[2]; p[2] < hi[2]; ++p[2]) {
+                        int nout = 0;
+                        for (int t = 0; t < 3; ++t) nout += p[t] < 0 || p[t] >= ohi[t];
+                        if (nout != stage || o->wm[trip ? d : 0][ib][IX(b, p[0], p[1], p[2])]) continue;
+                        int sb, q[3], sd, fac;
+                        if (!chain_source(o, Lb, trip, ib, d, p, &sb, q, &sd, &fac)) continue;
+                        for (int m = 0; m < nc; ++m)
+                            F(b, slot + m, p[0], p[1], p[2]) = fac * F(&o->b[sb], (trip ? gf[0].slot + sd : slot) + m, q[0], q[1], q[2]);
+                    }
+        }
     }
 }
 
@@ -937,27 +1410,54 @@ static void sync_group(Orc *o, int gid) {
         }
     for (int n = 0; n < nf; ++n)
         for (int ib = 0; ib < o->nb; ++ib) corner_default(&o->b[ib], &gf[n]);
-    for (int stage = 1; stage <= 3; ++stage)
-        for (int n = 0; n < nf; ++n) {
-            if (stage > gf[n].level) continue;
-            for (int ib = 0; ib < o->nb; ++ib) halo_stage(o, ib, &gf[n], stage);
+    if (o->npatch > 0) {
+        /* Halo::sync_group(name, stage) (HaloSyncFramework.cpp:1004-1035): the
+         * group's triplet first, then its component-copy fields */
+        const int trip = gf[0].loc != L_CELL;
+        for (int d = 0; d < 3; ++d)
+            for (int ib = 0; ib < o->nb; ++ib) o->wm[d][ib] = (char *)calloc(o->b[ib].V, 1);
+        /* rule 1 completes each level before the next one reads it */
+        if (trip) {
+            for (int stage = 1; stage <= gf[0].level; ++stage) {
+                patch_exchange_triplet(o, gf[0].slot, gf[0].loc == L_FX, stage);
+                if (o->rule) patch_complete(o, gf, nf, 1, stage);
+            }
+        } else {
+            /* the fields are independent: stage by stage per field */
+            for (int n = 0; n < nf; ++n) {
+                for (int ib = 0; ib < o->nb; ++ib) memset(o->wm[0][ib], 0, o->b[ib].V);
+                for (int stage = 1; stage <= gf[n].level; ++stage) {
+                    patch_exchange_component(o, &gf[n], stage);
+                    if (o->rule) patch_complete(o, &gf[n], 1, 0, stage);
+                }
+            }
         }
+        for (int d = 0; d < 3; ++d)
+            for (int ib = 0; ib < o->nb; ++ib) {
+                free(o->wm[d][ib]);
+                o->wm[d][ib] = NULL;
+            }
+    } else {
+        for (int stage = 1; stage <= 3; ++stage)
+            for (int n = 0; n < nf; ++n) {
+                if (stage > gf[n].level) continue;
+                for (int ib = 0; ib < o->nb; ++ib) halo_stage(o, ib, &gf[n], stage);
+            }
+    }
     for (int n = 0; n < nf; ++n)
         if (gf[n].owner_sync) owner_sync(o, &gf[n]);
 }
 
 /* ReduceEdgeAliasCandidatesToOwners_ (MercurySolver_Current.cpp:3-57): the owner
- * of each shared edge class takes the signed mean of all members, summed in
- * ascending EntityKey order. */
+ * of each shared edge class takes its orientation sign times the mean of
+ * sign * value over all members, summed in ascending EntityKey order. */
 static void alias_reduce_edges(Orc *o, int slot0) {
-    for (int a = 0; a < 3; ++a) {
-        const int L = L_EX + a;
-        PAR1
+    for (int L = L_EX; L <= L_EZ; ++L) {
         for (int c = 0; c < o->ncls[L]; ++c) {
             const long s0 = o->cls_start[L][c], s1 = o->cls_start[L][c + 1];
             double sum = 0.0;
-            for (long t = s0; t < s1; ++t) sum += 1 * o->b[o->cls_b[L][t]].f[slot0 + a][o->cls_q[L][t]];
-            o->b[o->cls_b[L][s0]].f[slot0 + a][o->cls_q[L][s0]] = 1 * sum / (double)(s1 - s0);
+            for (long t = s0; t < s1; ++t) sum += o->cls_sg[L][t] * o->b[o->cls_b[L][t]].f[slot0 + o->cls_ax[L][t]][o->cls_q[L][t]];
+            o->b[o->cls_b[L][s0]].f[slot0 + o->cls_ax[L][s0]][o->cls_q[L][s0]] = o->cls_sg[L][s0] * sum / (double)(s1 - s0);
         }
     }
 }
@@ -988,6 +1488,8 @@ static void build_classes(Orc *o) {
         o->cls_start[L] = (long *)calloc(1, sizeof(long));
         o->cls_b[L] = NULL;
         o->cls_q[L] = NULL;
+        o->cls_ax[L] = NULL;
+        o->cls_sg[L] = NULL;
     }
     if (!any) return;
     long *base = (long *)calloc(o->nb + 1, sizeof(long));
@@ -1052,6 +1554,13 @@ static void build_classes(Orc *o) {
         o->cls_start[L] = st;
         o->cls_b[L] = cb;
         o->cls_q[L] = cq;
+        o->cls_ax[L] = (int *)malloc(sizeof(int) * (nmem ? nmem : 1));
+        o->cls_sg[L] = (signed char *)malloc(nmem ? nmem : 1);
+        const int ax = L >= L_EX && L <= L_EZ ? L - L_EX : (L >= L_FX && L <= L_FZ ? L - L_FX : 0);
+        for (long u = 0; u < nmem; ++u) {
+            o->cls_ax[L][u] = ax;
+            o->cls_sg[L][u] = 1;
+        }
         free(key);
     }
     free(hit);
@@ -1935,11 +2444,10 @@ static void resist_dofs(Orc *o) {
         alias[a] = (char **)calloc(o->nb, sizeof(char *));
         for (int ib = 0; ib < o->nb; ++ib) alias[a][ib] = (char *)calloc(o->b[ib].V, 1);
     }
-    for (int a = 0; a < 3; ++a) {
-        const int L = L_EX + a;
+    for (int L = L_EX; L <= L_EZ; ++L)
         for (int c = 0; c < o->ncls[L]; ++c)
-            for (long t = o->cls_start[L][c] + 1; t < o->cls_start[L][c + 1]; ++t) alias[a][o->cls_b[L][t]][o->cls_q[L][t]] = 1;
-    }
+            for (long t = o->cls_start[L][c] + 1; t < o->cls_start[L][c + 1]; ++t)
+                alias[o->cls_ax[L][t]][o->cls_b[L][t]][o->cls_q[L][t]] = 1;
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
         for (int a = 0; a < 3; ++a) {
@@ -2178,6 +2686,208 @@ static int stage(Orc *o, double dt, int resist) {
  * neighbour over its whole extent, declared from both sides, with at least
  * as many cells as ghost layers along the connection.  Returns NULL on
  * anything else. */
+/* ------------------------------------------------------------------------- */
+/* patch connections: shared-DOF classes and the context                      */
+/* ------------------------------------------------------------------------- */
+/* Equivalence classes through patches (TOPO::detail::build_equivalence,
+ * src/2_topology/Topology.cpp:280-323): nodes joined by every patch's node map
+ * form classes whose representative is the smallest EntityKey (block, i, j, k)
+ * (TopologyNode.cpp:180-235).  An edge's key is its two endpoint
+ * representatives in ascending order, its sign +1 when they come in that order
+ * (TOPO::make_edge_key, TopologyQuery.cpp:104-126); a face's key is its four
+ * sorted corner representatives (make_face_key, :128-184) and its sign
+ * compares its boundary-edge stencil with the owner's: +1 equal, -1 opposite
+ * (orient_face_members_to_owner, TopologyFaceEquiv.cpp:298-351; boundary of a
+ * face, include/2_topology/LocalIncidence.h:44-79).  Owner = smallest member
+ * (block, i, j, k, axis); edge owners keep their own sign, face owners +1
+ * (TopologyEdgeEquiv.cpp:176-262, TopologyFaceEquiv.cpp:353-461).  Only
+ * classes of two or more members are kept: the others take part in no copy. */
+typedef struct { long k0, k1, k2, k3; int b, i, j, k, ax, sg; long st[4][3]; } PEnt; /* st: (edge key lo, hi, coefficient) */
+static int pent_cmp(const void *x, const void *y) {
+    const PEnt *a = (const PEnt *)x, *b = (const PEnt *)y;
+    const long ka[8] = {a->k0, a->k1, a->k2, a->k3, a->b, a->i, a->j, a->k};
+    const long kb[8] = {b->k0, b->k1, b->k2, b->k3, b->b, b->i, b->j, b->k};
+    for (int t = 0; t < 8; ++t)
+        if (ka[t] != kb[t]) return ka[t] < kb[t] ? -1 : 1;
+    return (a->ax > b->ax) - (a->ax < b->ax);
+}
+static int st_cmp(const void *x, const void *y) {
+    const long *a = (const long *)x, *b = (const long *)y;
+    if (a[0] != b[0]) return a[0] < b[0] ? -1 : 1;
+    return (a[1] > b[1]) - (a[1] < b[1]);
+}
+static int build_classes_patched(Orc *o) {
+    for (int L = 0; L < 8; ++L) {
+        o->ncls[L] = 0;
+        o->cls_start[L] = (long *)calloc(1, sizeof(long));
+        o->cls_b[L] = NULL;
+        o->cls_q[L] = NULL;
+        o->cls_ax[L] = NULL;
+        o->cls_sg[L] = NULL;
+    }
+    long *nbase = (long *)calloc(o->nb + 1, sizeof(long));
+    for (int ib = 0; ib < o->nb; ++ib) nbase[ib + 1] = nbase[ib] + (long)(o->b[ib].m[0] + 1) * (o->b[ib].m[1] + 1) * (o->b[ib].m[2] + 1);
+#define NODE(ib, i, j, k) (nbase[ib] + (((long)(i) * (o->b[ib].m[1] + 1) + (j)) * (o->b[ib].m[2] + 1) + (k)))
+    const long nn = nbase[o->nb];
+    long *par = (long *)malloc(sizeof(long) * nn);
+    long *cnt = (long *)calloc(nn, sizeof(long));
+    for (long x = 0; x < nn; ++x) par[x] = x;
+    for (int p = 0; p < o->npatch; ++p) {
+        const int *P = o->patch + PT_N * p;
+        int q[3], s[3];
+        for (q[0] = P[PT_LO]; q[0] < P[PT_HI]; ++q[0])
+            for (q[1] = P[PT_LO + 1]; q[1] < P[PT_HI + 1]; ++q[1])
+                for (q[2] = P[PT_LO + 2]; q[2] < P[PT_HI + 2]; ++q[2]) {
+                    tr_map(P + PT_PERM, P + PT_SIGN, P + PT_OFF, q, s);
+                    const long rx = uf_find(par, NODE(P[PT_BLOCK], q[0], q[1], q[2])), ry = uf_find(par, NODE(P[PT_NB], s[0], s[1], s[2]));
+                    if (rx != ry) par[rx < ry ? ry : rx] = rx < ry ? rx : ry;
+                }
+    }
+    for (long x = 0; x < nn; ++x) cnt[uf_find(par, x)] += 1;
+    for (int dim = 0; dim < 2; ++dim) { /* 0: edges, 1: faces */
+        PEnt *e = NULL;
+        long ne = 0, cap = 0;
+        for (int ib = 0; ib < o->nb; ++ib) {
+            const OBlock *b = &o->b[ib];
+            for (int ax = 0; ax < 3; ++ax) {
+                const int L = (dim ? L_FX : L_EX) + ax;
+                int hi[3];
+                owned_hi(b, L, hi);
+                for (int i = 0; i < hi[0]; ++i)
+                    for (int j = 0; j < hi[1]; ++j)
+                        for (int k = 0; k < hi[2]; ++k) {
+                            PEnt x;
+                            memset(&x, 0, sizeof x);
+                            x.b = ib, x.i = i, x.j = j, x.k = k, x.ax = ax, x.sg = 1;
+                            if (!dim) {
+                                const long r0 = uf_find(par, NODE(ib, i, j, k));
+                                if (cnt[r0] <= 1) continue;
+                                const long r1 = uf_find(par, NODE(ib, i + (ax == 0), j + (ax == 1), k + (ax == 2)));
+                                if (cnt[r1] <= 1 || r0 == r1) continue;
+                                x.k0 = r0 < r1 ? r0 : r1;
+                                x.k1 = r0 < r1 ? r1 : r0;
+                                x.sg = r0 < r1 ? 1 : -1;
+                            } else {
+                                /* corners n0..n3 and the four boundary edges (start, axis, incidence) */
+                                const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+                                long c[4];
+                                int ok = 1;
+                                for (int n = 0; n < 4 && ok; ++n) {
+                                    int q[3] = {i, j, k};
+                                    q[t1] += n & 1;
+                                    q[t2] += n >> 1;
+                                    c[n] = uf_find(par, NODE(ib, q[0], q[1], q[2]));
+                                    ok = cnt[c[n]] > 1;
+                                }
+                                if (!ok) continue;
+                                long srt[4] = {c[0], c[1], c[2], c[3]};
+                                for (int u = 1; u < 4; ++u)
+                                    for (int w = u; w > 0 && srt[w] < srt[w - 1]; --w) {
+                                        const long z = srt[w];
+                                        srt[w] = srt[w - 1];
+                                        srt[w - 1] = z;
+                                    }
+                                x.k0 = srt[0], x.k1 = srt[1], x.k2 = srt[2], x.k3 = srt[3];
+                                /* boundary_of_face: (+ t1-edge at start, + t2-edge at start+e_t1, - t1-edge at start+e_t2, - t2-edge at start) for
+                                 * Xi (t1 eta, t2 zeta) and Zeta (t1 xi, t2 eta); Eta lists zeta first: (+ zeta at start, + xi at +e_zeta,
+                                 * - zeta at +e_xi, - xi at start) */
+                                int ea[4], es[4][3], ei[4];
+                                const int u1 = ax == 1 ? 2 : t1, u2 = ax == 1 ? 0 : t2;
+                                for (int n = 0; n < 4; ++n)
+                                    for (int a = 0; a < 3; ++a) es[n][a] = (a == 0 ? i : (a == 1 ? j : k));
+                                ea[0] = u1, ei[0] = 1;
+                                ea[1] = u2, ei[1] = 1, es[1][u1] += 1;
+                                ea[2] = u1, ei[2] = -1, es[2][u2] += 1;
+                                ea[3] = u2, ei[3] = -1;
+                                long st[4][3];
+                                int ns = 0;
+                                for (int n = 0; n < 4; ++n) {
+                                    const long a0 = uf_find(par, NODE(ib, es[n][0], es[n][1], es[n][2]));
+                                    const long a1 = uf_find(par, NODE(ib, es[n][0] + (ea[n] == 0), es[n][1] + (ea[n] == 1), es[n][2] + (ea[n] == 2)));
+                                    const long lo = a0 < a1 ? a0 : a1, hi2 = a0 < a1 ? a1 : a0;
+                                    const long coef = ei[n] * (a0 < a1 ? 1 : -1);
+                                    int hit = -1;
+                                    for (int w = 0; w < ns; ++w)
+                                        if (st[w][0] == lo && st[w][1] == hi2) hit = w;
+                                    if (hit < 0) {
+                                        st[ns][0] = lo, st[ns][1] = hi2, st[ns][2] = coef;
+                                        ++ns;
+                                    } else {
+                                        st[hit][2] += coef;
+                                    }
+                                }
+                                int w2 = 0;
+                                for (int w = 0; w < ns; ++w)
+                                    if (st[w][2] != 0) {
+                                        for (int z = 0; z < 3; ++z) st[w2][z] = st[w][z];
+                                        ++w2;
+                                    }
+                                qsort(st, w2, sizeof st[0], st_cmp);
+                                for (int w = 0; w < 4; ++w)
+                                    for (int z = 0; z < 3; ++z) x.st[w][z] = w < w2 ? st[w][z] : 0;
+                            }
+                            if (ne == cap) {
+                                cap = cap ? 2 * cap : 1024;
+                                e = (PEnt *)realloc(e, sizeof(PEnt) * cap);
+                            }
+                            e[ne++] = x;
+                        }
+            }
+        }
+        qsort(e, ne, sizeof(PEnt), pent_cmp);
+        const int L = dim ? L_FX : L_EX;
+        long *st = (long *)malloc(sizeof(long) * (ne + 1));
+        int *cb = (int *)malloc(sizeof(int) * (ne + 1));
+        long *cq = (long *)malloc(sizeof(long) * (ne + 1));
+        int *ca = (int *)malloc(sizeof(int) * (ne + 1));
+        signed char *cs = (signed char *)malloc(ne + 1);
+        int nc = 0;
+        long nm = 0;
+        for (long u = 0; u < ne;) {
+            long v = u + 1;
+            while (v < ne && e[v].k0 == e[u].k0 && e[v].k1 == e[u].k1 && e[v].k2 == e[u].k2 && e[v].k3 == e[u].k3) ++v;
+            if (v - u >= 2) {
+                st[nc++] = nm;
+                for (long w = u; w < v; ++w) {
+                    int sg = e[w].sg;
+                    if (dim) {
+                        int same = 1, opp = 1;
+                        for (int z = 0; z < 4; ++z) {
+                            same &= e[w].st[z][0] == e[u].st[z][0] && e[w].st[z][1] == e[u].st[z][1] && e[w].st[z][2] == e[u].st[z][2];
+                            opp &= e[w].st[z][0] == e[u].st[z][0] && e[w].st[z][1] == e[u].st[z][1] && e[w].st[z][2] == -e[u].st[z][2];
+                        }
+                        if (!same && !opp) {
+                            free(e), free(st), free(cb), free(cq), free(ca), free(cs), free(par), free(cnt), free(nbase);
+                            return 0;
+                        }
+                        sg = same ? 1 : -1;
+                    }
+                    cb[nm] = e[w].b;
+                    cq[nm] = IX(&o->b[e[w].b], e[w].i, e[w].j, e[w].k);
+                    ca[nm] = e[w].ax;
+                    cs[nm] = (signed char)sg;
+                    ++nm;
+                }
+            }
+            u = v;
+        }
+        st[nc] = nm;
+        free(o->cls_start[L]);
+        o->ncls[L] = nc;
+        o->cls_start[L] = st;
+        o->cls_b[L] = cb;
+        o->cls_q[L] = cq;
+        o->cls_ax[L] = ca;
+        o->cls_sg[L] = cs;
+        free(e);
+    }
+#undef NODE
+    free(par);
+    free(cnt);
+    free(nbase);
+    return 1;
+}
+
 void *orc_create_conn(int nb, const int *dims, int g, const int *bc, const int *nbr, const int *trans, const double *nodes,
                       const OrcPhys *P);
 void *orc_create(int nb, const int *dims, int g, const int *bc, const double *nodes, const OrcPhys *P) {
@@ -2268,9 +2978,102 @@ void orc_destroy(void *h) {
         free(o->cls_start[L]);
         free(o->cls_b[L]);
         free(o->cls_q[L]);
+        free(o->cls_ax[L]);
+        free(o->cls_sg[L]);
     }
+    free(o->patch);
+    for (int l = 0; l < 4; ++l) free(o->reg[l]);
     free(o->b);
     free(o);
+}
+
+
+/* A context whose block faces connect through patches (18 ints each, see
+ * PT_*): every face with code BC_INTERFACE must be covered by the node boxes of
+ * its patches; periodicity is a patch from a block to itself.  rule 0 maps
+ * staggered DOFs as the reference does, rule 1 by position (region_src).
+ * NULL when the patches are not valid. */
+static int patches_valid(int nb, const int *dims, int g, const int *bc, int npatch, const int *pt) {
+    (void)g;
+    if (nb <= 0 || nb > 64 || npatch < 0 || (npatch > 0 && !pt)) return 0;
+    for (int ib = 0; ib < nb; ++ib)
+        for (int f = 0; f < 6; ++f)
+            if (bc[6 * ib + f] < BC_FARFIELD || bc[6 * ib + f] > BC_INTERFACE) return 0;
+    for (int p = 0; p < npatch; ++p) {
+        const int *P = pt + PT_N * p;
+        const int b = P[PT_BLOCK], d = P[PT_DIR], n = P[PT_NB];
+        if (b < 0 || b >= nb || n < 0 || n >= nb || d == 0 || abs(d) > 3) return 0;
+        const int *m = dims + 3 * b, *mn = dims + 3 * n;
+        int seen[3] = {0, 0, 0};
+        for (int a = 0; a < 3; ++a) {
+            const int q = P[PT_PERM + a];
+            if (q < 0 || q > 2 || seen[q]++) return 0;
+            if (P[PT_SIGN + a] != 1 && P[PT_SIGN + a] != -1) return 0;
+            if (P[PT_LO + a] < 0 || P[PT_HI + a] <= P[PT_LO + a] || P[PT_HI + a] > m[a] + 1) return 0;
+        }
+        if (face_dir_of(P + PT_LO, P + PT_HI, m) != d) return 0;
+        if (bc[6 * b + 2 * (abs(d) - 1) + (d > 0)] != BC_INTERFACE) return 0;
+        int lo[3], hi[3];
+        tr_box(P + PT_PERM, P + PT_SIGN, P + PT_OFF, P + PT_LO, P + PT_HI, lo, hi);
+        for (int a = 0; a < 3; ++a)
+            if (lo[a] < 0 || hi[a] > mn[a] + 1) return 0;
+        const int dn = face_dir_of(lo, hi, mn);
+        if (dn == 0 || bc[6 * n + 2 * (abs(dn) - 1) + (dn > 0)] != BC_INTERFACE) return 0;
+    }
+    /* every node of an interface face lies in a patch of that face */
+    for (int ib = 0; ib < nb; ++ib)
+        for (int f = 0; f < 6; ++f) {
+            if (bc[6 * ib + f] != BC_INTERFACE) continue;
+            const int a = f / 2, d = (f & 1) ? a + 1 : -(a + 1), *m = dims + 3 * ib;
+            const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+            for (int u = 0; u <= m[t1]; ++u)
+                for (int w = 0; w <= m[t2]; ++w) {
+                    int q[3];
+                    q[a] = (f & 1) ? m[a] : 0;
+                    q[t1] = u;
+                    q[t2] = w;
+                    int cov = 0;
+                    for (int p = 0; p < npatch && !cov; ++p) {
+                        const int *P = pt + PT_N * p;
+                        if (P[PT_BLOCK] != ib || P[PT_DIR] != d) continue;
+                        cov = 1;
+                        for (int c = 0; c < 3; ++c) cov &= P[PT_LO + c] <= q[c] && q[c] < P[PT_HI + c];
+                    }
+                    if (!cov) return 0;
+                }
+        }
+    return 1;
+}
+void *orc_create_patched(int nb, const int *dims, int g, const int *bc, int npatch, const int *pt, int rule, const double *nodes,
+                         const OrcPhys *P) {
+    if (!patches_valid(nb, dims, g, bc, npatch, pt) || (rule != 0 && rule != 1)) return NULL;
+    /* faces carry no per-face connection: the context is built without one and
+     * the patch halo and classes replace the translation ones */
+    int *bc0 = (int *)malloc(sizeof(int) * 6 * nb);
+    for (int t = 0; t < 6 * nb; ++t) bc0[t] = bc[t] == BC_INTERFACE ? BC_FARFIELD : bc[t];
+    Orc *o = (Orc *)orc_create_conn(nb, dims, g, bc0, NULL, NULL, nodes, P);
+    free(bc0);
+    if (!o) return NULL;
+    for (int ib = 0; ib < nb; ++ib)
+        for (int f = 0; f < 6; ++f) o->b[ib].bc[f] = bc[6 * ib + f];
+    o->npatch = npatch;
+    o->rule = rule;
+    o->patch = (int *)malloc(sizeof(int) * PT_N * (npatch ? npatch : 1));
+    memcpy(o->patch, pt, sizeof(int) * PT_N * npatch);
+    build_patch_regions(o);
+    for (int L = 0; L < 8; ++L) {
+        free(o->cls_start[L]);
+        free(o->cls_b[L]);
+        free(o->cls_q[L]);
+        free(o->cls_ax[L]);
+        free(o->cls_sg[L]);
+    }
+    if (!build_classes_patched(o)) {
+        orc_destroy(o);
+        return NULL;
+    }
+    resist_dofs(o);
+    return o;
 }
 
 int orc_nslot(void) { return S_NSLOT; }

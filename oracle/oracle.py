@@ -42,6 +42,9 @@ def lib():
         L.orc_create.argtypes = [ctypes.c_int, ip, ctypes.c_int, ip, dp, ctypes.POINTER(MercuryPhysics)]
         L.orc_create_conn.restype = vp
         L.orc_create_conn.argtypes = [ctypes.c_int, ip, ctypes.c_int, ip, ip, ip, dp, ctypes.POINTER(MercuryPhysics)]
+        L.orc_create_patched.restype = vp
+        L.orc_create_patched.argtypes = [ctypes.c_int, ip, ctypes.c_int, ip, ctypes.c_int, ip, ctypes.c_int, dp,
+                                         ctypes.POINTER(MercuryPhysics)]
         L.orc_destroy.argtypes = [vp]
         L.orc_slot.argtypes = [ctypes.c_char_p]
         L.orc_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
@@ -80,9 +83,14 @@ class Oracle:
         bc = case.bc_array()
         nodes = case.nodes_flat()
         ip = ctypes.POINTER(ctypes.c_int)
-        nbr, trans = case.nbr_array(), case.trans_array()
-        self.h = L.orc_create_conn(case.nb, dims.ctypes.data_as(ip), case.g, bc.ctypes.data_as(ip), nbr.ctypes.data_as(ip),
-                                   trans.ctypes.data_as(ip), _dp(nodes), ctypes.byref(self.phys))
+        if getattr(case, "patches", None) is not None:
+            pt = np.ascontiguousarray(case.patches, dtype=np.int32).reshape(-1)
+            self.h = L.orc_create_patched(case.nb, dims.ctypes.data_as(ip), case.g, bc.ctypes.data_as(ip), len(pt) // 18,
+                                          pt.ctypes.data_as(ip), int(case.patch_rule), _dp(nodes), ctypes.byref(self.phys))
+        else:
+            nbr, trans = case.nbr_array(), case.trans_array()
+            self.h = L.orc_create_conn(case.nb, dims.ctypes.data_as(ip), case.g, bc.ctypes.data_as(ip), nbr.ctypes.data_as(ip),
+                                       trans.ctypes.data_as(ip), _dp(nodes), ctypes.byref(self.phys))
         if not self.h:
             raise ValueError("oracle: unsupported block connections")
         for ib, fl in enumerate(case.fields):

@@ -1,0 +1,250 @@
+"""The oracle's patch connections (rotated, mirrored and partial-face block
+interfaces) on the CPU.
+
+Pins:
+* translations written as patches give the per-face path's result bit for bit
+  (syncs and steps, both staggered-DOF rules);
+* on the reference's own sample geometries (Cartesian blocks of the unit cube
+  with every axis permutation and reversal, faces split into several patches,
+  tests/golden/z0_null_case_geometry.tar.gz) with the position rule (1): after
+  a sync every ghost DOF of every block up to the group's level, and every
+  shared DOF after the owner -> alias pass, holds the code of the global cell,
+  face or edge at its physical position, face and edge values signed by the
+  block axis's direction (closed form: the lattice map of each block);
+* the position rule on those geometries takes a step exactly like the
+  one-block periodic cube of the same lattice (to round-off): every sync of the
+  step (state, faces, edges, cell fields, edge alias means) goes through the
+  patches.  Later steps part from the cube by design: the state is synced at
+  the face level only (Corner1D, Z4_Mercury/include/0_MercuryFieldCatalog.h:
+  77-82), so corner ghost cells keep older values, and U_plus over the whole
+  box (MercurySolver_Tools.cpp:215-301) reads them at block edges that are
+  interior points of the one-block cube;
+* the reference's node-index rule (0) differs from it there (its staggered DOFs
+  read one index off along reversed axes), and it leaves no face-strip ghost
+  unwritten (the reference's NaN-overwrite check, Z0_NULL/test/halo_comm/
+  Z0_HaloCommTest.cpp:701-760).
+"""
+import sys
+import tarfile
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+from mpcns_mercury_amd import cases, hip  # noqa: E402
+import oracle as orc  # noqa: E402
+
+GOLDEN = Path(__file__).parent / "golden" / "z0_null_case_geometry.tar.gz"
+
+
+@pytest.fixture(scope="module")
+def samples(tmp_path_factory):
+    d = tmp_path_factory.mktemp("geometry")
+    with tarfile.open(GOLDEN) as t:
+        t.extractall(d, members=[m for m in t.getmembers() if "Zone.Identifier" not in m.name])
+    return d
+
+
+def _geo(samples, which):
+    return hip.CaseGeometry(samples / which)
+
+
+def _all_fields(o, case, names):
+    return {(n, b): o.get(n, b) for n in names for b in range(case.nb)}
+
+
+STATE = ["U_H", "U_Na", "B_xi", "B_eta", "B_zeta", "E_xi", "E_eta", "E_zeta", "J_xi", "J_eta", "J_zeta", "B_cell"]
+
+
+@pytest.mark.parametrize("rule", [0, 1])
+def test_translation_patches_equal_per_face_connections(rule):
+    c = cases.make_split_cube((12, 10, 8), (2, 2, 1), g=2, hall=True, steps=3)
+    a, b = orc.Oracle(c), orc.Oracle(cases.patched_case(c, rule))
+    a.prepare()
+    b.prepare()
+    for _ in range(3):
+        a.step(1)
+        b.step(1)
+    fa, fb = _all_fields(a, c, STATE), _all_fields(b, c, STATE)
+    for k in fa:
+        np.testing.assert_array_equal(fa[k], fb[k], err_msg=str(k))
+
+
+# --- closed-form codes on the lattice ------------------------------------
+
+DELTA = {"cell": (1, 1, 1), 0: (0, 1, 1), 1: (1, 0, 1), 2: (1, 1, 0), 3: (1, 0, 0), 4: (0, 1, 0), 5: (0, 0, 1)}
+
+
+def _codes(nodes, n, dims, g, kind, a=0):
+    """Expected value of every box point of a block: the global code of the
+    DOF at its physical position (kind 'cell', 'face' or 'edge' along local
+    axis a), signed for faces and edges by the direction of local axis a."""
+    g0, R = cases._lattice_map(nodes, n)
+    I, J, K = cases._box_index(dims, g)
+    P = np.stack([I, J, K]).reshape(3, -1)
+    base = g0[:, None] + R @ P
+    mins = np.minimum(R, 0)
+    if kind == "cell":
+        gc = base + mins.sum(axis=1)[:, None]
+        c, s = 0, 1
+    else:
+        col = R[:, a]
+        c = int(np.nonzero(col)[0][0])
+        s = int(col[c])
+        if kind == "face":  # cell-like along the two other local axes
+            gc = base + (mins.sum(axis=1) - mins[:, a])[:, None]
+        else:  # edge: cell-like along its own axis
+            gc = base + mins[:, a][:, None]
+    w = np.mod(gc, n)
+    code = 1.0 + ((c * n + w[0]) * n + w[1]) * n + w[2]
+    return (s * code).reshape(I.shape)
+
+
+def _alloc_mask(dims, g, delta):
+    """Points of the box [-g, m+g] inside a location's allocated box [-g, m+1-delta+g)."""
+    I, J, K = cases._box_index(dims, g)
+    ok = np.ones(I.shape, dtype=bool)
+    for X, m, d in zip((I, J, K), dims, delta):
+        ok &= X < m + 1 - d + g
+    return ok
+
+
+def _owned_mask(dims, g, delta):
+    I, J, K = cases._box_index(dims, g)
+    ok = np.ones(I.shape, dtype=bool)
+    for X, m, d in zip((I, J, K), dims, delta):
+        ok &= (X >= 0) & (X < m + 1 - d)
+    return ok
+
+
+GROUPS = {
+    "sync_ucell": [("U_H", "cell", None)],
+    "sync_bface": [("B_xi", "face", 0), ("B_eta", "face", 1), ("B_zeta", "face", 2)],
+    "sync_eedge": [("E_xi", "edge", 0), ("E_eta", "edge", 1), ("E_zeta", "edge", 2)],
+    "sync_bcell": [("B_cell", "cell", None)],
+}
+
+
+def _run_codes(samples, which, rule, op):
+    geo = _geo(samples, which)
+    try:
+        n = 64
+        cube = cases.cartesian_periodic_cube(n, g=4)
+        case = cases.cube_on_geometry(geo, cube, rule=rule)
+        o = orc.Oracle(case)
+        want, got, owned, alloc = {}, {}, {}, {}
+        for name, kind, a in GROUPS[op]:
+            nc = orc._NCOMP.get(name, 1)
+            delta = DELTA["cell"] if kind == "cell" else DELTA[a if kind == "face" else 3 + a]
+            for b in range(case.nb):
+                dims = case.blocks[b].dims
+                exp = _codes(case.blocks[b].nodes, n, dims, case.g, kind, a if a is not None else 0)
+                own = _owned_mask(dims, case.g, delta)
+                arr = np.where(own, exp, np.nan)
+                o.set(name, b, np.broadcast_to(arr, (nc,) + arr.shape).copy())
+                want[name, b], owned[name, b], alloc[name, b] = exp, own, _alloc_mask(dims, case.g, delta)
+        o.op(op)
+        for name, _, _ in GROUPS[op]:
+            for b in range(case.nb):
+                got[name, b] = o.get(name, b)[0]
+        return case, want, got, owned, alloc
+    finally:
+        geo.close()
+
+
+@pytest.mark.parametrize("which", ["geometry_3", "geometry_6"])
+@pytest.mark.parametrize("op", list(GROUPS))
+def test_position_rule_fills_every_ghost_with_its_physical_dof(samples, which, op):
+    case, want, got, owned, alloc = _run_codes(samples, which, 1, op)
+    level = 1 if op == "sync_ucell" else 3
+    for (name, b), w in want.items():
+        kind, a = [(x[1], x[2]) for x in GROUPS[op] if x[0] == name][0]
+        delta = DELTA["cell"] if kind == "cell" else DELTA[a if kind == "face" else 3 + a]
+        dims = case.blocks[b].dims
+        I, J, K = cases._box_index(dims, case.g)
+        nout = sum(((X < 0) | (X >= m + 1 - d)).astype(int) for X, m, d in zip((I, J, K), dims, delta))
+        # every DOF up to the group's level (owned, halo ghosts, owner -> alias) is the DOF at its position
+        m = alloc[name, b] & (nout <= level)
+        np.testing.assert_array_equal(got[name, b][m], w[m], err_msg=str((name, b)))
+        # and the levels above it are left alone
+        assert np.isnan(got[name, b][alloc[name, b] & (nout > level)]).all()
+
+
+def _strip_mask(dims, g, delta):
+    """Ghost points of the allocated box with exactly one axis outside the owned range."""
+    I, J, K = cases._box_index(dims, g)
+    out = np.zeros(I.shape, dtype=int)
+    for X, m, d in zip((I, J, K), dims, delta):
+        out += (X < 0) | (X >= m + 1 - d)
+    return (out == 1) & _alloc_mask(dims, g, delta)
+
+
+@pytest.mark.parametrize("which", ["geometry_3", "geometry_6"])
+@pytest.mark.parametrize("op", ["sync_ucell", "sync_bface", "sync_eedge"])
+def test_node_rule_writes_every_face_strip(samples, which, op):
+    """The reference's rule reads its staggered sources at the node-mapped
+    index; its NaN-overwrite check holds where that source is an owned DOF:
+    with owned DOFs coded and every ghost NaN, a face-strip ghost is left
+    NaN only where the source is itself a ghost of the same level."""
+    case, want, got, owned, alloc = _run_codes(samples, which, 0, op)
+    nnan = nall = 0
+    for (name, b), v in got.items():
+        kind = GROUPS[op][0][1]
+        a = [x for x in GROUPS[op] if x[0] == name][0][2]
+        delta = DELTA["cell"] if kind == "cell" else DELTA[a if kind == "face" else 3 + a]
+        s = _strip_mask(case.blocks[b].dims, case.g, delta)
+        nnan += int(np.isnan(v[s]).sum())
+        nall += int(s.sum())
+        assert np.isfinite(v[owned[name, b]]).all()
+    # every strip ghost is written except those whose node-mapped source is a ghost
+    assert nall > 0 and nnan < nall
+
+
+def _cube_view(case, cube_state, b, name, n, g):
+    """The cube's values at block b's owned cells (cell fields)."""
+    g0, R = cases._lattice_map(case.blocks[b].nodes, n)
+    dims = case.blocks[b].dims
+    I, J, K = np.meshgrid(*(np.arange(m) for m in dims), indexing="ij")
+    P = np.stack([I, J, K]).reshape(3, -1)
+    c = np.mod(g0[:, None] + R @ P + np.minimum(R, 0).sum(axis=1)[:, None], n) + g
+    return cube_state[:, c[0], c[1], c[2]].reshape((cube_state.shape[0],) + I.shape)
+
+
+def _step_pair(samples, which, rule, steps=1):
+    geo = _geo(samples, which)
+    try:
+        n = 64
+        cube = cases.cartesian_periodic_cube(n, g=4, steps=steps)
+        case = cases.cube_on_geometry(geo, cube, rule=rule)
+    finally:
+        geo.close()
+    oc, op_ = orc.Oracle(cube), orc.Oracle(case)
+    oc.prepare()
+    op_.prepare()
+    oc.step(steps)
+    op_.step(steps)
+    worst = 0.0
+    for name in ("U_H", "U_Na"):
+        ref = oc.get(name, 0)
+        for b in range(case.nb):
+            m = case.blocks[b].dims
+            g = case.g
+            got = op_.get(name, b)[:, g:g + m[0], g:g + m[1], g:g + m[2]]
+            want = _cube_view(case, ref, b, name, n, g)
+            scale = np.abs(want).max(axis=(1, 2, 3), keepdims=True) + 1e-300
+            worst = max(worst, float((np.abs(got - want) / scale).max()))
+    return worst
+
+
+@pytest.mark.parametrize("which", ["geometry_3"])
+def test_position_rule_steps_like_the_one_block_cube(samples, which):
+    assert _step_pair(samples, which, 1) < 1e-11
+
+
+@pytest.mark.parametrize("which", ["geometry_3"])
+def test_node_rule_departs_from_the_one_block_cube(samples, which):
+    assert _step_pair(samples, which, 0) > 1e-7
