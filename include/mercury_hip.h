@@ -114,6 +114,47 @@ int mercury_create(MercuryContext **ctx, int nblocks, const int *dims, int nghos
  * NULL when no face is an interface. */
 int mercury_create_connected(MercuryContext **ctx, int nblocks, const int *dims, int nghost, const int *bc, const int *nbr,
                              const int *transform, const double *owned_nodes, const MercuryPhysics *phys);
+
+/* mercury_create for blocks joined by patches: any axis permutation and
+ * reversal, faces split into several patches, a block joined to itself for
+ * periodicity; the topology TOPO::build_topology derives from the grid's
+ * inner-connection files (src/2_topology/TopologyBuilder.cpp:64-260, patches
+ * read by mercury_case_interfaces).  patches: npatch records of 18 ints
+ * (block, direction +-1..3, node box lo[3] hi[3] on the block face (hi
+ * exclusive), neighbour block, perm[3], sign[3], offset[3]: neighbour node
+ * index[perm[a]] = sign[a] * node index[a] + offset[a]), in the order of the
+ * reference's inner patch list (it decides which face fills a corner).  Faces
+ * with code MERCURY_BC_INTERFACE must be covered by their patches; other codes
+ * are MERCURY_BC_FARFIELD or MERCURY_BC_WALL (no MERCURY_BC_PERIODIC: periodicity
+ * is a patch).  The halo replays the reference's three levels of regions
+ * (Halo::build_inner_{1,2,3}DCorner_pattern, src/4_halo/build/HaloBuildFace.cpp,
+ * HaloBuildEdge.cpp, HaloBuildVertex.cpp; copies of HaloExchangeInner.cpp and
+ * src/4_halo/forms/HaloTriplet.cpp) and runs as one flat copy list per group
+ * and stage (mercury_sync_plan reads them back); shared faces and edges take
+ * their owner's value with the topology's orientation signs
+ * (src/2_topology/Topology.cpp:280-323).
+ * rule 0 reproduces the reference: staggered DOFs are mapped as node indices,
+ * face 2-forms take the factor sign[0]*sign[1]*sign[2]*sign[src] and read, at
+ * the edge and vertex levels, the axis a with perm[a] = the destination axis.
+ * Along a reversed axis such a DOF lands one index away from the DOF at its
+ * position, an odd permutation flips the 2-form factor, and corner ghosts the
+ * reference's regions skip keep older values.  rule 1 maps every DOF by
+ * position, carries the area vector's orientation (permutation parity) and
+ * fills the ghosts no region writes through chains of patches: on the
+ * reference's sample geometries a step then equals the one-block cube's. */
+int mercury_create_patched(MercuryContext **ctx, int nblocks, const int *dims, int nghost, const int *bc, int npatch,
+                           const int *patches, int rule, const double *owned_nodes, const MercuryPhysics *phys);
+
+/* Host only (no GPU, no context): the copies mercury_create_patched derives
+ * for one kind of field group (0 cell component copies, 1 face 2-form triplet,
+ * 2 edge 1-form triplet) up to a halo level (1 face strips, 3 with edges and
+ * vertices), stage 0 the halo and stage 1 owner -> alias.  Entries of 11 ints:
+ * destination (field axis, block, i, j, k), source (field axis, block, i, j, k),
+ * sign; a halo source is the DOF's value when the halo starts, so an entry list
+ * whose sources include its destinations is applied with every read before
+ * every write.  *n receives the count; entries are written when capacity >= *n. */
+int mercury_patch_plan(int nblocks, const int *dims, int nghost, const int *bc, int npatch, const int *patches, int rule, int kind,
+                       int level, int stage, long long capacity, long long *n, int *entries);
 /* An ensemble of independent runs in ONE context (no reference counterpart: the
  * reference runs one case per MPI job): run r owns the next run_nblocks[r] blocks
  * of the concatenated block list (dims, bc, owned_nodes as for
@@ -169,7 +210,9 @@ int mercury_stream(MercuryContext *ctx, void **stream);
  * with slot = the slot of mercury_field_info + the component, points (i, j, k)
  * in the reference's index space of the slot's location (ghosts negative or
  * beyond the owned range, inside the block's box), sign +1 or -1; the entry
- * sets dst = sign * src, and the entries of one stage run concurrently.
+ * sets dst = sign * src, and the entries of one stage act as if every source
+ * were read before any destination is written (a stage whose sources include
+ * its own destinations runs as a gather and a scatter through a scratch list).
  * Setting a stage replaces the context's own copies of that stage for that
  * group (the other stage keeps what it had); the physical handlers and corner
  * defaults of the group still run first.  Drops a captured step graph.

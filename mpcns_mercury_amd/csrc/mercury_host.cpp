@@ -28,6 +28,7 @@
 #include "mercury_geometry.h"
 #include "mercury_kernels.h"
 #include "mercury_fused.h"
+#include "mercury_patches.h"
 #include "mercury_case.h"
 #include "mercury_io.h"
 #include "mercury_output.h"
@@ -160,7 +161,7 @@ public:
                 if (k.bc[f] == MBC_PERIODIC) {
                     cnb[6 * b + f] = b;
                     coff[6 * b + f][a] = (f & 1) ? -k.m[a] : k.m[a];
-                } else if (k.bc[f] == MBC_INTERFACE) {
+                } else if (k.bc[f] == MBC_INTERFACE && nbr && trans) {
                     cnb[6 * b + f] = nbr[6 * b + f];
                     for (int c = 0; c < 3; ++c) coff[6 * b + f][c] = trans[9 * (6 * b + f) + 6 + c];
                 }
@@ -593,44 +594,59 @@ struct Classes {
     }
 };
 
-// edge alias classes (ReduceEdgeAliasCandidatesToOwners_, MercurySolver_Current.cpp:3-57)
+// edge alias classes (ReduceEdgeAliasCandidatesToOwners_, MercurySolver_Current.cpp:3-57):
+// per member its arena index, orientation sign and axis (the triplet field it
+// lives in); per class the owner's index, sign and axis
 struct AliasClasses {
-    DBuf<int> start, member, msign, owner, osign, slot_of;
+    DBuf<int> start, member, msign, maxis, owner, osign, oaxis;
     long long n = 0;
-    // host copies (arena indices), for the export
-    std::vector<int> hst{0}, hmem, hms, how, hos, hsl;
-    void set(std::vector<int> st, std::vector<int> mem, std::vector<int> ms, std::vector<int> ow, std::vector<int> os,
-             std::vector<int> sl) {
+    // host copies (arena indices), for the export and the resistive unknowns
+    std::vector<int> hst{0}, hmem, hms, hmax, how, hos, hoax;
+    void set(std::vector<int> st, std::vector<int> mem, std::vector<int> ms, std::vector<int> max, std::vector<int> ow,
+             std::vector<int> os, std::vector<int> oax) {
         n = (long long)ow.size();
         start.upload(st);
         member.upload(mem);
         msign.upload(ms);
+        maxis.upload(max);
         owner.upload(ow);
         osign.upload(os);
-        slot_of.upload(sl);
+        oaxis.upload(oax);
         hst = std::move(st);
         hmem = std::move(mem);
         hms = std::move(ms);
+        hmax = std::move(max);
         how = std::move(ow);
         hos = std::move(os);
-        hsl = std::move(sl);
+        hoax = std::move(oax);
     }
     void build(const Classes &C) {
-        std::vector<int> st{0}, mem, ms, ow, os, sl;
+        std::vector<int> st{0}, mem, ms, max, ow, os, oax;
         for (int a = 0; a < 3; ++a) {
             const int L = LOC_EX + a;
             for (int c = 0; c < C.ncls(L); ++c) {
                 for (int t = C.start[L][c]; t < C.start[L][c + 1]; ++t) {
                     mem.push_back(C.member[L][t]);
                     ms.push_back(1);
+                    max.push_back(a);
                 }
                 st.push_back((int)mem.size());
                 ow.push_back(C.member[L][C.start[L][c]]);
                 os.push_back(1);
-                sl.push_back(a);
+                oax.push_back(a);
             }
         }
-        set(std::move(st), std::move(mem), std::move(ms), std::move(ow), std::move(os), std::move(sl));
+        set(std::move(st), std::move(mem), std::move(ms), std::move(max), std::move(ow), std::move(os), std::move(oax));
+    }
+    // non-owner members of each axis (arena index flags)
+    void alias_flags(long long V, std::vector<char> flags[3]) const {
+        for (int a = 0; a < 3; ++a) flags[a].assign(V, 0);
+        for (size_t c = 0; c + 1 < hst.size(); ++c)
+            for (int t = hst[c]; t < hst[c + 1]; ++t)
+                if (!(hmem[t] == how[c] && hmax[t] == hoax[c])) flags[hmax[t]][hmem[t]] = 1;
+    }
+    void launch(hipStream_t st, double *const *F, int slot0) const {
+        launch_alias_reduce(st, start.p, member.p, msign.p, maxis.p, owner.p, osign.p, oaxis.p, n, F, slot0);
     }
 };
 
@@ -685,6 +701,13 @@ public:
     bool caller_plan[G_NGROUP] = {};
     std::vector<int> plan_h[G_NGROUP][2];
     DBuf<int> plan_d[G_NGROUP][2];
+    // a plan whose sources include its own destinations runs staged (every
+    // read before every write) through plan_tmp
+    bool plan_staged[G_NGROUP][2] = {};
+    DBuf<double> plan_tmp[G_NGROUP][2];
+    // patch connections (rotated, mirrored, partial-face interfaces): their
+    // halo and owner -> alias copies are the groups' plans
+    std::unique_ptr<patch::Topo> ptopo;
     bool any_caller_plan() const {
         for (bool b : caller_plan)
             if (b) return true;
@@ -796,11 +819,17 @@ public:
 
     // block_run: run of each block (NULL with one run); prun: physics of each run
     MercurySolver(int nb, const int *dims, int g, const int *bc, const int *nbr, const int *trans, const double *nodes, int nruns_,
-                  const int *block_run, const MercPhys *prun)
+                  const int *block_run, const MercPhys *prun, const patch::Topo *pt = nullptr)
         : P(prun[0]), nruns(nruns_) {
         if (g < 2) throw std::runtime_error("nghost must be >= 2");
         if (nruns < 1) throw std::runtime_error("an ensemble needs at least one run");
-        check_topology(nb, dims, g, bc, nbr, trans);
+        if (pt) {
+            if (nruns != 1) throw std::runtime_error("patch connections need a one-run context");
+            ptopo.reset(new patch::Topo(*pt));
+            nbr = trans = nullptr;
+        } else {
+            check_topology(nb, dims, g, bc, nbr, trans);
+        }
         brun.assign(nb, 0);
         if (block_run)
             for (int b = 0; b < nb; ++b) {
@@ -809,7 +838,7 @@ public:
             }
         for (int b = 0; b < nb; ++b)
             for (int f = 0; f < 6; ++f)
-                if (bc[6 * b + f] == MBC_INTERFACE && brun[nbr[6 * b + f]] != brun[b])
+                if (nbr && bc[6 * b + f] == MBC_INTERFACE && brun[nbr[6 * b + f]] != brun[b])
                     throw std::runtime_error("block " + std::to_string(b) + ": an interface joins two runs of the ensemble");
         Prun.assign(prun, prun + nruns);
         check_run_physics(nruns, prun);
@@ -898,6 +927,7 @@ public:
         cls.build(*F);
         build_programs();
         alias.build(cls);
+        if (ptopo) apply_patch_plans();
         build_resist();
     }
     ~MercurySolver() {
@@ -1182,9 +1212,67 @@ public:
             caller_plan[gid] = true;
         }
         plan_h[gid][stage] = std::move(ent);
-        for (int t = 0; t < 2; ++t) plan_d[gid][t].upload(plan_h[gid][t]);
+        upload_plans(gid);
         build_all_merged();
         drop_graph();
+    }
+    void upload_plans(int gid) {
+        for (int t = 0; t < 2; ++t) {
+            const std::vector<int> &e = plan_h[gid][t];
+            plan_d[gid][t].upload(e);
+            std::vector<std::pair<int, int>> dst;
+            for (size_t q = 0; q + 4 < e.size(); q += 5) dst.push_back({e[q], e[q + 1]});
+            std::sort(dst.begin(), dst.end());
+            bool staged = false;
+            for (size_t q = 0; q + 4 < e.size() && !staged; q += 5) staged = std::binary_search(dst.begin(), dst.end(), std::make_pair(e[q + 2], e[q + 3]));
+            plan_staged[gid][t] = staged;
+            plan_tmp[gid][t].upload(std::vector<double>(staged ? e.size() / 5 : 0, 0.0));
+        }
+    }
+    // Plans of every group from the patch connections (rule and regions as
+    // merc::patch states): stage 0 the halo, stage 1 owner -> alias; and the
+    // edge alias classes of the current and EMF means
+    void apply_patch_plans() {
+        const patch::Topo &T = *ptopo;
+        const Field &Fl = *F;
+        for (int gid = 0; gid < G_NGROUP; ++gid) {
+            const auto gfs = group_fields(gid, nspecies());
+            const int loc = gfs[0].loc;
+            const int kind = loc == LOC_CELL ? patch::KIND_COMPONENT : (loc >= LOC_FX && loc <= LOC_FZ ? patch::KIND_FACE : patch::KIND_EDGE);
+            for (int t = 0; t < 2; ++t) {
+                const std::vector<patch::Entry> ent = t == 0 ? patch::halo_entries(T, kind, gfs[0].level) : patch::owner_entries(T, kind);
+                std::vector<int> out;
+                for (const patch::Entry &e : ent) {
+                    const int d = Fl.gidx(e.dst.b, e.dst.i, e.dst.j, e.dst.k), s = Fl.gidx(e.src.b, e.src.i, e.src.j, e.src.k);
+                    if (kind == patch::KIND_COMPONENT) {
+                        for (const GField &gf : gfs)
+                            for (int c = 0; c < gf.ncomp; ++c) out.insert(out.end(), {gf.slot + c, d, gf.slot + c, s, e.sign});
+                    } else {
+                        out.insert(out.end(), {gfs[e.dst.f].slot, d, gfs[e.src.f].slot, s, e.sign});
+                    }
+                }
+                plan_h[gid][t] = std::move(out);
+            }
+            caller_plan[gid] = true;
+            upload_plans(gid);
+        }
+        const patch::EdgeClasses E = patch::edge_classes(T);
+        std::vector<int> st{0}, mem, ms, max, ow, os, oax;
+        for (size_t c = 0; c + 1 < E.start.size(); ++c) {
+            for (int t = E.start[c]; t < E.start[c + 1]; ++t) {
+                const patch::Dof &m = E.mem[t];
+                mem.push_back(Fl.gidx(m.b, m.i, m.j, m.k));
+                ms.push_back(E.sign[t]);
+                max.push_back(m.f);
+            }
+            st.push_back((int)mem.size());
+            const patch::Dof &o = E.mem[E.start[c]];
+            ow.push_back(Fl.gidx(o.b, o.i, o.j, o.k));
+            os.push_back(E.sign[E.start[c]]);
+            oax.push_back(o.f);
+        }
+        alias.set(std::move(st), std::move(mem), std::move(ms), std::move(max), std::move(ow), std::move(os), std::move(oax));
+        build_all_merged();
     }
     void drop_graph() {
         ++generation;
@@ -1290,8 +1378,10 @@ public:
         // caller plans: the halo stage of every group of the set, then the owner stage
         for (int t = 0; t < 2; ++t)
             for (int gid = 0; gid < G_NGROUP; ++gid)
-                if ((gset & (1u << gid)) && caller_plan[gid] && plan_d[gid][t].n > 0)
-                    launch_copy_plan(s, plan_d[gid][t].p, plan_d[gid][t].n / 5, Fd());
+                if ((gset & (1u << gid)) && caller_plan[gid] && plan_d[gid][t].n > 0) {
+                    if (plan_staged[gid][t]) launch_copy_plan_staged(s, plan_d[gid][t].p, plan_d[gid][t].n / 5, Fd(), plan_tmp[gid][t].p);
+                    else launch_copy_plan(s, plan_d[gid][t].p, plan_d[gid][t].n / 5, Fd());
+                }
     }
 
     void sync(int gid) { run_merged(1u << gid); }
@@ -1313,11 +1403,7 @@ public:
         std::vector<double> eta;
         const Field &Fl = *F;
         std::vector<char> alias_ax[3];  // non-owner members of the edge classes, per axis
-        for (int a = 0; a < 3; ++a) {
-            alias_ax[a].assign(Fl.Vtot, 0);
-            for (int c = 0; c < cls.ncls(LOC_EX + a); ++c)
-                for (int t = cls.start[LOC_EX + a][c] + 1; t < cls.start[LOC_EX + a][c + 1]; ++t) alias_ax[a][cls.member[LOC_EX + a][t]] = 1;
-        }
+        alias.alias_flags(Fl.Vtot, alias_ax);
         for (int b = 0; b < Fl.nb(); ++b) {
             const MercBlk &k = Fl.blk[b];
             const BlockGeom &G = *geom[b];
@@ -1429,7 +1515,7 @@ public:
         launch_dec_d1(st, R_res.h, Bd(), Fd(), SL_ERES, SL_DB, -dt);
         run_merged(1u << G_DB);
         launch_jedge(st, R_res.h, Bd(), Fd(), SL_DB, SL_DJ);
-        launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_DJ);
+        alias.launch(st, Fd(), SL_DJ);
         launch_res_gather(st, rd, Fd(), y, x, w, r_part.p);
     }
 
@@ -1447,7 +1533,7 @@ public:
         // Calc_J_Edge on B* around the unknowns (its Sync(Jedge) touches no
         // unknown; the derived pass recomputes J everywhere afterwards)
         launch_jedge(st, R_res.h, Bd(), Fd());
-        launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
+        alias.launch(st, Fd(), SL_J);
         launch_res_rhs(st, rd, Fd(), dt, rb, rdg, rw, rwd);
         res_apply(rx, rAp, dt, false, nullptr);
         launch_cg_init(st, n, rb, rAp, rdg, rwd, rr, rp, r_part.p, r_S.p);
@@ -1488,7 +1574,7 @@ public:
     // last derived pass of a step also reduces the next step's dt
     void derived_fused(bool final) {
         launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix(), mon_part(), d_mon);
-        launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
+        alias.launch(st, Fd(), SL_J);
         run_merged(MG_DERIVED_PAIR);
         launch_jcell_t(st, R_cell.h, Bd(), Fd(), Gd(), mix());
         sync(G_JCELL);
@@ -1513,7 +1599,7 @@ public:
         run_merged(1u << G_UCELL, st2);
         run_merged(1u << G_BFACE, st);
         launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix(), mon_part(), d_mon);
-        launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
+        alias.launch(st, Fd(), SL_J);
         if (alias.n == 0 && !F->connected() && !any_caller_plan()) {
             HIPCHK(hipEventRecord(ev_fork2, st));
             HIPCHK(hipStreamWaitEvent(st2, ev_fork2, 0));
@@ -1554,7 +1640,7 @@ public:
             // shared edges (periodic seams, interfaces): the alias mean of the
             // non-Hall EMF before the Hall part is added; one pass writes both
             launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, 2, mix());
-            launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_E);
+            alias.launch(st, Fd(), SL_E);
             sync(G_EHALL);
             launch_add_hall_wall(st, R_node.h, Bd(), Fd());
         }
@@ -1569,7 +1655,7 @@ public:
         launch_bcell(st, R_cell.h, Bd(), Fd());
         sync(G_BCELL);
         launch_jedge(st, R_node.h, Bd(), Fd());
-        launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
+        alias.launch(st, Fd(), SL_J);
         sync(G_JEDGE);
         launch_jcell(st, R_cell.h, Bd(), Fd());
         sync(G_JCELL);
@@ -1582,7 +1668,7 @@ public:
         launch_zero_span(st, Fd(), es, 3, F->Vtot);
         launch_eface(st, R_eface.h, Bd(), Fd());
         launch_edge_emf(st, R_node.h, Bd(), Fd(), P);
-        launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_E);
+        alias.launch(st, Fd(), SL_E);
         launch_hall_cell(st, R_box.h, Bd(), Fd(), P);
         launch_hall_edge(st, R_node.h, Bd(), Fd());
         sync(G_EHALL);
@@ -1770,21 +1856,38 @@ public:
     // function of U and B only: a fused form recomputes the whole derived chain
     // with its own passes, so a state read from a restart continues exactly as
     // the run that wrote it
+    // Syncs Badd and Bface and runs UpdateDerivedFields_ once, as the
+    // reference's set-up does; the cell-end pass adds the fused forms'
+    // intermediates and the first dt.  A sync whose halo reads ghosts of its own
+    // level (the node-index rule of patch connections) is not idempotent, so
+    // neither this nor set_form syncs anything twice.
+    bool prepared = false;
     void prepare() {
         clear_status();
         run_merged(MG_PREPARE_PAIR);
-        derived();
-        refresh_float_statics();
-        if (form >= 1) derived_fused(true);
-        else launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, false, nruns);
+        if (form == 2) {
+            // the mixed form's own derived fields, as its steps leave them
+            refresh_float_statics();
+            derived_fused(true);
+        } else {
+            derived();
+            refresh_float_statics();
+            launch_cell_end(st, R_box.h, Bd(), Fd(), Gd(), P, d_dt_next, 1, false, nruns);
+        }
+        prepared = true;
         HIPCHK(hipStreamSynchronize(st));
     }
 
     void set_form(int f) {
         if (f < 0 || f > 2) throw std::runtime_error("form must be 0 (per-function FP64), 1 (fused FP64) or 2 (mixed)");
+        // the same form, or a context not prepared yet: nothing to recompute
+        // (a context of patch connections under the node-index rule should get
+        // its form before prepare: switching after it syncs the derived fields
+        // a second time)
+        if (f == form) return;
         form = f;
         ++generation;
-        if (f >= 1) {
+        if (f >= 1 && prepared) {
             refresh_float_statics();
             derived_fused(true);
         }
@@ -1812,9 +1915,9 @@ public:
             induction_emf();
             launch_curl_update(st, R_node.h, Bd(), Fd(), dtp(), 0);
         } else if (op == "alias_e")
-            launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_E);
+            alias.launch(st, Fd(), SL_E);
         else if (op == "alias_j")
-            launch_alias_reduce(st, alias.start.p, alias.member.p, alias.msign.p, alias.owner.p, alias.osign.p, alias.slot_of.p, alias.n, Fd(), SL_J);
+            alias.launch(st, Fd(), SL_J);
         else if (op == "timestep") launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt, nruns);
         else if (op == "stage") stage();
         else if (op == "rhs_cell") launch_flux(st);
@@ -1948,6 +2051,42 @@ int mercury_create_connected(MercuryContext **out, int nblocks, const int *dims,
         std::unique_ptr<MercuryContext> c(new MercuryContext());
         c->s.reset(new merc::MercurySolver(nblocks, dims, nghost, bc, nbr, transform, owned_nodes, P));
         *out = c.release();
+    })
+}
+
+int mercury_create_patched(MercuryContext **out, int nblocks, const int *dims, int nghost, const int *bc, int npatch, const int *patches,
+                           int rule, const double *owned_nodes, const MercuryPhysics *phys) {
+    if (!out || nblocks <= 0 || !dims || !bc || !owned_nodes || !phys) {
+        merc::g_last_error = "mercury_create_patched: invalid argument";
+        return -1;
+    }
+    ABI_TRY({
+        const merc::patch::Topo T = merc::patch::make_topo(nblocks, dims, nghost, bc, npatch, patches, rule);
+        MercPhys P;
+        std::memcpy(&P, phys, sizeof(P));
+        std::unique_ptr<MercuryContext> c(new MercuryContext());
+        c->s.reset(new merc::MercurySolver(nblocks, dims, nghost, bc, nullptr, nullptr, owned_nodes, 1, nullptr, &P, &T));
+        *out = c.release();
+    })
+}
+
+int mercury_patch_plan(int nblocks, const int *dims, int nghost, const int *bc, int npatch, const int *patches, int rule, int kind,
+                       int level, int stage, long long capacity, long long *n, int *entries) {
+    if (!n || (stage != 0 && stage != 1)) {
+        merc::g_last_error = "mercury_patch_plan: invalid argument";
+        return -1;
+    }
+    ABI_TRY({
+        const merc::patch::Topo T = merc::patch::make_topo(nblocks, dims, nghost, bc, npatch, patches, rule);
+        const std::vector<merc::patch::Entry> e = stage == 0 ? merc::patch::halo_entries(T, kind, level) : merc::patch::owner_entries(T, kind);
+        *n = (long long)e.size();
+        if (!entries || capacity < *n) return 0;
+        for (size_t t = 0; t < e.size(); ++t) {
+            int *o = entries + 11 * t;
+            const merc::patch::Dof &d = e[t].dst, &s = e[t].src;
+            const int v[11] = {d.f, d.b, d.i, d.j, d.k, s.f, s.b, s.i, s.j, s.k, e[t].sign};
+            std::copy(v, v + 11, o);
+        }
     })
 }
 
@@ -2107,7 +2246,7 @@ int mercury_set_alias_classes(MercuryContext *ctx, int ncls, const int *start, c
         if (ncls < 0 || (ncls > 0 && (!start || !axis || !members || !owners)))
             throw std::runtime_error("mercury_set_alias_classes: invalid argument");
         merc::MercurySolver &S = *ctx->s;
-        std::vector<int> st{0}, mem, ms, ow, os, sl;
+        std::vector<int> st{0}, mem, ms, max, ow, os, oax;
         if (ncls > 0 && start[0] != 0) throw std::runtime_error("mercury_set_alias_classes: start[0] must be 0");
         for (int c = 0; c < ncls; ++c) {
             if (start[c + 1] <= start[c]) throw std::runtime_error("mercury_set_alias_classes: empty class " + std::to_string(c));
@@ -2117,16 +2256,17 @@ int mercury_set_alias_classes(MercuryContext *ctx, int ncls, const int *start, c
                 if (m[4] != 1 && m[4] != -1) throw std::runtime_error("mercury_set_alias_classes: sign must be +1 or -1");
                 mem.push_back(S.F->gidx_checked(m[0], m[1], m[2], m[3], "mercury_set_alias_classes member"));
                 ms.push_back(m[4]);
+                max.push_back(axis[c]);
             }
             st.push_back((int)mem.size());
             const int *o = owners + 5 * (long long)c;
             if (o[4] != 1 && o[4] != -1) throw std::runtime_error("mercury_set_alias_classes: sign must be +1 or -1");
             ow.push_back(S.F->gidx_checked(o[0], o[1], o[2], o[3], "mercury_set_alias_classes owner"));
             os.push_back(o[4]);
-            sl.push_back(axis[c]);
+            oax.push_back(axis[c]);
         }
         HIPCHK(hipStreamSynchronize(S.st));
-        S.alias.set(std::move(st), std::move(mem), std::move(ms), std::move(ow), std::move(os), std::move(sl));
+        S.alias.set(std::move(st), std::move(mem), std::move(ms), std::move(max), std::move(ow), std::move(os), std::move(oax));
         S.drop_graph();
     })
 }
@@ -2141,9 +2281,14 @@ int mercury_alias_classes(MercuryContext *ctx, int cls_capacity, long long membe
         *nmembers = (long long)A.hmem.size();
         if (!start || !axis || !members || !owners || cls_capacity < *ncls || member_capacity < *nmembers) return 0;
         for (int c = 0; c <= *ncls; ++c) start[c] = A.hst[c];
+        for (int c = 0; c < *ncls; ++c)
+            for (int t = A.hst[c]; t < A.hst[c + 1]; ++t)
+                if (A.hmax[t] != A.hoax[c])
+                    throw std::runtime_error("mercury_alias_classes: class " + std::to_string(c) +
+                                             " joins edges of different axes (a patch-connected context); one axis per class is exported");
         for (int c = 0; c < *ncls; ++c) {
             int p[4];
-            axis[c] = A.hsl[c];
+            axis[c] = A.hoax[c];
             ctx->s->F->point_of(A.how[c], p);
             for (int q = 0; q < 4; ++q) owners[5 * c + q] = p[q];
             owners[5 * c + 4] = A.hos[c];

@@ -38,8 +38,8 @@ int divb_partials();
 void launch_divb(hipStream_t st, const RangeH &R, const MercBlk *B, double *const *F, double *partial, unsigned long long *out,
                  int write_field);
 
-void launch_alias_reduce(hipStream_t st, const int *cls_start, const int *member, const int *msign, const int *owner, const int *osign,
-                         const int *slot_of, long long ncls, double *const *F, int slot0);
+void launch_alias_reduce(hipStream_t st, const int *cls_start, const int *member, const int *msign, const int *maxis, const int *owner,
+                         const int *osign, const int *oaxis, long long ncls, double *const *F, int slot0);
 void launch_zero_span(hipStream_t st, double *const *F, const int *slots, int ns, long long n);
 void launch_copy_multi(hipStream_t st, const int *ent, long long n, double *const *F, const MultiSets &S);
 void launch_fill_multi(hipStream_t st, const int *ent, long long n, double *const *F, const MultiSets &S);
@@ -47,5 +47,8 @@ void launch_setv_multi(hipStream_t st, const int *ent, const double *val, long l
 void launch_wall_multi(hipStream_t st, const int *ent, long long n, int g, double *const *F, const MercPhys &P);
 // n entries (dst slot, dst, src slot, src, sign): dst <- sign * src
 void launch_copy_plan(hipStream_t st, const int *ent, long long n, double *const *F);
+// the same entries with every read before every write (sources that are also
+// destinations read their old values): tmp holds n doubles
+void launch_copy_plan_staged(hipStream_t st, const int *ent, long long n, double *const *F, double *tmp);
 
 }  // namespace merc

@@ -641,6 +641,22 @@ __global__ void k_copy_plan(const int *__restrict__ ent, long long n, double *co
     const int *e = ent + 5 * t;
     F[e[0]][e[1]] = (double)e[4] * F[e[2]][e[3]];
 }
+// a plan applied with every read before every write (its sources include
+// destinations of the same plan): gather into tmp, then scatter
+__global__ void k_plan_gather(const int *__restrict__ ent, long long n, double *const *__restrict__ F_, double *__restrict__ tmp) {
+    gdouble *const *__restrict__ F = gslots(F_);
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int *e = ent + 5 * t;
+    tmp[t] = (double)e[4] * F[e[2]][e[3]];
+}
+__global__ void k_plan_scatter(const int *__restrict__ ent, long long n, double *const *__restrict__ F_, const double *__restrict__ tmp) {
+    gdouble *const *__restrict__ F = gslots(F_);
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int *e = ent + 5 * t;
+    F[e[0]][e[1]] = tmp[t];
+}
 // BC_Solid_Surface_ (Boundary_Condition_Fluid.cpp:36-260): per wall cell the
 // entry holds (species base slot, species (0 H+, 1 Na+, 2 He++), first layer,
 // one deeper, g ghosts); He++ takes the H+ rule with its own density floor
@@ -687,18 +703,18 @@ __global__ void k_wall_multi(const int *__restrict__ ent, long long n, int g, do
 }
 
 // owner <- signed mean of an equivalence class (ReduceEdgeAliasCandidatesToOwners_,
-// MercurySolver_Current.cpp:3-57); members summed in the listed order
+// MercurySolver_Current.cpp:3-57); members summed in the listed order, each
+// read from the triplet field of its own axis
 __global__ void k_alias_reduce(const int *__restrict__ cls_start, const int *__restrict__ member, const int *__restrict__ msign,
-                               const int *__restrict__ owner, const int *__restrict__ osign, const int *__restrict__ slot_of,
-                               long long ncls, double *const *__restrict__ F_, int slot0) {
+                               const int *__restrict__ maxis, const int *__restrict__ owner, const int *__restrict__ osign,
+                               const int *__restrict__ oaxis, long long ncls, double *const *__restrict__ F_, int slot0) {
     gdouble *const *__restrict__ F = gslots(F_);
     const long long t = TID;
     if (t >= ncls) return;
     const int b = cls_start[t], e = cls_start[t + 1];
-    gdouble *f = F[slot0 + slot_of[t]];
     double sum = 0.0;
-    for (int m = b; m < e; ++m) sum += msign[m] * f[member[m]];
-    f[owner[t]] = osign[t] * sum / (double)(e - b);
+    for (int m = b; m < e; ++m) sum += msign[m] * F[slot0 + maxis[m]][member[m]];
+    F[slot0 + oaxis[t]][owner[t]] = osign[t] * sum / (double)(e - b);
 }
 
 // ---------------------------------------------------------------------------
@@ -876,10 +892,11 @@ void launch_divb(hipStream_t st, const RangeH &R, const MercBlk *B, double *cons
     hipLaunchKernelGGL(k_divb_final, dim3(1), dim3(256), 0, st, partial, groups, out);
 }
 
-void launch_alias_reduce(hipStream_t st, const int *cls_start, const int *member, const int *msign, const int *owner, const int *osign,
-                         const int *slot_of, long long ncls, double *const *F, int slot0) {
-    if (ncls <= 0) return;
-    hipLaunchKernelGGL(k_alias_reduce, dim3(nblk(ncls, 256)), dim3(256), 0, st, cls_start, member, msign, owner, osign, slot_of, ncls, F, slot0);
+void launch_alias_reduce(hipStream_t st, const int *cls_start, const int *member, const int *msign, const int *maxis, const int *owner,
+                         const int *osign, const int *oaxis, long long ncls, double *const *F, int slot0) {
+    if (ncls > 0)
+        hipLaunchKernelGGL(k_alias_reduce, dim3(nblk(ncls, 256)), dim3(256), 0, st, cls_start, member, msign, maxis, owner, osign, oaxis, ncls,
+                           F, slot0);
 }
 void launch_copy_multi(hipStream_t st, const int *ent, long long n, double *const *F, const MultiSets &S) {
     if (n > 0) hipLaunchKernelGGL(k_copy_multi, dim3(nblk(n, 256)), dim3(256), 0, st, ent, n, F, S);
@@ -892,6 +909,11 @@ void launch_setv_multi(hipStream_t st, const int *ent, const double *val, long l
 }
 void launch_copy_plan(hipStream_t st, const int *ent, long long n, double *const *F) {
     if (n > 0) hipLaunchKernelGGL(k_copy_plan, dim3(nblk(n, 256)), dim3(256), 0, st, ent, n, F);
+}
+void launch_copy_plan_staged(hipStream_t st, const int *ent, long long n, double *const *F, double *tmp) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_plan_gather, dim3(nblk(n, 256)), dim3(256), 0, st, ent, n, F, tmp);
+    hipLaunchKernelGGL(k_plan_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, ent, n, F, (const double *)tmp);
 }
 void launch_wall_multi(hipStream_t st, const int *ent, long long n, int g, double *const *F, const MercPhys &P) {
     if (n > 0) hipLaunchKernelGGL(k_wall_multi, dim3(nblk(n, 256)), dim3(256), 0, st, ent, n, g, F, P);

@@ -19,7 +19,8 @@ LIB_PATH = os.environ.get("MERCURY_HIP_LIB") or os.path.join(_HERE, "libmercury_
 _lib = None
 
 # every symbol declared in include/mercury_hip.h
-ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercury_create_connected", "mercury_create_ensemble", "mercury_run_dt", "mercury_destroy",
+ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercury_create_connected", "mercury_create_patched",
+               "mercury_patch_plan", "mercury_create_ensemble", "mercury_run_dt", "mercury_destroy",
                "mercury_field_info",
                "mercury_block_volume", "mercury_block_shape", "mercury_field_device", "mercury_stream", "mercury_set_stream", "mercury_total_volume", "mercury_upload", "mercury_download",
                "mercury_init_bface", "mercury_init_bface_dipoles", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
@@ -52,6 +53,10 @@ def load():
     L.mercury_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ip, ctypes.c_int, ip, dp, ctypes.POINTER(MercuryPhysics)]
     L.mercury_create_connected.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ip, ctypes.c_int, ip, ip, ip, dp,
                                            ctypes.POINTER(MercuryPhysics)]
+    L.mercury_create_patched.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ip, ctypes.c_int, ip, ctypes.c_int, ip, ctypes.c_int, dp,
+                                         ctypes.POINTER(MercuryPhysics)]
+    L.mercury_patch_plan.argtypes = [ctypes.c_int, ip, ctypes.c_int, ip, ctypes.c_int, ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong), ip]
     L.mercury_destroy.argtypes = [vp]
     L.mercury_create_ensemble.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ip, ip, ctypes.c_int, ip, ip, ip, dp,
                                           ctypes.POINTER(MercuryPhysics)]
@@ -168,6 +173,25 @@ def geometry_block(case, ib, name):
     return out
 
 
+PATCH_KINDS = {"component": 0, "face": 1, "edge": 2}
+
+
+def patch_plan(case, kind: str, level: int, stage: int) -> np.ndarray:
+    """The copies a patch-connected context derives for one kind of field group
+    (host only, no GPU): rows (dst axis, block, i, j, k, src axis, block, i, j, k, sign)."""
+    L = load()
+    ip = ctypes.POINTER(ctypes.c_int)
+    dims, bc = case.dims_array(), case.bc_array()
+    pt = np.ascontiguousarray(case.patches, dtype=np.int32).reshape(-1)
+    n = ctypes.c_longlong(0)
+    args = (case.nb, dims.ctypes.data_as(ip), case.g, bc.ctypes.data_as(ip), len(pt) // 18, pt.ctypes.data_as(ip), int(case.patch_rule),
+            PATCH_KINDS[kind], int(level), int(stage))
+    _chk(L.mercury_patch_plan(*args, 0, ctypes.byref(n), None))
+    out = np.zeros((n.value, 11), dtype=np.int32)
+    _chk(L.mercury_patch_plan(*args, n.value, ctypes.byref(n), out.ctypes.data_as(ip)))
+    return out
+
+
 class Mercury:
     """All blocks of one case resident on the GPU."""
 
@@ -189,6 +213,11 @@ class Mercury:
             _chk(L.mercury_create_ensemble(ctypes.byref(self._h), len(runs), nb_arr.ctypes.data_as(ip), dims.ctypes.data_as(ip),
                                            case.g, bc.ctypes.data_as(ip), nbr.ctypes.data_as(ip), trans.ctypes.data_as(ip),
                                            _dp(nodes), phys))
+        elif getattr(case, "patches", None) is not None:
+            pt = np.ascontiguousarray(case.patches, dtype=np.int32).reshape(-1)
+            _chk(L.mercury_create_patched(ctypes.byref(self._h), case.nb, dims.ctypes.data_as(ip), case.g, bc.ctypes.data_as(ip),
+                                          len(pt) // 18, pt.ctypes.data_as(ip), int(case.patch_rule), _dp(nodes),
+                                          ctypes.byref(self.phys)))
         elif case.has_interfaces():
             nbr, trans = case.nbr_array(), case.trans_array()
             _chk(L.mercury_create_connected(ctypes.byref(self._h), case.nb, dims.ctypes.data_as(ip), case.g, bc.ctypes.data_as(ip),

@@ -248,3 +248,61 @@ def test_position_rule_steps_like_the_one_block_cube(samples, which):
 @pytest.mark.parametrize("which", ["geometry_3"])
 def test_node_rule_departs_from_the_one_block_cube(samples, which):
     assert _step_pair(samples, which, 0) > 1e-7
+
+
+# --- the library's plans against the oracle ------------------------------
+
+PLAN_GROUPS = {
+    "sync_ucell": ("component", 1, [["U_H"], ["U_Na"]]),
+    "sync_bcell": ("component", 3, [["B_cell"], ["Bind_cell"]]),
+    "sync_bface": ("face", 3, [["B_xi", "B_eta", "B_zeta"]]),
+    "sync_eedge": ("edge", 3, [["E_xi", "E_eta", "E_zeta"]]),
+}
+
+
+def _apply(arrs, ent, g):
+    """dst = sign * src for every entry, every source read before any write."""
+    if len(ent) == 0:
+        return
+    nc = next(iter(arrs.values())).shape[0]
+    src = np.empty((nc, len(ent)))
+    for (f, b), a in arrs.items():
+        sel = (ent[:, 5] == f) & (ent[:, 6] == b)
+        src[:, sel] = a[:, ent[sel, 7] + g, ent[sel, 8] + g, ent[sel, 9] + g]
+    src *= ent[:, 10]
+    for (f, b), a in arrs.items():
+        sel = (ent[:, 0] == f) & (ent[:, 1] == b)
+        a[:, ent[sel, 2] + g, ent[sel, 3] + g, ent[sel, 4] + g] = src[:, sel]
+
+
+@pytest.mark.parametrize("which", ["geometry_3", "geometry_6"])
+@pytest.mark.parametrize("rule", [0, 1])
+@pytest.mark.parametrize("op", list(PLAN_GROUPS))
+def test_library_plan_matches_the_oracle_sync(samples, which, rule, op):
+    """mercury_patch_plan (the copy lists a patch-connected context runs) applied
+    to random data on every box point gives the oracle's sync bit for bit."""
+    kind, level, fieldsets = PLAN_GROUPS[op]
+    geo = _geo(samples, which)
+    try:
+        case = cases.cube_on_geometry(geo, cases.cartesian_periodic_cube(64, g=4), rule=rule)
+    finally:
+        geo.close()
+    halo = hip.patch_plan(case, kind, level, 0)
+    own = hip.patch_plan(case, kind, level, 1) if kind != "component" else np.zeros((0, 11), np.int32)
+    assert len(np.unique(halo[:, 0:5], axis=0)) == len(halo)  # one entry per destination
+    o = orc.Oracle(case)
+    rng = np.random.default_rng(7)
+    for names in fieldsets:
+        arrs = {}
+        for f, name in enumerate(names):
+            nc = orc._NCOMP.get(name, 1)
+            for b in range(case.nb):
+                a = rng.standard_normal((nc,) + case.box(b))
+                o.set(name, b, a)
+                arrs[f, b] = a.copy()
+        _apply(arrs, halo, case.g)
+        _apply(arrs, own, case.g)
+        o.op(op)
+        for f, name in enumerate(names):
+            for b in range(case.nb):
+                np.testing.assert_array_equal(o.get(name, b), arrs[f, b], err_msg=f"{name} block {b}")
