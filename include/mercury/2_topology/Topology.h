@@ -15,6 +15,10 @@ struct Topology {
     std::vector<int> bc;         // 6 per block: MERCURY_BC_* codes, -xi,+xi,-eta,+eta,-zeta,+zeta
     std::vector<int> nbr;        // 6 per block: neighbour block of a MERCURY_BC_INTERFACE face, else -1
     std::vector<int> transform;  // 54 per block: IndexTransform of each face
+    // patch connections instead of per-face ones (rotated, mirrored, split faces):
+    // 18 ints per patch as mercury_create_patched takes them; rule 0 is the reference's halo
+    std::vector<int> patches;
+    int rule = 0;
     int num_blocks() const { return (int)bc.size() / 6; }
 };
 
@@ -31,6 +35,25 @@ inline Topology build_topology(const Grid &grid, int myid = 0, int dimension = 3
     t.transform.assign(54 * nb, 0);
     mercury_hip_detail::check(mercury_case_topology(grid.case_files(), t.bc.data(), t.nbr.data(), t.transform.data()),
                               "build_topology");
+    return t;
+}
+
+// Any CASE topology as patches (inner_patches of TOPO::build_topology,
+// src/2_topology/TopologyBuilder.cpp:64-260): split, rotated and mirrored
+// connections included.
+inline Topology build_patch_topology(const Grid &grid, int rule = 0) {
+    if (!grid.case_files()) throw std::runtime_error("build_patch_topology: the grid was not read from a CASE directory");
+    Topology t;
+    const int nb = grid.num_blocks();
+    t.bc.assign(6 * nb, 0);
+    t.nbr.assign(6 * nb, -1);
+    t.transform.assign(54 * nb, 0);
+    t.rule = rule;
+    int np = 0;
+    mercury_hip_detail::check(mercury_case_patch_topology(grid.case_files(), t.bc.data(), 0, &np, nullptr), "build_patch_topology");
+    t.patches.assign(18 * (size_t)np, 0);
+    mercury_hip_detail::check(mercury_case_patch_topology(grid.case_files(), t.bc.data(), np, &np, t.patches.data()),
+                              "build_patch_topology");
     return t;
 }
 

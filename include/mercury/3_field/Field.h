@@ -103,10 +103,16 @@ public:
     Field(Grid *grid, const TOPO::Topology &topo, const MercuryPhysics &phys, int ngg) : grid_(grid), ngg_(ngg) {
         if (!grid || grid->num_blocks() == 0) throw std::runtime_error("Field: empty grid");
         if (topo.num_blocks() != grid->num_blocks()) throw std::runtime_error("Field: topology and grid disagree on the block count");
-        mercury_hip_detail::check(mercury_create_connected(&ctx_, grid->num_blocks(), grid->all_dims().data(), ngg, topo.bc.data(),
-                                                           topo.nbr.data(), topo.transform.data(), grid->owned_nodes().data(),
-                                                           &phys),
-                                  "Field");
+        if (!topo.patches.empty())
+            mercury_hip_detail::check(mercury_create_patched(&ctx_, grid->num_blocks(), grid->all_dims().data(), ngg, topo.bc.data(),
+                                                             (int)topo.patches.size() / 18, topo.patches.data(), topo.rule,
+                                                             grid->owned_nodes().data(), &phys),
+                                      "Field");
+        else
+            mercury_hip_detail::check(mercury_create_connected(&ctx_, grid->num_blocks(), grid->all_dims().data(), ngg, topo.bc.data(),
+                                                               topo.nbr.data(), topo.transform.data(), grid->owned_nodes().data(),
+                                                               &phys),
+                                      "Field");
     }
     ~Field() {
         if (ctx_) mercury_destroy(ctx_);

@@ -402,6 +402,12 @@ int mercury_case_nodes(const MercuryCase *c, double *owned_nodes);
  * MERCURY_BC_FARFIELD, MERCURY_BC_WALL) or a connection (MERCURY_BC_INTERFACE,
  * periodic ones included); anything else fails with a message naming the face. */
 int mercury_case_topology(const MercuryCase *c, int *bc, int *nbr, int *transform);
+/* bc and patches for mercury_create_patched (any connection patches: split,
+ * rotated, mirrored, periodic): faces holding connection patches get
+ * MERCURY_BC_INTERFACE, the others must be one whole-face Farfield or
+ * Solid_Surface patch; *npatch receives the number of 18-int patch records,
+ * written to patches when capacity >= *npatch (the reader's order). */
+int mercury_case_patch_topology(const MercuryCase *c, int *bc, int capacity, int *npatch, int *patches);
 /* patch listings (records may be NULL to query n).  Connection patch, 27 ints:
  * kind (0 inner, 1 parallel), periodic, block, nb_block, direction,
  * nb_direction (+-(axis+1)), node box lo[3] hi[3], neighbour node box lo[3]

@@ -318,6 +318,41 @@ Geometry Geometry::read(const std::string &dir, int nrank, bool binary) {
     return G;
 }
 
+void Geometry::patch_topology(int *bc, std::vector<int> &patches) const {
+    const int nb = (int)blocks.size();
+    patches.clear();
+    for (const Interface &p : interfaces) {
+        const int rec[18] = {p.block, p.direction, p.lo[0], p.lo[1], p.lo[2], p.hi[0], p.hi[1], p.hi[2], p.nb_block,
+                             p.perm[0], p.perm[1], p.perm[2], p.sign[0], p.sign[1], p.sign[2], p.offset[0], p.offset[1], p.offset[2]};
+        patches.insert(patches.end(), rec, rec + 18);
+    }
+    for (int b = 0; b < nb; ++b) {
+        const Block &B = blocks[b];
+        for (int f = 0; f < 6; ++f) {
+            const int a = f / 2, dir = (f & 1) ? a + 1 : -(a + 1);
+            const std::string where = "block " + std::to_string(b) + " (rank " + std::to_string(B.rank) + ", block " +
+                                      std::to_string(B.local) + ") face " + std::to_string(f);
+            bool conn = false;
+            for (const Interface &p : interfaces) conn |= p.block == b && p.direction == dir;
+            const Physical *ph = nullptr;
+            int nph = 0;
+            for (const Physical &p : physical)
+                if (p.block == b && p.direction == dir) ph = &p, ++nph;
+            if (conn && nph) fail(where + " mixes connections and physical boundaries");
+            if (conn) {
+                bc[6 * b + f] = 3;
+                continue;
+            }
+            if (nph != 1) fail(where + (nph ? " is split into several physical patches" : " has no boundary or connection"));
+            for (int d = 0; d < 3; ++d)
+                if (d != a && (ph->sub[d] != 0 || ph->sup[d] + 1 != B.nn[d])) fail(where + ": a physical patch must cover the whole face");
+            if (ph->bc_name == "Farfield") bc[6 * b + f] = 1;
+            else if (ph->bc_name == "Solid_Surface") bc[6 * b + f] = 2;
+            else fail(where + ": boundary '" + ph->bc_name + "' is not supported (Farfield and Solid_Surface are)");
+        }
+    }
+}
+
 void Geometry::face_topology(int *bc, int *nbr, int *transform) const {
     const int nb = (int)blocks.size();
     for (int b = 0; b < nb; ++b) {

@@ -51,6 +51,10 @@ struct Geometry {
     // patch or one connection patch covering the whole face; codes of
     // include/mercury_hip.h, neighbour and IndexTransform (perm, sign, offset)
     void face_topology(int *bc, int *nbr, int *transform) const;
+    // face codes (interface where connection patches lie, the physical code of a
+    // whole-face Farfield / Solid_Surface patch) and every connection patch as 18
+    // ints (block, direction, lo[3], hi[3], neighbour, perm[3], sign[3], offset[3])
+    void patch_topology(int *bc, std::vector<int> &patches) const;
 };
 
 }  // namespace casefile

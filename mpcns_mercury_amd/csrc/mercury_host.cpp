@@ -2696,6 +2696,19 @@ int mercury_case_topology(const MercuryCase *c, int *bc, int *nbr, int *transfor
     ABI_TRY({ c->G.face_topology(bc, nbr, transform); })
 }
 
+int mercury_case_patch_topology(const MercuryCase *c, int *bc, int capacity, int *npatch, int *patches) {
+    if (!c || !bc || !npatch) {
+        merc::g_last_error = "mercury_case_patch_topology: null argument";
+        return -1;
+    }
+    ABI_TRY({
+        std::vector<int> pt;
+        c->G.patch_topology(bc, pt);
+        *npatch = (int)pt.size() / 18;
+        if (patches && capacity >= *npatch) std::copy(pt.begin(), pt.end(), patches);
+    })
+}
+
 int mercury_case_interfaces(const MercuryCase *c, int *n, int *records) {
     if (!c || !n) {
         merc::g_last_error = "mercury_case_interfaces: null argument";

@@ -32,7 +32,7 @@ ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercu
                "mercury_write_restart", "mercury_read_restart", "mercury_write_rundata", "mercury_read_rundata",
                "mercury_write_tecplot", "mercury_case_read", "mercury_case_destroy", "mercury_case_blocks",
                "mercury_case_block_name", "mercury_case_nodes", "mercury_case_topology", "mercury_case_interfaces",
-               "mercury_case_physical", "mercury_dec_d0", "mercury_dec_d1", "mercury_dec_d2", "mercury_dec_faraday_update",
+               "mercury_case_physical", "mercury_case_patch_topology", "mercury_dec_d0", "mercury_dec_d1", "mercury_dec_d2", "mercury_dec_faraday_update",
                "mercury_dec_max_abs_div_curl", "mercury_dec_max_abs_curl_grad", "mercury_dec_max_abs_div_curl_noalloc")
 
 
@@ -498,6 +498,17 @@ class CaseGeometry:
                         "nb_lo": tuple(r[12:15]), "nb_hi": tuple(r[15:18]), "perm": tuple(r[18:21]),
                         "sign": tuple(r[21:24]), "offset": tuple(r[24:27])})
         return out
+
+    def patch_topology(self):
+        """(bc, patches (n, 18)) for mercury_create_patched: any connection patches."""
+        L = load()
+        ip = ctypes.POINTER(ctypes.c_int)
+        bc = np.zeros(6 * self.nb, dtype=np.int32)
+        n = ctypes.c_int(0)
+        _chk(L.mercury_case_patch_topology(self._h, bc.ctypes.data_as(ip), 0, ctypes.byref(n), None))
+        pt = np.zeros((max(n.value, 1), 18), dtype=np.int32)
+        _chk(L.mercury_case_patch_topology(self._h, bc.ctypes.data_as(ip), n.value, ctypes.byref(n), pt.ctypes.data_as(ip)))
+        return bc, pt[:n.value]
 
     def physical(self):
         """Physical patches: dicts with block, direction, bc_id, lo, hi (half-open node box)."""

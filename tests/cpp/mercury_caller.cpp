@@ -56,6 +56,9 @@ int main(int argc, char **argv) {
         topo.bc = get_n<int>(in, 6 * nb);
         topo.nbr = get_n<int>(in, 6 * nb);
         topo.transform = get_n<int>(in, 54 * nb);
+        const int npatch = get<int>(in);
+        topo.rule = get<int>(in);
+        topo.patches = get_n<int>(in, 18 * (size_t)npatch);
         size_t nn = 0;
         for (int b = 0; b < nb; ++b) nn += 3 * (size_t)(dims[3 * b] + 1) * (dims[3 * b + 1] + 1) * (dims[3 * b + 2] + 1);
         const std::vector<double> nodes = get_n<double>(in, nn);
@@ -86,7 +89,7 @@ int main(int argc, char **argv) {
         solver.Prepare();
         std::vector<double> dts;
         solver.Advance(std::stoi(argv[3]), std::stoi(argv[4]), &dts);
-        halo.sync_group("Ucell");  // idempotent after a step: ghosts already hold these values
+        halo.sync_group("Ucell");  // idempotent after a step (per-face connections, patches under the position rule)
         const double divb = solver.NormalisedDivB();
         const double dc = DEC::max_abs_div_curl(field, DEC::EdgeFormNames{"E_xi", "E_eta", "E_zeta"});
         // d1 d0 of the node scratch field: the reference's DEC chain identity on solver storage

@@ -16,6 +16,9 @@ def write_input(path, case, fields):
         f.write(case.bc_array().astype("<i4").tobytes())
         f.write(case.nbr_array().astype("<i4").tobytes())
         f.write(case.trans_array().astype("<i4").tobytes())
+        pt = np.zeros((0, 18), np.int32) if case.patches is None else np.asarray(case.patches, dtype=np.int32).reshape(-1, 18)
+        f.write(struct.pack("<ii", len(pt), int(case.patch_rule)))
+        f.write(pt.astype("<i4").tobytes())
         f.write(case.nodes_flat().astype("<f8").tobytes())
         f.write(bytes(ctypes.string_at(ctypes.addressof(case.phys), ctypes.sizeof(case.phys))))
         dip = case.dipole_array()

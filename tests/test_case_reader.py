@@ -103,6 +103,10 @@ def test_reference_sample_geometry(samples, which, nrank, nblock):
         assert nperiodic > 0
         with pytest.raises(hip.MercuryError, match="split into"):
             geo.topology()
+        # as patches: every face a connection, the reader's records in order
+        bc, pt = geo.patch_topology()
+        assert (bc == cases.BC_INTERFACE).all()
+        np.testing.assert_array_equal(pt, cases.patches_of_geometry(geo))
     finally:
         geo.close()
 

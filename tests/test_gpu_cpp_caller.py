@@ -18,11 +18,29 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CALLER = os.path.join(ROOT, "tests", "cpp", "mercury_caller")
 
 
-@pytest.mark.parametrize("which", ["cube", "shell_dipole"])
+def _patched_sample(tmp_path):
+    """The reference's 8-block sample geometry as patches (position rule)."""
+    import tarfile
+    from mpcns_mercury_amd import hip
+    with tarfile.open(os.path.join(ROOT, "tests", "golden", "z0_null_case_geometry.tar.gz")) as t:
+        t.extractall(tmp_path, members=[m for m in t.getmembers() if "Zone.Identifier" not in m.name])
+    geo = hip.CaseGeometry(tmp_path / "geometry_3")
+    try:
+        return cases.cube_on_geometry(geo, cases.cartesian_periodic_cube(64, g=4), rule=1)
+    finally:
+        geo.close()
+
+
+@pytest.mark.parametrize("which", ["cube", "shell_dipole", "patched_sample"])
 def test_cpp_caller_steps_like_the_oracle(oracle_mod, tmp_path, which):
     if not os.path.exists(CALLER):
         pytest.fail(f"{CALLER} is missing: run __graft_entry__.build()")
-    case = cases.make_periodic_cube(n=(10, 8, 6), g=4) if which == "cube" else cases.make_shell(scale=0.1, dipole=True)
+    if which == "cube":
+        case = cases.make_periodic_cube(n=(10, 8, 6), g=4)
+    elif which == "shell_dipole":
+        case = cases.make_shell(scale=0.1, dipole=True)
+    else:
+        case = _patched_sample(tmp_path)
     nsteps = 4
     print(f"[case] C++ caller {case.name} {nsteps} steps", flush=True)
     o = oracle_mod.Oracle(case)
