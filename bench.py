@@ -154,6 +154,8 @@ def main():
                     help="steps of the three-species resistive shell in the mixed form (0: skip)")
     ap.add_argument("--cut-shell", type=int, default=1,
                     help="also time the six sectors cut into 24 interface-glued blocks (0: skip)")
+    ap.add_argument("--patched", type=int, default=1,
+                    help="also time the reference's 8-block sample geometry as patch connections against its one-block cube (0: skip)")
     ap.add_argument("--forms", default="fp64_plain,mixed,fp64_fused",
                     help="comma list of forms to run (fp64_fused is always run: it gives the value)")
     args = ap.parse_args()
@@ -274,6 +276,45 @@ def main():
                                                              "sync_bcell", "sync_jcell")}
         cg.close()
         print(f"[bench] cut shell setup+run {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    # the reference's 8-block sample geometry (Cartesian blocks of one periodic 64^3 cube
+    # with every axis permutation and reversal, split faces) as patch connections, under
+    # the node-index rule (the reference's halo) and the position rule, against the
+    # one-block cube of the same cells
+    patched = None
+    if args.patched:
+        import tarfile
+        import tempfile
+        t0 = time.perf_counter()
+        patched = {}
+        with tempfile.TemporaryDirectory() as td:
+            with tarfile.open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                           "z0_null_case_geometry.tar.gz")) as tf:
+                tf.extractall(td, members=[m for m in tf.getmembers() if "Zone.Identifier" not in m.name])
+            from mpcns_mercury_amd.hip import CaseGeometry  # noqa: E402
+            geo = CaseGeometry(os.path.join(td, "geometry_3"))
+            cube = cases.cartesian_periodic_cube(64, g=4)
+            runs = [("one_block_cube", cube)] + [(f"patched_rule{r}", cases.cube_on_geometry(geo, cube, rule=r)) for r in (0, 1)]
+            geo.close()
+        for label, pc in runs:
+            pg = Mercury(pc)
+            pg.set_form(1)
+            pg.prepare()
+            pg.graph_capture(0)
+            pg.graph_launch(args.warmup)
+            pg.synchronize()
+            t1 = time.perf_counter()
+            pg.graph_launch(args.steps)
+            pg.synchronize()
+            el = time.perf_counter() - t1
+            patched[label] = {"blocks": pc.nb, "cells": pc.ncells(), "ms_per_step": el * 1e3 / args.steps,
+                              "cell_updates_per_s": pc.ncells() * args.steps / el, "divB_normalised": pg.divb()[0]}
+            if pc.patches is not None:
+                patched[label]["patches"] = int(len(pc.patches))
+                patched[label]["sync_ms"] = {op: pg.time_op(op, 20) for op in ("sync_ucell", "sync_bface", "sync_eedge")}
+            print(f"[bench] {label}: {pc.nb} blocks, {el * 1e3 / args.steps:.3f} ms/step, div B {patched[label]['divB_normalised']:.2e}",
+                  file=sys.stderr, flush=True)
+            pg.close()
+        print(f"[bench] patched sample setup+run {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     # visualisation output of the stepped state (Tecplot, the Mercury variable list at the
     # nodes): node means on the device, file written by the host; outside the timed steps
     output = None
@@ -549,6 +590,7 @@ def main():
         "resistive": resist,
         "ensemble": ensemble,
         "cut_shell": cut,
+        "patched_sample": patched,
         "resistive_shell": rshell,
         "mixed_horizon": horizon,
         "output": output,
