@@ -646,3 +646,96 @@ def cube_on_geometry(geo, cube: Case, rule: int = 0, name: str | None = None) ->
     return Case(name=name or f"{cube.name}_on_{geo.nb}_blocks", g=g, blocks=blocks, phys=cube.phys, fields=fields, A=A, Bu=cube.Bu,
                 steps=cube.steps, integrator=cube.integrator, meta=dict(cube.meta), patches=patches_of_geometry(geo),
                 patch_rule=rule)
+
+
+def _orient_map(perm, sign, m):
+    """Node-index map of a block reorientation: new[perm[a]] = sign[a] * old[a] + c[a]."""
+    c = [0 if sign[a] > 0 else m[a] for a in range(3)]
+    return list(perm), list(sign), c
+
+
+def _reorient_box_array(arr, perm, sign, loc_delta, lead=1):
+    """Values on the box [-g, m+g] of a block, rearranged to the reoriented
+    block's box: along a reversed axis a DOF between nodes (delta 1) at index i
+    goes to m-1-i, a node-like one to m-i (the box's extra last point of a
+    cell-like axis is carried over unchanged)."""
+    out = arr
+    for a in range(3):
+        if sign[a] > 0:
+            continue
+        ax = lead + a
+        n = out.shape[ax]
+        idx = np.arange(n)
+        idx = idx[::-1].copy() if loc_delta[a] == 0 else np.r_[np.arange(n - 2, -1, -1), n - 1]
+        out = np.take(out, idx, axis=ax)
+    inv = [0, 0, 0]
+    for a in range(3):
+        inv[perm[a]] = a
+    return np.ascontiguousarray(np.transpose(out, list(range(lead)) + [lead + inv[q] for q in range(3)]))
+
+
+def reorient_blocks(case: Case, orient: dict, rule: int = 0, name: str | None = None) -> Case:
+    """The same case with the index axes of some blocks permuted and reversed:
+    orient[b] = (perm, sign) (new axis perm[a] = old axis a, direction sign[a]).
+    Node coordinates, the cell fields, the edge potential (signed along reversed
+    axes) and the face codes move with the axes; every connection (periodic
+    faces and 1:1 interfaces) becomes a patch whose transform composes the two
+    blocks' reorientations, so the connections between reoriented blocks rotate
+    and mirror.  The physical problem is unchanged."""
+    R = {}
+    blocks, fields, A = [], [], []
+    for b, blk in enumerate(case.blocks):
+        perm, sign = orient.get(b, ((0, 1, 2), (1, 1, 1)))
+        m = blk.dims
+        R[b] = _orient_map(perm, sign, m)
+        nd = [0, 0, 0]
+        for a in range(3):
+            nd[perm[a]] = m[a]
+        nodes = _reorient_box_array(blk.nodes, perm, sign, (0, 0, 0))
+        fl = {k: _reorient_box_array(v, perm, sign, (1, 1, 1)) for k, v in case.fields[b].items()}
+        fields.append(fl)
+        if case.A is not None:
+            Ab = np.zeros((3,) + tuple(mm + 1 + 2 * case.g for mm in nd))
+            for a in range(3):
+                delta = [0, 0, 0]
+                delta[a] = 1
+                Ab[perm[a]] = sign[a] * _reorient_box_array(case.A[b][a], perm, sign, delta, lead=0)
+            A.append(Ab)
+        bc = [0] * 6
+        for f in range(6):
+            a, side = f // 2, f & 1
+            nside = side if sign[a] > 0 else 1 - side
+            c = blk.bc[f]
+            bc[2 * perm[a] + nside] = BC_INTERFACE if c in (BC_PERIODIC, BC_INTERFACE) else c
+        blocks.append(Block(dims=tuple(nd), bc=tuple(bc), nodes=nodes))
+    rows = []
+    for row in patches_of_case(case):
+        b, nb = int(row[0]), int(row[8])
+        off = row[15:18]
+        pb, sb, cb = R[b]
+        pn, sn, cn = R[nb]
+        # new-b index q -> old-b p = Rb^-1(q) -> old-nb p + off -> new-nb Rn(p + off)
+        lo, hi = row[2:5], row[5:8]
+        corners = []
+        for q in ([lo[0], lo[1], lo[2]], [hi[0] - 1, hi[1] - 1, hi[2] - 1]):
+            r = [0, 0, 0]
+            for a in range(3):
+                r[pb[a]] = sb[a] * q[a] + cb[a]
+            corners.append(r)
+        nlo = [min(corners[0][a], corners[1][a]) for a in range(3)]
+        nhi = [max(corners[0][a], corners[1][a]) + 1 for a in range(3)]
+        perm, sign, offs = [0, 0, 0], [0, 0, 0], [0, 0, 0]
+        for ax in range(3):  # new-b axis ax comes from old-b axis a with perm_b[a] = ax
+            a = pb.index(ax)
+            # old-b p[a] = sb[a] * (q[ax] - cb[a]); old-nb axis a -> new-nb axis pn[a]
+            perm[ax] = pn[a]
+            sign[ax] = sn[a] * sb[a]
+            offs[ax] = sn[a] * (-sb[a] * cb[a] + off[a]) + cn[a]
+        d = int(row[1])
+        a0 = abs(d) - 1
+        nd_ = pb[a0] + 1
+        ndir = nd_ if (d > 0) == (sb[a0] > 0) else -nd_
+        rows.append(_patch_row(b, ndir, nlo, nhi, nb, perm, sign, offs))
+    return Case(name=name or f"{case.name}_reoriented", g=case.g, blocks=blocks, phys=case.phys, fields=fields,
+                A=A if case.A is not None else None, Bu=case.Bu, steps=case.steps, integrator=case.integrator, meta=dict(case.meta),
+                dipoles=case.dipoles, patches=np.array(rows, dtype=np.int32).reshape(-1, PATCH_INTS), patch_rule=rule)
