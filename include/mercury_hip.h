@@ -155,6 +155,23 @@ int mercury_create_patched(MercuryContext **ctx, int nblocks, const int *dims, i
  * every write.  *n receives the count; entries are written when capacity >= *n. */
 int mercury_patch_plan(int nblocks, const int *dims, int nghost, const int *bc, int npatch, const int *patches, int rule, int kind,
                        int level, int stage, long long capacity, long long *n, int *entries);
+/* Host only (no GPU, no context): the singular edges mercury_create_patched
+ * finds (SingularEdgeRegistry::build, src/7_metric/SingularEdgeRegistry.cpp:
+ * 78-268): shared edges whose aliases touch three, or more than four, owner
+ * faces, as on the corner lines of a cubed sphere.  On them the context
+ * assembles E and E_hall from the cells around every alias
+ * (Z4_Mercury/src/physics/MercurySolver_MagExplicit_Ideal.cpp:57-130,
+ * MercurySolver_MagExplicit_Hall.cpp:32-56) and J and dJ from the owner faces
+ * around them (MercurySolver_Current.cpp:59-138), and alpha on their aliases is
+ * length / dual area, the dual area being the polygon through the cells'
+ * centres.  Per entry 8 ints: owner (block, i, j, k, axis), the owner alias's
+ * orientation, cell count, face count; and 4 doubles: the inverse Hodge scale
+ * and the mean edge vector.  *n receives the count; entries are written when
+ * capacity >= *n.  Fails as the reference aborts for an entry without length,
+ * dual area or cell. */
+int mercury_singular_edges(int nblocks, const int *dims, int nghost, const int *bc, int npatch, const int *patches, int rule,
+                           const double *owned_nodes, const MercuryPhysics *phys, int capacity, int *n, int *ints,
+                           double *values);
 /* An ensemble of independent runs in ONE context (no reference counterpart: the
  * reference runs one case per MPI job): run r owns the next run_nblocks[r] blocks
  * of the concatenated block list (dims, bc, owned_nodes as for

@@ -15,8 +15,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmercury_hip.so")
-SOURCES = ["mercury_kernels.hip", "mercury_fused.hip", "mercury_resist.hip", "mercury_output.hip", "mercury_dec.hip", "mercury_geometry.cpp",
-           "mercury_io.cpp", "mercury_case.cpp", "mercury_patches.cpp", "mercury_host.cpp"]
+SOURCES = ["mercury_kernels.hip", "mercury_fused.hip", "mercury_resist.hip", "mercury_output.hip", "mercury_dec.hip", "mercury_singular_kernels.hip",
+           "mercury_geometry.cpp", "mercury_io.cpp", "mercury_case.cpp", "mercury_patches.cpp", "mercury_singular.cpp", "mercury_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CHECKED_LIB = os.path.join(HERE, "libmercury_hip_checked.so")
 # The checked build: the same device code, host code with bounds-checked

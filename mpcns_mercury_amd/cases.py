@@ -336,7 +336,8 @@ def sector_nodes(dims, panel, r0=1.0, r1=4.0, half_angle=np.deg2rad(40.0)):
 def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, integrator: int = 0,
                scale: float = 1.0, hall_coef: float | None = None, resist: float | None = None,
                inflow: float = 1.0, imf=(0.0, 0.0, -0.2), na_fraction: float = 0.01, panels=None,
-               name: str | None = None, resist_profile: str = "window", dipole: bool = False, he: bool = False) -> Case:
+               name: str | None = None, resist_profile: str = "window", dipole: bool = False, he: bool = False,
+               half_angle: float = np.deg2rad(40.0)) -> Case:
     """Two-fluid Hall MHD: H+ and Na+ (mass ratio 1:23, density ratio 1:0.01) plus
     electron pressure (ambipolar term), inflow u = (-1,0,0), IMF B = (0,0,-0.2),
     the fluid side of a perfectly conducting sphere at r = 1 on every block's
@@ -378,7 +379,7 @@ def make_shell(blocks=C2_BLOCKS, g: int = 4, seed: int = 1, steps: int = 200, in
     blks, fields = [], []
     for ib, dims0 in enumerate(blocks):
         dims = tuple(max(4, int(round(d * scale))) for d in dims0)
-        nodes = sector_nodes(dims, _PANELS[(panels[ib] if panels is not None else ib) % 6])
+        nodes = sector_nodes(dims, _PANELS[(panels[ib] if panels is not None else ib) % 6], half_angle=half_angle)
         blks.append(Block(dims=dims, bc=(BC_WALL,) + (BC_FARFIELD,) * 5, nodes=nodes))
         I, J, K = _box_index(dims, g)
         key = (((ib * 4096 + (I + g)) * 4096 + (J + g)) * 4096 + (K + g)).astype(np.uint64)
@@ -739,3 +740,107 @@ def reorient_blocks(case: Case, orient: dict, rule: int = 0, name: str | None = 
     return Case(name=name or f"{case.name}_reoriented", g=case.g, blocks=blocks, phys=case.phys, fields=fields,
                 A=A if case.A is not None else None, Bu=case.Bu, steps=case.steps, integrator=case.integrator, meta=dict(case.meta),
                 dipoles=case.dipoles, patches=np.array(rows, dtype=np.int32).reshape(-1, PATCH_INTS), patch_rule=rule)
+
+
+def glue_patches(blocks, tol: float = 1e-9) -> np.ndarray:
+    """Patches joining every interface face of ``blocks`` to the face of
+    another block whose nodes coincide with its own (whole faces, conforming
+    nodes): the IndexTransform maps each node onto the node at the same point
+    and continues the block across the face into the neighbour."""
+    faces = []
+    for b, blk in enumerate(blocks):
+        for f in range(6):
+            if blk.bc[f] != BC_INTERFACE:
+                continue
+            a, side = f // 2, f & 1
+            sl = [slice(None)] * 3
+            sl[a] = blk.dims[a] if side else 0
+            pts = blk.nodes[(slice(None),) + tuple(sl)].reshape(3, -1).T
+            faces.append((b, f, pts))
+    scale = max(np.abs(blk.nodes).max() for blk in blocks)
+    key = lambda x: tuple(np.rint(np.asarray(x) / (tol * scale)).astype(np.int64))
+    rows = []
+    for b, f, pts in faces:
+        blk = blocks[b]
+        a, side = f // 2, f & 1
+        corners = sorted(key(p) for p in pts[[0, -1]])
+        match = None
+        for nb, fn, npts in faces:
+            if (nb, fn) == (b, f) or len(npts) != len(pts):
+                continue
+            if {key(p) for p in npts[[0, -1]]} <= {key(p) for p in npts} and set(corners) <= {key(p) for p in npts}:
+                if {key(p) for p in pts} == {key(p) for p in npts}:
+                    match = (nb, fn)
+                    break
+        if match is None:
+            raise ValueError(f"block {b} face {f}: no coinciding face")
+        nb, fn = match
+        nblk = blocks[nb]
+        where = {}
+        an, sn = fn // 2, fn & 1
+        for idx in np.ndindex(*(nblk.dims[t] + 1 for t in range(3))):
+            if idx[an] == (nblk.dims[an] if sn else 0):
+                where[key(nblk.nodes[(slice(None),) + idx])] = idx
+        p0 = [0, 0, 0]
+        p0[a] = blk.dims[a] if side else 0
+        q0 = where[key(blk.nodes[(slice(None),) + tuple(p0)])]
+        perm, sign = [0, 0, 0], [0, 0, 0]
+        for t in range(3):
+            if t == a:
+                continue
+            p1 = list(p0)
+            p1[t] += 1
+            d = np.subtract(where[key(blk.nodes[(slice(None),) + tuple(p1)])], q0)
+            perm[t] = int(np.nonzero(d)[0][0])
+            sign[t] = int(d[perm[t]])
+        perm[a] = an
+        inward_nb = -1 if sn else 1
+        sign[a] = inward_nb if side else -inward_nb
+        off = [int(q0[perm[t]] - sign[t] * p0[t]) for t in range(3)]
+        lo, hi = [0, 0, 0], [d + 1 for d in blk.dims]
+        lo[a] = blk.dims[a] if side else 0
+        hi[a] = lo[a] + 1
+        rows.append(_patch_row(b, (a + 1) * (1 if side else -1), lo, hi, nb, perm, sign, off))
+    return np.array(rows, dtype=np.int32).reshape(-1, PATCH_INTS)
+
+
+def edge_potential(case: Case, A) -> List[np.ndarray]:
+    """Edge line integrals of a vector potential A(x, y, z) -> (Ax, Ay, Az)
+    (quadratic or lower: Simpson's rule is exact) along the straight edges
+    between owned nodes, per block (3, box) in the layout of Case.A; edges
+    with a node outside the owned range stay 0 (the faces they bound are
+    ghosts)."""
+    out = []
+    g = case.g
+    for blk in case.blocks:
+        nd = np.asarray(blk.nodes, dtype=np.float64)
+        shape = tuple(m + 1 + 2 * g for m in blk.dims)
+        arr = np.zeros((3,) + shape)
+        for a in range(3):
+            lo = [slice(None)] * 3
+            hi = [slice(None)] * 3
+            lo[a] = slice(0, -1)
+            hi[a] = slice(1, None)
+            p, q = nd[(slice(None),) + tuple(lo)], nd[(slice(None),) + tuple(hi)]
+            mid = 0.5 * (p + q)
+            fp, fm, fq = (np.stack(A(*x)) for x in (p, mid, q))
+            val = ((q - p) * (fp + 4.0 * fm + fq) / 6.0).sum(axis=0)
+            sl = tuple(slice(g, g + n) for n in val.shape)
+            arr[(a,) + sl] = val
+        out.append(arr)
+    return out
+
+
+def make_cubed_sphere(n: int = 8, nr: int = 8, g: int = 4, rule: int = 0, **kw) -> Case:
+    """The two-fluid Hall problem of the C2 shell on a cubed sphere: six
+    gnomonic panels of nr x n x n cells (radial, two angular axes) over the
+    whole shell, the perfectly conducting sphere at r = 1 and the far field
+    outside, the panels joined by rotated patches along their sides.  The
+    eight corner lines where three panels meet are singular edges (valence 3)."""
+    kw.setdefault("name", f"cubed_sphere_{n}x{nr}")
+    c = make_shell(blocks=((nr, n, n),) * 6, panels=list(range(6)), g=g, half_angle=np.pi / 4, **kw)
+    for blk in c.blocks:
+        blk.bc = (BC_WALL, BC_FARFIELD) + (BC_INTERFACE,) * 4
+    c.patches = glue_patches(c.blocks)
+    c.patch_rule = rule
+    return c

@@ -29,6 +29,8 @@
 #include "mercury_kernels.h"
 #include "mercury_fused.h"
 #include "mercury_patches.h"
+#include "mercury_singular.h"
+#include "mercury_singular_dev.h"
 #include "mercury_case.h"
 #include "mercury_io.h"
 #include "mercury_output.h"
@@ -715,6 +717,14 @@ public:
     }
     Classes cls;          // shared-DOF classes (owner / aliases) of every location
     AliasClasses alias;
+    // singular edges of a patch-connected context (none otherwise)
+    struct SingularEdges {
+        DBuf<MercSingEntry> e;
+        DBuf<MercSingCell> c;
+        DBuf<MercSingFace> f;
+        int n = 0;
+        singular::Registry R;
+    } sing;
     hipStream_t st = nullptr;      // the stream all of the context's work goes to
     hipStream_t own_st = nullptr;  // the context's own stream (st unless a caller's stream is set)
     // side stream of the fused stage: the fluid RHS runs there while the
@@ -1273,6 +1283,80 @@ public:
         }
         alias.set(std::move(st), std::move(mem), std::move(ms), std::move(max), std::move(ow), std::move(os), std::move(oax));
         build_all_merged();
+        build_singular();
+    }
+
+    // The singular-edge registry from the static geometry on the device; the
+    // entries' inverse Hodge scale replaces alpha on their owned aliases
+    // (SingularEdgeRegistry::build, src/7_metric/SingularEdgeRegistry.cpp:253-266)
+    void build_singular() {
+        const patch::Topo &T = *ptopo;
+        const Field &Fl = *F;
+        const int nb = T.nb();
+        static const int kSlots[19] = {SL_DL, SL_DL + 1, SL_DL + 2, SL_DR, SL_DR + 1, SL_DR + 2, SL_DR + 3, SL_DR + 4,
+                                       SL_DR + 5, SL_DR + 6, SL_DR + 7, SL_DR + 8, SL_JAC, SL_CC, SL_CC + 1, SL_CC + 2,
+                                       SL_AREA, SL_AREA + 1, SL_AREA + 2};
+        std::vector<std::vector<double>> store((size_t)nb * 19);
+        std::vector<singular::BlockGeo> G(nb);
+        for (int b = 0; b < nb; ++b) {
+            for (int u = 0; u < 19; ++u) {
+                std::vector<double> &v = store[(size_t)b * 19 + u];
+                v.resize(Fl.vol(b));
+                Fl.download_block(kSlots[u], b, v.data());
+            }
+            singular::set_block_geo(G[b], T.m[b], T.g, &store[(size_t)b * 19]);
+        }
+        sing.R = singular::build(T, G);
+        set_singular(G);
+    }
+    void set_singular(const std::vector<singular::BlockGeo> &G) {
+        const singular::Registry &R = sing.R;
+        const Field &Fl = *F;
+        std::vector<MercSingEntry> he;
+        std::vector<MercSingCell> hc;
+        std::vector<MercSingFace> hf;
+        for (const singular::Owner &o : R.e) {
+            MercSingEntry x{};
+            x.oq = Fl.gidx(o.dof.b, o.dof.i, o.dof.j, o.dof.k);
+            x.oax = o.dof.f;
+            x.c0 = o.c0, x.c1 = o.c1, x.f0 = o.f0, x.f1 = o.f1;
+            x.osign = o.sign;
+            for (int u = 0; u < 3; ++u) x.cv[u] = o.cv[u];
+            he.push_back(x);
+        }
+        for (const singular::Cell &c : R.c) {
+            const MercBlk &k = Fl.blk[c.dof.b];
+            const MercPhys &Pc = Pb(c.dof.b);
+            const singular::BlockGeo &g = G[c.dof.b];
+            const long long q = g.idx(c.dof.i, c.dof.j, c.dof.k);
+            MercSingCell y{};
+            y.q = Fl.gidx(c.dof.b, c.dof.i, c.dof.j, c.dof.k);
+            y.d = c.dof.f == 0 ? k.sI : (c.dof.f == 1 ? k.sJ : 1);
+            y.blk = c.dof.b;
+            y.w = c.w;
+            y.orient = c.orient;
+            const double x0 = g.cc[0][q], x1 = g.cc[1][q], x2 = g.cc[2][q];
+            y.taper = singular::hall_taper(Pc, x0, x1, x2);
+            y.leta_on = singular::local_eta(Pc, x0, x1, x2, y.leta) ? 1 : 0;
+            hc.push_back(y);
+        }
+        for (const singular::Face &f : R.f) hf.push_back(MercSingFace{Fl.gidx(f.dof.b, f.dof.i, f.dof.j, f.dof.k), f.dof.f, f.coef});
+        sing.n = (int)he.size();
+        sing.e.upload(he);
+        sing.c.upload(hc);
+        sing.f.upload(hf);
+        for (size_t t = 0; t < R.alpha_dof.size(); ++t) {
+            const patch::Dof &d = R.alpha_dof[t];
+            HIPCHK(hipMemcpy(F->hptr[SL_ALPHA + d.f] + Fl.gidx(d.b, d.i, d.j, d.k), &R.alpha_val[t], sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    // E and E_hall (after the alias mean of E and the Hall edge EMF) and the
+    // current or its increment (after the alias mean) on the singular owners
+    void singular_emf(bool mixed) {
+        if (sing.n) launch_singular_emf(st, sing.n, sing.e.p, sing.c.p, Bd(), Fd(), Gd(), P, mixed);
+    }
+    void singular_current(int sB, int sJ) {
+        if (sing.n) launch_singular_current(st, sing.n, sing.e.p, sing.f.p, Fd(), sB, sJ);
     }
     void drop_graph() {
         ++generation;
@@ -1516,6 +1600,7 @@ public:
         run_merged(1u << G_DB);
         launch_jedge(st, R_res.h, Bd(), Fd(), SL_DB, SL_DJ);
         alias.launch(st, Fd(), SL_DJ);
+        singular_current(SL_DB, SL_DJ);
         launch_res_gather(st, rd, Fd(), y, x, w, r_part.p);
     }
 
@@ -1534,6 +1619,7 @@ public:
         // unknown; the derived pass recomputes J everywhere afterwards)
         launch_jedge(st, R_res.h, Bd(), Fd());
         alias.launch(st, Fd(), SL_J);
+        singular_current(SL_B, SL_J);
         launch_res_rhs(st, rd, Fd(), dt, rb, rdg, rw, rwd);
         res_apply(rx, rAp, dt, false, nullptr);
         launch_cg_init(st, n, rb, rAp, rdg, rwd, rr, rp, r_part.p, r_S.p);
@@ -1575,6 +1661,7 @@ public:
     void derived_fused(bool final) {
         launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix(), mon_part(), d_mon);
         alias.launch(st, Fd(), SL_J);
+        singular_current(SL_B, SL_J);
         run_merged(MG_DERIVED_PAIR);
         launch_jcell_t(st, R_cell.h, Bd(), Fd(), Gd(), mix());
         sync(G_JCELL);
@@ -1600,6 +1687,7 @@ public:
         run_merged(1u << G_BFACE, st);
         launch_bcell_jedge(st, R_node.h, Bd(), Fd(), Gd(), mix(), mon_part(), d_mon);
         alias.launch(st, Fd(), SL_J);
+        singular_current(SL_B, SL_J);
         if (alias.n == 0 && !F->connected() && !any_caller_plan()) {
             HIPCHK(hipEventRecord(ev_fork2, st));
             HIPCHK(hipStreamWaitEvent(st2, ev_fork2, 0));
@@ -1641,6 +1729,7 @@ public:
             // non-Hall EMF before the Hall part is added; one pass writes both
             launch_edge_total(st, R_node.h, Bd(), Fd(), Gd(), P, 2, mix());
             alias.launch(st, Fd(), SL_E);
+            singular_emf(mix());
             sync(G_EHALL);
             launch_add_hall_wall(st, R_node.h, Bd(), Fd());
         }
@@ -1656,6 +1745,7 @@ public:
         sync(G_BCELL);
         launch_jedge(st, R_node.h, Bd(), Fd());
         alias.launch(st, Fd(), SL_J);
+        singular_current(SL_B, SL_J);
         sync(G_JEDGE);
         launch_jcell(st, R_cell.h, Bd(), Fd());
         sync(G_JCELL);
@@ -1671,6 +1761,7 @@ public:
         alias.launch(st, Fd(), SL_E);
         launch_hall_cell(st, R_box.h, Bd(), Fd(), P);
         launch_hall_edge(st, R_node.h, Bd(), Fd());
+        singular_emf(false);
         sync(G_EHALL);
         launch_add_hall_wall(st, R_node.h, Bd(), Fd());
         sync(G_EEDGE);
@@ -1916,6 +2007,10 @@ public:
             launch_curl_update(st, R_node.h, Bd(), Fd(), dtp(), 0);
         } else if (op == "alias_e")
             alias.launch(st, Fd(), SL_E);
+        else if (op == "singular_emf")
+            singular_emf(false);
+        else if (op == "singular_j")
+            singular_current(SL_B, SL_J);
         else if (op == "alias_j")
             alias.launch(st, Fd(), SL_J);
         else if (op == "timestep") launch_timestep(st, R_cell.h, Bd(), Fd(), P, d_dt, nruns);
@@ -2086,6 +2181,55 @@ int mercury_patch_plan(int nblocks, const int *dims, int nghost, const int *bc, 
             const merc::patch::Dof &d = e[t].dst, &s = e[t].src;
             const int v[11] = {d.f, d.b, d.i, d.j, d.k, s.f, s.b, s.i, s.j, s.k, e[t].sign};
             std::copy(v, v + 11, o);
+        }
+    })
+}
+
+int mercury_singular_edges(int nblocks, const int *dims, int nghost, const int *bc, int npatch, const int *patches, int rule,
+                           const double *owned_nodes, const MercuryPhysics *phys, int capacity, int *n, int *ints,
+                           double *values) {
+    if (!n || !dims || !owned_nodes || !phys || nblocks <= 0) {
+        merc::g_last_error = "mercury_singular_edges: invalid argument";
+        return -1;
+    }
+    ABI_TRY({
+        const merc::patch::Topo T = merc::patch::make_topo(nblocks, dims, nghost, bc, npatch, patches, rule);
+        T.validate();
+        MercPhys P;
+        std::memcpy(&P, phys, sizeof(MercPhys));
+        // the static geometry each block's context computes, kept on the host
+        std::vector<std::vector<double>> store((size_t)nblocks * 19);
+        std::vector<merc::singular::BlockGeo> G(nblocks);
+        long long noff = 0;
+        for (int b = 0; b < nblocks; ++b) {
+            const int m[3] = {dims[3 * b], dims[3 * b + 1], dims[3 * b + 2]};
+            merc::BlockGeom geo(m, nghost, owned_nodes + noff);
+            noff += 3LL * (m[0] + 1) * (m[1] + 1) * (m[2] + 1);
+            struct W : merc::SlotWriter {
+                std::vector<double> *out;
+                void put(int slot, const std::vector<double> &v) override {
+                    int u = -1;
+                    if (slot >= SL_DL && slot < SL_DL + 3) u = slot - SL_DL;
+                    else if (slot >= SL_DR && slot < SL_DR + 9) u = 3 + slot - SL_DR;
+                    else if (slot == SL_JAC) u = 12;
+                    else if (slot >= SL_CC && slot < SL_CC + 3) u = 13 + slot - SL_CC;
+                    else if (slot >= SL_AREA && slot < SL_AREA + 3) u = 16 + slot - SL_AREA;
+                    if (u >= 0) out[u] = v;
+                }
+            } w;
+            w.out = &store[(size_t)b * 19];
+            geo.compute(P, w);
+            merc::singular::set_block_geo(G[b], T.m[b], nghost, &store[(size_t)b * 19]);
+        }
+        const merc::singular::Registry R = merc::singular::build(T, G);
+        *n = (int)R.e.size();
+        if (!ints || !values || capacity < *n) return 0;
+        for (size_t t = 0; t < R.e.size(); ++t) {
+            const merc::singular::Owner &o = R.e[t];
+            const int v[8] = {o.dof.b, o.dof.i, o.dof.j, o.dof.k, o.dof.f, o.sign, o.c1 - o.c0, o.f1 - o.f0};
+            std::copy(v, v + 8, ints + 8 * t);
+            const double d[4] = {o.ih, o.cv[0], o.cv[1], o.cv[2]};
+            std::copy(d, d + 4, values + 4 * t);
         }
     })
 }

@@ -711,5 +711,11 @@ EdgeClasses edge_classes(const Topo &T) {
     return E;
 }
 
+EdgeClasses face_classes(const Topo &T) {
+    EdgeClasses E;
+    classes(T, 1, E.start, E.mem, E.sign);
+    return E;
+}
+
 }  // namespace patch
 }  // namespace merc

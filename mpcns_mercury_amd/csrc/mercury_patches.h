@@ -64,6 +64,9 @@ struct EdgeClasses {
     std::vector<int> sign;
 };
 EdgeClasses edge_classes(const Topo &T);
+// the same for shared face DOFs (f = axis; sign +1 or -1 as the face's
+// boundary-edge stencil equals or opposes the owner's)
+EdgeClasses face_classes(const Topo &T);
 
 }  // namespace patch
 }  // namespace merc

@@ -20,7 +20,7 @@ _lib = None
 
 # every symbol declared in include/mercury_hip.h
 ABI_SYMBOLS = ("mercury_last_error", "mercury_version", "mercury_create", "mercury_create_connected", "mercury_create_patched",
-               "mercury_patch_plan", "mercury_create_ensemble", "mercury_run_dt", "mercury_destroy",
+               "mercury_patch_plan", "mercury_singular_edges", "mercury_create_ensemble", "mercury_run_dt", "mercury_destroy",
                "mercury_field_info",
                "mercury_block_volume", "mercury_block_shape", "mercury_field_device", "mercury_stream", "mercury_set_stream", "mercury_total_volume", "mercury_upload", "mercury_download",
                "mercury_init_bface", "mercury_init_bface_dipoles", "mercury_prepare", "mercury_step", "mercury_op", "mercury_timestep",
@@ -57,6 +57,8 @@ def load():
                                          ctypes.POINTER(MercuryPhysics)]
     L.mercury_patch_plan.argtypes = [ctypes.c_int, ip, ctypes.c_int, ip, ctypes.c_int, ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong), ip]
+    L.mercury_singular_edges.argtypes = [ctypes.c_int, ip, ctypes.c_int, ip, ctypes.c_int, ip, ctypes.c_int, dp,
+                                         ctypes.POINTER(MercuryPhysics), ctypes.c_int, ip, ip, dp]
     L.mercury_destroy.argtypes = [vp]
     L.mercury_create_ensemble.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ip, ip, ctypes.c_int, ip, ip, ip, dp,
                                           ctypes.POINTER(MercuryPhysics)]
@@ -190,6 +192,27 @@ def patch_plan(case, kind: str, level: int, stage: int) -> np.ndarray:
     out = np.zeros((n.value, 11), dtype=np.int32)
     _chk(L.mercury_patch_plan(*args, n.value, ctypes.byref(n), out.ctypes.data_as(ip)))
     return out
+
+
+def singular_edges(case) -> list:
+    """The singular edges a patch-connected context assembles specially (host
+    only, no GPU): per entry the owner DOF (block, i, j, k, axis), its
+    orientation sign, the incident cell and face counts, the inverse Hodge
+    scale and the mean edge vector."""
+    L = load()
+    ip = ctypes.POINTER(ctypes.c_int)
+    dims, bc = case.dims_array(), case.bc_array()
+    pt = np.ascontiguousarray(case.patches, dtype=np.int32).reshape(-1)
+    nodes = case.nodes_flat()
+    n = ctypes.c_int(0)
+    args = (case.nb, dims.ctypes.data_as(ip), case.g, bc.ctypes.data_as(ip), len(pt) // 18, pt.ctypes.data_as(ip), int(case.patch_rule),
+            _dp(nodes), ctypes.byref(case.phys))
+    _chk(L.mercury_singular_edges(*args, 0, ctypes.byref(n), None, None))
+    iv = np.zeros((max(n.value, 1), 8), dtype=np.int32)
+    dv = np.zeros((max(n.value, 1), 4))
+    _chk(L.mercury_singular_edges(*args, n.value, ctypes.byref(n), iv.ctypes.data_as(ip), _dp(dv)))
+    return [dict(owner=tuple(int(x) for x in iv[t, :5]), sign=int(iv[t, 5]), cells=int(iv[t, 6]), faces=int(iv[t, 7]),
+                 inverse_hodge=float(dv[t, 0]), edge_vector=dv[t, 1:4].copy()) for t in range(n.value)]
 
 
 class Mercury:

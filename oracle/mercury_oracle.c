@@ -149,9 +149,12 @@ typedef struct {
     /* during a patch sync: per field axis and block, the box points the halo
      * regions wrote (rule 1 completes the others) */
     char *wm[3][64];
+    /* singular edges (patch contexts): entries, their cells and faces */
+    struct OSingRegT *singp;
 } Orc;
 
 #define IX(b, i, j, k) ((((long)(i) + (b)->g) * (b)->N[1] + ((j) + (b)->g)) * (b)->N[2] + ((k) + (b)->g))
+static void singular_current(Orc *o, int sB, int sJ);
 #define NX(b, i, j, k) ((((long)(i) + (b)->g + 1) * (b)->Nn[1] + ((j) + (b)->g + 1)) * (b)->Nn[2] + ((k) + (b)->g + 1))
 #define F(b, s, i, j, k) ((b)->f[(s)][IX(b, i, j, k)])
 #define XN(b, i, j, k) ((b)->x[NX(b, i, j, k)])
@@ -1766,6 +1769,7 @@ static void update_derived(Orc *o) {
     sync_group(o, G_BCELL);
     calc_jedge(o);
     alias_reduce_edges(o, S_J);
+    singular_current(o, S_B, S_J);
     sync_group(o, G_JEDGE);
     calc_jcell(o);
     sync_group(o, G_JCELL);
@@ -1981,6 +1985,306 @@ static void add_sources(Orc *o) {
                         F(b, sr + 4, i, j, k) += P->a4 * rho * u2 * b2;
                     }
                 }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* singular edges (valence 3 or more than 4: the corner lines of a cubed      */
+/* sphere)                                                                    */
+/* ------------------------------------------------------------------------- */
+/* SingularEdgeRegistry::build (src/7_metric/SingularEdgeRegistry.cpp:78-268):
+ * an edge class of three or more members whose aliases touch 3 or more than 4
+ * distinct owner faces of positive area.  Per entry: the cells around every
+ * alias (owned, |Jac| > 1e-30, weight = |Jac| / sum), the owner faces around
+ * every alias (with the alias's orientation and the face's incidence on it),
+ * the mean alias length, the orientation-signed mean of the aliases' dr, and
+ * the dual area: the polygon through the cells' centres ordered by angle about
+ * the edge (half the norm of the summed cross products), else sum|Jac| / length;
+ * inverse_hodge = length / dual area, which also replaces alpha on every owned
+ * alias. */
+typedef struct {
+    int ob, oax, osign;
+    long oq;
+    double cv[3], ih;
+    int c0, c1, f0, f1;
+} OSing;
+typedef struct OSingRegT {
+    int nsing, ncell, nface;
+    OSing *e;
+    int *cb, *cax, *cor;
+    long *cq;
+    double *cw;
+    int *fb, *fax;
+    long *fq;
+    double *fcoef;
+} OSingReg;
+
+static void ijk_of(const OBlock *b, long q, int p[3]) {
+    p[2] = (int)(q % b->N[2]) - b->g;
+    p[1] = (int)((q / b->N[2]) % b->N[1]) - b->g;
+    p[0] = (int)(q / ((long)b->N[1] * b->N[2])) - b->g;
+}
+static int in_owned(const OBlock *b, int L, const int p[3]) {
+    int hi[3];
+    owned_hi(b, L, hi);
+    for (int a = 0; a < 3; ++a)
+        if (p[a] < 0 || p[a] >= hi[a]) return 0;
+    return 1;
+}
+/* incidence of a face on an edge of the same block (AssembleSingularEdgeCurrent_,
+ * Z4_Mercury/src/physics/MercurySolver_Current.cpp:64-110) */
+static int incidence(int ea, const int e[3], int fa, const int f[3]) {
+    if (ea == 0) {
+        if (fa == 1 && f[0] == e[0] && f[1] == e[1]) return f[2] == e[2] - 1 ? 1 : (f[2] == e[2] ? -1 : 0);
+        if (fa == 2 && f[0] == e[0] && f[2] == e[2]) return f[1] == e[1] ? 1 : (f[1] == e[1] - 1 ? -1 : 0);
+    } else if (ea == 1) {
+        if (fa == 0 && f[0] == e[0] && f[1] == e[1]) return f[2] == e[2] ? 1 : (f[2] == e[2] - 1 ? -1 : 0);
+        if (fa == 2 && f[1] == e[1] && f[2] == e[2]) return f[0] == e[0] - 1 ? 1 : (f[0] == e[0] ? -1 : 0);
+    } else {
+        if (fa == 0 && f[0] == e[0] && f[2] == e[2]) return f[1] == e[1] - 1 ? 1 : (f[1] == e[1] ? -1 : 0);
+        if (fa == 1 && f[1] == e[1] && f[2] == e[2]) return f[0] == e[0] ? 1 : (f[0] == e[0] - 1 ? -1 : 0);
+    }
+    return 0;
+}
+static int face_is_alias(const Orc *o, int b, int fa, long fq) {
+    for (int L = L_FX; L <= L_FZ; ++L)
+        for (int c = 0; c < o->ncls[L]; ++c)
+            for (long t = o->cls_start[L][c] + 1; t < o->cls_start[L][c + 1]; ++t)
+                if (o->cls_b[L][t] == b && o->cls_q[L][t] == fq && o->cls_ax[L][t] == fa) return 1;
+    return 0;
+}
+typedef struct { double ang, p[3]; } OAng;
+static int ang_cmp(const void *x, const void *y) {
+    const double a = ((const OAng *)x)->ang, b = ((const OAng *)y)->ang;
+    return (a > b) - (a < b);
+}
+static int build_singular(Orc *o) {
+    OSingReg *S = (OSingReg *)calloc(1, sizeof(OSingReg));
+    o->singp = S;
+    const int L = L_EX;
+    for (int c = 0; c < o->ncls[L]; ++c) {
+        const long s0 = o->cls_start[L][c], s1 = o->cls_start[L][c + 1];
+        if (s1 - s0 < 3) continue;
+        int nc = 0, nf = 0, nlen = 0;
+        int cbuf[64][4], fbuf[64][5];
+        double cw[64], len = 0.0, dr[3] = {0, 0, 0}, vsum = 0.0, fsum = 0.0;
+        int cax[64], cor[64], fcoef_sign[64];
+        double cen[64][3];
+        int ok = 1;
+        for (long t = s0; t < s1 && ok; ++t) {
+            const int b = o->cls_b[L][t], ax = o->cls_ax[L][t], orient = o->cls_sg[L][t];
+            OBlock *bk = &o->b[b];
+            int e[3];
+            ijk_of(bk, o->cls_q[L][t], e);
+            if (in_owned(bk, L_EX + ax, e)) {
+                const double Lg = F(bk, S_DL + ax, e[0], e[1], e[2]);
+                if (isfinite(Lg) && Lg > 0.0) len += Lg, ++nlen;
+                for (int q = 0; q < 3; ++q) dr[q] += orient * F(bk, S_DR + 3 * ax + q, e[0], e[1], e[2]);
+            }
+            for (int s0c = -1; s0c <= 0; ++s0c)
+                for (int s1c = -1; s1c <= 0; ++s1c) {
+                    int p[3] = {e[0], e[1], e[2]};
+                    const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+                    p[t1] += s0c;
+                    p[t2] += s1c;
+                    int dup = 0;
+                    for (int u = 0; u < nc && !dup; ++u) dup = cbuf[u][0] == b && cbuf[u][1] == p[0] && cbuf[u][2] == p[1] && cbuf[u][3] == p[2];
+                    if (dup || !in_owned(bk, L_CELL, p)) continue;
+                    const double v = fabs(F(bk, S_JAC, p[0], p[1], p[2]));
+                    if (!(isfinite(v) && v > 1.e-30)) continue;
+                    if (nc == 64) { ok = 0; break; }
+                    cbuf[nc][0] = b, cbuf[nc][1] = p[0], cbuf[nc][2] = p[1], cbuf[nc][3] = p[2];
+                    cw[nc] = v, cax[nc] = ax, cor[nc] = orient;
+                    for (int q = 0; q < 3; ++q) cen[nc][q] = F(bk, S_CC + q, p[0], p[1], p[2]);
+                    vsum += v;
+                    ++nc;
+                }
+            /* the four faces around the edge (incident_faces, SingularEdgeRegistry.cpp:43-68) */
+            int fl[4][4];
+            if (ax == 0) {
+                int t0[4][4] = {{1, e[0], e[1], e[2]}, {1, e[0], e[1], e[2] - 1}, {2, e[0], e[1], e[2]}, {2, e[0], e[1] - 1, e[2]}};
+                memcpy(fl, t0, sizeof fl);
+            } else if (ax == 1) {
+                int t0[4][4] = {{0, e[0], e[1], e[2]}, {0, e[0], e[1], e[2] - 1}, {2, e[0], e[1], e[2]}, {2, e[0] - 1, e[1], e[2]}};
+                memcpy(fl, t0, sizeof fl);
+            } else {
+                int t0[4][4] = {{0, e[0], e[1], e[2]}, {0, e[0], e[1] - 1, e[2]}, {1, e[0], e[1], e[2]}, {1, e[0] - 1, e[1], e[2]}};
+                memcpy(fl, t0, sizeof fl);
+            }
+            for (int u = 0; u < 4; ++u) {
+                const int fa = fl[u][0], fp[3] = {fl[u][1], fl[u][2], fl[u][3]};
+                int dup = 0;
+                for (int w = 0; w < nf && !dup; ++w)
+                    dup = fbuf[w][0] == b && fbuf[w][1] == fa && fbuf[w][2] == fp[0] && fbuf[w][3] == fp[1] && fbuf[w][4] == fp[2];
+                if (dup || !in_owned(bk, L_FX + fa, fp)) continue;
+                /* only owner faces (Topology::is_owner): no later member of a face class */
+                if (face_is_alias(o, b, fa, IX(bk, fp[0], fp[1], fp[2]))) continue;
+                const double av = F(bk, S_AREA + fa, fp[0], fp[1], fp[2]);
+                if (!(isfinite(av) && av > 1.e-30)) continue;
+                if (nf == 64) { ok = 0; break; }
+                fbuf[nf][0] = b, fbuf[nf][1] = fa, fbuf[nf][2] = fp[0], fbuf[nf][3] = fp[1], fbuf[nf][4] = fp[2];
+                fcoef_sign[nf] = orient * incidence(ax, e, fa, fp);
+                fsum += av;
+                ++nf;
+            }
+        }
+        if (!ok) return 0;
+        if (!(nf == 3 || nf > 4)) continue;
+        OSing E;
+        E.ob = o->cls_b[L][s0], E.oq = o->cls_q[L][s0], E.oax = o->cls_ax[L][s0], E.osign = o->cls_sg[L][s0];
+        const double plen = nlen ? len / nlen : 0.0;
+        double dual = 0.0;
+        if (nc >= 3) {
+            double ctr[3] = {0, 0, 0}, tg[3] = {dr[0], dr[1], dr[2]};
+            for (int u = 0; u < nc; ++u)
+                for (int q = 0; q < 3; ++q) ctr[q] += cen[u][q];
+            for (int q = 0; q < 3; ++q) ctr[q] /= nc;
+            const double tn = sqrt(tg[0] * tg[0] + tg[1] * tg[1] + tg[2] * tg[2]);
+            if (tn > 0.0)
+                for (int q = 0; q < 3; ++q) tg[q] /= tn;
+            double ref[3] = {1, 0, 0};
+            if (!(fabs(tg[0]) < 0.9)) ref[0] = 0, ref[1] = 1;
+            double uu[3] = {tg[1] * ref[2] - tg[2] * ref[1], tg[2] * ref[0] - tg[0] * ref[2], tg[0] * ref[1] - tg[1] * ref[0]};
+            const double un = sqrt(uu[0] * uu[0] + uu[1] * uu[1] + uu[2] * uu[2]);
+            if (un > 0.0)
+                for (int q = 0; q < 3; ++q) uu[q] /= un;
+            const double vv[3] = {tg[1] * uu[2] - tg[2] * uu[1], tg[2] * uu[0] - tg[0] * uu[2], tg[0] * uu[1] - tg[1] * uu[0]};
+            OAng pts[64];
+            for (int u = 0; u < nc; ++u) {
+                double du = 0.0, dv = 0.0;
+                for (int q = 0; q < 3; ++q) {
+                    du += (cen[u][q] - ctr[q]) * uu[q];
+                    dv += (cen[u][q] - ctr[q]) * vv[q];
+                }
+                pts[u].ang = atan2(dv, du);
+                for (int q = 0; q < 3; ++q) pts[u].p[q] = cen[u][q];
+            }
+            qsort(pts, nc, sizeof(OAng), ang_cmp);
+            double av[3] = {0, 0, 0};
+            for (int u = 0; u < nc; ++u) {
+                const double *a = pts[u].p, *bb = pts[(u + 1) % nc].p;
+                av[0] += a[1] * bb[2] - a[2] * bb[1];
+                av[1] += a[2] * bb[0] - a[0] * bb[2];
+                av[2] += a[0] * bb[1] - a[1] * bb[0];
+            }
+            dual = 0.5 * sqrt(av[0] * av[0] + av[1] * av[1] + av[2] * av[2]);
+        }
+        if (!(dual > 0.0)) dual = plen > 0.0 ? vsum / plen : 0.0;
+        E.ih = dual > 0.0 ? plen / dual : 0.0;
+        for (int q = 0; q < 3; ++q) E.cv[q] = nlen > 0 ? dr[q] / nlen : 0.0;
+        /* validate_or_abort (SingularEdgeRegistry.cpp:270-282) */
+        if (!(plen > 0.0) || !(dual > 0.0) || !isfinite(E.ih) || nc < 1) return 0;
+        (void)fsum;
+        E.c0 = S->ncell, E.c1 = S->ncell + nc, E.f0 = S->nface, E.f1 = S->nface + nf;
+        S->e = (OSing *)realloc(S->e, sizeof(OSing) * (S->nsing + 1));
+        S->e[S->nsing++] = E;
+        S->cb = realloc(S->cb, sizeof(int) * (S->ncell + nc)), S->cax = realloc(S->cax, sizeof(int) * (S->ncell + nc));
+        S->cor = realloc(S->cor, sizeof(int) * (S->ncell + nc)), S->cq = realloc(S->cq, sizeof(long) * (S->ncell + nc));
+        S->cw = realloc(S->cw, sizeof(double) * (S->ncell + nc));
+        for (int u = 0; u < nc; ++u) {
+            const int n = S->ncell + u;
+            S->cb[n] = cbuf[u][0], S->cq[n] = IX(&o->b[cbuf[u][0]], cbuf[u][1], cbuf[u][2], cbuf[u][3]);
+            S->cw[n] = vsum > 0.0 ? cw[u] / vsum : 0.0, S->cax[n] = cax[u], S->cor[n] = cor[u];
+        }
+        S->ncell += nc;
+        S->fb = realloc(S->fb, sizeof(int) * (S->nface + nf)), S->fax = realloc(S->fax, sizeof(int) * (S->nface + nf));
+        S->fq = realloc(S->fq, sizeof(long) * (S->nface + nf)), S->fcoef = realloc(S->fcoef, sizeof(double) * (S->nface + nf));
+        for (int w = 0; w < nf; ++w) {
+            const int n = S->nface + w;
+            S->fb[n] = fbuf[w][0], S->fax[n] = fbuf[w][1], S->fq[n] = IX(&o->b[fbuf[w][0]], fbuf[w][2], fbuf[w][3], fbuf[w][4]);
+            S->fcoef[n] = E.ih * fcoef_sign[w];
+        }
+        S->nface += nf;
+        /* the corrected inverse Hodge on every owned alias */
+        for (long t = s0; t < s1; ++t) {
+            OBlock *bk = &o->b[o->cls_b[L][t]];
+            int e[3];
+            ijk_of(bk, o->cls_q[L][t], e);
+            if (in_owned(bk, L_EX + o->cls_ax[L][t], e)) bk->f[S_ALPHA + o->cls_ax[L][t]][o->cls_q[L][t]] = E.ih;
+        }
+    }
+    return 1;
+}
+
+static double hall_taper_at(const OrcPhys *P, double x, double y, double z) {
+    const double r = sqrt(x * x + y * y + z * z);
+    const double width = P->taper_rmax - P->taper_rmin;
+    if (width <= 0.0) return (r >= P->taper_rmin) ? 1.0 : 0.0;
+    const double mid = 0.5 * (P->taper_rmin + P->taper_rmax);
+    return 0.5 * (1.0 + tanh(2.0 * (r - mid) / width));
+}
+/* AssembleSingularEdgeEMF_NonHall_ (Z4_Mercury/src/physics/MercurySolver_MagExplicit_Ideal.cpp:57-130)
+ * (hall = 0) and AssembleSingularEdgeEMF_HallExplicit_ (MercurySolver_MagExplicit_Hall.cpp:32-56)
+ * (hall = 1): owner = owner sign * sum over the cells of weight * the cell's EMF
+ * along the mean edge vector (assemble_cell_field_to_local_owners,
+ * SingularEdgeRegistry.cpp:288-310).  The electron pressure and the number
+ * limiter take the H and Na species only, as there. */
+static void singular_emf(Orc *o, int hall, int slot0) {
+    const OrcPhys *P = &o->P;
+    const OSingReg *S = o->singp;
+    if (!S) return;
+    for (int n = 0; n < S->nsing; ++n) {
+        const OSing *E = &S->e[n];
+        double sum = 0.0;
+        for (int t = E->c0; t < E->c1; ++t) {
+            const OBlock *b = &o->b[S->cb[t]];
+            const long q = S->cq[t];
+            const double *cv = E->cv;
+            double emf;
+            const double b0 = b->f[S_BC][q], b1 = b->f[S_BC + 1][q], b2 = b->f[S_BC + 2][q];
+            const double j0 = b->f[S_JC][q], j1 = b->f[S_JC + 1][q], j2 = b->f[S_JC + 2][q];
+            const double rho2[NSMAX] = {b->f[S_UH][q], b->f[S_UN][q], 0.0};
+            if (!hall) {
+                const double u0 = b->f[S_UP][q], u1 = b->f[S_UP + 1][q], u2 = b->f[S_UP + 2][q];
+                const double ex = -(u1 * b2 - u2 * b1), ey = -(u2 * b0 - u0 * b2), ez = -(u0 * b1 - u1 * b0);
+                emf = ex * cv[0] + ey * cv[1] + ez * cv[2];
+                if (P->ambipolar_enabled != 0.0) {
+                    const long d = S->cax[t] == 0 ? (long)b->N[1] * b->N[2] : (S->cax[t] == 1 ? (long)b->N[2] : 1L);
+                    const double dp = 0.5 * ((b->f[S_PVH + 3][q + d] + b->f[S_PVN + 3][q + d]) -
+                                             (b->f[S_PVH + 3][q - d] + b->f[S_PVN + 3][q - d]));
+                    const NumInfo num = num_limiter(P, rho2, 2);
+                    emf -= P->pressure_coef * S->cor[t] * dp / num.ne_eff;
+                }
+                const double jedge = j0 * cv[0] + j1 * cv[1] + j2 * cv[2];
+                const double jmag = sqrt(j0 * j0 + j1 * j1 + j2 * j2);
+                if (P->arti_eta_max > 0.0) {
+                    double qq = (jmag - P->J_start) / dmax(P->J_on - P->J_start, 1.e-30);
+                    qq = dmax(0.0, dmin(1.0, qq));
+                    emf += P->arti_eta_max * qq * qq * (3.0 - 2.0 * qq) * jedge;
+                }
+                if (P->local_enabled != 0.0 && P->local_eta_max > 0.0) {
+                    const double dx = b->f[S_CC][q] - P->local_center[0], dy = b->f[S_CC + 1][q] - P->local_center[1],
+                                 dz = b->f[S_CC + 2][q] - P->local_center[2];
+                    const double rr = sqrt(dx * dx + dy * dy + dz * dz);
+                    if (P->local_r_cutoff <= 0.0 || rr < P->local_r_cutoff) emf += P->local_eta_max * exp(-rr / P->local_r_decay) * jedge;
+                }
+            } else {
+                const NumInfo num = num_limiter(P, rho2, 2);
+                const double alpha = P->hall_coef / num.ne_eff;
+                const double ex = alpha * (j1 * b2 - j2 * b1), ey = alpha * (j2 * b0 - j0 * b2), ez = alpha * (j0 * b1 - j1 * b0);
+                const double taper = hall_taper_at(P, b->f[S_CC][q], b->f[S_CC + 1][q], b->f[S_CC + 2][q]);
+                emf = taper * (ex * cv[0] + ey * cv[1] + ez * cv[2]);
+            }
+            sum += S->cw[t] * emf;
+        }
+        o->b[E->ob].f[slot0 + E->oax][E->oq] = E->osign * sum;
+    }
+}
+/* AssembleSingularEdgeCurrent_ (Z4_Mercury/src/physics/MercurySolver_Current.cpp:59-138):
+ * owner = owner sign * sum over the owner faces of inverse_hodge * alias
+ * orientation * incidence * beta * B (assemble_face_triplet_to_local_owners,
+ * SingularEdgeRegistry.cpp:312-330) */
+static void singular_current(Orc *o, int sB, int sJ) {
+    const OSingReg *S = o->singp;
+    if (!S) return;
+    for (int n = 0; n < S->nsing; ++n) {
+        const OSing *E = &S->e[n];
+        double sum = 0.0;
+        for (int t = E->f0; t < E->f1; ++t) {
+            const OBlock *b = &o->b[S->fb[t]];
+            sum += S->fcoef[t] * b->f[S_BETA + S->fax[t]][S->fq[t]] * b->f[sB + S->fax[t]][S->fq[t]];
+        }
+        o->b[E->ob].f[sJ + E->oax][E->oq] = E->osign * sum;
     }
 }
 
@@ -2291,7 +2595,9 @@ static void induction_ct(Orc *o) {
     ambipolar_emf(o);
     arti_emf(o);
     alias_reduce_edges(o, S_E);
+    singular_emf(o, 0, S_E);
     hall_emf(o);
+    singular_emf(o, 1, S_EH);
     sync_group(o, G_EHALL);
     for (int ib = 0; ib < o->nb; ++ib) {
         OBlock *b = &o->b[ib];
@@ -2539,6 +2845,7 @@ static void res_apply(Orc *o, const DofList *d, const double *x, double *y, doub
     sync_group(o, G_DB);
     adj_curl(o, S_DB, S_DJ);
     alias_reduce_edges(o, S_DJ);
+    singular_current(o, S_DB, S_DJ);
     sync_group(o, G_DJ);
     for (int n = 0; n < d->n; ++n) y[n] = DOF(o, d, S_ERES, n) - DOF(o, d, S_ETAR, n) * DOF(o, d, S_DJ, n);
 }
@@ -2573,6 +2880,7 @@ static int res_solve(Orc *o, double dt, int *its_out, double *rel_out) {
     /* Calc_J_Edge on B* (the synchronised state after the explicit update) */
     calc_jedge(o);
     alias_reduce_edges(o, S_J);
+    singular_current(o, S_B, S_J);
     sync_group(o, G_JEDGE);
     for (int q = 0; q < n; ++q) {
         b[q] = 0.0 + DOF(o, &d, S_ETAR, q) * DOF(o, &d, S_J, q);
@@ -2983,6 +3291,11 @@ void orc_destroy(void *h) {
     }
     free(o->patch);
     for (int l = 0; l < 4; ++l) free(o->reg[l]);
+    if (o->singp) {
+        OSingReg *S = o->singp;
+        free(S->e), free(S->cb), free(S->cax), free(S->cor), free(S->cq), free(S->cw);
+        free(S->fb), free(S->fax), free(S->fq), free(S->fcoef), free(S);
+    }
     free(o->b);
     free(o);
 }
@@ -3072,11 +3385,34 @@ void *orc_create_patched(int nb, const int *dims, int g, const int *bc, int npat
         orc_destroy(o);
         return NULL;
     }
+    if (!build_singular(o)) {
+        orc_destroy(o);
+        return NULL;
+    }
     resist_dofs(o);
     return o;
 }
 
 int orc_nslot(void) { return S_NSLOT; }
+
+/* the singular-edge registry of a patch context: the entry count, and per
+ * entry (owner block, i, j, k, axis, owner sign, cells, faces) and
+ * (inverse Hodge, mean edge vector x, y, z) */
+int orc_singular_count(void *h) {
+    const Orc *o = (const Orc *)h;
+    return o->singp ? o->singp->nsing : 0;
+}
+int orc_singular_entry(void *h, int n, int *iv, double *dv) {
+    const Orc *o = (const Orc *)h;
+    if (!o->singp || n < 0 || n >= o->singp->nsing) return -1;
+    const OSing *E = &o->singp->e[n];
+    int p[3];
+    ijk_of(&o->b[E->ob], E->oq, p);
+    iv[0] = E->ob, iv[1] = p[0], iv[2] = p[1], iv[3] = p[2], iv[4] = E->oax, iv[5] = E->osign;
+    iv[6] = E->c1 - E->c0, iv[7] = E->f1 - E->f0;
+    dv[0] = E->ih, dv[1] = E->cv[0], dv[2] = E->cv[1], dv[3] = E->cv[2];
+    return 0;
+}
 long orc_block_volume(void *h, int ib) { return ((Orc *)h)->b[ib].V; }
 
 /* slot lookup by reference field name (Z4_Mercury/include/0_MercuryFieldCatalog.h:70-202,
@@ -3316,6 +3652,9 @@ int orc_op(void *h, const char *op, double arg) {
     else if (!strcmp(op, "induction")) induction_ct(o);
     else if (!strcmp(op, "alias_e")) alias_reduce_edges(o, S_E);
     else if (!strcmp(op, "alias_j")) alias_reduce_edges(o, S_J);
+    else if (!strcmp(op, "singular_e")) singular_emf(o, 0, S_E);
+    else if (!strcmp(op, "singular_ehall")) singular_emf(o, 1, S_EH);
+    else if (!strcmp(op, "singular_j")) singular_current(o, S_B, S_J);
     else if (!strcmp(op, "update")) apply_update(o, arg);
     else if (!strcmp(op, "sync_ucell")) sync_group(o, G_UCELL);
     else if (!strcmp(op, "sync_bface")) sync_group(o, G_BFACE);

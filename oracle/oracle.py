@@ -64,6 +64,10 @@ def lib():
         L.orc_res_solve.argtypes = [vp, ctypes.c_double, ip, dp]
         L.orc_threads.restype = ctypes.c_int
         L.orc_threads.argtypes = []
+        L.orc_singular_count.restype = ctypes.c_int
+        L.orc_singular_count.argtypes = [vp]
+        L.orc_singular_entry.restype = ctypes.c_int
+        L.orc_singular_entry.argtypes = [vp, ctypes.c_int, ip, dp]
         _lib = L
     return _lib
 
@@ -164,6 +168,20 @@ class Oracle:
         m = ctypes.c_double(0.0)
         r = lib().orc_divb(self.h, ctypes.byref(m))
         return r, m.value
+
+    def singular_edges(self):
+        """The singular-edge registry: per entry a dict of the owner DOF
+        (block, i, j, k, axis), its orientation sign, the incident cell and
+        face counts, the inverse Hodge scale and the mean edge vector."""
+        L = lib()
+        out = []
+        for n in range(L.orc_singular_count(self.h)):
+            iv = np.zeros(8, dtype=np.int32)
+            dv = np.zeros(4)
+            L.orc_singular_entry(self.h, n, iv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(dv))
+            out.append(dict(owner=tuple(int(x) for x in iv[:5]), sign=int(iv[5]), cells=int(iv[6]),
+                            faces=int(iv[7]), inverse_hodge=float(dv[0]), edge_vector=dv[1:4].copy()))
+        return out
 
     def res_solve(self, dt):
         """One implicit resistive correction; returns (iterations, relative residual)."""
