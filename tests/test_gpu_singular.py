@@ -32,6 +32,18 @@ def _sphere(rule=0, resistive=False, resist=None):
     return c
 
 
+def _owned(case, name, ib, arr):
+    """The owned DOFs of an edge field (the fused forms leave ghost edges that
+    no halo region writes at older values, the oracle at zero; the update never
+    reads them)."""
+    if name not in EDGES:
+        return arr
+    a = ("xi", "eta", "zeta").index(name.split("_")[1])
+    g, m = case.g, case.blocks[ib].dims
+    sl = tuple(slice(g, g + m[t] + (0 if t == a else 1)) for t in range(3))
+    return arr[(slice(None),) + sl]
+
+
 def _run(oracle_mod, case, form, nstep=3):
     g = Mercury(case)
     o = oracle_mod.Oracle(case)
@@ -43,7 +55,7 @@ def _run(oracle_mod, case, form, nstep=3):
         fields = {}
         for ib in range(case.nb):
             for name in STATE + EDGES:
-                fields[(name, ib)] = (g.get(name, ib), o.get(name, ib))
+                fields[(name, ib)] = (_owned(case, name, ib, g.get(name, ib)), _owned(case, name, ib, o.get(name, ib)))
     finally:
         g.close()
     return dg, do, fields, o
@@ -72,7 +84,7 @@ def test_cubed_sphere_steps_like_the_oracle(oracle_mod, rule, form):
     S = o.singular_edges()
     assert len(S) == 48
     g0 = case.g
-    jmax = max(abs(fields[(f"J_{('xi', 'eta', 'zeta')[e['owner'][4]]}", e["owner"][0])][1][0][
+    jmax = max(abs(o.get(f"J_{('xi', 'eta', 'zeta')[e['owner'][4]]}", e["owner"][0])[0][
         tuple(x + g0 for x in e["owner"][1:4])]) for e in S)
     assert jmax > 0.0
 
