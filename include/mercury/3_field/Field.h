@@ -100,7 +100,7 @@ class Field {
 public:
     // Field(Grid*, Param*, ngg) + MercurySolver::RegisterFields + METRIC::build_field_geometry:
     // the device context with every Mercury field and the static geometry.
-    Field(Grid *grid, const TOPO::Topology &topo, const MercuryPhysics &phys, int ngg) : grid_(grid), ngg_(ngg) {
+    Field(Grid *grid, const TOPO::Topology &topo, const MercuryPhysics &phys, int ngg) : grid_(grid), ngg_(ngg), phys_(phys) {
         if (!grid || grid->num_blocks() == 0) throw std::runtime_error("Field: empty grid");
         if (topo.num_blocks() != grid->num_blocks()) throw std::runtime_error("Field: topology and grid disagree on the block count");
         if (!topo.patches.empty())
@@ -140,9 +140,12 @@ public:
     }
     MercuryContext *context() const { return ctx_; }
     Grid *grid() const { return grid_; }
+    int nghost() const { return ngg_; }
+    const MercuryPhysics &physics() const { return phys_; }
 
 private:
     Grid *grid_ = nullptr;
     int ngg_ = 0;
+    MercuryPhysics phys_;
     MercuryContext *ctx_ = nullptr;
 };

@@ -25,6 +25,7 @@
 #include "MercurySolver.h"
 #include "8_dec/DecDiagnostics.h"
 #include "8_dec/DecOps.h"
+#include "7_metric/SingularEdgeRegistry.h"
 
 namespace {
 template <typename T> T get(std::ifstream &in) {
@@ -70,6 +71,10 @@ int main(int argc, char **argv) {
         grid.SetBlocks(dims, nodes);
         Field field(&grid, topo, phys, g);
         MercurySolver::RegisterFields(&field, g);
+        METRIC::SingularEdgeRegistry singular_edges;
+        singular_edges.build(topo, field, grid, 0);
+        singular_edges.validate_or_abort();
+        std::cout << "[SingularEdgeRegistry] physical singular edges=" << singular_edges.size() << "\n";
         Halo halo(&field, &topo);
         MercurySolver::RegisterHaloFields(&field, &halo);
         MercurySolver solver(&grid, &topo, &field, &halo, nullptr, dip);

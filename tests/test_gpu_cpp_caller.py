@@ -31,7 +31,16 @@ def _patched_sample(tmp_path):
         geo.close()
 
 
-@pytest.mark.parametrize("which", ["cube", "shell_dipole", "patched_sample"])
+def _cubed_sphere():
+    """Six gnomonic panels glued by rotated patches (position rule), with a
+    uniform field plus a uniform current: the corner lines are singular edges."""
+    c = cases.make_cubed_sphere(8, 6, rule=1)
+    c.Bu = (0.05, -0.03, 0.08)
+    c.A = cases.edge_potential(c, lambda x, y, z: (0.01 * y * z, 0.0 * x, -0.02 * (x * x + y * y)))
+    return c
+
+
+@pytest.mark.parametrize("which", ["cube", "shell_dipole", "patched_sample", "cubed_sphere"])
 def test_cpp_caller_steps_like_the_oracle(oracle_mod, tmp_path, which):
     if not os.path.exists(CALLER):
         pytest.fail(f"{CALLER} is missing: run __graft_entry__.build()")
@@ -39,8 +48,10 @@ def test_cpp_caller_steps_like_the_oracle(oracle_mod, tmp_path, which):
         case = cases.make_periodic_cube(n=(10, 8, 6), g=4)
     elif which == "shell_dipole":
         case = cases.make_shell(scale=0.1, dipole=True)
-    else:
+    elif which == "patched_sample":
         case = _patched_sample(tmp_path)
+    else:
+        case = _cubed_sphere()
     nsteps = 4
     print(f"[case] C++ caller {case.name} {nsteps} steps", flush=True)
     o = oracle_mod.Oracle(case)
@@ -54,6 +65,10 @@ def test_cpp_caller_steps_like_the_oracle(oracle_mod, tmp_path, which):
     r = subprocess.run([CALLER, str(inp), str(out), str(nsteps), "0"], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
+    # the registry the drop-in METRIC::SingularEdgeRegistry reads: the eight
+    # corner lines of the cubed sphere, one entry per radial edge
+    nsing = 48 if which == "cubed_sphere" else 0
+    assert f"[SingularEdgeRegistry] physical singular edges={nsing}" in r.stdout
     dts, divb, div_curl, curl_grad, state = cpp_case_io.read_output(out, case, nsteps)
     o.prepare()
     np.testing.assert_array_equal(dts, o.step(nsteps, 0))
