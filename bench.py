@@ -156,6 +156,9 @@ def main():
                     help="also time the six sectors cut into 24 interface-glued blocks (0: skip)")
     ap.add_argument("--patched", type=int, default=1,
                     help="also time the reference's 8-block sample geometry as patch connections against its one-block cube (0: skip)")
+    ap.add_argument("--cubed-sphere", type=int, default=64,
+                    help="cells per panel side of the cubed-sphere line (6 panels of n^3 cells glued by rotated "
+                         "patches, singular corner edges; 0: skip)")
     ap.add_argument("--forms", default="fp64_plain,mixed,fp64_fused",
                     help="comma list of forms to run (fp64_fused is always run: it gives the value)")
     args = ap.parse_args()
@@ -315,6 +318,37 @@ def main():
                   file=sys.stderr, flush=True)
             pg.close()
         print(f"[bench] patched sample setup+run {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    # a whole cubed sphere: six gnomonic panels of n^3 cells (radial x two angular axes)
+    # joined by rotated patches, whose eight corner lines are singular edges (three
+    # panels meeting), under both staggered-DOF rules, FP64 fused form from a graph
+    sphere = None
+    if args.cubed_sphere:
+        from mpcns_mercury_amd.hip import singular_edges  # noqa: E402
+        t0 = time.perf_counter()
+        sphere = {}
+        n = args.cubed_sphere
+        for rule in (0, 1):
+            sc = cases.make_cubed_sphere(n, n, rule=rule)
+            nsing = len(singular_edges(sc))
+            pg = Mercury(sc)
+            pg.set_form(1)
+            pg.prepare()
+            pg.graph_capture(0)
+            pg.graph_launch(args.warmup)
+            pg.synchronize()
+            t1 = time.perf_counter()
+            pg.graph_launch(args.steps)
+            pg.synchronize()
+            el = time.perf_counter() - t1
+            label = f"rule{rule}"
+            sphere[label] = {"panels": sc.nb, "cells": sc.ncells(), "patches": int(len(sc.patches)), "singular_edges": nsing,
+                             "ms_per_step": el * 1e3 / args.steps, "cell_updates_per_s": sc.ncells() * args.steps / el,
+                             "divB_normalised": pg.divb()[0],
+                             "op_ms": {op: pg.time_op(op, 20) for op in ("singular_emf", "singular_j", "sync_eedge")}}
+            print(f"[bench] cubed sphere {n}^3 x 6 {label}: {nsing} singular edges, {el * 1e3 / args.steps:.3f} ms/step, "
+                  f"div B {sphere[label]['divB_normalised']:.2e}", file=sys.stderr, flush=True)
+            pg.close()
+        print(f"[bench] cubed sphere setup+run {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     # visualisation output of the stepped state (Tecplot, the Mercury variable list at the
     # nodes): node means on the device, file written by the host; outside the timed steps
     output = None
@@ -591,6 +625,7 @@ def main():
         "ensemble": ensemble,
         "cut_shell": cut,
         "patched_sample": patched,
+        "cubed_sphere": sphere,
         "resistive_shell": rshell,
         "mixed_horizon": horizon,
         "output": output,
