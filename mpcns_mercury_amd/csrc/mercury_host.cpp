@@ -724,6 +724,10 @@ public:
         DBuf<MercSingFace> f;
         int n = 0;
         singular::Registry R;
+        // Tecplot: stencils of the singular end nodes and their owned aliases
+        DBuf<int> nstart, ncell;
+        int nnode = 0;
+        std::vector<patch::Dof> nalias;
     } sing;
     hipStream_t st = nullptr;      // the stream all of the context's work goes to
     hipStream_t own_st = nullptr;  // the context's own stream (st unless a caller's stream is set)
@@ -1308,6 +1312,13 @@ public:
         }
         sing.R = singular::build(T, G);
         set_singular(G);
+        const singular::NodeStencils NS = singular::node_stencils(T, sing.R);
+        std::vector<int> cq;
+        for (const patch::Dof &c : NS.cells) cq.push_back(Fl.gidx(c.b, c.i, c.j, c.k));
+        sing.nnode = (int)NS.start.size() - 1;
+        sing.nstart.upload(NS.start);
+        sing.ncell.upload(cq);
+        sing.nalias = NS.alias;
     }
     void set_singular(const std::vector<singular::BlockGeo> &G) {
         const singular::Registry &R = sing.R;
@@ -2729,6 +2740,20 @@ int mercury_write_tecplot(MercuryContext *ctx, const char *path, double time, co
         }
         merc::io::TecplotWriter w(path, "flow_field", vnames, zones, time);
         DeviceScratch dnode((size_t)nmax);
+        // singular nodes (the end nodes of singular edges): the |Jac|-weighted mean
+        // over the cells of the physical node instead of the block's own cells
+        // (EvalTecplotSingularNodeAverage_, src/6_io/IOModule_Tecplot.cpp:611-631, 358-365)
+        std::vector<std::vector<double>> smean(vars.size());
+        if (S.sing.nnode > 0) {
+            DeviceScratch dm((size_t)S.sing.nnode);
+            for (size_t v = 0; v < vars.size(); ++v) {
+                if (vars[v].loc != LOC_CELL) continue;
+                launch_node_stencil_mean(S.st, S.sing.nnode, S.sing.nstart.p, S.sing.ncell.p, F.hptr[vars[v].slot], F.hptr[SL_JAC], dm.p);
+                smean[v].resize(S.sing.nnode);
+                HIPCHK(hipMemcpyAsync(smean[v].data(), dm.p, sizeof(double) * S.sing.nnode, hipMemcpyDeviceToHost, S.st));
+            }
+            HIPCHK(hipStreamSynchronize(S.st));
+        }
         std::vector<double> tmp, box;
         for (int b = 0; b < F.nb(); ++b) {
             const MercBlk &k = F.blk[b];
@@ -2753,6 +2778,9 @@ int mercury_write_tecplot(MercuryContext *ctx, const char *path, double time, co
                     for (int i = 0; i < ni; ++i)
                         for (int j = 0; j < nj; ++j)
                             for (int kk = 0; kk < nk; ++kk) dst[file_index(i, j, kk)] = tmp[((size_t)i * nj + j) * nk + kk];
+                    if (!smean[v].empty())
+                        for (const merc::patch::Dof &a : S.sing.nalias)
+                            if (a.b == b) dst[file_index(a.i, a.j, a.k)] = smean[v][a.f];
                 } else {
                     box.resize(F.vol(b));
                     F.download_block(vars[v].slot, b, box.data());

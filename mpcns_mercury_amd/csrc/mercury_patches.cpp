@@ -711,6 +711,15 @@ EdgeClasses edge_classes(const Topo &T) {
     return E;
 }
 
+std::vector<long long> node_reps(const Topo &T, std::vector<long long> &off) {
+    NodeClasses N(T);
+    off = N.base;
+    off.push_back((long long)N.par.size());
+    std::vector<long long> r(N.par.size());
+    for (long long x = 0; x < (long long)r.size(); ++x) r[x] = N.find(x);
+    return r;
+}
+
 EdgeClasses face_classes(const Topo &T) {
     EdgeClasses E;
     classes(T, 1, E.start, E.mem, E.sign);

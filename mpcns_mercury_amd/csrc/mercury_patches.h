@@ -64,6 +64,10 @@ struct EdgeClasses {
     std::vector<int> sign;
 };
 EdgeClasses edge_classes(const Topo &T);
+// the class representative of every owned node (nodes of one class are one
+// physical node), per block in (i, j, k) order, k fastest; off[b] is block b's
+// first entry (off has nb + 1 entries)
+std::vector<long long> node_reps(const Topo &T, std::vector<long long> &off);
 // the same for shared face DOFs (f = axis; sign +1 or -1 as the face's
 // boundary-edge stencil equals or opposes the owner's)
 EdgeClasses face_classes(const Topo &T);

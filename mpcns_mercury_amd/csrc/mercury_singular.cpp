@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <map>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -221,6 +222,47 @@ Registry build(const patch::Topo &T, const std::vector<BlockGeo> &G) {
         }
     }
     return R;
+}
+
+NodeStencils node_stencils(const patch::Topo &T, const Registry &R) {
+    NodeStencils S;
+    if (R.e.empty()) return S;
+    std::vector<long long> off;
+    const std::vector<long long> rep = patch::node_reps(T, off);
+    auto node_rep = [&](int b, int i, int j, int k) {
+        const std::array<int, 3> &m = T.m[b];
+        return rep[off[b] + ((long long)i * (m[1] + 1) + j) * (m[2] + 1) + k];
+    };
+    // per end-node class: the cells of the singular edges ending there
+    std::map<long long, std::set<Key>> cells;
+    for (const Owner &o : R.e) {
+        int q[3] = {o.dof.i, o.dof.j, o.dof.k};
+        const long long r0 = node_rep(o.dof.b, q[0], q[1], q[2]);
+        ++q[o.dof.f];
+        const long long r1 = node_rep(o.dof.b, q[0], q[1], q[2]);
+        if (r0 == r1) throw std::runtime_error("singular edge whose two ends are one node");
+        for (const long long r : {r0, r1})
+            for (int t = o.c0; t < o.c1; ++t) {
+                const patch::Dof &c = R.c[t].dof;
+                cells[r].insert(Key{c.b, c.i, c.j, c.k, 0});
+            }
+    }
+    std::map<long long, int> index;
+    for (const auto &kv : cells) {
+        index[kv.first] = (int)S.start.size() - 1;
+        for (const Key &c : kv.second) S.cells.push_back(patch::Dof{0, std::get<0>(c), std::get<1>(c), std::get<2>(c), std::get<3>(c)});
+        S.start.push_back((int)S.cells.size());
+    }
+    for (int b = 0; b < T.nb(); ++b) {
+        const std::array<int, 3> &m = T.m[b];
+        for (int i = 0; i <= m[0]; ++i)
+            for (int j = 0; j <= m[1]; ++j)
+                for (int k = 0; k <= m[2]; ++k) {
+                    const auto it = index.find(node_rep(b, i, j, k));
+                    if (it != index.end()) S.alias.push_back(patch::Dof{it->second, b, i, j, k});
+                }
+    }
+    return S;
 }
 
 }  // namespace singular

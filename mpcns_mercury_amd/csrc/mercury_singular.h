@@ -56,6 +56,17 @@ double hall_taper(const MercPhys &P, double x, double y, double z);
 // (MercurySolver_MagExplicit_Ideal.cpp:115-124): false where it does not apply
 bool local_eta(const MercPhys &P, double x, double y, double z, double &eta);
 
+// Tecplot node stencils of the singular edges' end nodes
+// (IOModule::BuildTecplotSingularNodeStencils_, src/6_io/IOModule_Tecplot.cpp:430-526):
+// per physical node the cells of every singular edge ending there, in
+// (block, i, j, k) order, and every owned alias of the node
+struct NodeStencils {
+    std::vector<int> start{0};
+    std::vector<patch::Dof> cells;
+    std::vector<patch::Dof> alias;  // f = stencil index
+};
+NodeStencils node_stencils(const patch::Topo &T, const Registry &R);
+
 // throws std::runtime_error for an entry the reference's validate_or_abort
 // rejects (no length, no dual area, no cell)
 Registry build(const patch::Topo &T, const std::vector<BlockGeo> &G);

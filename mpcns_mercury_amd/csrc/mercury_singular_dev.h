@@ -30,3 +30,7 @@ void launch_singular_emf(hipStream_t s, int n, const MercSingEntry *E, const Mer
 // the current (or its increment) owners of every singular edge from the face
 // triplet at slot sB into the edge triplet at slot sJ
 void launch_singular_current(hipStream_t s, int n, const MercSingEntry *E, const MercSingFace *Fc, double *const *F, int sB, int sJ);
+// Tecplot node values of singular nodes: per stencil the |Jac|-weighted mean of
+// the field over its cells (arena indices cq[start[s] .. start[s+1]))
+void launch_node_stencil_mean(hipStream_t s, int n, const int *start, const int *cq, const double *field, const double *jac,
+                              double *out);

@@ -77,7 +77,32 @@ __global__ void __launch_bounds__(64) k_singular_current(int n, const MercSingEn
     F[sJ + x.oax][x.oq] = x.osign * s;
 }
 
+// PrepareTecplotSingularNodeAverages_ (src/6_io/IOModule_Tecplot.cpp:528-609): the
+// |Jac|-weighted mean of a cell field over a singular node's cells, in stencil
+// order; cells of non-finite or non-positive |Jac| are skipped, no weight gives 0
+__global__ void __launch_bounds__(64) k_node_stencil_mean(int n, const int *__restrict__ start, const int *__restrict__ cq,
+                                                          const double *__restrict__ v, const double *__restrict__ jac,
+                                                          double *__restrict__ out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    double sum = 0.0, wsum = 0.0;
+    for (int t = start[s]; t < start[s + 1]; ++t) {
+        const int q = cq[t];
+        const double w = fabs(jac[q]);
+        if (!isfinite(w) || w <= 0.0) continue;
+        sum += w * v[q];
+        wsum += w;
+    }
+    out[s] = wsum > 0.0 ? sum / wsum : 0.0;
+}
+
 }  // namespace
+
+void launch_node_stencil_mean(hipStream_t s, int n, const int *start, const int *cq, const double *field, const double *jac,
+                              double *out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_node_stencil_mean, dim3((n + 63) / 64), dim3(64), 0, s, n, start, cq, field, jac, out);
+}
 
 void launch_singular_emf(hipStream_t s, int n, const MercSingEntry *E, const MercSingCell *C, const MercBlk *B, double *const *F,
                          const float *const *G, const MercPhys &P, bool mix) {

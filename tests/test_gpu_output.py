@@ -127,3 +127,73 @@ def test_tecplot_linear_field_closed_form(tmp_path):
             np.testing.assert_allclose(d["values"][3 + q], want.transpose(2, 1, 0), rtol=2e-7, atol=1e-6)
     finally:
         g.close()
+
+
+def test_tecplot_singular_nodes_of_a_cubed_sphere(tmp_path):
+    """On the corner lines of a cubed sphere (singular edges) a node's value is
+    the |Jac|-weighted mean over the cells of the physical node, from every
+    panel, in (block, i, j, k) order, at every alias of the node
+    (IOModule_Tecplot.cpp:358-365, 430-631); elsewhere the block's own mean."""
+    from mpcns_mercury_amd import hip
+    c = cases.make_cubed_sphere(6, 4, rule=1)
+    c.Bu = (0.05, -0.03, 0.08)
+    c.A = cases.edge_potential(c, lambda x, y, z: (0.01 * y * z, 0.0 * x, -0.02 * (x * x + y * y)))
+    print(f"[case] {c.name} tecplot singular nodes", flush=True)
+    g = Mercury(c)
+    try:
+        g.set_form(1)
+        g.prepare()
+        g.step(2, 0)
+        path = tmp_path / "sphere.plt"
+        g.write_tecplot(str(path), 0.5)
+        f = tf.read_plt(str(path))
+        # the singular points: both ends of every singular edge
+        pts = []
+        for e in hip.singular_edges(c):
+            b, i, j, k, a = e["owner"]
+            q = [i, j, k]
+            q[a] += 1
+            for p in ((i, j, k), tuple(q)):
+                x = c.blocks[b].nodes[(slice(None),) + p]
+                if not any(np.abs(x - y).max() < 1e-12 for y in pts):
+                    pts.append(x)
+        assert len(pts) == 8 * 5
+        # per point: the owned cells having it as a corner, and its node aliases
+        stencil, alias = [], []
+        for x in pts:
+            cells, al = [], []
+            for ib, blk in enumerate(c.blocks):
+                X = blk.nodes
+                hit = np.argwhere(np.abs(X - x[:, None, None, None]).max(axis=0) < 1e-12)
+                for (i, j, k) in hit:
+                    al.append((ib, i, j, k))
+                    for di in (-1, 0):
+                        for dj in (-1, 0):
+                            for dk in (-1, 0):
+                                cc = (i + di, j + dj, k + dk)
+                                if all(0 <= cc[t] < blk.dims[t] for t in range(3)):
+                                    cells.append((ib,) + cc)
+            stencil.append(sorted(set(cells)))
+            alias.append(al)
+        jac = {ib: g.get("Jac", ib)[0] for ib in range(c.nb)}
+        names = ["x", "y", "z"] + [n for _, comps in MERCURY for n in comps]
+        want = {ib: [c.blocks[ib].nodes[a] for a in range(3)] for ib in range(c.nb)}
+        G = c.g
+        for fld, comps in MERCURY:
+            arrs = {ib: g.get(fld, ib) for ib in range(c.nb)}
+            for cc in range(len(comps)):
+                for ib in range(c.nb):
+                    want[ib].append(tf.node_average(arrs[ib][cc], jac[ib], G, c.blocks[ib].dims).copy())
+                for st, al in zip(stencil, alias):
+                    s = w = 0.0
+                    for (ib, i, j, k) in st:
+                        wt = abs(float(jac[ib][i + G, j + G, k + G]))
+                        s += wt * float(arrs[ib][cc][i + G, j + G, k + G])
+                        w += wt
+                    for (ib, i, j, k) in al:
+                        want[ib][-1][i, j, k] = s / w
+        for ib, d in enumerate(f["data"]):
+            for v, (w, got) in enumerate(zip(want[ib], d["values"])):
+                np.testing.assert_array_equal(got, w.astype(np.float32).transpose(2, 1, 0), err_msg=f"{names[v]}[{ib}]")
+    finally:
+        g.close()
