@@ -46,6 +46,40 @@ def _ranges(dims, f):
     return sub, sup
 
 
+def _patch_ranges(row):
+    """(sub, sup) of a patch (18 ints: block, direction, node box lo, hi, neighbour,
+    perm, sign, offset) and of its image on the neighbour, as the reader decodes
+    them (make_index_transform_from_boundary_arrays, src/2_topology/TopologyOps.cpp:
+    118-219): the normal axis fixed, the first in-face axis positive and the second
+    negative; on the neighbour the image axes take the same marks, ascending or
+    descending with the transform's sign."""
+    lo, hi, perm, sign, off = row[2:5], row[5:8], row[9:12], row[12:15], row[15:18]
+    a = abs(int(row[1])) - 1
+    t1, t2 = [d for d in range(3) if d != a]
+    sub, sup, tsub, tsup = [0] * 3, [0] * 3, [0] * 3, [0] * 3
+    for d in range(3):
+        ms, mp = int(lo[d]), int(hi[d]) - 1
+        ts, tp = int(sign[d]) * ms + int(off[d]), int(sign[d]) * mp + int(off[d])
+        mark = -1 if d == t2 else 1
+        sub[d], sup[d] = mark * (ms + 1), mark * (mp + 1)
+        tsub[int(perm[d])], tsup[int(perm[d])] = mark * (ts + 1), mark * (tp + 1)
+    return sub, sup, tsub, tsup
+
+
+def _patch_nodes(case, row, neighbour=False):
+    """Node coordinates of a patch box, or of its image on the neighbour in the
+    patch's own index order."""
+    lo, hi, perm, sign, off = row[2:5], row[5:8], row[9:12], row[12:15], row[15:18]
+    I, J, K = np.meshgrid(*(np.arange(int(lo[d]), int(hi[d])) for d in range(3)), indexing="ij")
+    P = [I, J, K]
+    if not neighbour:
+        return case.blocks[int(row[0])].nodes[:, I, J, K]
+    Q = [None] * 3
+    for d in range(3):
+        Q[int(perm[d])] = int(sign[d]) * P[d] + int(off[d])
+    return case.blocks[int(row[8])].nodes[:, Q[0], Q[1], Q[2]]
+
+
 def _fmt(sub, sup):
     return "\t".join(str(x) for d in range(3) for x in (sub[d], sup[d]))
 
@@ -77,6 +111,8 @@ def write_case(path, case, ranks, binary=True):
                 phys[r][local[b]].append("\t".join(str(x) for d in range(3) for x in (lo[d], hi[d])) + f"\t{BC_ID[name]}")
                 continue
             assert code == cases.BC_INTERFACE, "periodic faces must be written as connections"
+            if case.patches is not None:
+                continue  # written from the patch list below
             n, fo = blk.nbr[f], f ^ 1
             sub, sup = _ranges(blk.dims, f)
             tsub, tsup = _ranges(case.blocks[n].dims, fo)
@@ -89,6 +125,14 @@ def write_case(path, case, ranks, binary=True):
                 if key not in flag:
                     flag[key], flag[rkey] = len(flag) + 1, len(flag) + 2
                 par[r][local[b]].append(f"{_fmt(sub, sup)}\t{ranks[n]}\t{flag[key]}\t{flag[rkey]}\tFluid")
+    if case.patches is not None:
+        assert nr == 1, "patch lists are written for one rank"
+        for row in np.asarray(case.patches).reshape(-1, 18):
+            b, n = int(row[0]), int(row[8])
+            sub, sup, tsub, tsup = _patch_ranges(row)
+            same = np.array_equal(_patch_nodes(case, row), _patch_nodes(case, row, neighbour=True))
+            sgn = 1 if same else -1
+            inner[0][local[b]].append(f"{_fmt(sub, sup)}\t{sgn * (local[b] + 1)}\tFluid\t{_fmt(tsub, tsup)}\t{sgn * (local[n] + 1)}\tFluid")
     for r in range(nr):
         mine = [b for b in range(case.nb) if ranks[b] == r]
         if binary:

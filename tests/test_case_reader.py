@@ -182,3 +182,31 @@ def test_unsupported_boundaries_are_named(tmp_path):
         geo.close()
     with pytest.raises(hip.MercuryError, match="no grid file|cannot open"):
         hip.CaseGeometry(tmp_path / "missing")
+
+
+@pytest.mark.parametrize("binary", [True, False])
+def test_cubed_sphere_through_the_case_files(tmp_path, binary):
+    """A cubed sphere written as the reference's CASE files (rotated connections
+    as inner records) reads back with the same nodes, face codes and, by the
+    reference's IndexTransform rule, the same patch list in the same order; the
+    singular edges found from the read-back topology are the in-memory case's."""
+    c = cases.make_cubed_sphere(6, 4)
+    cf.write_case(tmp_path, c, [0] * c.nb, binary=binary)
+    geo = hip.CaseGeometry(tmp_path, binary=binary)
+    try:
+        assert geo.nb == c.nb and geo.dims == [b.dims for b in c.blocks]
+        for b in range(c.nb):
+            np.testing.assert_array_equal(geo.nodes[b], c.blocks[b].nodes)
+        bc, pt = geo.patch_topology()
+        np.testing.assert_array_equal(bc, c.bc_array())
+        np.testing.assert_array_equal(pt, np.asarray(c.patches).reshape(-1, 18))
+        np.testing.assert_array_equal(cases.patches_of_geometry(geo), pt)
+    finally:
+        geo.close()
+    r = cases.make_cubed_sphere(6, 4)
+    r.patches = pt
+    want = {e["owner"]: e for e in hip.singular_edges(c)}
+    got = {e["owner"]: e for e in hip.singular_edges(r)}
+    assert len(got) == 32 and set(got) == set(want)
+    for k, e in got.items():
+        assert e["inverse_hodge"] == want[k]["inverse_hodge"]
