@@ -1287,7 +1287,11 @@ public:
         }
         alias.set(std::move(st), std::move(mem), std::move(ms), std::move(max), std::move(ow), std::move(os), std::move(oax));
         build_all_merged();
-        build_singular();
+        // a singular edge has three or more aliases: without such a class there
+        // is no registry to build (and no geometry to download)
+        bool candidates = false;
+        for (size_t c = 0; c + 1 < E.start.size(); ++c) candidates |= E.start[c + 1] - E.start[c] >= 3;
+        if (candidates) build_singular();
     }
 
     // The singular-edge registry from the static geometry on the device; the
