@@ -19,10 +19,25 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "mercury_oracle.c")
 PINS = ["tests/test_oracle_pins.py", "tests/test_oracle_term_pins.py", "tests/test_emf_pins.py", "tests/test_dipole_oracle.py",
-        "tests/test_dec_oracle.py", "tests/test_species_oracle.py"]
+        "tests/test_dec_oracle.py", "tests/test_species_oracle.py", "tests/test_singular_oracle.py"]
 
 # (what the mistake is, exact fragment, replacement)
 MUTANTS = [
+    # singular edges (SingularEdgeRegistry.cpp:78-330, MagExplicit_Ideal.cpp:57-130,
+    # MagExplicit_Hall.cpp:32-56, Current.cpp:59-138)
+    ("singular: incidence sign flipped", "if (fa == 1 && f[0] == e[0] && f[1] == e[1]) return f[2] == e[2] - 1 ? 1 : (f[2] == e[2] ? -1 : 0);",
+     "if (fa == 1 && f[0] == e[0] && f[1] == e[1]) return f[2] == e[2] - 1 ? -1 : (f[2] == e[2] ? 1 : 0);"),
+    ("singular: dual area from volume / length", "        if (nc >= 3) {\n            double ctr[3]", "        if (nc >= 30) {\n            double ctr[3]"),
+    ("singular: cell weights not normalised", "S->cw[n] = vsum > 0.0 ? cw[u] / vsum : 0.0", "S->cw[n] = cw[u]"),
+    ("singular: face coefficient without inverse Hodge", "S->fcoef[n] = E.ih * fcoef_sign[w];", "S->fcoef[n] = fcoef_sign[w];"),
+    ("singular: edge vector without orientation", "dr[q] += orient * F(bk, S_DR + 3 * ax + q, e[0], e[1], e[2]);",
+     "dr[q] += F(bk, S_DR + 3 * ax + q, e[0], e[1], e[2]);"),
+    ("singular: Hall taper dropped", "emf = taper * (ex * cv[0] + ey * cv[1] + ez * cv[2]);", "emf = (ex * cv[0] + ey * cv[1] + ez * cv[2]);"),
+    ("singular: ideal EMF sign", "const double ex = -(u1 * b2 - u2 * b1), ey = -(u2 * b0 - u0 * b2), ez = -(u0 * b1 - u1 * b0);",
+     "const double ex = (u1 * b2 - u2 * b1), ey = (u2 * b0 - u0 * b2), ez = (u0 * b1 - u1 * b0);"),
+    ("singular: pressure difference halved twice", "const double dp = 0.5 * ((b->f[S_PVH + 3][q + d]", "const double dp = 0.25 * ((b->f[S_PVH + 3][q + d]"),
+    ("singular: local resistivity decay", "emf += P->local_eta_max * exp(-rr / P->local_r_decay) * jedge;",
+     "emf += P->local_eta_max * exp(-rr * P->local_r_decay) * jedge;"),
     # A7 Scheme_U_ Rusanov flux (MercurySolver_FluidRHS.cpp:32-102)
     ("flux: dissipation dropped", "- 0.5 * r * (Uc[1][m] - Uc[0][m])", "- 0.0 * r * (Uc[1][m] - Uc[0][m])"),
     ("flux: dissipation sign", "- 0.5 * r * (Uc[1][m] - Uc[0][m])", "- 0.5 * r * (Uc[0][m] - Uc[1][m])"),

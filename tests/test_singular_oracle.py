@@ -32,8 +32,23 @@ AX = ("xi", "eta", "zeta")
 GOLDEN = Path(__file__).parent / "golden" / "z0_null_case_geometry.tar.gz"
 
 
-def _sphere(n=8, nr=6, **kw):
-    return cases.make_cubed_sphere(n, nr, **kw)
+def _sphere(n=8, nr=6, flip=False, **kw):
+    """The cubed sphere; flip reverses the radial axis of panel 0 and an angular
+    axis of panel 3 (nodes, fields and face codes) before the panels are glued,
+    so that aliases of one edge run in opposite index directions (orientation
+    -1 against the class's direction).  The owners keep +1: the owner and the
+    node representatives both come from the smallest block."""
+    c = cases.make_cubed_sphere(n, nr, **kw)
+    if flip:
+        for b, a in ((0, 0), (3, 2)):
+            blk = c.blocks[b]
+            blk.nodes = np.flip(blk.nodes, axis=1 + a).copy()
+            bc = list(blk.bc)
+            bc[2 * a], bc[2 * a + 1] = bc[2 * a + 1], bc[2 * a]
+            blk.bc = tuple(bc)
+            c.fields[b] = {k: np.flip(v, axis=1 + a).copy() for k, v in c.fields[b].items()}
+        c.patches = cases.glue_patches(c.blocks)
+    return c
 
 
 def _node(case, b, p):
@@ -92,9 +107,10 @@ def _faces_with_edge(case, p0, p1, tol=1e-12):
     return out
 
 
-def test_registry_of_the_cubed_sphere(oracle_mod):
+@pytest.mark.parametrize("flip", [False, True])
+def test_registry_of_the_cubed_sphere(oracle_mod, flip):
     nr = 6
-    case = _sphere(8, nr)
+    case = _sphere(8, nr, flip=flip)
     o = oracle_mod.Oracle(case)
     S = o.singular_edges()
     assert len(S) == 8 * nr
@@ -110,8 +126,11 @@ def test_registry_of_the_cubed_sphere(oracle_mod):
         assert e["inverse_hodge"] == pytest.approx(np.linalg.norm(p1 - p0) / tri, rel=1e-12)
         b, i, j, k, a = e["owner"]
         assert o.get(f"alpha_{AX[a]}", b)[0][i + g, j + g, k + g] == e["inverse_hodge"]
-    # every corner line, every radial layer
-    corners = {tuple(np.sign(np.round(e["edge_vector"] * e["sign"], 12)).astype(int)) for e in S}
+    # every corner line (the edge vector made to point outward), every radial layer
+    corners = set()
+    for e in S:
+        v = e["edge_vector"]
+        corners.add(tuple(np.sign(np.round(v * np.sign(v @ _edge_ends(case, e["owner"])[0]), 12)).astype(int)))
     assert len(corners) == 8
 
 
@@ -168,8 +187,9 @@ def _owner_dr(o, case, e):
     return o.get(f"dr_{AX[a]}", b)[:, i + g, j + g, k + g]
 
 
-def test_emf_of_uniform_fields(oracle_mod):
-    case = _sphere(6, 4)
+@pytest.mark.parametrize("flip", [False, True])
+def test_emf_of_uniform_fields(oracle_mod, flip):
+    case = _sphere(6, 4, flip=flip)
     P = case.phys
     P.ambipolar_enabled = 0.0
     P.arti_eta_max = 0.0
@@ -228,8 +248,9 @@ def test_emf_pressure_and_resistivity_terms(oracle_mod):
         assert _owner_value(o, case, "E", e) == pytest.approx(want, rel=1e-11)
 
 
-def test_current_of_one_face_flux(oracle_mod):
-    case = _sphere(6, 4)
+@pytest.mark.parametrize("flip", [False, True])
+def test_current_of_one_face_flux(oracle_mod, flip):
+    case = _sphere(6, 4, flip=flip)
     o = oracle_mod.Oracle(case)
     g = case.g
     S = o.singular_edges()
@@ -263,3 +284,60 @@ def test_current_of_one_face_flux(oracle_mod):
         o.op("singular_j")
         for n, e in enumerate(S):
             assert e["sign"] * _owner_value(o, case, "J", e) == pytest.approx(want[n], rel=1e-12)
+
+
+def _entry_cells(case, e):
+    """The cells around a singular edge, found from node coordinates, as
+    (block, i, j, k) with their centres (mean of 8 nodes)."""
+    p0, p1 = _edge_ends(case, e["owner"])
+    out = []
+    for ib, blk in enumerate(case.blocks):
+        X = blk.nodes
+        corners = [X[:, di:X.shape[1] - 1 + di, dj:X.shape[2] - 1 + dj, dk:X.shape[3] - 1 + dk]
+                   for di in (0, 1) for dj in (0, 1) for dk in (0, 1)]
+        has0 = np.zeros(corners[0].shape[1:], dtype=bool)
+        has1 = np.zeros_like(has0)
+        for c in corners:
+            has0 |= np.abs(c - p0[:, None, None, None]).max(axis=0) < 1e-12
+            has1 |= np.abs(c - p1[:, None, None, None]).max(axis=0) < 1e-12
+        for idx in zip(*np.nonzero(has0 & has1)):
+            out.append(((ib,) + tuple(int(x) for x in idx), np.mean([c[(slice(None),) + idx] for c in corners], axis=0)))
+    return out
+
+
+@pytest.mark.parametrize("flip", [False, True])
+def test_emf_taper_and_local_resistivity_at_the_cells(oracle_mod, flip):
+    """The Hall taper and the local resistivity are evaluated at each cell's
+    centre: E_hall = sum_c w_c taper(r_c) alpha (J x B).dr and the local term
+    sum_c w_c eta_max exp(-|x_c - x0| / r_decay) J.dr, w_c = |Jac_c| / sum."""
+    case = _sphere(6, 4, flip=flip)
+    P = case.phys
+    P.ambipolar_enabled = 0.0
+    P.arti_eta_max = 0.0
+    P.taper_rmin, P.taper_rmax = 1.0, 1.6
+    P.local_enabled, P.local_eta_max, P.local_r_decay, P.local_r_cutoff = 1.0, 0.02, 0.4, 0.0
+    x0 = np.array([0.7, 0.5, 0.4])
+    for d in range(3):
+        P.local_center[d] = x0[d]
+    o = oracle_mod.Oracle(case)
+    rhoH, rhoN = _uniform_densities(o, case)
+    B, J = np.array([0.3, -0.2, 0.5]), np.array([0.1, 0.4, -0.3])
+    _uniform(o, case, "U_plus", [0.0, 0.0, 0.0])
+    _uniform(o, case, "B_cell", B)
+    _uniform(o, case, "J_cell", J)
+    o.op("singular_e")
+    o.op("singular_ehall")
+    ne = rhoH / P.M_H * P.M_ref + rhoN / P.M_Na * P.M_ref
+    alpha = P.hall_coef / np.sqrt(ne * ne + P.ne_floor * P.ne_floor)
+    g = case.g
+    mid, width = 0.5 * (P.taper_rmin + P.taper_rmax), P.taper_rmax - P.taper_rmin
+    for e in o.singular_edges():
+        dr = _owner_dr(o, case, e)
+        cells = _entry_cells(case, e)
+        assert len(cells) == 3
+        w = np.array([abs(o.get("Jac", b)[0][i + g, j + g, k + g]) for (b, i, j, k), _ in cells])
+        w /= w.sum()
+        taper = np.array([0.5 * (1.0 + np.tanh(2.0 * (np.linalg.norm(x) - mid) / width)) for _, x in cells])
+        leta = np.array([P.local_eta_max * np.exp(-np.linalg.norm(x - x0) / P.local_r_decay) for _, x in cells])
+        assert _owner_value(o, case, "Ehall", e) == pytest.approx((w * taper).sum() * alpha * (np.cross(J, B) @ dr), rel=1e-11)
+        assert _owner_value(o, case, "E", e) == pytest.approx((w * leta).sum() * (J @ dr), rel=1e-11)
